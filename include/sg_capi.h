@@ -1,0 +1,195 @@
+/*
+ * sg_capi.h -- C ABI of the MI355X Schwarz-(F)GMRES solver (libsgamd.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an sg_status; the
+ * message of the last failure on a context is available from sg_last_error().
+ * Device pointers ("d_" prefix) are HIP device allocations (from sg_malloc or
+ * any other HIP allocation on the same device); host pointers are ordinary
+ * memory.  A context owns one HIP stream; every call made through a context
+ * and the objects created from it is ordered on that stream and is
+ * single-threaded.  Separate contexts may be used from separate threads.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (izbassarn7/schwarz-gmres-dashboard, paths relative to its root).
+ */
+#ifndef SG_CAPI_H
+#define SG_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SG_OK = 0,
+    SG_E_ARG = 1,        /* invalid argument / shape                           */
+    SG_E_HIP = 2,        /* HIP runtime failure                                */
+    SG_E_SINGULAR = 3,   /* "Coarse matrix is singular" (coarse_solve.h:31-32) */
+    SG_E_PARTITION = 4,  /* partitioning failed (domain_decomp.h:54-55)        */
+    SG_E_IO = 5,
+    SG_E_NOCONV = 6,     /* reserved: non-convergence is reported, not raised  */
+    SG_E_STATE = 7       /* object used before setup / wrong order             */
+} sg_status;
+
+typedef struct sg_context sg_context;
+typedef struct sg_csr sg_csr;
+typedef struct sg_precond sg_precond;
+
+/* ---- context, memory, errors ----------------------------------------------
+ * Replaces gpu::CudaHandles (src/gpu/cuda_common.h:42-66), DeviceBuffer
+ * (src/gpu/cuda_common.h:69-107) and the *_CHECK error macros (:13-40). */
+sg_status sg_context_create(int device, sg_context** out);
+sg_status sg_context_destroy(sg_context* ctx);
+sg_status sg_context_synchronize(sg_context* ctx);
+const char* sg_last_error(const sg_context* ctx); /* ctx may be NULL: last global error */
+sg_status sg_malloc(sg_context* ctx, size_t bytes, void** d_ptr);
+sg_status sg_free(sg_context* ctx, void* d_ptr);
+sg_status sg_memcpy_h2d(sg_context* ctx, void* d_dst, const void* h_src, size_t bytes);
+sg_status sg_memcpy_d2h(sg_context* ctx, void* h_dst, const void* d_src, size_t bytes);
+sg_status sg_memcpy_d2d(sg_context* ctx, void* d_dst, const void* d_src, size_t bytes);
+sg_status sg_memset_zero(sg_context* ctx, void* d_ptr, size_t bytes);
+
+/* ---- CSR matrix -----------------------------------------------------------
+ * Replaces CSRMatrix + DeviceCSRMatrix::upload (src/core/sparse_matrix.h:13-28,
+ * src/gpu/cuda_sparse.h:15-51).  Copies the host arrays (0-based, int32
+ * indices) to the device and keeps a host copy for the preconditioner setup.
+ * Columns must be sorted within each row for ILU(0)-based preconditioners
+ * (src/precond/ilu0.h:31-51 relies on it). */
+sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const int* row_ptr,
+                        const int* col_idx, const double* vals, sg_csr** out);
+sg_status sg_csr_destroy(sg_csr* A);
+sg_status sg_csr_info(const sg_csr* A, int* nrows, int* ncols, int* nnz);
+
+/* y = alpha*A*x + beta*y (FP64).  Replaces CSRMatrix::spmv
+ * (src/core/sparse_matrix.h:30-43) and DeviceCSRMatrix::spmv via cusparseSpMV
+ * (src/gpu/cuda_sparse.h:53-72).  Each row is summed sequentially in CSR order,
+ * so with alpha=1, beta=0 the result equals the reference's serial SpMV bit for
+ * bit. */
+sg_status sg_spmv(sg_csr* A, double alpha, const double* d_x, double beta, double* d_y);
+
+/* ---- BLAS-1 (FP64, device vectors) ----------------------------------------
+ * Replace schwarz::dot/norm2/axpy/scale (src/core/vector.h:11-48) and
+ * gpu::cublas_dot/nrm2/axpy/scal (src/gpu/cuda_blas.h:10-39).  Reductions use a
+ * fixed tree, so results are deterministic run to run. */
+sg_status sg_dot(sg_context* ctx, int n, const double* d_a, const double* d_b, double* result);
+sg_status sg_nrm2(sg_context* ctx, int n, const double* d_a, double* result);
+sg_status sg_axpy(sg_context* ctx, int n, double alpha, const double* d_x, double* d_y);
+sg_status sg_scal(sg_context* ctx, int n, double alpha, double* d_x);
+
+/* ---- preconditioners --------------------------------------------------------
+ * Replace the Preconditioner hierarchy (src/precond/preconditioner.h:7-20):
+ * JacobiPrecond (jacobi.h:10-27), ILU0Precond (ilu0.h:11-93), ASMPrecond
+ * (asm.h:15-71), RASPrecond (ras.h:16-82), TwoLevelSchwarzPrecond
+ * (two_level_schwarz.h:21-75) and the device ASMGPUPrecond (src/gpu/asm_gpu.h:30-186). */
+typedef enum {
+    SG_PC_NONE = 0,
+    SG_PC_JACOBI = 1,
+    SG_PC_ILU0 = 2,
+    SG_PC_ASM = 3,
+    SG_PC_RAS = 4,
+    SG_PC_TWOLEVEL_ASM = 5,
+    SG_PC_TWOLEVEL_RAS = 6
+} sg_precond_kind;
+
+typedef enum { SG_FP64 = 0, SG_FP32 = 1 } sg_precision;
+
+typedef struct {
+    int nparts;            /* subdomains (RASPrecond(nparts, overlap), ras.h:18-19)      */
+    int overlap;           /* BFS overlap layers (domain_decomp.h:72-86)                  */
+    int precision;         /* sg_precision of ILU factors, subdomain and coarse apply    */
+    const int* partition;  /* host int[nrows] subdomain of each row, or NULL: built-in
+                              partitioner (stands in for partition_metis,
+                              domain_decomp.h:23-58)                                      */
+    int ufactor;           /* allowed imbalance in 1/1000 (METIS_OPTION_UFACTOR); 0 = 30 */
+    int coarse_solve;      /* two-level only: 0 = LU forward/back solve (coarse_solve.h
+                              order), 1 = precomputed inverse (one dense mat-vec)         */
+} sg_precond_opts;
+
+sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_precond_opts* opts,
+                            sg_precond** out);
+/* Host decomposition (partition, overlap, extract), device ILU(0) factorisation
+ * and coarse-matrix LU.  Replaces Preconditioner::setup. */
+sg_status sg_precond_setup(sg_precond* M, const sg_csr* A);
+/* d_y = M^{-1} d_x.  Replaces Preconditioner::apply (host pointers) with device
+ * pointers on the context stream. */
+sg_status sg_precond_apply(sg_precond* M, const double* d_x, double* d_y);
+sg_status sg_precond_destroy(sg_precond* M);
+
+/* Host callback preconditioner: fn(user, x, y, n) computes y = M^{-1} x.  With
+ * device_pointers = 0 the library stages x and y through host memory, so any
+ * host-side Preconditioner::apply can be plugged in (FlexApply,
+ * src/core/fgmres.h:37). */
+typedef void (*sg_apply_fn)(void* user, const double* x, double* y, int n);
+sg_status sg_precond_create_callback(sg_context* ctx, int n, sg_apply_fn fn, void* user,
+                                     int device_pointers, sg_precond** out);
+
+/* Introspection for parity tests and tooling (all host outputs). */
+sg_status sg_precond_partition(const sg_precond* M, int* part /* int[nrows] */);
+/* Total overlapped size sum_s |subdomain_dofs[s]|, then the sets (sorted global
+ * ids, concatenated; ptr has nparts+1 entries).  Pass NULL arrays to query. */
+sg_status sg_precond_subdomains(const sg_precond* M, int* total, int* ptr, int* idx);
+/* ILU(0) factor values of subdomain s in that subdomain's local CSR order
+ * (ILU0Precond::lu_vals, ilu0.h:83); s = 0 for the global ILU(0). */
+sg_status sg_precond_ilu_values(const sg_precond* M, int s, int* nnz, double* vals);
+/* Coarse matrix A0 (row-major nc x nc) and its LU factors / pivots. */
+sg_status sg_precond_coarse(const sg_precond* M, int* nc, double* A0, double* LU, int* piv);
+
+/* ---- Krylov solve ------------------------------------------------------------
+ * Replaces gmres (src/core/gmres.h:28-33), fgmres / fgmres_flex
+ * (src/core/fgmres.h:42-55) and gpu::gmres_gpu / solve_gpu
+ * (src/gpu/gmres_gpu.h:25-72). */
+typedef enum { SG_GMRES_LEFT = 0, SG_FGMRES = 1 } sg_method;
+typedef enum { SG_ORTHO_MGS = 0, SG_ORTHO_CGS2 = 1 } sg_ortho;
+
+typedef struct {
+    int method;      /* sg_method                                                     */
+    int restart;     /* m (GMRESParams::restart, default 30)                          */
+    int max_iter;    /* GMRESParams::max_iter (cycle bound; total steps checked per
+                        cycle, so iterations may exceed it by up to m-1)              */
+    double tol_abs;  /* GMRESParams::tol: absolute test on the residual estimate
+                        (||M^-1 r|| for left GMRES, ||r|| for FGMRES)                  */
+    double rtol;     /* > 0: stop only when the FP64 true residual ||b-Ax||/||b|| <= rtol,
+                        verified with an FP64 SpMV (tol_abs is then ignored)           */
+    int ortho;       /* sg_ortho: MGS (reference order) or CGS2                       */
+    int lookahead;   /* Arnoldi steps the host may run ahead of the device (0 = 2)     */
+    int flags;       /* reserved, 0                                                   */
+} sg_solve_params;
+
+typedef struct {
+    int iterations;           /* GMRESResult::iterations                              */
+    int converged;            /* GMRESResult::converged                               */
+    double residual_norm;     /* GMRESResult::residual_norm (last estimate)           */
+    double rel_true_residual; /* ||b - A x|| / ||b|| in FP64 at exit                  */
+    double time_solve_s;      /* GMRESResult::time_solve_s (wall, includes the sync)  */
+    int cycles;               /* restart cycles run                                   */
+    int precond_applies;
+} sg_solve_result;
+
+/* x (device, in/out) is the initial guess and the solution.  hist (host, may be
+ * NULL) receives the residual estimate after every Arnoldi step
+ * (GMRESResult::residual_history), up to hist_cap entries. */
+sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                   const sg_solve_params* params, sg_solve_result* result, double* hist,
+                   int hist_cap);
+
+/* ---- kernel timing (HIP events on the context stream) ------------------------
+ * With profiling on, every launch of the listed kernel families is bracketed
+ * by HIP events; totals are collected at each synchronisation. */
+typedef enum {
+    SG_K_SPMV = 0,     /* CSR SpMV (Arnoldi and residual)          */
+    SG_K_PRECOND = 1,  /* whole preconditioner apply                */
+    SG_K_TRSV = 2,     /* fused gather + ILU(0) sweeps + scatter    */
+    SG_K_ORTHO = 3,    /* Gram-Schmidt + norm + Givens              */
+    SG_K_UPDATE = 4,   /* restart residual, basis scaling, x update */
+    SG_K_COUNT = 5
+} sg_kernel_family;
+sg_status sg_context_set_profiling(sg_context* ctx, int enable);
+sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
+                                  double* total_ms);
+sg_status sg_context_reset_stats(sg_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SG_CAPI_H */

@@ -1,0 +1,244 @@
+// Golden-vector harness over the reference's METIS-free sources.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the
+// unmodified reference headers and translation units (core/gmres.cpp,
+// core/fgmres.cpp, precond/ilu0.h, precond/jacobi.h, precond/coarse_solve.h,
+// mesh/poisson.h, io/metrics_logger.h, core/sparse_matrix.h).  The binary goes
+// to oracle/_ref/ and is driven by tests/golden/make_golden.py, which writes the
+// committed fixtures.  Nothing under the reference's decomp/ (needs METIS) is
+// included: RAS/ASM goldens are produced from the reference's own
+// extract_submatrix and ILU0Precond on subdomain index sets that the caller
+// supplies.
+#include "core/sparse_matrix.h"
+#include "core/vector.h"
+#include "core/gmres.h"
+#include "core/fgmres.h"
+#include "precond/ilu0.h"
+#include "precond/jacobi.h"
+#include "precond/coarse_solve.h"
+#include "mesh/poisson.h"
+#include "io/metrics_logger.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace schwarz;
+
+namespace {
+
+// SplitMix64 (Steele, Lea, Flood 2014); u = (x >> 11) * 2^-53.
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uniform01() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+void print_vec(const char* key, const std::vector<double>& v, bool last = false) {
+    std::printf("\"%s\":[", key);
+    for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? "," : "", v[i]);
+    std::printf("]%s", last ? "" : ",");
+}
+void print_ivec(const char* key, const std::vector<int>& v, bool last = false) {
+    std::printf("\"%s\":[", key);
+    for (size_t i = 0; i < v.size(); ++i) std::printf("%s%d", i ? "," : "", v[i]);
+    std::printf("]%s", last ? "" : ",");
+}
+
+CSRMatrix grid(const std::string& spec) {
+    // "2d:NX:NY" or "3d:NX:NY:NZ"
+    int a = 0, b = 0, c = 0;
+    if (spec.rfind("2d:", 0) == 0) {
+        std::sscanf(spec.c_str() + 3, "%d:%d", &a, &b);
+        return make_poisson_2d(a, b);
+    }
+    std::sscanf(spec.c_str() + 3, "%d:%d:%d", &a, &b, &c);
+    return make_poisson_3d(a, b, c);
+}
+
+int cmd_ilu0(const std::string& spec) {
+    CSRMatrix A = grid(spec);
+    ILU0Precond p;
+    p.setup(A);
+    std::vector<double> x(A.nrows), y(A.nrows);
+    SplitMix64 rng(7);
+    for (auto& t : x) t = 2.0 * rng.uniform01() - 1.0;
+    p.apply(x.data(), y.data(), A.nrows);
+    std::printf("{");
+    print_vec("lu", p.lu_vals());
+    print_ivec("diag", p.diag_ptr());
+    print_vec("x", x);
+    print_vec("y", y, true);
+    std::printf("}\n");
+    return 0;
+}
+
+int cmd_solve(int argc, char** argv) {
+    // solve METHOD PC GRID RHS RESTART MAXITER TOL
+    if (argc < 9) return 2;
+    std::string method = argv[2], pc = argv[3], rhs = argv[5];
+    CSRMatrix A = grid(argv[4]);
+    GMRESParams prm;
+    prm.restart = std::atoi(argv[6]);
+    prm.max_iter = std::atoi(argv[7]);
+    prm.tol = std::atof(argv[8]);
+    int n = A.nrows;
+    std::vector<double> xs(n, 1.0), b(n), x(n, 0.0);
+    if (rhs.rfind("uniform:", 0) == 0) {
+        SplitMix64 rng(std::strtoull(rhs.c_str() + 8, nullptr, 10));
+        for (auto& t : xs) t = 2.0 * rng.uniform01() - 1.0;
+    }
+    A.spmv(xs, b);
+    std::unique_ptr<Preconditioner> M;
+    if (pc == "jacobi") M.reset(new JacobiPrecond());
+    else if (pc == "ilu0") M.reset(new ILU0Precond());
+    if (M) M->setup(A);
+    // A negative TOL means relative: |TOL| * ||M^-1 b|| for GMRES (the quantity
+    // its absolute test measures at x0 = 0) and |TOL| * ||b|| for FGMRES.
+    if (prm.tol < 0) {
+        std::vector<double> mb(n);
+        if (M && method != "fgmres") M->apply(b.data(), mb.data(), n);
+        else mb = b;
+        prm.tol = -prm.tol * norm2(mb);
+    }
+    GMRESResult r = method == "fgmres" ? fgmres(A, b, x, M.get(), prm)
+                                       : gmres(A, b, x, M.get(), prm);
+    std::vector<double> ax;
+    A.spmv(x, ax);
+    double rn = 0, bn = 0, en = 0, xn = 0;
+    for (int i = 0; i < n; ++i) {
+        rn += (b[i] - ax[i]) * (b[i] - ax[i]);
+        bn += b[i] * b[i];
+        en += (x[i] - xs[i]) * (x[i] - xs[i]);
+        xn += xs[i] * xs[i];
+    }
+    std::printf("{\"iterations\":%d,\"residual_norm\":%.17g,\"converged\":%d,\"tol\":%.17g,",
+                r.iterations, r.residual_norm, r.converged ? 1 : 0, prm.tol);
+    std::printf("\"rel_true_residual\":%.17g,\"x_rel_error\":%.17g,", std::sqrt(rn / bn),
+                std::sqrt(en / xn));
+    if (n <= 4096) print_vec("x", x);
+    print_vec("hist", r.residual_history, true);
+    std::printf("}\n");
+    return 0;
+}
+
+// Reads from stdin: GRID nsub; per subdomain: m, then m sorted global dofs,
+// then m owned flags (0/1); then n input values.  Writes the RAS and ASM
+// outputs computed with the reference extract_submatrix + ILU0Precond, with
+// gather / restricted scatter / additive scatter as src/precond/ras.h:45-70 and
+// src/precond/asm.h:35-62 do it (ascending subdomain order).
+int cmd_blocks() {
+    char spec[64];
+    int nsub;
+    if (std::scanf("%63s %d", spec, &nsub) != 2) return 2;
+    CSRMatrix A = grid(spec);
+    int n = A.nrows;
+    std::vector<std::vector<int>> dofs(nsub), owned(nsub);
+    for (int s = 0; s < nsub; ++s) {
+        int m;
+        if (std::scanf("%d", &m) != 1) return 2;
+        dofs[s].resize(m);
+        owned[s].resize(m);
+        for (int i = 0; i < m; ++i) if (std::scanf("%d", &dofs[s][i]) != 1) return 2;
+        for (int i = 0; i < m; ++i) if (std::scanf("%d", &owned[s][i]) != 1) return 2;
+    }
+    std::vector<double> x(n), yr(n, 0.0), ya(n, 0.0);
+    for (int i = 0; i < n; ++i) if (std::scanf("%lf", &x[i]) != 1) return 2;
+    for (int s = 0; s < nsub; ++s) {
+        CSRMatrix Al = A.extract_submatrix(dofs[s]);
+        ILU0Precond p;
+        p.setup(Al);
+        int m = (int)dofs[s].size();
+        std::vector<double> rl(m), zl(m);
+        for (int i = 0; i < m; ++i) rl[i] = x[dofs[s][i]];
+        p.apply(rl.data(), zl.data(), m);
+        for (int i = 0; i < m; ++i) {
+            if (owned[s][i]) yr[dofs[s][i]] = zl[i];
+            ya[dofs[s][i]] += zl[i];
+        }
+    }
+    std::printf("{");
+    print_vec("ras", yr);
+    print_vec("asm", ya, true);
+    std::printf("}\n");
+    return 0;
+}
+
+int cmd_lu(int n, uint64_t seed, int singular) {
+    SplitMix64 rng(seed);
+    std::vector<double> A((size_t)n * n);
+    for (auto& t : A) t = 2.0 * rng.uniform01() - 1.0;
+    if (singular) for (int j = 0; j < n; ++j) A[(size_t)(n - 1) * n + j] = A[j];
+    std::vector<double> b(n);
+    for (auto& t : b) t = 2.0 * rng.uniform01() - 1.0;
+    CoarseSolver cs;
+    try {
+        cs.setup(A, n);
+    } catch (const std::exception& e) {
+        std::printf("{\"error\":\"%s\"}\n", e.what());
+        return 0;
+    }
+    std::vector<double> x = cs.solve(b);
+    std::printf("{");
+    print_vec("A", A);
+    print_vec("b", b);
+    print_vec("x", x, true);
+    std::printf("}\n");
+    return 0;
+}
+
+int cmd_metrics() {
+    Metrics m;
+    m.matrix = "poisson2d_256";
+    m.method = "GMRES+ilu0";
+    m.n = 65536;
+    m.nnz = 326656;
+    m.threads = 8;
+    m.overlap = 1;
+    m.iterations = 412;
+    m.tol = 1e-8;
+    m.residual_norm = 9.87654321e-9;
+    m.time_setup_s = 0.0123456789;
+    m.time_solve_s = 1.5;
+    std::printf("%s\n", m.to_json().c_str());
+    Metrics d;  // defaults
+    std::printf("%s\n", d.to_json().c_str());
+    Metrics g;
+    g.matrix = "poisson3d_128";
+    g.method = "GPU_GMRES+ras";
+    g.n = 2097152;
+    g.nnz = 14581760;
+    g.gpus = 1;
+    g.restart = 50;
+    g.tol = 0.000123;
+    g.residual_norm = 0.0;
+    g.time_solve_s = 123456.5;
+    g.speedup = 3.25;
+    g.efficiency = 1e-7;
+    std::printf("%s\n", g.to_json().c_str());
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    std::string c = argv[1];
+    if (c == "ilu0" && argc >= 3) return cmd_ilu0(argv[2]);
+    if (c == "solve") return cmd_solve(argc, argv);
+    if (c == "blocks") return cmd_blocks();
+    if (c == "lu" && argc >= 5) return cmd_lu(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]));
+    if (c == "metrics") return cmd_metrics();
+    return 2;
+}

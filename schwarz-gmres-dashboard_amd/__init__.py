@@ -1,0 +1,309 @@
+"""ctypes binding of libsgamd.so, the MI355X Schwarz-(F)GMRES C ABI (include/sg_capi.h).
+
+The product path is the HIP library; this module only moves arrays and calls
+the C entry points.  It never falls back to a CPU implementation: when the
+library is missing or no GPU is visible, the calls raise.
+
+Names mirror the reference's host interface (izbassarn7/schwarz-gmres-dashboard):
+``Precond`` covers JacobiPrecond / ILU0Precond / ASMPrecond / RASPrecond /
+TwoLevelSchwarzPrecond (src/precond/*.h), ``solve`` covers gmres / fgmres
+(src/core/gmres.h:28-33, src/core/fgmres.h:42-55) with GMRESParams /
+GMRESResult fields.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgamd.so")
+
+PC_KIND = {"none": 0, "jacobi": 1, "ilu0": 2, "asm": 3, "ras": 4, "twolevel_asm": 5,
+           "twolevel_ras": 6}
+STATUS = {0: "SG_OK", 1: "SG_E_ARG", 2: "SG_E_HIP", 3: "SG_E_SINGULAR", 4: "SG_E_PARTITION",
+          5: "SG_E_IO", 6: "SG_E_NOCONV", 7: "SG_E_STATE"}
+K_FAMILY = {"spmv": 0, "precond": 1, "trsv": 2, "ortho": 3, "update": 4}
+
+
+class SgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class PrecondOpts(C.Structure):
+    _fields_ = [("nparts", C.c_int), ("overlap", C.c_int), ("precision", C.c_int),
+                ("partition", C.POINTER(C.c_int)), ("ufactor", C.c_int),
+                ("coarse_solve", C.c_int)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("method", C.c_int), ("restart", C.c_int), ("max_iter", C.c_int),
+                ("tol_abs", C.c_double), ("rtol", C.c_double), ("ortho", C.c_int),
+                ("lookahead", C.c_int), ("flags", C.c_int)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int),
+                ("residual_norm", C.c_double), ("rel_true_residual", C.c_double),
+                ("time_solve_s", C.c_double), ("cycles", C.c_int),
+                ("precond_applies", C.c_int)]
+
+
+_lib = None
+_VP = C.c_void_p
+
+
+def lib():
+    """Load libsgamd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "sg_context_create": ([C.c_int, C.POINTER(_VP)], C.c_int),
+        "sg_context_destroy": ([_VP], C.c_int),
+        "sg_context_synchronize": ([_VP], C.c_int),
+        "sg_last_error": ([_VP], C.c_char_p),
+        "sg_malloc": ([_VP, C.c_size_t, C.POINTER(_VP)], C.c_int),
+        "sg_free": ([_VP, _VP], C.c_int),
+        "sg_memcpy_h2d": ([_VP, _VP, _VP, C.c_size_t], C.c_int),
+        "sg_memcpy_d2h": ([_VP, _VP, _VP, C.c_size_t], C.c_int),
+        "sg_memcpy_d2d": ([_VP, _VP, _VP, C.c_size_t], C.c_int),
+        "sg_memset_zero": ([_VP, _VP, C.c_size_t], C.c_int),
+        "sg_csr_create": ([_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)], C.c_int),
+        "sg_csr_destroy": ([_VP], C.c_int),
+        "sg_csr_info": ([_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "sg_spmv": ([_VP, C.c_double, _VP, C.c_double, _VP], C.c_int),
+        "sg_dot": ([_VP, C.c_int, _VP, _VP, C.POINTER(C.c_double)], C.c_int),
+        "sg_nrm2": ([_VP, C.c_int, _VP, C.POINTER(C.c_double)], C.c_int),
+        "sg_axpy": ([_VP, C.c_int, C.c_double, _VP, _VP], C.c_int),
+        "sg_scal": ([_VP, C.c_int, C.c_double, _VP], C.c_int),
+        "sg_precond_create": ([_VP, C.c_int, C.POINTER(PrecondOpts), C.POINTER(_VP)], C.c_int),
+        "sg_precond_setup": ([_VP, _VP], C.c_int),
+        "sg_precond_apply": ([_VP, _VP, _VP], C.c_int),
+        "sg_precond_destroy": ([_VP], C.c_int),
+        "sg_precond_create_callback": ([_VP, C.c_int, _VP, _VP, C.c_int, C.POINTER(_VP)], C.c_int),
+        "sg_precond_partition": ([_VP, _VP], C.c_int),
+        "sg_precond_subdomains": ([_VP, C.POINTER(C.c_int), _VP, _VP], C.c_int),
+        "sg_precond_ilu_values": ([_VP, C.c_int, C.POINTER(C.c_int), _VP], C.c_int),
+        "sg_precond_coarse": ([_VP, C.POINTER(C.c_int), _VP, _VP, _VP], C.c_int),
+        "sg_solve": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult), _VP,
+                      C.c_int], C.c_int),
+        "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
+        "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
+                                     C.POINTER(C.c_double)], C.c_int),
+        "sg_context_reset_stats": ([_VP], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/sg_capi.h that the binding knows."""
+    return [n for n in dir(lib()) if n.startswith("sg_")]
+
+
+def _p(a):
+    return a.ctypes.data_as(_VP) if a is not None else None
+
+
+class Context:
+    """One HIP device + one stream (sg_context)."""
+
+    def __init__(self, device=0):
+        self.h = _VP()
+        st = lib().sg_context_create(device, C.byref(self.h))
+        if st:
+            raise SgError(st, lib().sg_last_error(None).decode())
+
+    def check(self, st):
+        if st:
+            raise SgError(st, lib().sg_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(lib().sg_context_synchronize(self.h))
+
+    def profiling(self, on=True):
+        self.check(lib().sg_context_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self, family):
+        n = C.c_longlong()
+        ms = C.c_double()
+        self.check(lib().sg_context_kernel_stats(self.h, K_FAMILY[family], C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def reset_stats(self):
+        self.check(lib().sg_context_reset_stats(self.h))
+
+    def close(self):
+        if self.h:
+            lib().sg_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation holding a 1-D array (float64 by default)."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = _VP()
+        ctx.check(lib().sg_malloc(ctx.h, max(self.nbytes, 8), C.byref(self.ptr)))
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        self.ctx.check(lib().sg_memcpy_h2d(self.ctx.h, self.ptr, _p(a), self.nbytes))
+
+    def get(self):
+        out = np.empty(self.n, self.dtype)
+        self.ctx.check(lib().sg_memcpy_d2h(self.ctx.h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx.check(lib().sg_memset_zero(self.ctx.h, self.ptr, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().sg_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Csr:
+    """Device CSR matrix (sg_csr); host arrays are copied."""
+
+    def __init__(self, ctx, rp, ci, v, ncols=None):
+        self.ctx = ctx
+        self.rp = np.ascontiguousarray(rp, np.int32)
+        self.ci = np.ascontiguousarray(ci, np.int32)
+        self.v = np.ascontiguousarray(v, np.float64)
+        self.n = len(self.rp) - 1
+        self.ncols = self.n if ncols is None else ncols
+        self.h = _VP()
+        ctx.check(lib().sg_csr_create(ctx.h, self.n, self.ncols, int(self.rp[-1]), _p(self.rp),
+                                      _p(self.ci), _p(self.v), C.byref(self.h)))
+
+    @property
+    def nnz(self):
+        return int(self.rp[-1])
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0):
+        self.ctx.check(lib().sg_spmv(self.h, alpha, x.ptr, beta, y.ptr))
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().sg_csr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Precond:
+    """Device preconditioner (sg_precond): kind in PC_KIND."""
+
+    def __init__(self, ctx, kind, nparts=4, overlap=1, precision="fp64", partition=None,
+                 ufactor=0, coarse_solve=0):
+        self.ctx = ctx
+        self.kind = kind
+        self._part = None if partition is None else np.ascontiguousarray(partition, np.int32)
+        self.opts = PrecondOpts(nparts, overlap, 1 if precision == "fp32" else 0,
+                                None if self._part is None else
+                                self._part.ctypes.data_as(C.POINTER(C.c_int)),
+                                ufactor, coarse_solve)
+        self.h = _VP()
+        ctx.check(lib().sg_precond_create(ctx.h, PC_KIND[kind], C.byref(self.opts),
+                                          C.byref(self.h)))
+        self.n = None
+
+    def setup(self, A):
+        self.ctx.check(lib().sg_precond_setup(self.h, A.h))
+        self.n = A.n
+        return self
+
+    def apply(self, x, y):
+        self.ctx.check(lib().sg_precond_apply(self.h, x.ptr, y.ptr))
+
+    def partition(self):
+        out = np.zeros(self.n, np.int32)
+        self.ctx.check(lib().sg_precond_partition(self.h, _p(out)))
+        return out
+
+    def subdomains(self):
+        tot = C.c_int()
+        self.ctx.check(lib().sg_precond_subdomains(self.h, C.byref(tot), None, None))
+        nparts = self.opts.nparts if self.kind not in ("ilu0",) else 1
+        ptr = np.zeros(nparts + 1, np.int32)
+        idx = np.zeros(tot.value, np.int32)
+        self.ctx.check(lib().sg_precond_subdomains(self.h, C.byref(tot), _p(ptr), _p(idx)))
+        return [idx[ptr[s]:ptr[s + 1]] for s in range(nparts)]
+
+    def ilu_values(self, s=0):
+        nnz = C.c_int()
+        self.ctx.check(lib().sg_precond_ilu_values(self.h, s, C.byref(nnz), None))
+        out = np.zeros(nnz.value)
+        self.ctx.check(lib().sg_precond_ilu_values(self.h, s, C.byref(nnz), _p(out)))
+        return out
+
+    def coarse(self):
+        nc = C.c_int()
+        self.ctx.check(lib().sg_precond_coarse(self.h, C.byref(nc), None, None, None))
+        k = nc.value
+        A0 = np.zeros(k * k)
+        LU = np.zeros(k * k)
+        piv = np.zeros(k, np.int32)
+        self.ctx.check(lib().sg_precond_coarse(self.h, C.byref(nc), _p(A0), _p(LU), _p(piv)))
+        return A0.reshape(k, k), LU.reshape(k, k), piv
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().sg_precond_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10, rtol=0.0,
+          ortho="mgs", lookahead=0, hist_cap=None):
+    """Run sg_solve.  b, x are DeviceArrays (x in/out).  Returns a dict shaped
+    like GMRESResult plus the FP64 true relative residual."""
+    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
+                    1 if ortho == "cgs2" else 0, lookahead, 0)
+    r = SolveResult()
+    cap = hist_cap if hist_cap is not None else max_iter + restart + 2
+    hist = np.zeros(max(cap, 1))
+    A.ctx.check(lib().sg_solve(A.h, M.h if M is not None else None, b.ptr, x.ptr, C.byref(p),
+                               C.byref(r), _p(hist), cap))
+    return dict(iterations=r.iterations, converged=bool(r.converged),
+                residual_norm=r.residual_norm, rel_true_residual=r.rel_true_residual,
+                time_solve_s=r.time_solve_s, cycles=r.cycles,
+                precond_applies=r.precond_applies,
+                hist=hist[:min(r.iterations, cap)].copy())

@@ -1,0 +1,471 @@
+// Host-side setup: partition, overlap sets, extraction, level schedules.
+#include "host_setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <thread>
+#include <utility>
+
+namespace sg {
+
+namespace {
+
+// Undirected weighted graph in CSR form.
+struct Graph {
+    int nv = 0;
+    std::vector<int> xadj, adj, ew, vw;
+    long long total_vw() const {
+        long long t = 0;
+        for (int w : vw) t += w;
+        return t;
+    }
+};
+
+Graph graph_from_csr(const HostCsr& A) {
+    // symmetrise the pattern, drop self loops and duplicates
+    int n = A.n;
+    std::vector<std::pair<int, int>> e;
+    e.reserve((size_t)A.nnz() * 2);
+    for (int i = 0; i < n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            int j = A.ci[k];
+            if (j == i || j < 0 || j >= n) continue;
+            e.emplace_back(i, j);
+            e.emplace_back(j, i);
+        }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    Graph g;
+    g.nv = n;
+    g.xadj.assign(n + 1, 0);
+    for (auto& p : e) g.xadj[p.first + 1]++;
+    for (int i = 0; i < n; ++i) g.xadj[i + 1] += g.xadj[i];
+    g.adj.resize(e.size());
+    g.ew.assign(e.size(), 1);
+    for (size_t k = 0; k < e.size(); ++k) g.adj[k] = e[k].second;
+    g.vw.assign(n, 1);
+    return g;
+}
+
+// Heavy-edge matching coarsening.  Returns coarse graph and fine->coarse map.
+bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
+    int n = g.nv;
+    std::vector<int> match(n, -1);
+    // visit in order of increasing degree (ties by index): deterministic
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (g.xadj[a + 1] - g.xadj[a]) < (g.xadj[b + 1] - g.xadj[b]);
+    });
+    long long maxvw = std::max<long long>(1, g.total_vw() / 20);
+    for (int v : order) {
+        if (match[v] >= 0) continue;
+        int best = -1, bw = -1;
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+            int u = g.adj[k];
+            if (match[u] >= 0 || u == v) continue;
+            if (g.vw[u] + g.vw[v] > maxvw) continue;
+            if (g.ew[k] > bw) { bw = g.ew[k]; best = u; }
+        }
+        if (best >= 0) { match[v] = best; match[best] = v; }
+        else match[v] = v;
+    }
+    cmap.assign(n, -1);
+    int nc = 0;
+    for (int v = 0; v < n; ++v) {
+        if (cmap[v] >= 0) continue;
+        cmap[v] = nc;
+        if (match[v] != v) cmap[match[v]] = nc;
+        ++nc;
+    }
+    if (nc > 0.95 * n) return false;
+    c.nv = nc;
+    c.vw.assign(nc, 0);
+    for (int v = 0; v < n; ++v) c.vw[cmap[v]] += g.vw[v];
+    // members of each coarse vertex in ascending fine order
+    std::vector<int> first(nc, -1), second(nc, -1);
+    for (int v = 0; v < n; ++v) {
+        int cv = cmap[v];
+        if (first[cv] < 0) first[cv] = v; else second[cv] = v;
+    }
+    c.xadj.assign(nc + 1, 0);
+    c.adj.clear();
+    c.ew.clear();
+    std::vector<int> mark(nc, -1), slot(nc, 0);
+    for (int cv = 0; cv < nc; ++cv) {
+        int start = (int)c.adj.size();
+        for (int m = 0; m < 2; ++m) {
+            int v = m == 0 ? first[cv] : second[cv];
+            if (v < 0) continue;
+            for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+                int cu = cmap[g.adj[k]];
+                if (cu == cv) continue;
+                if (mark[cu] != cv) {
+                    mark[cu] = cv;
+                    slot[cu] = (int)c.adj.size();
+                    c.adj.push_back(cu);
+                    c.ew.push_back(g.ew[k]);
+                } else {
+                    c.ew[slot[cu]] += g.ew[k];
+                }
+            }
+        }
+        (void)start;
+        c.xadj[cv + 1] = (int)c.adj.size();
+    }
+    return true;
+}
+
+long long cut_of(const Graph& g, const std::vector<int>& side) {
+    long long cut = 0;
+    for (int v = 0; v < g.nv; ++v)
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
+            if (side[g.adj[k]] != side[v]) cut += g.ew[k];
+    return cut / 2;
+}
+
+// Boundary Fiduccia-Mattheyses refinement of a bisection with balance window
+// [lo, hi] on the weight of side 0.
+void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long hi, int passes) {
+    int n = g.nv;
+    long long w0 = 0;
+    for (int v = 0; v < n; ++v) if (side[v] == 0) w0 += g.vw[v];
+    std::vector<int> gain(n), locked(n);
+    auto compute_gain = [&](int v) {
+        int ext = 0, in = 0;
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
+            (side[g.adj[k]] != side[v] ? ext : in) += g.ew[k];
+        return ext - in;
+    };
+    auto boundary = [&](int v) {
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
+            if (side[g.adj[k]] != side[v]) return true;
+        return false;
+    };
+    for (int pass = 0; pass < passes; ++pass) {
+        std::set<std::pair<int, int>> q[2];  // (-gain, v)
+        std::fill(locked.begin(), locked.end(), 0);
+        for (int v = 0; v < n; ++v) {
+            if (!boundary(v)) continue;
+            gain[v] = compute_gain(v);
+            q[side[v]].insert({-gain[v], v});
+        }
+        std::vector<int> moved;
+        long long cur = 0, best = 0;
+        size_t best_len = 0;
+        long long best_imb = std::llabs(w0 - (lo + hi) / 2);
+        int since_best = 0;
+        const int limit = std::max(50, std::min(n / 20 + 1, 400));
+        while (since_best < limit) {
+            // candidate from each side that keeps the balance window
+            int pick = -1;
+            int pside = -1;
+            for (int s = 0; s < 2; ++s) {
+                for (auto it = q[s].begin(); it != q[s].end(); ++it) {
+                    int v = it->second;
+                    long long nw0 = s == 0 ? w0 - g.vw[v] : w0 + g.vw[v];
+                    if (nw0 < lo || nw0 > hi) continue;
+                    if (pick < 0 || gain[v] > gain[pick] ||
+                        (gain[v] == gain[pick] && std::llabs(nw0 - (lo + hi) / 2) <
+                                                      std::llabs((pside == 0 ? w0 - g.vw[pick] : w0 + g.vw[pick]) - (lo + hi) / 2))) {
+                        pick = v;
+                        pside = s;
+                    }
+                    break;  // only the best-gain feasible-first entry per side
+                }
+            }
+            if (pick < 0) break;
+            q[pside].erase({-gain[pick], pick});
+            cur -= gain[pick];
+            w0 += pside == 0 ? -g.vw[pick] : g.vw[pick];
+            side[pick] ^= 1;
+            locked[pick] = 1;
+            moved.push_back(pick);
+            for (int k = g.xadj[pick]; k < g.xadj[pick + 1]; ++k) {
+                int u = g.adj[k];
+                if (locked[u]) continue;
+                bool inq = q[side[u]].count({-gain[u], u}) > 0;
+                if (inq) q[side[u]].erase({-gain[u], u});
+                gain[u] = compute_gain(u);
+                if (boundary(u)) q[side[u]].insert({-gain[u], u});
+            }
+            long long imb = std::llabs(w0 - (lo + hi) / 2);
+            if (cur < best || (cur == best && imb < best_imb)) {
+                best = cur;
+                best_len = moved.size();
+                best_imb = imb;
+                since_best = 0;
+            } else {
+                ++since_best;
+            }
+        }
+        // roll back past the best prefix
+        for (size_t i = moved.size(); i > best_len; --i) {
+            int v = moved[i - 1];
+            w0 += side[v] == 0 ? -g.vw[v] : g.vw[v];
+            side[v] ^= 1;
+        }
+        if (best_len == 0) break;
+    }
+}
+
+// Greedy graph growing from a seed; grows side 0 until its weight >= target.
+std::vector<int> grow(const Graph& g, int seed, long long target) {
+    int n = g.nv;
+    std::vector<int> side(n, 1), gain(n, 0), inq(n, 0);
+    std::set<std::pair<int, int>> q;
+    long long w0 = 0;
+    auto push = [&](int v) {
+        int ext = 0, in = 0;
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
+            (side[g.adj[k]] == 0 ? in : ext) += g.ew[k];
+        gain[v] = in - ext;
+        q.insert({-gain[v], v});
+        inq[v] = 1;
+    };
+    push(seed);
+    int next_unreached = 0;
+    while (w0 < target) {
+        if (q.empty()) {  // disconnected: restart from the next side-1 vertex
+            while (next_unreached < n && side[next_unreached] == 0) ++next_unreached;
+            if (next_unreached >= n) break;
+            push(next_unreached);
+        }
+        int v = q.begin()->second;
+        q.erase(q.begin());
+        inq[v] = 0;
+        side[v] = 0;
+        w0 += g.vw[v];
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+            int u = g.adj[k];
+            if (side[u] == 0) continue;
+            if (inq[u]) q.erase({-gain[u], u});
+            push(u);
+        }
+    }
+    return side;
+}
+
+int bfs_far(const Graph& g, int src) {
+    std::vector<int> d(g.nv, -1);
+    std::vector<int> qu;
+    qu.push_back(src);
+    d[src] = 0;
+    int last = src;
+    for (size_t h = 0; h < qu.size(); ++h) {
+        int v = qu[h];
+        last = v;
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
+            if (d[g.adj[k]] < 0) { d[g.adj[k]] = d[v] + 1; qu.push_back(g.adj[k]); }
+    }
+    return last;
+}
+
+std::vector<int> initial_bisect(const Graph& g, long long target, long long lo, long long hi) {
+    std::vector<int> seeds;
+    int a = bfs_far(g, 0);
+    int b = bfs_far(g, a);
+    seeds.push_back(a);
+    seeds.push_back(b);
+    for (int t = 1; t <= 4 && g.nv > 8; ++t) seeds.push_back((int)((long long)g.nv * t / 5));
+    std::vector<int> best;
+    long long bestcut = -1;
+    for (int s : seeds) {
+        std::vector<int> side = grow(g, s, target);
+        fm_refine(g, side, lo, hi, 4);
+        long long c = cut_of(g, side);
+        if (bestcut < 0 || c < bestcut) { bestcut = c; best = side; }
+    }
+    return best;
+}
+
+// Multilevel bisection: side[v] in {0,1}, side 0 weight ~ frac * total.
+std::vector<int> bisect(const Graph& g, double frac, double eps) {
+    long long tot = g.total_vw();
+    long long target = (long long)std::llround(frac * (double)tot);
+    std::vector<Graph> levels;
+    std::vector<std::vector<int>> maps;
+    levels.push_back(g);
+    while (levels.back().nv > 160) {
+        Graph c;
+        std::vector<int> cm;
+        if (!coarsen(levels.back(), c, cm)) break;
+        maps.push_back(std::move(cm));
+        levels.push_back(std::move(c));
+    }
+    auto window = [&](const Graph& h) {
+        long long maxv = 1;
+        for (int w : h.vw) maxv = std::max<long long>(maxv, w);
+        long long slack = std::max<long long>((long long)(eps * (double)tot), maxv);
+        return std::make_pair(target - slack, target + slack);
+    };
+    auto w = window(levels.back());
+    std::vector<int> side = initial_bisect(levels.back(), target, w.first, w.second);
+    for (int l = (int)levels.size() - 2; l >= 0; --l) {
+        std::vector<int> fine(levels[l].nv);
+        for (int v = 0; v < levels[l].nv; ++v) fine[v] = side[maps[l][v]];
+        side.swap(fine);
+        auto wl = window(levels[l]);
+        fm_refine(levels[l], side, wl.first, wl.second, l == 0 ? 6 : 3);
+    }
+    return side;
+}
+
+Graph induced(const Graph& g, const std::vector<int>& verts, std::vector<int>& local) {
+    // verts ascending; local[v] = new id or -1
+    Graph s;
+    s.nv = (int)verts.size();
+    s.xadj.assign(s.nv + 1, 0);
+    s.vw.resize(s.nv);
+    for (int i = 0; i < s.nv; ++i) {
+        int v = verts[i];
+        s.vw[i] = g.vw[v];
+        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+            int u = local[g.adj[k]];
+            if (u >= 0) { s.adj.push_back(u); s.ew.push_back(g.ew[k]); }
+        }
+        s.xadj[i + 1] = (int)s.adj.size();
+    }
+    return s;
+}
+
+void recurse(const Graph& g, const std::vector<int>& ids, int nparts, int first, double eps,
+             std::vector<int>& part, int depth) {
+    if (nparts <= 1 || g.nv == 0) {
+        for (int id : ids) part[id] = first;
+        return;
+    }
+    int k0 = nparts / 2, k1 = nparts - k0;
+    std::vector<int> side = bisect(g, (double)k0 / nparts, eps);
+    std::vector<int> v0, v1, loc0(g.nv, -1), loc1(g.nv, -1);
+    for (int v = 0; v < g.nv; ++v) {
+        if (side[v] == 0) { loc0[v] = (int)v0.size(); v0.push_back(v); }
+        else { loc1[v] = (int)v1.size(); v1.push_back(v); }
+    }
+    Graph g0 = induced(g, v0, loc0), g1 = induced(g, v1, loc1);
+    std::vector<int> ids0(v0.size()), ids1(v1.size());
+    for (size_t i = 0; i < v0.size(); ++i) ids0[i] = ids[v0[i]];
+    for (size_t i = 0; i < v1.size(); ++i) ids1[i] = ids[v1[i]];
+    // the two halves are independent: recurse concurrently near the top
+    if (depth < 3 && g.nv > 20000) {
+        std::thread t([&] { recurse(g0, ids0, k0, first, eps, part, depth + 1); });
+        recurse(g1, ids1, k1, first + k0, eps, part, depth + 1);
+        t.join();
+    } else {
+        recurse(g0, ids0, k0, first, eps, part, depth + 1);
+        recurse(g1, ids1, k1, first + k0, eps, part, depth + 1);
+    }
+}
+
+}  // namespace
+
+std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor) {
+    if (nparts <= 1) return std::vector<int>(A.n, 0);
+    if (nparts > A.n) throw std::runtime_error("partitioning failed: more parts than rows");
+    if (ufactor <= 0) ufactor = 30;
+    Graph g = graph_from_csr(A);
+    int depth = (int)std::ceil(std::log2((double)nparts));
+    double eps = std::pow(1.0 + ufactor / 1000.0, 1.0 / std::max(1, depth)) - 1.0;
+    std::vector<int> part(A.n, -1), ids(A.n);
+    std::iota(ids.begin(), ids.end(), 0);
+    recurse(g, ids, nparts, 0, eps, part, 0);
+    std::vector<int> count(nparts, 0);
+    for (int p : part) {
+        if (p < 0 || p >= nparts) throw std::runtime_error("partitioning failed");
+        count[p]++;
+    }
+    for (int c : count)
+        if (c == 0) throw std::runtime_error("partitioning failed: empty subdomain");
+    return part;
+}
+
+void build_sets(const HostCsr& A, const std::vector<int>& part, int nparts, int overlap,
+                std::vector<std::vector<int>>& owned, std::vector<std::vector<int>>& dofs) {
+    int n = A.n;
+    owned.assign(nparts, {});
+    for (int i = 0; i < n; ++i) owned[part[i]].push_back(i);
+    dofs.assign(nparts, {});
+    std::vector<int> mark(std::max(n, A.ncols), -1);
+    for (int s = 0; s < nparts; ++s) {
+        std::vector<int> cur = owned[s], frontier = owned[s];
+        for (int i : cur) mark[i] = s;
+        // Layer l adds every column of every row already in the set that is
+        // not yet in it; only the previous layer can contribute new columns.
+        for (int layer = 0; layer < overlap; ++layer) {
+            std::vector<int> add;
+            for (int i : frontier)
+                for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                    int c = A.ci[k];
+                    if (mark[c] != s) { mark[c] = s; add.push_back(c); }
+                }
+            cur.insert(cur.end(), add.begin(), add.end());
+            frontier.swap(add);
+            if (frontier.empty()) break;
+        }
+        std::sort(cur.begin(), cur.end());
+        dofs[s] = std::move(cur);
+    }
+}
+
+HostCsr extract_submatrix(const HostCsr& A, const std::vector<int>& idx) {
+    int m = (int)idx.size();
+    thread_local std::vector<int> inv;
+    if ((int)inv.size() < std::max(A.n, A.ncols)) inv.assign(std::max(A.n, A.ncols), -1);
+    for (int i = 0; i < m; ++i) inv[idx[i]] = i;
+    HostCsr s;
+    s.n = m;
+    s.ncols = m;
+    s.rp.assign(m + 1, 0);
+    for (int li = 0; li < m; ++li) {
+        int gi = idx[li];
+        s.rp[li] = (int)s.ci.size();
+        for (int k = A.rp[gi]; k < A.rp[gi + 1]; ++k) {
+            int lj = inv[A.ci[k]];
+            if (lj >= 0) { s.ci.push_back(lj); s.v.push_back(A.v[k]); }
+        }
+    }
+    s.rp[m] = (int)s.ci.size();
+    for (int i = 0; i < m; ++i) inv[idx[i]] = -1;
+    return s;
+}
+
+std::vector<int> diag_positions(const HostCsr& A) {
+    std::vector<int> d(A.n, -1);
+    for (int i = 0; i < A.n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] == i) { d[i] = k; break; }
+    return d;
+}
+
+std::vector<int> lower_levels(const HostCsr& A) {
+    std::vector<int> lev(A.n, 0);
+    for (int i = 0; i < A.n; ++i) {
+        int l = 0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            int c = A.ci[k];
+            if (c >= i) break;  // sorted columns
+            l = std::max(l, lev[c] + 1);
+        }
+        lev[i] = l;
+    }
+    return lev;
+}
+
+std::vector<int> upper_levels(const HostCsr& A) {
+    std::vector<int> lev(A.n, 0);
+    std::vector<int> d = diag_positions(A);
+    for (int i = A.n - 1; i >= 0; --i) {
+        int l = 0;
+        int k0 = d[i] >= 0 ? d[i] + 1 : A.rp[i + 1];
+        for (int k = k0; k < A.rp[i + 1]; ++k) l = std::max(l, lev[A.ci[k]] + 1);
+        lev[i] = l;
+    }
+    return lev;
+}
+
+}  // namespace sg

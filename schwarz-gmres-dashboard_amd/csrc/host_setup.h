@@ -1,0 +1,49 @@
+// Host-side setup of the Schwarz preconditioners: graph partition, overlap
+// sets, subdomain extraction, ILU(0) level schedules and the packed layouts
+// the HIP kernels consume.  Plain C++ (built with g++), integer work only,
+// except for copying matrix values.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace sg {
+
+struct HostCsr {
+    int n = 0;      // rows
+    int ncols = 0;  // columns
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+    int nnz() const { return rp.empty() ? 0 : rp[n]; }
+};
+
+// Balanced k-way partition of the graph of A (self loops ignored, pattern
+// symmetrised).  Multilevel recursive bisection: heavy-edge coarsening, greedy
+// graph-growing initial bisection and boundary Fiduccia-Mattheyses refinement.
+// Deterministic.  Stands in for METIS_PartGraphKway
+// (reference src/decomp/domain_decomp.h:23-58); nparts <= 1 gives all zeros as
+// there (:24-26).  ufactor is the allowed imbalance in 1/1000 (30 = 1.03).
+std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor);
+
+// Owned and overlapped index sets exactly as build_decomposition
+// (reference src/decomp/domain_decomp.h:61-93): owned_dofs[s] ascending,
+// subdomain_dofs[s] = owned_dofs[s] grown by `overlap` breadth-first layers
+// over the CSR columns of A (explicit zeros included), sorted ascending.
+void build_sets(const HostCsr& A, const std::vector<int>& part, int nparts, int overlap,
+                std::vector<std::vector<int>>& owned, std::vector<std::vector<int>>& dofs);
+
+// Local CSR of A[idx, idx] in CSR order (reference
+// src/core/sparse_matrix.h:53-83).
+HostCsr extract_submatrix(const HostCsr& A, const std::vector<int>& idx);
+
+// Position of the diagonal in each row, -1 when absent.
+std::vector<int> diag_positions(const HostCsr& A);
+
+// Level of each row for the forward (unit-L) sweep: 0 without strictly-lower
+// entries, else 1 + max over them.  The IKJ ILU(0) factorisation has the same
+// dependencies, so rows of one level can be factored in parallel.
+std::vector<int> lower_levels(const HostCsr& A);
+// Level of each row for the backward (U) sweep (dependencies on columns > row).
+std::vector<int> upper_levels(const HostCsr& A);
+
+}  // namespace sg

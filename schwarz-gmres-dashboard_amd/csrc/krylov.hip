@@ -1,0 +1,592 @@
+// Restarted GMRES(m) (left preconditioning) and FGMRES(m) (flexible right
+// preconditioning) with every scalar on the device.
+//
+// Semantics follow src/core/gmres.cpp:34-160 and src/core/fgmres.cpp:28-172:
+// absolute test on beta at each restart and on |g_{j+1}| after each Arnoldi
+// step, breakdown when h_{j+1,j} < 1e-30 (no rotation for that column),
+// tangent-form Givens rotations (gmres.cpp:11-23), back substitution and the
+// x update in the reference's order, max_iter bounding cycles with the total
+// step count checked after each cycle.  Gram-Schmidt runs either as MGS in the
+// reference's order (one fused kernel per projection) or as CGS2 (three fused
+// passes per step).  All reductions use a fixed tree, so a solve is
+// deterministic run to run.
+//
+// The host launches a whole restart cycle without reading anything back; each
+// kernel of a step checks a device flag and returns at once when the cycle has
+// ended (converged or broke down).  The host may run `lookahead` steps ahead of
+// the device and reads the flag asynchronously to stop launching, and
+// synchronises once per cycle.
+#include "sg_internal.h"
+
+#include <chrono>
+#include <cmath>
+
+namespace sg {
+
+sg_context* precond_ctx(const sg_precond* M);
+int precond_n(const sg_precond* M);
+bool precond_ready(const sg_precond* M);
+
+constexpr int kT = 256;           // threads per reduction block
+constexpr int kE = 16;            // elements per thread in the CGS2 passes
+constexpr int kMaxBlocks = 4096;  // partial-sum slots per column
+constexpr int kMaxCols = 128;     // restart + 1
+
+struct KState {
+    int active;     // current cycle continues
+    int k;          // Arnoldi columns completed in this cycle
+    int converged;
+    int total;      // Arnoldi steps over all cycles (GMRESResult::iterations)
+    int est_conv;   // rtol mode: the estimate met the internal tolerance
+    int first;      // no restart done yet
+    int method;     // 0 left GMRES, 1 FGMRES
+    int m;
+    double tol;     // test threshold on beta and |g|
+    double rtol;    // > 0: true-residual mode
+    double bnorm;
+    double rel_true;
+    double scale;   // divisor (GMRES) or multiplier (FGMRES) for the next basis vector
+    double res;
+};
+
+struct Work {
+    double* H;     // (m+1) x m, column-major H[i + j*(m+1)]
+    double* g;     // m+1
+    double* cs;    // m
+    double* sn;    // m
+    double* y;     // m
+    double* h1;    // kMaxCols (CGS2 pass 1)
+    double* h2;    // kMaxCols (CGS2 pass 2)
+    double* hist;  // residual history
+    double* part;  // kMaxCols * kMaxBlocks
+    unsigned* ticket;
+    KState* st;
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Givens rotation, tangent form (src/core/gmres.cpp:11-23, fgmres.cpp:12-20).
+__device__ inline void givens_compute(double a, double b, double& c, double& s) {
+    if (fabs(b) < 1e-30) {
+        c = 1.0;
+        s = 0.0;
+    } else if (fabs(b) > fabs(a)) {
+        double t = -a / b;
+        s = 1.0 / sqrt(1.0 + t * t);
+        c = s * t;
+    } else {
+        double t = -b / a;
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = c * t;
+    }
+}
+
+__device__ inline void givens_apply(double c, double s, double& h1, double& h2) {
+    double t1 = c * h1 - s * h2;
+    double t2 = s * h1 + c * h2;
+    h1 = t1;
+    h2 = t2;
+}
+
+// End of Arnoldi step j once h_{j+1,j} is known (gmres.cpp:99-127).
+__device__ void step_epilogue(const Work& W, int j, double hn) {
+    KState* st = W.st;
+    const int m = st->m;
+    double* H = W.H;
+    H[(j + 1) + j * (m + 1)] = hn;
+    if (hn < 1e-30) {  // breakdown: column j is used unrotated
+        st->k = j + 1;
+        st->active = 0;
+        return;
+    }
+    st->scale = st->method == 0 ? hn : 1.0 / hn;
+    for (int i = 0; i < j; ++i)
+        givens_apply(W.cs[i], W.sn[i], H[i + j * (m + 1)], H[(i + 1) + j * (m + 1)]);
+    double c, s;
+    givens_compute(H[j + j * (m + 1)], H[(j + 1) + j * (m + 1)], c, s);
+    W.cs[j] = c;
+    W.sn[j] = s;
+    givens_apply(c, s, H[j + j * (m + 1)], H[(j + 1) + j * (m + 1)]);
+    givens_apply(c, s, W.g[j], W.g[j + 1]);
+    double res = fabs(W.g[j + 1]);
+    W.hist[st->total] = res;
+    st->total += 1;
+    st->res = res;
+    st->k = j + 1;
+    if (res < st->tol) {
+        if (st->rtol > 0.0) st->est_conv = 1;
+        else st->converged = 1;
+        st->active = 0;
+    }
+}
+
+// Block sums of ncols partial values in red[wave][col] -> partials, then the
+// last block reduces the partials of every column in fixed order into out[].
+__device__ bool finish_multi(const Work& W, int ncols, double (*red)[kMaxCols], double* out) {
+    const int tid = threadIdx.x;
+    __shared__ bool last;
+    __syncthreads();
+    for (int i = tid; i < ncols; i += kT) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kT / 64; ++w) s += red[w][i];
+        W.part[i * kMaxBlocks + blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(W.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int i = 0; i < ncols; ++i) {
+        double s = 0.0;
+        for (int b = tid; b < (int)gridDim.x; b += kT) s += W.part[i * kMaxBlocks + b];
+        s = wave_sum(s);
+        if (lane == 0) red[wv][i] = s;
+    }
+    __syncthreads();
+    for (int i = tid; i < ncols; i += kT) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kT / 64; ++w) s += red[w][i];
+        out[i] = s;
+    }
+    if (tid == 0) *W.ticket = 0u;
+    __syncthreads();
+    return true;
+}
+
+// CGS2 pass.  MODE 0: h1 = V^T w.  MODE 1: w -= V h1, then h2 = V^T w and
+// H[:,j] = h1 + h2.  MODE 2: w -= V h2, then h_{j+1,j} = ||w|| and the step
+// epilogue.  Each thread keeps kE elements of w in registers.
+template <int MODE>
+__global__ void __launch_bounds__(kT)
+k_cgs(int n, int ncols, const double* __restrict__ V, size_t ld, double* __restrict__ w,
+      Work W, int j) {
+    if (!W.st->active) return;
+    __shared__ double red[kT / 64][kMaxCols];
+    __shared__ double hs[kMaxCols];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int base = blockIdx.x * (kT * kE) + tid;
+    double wr[kE];
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+        int e = base + r * kT;
+        wr[r] = e < n ? w[e] : 0.0;
+    }
+    if (MODE >= 1) {
+        const double* h = MODE == 1 ? W.h1 : W.h2;
+        for (int i = tid; i < ncols; i += kT) hs[i] = h[i];
+        __syncthreads();
+        for (int i = 0; i < ncols; ++i) {
+            const double hi = hs[i];
+            const double* Vi = V + (size_t)i * ld;
+#pragma unroll
+            for (int r = 0; r < kE; ++r) {
+                int e = base + r * kT;
+                if (e < n) wr[r] -= hi * Vi[e];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kE; ++r) {
+            int e = base + r * kT;
+            if (e < n) w[e] = wr[r];
+        }
+    }
+    if (MODE <= 1) {
+        for (int i = 0; i < ncols; ++i) {
+            const double* Vi = V + (size_t)i * ld;
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < kE; ++r) {
+                int e = base + r * kT;
+                if (e < n) s += Vi[e] * wr[r];
+            }
+            s = wave_sum(s);
+            if (lane == 0) red[wv][i] = s;
+        }
+        double* out = MODE == 0 ? W.h1 : W.h2;
+        if (!finish_multi(W, ncols, red, out)) return;
+        if (MODE == 1) {
+            const int m = W.st->m;
+            for (int i = tid; i < ncols; i += kT) W.H[i + j * (m + 1)] = W.h1[i] + W.h2[i];
+        }
+    } else {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < kE; ++r) s += wr[r] * wr[r];
+        s = wave_sum(s);
+        if (lane == 0) red[wv][0] = s;
+        if (!finish_multi(W, 1, red, W.h1 + kMaxCols - 1)) return;
+        if (tid == 0) step_epilogue(W, j, sqrt(W.h1[kMaxCols - 1]));
+    }
+}
+
+// MGS projection i of step j (gmres.cpp:93-97): w -= h_{i-1} V_{i-1} (when
+// i > 0), then H[i,j] = w . V_i; with Vi == null the norm and the epilogue.
+__global__ void __launch_bounds__(kT)
+k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
+      double* __restrict__ w, Work W, int i, int j) {
+    if (!W.st->active) return;
+    __shared__ double red[kT / 64][kMaxCols];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m = W.st->m;
+    const double hprev = Vprev ? W.H[(i - 1) + j * (m + 1)] : 0.0;
+    double s = 0.0;
+    for (int e = blockIdx.x * kT + tid; e < n; e += gridDim.x * kT) {
+        double we = w[e];
+        if (Vprev) {
+            we -= hprev * Vprev[e];
+            w[e] = we;
+        }
+        s += Vi ? we * Vi[e] : we * we;
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wv][0] = s;
+    double* out = W.h1 + kMaxCols - 1;
+    if (!finish_multi(W, 1, red, out)) return;
+    if (tid == 0) {
+        if (Vi) W.H[i + j * (m + 1)] = *out;
+        else step_epilogue(W, j, sqrt(*out));
+    }
+}
+
+// Norms at a restart: u.u and, in true-residual mode, r.r; then the
+// beta test and the start of the cycle (gmres.cpp:59-81, fgmres.cpp:54-71).
+__global__ void __launch_bounds__(kT)
+k_restart(int n, const double* __restrict__ u, const double* __restrict__ r, Work W) {
+    __shared__ double red[kT / 64][kMaxCols];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double su = 0.0, sr = 0.0;
+    for (int e = blockIdx.x * kT + tid; e < n; e += gridDim.x * kT) {
+        double a = u[e];
+        su += a * a;
+        if (r) {
+            double b = r[e];
+            sr += b * b;
+        }
+    }
+    su = wave_sum(su);
+    sr = wave_sum(sr);
+    if (lane == 0) {
+        red[wv][0] = su;
+        red[wv][1] = sr;
+    }
+    double* out = W.h1 + kMaxCols - 2;
+    if (!finish_multi(W, 2, red, out)) return;
+    if (tid != 0) return;
+    KState* st = W.st;
+    const double beta = sqrt(out[0]);
+    const double rn = r ? sqrt(out[1]) : beta;
+    st->k = 0;
+    if (st->rtol > 0.0) {
+        const double rel = rn / st->bnorm;
+        st->rel_true = rel;
+        if (rel <= st->rtol || !(beta > 0.0)) {
+            st->converged = rel <= st->rtol;
+            st->active = 0;
+            return;
+        }
+        if (st->first) {
+            st->tol = st->method == 0 ? st->rtol * beta : st->rtol * st->bnorm;
+        } else if (st->est_conv) {
+            double f = 0.5 * st->rtol / rel;
+            st->tol *= f < 1e-3 ? 1e-3 : (f > 0.9 ? 0.9 : f);
+        }
+        st->est_conv = 0;
+    } else if (beta < st->tol) {
+        st->converged = 1;
+        st->active = 0;
+        return;
+    }
+    st->first = 0;
+    st->active = 1;
+    const int m = st->m;
+    W.g[0] = beta;
+    for (int i = 1; i <= m; ++i) W.g[i] = 0.0;
+    st->scale = st->method == 0 ? beta : 1.0 / beta;
+}
+
+// v = u / scale (GMRES, gmres.cpp:76-77,107-108) or u * scale (FGMRES,
+// fgmres.cpp:66-67,97-98).
+__global__ void k_scale_basis(int n, const double* __restrict__ u, double* __restrict__ v,
+                              const KState* __restrict__ st) {
+    if (!st->active) return;
+    const double sc = st->scale;
+    const bool div = st->method == 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        v[e] = div ? u[e] / sc : u[e] * sc;
+}
+
+// Back substitution H y = g (gmres.cpp:131-137), one thread.
+__global__ void k_backsolve(Work W) {
+    const KState* st = W.st;
+    const int k = st->k, m = st->m;
+    for (int i = k - 1; i >= 0; --i) {
+        double yi = W.g[i];
+        for (int l = i + 1; l < k; ++l) yi -= W.H[i + l * (m + 1)] * W.y[l];
+        W.y[i] = yi / W.H[i + i * (m + 1)];
+    }
+}
+
+// x += sum_i y_i B_i in the reference's axpy order (gmres.cpp:140-141).
+__global__ void k_update(int n, double* __restrict__ x, const double* __restrict__ B, size_t ld,
+                         Work W) {
+    const int k = W.st->k;
+    if (k == 0) return;
+    __shared__ double ys[kMaxCols];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) ys[i] = W.y[i];
+    __syncthreads();
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        double xe = x[e];
+        for (int i = 0; i < k; ++i) xe += ys[i] * B[(size_t)i * ld + e];
+        x[e] = xe;
+    }
+}
+
+__global__ void k_init_state(KState* st, int method, int m, double tol, double rtol) {
+    st->active = 0;
+    st->k = 0;
+    st->converged = 0;
+    st->total = 0;
+    st->est_conv = 0;
+    st->first = 1;
+    st->method = method;
+    st->m = m;
+    st->tol = tol;
+    st->rtol = rtol;
+    st->rel_true = -1.0;
+    st->scale = 1.0;
+    st->res = 0.0;
+}
+
+__global__ void k_rel_from_norm(KState* st, const double* rn) {
+    st->rel_true = *rn / st->bnorm;
+}
+
+static int nblocks_for(int n, int per_thread) {
+    long long g = ((long long)n + (long long)kT * per_thread - 1) / ((long long)kT * per_thread);
+    if (g < 1) g = 1;
+    if (g > kMaxBlocks) g = kMaxBlocks;
+    return (int)g;
+}
+
+struct SolveWs {
+    DevBuf<double> vecs;  // V, Z, r, z, w, t
+    DevBuf<double> small;
+    DevBuf<unsigned> ticket;
+    DevBuf<KState> st;
+    int* pinned = nullptr;
+    ~SolveWs() {
+        if (pinned) (void)hipHostFree(pinned);
+    }
+};
+
+}  // namespace sg
+
+using namespace sg;
+
+extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                              const sg_solve_params* prm, sg_solve_result* res, double* hist,
+                              int hist_cap) {
+    if (!A || !prm || !res || (A->nrows && (!d_b || !d_x))) return SG_E_ARG;
+    sg_context* ctx = A->ctx;
+    return run_guarded(ctx, [&] {
+        if (A->nrows != A->ncols) fail(SG_E_ARG, "GMRES needs a square matrix");
+        if (M && !precond_ready(M)) fail(SG_E_STATE, "preconditioner used before setup");
+        if (M && precond_n(M) != A->nrows) fail(SG_E_ARG, "preconditioner size mismatch");
+        const int n = A->nrows;
+        const int m = prm->restart;
+        const int method = prm->method == SG_FGMRES ? 1 : 0;
+        if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
+        if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
+        const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
+        if (cgs2 && (long long)n > (long long)kMaxBlocks * kT * kE)
+            fail(SG_E_ARG, "CGS2 supports n <= 16777216; use MGS");
+        const int L = prm->lookahead > 0 ? prm->lookahead : 2;
+        hipStream_t s = ctx->stream;
+        SG_HIP(hipSetDevice(ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+
+        const size_t ld = ((size_t)n + 63) / 64 * 64;
+        const int nz = (method && M) ? m : 0;
+        const int nvec = (m + 1) + nz + 4;
+        SolveWs ws;
+        ws.vecs.alloc(ld * nvec);
+        double* V = ws.vecs.p;
+        double* Z = nz ? V + ld * (m + 1) : nullptr;
+        double* r = V + ld * ((m + 1) + nz);
+        double* z = r + ld;
+        double* w = z + ld;
+        double* t = w + ld;
+        const int hcap = prm->max_iter + m + 2;
+        size_t nsmall = (size_t)(m + 1) * m + (m + 1) + 3 * m + 2 * kMaxCols + hcap +
+                        (size_t)kMaxCols * kMaxBlocks;
+        ws.small.alloc(nsmall);
+        ws.ticket.alloc(1);
+        ws.st.alloc(1);
+        SG_HIP(hipMemsetAsync(ws.ticket.p, 0, sizeof(unsigned), s));
+        SG_HIP(hipHostMalloc((void**)&ws.pinned, sizeof(int) * (m + 1), hipHostMallocDefault));
+        Work W;
+        W.H = ws.small.p;
+        W.g = W.H + (size_t)(m + 1) * m;
+        W.cs = W.g + (m + 1);
+        W.sn = W.cs + m;
+        W.y = W.sn + m;
+        W.h1 = W.y + m;
+        W.h2 = W.h1 + kMaxCols;
+        W.hist = W.h2 + kMaxCols;
+        W.part = W.hist + hcap;
+        W.ticket = ws.ticket.p;
+        W.st = ws.st.p;
+        KState* st = ws.st.p;
+        const int* active = &st->active;
+
+        double tol = prm->rtol > 0.0 ? 0.0 : prm->tol_abs;
+        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, st, method, m, tol, prm->rtol);
+        SG_HIP(hipGetLastError());
+        // ||b|| for the relative test and the report
+        dot_launch(ctx, n, d_b, d_b, ctx->d_result);
+        double bb = 0.0;
+        SG_HIP(hipMemcpyAsync(&bb, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+        SG_HIP(hipStreamSynchronize(s));
+        const double bnorm = std::sqrt(bb);
+        SG_HIP(hipMemcpyAsync(&st->bnorm, &bnorm, sizeof(double), hipMemcpyHostToDevice, s));
+
+        const int gred = nblocks_for(n, 4);
+        const int gcgs = nblocks_for(n, kE);
+        const int gew = nblocks_for(n, 4);
+        std::vector<hipEvent_t> evs(m);
+        for (int j = 0; j < m; ++j) evs[j] = ctx->get_event();
+        struct EvBack {
+            sg_context* c;
+            std::vector<hipEvent_t>& e;
+            ~EvBack() {
+                for (auto x : e) c->event_pool.push_back(x);
+            }
+        } evback{ctx, evs};
+
+        const bool rmode = prm->rtol > 0.0;
+        KState hst{};
+        int cycles = 0, applies = 0;
+        bool verify_only = false;
+        for (int cycle = 0;; ++cycle) {
+            if (!verify_only && cycle >= prm->max_iter) {
+                if (rmode && hst.est_conv) verify_only = true;
+                else break;
+            }
+            // r = b - A x
+            {
+                ProfScope ps(ctx, SG_K_UPDATE);
+                spmv_launch(A, -1.0, d_x, 1.0, d_b, r, nullptr, s);
+            }
+            const double* u = r;
+            if (method == 0 && M) {
+                precond_apply_launch(M, r, z, nullptr);
+                ++applies;
+                u = z;
+            }
+            {
+                ProfScope ps(ctx, SG_K_UPDATE);
+                hipLaunchKernelGGL(k_restart, dim3(gred), dim3(kT), 0, s, n, u,
+                                   (rmode && u != r) ? r : nullptr, W);
+                if (!verify_only)
+                    hipLaunchKernelGGL(k_scale_basis, dim3(gew), dim3(kT), 0, s, n, u, V, st);
+                SG_HIP(hipGetLastError());
+            }
+            for (int j = 0; j < m && !verify_only; ++j) {
+                if (j >= L) {
+                    SG_HIP(hipEventSynchronize(evs[j - L]));
+                    if (!ws.pinned[j - L]) break;
+                }
+                double* Vj = V + (size_t)j * ld;
+                if (method == 0) {
+                    if (M) {
+                        spmv_launch(A, 1.0, Vj, 0.0, nullptr, t, active, s);
+                        precond_apply_launch(M, t, w, active);
+                        ++applies;
+                    } else {
+                        spmv_launch(A, 1.0, Vj, 0.0, nullptr, w, active, s);
+                    }
+                } else {
+                    double* Zj = M ? Z + (size_t)j * ld : Vj;
+                    if (M) {
+                        precond_apply_launch(M, Vj, Zj, active);
+                        ++applies;
+                    }
+                    spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, active, s);
+                }
+                {
+                    ProfScope ps(ctx, SG_K_ORTHO);
+                    if (cgs2) {
+                        hipLaunchKernelGGL(k_cgs<0>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_cgs<1>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_cgs<2>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                    } else {
+                        for (int i = 0; i <= j; ++i)
+                            hipLaunchKernelGGL(k_mgs, dim3(gred), dim3(kT), 0, s, n,
+                                               i ? V + (size_t)(i - 1) * ld : nullptr,
+                                               V + (size_t)i * ld, w, W, i, j);
+                        hipLaunchKernelGGL(k_mgs, dim3(gred), dim3(kT), 0, s, n,
+                                           V + (size_t)j * ld, nullptr, w, W, j + 1, j);
+                    }
+                    SG_HIP(hipGetLastError());
+                }
+                if (j + 1 < m) {
+                    ProfScope ps(ctx, SG_K_UPDATE);
+                    hipLaunchKernelGGL(k_scale_basis, dim3(gew), dim3(kT), 0, s, n, w,
+                                       V + (size_t)(j + 1) * ld, st);
+                }
+                SG_HIP(hipMemcpyAsync(&ws.pinned[j], active, sizeof(int), hipMemcpyDeviceToHost, s));
+                SG_HIP(hipEventRecord(evs[j], s));
+            }
+            if (!verify_only) {
+                ProfScope ps(ctx, SG_K_UPDATE);
+                hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(1), 0, s, W);
+                hipLaunchKernelGGL(k_update, dim3(gew), dim3(kT), 0, s, n, d_x,
+                                   (method && M) ? Z : V, ld, W);
+                SG_HIP(hipGetLastError());
+            }
+            SG_HIP(hipMemcpyAsync(&hst, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+            SG_HIP(hipStreamSynchronize(s));
+            ++cycles;
+            if (hst.converged || verify_only) break;
+            if (rmode && hst.est_conv) {
+                if (hst.total >= prm->max_iter) verify_only = true;
+                continue;  // the next restart checks the true residual
+            }
+            if (hst.total >= prm->max_iter) break;  // gmres.cpp:146-147
+        }
+        SG_HIP(hipStreamSynchronize(s));
+        auto t1 = std::chrono::steady_clock::now();
+        ctx->collect_profiles();
+
+        res->iterations = hst.total;
+        res->converged = hst.converged;
+        res->cycles = cycles;
+        res->precond_applies = applies;
+        res->time_solve_s = std::chrono::duration<double>(t1 - t0).count();
+        std::vector<double> h(hst.total);
+        if (hst.total)
+            SG_HIP(hipMemcpy(h.data(), W.hist, sizeof(double) * hst.total, hipMemcpyDeviceToHost));
+        res->residual_norm = hst.total ? h.back() : 0.0;  // gmres.cpp:150-155
+        if (hist)
+            for (int i = 0; i < hst.total && i < hist_cap; ++i) hist[i] = h[i];
+        // FP64 true residual at exit (reported; inside the solve in rtol mode)
+        if (prm->rtol > 0.0 && hst.converged) {
+            res->rel_true_residual = hst.rel_true;
+        } else {
+            spmv_launch(A, -1.0, d_x, 1.0, d_b, r, nullptr, s);
+            dot_launch(ctx, n, r, r, ctx->d_result);
+            double rr = 0.0;
+            SG_HIP(hipMemcpyAsync(&rr, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+            SG_HIP(hipStreamSynchronize(s));
+            res->rel_true_residual = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+        }
+    });
+}

@@ -1,0 +1,121 @@
+// Internal types of libsgamd.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sg_capi.h"
+#include "host_setup.h"
+
+namespace sg {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kRedBlocks = 1024;   // fixed grid of the deterministic reductions
+constexpr int kRedThreads = 256;
+constexpr int kMaxRedCols = 128;   // widest multi-dot (restart + 1)
+
+struct Error {
+    sg_status st;
+    std::string msg;
+};
+
+// Throws Error when a HIP call fails; the C ABI boundary converts it.
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define SG_HIP(call) ::sg::hip_check((call), #call, __FILE__, __LINE__)
+[[noreturn]] void fail(sg_status st, const std::string& msg);
+// Runs f; exceptions become a status plus the context's last-error message.
+sg_status run_guarded(sg_context* ctx, const std::function<void()>& f);
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) SG_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (n) SG_HIP(hipMemcpyAsync(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Prof {
+    long long launches = 0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+}  // namespace sg
+
+struct sg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cu = 256;
+    int max_lds = 65536;  // bytes of LDS one workgroup may allocate
+    // deterministic reduction scratch: partials[kMaxRedCols][kRedBlocks], ticket, result
+    double* d_partials = nullptr;
+    unsigned* d_ticket = nullptr;
+    double* d_result = nullptr;  // kMaxRedCols doubles
+    double* h_pinned = nullptr;  // pinned staging for small reads
+    bool profiling = false;
+    sg::Prof prof[SG_K_COUNT];
+    std::vector<hipEvent_t> event_pool;
+
+    hipEvent_t get_event();
+    void collect_profiles();  // stream must be synchronised
+};
+
+struct sg_csr {
+    sg_context* ctx = nullptr;
+    int nrows = 0, ncols = 0, nnz = 0;
+    sg::HostCsr host;
+    sg::DevBuf<int> rp, ci;
+    sg::DevBuf<double> v;
+    // row blocks for the LDS-staged SpMV: block b covers rows [blk[b], blk[b+1])
+    int nblk = 0;
+    sg::DevBuf<int> blk;
+};
+
+namespace sg {
+
+// RAII timing of one launch family on the context stream.
+struct ProfScope {
+    sg_context* ctx;
+    int fam;
+    hipEvent_t a = nullptr;
+    ProfScope(sg_context* c, int f);
+    ~ProfScope();
+};
+
+// ---- SpMV (spmv.hip) ---------------------------------------------------------
+// y = alpha*A*x + beta*y; guard (device int, may be null): skip when *guard == 0.
+// y0 may be null (then y0 = y); beta == 0 never reads y0.
+void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,
+                 double* y, const int* guard, hipStream_t s);
+void spmv_prepare(sg_csr* A);
+
+// ---- reductions / BLAS-1 (blas.hip) -------------------------------------------
+// result[0] = dot(a, b) on the device (deterministic tree), stream-ordered.
+void dot_launch(sg_context* ctx, int n, const double* a, const double* b, double* result);
+
+// ---- preconditioner apply (precond.hip) ----------------------------------------
+void precond_apply_launch(sg_precond* M, const double* x, double* y, const int* guard);
+
+}  // namespace sg

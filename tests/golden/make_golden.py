@@ -1,0 +1,153 @@
+"""Write the golden fixtures under tests/golden/ from the reference's own code.
+
+Runs oracle/_ref/ref_harness (built by ``make -C oracle ref`` from the
+reference's METIS-free sources: core/gmres.cpp, core/fgmres.cpp,
+precond/ilu0.h, precond/jacobi.h, precond/coarse_solve.h, mesh/poisson.h,
+io/metrics_logger.h, core/sparse_matrix.h) and stores inputs and outputs as
+small JSON fixtures.  Subdomain index sets for the RAS/ASM fixtures are built
+here by a brute-force breadth-first search over the dense adjacency of the
+grid, independent of the oracle.
+
+Usage:  python tests/golden/make_golden.py      (needs /root/reference)
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def run(*args, stdin=None):
+    out = subprocess.run([HARNESS, *map(str, args)], input=stdin, capture_output=True,
+                         text=True, check=True)
+    return json.loads(out.stdout)
+
+
+def dump(name, obj):
+    with open(os.path.join(HERE, name), "w") as f:
+        json.dump(obj, f, separators=(",", ":"))
+
+
+def grid_adjacency(spec):
+    """Dense boolean adjacency (pattern incl. diagonal) of the Poisson grid."""
+    dims = [int(t) for t in spec.split(":")[1:]]
+    if len(dims) == 2:
+        nx, ny = dims
+        n = nx * ny
+        A = np.zeros((n, n), dtype=bool)
+        for iy in range(ny):
+            for ix in range(nx):
+                i = iy * nx + ix
+                A[i, i] = True
+                for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                    jx, jy = ix + dx, iy + dy
+                    if 0 <= jx < nx and 0 <= jy < ny:
+                        A[i, jy * nx + jx] = True
+        return A
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    A = np.zeros((n, n), dtype=bool)
+    for iz in range(nz):
+        for iy in range(ny):
+            for ix in range(nx):
+                i = (iz * ny + iy) * nx + ix
+                A[i, i] = True
+                for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    jx, jy, jz = ix + d[0], iy + d[1], iz + d[2]
+                    if 0 <= jx < nx and 0 <= jy < ny and 0 <= jz < nz:
+                        A[i, (jz * ny + jy) * nx + jx] = True
+    return A
+
+
+def bfs_sets(adj, part, nparts, overlap):
+    sets = []
+    for s in range(nparts):
+        cur = part == s
+        for _ in range(overlap):
+            cur = cur | adj[cur].any(axis=0)
+        sets.append(np.flatnonzero(cur))
+    return sets
+
+
+def blocks_case(spec, part, nparts, overlap, seed):
+    adj = grid_adjacency(spec)
+    n = adj.shape[0]
+    sets = bfs_sets(adj, part, nparts, overlap)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, n)
+    lines = [f"{spec} {nparts}"]
+    for s, d in enumerate(sets):
+        lines.append(str(len(d)))
+        lines.append(" ".join(map(str, d)))
+        lines.append(" ".join("1" if part[i] == s else "0" for i in d))
+    lines.append(" ".join(repr(float(t)) for t in x))
+    out = run("blocks", stdin="\n".join(lines) + "\n")
+    return {"grid": spec, "nparts": nparts, "overlap": overlap, "part": part.tolist(),
+            "sets": [d.tolist() for d in sets], "x": x.tolist(), "ras": out["ras"],
+            "asm": out["asm"]}
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the harness first: make -C oracle ref")
+
+    # ILU(0) factors and one apply (src/precond/ilu0.h).
+    ilu = {spec: run("ilu0", spec) for spec in ("2d:8:8", "2d:16:16", "3d:6:6:6", "3d:5:4:3")}
+    dump("ilu0.json", ilu)
+
+    # GMRES / FGMRES iteration counts and residual histories.
+    cases = [
+        ("gmres", "ilu0", "2d:100:100", "ones", 30, 1000, 1e-10),
+        ("gmres", "ilu0", "2d:150:150", "ones", 30, 1000, 1e-10),
+        ("gmres", "ilu0", "2d:256:256", "uniform:1", 30, 2000, -1e-8),
+        ("fgmres", "ilu0", "2d:256:256", "uniform:1", 30, 2000, -1e-8),
+        ("gmres", "none", "2d:20:20", "ones", 30, 1000, 1e-10),
+        ("gmres", "jacobi", "2d:20:20", "uniform:3", 25, 1000, 1e-10),
+        ("fgmres", "none", "2d:20:20", "uniform:4", 30, 1000, 1e-10),
+        ("fgmres", "jacobi", "3d:8:8:8", "uniform:5", 20, 1000, 1e-10),
+        ("gmres", "ilu0", "3d:16:16:16", "uniform:6", 50, 2000, -1e-8),
+        ("fgmres", "ilu0", "3d:16:16:16", "uniform:6", 50, 2000, -1e-8),
+        ("gmres", "ilu0", "2d:5:5", "ones", 25, 1000, 1e-10),
+        ("gmres", "none", "2d:30:30", "ones", 10, 7, 1e-14),  # max_iter overshoot
+    ]
+    solves = []
+    for c in cases:
+        r = run("solve", *c)
+        r.update(dict(method=c[0], pc=c[1], grid=c[2], rhs=c[3], restart=c[4],
+                      max_iter=c[5], tol_arg=c[6]))
+        solves.append(r)
+    dump("solve.json", solves)
+
+    # RAS / ASM applies on reference extract_submatrix + ILU0Precond blocks.
+    part20 = np.array([(0 if (i % 20) < 10 else 1) + (0 if (i // 20) < 10 else 2)
+                       for i in range(400)])
+    part3d = np.array([((i % 6) // 3) + 2 * (((i // 36) % 6) // 3) for i in range(216)])
+    rng = np.random.default_rng(11)
+    part_rand = rng.integers(0, 5, 144)
+    blocks = [
+        blocks_case("2d:20:20", part20, 4, 1, 1),
+        blocks_case("2d:20:20", part20, 4, 2, 2),
+        blocks_case("2d:20:20", part20, 4, 0, 3),
+        blocks_case("3d:6:6:6", part3d, 4, 1, 4),
+        blocks_case("2d:12:12", part_rand, 5, 2, 5),
+    ]
+    dump("blocks.json", blocks)
+
+    # Dense coarse LU with partial pivoting (src/precond/coarse_solve.h).
+    dump("lu.json", {"n16": run("lu", 16, 5, 0), "n40": run("lu", 40, 6, 0),
+                     "singular": run("lu", 8, 9, 1)})
+
+    # JSONL records (src/io/metrics_logger.h).
+    lines = subprocess.run([HARNESS, "metrics"], capture_output=True, text=True,
+                           check=True).stdout.splitlines()
+    dump("metrics.json", lines)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,309 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference's golden vectors.
+
+Bars: bit-exact for SpMV, ILU(0) factors and the FP64 sweeps (same operation
+order as the reference, no FMA contraction); iteration counts within +-1 and
+solutions within 1e-6 relative for GMRES/FGMRES (reductions are trees on the
+device, sequential sums on the CPU); 1e-12 relative for FP64 preconditioner
+applies that contain reductions (coarse restriction); 1e-5 for the FP32 modes.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sg():
+    import __graft_entry__
+    return __graft_entry__.load_package()
+
+
+@pytest.fixture(scope="module")
+def ctx(sg):
+    c = sg.Context(0)
+    yield c
+    c.close()
+
+
+def dev(sg, ctx, a):
+    return sg.DeviceArray.from_host(ctx, np.ascontiguousarray(a, np.float64))
+
+
+def test_library_exports_every_declared_symbol(sg):
+    import re
+    import os
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "sg_capi.h")).read()
+    declared = set(re.findall(r"\b(sg_[a-z0-9_]+)\s*\(", hdr))
+    for name in declared:
+        assert hasattr(sg.lib(), name), name
+
+
+@pytest.mark.parametrize("spec", ["2d:33:17", "3d:12:9:7", "2d:256:256"])
+def test_spmv_bitwise(sg, ctx, spec):
+    A = O.grid(spec)
+    rng = np.random.default_rng(1)
+    v = A.v * rng.uniform(0.5, 1.5, A.nnz)  # non-trivial values
+    Ao = O.Csr(A.rp, A.ci, v)
+    x = rng.standard_normal(A.n)
+    dA = sg.Csr(ctx, A.rp, A.ci, v)
+    dx, dy = dev(sg, ctx, x), dev(sg, ctx, np.zeros(A.n))
+    dA.spmv(dx, dy)
+    assert np.array_equal(dy.get(), O.spmv(Ao, x))
+    # r = b - A x form used at every restart (alpha = -1, beta = 1)
+    b = rng.standard_normal(A.n)
+    db = dev(sg, ctx, b)
+    dA.spmv(dx, db, alpha=-1.0, beta=1.0)
+    assert np.array_equal(db.get(), b - O.spmv(Ao, x))
+
+
+def test_spmv_long_rows(sg, ctx):
+    # one dense row longer than the LDS staging capacity + empty rows
+    n = 5000
+    rows = [[0, 1], list(range(n)), [], [3]] + [[i] for i in range(4, n)]
+    rp = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    ci = np.concatenate([np.array(r, np.int32) for r in rows if r])
+    v = np.random.default_rng(2).standard_normal(len(ci))
+    x = np.random.default_rng(3).standard_normal(n)
+    dA = sg.Csr(ctx, rp, ci, v)
+    dx, dy = dev(sg, ctx, x), dev(sg, ctx, np.zeros(n))
+    dA.spmv(dx, dy)
+    ref = O.spmv(O.Csr(rp, ci, v), x)
+    y = dy.get()
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-12)
+    ref_rows = np.ones(n, bool)
+    ref_rows[1] = False
+    assert np.array_equal(y[ref_rows], ref[ref_rows])
+
+
+def test_blas1_closed_forms(sg, ctx):
+    a = dev(sg, ctx, np.arange(1, 101, dtype=float))
+    ones = dev(sg, ctx, np.ones(100))
+    import ctypes as C
+    r = C.c_double()
+    ctx.check(sg.lib().sg_dot(ctx.h, 100, a.ptr, ones.ptr, C.byref(r)))
+    assert r.value == 5050.0
+    ctx.check(sg.lib().sg_nrm2(ctx.h, 100, ones.ptr, C.byref(r)))
+    assert r.value == 10.0
+    x = np.random.default_rng(4).standard_normal(3_000_001)
+    dx = dev(sg, ctx, x)
+    ctx.check(sg.lib().sg_nrm2(ctx.h, x.size, dx.ptr, C.byref(r)))
+    assert abs(r.value - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("spec", ["2d:8:8", "2d:16:16", "3d:6:6:6", "3d:5:4:3"])
+def test_ilu0_factors_bitwise_golden(sg, ctx, gold, spec):
+    g = gold("ilu0.json")[spec]
+    A = O.grid(spec)
+    M = sg.Precond(ctx, "ilu0").setup(sg.Csr(ctx, A.rp, A.ci, A.v))
+    assert np.array_equal(M.ilu_values(0), np.array(g["lu"]))
+    x, y = dev(sg, ctx, g["x"]), dev(sg, ctx, np.zeros(A.n))
+    M.apply(x, y)
+    assert np.array_equal(y.get(), np.array(g["y"]))
+
+
+def test_ilu0_random_values_bitwise(sg, ctx):
+    A = O.poisson3d(20, 18, 16)
+    rng = np.random.default_rng(5)
+    v = A.v * rng.uniform(0.8, 1.2, A.nnz)
+    Ao = O.Csr(A.rp, A.ci, v)
+    lu, diag = O.ilu0(Ao)
+    M = sg.Precond(ctx, "ilu0").setup(sg.Csr(ctx, A.rp, A.ci, v))
+    assert np.array_equal(M.ilu_values(0), lu)
+    x = rng.standard_normal(A.n)
+    dx, dy = dev(sg, ctx, x), dev(sg, ctx, np.zeros(A.n))
+    M.apply(dx, dy)
+    assert np.array_equal(dy.get(), O.ilu0_solve(Ao, lu, diag, x))
+
+
+def test_ras_asm_match_reference_blocks(sg, ctx, gold):
+    for case in gold("blocks.json"):
+        A = O.grid(case["grid"])
+        dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+        x = dev(sg, ctx, case["x"])
+        for kind in ("ras", "asm"):
+            M = sg.Precond(ctx, kind, case["nparts"], case["overlap"],
+                           partition=case["part"]).setup(dA)
+            for s, ref in zip(M.subdomains(), case["sets"]):
+                assert np.array_equal(s, np.array(ref))
+            y = dev(sg, ctx, np.zeros(A.n))
+            M.apply(x, y)
+            assert np.array_equal(y.get(), np.array(case[kind])), (case["grid"], kind)
+
+
+def test_partitioner_properties_and_overlap_sets(sg, ctx):
+    A = O.poisson3d(24, 20, 16)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "ras", nparts=12, overlap=2).setup(dA)
+    part = M.partition()
+    counts = np.bincount(part, minlength=12)
+    assert counts.min() > 0 and counts.max() <= 1.03 * A.n / 12 + 8
+    # deterministic
+    M2 = sg.Precond(ctx, "ras", nparts=12, overlap=2).setup(dA)
+    assert np.array_equal(part, M2.partition())
+    # overlap sets equal the oracle's given the same partition (bit-exact)
+    for s, ref in zip(M.subdomains(), O.overlap_sets(A, part, 12, 2)):
+        assert np.array_equal(s, ref)
+    # RAS apply equals the oracle's on that partition
+    x = np.random.default_rng(6).standard_normal(A.n)
+    y = dev(sg, ctx, np.zeros(A.n))
+    M.apply(dev(sg, ctx, x), y)
+    assert np.array_equal(y.get(), O.Precond("ras", A, 12, 2, part).apply(x))
+
+
+@pytest.mark.parametrize("fine", ["twolevel_ras", "twolevel_asm"])
+@pytest.mark.parametrize("coarse_solve", [0, 1])
+def test_two_level_fp64(sg, ctx, fine, coarse_solve):
+    A = O.poisson2d(40, 36)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, fine, nparts=9, overlap=1, coarse_solve=coarse_solve).setup(dA)
+    part = M.partition()
+    ref = O.Precond(fine, A, 9, 1, part)
+    A0, LU, piv = M.coarse()
+    rA0, rLU, rpiv = ref.coarse()
+    assert np.array_equal(A0, rA0)
+    assert np.array_equal(piv, rpiv)
+    assert np.array_equal(LU, rLU)
+    x = np.random.default_rng(7).standard_normal(A.n)
+    y = dev(sg, ctx, np.zeros(A.n))
+    M.apply(dev(sg, ctx, x), y)
+    want = ref.apply(x)
+    assert np.linalg.norm(y.get() - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("kind", ["ilu0", "ras", "asm", "twolevel_ras"])
+def test_fp32_apply_matches_oracle_mixed_mode(sg, ctx, kind):
+    A = O.poisson3d(16, 16, 12)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, kind, nparts=6, overlap=1, precision="fp32").setup(dA)
+    part = M.partition() if kind != "ilu0" else None
+    ref = O.Precond(kind, A, 6 if part is not None else 1, 1, part, precision=1)
+    x = np.random.default_rng(8).standard_normal(A.n)
+    y = dev(sg, ctx, np.zeros(A.n))
+    M.apply(dev(sg, ctx, x), y)
+    want = ref.apply(x)
+    assert np.linalg.norm(y.get() - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def _rhs(g, A):
+    from rng_ref import splitmix_uniform
+    if g["rhs"] == "ones":
+        return O.spmv(A, np.ones(A.n)), np.ones(A.n)
+    xs = 2.0 * splitmix_uniform(int(g["rhs"].split(":")[1]), A.n) - 1.0
+    return O.spmv(A, xs), xs
+
+
+@pytest.mark.parametrize("idx", range(12))
+def test_solve_matches_reference_golden(sg, ctx, gold, idx):
+    g = gold("solve.json")[idx]
+    A = O.grid(g["grid"])
+    b, xs = _rhs(g, A)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = None if g["pc"] == "none" else sg.Precond(ctx, g["pc"]).setup(dA)
+    db, dx = dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n))
+    r = sg.solve(dA, db, dx, M, method=g["method"], restart=g["restart"],
+                 max_iter=g["max_iter"], tol=g["tol"])
+    assert r["converged"] == bool(g["converged"])
+    assert abs(r["iterations"] - g["iterations"]) <= 1, (r["iterations"], g["iterations"])
+    k = min(len(r["hist"]), len(g["hist"]))
+    h, gh = r["hist"][:k], np.array(g["hist"][:k])
+    assert np.all(np.abs(h - gh) <= 1e-6 * gh + 1e-300)
+    if "x" in g and g["converged"]:
+        x = dx.get()
+        assert np.linalg.norm(x - np.array(g["x"])) <= 1e-6 * np.linalg.norm(g["x"])
+    if g["converged"]:
+        assert r["rel_true_residual"] <= 2.0 * g["rel_true_residual"] + 1e-12
+
+
+@pytest.mark.parametrize("method", ["gmres", "fgmres"])
+@pytest.mark.parametrize("kind", ["ras", "twolevel_ras", "asm"])
+def test_schwarz_solve_vs_oracle(sg, ctx, method, kind):
+    A = O.poisson3d(24, 24, 20)
+    from rng_ref import splitmix_uniform
+    xs = 2.0 * splitmix_uniform(11, A.n) - 1.0
+    b = O.spmv(A, xs)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, kind, nparts=8, overlap=2).setup(dA)
+    part = M.partition()
+    ref = O.solve(method, A, b, O.Precond(kind, A, 8, 2, part), 30, 1000, 1e-9)
+    db, dx = dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n))
+    r = sg.solve(dA, db, dx, M, method=method, restart=30, max_iter=1000, tol=1e-9)
+    assert r["converged"] and ref["converged"]
+    assert abs(r["iterations"] - ref["iterations"]) <= 1
+    x = dx.get()
+    assert np.linalg.norm(x - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
+
+
+@pytest.mark.parametrize("method", ["gmres", "fgmres"])
+def test_rtol_mode_verifies_true_residual(sg, ctx, method):
+    A = O.poisson3d(32, 32, 32)
+    from rng_ref import splitmix_uniform
+    xs = 2.0 * splitmix_uniform(2, A.n) - 1.0
+    b = O.spmv(A, xs)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "ras", nparts=8, overlap=2).setup(dA)
+    for ortho in ("mgs", "cgs2"):
+        db, dx = dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n))
+        r = sg.solve(dA, db, dx, M, method=method, restart=50, max_iter=2000, rtol=1e-8,
+                     ortho=ortho)
+        assert r["converged"], r
+        x = dx.get()
+        true = np.linalg.norm(b - O.spmv(A, x)) / np.linalg.norm(b)
+        assert true <= 1e-8 and abs(true - r["rel_true_residual"]) <= 1e-3 * true
+
+
+def test_mixed_fgmres_converges_to_fp64_accuracy(sg, ctx):
+    A = O.poisson3d(32, 32, 24)
+    from rng_ref import splitmix_uniform
+    xs = 2.0 * splitmix_uniform(3, A.n) - 1.0
+    b = O.spmv(A, xs)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "twolevel_ras", nparts=16, overlap=1, precision="fp32",
+                   coarse_solve=1).setup(dA)
+    db, dx = dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n))
+    r = sg.solve(dA, db, dx, M, method="fgmres", restart=40, max_iter=2000, rtol=1e-8,
+                 ortho="cgs2")
+    assert r["converged"]
+    x = dx.get()
+    assert np.linalg.norm(b - O.spmv(A, x)) <= 1e-8 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xs) <= 1e-6 * np.linalg.norm(xs)
+
+
+def test_solve_is_deterministic(sg, ctx):
+    A = O.poisson2d(64, 64)
+    b = O.spmv(A, np.ones(A.n))
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "twolevel_ras", nparts=8, overlap=1).setup(dA)
+    outs = []
+    for _ in range(2):
+        db, dx = dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n))
+        r = sg.solve(dA, db, dx, M, method="fgmres", restart=30, tol=1e-10, ortho="cgs2")
+        outs.append((r["iterations"], dx.get(), r["hist"]))
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_errors_are_reported(sg, ctx):
+    A = O.poisson2d(8, 8)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "ras", nparts=4)
+    x = dev(sg, ctx, np.zeros(A.n))
+    with pytest.raises(sg.SgError) as e:
+        M.apply(x, x)  # before setup
+    assert e.value.status == 7
+    with pytest.raises(sg.SgError) as e:
+        sg.Precond(ctx, "ras", nparts=100).setup(dA)  # more parts than rows
+    assert e.value.status == 4
+    # singular coarse matrix: a zero block row makes A0 singular
+    rp = np.arange(0, 17, dtype=np.int32)
+    ci = np.arange(16, dtype=np.int32)
+    v = np.ones(16)
+    v[:4] = 0.0
+    dZ = sg.Csr(ctx, rp, ci, v)
+    part = (np.arange(16) // 4).astype(np.int32)
+    with pytest.raises(sg.SgError) as e:
+        sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=0, partition=part).setup(dZ)
+    assert e.value.status == 3
