@@ -209,7 +209,9 @@ def test_solve_matches_reference_golden(sg, ctx, gold, idx):
     assert abs(r["iterations"] - g["iterations"]) <= 1, (r["iterations"], g["iterations"])
     k = min(len(r["hist"]), len(g["hist"]))
     h, gh = r["hist"][:k], np.array(g["hist"][:k])
-    assert np.all(np.abs(h - gh) <= 1e-6 * gh + 1e-300)
+    # estimates agree to 1e-6 relative, or to 1e-12 of the initial residual once
+    # they are far below it (tree vs sequential reductions over hundreds of steps)
+    assert np.all(np.abs(h - gh) <= 1e-6 * gh + 1e-12 * gh[0])
     if "x" in g and g["converged"]:
         x = dx.get()
         assert np.linalg.norm(x - np.array(g["x"])) <= 1e-6 * np.linalg.norm(g["x"])
