@@ -1,0 +1,247 @@
+"""Time-to-solution benchmark on one MI355X.
+
+Workload (default): 3D jump-coefficient diffusion, 7-point finite-volume stencil
+on a 160^3 grid (n = 4,096,000, nnz = 28,518,400), kappa = 10^u with u uniform on
+{-3, 0, 3} per 16^3 block, harmonic-mean faces, b ~ N(0,1).  Measured path:
+FP64 FGMRES(40) with two-level RAS (256 subdomains, ufactor 300, overlap 1,
+FP32 ILU(0) factors and sweeps, Nicolaides coarse space with a 256 x 256 FP64
+LU), CGS2, stopping on an FP64-verified ||b - A x|| / ||b|| <= 1e-8.  Beside it,
+on the same GPU and matrix: plain FP64 GMRES(40) with one-level RAS and MGS
+(the reference's method).  A "step" is one solve from x0 = 0 with the inputs
+resident in HBM; ``value`` is seconds per solve (lower is better).
+
+The dominant kernel family's achieved bandwidth is measured live with HIP
+events on the solver stream; the CPU oracle (the reference algorithm restated
+in C++) is timed on a bounded number of iterations on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--no-baseline] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def load_problems():
+    spec = importlib.util.spec_from_file_location(
+        "sgamd_problems", os.path.join(__graft_entry__.PKG_DIR, "problems.py"))
+    P = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(P)
+    return P
+
+
+CONFIGS = {
+    "c3": dict(workload="diffusion3d_jump_160^3", restart=40, nparts=256, overlap=1,
+               ufactor=300, weighted=True),
+    "c2": dict(workload="poisson3d_128^3", restart=50, nparts=64, overlap=2, ufactor=0,
+               weighted=False),
+    "c1": dict(workload="poisson2d_256^2", restart=30, nparts=1, overlap=0, ufactor=0,
+               weighted=False),
+}
+
+
+def make_problem(P, cfg):
+    if cfg == "c3":
+        k = P.block_jump_kappa(160, 16, 3)
+        rp, ci, v = P.diffusion3d(k)
+        b = P.normal(33, len(rp) - 1)
+        xs = None
+    elif cfg == "c2":
+        rp, ci, v = P.poisson3d(128, 128, 128)
+        xs = P.uniform_pm1(2, len(rp) - 1)
+        b = None
+    else:
+        rp, ci, v = P.poisson2d(256, 256)
+        xs = P.uniform_pm1(1, len(rp) - 1)
+        b = None
+    return rp, ci, v, b, xs
+
+
+def spmv_np(rp, ci, v, x):
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return np.bincount(rows, weights=v * x[ci], minlength=len(rp) - 1)
+
+
+def run_solves(sg, ctx, A, M, db, dx, k, **kw):
+    out = []
+    for _ in range(k):
+        dx.zero()
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = sg.solve(A, db, dx, M, **kw)
+        ctx.sync()
+        r["wall_s"] = time.perf_counter() - t0
+        out.append(r)
+    return out
+
+
+def traffic_models(n, nnz, info, restart, fp32):
+    v = 4 if fp32 else 8
+    return {
+        # SpMV (FP64 CSR): vals+cols, row_ptr, x, y
+        "spmv": 12.0 * nnz + 20.0 * n + 4,
+        # fused RAS apply: factors+cols, level/row metadata, gather index + x, scatter + y
+        "trsv": (v + 4.0) * info["nnz_ext"] + 8.0 * info["n_ext"] + 12.0 * info["n_ext"]
+                + 12.0 * n,
+    }
+
+
+def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, gpu_iters):
+    """Oracle FGMRES + two-level RAS (FP64, reference algorithm) on the host."""
+    A = O.Csr(rp, ci, v)
+    t0 = time.perf_counter()
+    pc = O.Precond("twolevel_ras", A, cfg["nparts"], cfg["overlap"], part, precision=0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = O.solve("fgmres", A, b, pc, cfg["restart"], iters, 0.0)
+    t_iter = (time.perf_counter() - t0) / max(1, r["iterations"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(t_setup * 0 + t_iter * gpu_iters, 3), "unit": "s",
+            "cores": threads, "kind": "port",
+            "sample": f"{r['iterations']} FGMRES({cfg['restart']}) iterations of the oracle's "
+                      f"FP64 two-level RAS on the same matrix and partition "
+                      f"({t_iter * 1e3:.1f} ms/iteration, setup {t_setup:.1f} s not included), "
+                      f"scaled to the {gpu_iters} iterations of the GPU solve"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
+              "multi-GPU path (RCCL over xGMI) is a design in DESIGN.md, not built.",
+              file=sys.stderr)
+        sys.exit(1)
+
+    sg = __graft_entry__.load_package()
+    P = load_problems()
+    cfg = CONFIGS[args.config]
+    t0 = time.perf_counter()
+    rp, ci, v, b, xs = make_problem(P, args.config)
+    if b is None:
+        b = spmv_np(rp, ci, v, xs)
+    n, nnz = len(rp) - 1, int(rp[-1])
+    print(f"# problem {cfg['workload']}: n={n} nnz={nnz} ({time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    ctx = sg.Context(0)
+    A = sg.Csr(ctx, rp, ci, v)
+    db = sg.DeviceArray.from_host(ctx, b)
+    dx = sg.DeviceArray(ctx, n)
+
+    t0 = time.perf_counter()
+    if args.config == "c1":
+        part = None
+        M = sg.Precond(ctx, "ilu0").setup(A)
+        method, kind = "gmres", "ilu0"
+        solve_kw = dict(method="gmres", restart=cfg["restart"], max_iter=2000, rtol=1e-8,
+                        ortho="cgs2")
+    else:
+        part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v,
+                            weighted=cfg["weighted"])
+        kind = "twolevel_ras"
+        M = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
+                       precision="fp32", partition=part, coarse_solve=1).setup(A)
+        method = "fgmres"
+        solve_kw = dict(method="fgmres", restart=cfg["restart"], max_iter=2000, rtol=1e-8,
+                        ortho="cgs2")
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    info = M.stats()
+    print(f"# setup {t_setup:.2f} s  {info}", file=sys.stderr, flush=True)
+
+    run_solves(sg, ctx, A, M, db, dx, args.warmup, **solve_kw)
+    rs = run_solves(sg, ctx, A, M, db, dx, args.steps, **solve_kw)
+    times = [r["wall_s"] for r in rs]
+    r = rs[-1]
+    x = dx.get()
+    rel = np.linalg.norm(b - spmv_np(rp, ci, v, x)) / np.linalg.norm(b)
+    assert r["converged"] and rel <= 1e-8 * 1.0001, (r, rel)
+    print(f"# {method}+{kind}: {r['iterations']} it, {np.mean(times):.4f} s/solve, "
+          f"rel {rel:.2e}", file=sys.stderr, flush=True)
+
+    # per-family kernel time (HIP events on the solver stream), one profiled solve
+    ctx.profiling(True)
+    ctx.reset_stats()
+    dx.zero()
+    rp_ = sg.solve(A, db, dx, M, **solve_kw)
+    fams = {f: ctx.kernel_stats(f) for f in ("spmv", "trsv", "precond", "ortho", "update")}
+    ctx.profiling(False)
+    models = traffic_models(n, nnz, info, cfg["restart"], True)
+    dom = max(("spmv", "trsv"), key=lambda f: fams[f][1])
+    launches, ms = fams[dom]
+    avg_s = ms * 1e-3 / max(1, launches)
+    achieved = models[dom] / avg_s / 1e9
+    print(f"# kernel families (launches, ms): {fams}", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_baseline and args.config != "c1":
+        Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                        precision="fp64", partition=part).setup(A)
+        base_kw = dict(method="gmres", restart=cfg["restart"], max_iter=4000, rtol=1e-8,
+                       ortho="mgs")
+        rb = run_solves(sg, ctx, A, Mb, db, dx, 1, **base_kw)[0]
+        extra["fp64_ras_gmres"] = {"s": round(rb["wall_s"], 4), "iterations": rb["iterations"],
+                                   "converged": rb["converged"],
+                                   "rel_true_residual": rb["rel_true_residual"]}
+        extra["speedup_vs_fp64_ras_gmres"] = round(rb["wall_s"] / np.mean(times), 2)
+        print(f"# baseline gmres+ras fp64: {rb['iterations']} it {rb['wall_s']:.3f} s "
+              f"conv={rb['converged']}", file=sys.stderr, flush=True)
+        del Mb
+
+    cpu = None
+    if not args.no_cpu and args.config != "c1":
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind as O
+        cpu = cpu_baseline(O, rp, ci, v, b, part, cfg, args.cpu_iters, r["iterations"])
+
+    out = {
+        "metric": "time to ||r||/||b||<=1e-8 (s): mixed/learned FGMRES vs FP64 RAS-GMRES, 1 GPU",
+        "value": round(float(np.mean(times)), 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * float(np.mean(times)), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer / f32 preconditioner",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n": n, "nnz": nnz,
+                   "solver": f"{method}({cfg['restart']})+{kind}",
+                   "nparts": cfg["nparts"], "overlap": cfg["overlap"],
+                   "partition": "coefficient-weighted" if cfg["weighted"] else "unit-weight",
+                   "iterations": r["iterations"], "rel_true_residual": rel,
+                   "setup_s": round(t_setup, 3), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": models[dom],
+                     "avg_launch_us": round(avg_s * 1e6, 2)},
+        "kernel_ms": {f: round(fams[f][1], 3) for f in fams},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -104,6 +104,9 @@ typedef struct {
     int ufactor;           /* allowed imbalance in 1/1000 (METIS_OPTION_UFACTOR); 0 = 30 */
     int coarse_solve;      /* two-level only: 0 = LU forward/back solve (coarse_solve.h
                               order), 1 = precomputed inverse (one dense mat-vec)         */
+    int weighted_partition; /* built-in partitioner only: 1 weighs edge (i,j) by |a_ij| so
+                              cuts follow weak couplings; 0 = unit weights as the
+                              reference's METIS call (domain_decomp.h:48-52)              */
 } sg_precond_opts;
 
 sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_precond_opts* opts,
@@ -123,6 +126,31 @@ sg_status sg_precond_destroy(sg_precond* M);
 typedef void (*sg_apply_fn)(void* user, const double* x, double* y, int n);
 sg_status sg_precond_create_callback(sg_context* ctx, int n, sg_apply_fn fn, void* user,
                                      int device_pointers, sg_precond** out);
+
+/* Host-only graph partition of an n x n CSR matrix into nparts parts
+ * (part[n] out).  Multilevel recursive bisection; deterministic.  Stands in
+ * for partition_metis (src/decomp/domain_decomp.h:23-58): nparts <= 1 gives
+ * all zeros, self loops are ignored.  ufactor: allowed imbalance in 1/1000
+ * (0 = 30, METIS's default).  weighted = 1 weighs edges by |a_ij| (vals
+ * required); 0 ignores vals (may be NULL).  Needs no GPU. */
+sg_status sg_partition(int n, const int* row_ptr, const int* col_idx, const double* vals,
+                       int nparts, int ufactor, int weighted, int* part);
+
+/* Sizes of a set-up preconditioner, for traffic models: number of subdomains,
+ * sum of overlapped subdomain sizes (N_ext), stored local nonzeros (nnz_ext),
+ * forward and backward sweep levels summed over subdomains, and whether the
+ * subdomain vector lives in LDS. */
+typedef struct {
+    int nsub;
+    long long n_ext;
+    long long nnz_ext;
+    long long levels_fwd;
+    long long levels_bwd;
+    int max_levels;
+    int z_in_lds;
+    int nc;
+} sg_precond_info;
+sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info);
 
 /* Introspection for parity tests and tooling (all host outputs). */
 sg_status sg_precond_partition(const sg_precond* M, int* part /* int[nrows] */);

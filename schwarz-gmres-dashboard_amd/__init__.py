@@ -34,7 +34,7 @@ class SgError(RuntimeError):
 class PrecondOpts(C.Structure):
     _fields_ = [("nparts", C.c_int), ("overlap", C.c_int), ("precision", C.c_int),
                 ("partition", C.POINTER(C.c_int)), ("ufactor", C.c_int),
-                ("coarse_solve", C.c_int)]
+                ("coarse_solve", C.c_int), ("weighted_partition", C.c_int)]
 
 
 class SolveParams(C.Structure):
@@ -48,6 +48,12 @@ class SolveResult(C.Structure):
                 ("residual_norm", C.c_double), ("rel_true_residual", C.c_double),
                 ("time_solve_s", C.c_double), ("cycles", C.c_int),
                 ("precond_applies", C.c_int)]
+
+
+class PrecondInfo(C.Structure):
+    _fields_ = [("nsub", C.c_int), ("n_ext", C.c_longlong), ("nnz_ext", C.c_longlong),
+                ("levels_fwd", C.c_longlong), ("levels_bwd", C.c_longlong),
+                ("max_levels", C.c_int), ("z_in_lds", C.c_int), ("nc", C.c_int)]
 
 
 _lib = None
@@ -92,6 +98,8 @@ def lib():
         "sg_precond_coarse": ([_VP, C.POINTER(C.c_int), _VP, _VP, _VP], C.c_int),
         "sg_solve": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult), _VP,
                       C.c_int], C.c_int),
+        "sg_partition": ([C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP], C.c_int),
+        "sg_precond_stats": ([_VP, C.POINTER(PrecondInfo)], C.c_int),
         "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
         "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double)], C.c_int),
@@ -230,14 +238,14 @@ class Precond:
     """Device preconditioner (sg_precond): kind in PC_KIND."""
 
     def __init__(self, ctx, kind, nparts=4, overlap=1, precision="fp64", partition=None,
-                 ufactor=0, coarse_solve=0):
+                 ufactor=0, coarse_solve=0, weighted_partition=False):
         self.ctx = ctx
         self.kind = kind
         self._part = None if partition is None else np.ascontiguousarray(partition, np.int32)
         self.opts = PrecondOpts(nparts, overlap, 1 if precision == "fp32" else 0,
                                 None if self._part is None else
                                 self._part.ctypes.data_as(C.POINTER(C.c_int)),
-                                ufactor, coarse_solve)
+                                ufactor, coarse_solve, 1 if weighted_partition else 0)
         self.h = _VP()
         ctx.check(lib().sg_precond_create(ctx.h, PC_KIND[kind], C.byref(self.opts),
                                           C.byref(self.h)))
@@ -272,6 +280,11 @@ class Precond:
         self.ctx.check(lib().sg_precond_ilu_values(self.h, s, C.byref(nnz), _p(out)))
         return out
 
+    def stats(self):
+        info = PrecondInfo()
+        self.ctx.check(lib().sg_precond_stats(self.h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in PrecondInfo._fields_}
+
     def coarse(self):
         nc = C.c_int()
         self.ctx.check(lib().sg_precond_coarse(self.h, C.byref(nc), None, None, None))
@@ -289,6 +302,20 @@ class Precond:
                 self.h = None
         except Exception:
             pass
+
+
+def partition(rp, ci, nparts, ufactor=0, vals=None, weighted=False):
+    """Host graph partition (sg_partition); no GPU needed."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = None if vals is None else np.ascontiguousarray(vals, np.float64)
+    n = len(rp) - 1
+    part = np.zeros(n, np.int32)
+    st = lib().sg_partition(n, _p(rp), _p(ci), _p(v), nparts, ufactor, 1 if weighted else 0,
+                            _p(part))
+    if st:
+        raise SgError(st, lib().sg_last_error(None).decode())
+    return part
 
 
 def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10, rtol=0.0,

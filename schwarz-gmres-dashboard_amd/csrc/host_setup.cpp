@@ -18,7 +18,8 @@ namespace {
 // Undirected weighted graph in CSR form.
 struct Graph {
     int nv = 0;
-    std::vector<int> xadj, adj, ew, vw;
+    std::vector<int> xadj, adj, vw;
+    std::vector<long long> ew;
     long long total_vw() const {
         long long t = 0;
         for (int w : vw) t += w;
@@ -26,28 +27,50 @@ struct Graph {
     }
 };
 
-Graph graph_from_csr(const HostCsr& A) {
-    // symmetrise the pattern, drop self loops and duplicates
+Graph graph_from_csr(const HostCsr& A, bool weighted) {
+    // symmetrise the pattern, drop self loops and duplicates; with weighted,
+    // edge (i,j) weighs max(|a_ij|, |a_ji|) in units of the smallest nonzero
+    // off-diagonal magnitude (at least 1)
     int n = A.n;
-    std::vector<std::pair<int, int>> e;
+    double amin = 0.0;
+    if (weighted) {
+        amin = 1e300;
+        for (int i = 0; i < n; ++i)
+            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (A.ci[k] != i && A.v[k] != 0.0) amin = std::min(amin, std::fabs(A.v[k]));
+        if (!(amin < 1e300)) weighted = false;
+    }
+    struct E {
+        int a, b;
+        long long w;
+    };
+    std::vector<E> e;
     e.reserve((size_t)A.nnz() * 2);
     for (int i = 0; i < n; ++i)
         for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
             int j = A.ci[k];
             if (j == i || j < 0 || j >= n) continue;
-            e.emplace_back(i, j);
-            e.emplace_back(j, i);
+            long long w = 1;
+            if (weighted) {
+                double q = std::fabs(A.v[k]) / amin;
+                w = q >= 1e12 ? 1000000000000LL : std::max(1LL, (long long)std::llround(q));
+            }
+            e.push_back({i, j, w});
+            e.push_back({j, i, w});
         }
-    std::sort(e.begin(), e.end());
-    e.erase(std::unique(e.begin(), e.end()), e.end());
+    std::sort(e.begin(), e.end(), [](const E& x, const E& y) {
+        return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.w > y.w);
+    });
     Graph g;
     g.nv = n;
     g.xadj.assign(n + 1, 0);
-    for (auto& p : e) g.xadj[p.first + 1]++;
+    for (size_t k = 0; k < e.size(); ++k) {
+        if (k > 0 && e[k].a == e[k - 1].a && e[k].b == e[k - 1].b) continue;  // keep the max weight
+        g.adj.push_back(e[k].b);
+        g.ew.push_back(e[k].w);
+        g.xadj[e[k].a + 1]++;
+    }
     for (int i = 0; i < n; ++i) g.xadj[i + 1] += g.xadj[i];
-    g.adj.resize(e.size());
-    g.ew.assign(e.size(), 1);
-    for (size_t k = 0; k < e.size(); ++k) g.adj[k] = e[k].second;
     g.vw.assign(n, 1);
     return g;
 }
@@ -62,10 +85,11 @@ bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return (g.xadj[a + 1] - g.xadj[a]) < (g.xadj[b + 1] - g.xadj[b]);
     });
-    long long maxvw = std::max<long long>(1, g.total_vw() / 20);
+    long long maxvw = std::max<long long>(1, g.total_vw() / 100);
     for (int v : order) {
         if (match[v] >= 0) continue;
-        int best = -1, bw = -1;
+        int best = -1;
+        long long bw = -1;
         for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
             int u = g.adj[k];
             if (match[u] >= 0 || u == v) continue;
@@ -135,9 +159,10 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
     int n = g.nv;
     long long w0 = 0;
     for (int v = 0; v < n; ++v) if (side[v] == 0) w0 += g.vw[v];
-    std::vector<int> gain(n), locked(n);
+    std::vector<long long> gain(n);
+    std::vector<int> locked(n);
     auto compute_gain = [&](int v) {
-        int ext = 0, in = 0;
+        long long ext = 0, in = 0;
         for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
             (side[g.adj[k]] != side[v] ? ext : in) += g.ew[k];
         return ext - in;
@@ -148,7 +173,7 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
         return false;
     };
     for (int pass = 0; pass < passes; ++pass) {
-        std::set<std::pair<int, int>> q[2];  // (-gain, v)
+        std::set<std::pair<long long, int>> q[2];  // (-gain, v)
         std::fill(locked.begin(), locked.end(), 0);
         for (int v = 0; v < n; ++v) {
             if (!boundary(v)) continue;
@@ -159,6 +184,7 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
         long long cur = 0, best = 0;
         size_t best_len = 0;
         long long best_imb = std::llabs(w0 - (lo + hi) / 2);
+        bool best_ok = w0 >= lo && w0 <= hi;
         int since_best = 0;
         const int limit = std::max(50, std::min(n / 20 + 1, 400));
         while (since_best < limit) {
@@ -169,7 +195,10 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
                 for (auto it = q[s].begin(); it != q[s].end(); ++it) {
                     int v = it->second;
                     long long nw0 = s == 0 ? w0 - g.vw[v] : w0 + g.vw[v];
-                    if (nw0 < lo || nw0 > hi) continue;
+                    // feasible: stays in the window, or moves toward it from outside
+                    bool inside = nw0 >= lo && nw0 <= hi;
+                    bool closer = (w0 > hi && nw0 < w0 && nw0 >= lo) || (w0 < lo && nw0 > w0 && nw0 <= hi);
+                    if (!inside && !closer) continue;
                     if (pick < 0 || gain[v] > gain[pick] ||
                         (gain[v] == gain[pick] && std::llabs(nw0 - (lo + hi) / 2) <
                                                       std::llabs((pside == 0 ? w0 - g.vw[pick] : w0 + g.vw[pick]) - (lo + hi) / 2))) {
@@ -195,7 +224,9 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
                 if (boundary(u)) q[side[u]].insert({-gain[u], u});
             }
             long long imb = std::llabs(w0 - (lo + hi) / 2);
-            if (cur < best || (cur == best && imb < best_imb)) {
+            bool ok_now = w0 >= lo && w0 <= hi;
+            if ((ok_now && !best_ok) || (ok_now == best_ok && (cur < best || (cur == best && imb < best_imb)))) {
+                best_ok = ok_now;
                 best = cur;
                 best_len = moved.size();
                 best_imb = imb;
@@ -217,11 +248,12 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
 // Greedy graph growing from a seed; grows side 0 until its weight >= target.
 std::vector<int> grow(const Graph& g, int seed, long long target) {
     int n = g.nv;
-    std::vector<int> side(n, 1), gain(n, 0), inq(n, 0);
-    std::set<std::pair<int, int>> q;
+    std::vector<int> side(n, 1), inq(n, 0);
+    std::vector<long long> gain(n, 0);
+    std::set<std::pair<long long, int>> q;
     long long w0 = 0;
     auto push = [&](int v) {
-        int ext = 0, in = 0;
+        long long ext = 0, in = 0;
         for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k)
             (side[g.adj[k]] == 0 ? in : ext) += g.ew[k];
         gain[v] = in - ext;
@@ -298,11 +330,15 @@ std::vector<int> bisect(const Graph& g, double frac, double eps) {
         maps.push_back(std::move(cm));
         levels.push_back(std::move(c));
     }
+    // side s may exceed its target share t_s by eps * t_s (and by one coarse
+    // vertex while the graph is coarse)
     auto window = [&](const Graph& h) {
         long long maxv = 1;
         for (int w : h.vw) maxv = std::max<long long>(maxv, w);
-        long long slack = std::max<long long>((long long)(eps * (double)tot), maxv);
-        return std::make_pair(target - slack, target + slack);
+        const double t0 = (double)target, t1 = (double)(tot - target);
+        long long up0 = std::max<long long>((long long)(eps * t0), maxv);
+        long long up1 = std::max<long long>((long long)(eps * t1), maxv);
+        return std::make_pair(target - up1, target + up0);
     };
     auto w = window(levels.back());
     std::vector<int> side = initial_bisect(levels.back(), target, w.first, w.second);
@@ -364,11 +400,11 @@ void recurse(const Graph& g, const std::vector<int>& ids, int nparts, int first,
 
 }  // namespace
 
-std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor) {
+std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor, bool weighted) {
     if (nparts <= 1) return std::vector<int>(A.n, 0);
     if (nparts > A.n) throw std::runtime_error("partitioning failed: more parts than rows");
     if (ufactor <= 0) ufactor = 30;
-    Graph g = graph_from_csr(A);
+    Graph g = graph_from_csr(A, weighted);
     int depth = (int)std::ceil(std::log2((double)nparts));
     double eps = std::pow(1.0 + ufactor / 1000.0, 1.0 / std::max(1, depth)) - 1.0;
     std::vector<int> part(A.n, -1), ids(A.n);

@@ -23,7 +23,10 @@ struct HostCsr {
 // Deterministic.  Stands in for METIS_PartGraphKway
 // (reference src/decomp/domain_decomp.h:23-58); nparts <= 1 gives all zeros as
 // there (:24-26).  ufactor is the allowed imbalance in 1/1000 (30 = 1.03).
-std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor);
+// weighted = true weighs edge (i,j) by max(|a_ij|, |a_ji|) so that cuts prefer
+// weak couplings (coefficient-aware partitions for jump coefficients); the
+// reference's METIS call passes no weights, which weighted = false reproduces.
+std::vector<int> partition_graph(const HostCsr& A, int nparts, int ufactor, bool weighted = false);
 
 // Owned and overlapped index sets exactly as build_decomposition
 // (reference src/decomp/domain_decomp.h:61-93): owned_dofs[s] ascending,

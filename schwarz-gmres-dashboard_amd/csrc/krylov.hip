@@ -18,6 +18,7 @@
 // synchronises once per cycle.
 #include "sg_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -28,9 +29,11 @@ int precond_n(const sg_precond* M);
 bool precond_ready(const sg_precond* M);
 
 constexpr int kT = 256;           // threads per reduction block
-constexpr int kE = 16;            // elements per thread in the CGS2 passes
 constexpr int kMaxBlocks = 4096;  // partial-sum slots per column
 constexpr int kMaxCols = 128;     // restart + 1
+constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis vector)
+constexpr int kCB = 16;           // basis vectors per register block in the Gram passes
+constexpr int kEo = 2;            // elements per thread per tile in the Gram passes
 
 struct KState {
     int active;     // current cycle continues
@@ -55,10 +58,11 @@ struct Work {
     double* cs;    // m
     double* sn;    // m
     double* y;     // m
-    double* h1;    // kMaxCols (CGS2 pass 1)
-    double* h2;    // kMaxCols (CGS2 pass 2)
+    double* h1;    // kRedCols: projection coefficients h
+    double* h2;    // kRedCols: raw dots V^T w and V^T v_j
+    double* G;     // kMaxCols^2: Gram matrix V^T V, row-major, stride kMaxCols
     double* hist;  // residual history
-    double* part;  // kMaxCols * kMaxBlocks
+    double* part;  // kRedCols * kMaxBlocks
     unsigned* ticket;
     KState* st;
 };
@@ -126,7 +130,7 @@ __device__ void step_epilogue(const Work& W, int j, double hn) {
 
 // Block sums of ncols partial values in red[wave][col] -> partials, then the
 // last block reduces the partials of every column in fixed order into out[].
-__device__ bool finish_multi(const Work& W, int ncols, double (*red)[kMaxCols], double* out) {
+__device__ bool finish_multi(const Work& W, int ncols, double (*red)[kRedCols], double* out) {
     const int tid = threadIdx.x;
     __shared__ bool last;
     __syncthreads();
@@ -142,89 +146,154 @@ __device__ bool finish_multi(const Work& W, int ncols, double (*red)[kMaxCols], 
     __syncthreads();
     if (!last) return false;
     __threadfence();
-    const int lane = tid & 63, wv = tid >> 6;
-    for (int i = 0; i < ncols; ++i) {
-        double s = 0.0;
-        for (int b = tid; b < (int)gridDim.x; b += kT) s += W.part[i * kMaxBlocks + b];
-        s = wave_sum(s);
-        if (lane == 0) red[wv][i] = s;
-    }
-    __syncthreads();
-    for (int i = tid; i < ncols; i += kT) {
-        double s = 0.0;
+    if (ncols <= 8) {
+        // few columns: the whole block reduces each one
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int i = 0; i < ncols; ++i) {
+            double s = 0.0;
+            for (int b = tid; b < (int)gridDim.x; b += kT) s += W.part[i * kMaxBlocks + b];
+            s = wave_sum(s);
+            if (lane == 0) red[wv][i] = s;
+        }
+        __syncthreads();
+        for (int i = tid; i < ncols; i += kT) {
+            double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kT / 64; ++w) s += red[w][i];
-        out[i] = s;
+            for (int w = 0; w < kT / 64; ++w) s += red[w][i];
+            out[i] = s;
+        }
+    } else {
+        // many columns: one thread per column adds the block partials in order
+        for (int i = tid; i < ncols; i += kT) {
+            const double* pi = W.part + (size_t)i * kMaxBlocks;
+            double s = 0.0;
+            for (int b = 0; b < (int)gridDim.x; ++b) s += pi[b];
+            out[i] = s;
+        }
     }
     if (tid == 0) *W.ticket = 0u;
     __syncthreads();
     return true;
 }
 
-// CGS2 pass.  MODE 0: h1 = V^T w.  MODE 1: w -= V h1, then h2 = V^T w and
-// H[:,j] = h1 + h2.  MODE 2: w -= V h2, then h_{j+1,j} = ||w|| and the step
-// epilogue.  Each thread keeps kE elements of w in registers.
-template <int MODE>
+// Classical Gram-Schmidt with one re-orthogonalisation whose coefficients come
+// from the Gram matrix G = V^T V (two passes over V per step).
+// Pass 1 (k_gram_dot): p = V^T w and, lagged by one step, g = V^T v_j -- the
+// actual Gram row of the newest basis vector -- in one read of V; then
+// h = (2I - G) p, which equals G^{-1} p to first order in the loss of
+// orthogonality, i.e. the coefficients of projecting twice.
+// Pass 2 (k_gram_update): w -= V h and ||w||, then the step epilogue.
 __global__ void __launch_bounds__(kT)
-k_cgs(int n, int ncols, const double* __restrict__ V, size_t ld, double* __restrict__ w,
-      Work W, int j) {
+k_gram_dot(int n, int ncols, const double* __restrict__ V, size_t ld,
+           const double* __restrict__ w, Work W, int j) {
     if (!W.st->active) return;
-    __shared__ double red[kT / 64][kMaxCols];
+    __shared__ double red[kT / 64][kRedCols];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double* vj = V + (size_t)j * ld;
+    const int tile = kT * kEo;
+    for (int cb = 0; cb < ncols; cb += kCB) {
+        double aw[kCB], av[kCB];
+#pragma unroll
+        for (int c = 0; c < kCB; ++c) aw[c] = av[c] = 0.0;
+        for (int base = blockIdx.x * tile + tid; base < n; base += gridDim.x * tile) {
+            double wr[kEo], vr[kEo];
+#pragma unroll
+            for (int r = 0; r < kEo; ++r) {
+                const int e = base + r * kT;
+                wr[r] = e < n ? w[e] : 0.0;
+                vr[r] = e < n ? vj[e] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < kCB; ++c) {
+                if (cb + c < ncols) {
+                    const double* Vi = V + (size_t)(cb + c) * ld;
+#pragma unroll
+                    for (int r = 0; r < kEo; ++r) {
+                        const int e = base + r * kT;
+                        const double a = e < n ? Vi[e] : 0.0;
+                        aw[c] += a * wr[r];
+                        av[c] += a * vr[r];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kCB; ++c) {
+            if (cb + c < ncols) {
+                const double sw = wave_sum(aw[c]);
+                const double sv = wave_sum(av[c]);
+                if (lane == 0) {
+                    red[wv][cb + c] = sw;
+                    red[wv][ncols + cb + c] = sv;
+                }
+            }
+        }
+    }
+    // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
+    if (!finish_multi(W, 2 * ncols, red, W.h2)) return;
+    const int m = W.st->m;
+    double* Gm = W.G;
+    for (int i = tid; i < ncols; i += kT) {  // Gram row of v_j
+        const double g = W.h2[ncols + i];
+        Gm[j * kMaxCols + i] = g;
+        Gm[i * kMaxCols + j] = g;
+    }
+    __syncthreads();
+    for (int i = tid; i < ncols; i += kT) {
+        double gp = 0.0;
+        for (int k = 0; k < ncols; ++k) gp += Gm[i * kMaxCols + k] * W.h2[k];
+        const double h = 2.0 * W.h2[i] - gp;
+        W.h1[i] = h;
+        W.H[i + j * (m + 1)] = h;
+    }
+}
+
+__global__ void __launch_bounds__(kT)
+k_gram_update(int n, int ncols, const double* __restrict__ V, size_t ld, double* __restrict__ w,
+              Work W, int j) {
+    if (!W.st->active) return;
+    __shared__ double red[kT / 64][kRedCols];
     __shared__ double hs[kMaxCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int base = blockIdx.x * (kT * kE) + tid;
-    double wr[kE];
+    for (int i = tid; i < ncols; i += kT) hs[i] = W.h1[i];
+    __syncthreads();
+    const int tile = kT * kEo;
+    double acc = 0.0;
+    for (int base = blockIdx.x * tile + tid; base < n; base += gridDim.x * tile) {
+        double wr[kEo];
 #pragma unroll
-    for (int r = 0; r < kE; ++r) {
-        int e = base + r * kT;
-        wr[r] = e < n ? w[e] : 0.0;
-    }
-    if (MODE >= 1) {
-        const double* h = MODE == 1 ? W.h1 : W.h2;
-        for (int i = tid; i < ncols; i += kT) hs[i] = h[i];
-        __syncthreads();
-        for (int i = 0; i < ncols; ++i) {
-            const double hi = hs[i];
-            const double* Vi = V + (size_t)i * ld;
+        for (int r = 0; r < kEo; ++r) {
+            const int e = base + r * kT;
+            wr[r] = e < n ? w[e] : 0.0;
+        }
+        for (int cb = 0; cb < ncols; cb += kCB) {
 #pragma unroll
-            for (int r = 0; r < kE; ++r) {
-                int e = base + r * kT;
-                if (e < n) wr[r] -= hi * Vi[e];
+            for (int c = 0; c < kCB; ++c) {
+                if (cb + c < ncols) {
+                    const double hi = hs[cb + c];
+                    const double* Vi = V + (size_t)(cb + c) * ld;
+#pragma unroll
+                    for (int r = 0; r < kEo; ++r) {
+                        const int e = base + r * kT;
+                        if (e < n) wr[r] -= hi * Vi[e];
+                    }
+                }
             }
         }
 #pragma unroll
-        for (int r = 0; r < kE; ++r) {
-            int e = base + r * kT;
-            if (e < n) w[e] = wr[r];
-        }
-    }
-    if (MODE <= 1) {
-        for (int i = 0; i < ncols; ++i) {
-            const double* Vi = V + (size_t)i * ld;
-            double s = 0.0;
-#pragma unroll
-            for (int r = 0; r < kE; ++r) {
-                int e = base + r * kT;
-                if (e < n) s += Vi[e] * wr[r];
+        for (int r = 0; r < kEo; ++r) {
+            const int e = base + r * kT;
+            if (e < n) {
+                w[e] = wr[r];
+                acc += wr[r] * wr[r];
             }
-            s = wave_sum(s);
-            if (lane == 0) red[wv][i] = s;
         }
-        double* out = MODE == 0 ? W.h1 : W.h2;
-        if (!finish_multi(W, ncols, red, out)) return;
-        if (MODE == 1) {
-            const int m = W.st->m;
-            for (int i = tid; i < ncols; i += kT) W.H[i + j * (m + 1)] = W.h1[i] + W.h2[i];
-        }
-    } else {
-        double s = 0.0;
-#pragma unroll
-        for (int r = 0; r < kE; ++r) s += wr[r] * wr[r];
-        s = wave_sum(s);
-        if (lane == 0) red[wv][0] = s;
-        if (!finish_multi(W, 1, red, W.h1 + kMaxCols - 1)) return;
-        if (tid == 0) step_epilogue(W, j, sqrt(W.h1[kMaxCols - 1]));
     }
+    acc = wave_sum(acc);
+    if (lane == 0) red[wv][0] = acc;
+    double* out = W.h2 + kRedCols - 1;
+    if (!finish_multi(W, 1, red, out)) return;
+    if (tid == 0) step_epilogue(W, j, sqrt(*out));
 }
 
 // MGS projection i of step j (gmres.cpp:93-97): w -= h_{i-1} V_{i-1} (when
@@ -233,7 +302,7 @@ __global__ void __launch_bounds__(kT)
 k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
       double* __restrict__ w, Work W, int i, int j) {
     if (!W.st->active) return;
-    __shared__ double red[kT / 64][kMaxCols];
+    __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int m = W.st->m;
     const double hprev = Vprev ? W.H[(i - 1) + j * (m + 1)] : 0.0;
@@ -248,7 +317,7 @@ k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
     }
     s = wave_sum(s);
     if (lane == 0) red[wv][0] = s;
-    double* out = W.h1 + kMaxCols - 1;
+    double* out = W.h2 + kRedCols - 1;
     if (!finish_multi(W, 1, red, out)) return;
     if (tid == 0) {
         if (Vi) W.H[i + j * (m + 1)] = *out;
@@ -260,7 +329,7 @@ k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
 // beta test and the start of the cycle (gmres.cpp:59-81, fgmres.cpp:54-71).
 __global__ void __launch_bounds__(kT)
 k_restart(int n, const double* __restrict__ u, const double* __restrict__ r, Work W) {
-    __shared__ double red[kT / 64][kMaxCols];
+    __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double su = 0.0, sr = 0.0;
     for (int e = blockIdx.x * kT + tid; e < n; e += gridDim.x * kT) {
@@ -277,7 +346,7 @@ k_restart(int n, const double* __restrict__ u, const double* __restrict__ r, Wor
         red[wv][0] = su;
         red[wv][1] = sr;
     }
-    double* out = W.h1 + kMaxCols - 2;
+    double* out = W.h2 + kRedCols - 2;
     if (!finish_multi(W, 2, red, out)) return;
     if (tid != 0) return;
     KState* st = W.st;
@@ -406,8 +475,6 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
         if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
         const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
-        if (cgs2 && (long long)n > (long long)kMaxBlocks * kT * kE)
-            fail(SG_E_ARG, "CGS2 supports n <= 16777216; use MGS");
         const int L = prm->lookahead > 0 ? prm->lookahead : 2;
         hipStream_t s = ctx->stream;
         SG_HIP(hipSetDevice(ctx->device));
@@ -425,8 +492,8 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         double* w = z + ld;
         double* t = w + ld;
         const int hcap = prm->max_iter + m + 2;
-        size_t nsmall = (size_t)(m + 1) * m + (m + 1) + 3 * m + 2 * kMaxCols + hcap +
-                        (size_t)kMaxCols * kMaxBlocks;
+        size_t nsmall = (size_t)(m + 1) * m + (m + 1) + 3 * m + 2 * kRedCols + hcap +
+                        (size_t)kRedCols * kMaxBlocks + (size_t)kMaxCols * kMaxCols;
         ws.small.alloc(nsmall);
         ws.ticket.alloc(1);
         ws.st.alloc(1);
@@ -439,9 +506,10 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         W.sn = W.cs + m;
         W.y = W.sn + m;
         W.h1 = W.y + m;
-        W.h2 = W.h1 + kMaxCols;
-        W.hist = W.h2 + kMaxCols;
+        W.h2 = W.h1 + kRedCols;
+        W.hist = W.h2 + kRedCols;
         W.part = W.hist + hcap;
+        W.G = W.part + (size_t)kRedCols * kMaxBlocks;
         W.ticket = ws.ticket.p;
         W.st = ws.st.p;
         KState* st = ws.st.p;
@@ -459,7 +527,7 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         SG_HIP(hipMemcpyAsync(&st->bnorm, &bnorm, sizeof(double), hipMemcpyHostToDevice, s));
 
         const int gred = nblocks_for(n, 4);
-        const int gcgs = nblocks_for(n, kE);
+        const int gcgs = std::min(nblocks_for(n, kEo), 2 * ctx->num_cu);
         const int gew = nblocks_for(n, 4);
         std::vector<hipEvent_t> evs(m);
         for (int j = 0; j < m; ++j) evs[j] = ctx->get_event();
@@ -524,9 +592,8 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
                 {
                     ProfScope ps(ctx, SG_K_ORTHO);
                     if (cgs2) {
-                        hipLaunchKernelGGL(k_cgs<0>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
-                        hipLaunchKernelGGL(k_cgs<1>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
-                        hipLaunchKernelGGL(k_cgs<2>, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_gram_dot, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_gram_update, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
                     } else {
                         for (int i = 0; i <= j; ++i)
                             hipLaunchKernelGGL(k_mgs, dim3(gred), dim3(kT), 0, s, n,

@@ -30,30 +30,38 @@
 
 namespace sg {
 
-constexpr int kChunk = 64;
 constexpr int kTrsvThreads = 512;
+constexpr int kKreg = 4;  // factor entries per row prefetched in registers
 
 struct SubInfo {
     int ns;      // rows of the (overlapped) subdomain
     int row0;    // first row in the concatenated local numbering
     int zoff;    // offset of z in the global scratch (ns + 1 slots)
-    int flev0;   // first forward level (index into f_lev)
+    int flev0;   // first forward level (index into the level table)
     int nflev;
     int blev0;
     int nblev;
-    int own0;    // first owned entry (coarse restriction list)
+    int own0;    // first entry in the owned-row list
     int nown;
+};
+
+// One level of a sweep: rows [start, start + size) of the level-ordered row
+// list; entry k of the row at position start + q sits at ell + k * size + q.
+struct Level {
+    int start, size, K, ell;
 };
 
 struct Plan {
     // host-side staging, moved to the device at setup
     std::vector<SubInfo> subs;
-    std::vector<int> order;                        // subdomains by descending nnz
-    std::vector<int> f_lev, f_row, f_gid, f_off, f_K, f_col, f_src;
-    std::vector<int> b_lev, b_row, b_out, b_off, b_K, b_col, b_src, b_dsrc;
-    std::vector<int> lrp, lci, ldiag;              // concatenated local CSR
+    std::vector<int> order;  // subdomains by descending nnz
+    std::vector<Level> flev, blev;
+    std::vector<int> f_row, f_col, f_src;
+    std::vector<int> b_row, b_col, b_src, b_dsrc;
+    std::vector<int> lrp, lci, ldiag;  // concatenated local CSR
     std::vector<double> lval;
-    std::vector<int> own_idx;                      // owned global ids, per subdomain ascending
+    std::vector<int> dofs;   // global id of every local row, concatenated
+    std::vector<int> own;    // owned local rows, per subdomain ascending
     int total_rows = 0;
     int max_ns = 0;
 };
@@ -78,14 +86,18 @@ struct sg_precond {
     std::vector<int> part;
     std::vector<std::vector<int>> owned, dofs;
     std::vector<int> sub_nnz;  // local nnz per subdomain
+    long long levels_fwd = 0, levels_bwd = 0;
+    int max_levels = 0;
     // sweep plan (device)
     int nsub = 0;
     bool z_in_lds = false;
     size_t lds_bytes = 0;
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
-    sg::DevBuf<int> f_lev, f_row, f_gid, f_off, f_K, f_col, f_src;
-    sg::DevBuf<int> b_lev, b_row, b_out, b_off, b_K, b_col, b_src, b_dsrc;
+    sg::DevBuf<sg::Level> flev, blev;
+    sg::DevBuf<int> f_row, f_col, f_src;
+    sg::DevBuf<int> b_row, b_col, b_src, b_dsrc;
+    sg::DevBuf<int> d_dofs, d_own;
     sg::DevBuf<int> lrp, lci, ldiag;
     sg::DevBuf<double> lval;  // factored FP64 values in local CSR order
     sg::DevBuf<double> f_v64, b_v64, b_d64;
@@ -102,9 +114,10 @@ struct sg_precond {
     int nc = 0;
     std::vector<double> A0, LU;
     std::vector<int> piv;
-    sg::DevBuf<double> d_LU64, d_LUt64, d_inv64, d_c64, d_r0_64, d_cz64;
-    sg::DevBuf<float> d_LU32, d_LUt32, d_inv32, d_c32, d_r0_32, d_cz32;
-    sg::DevBuf<int> d_piv, d_own_ptr, d_own_idx;
+    sg::DevBuf<double> d_LUt64, d_inv64, d_c64, d_r0_64, d_cz64;
+    sg::DevBuf<float> d_LUt32, d_inv32, d_c32, d_r0_32, d_cz32;
+    sg::DevBuf<int> d_piv;
+    int restrict_split = 1;
     sg::DevBuf<int> d_lu_status;
 };
 
@@ -114,25 +127,21 @@ namespace sg {
 // host: per-subdomain sweep packing
 // ---------------------------------------------------------------------------
 
-struct SubPack {
-    std::vector<int> f_lev, f_row, f_gid, f_K, f_col, f_src;  // f_src local CSR positions
-    std::vector<int> b_lev, b_row, b_out, b_K, b_col, b_src, b_dsrc;
-    std::vector<int> f_off, b_off;  // relative to this subdomain's ELL start
+// Packs one sweep of a subdomain's local matrix L into level order.
+// lower = true: the strictly-lower entries of each row in CSR order (unit-L
+// forward sweep); lower = false: the entries right of the diagonal in CSR
+// order plus the diagonal (backward sweep).  Rows of a level are ascending.
+// Padding entries point at the zero slot (column ns) with source -1.
+struct SweepPack {
+    std::vector<Level> lev;  // start / ell relative to this subdomain
+    std::vector<int> row, col, src, dsrc;
 };
 
-// Packs one sweep.  lower=true: strictly-lower entries in CSR order;
-// lower=false: entries right of the diagonal in CSR order plus the diagonal.
-static void pack_sweep(const HostCsr& L, const std::vector<int>& diag, const std::vector<int>& lev,
-                       bool lower, const std::vector<int>& gid_of_row, std::vector<int>& out_lev,
-                       std::vector<int>& out_row, std::vector<int>& out_gid,
-                       std::vector<int>& out_off, std::vector<int>& out_K,
-                       std::vector<int>& out_col, std::vector<int>& out_src,
-                       std::vector<int>* out_dsrc) {
-    int ns = L.n;
+static void pack_levels(const HostCsr& L, const std::vector<int>& diag,
+                        const std::vector<int>& lev, bool lower, SweepPack& P) {
+    const int ns = L.n;
     int nlev = 0;
     for (int l : lev) nlev = std::max(nlev, l + 1);
-    // rows bucketed by level, ascending row order inside a level (forward) or
-    // descending levels processed first for backward (level 0 = no deps)
     std::vector<int> cnt(nlev + 1, 0);
     for (int i = 0; i < ns; ++i) cnt[lev[i] + 1]++;
     for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
@@ -141,56 +150,32 @@ static void pack_sweep(const HostCsr& L, const std::vector<int>& diag, const std
         std::vector<int> pos(cnt.begin(), cnt.end() - 1);
         for (int i = 0; i < ns; ++i) rows[pos[lev[i]]++] = i;
     }
-    int chunk = 0, ell = 0;
-    out_lev.push_back(0);
+    auto first = [&](int i) { return lower ? L.rp[i] : diag[i] + 1; };
+    auto count = [&](int i) {
+        if (!lower) return L.rp[i + 1] - (diag[i] + 1);
+        int c = 0;
+        for (int k = L.rp[i]; k < L.rp[i + 1] && L.ci[k] < i; ++k) ++c;
+        return c;
+    };
+    int ell = 0;
     for (int l = 0; l < nlev; ++l) {
-        for (int a = cnt[l]; a < cnt[l + 1]; a += kChunk) {
-            int b = std::min(cnt[l + 1], a + kChunk);
-            int K = 0;
-            for (int p = a; p < b; ++p) {
-                int i = rows[p];
-                int c = 0;
-                if (lower) {
-                    for (int k = L.rp[i]; k < L.rp[i + 1] && L.ci[k] < i; ++k) ++c;
-                } else {
-                    c = L.rp[i + 1] - (diag[i] + 1);
-                }
-                K = std::max(K, c);
+        const int a = cnt[l], size = cnt[l + 1] - cnt[l];
+        int K = 0;
+        for (int q = 0; q < size; ++q) K = std::max(K, count(rows[a + q]));
+        P.lev.push_back(Level{a, size, K, ell});
+        P.col.resize((size_t)ell + (size_t)K * size, ns);
+        P.src.resize((size_t)ell + (size_t)K * size, -1);
+        for (int q = 0; q < size; ++q) {
+            const int i = rows[a + q];
+            P.row.push_back(i);
+            if (!lower) P.dsrc.push_back(diag[i]);
+            const int k0 = first(i), c = count(i);
+            for (int e = 0; e < c; ++e) {
+                P.col[(size_t)ell + (size_t)e * size + q] = L.ci[k0 + e];
+                P.src[(size_t)ell + (size_t)e * size + q] = k0 + e;
             }
-            out_off.push_back(ell);
-            out_K.push_back(K);
-            for (int lane = 0; lane < kChunk; ++lane) {
-                int p = a + lane;
-                if (p < b) {
-                    int i = rows[p];
-                    out_row.push_back(i);
-                    out_gid.push_back(gid_of_row[i]);
-                    if (out_dsrc) out_dsrc->push_back(diag[i]);
-                } else {
-                    out_row.push_back(-1);
-                    out_gid.push_back(-1);
-                    if (out_dsrc) out_dsrc->push_back(-1);
-                }
-            }
-            // ELL slices: entry k of all 64 lanes contiguous
-            size_t base = out_col.size();
-            out_col.resize(base + (size_t)K * kChunk, ns);
-            out_src.resize(base + (size_t)K * kChunk, -1);
-            for (int lane = 0; lane < kChunk && a + lane < b; ++lane) {
-                int i = rows[a + lane];
-                int k0 = lower ? L.rp[i] : diag[i] + 1;
-                for (int e = 0; e < K; ++e) {
-                    int k = k0 + e;
-                    bool ok = lower ? (k < L.rp[i + 1] && L.ci[k] < i) : (k < L.rp[i + 1]);
-                    if (!ok) break;
-                    out_col[base + (size_t)e * kChunk + lane] = L.ci[k];
-                    out_src[base + (size_t)e * kChunk + lane] = k;
-                }
-            }
-            ell += K * kChunk;
-            ++chunk;
         }
-        out_lev.push_back(chunk);
+        ell += K * size;
     }
 }
 
@@ -229,19 +214,18 @@ __global__ void k_jacobi(int n, const double* __restrict__ inv, const double* __
 }
 
 // IKJ ILU(0) (src/precond/ilu0.h:31-51), one workgroup per subdomain, rows of
-// one level in parallel.  Per-row operation order equals the reference's.
+// one forward level in parallel.  Per-row operation order equals the
+// reference's, so the factors match it bit for bit.
 __global__ void __launch_bounds__(kTrsvThreads)
-k_ilu0_factor(const SubInfo* __restrict__ subs, const int* __restrict__ f_lev,
+k_ilu0_factor(const SubInfo* __restrict__ subs, const Level* __restrict__ flev,
               const int* __restrict__ f_row, const int* __restrict__ lrp,
               const int* __restrict__ lci, const int* __restrict__ ldiag,
               double* __restrict__ lu) {
     const SubInfo si = subs[blockIdx.x];
-    const int wave = threadIdx.x / kChunk, lane = threadIdx.x % kChunk;
-    const int nw = blockDim.x / kChunk;
     for (int l = si.flev0; l < si.flev0 + si.nflev; ++l) {
-        for (int c = f_lev[l] + wave; c < f_lev[l + 1]; c += nw) {
-            const int row = f_row[c * kChunk + lane];
-            if (row < 0) continue;
+        const Level L = flev[l];
+        for (int q = threadIdx.x; q < L.size; q += blockDim.x) {
+            const int row = f_row[L.start + q];
             const int gi = si.row0 + row;
             const int a = lrp[gi], e = lrp[gi + 1];
             for (int jp = a; jp < e; ++jp) {
@@ -276,20 +260,96 @@ __global__ void k_pack(int n, const int* __restrict__ src, const double* __restr
     }
 }
 
-// Fused restriction gather + unit-L sweep + U sweep + prolongation.
-// MODE 0: restricted (RAS / global ILU0): y[out] = z (+ coarse term cz[s]).
-// MODE 1: additive (ASM): zbuf[row0 + row] = z, summed by k_asm_gather.
+// Row data of one thread for one level, prefetched a level ahead.
+template <typename T>
+struct RowPre {
+    int row;
+    int col[kKreg];
+    T val[kKreg];
+    T diag;
+};
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void row_prefetch(RowPre<T>& p, const Level& L, int q,
+                                             const int* __restrict__ rows,
+                                             const int* __restrict__ cols,
+                                             const T* __restrict__ vals,
+                                             const T* __restrict__ diags) {
+    p.row = -1;
+    if (q < L.size) {
+        p.row = rows[L.start + q];
+#pragma unroll
+        for (int k = 0; k < kKreg; ++k)
+            if (k < L.K) {
+                p.col[k] = cols[L.ell + k * L.size + q];
+                p.val[k] = vals[L.ell + k * L.size + q];
+            }
+        if (BWD) p.diag = diags[L.start + q];
+    }
+}
+
+// One level-scheduled sweep over a subdomain: forward (unit L) or backward
+// (U, division by the diagonal), z in place.  Each row subtracts its entries
+// in CSR order, as src/precond/ilu0.h:62-76 does.  The next level's row data
+// is loaded while the current level computes; levels wider than the block
+// loop without prefetch.
+template <typename T, bool BWD>
+__device__ __forceinline__ void sweep(T* z, const Level* __restrict__ lev, int nlev,
+                                      const int* __restrict__ rows,
+                                      const int* __restrict__ cols,
+                                      const T* __restrict__ vals,
+                                      const T* __restrict__ diags) {
+    if (nlev == 0) return;
+    const int t = threadIdx.x, B = blockDim.x;
+    const Level none{0, 0, 0, 0};
+    Level L0 = lev[0];
+    Level L1 = nlev > 1 ? lev[1] : none;
+    RowPre<T> cur;
+    row_prefetch<T, BWD>(cur, L0, t, rows, cols, vals, diags);
+    for (int l = 0; l < nlev; ++l) {
+        const Level L2 = (l + 2 < nlev) ? lev[l + 2] : none;
+        RowPre<T> nxt;
+        row_prefetch<T, BWD>(nxt, L1, t, rows, cols, vals, diags);
+        if (cur.row >= 0) {
+            T acc = z[cur.row];
+#pragma unroll
+            for (int k = 0; k < kKreg; ++k)
+                if (k < L0.K) acc -= cur.val[k] * z[cur.col[k]];
+            for (int k = kKreg; k < L0.K; ++k)
+                acc -= vals[L0.ell + k * L0.size + t] * z[cols[L0.ell + k * L0.size + t]];
+            if (BWD) acc = acc / cur.diag;
+            z[cur.row] = acc;
+        }
+        for (int q = t + B; q < L0.size; q += B) {
+            const int row = rows[L0.start + q];
+            T acc = z[row];
+            for (int k = 0; k < L0.K; ++k)
+                acc -= vals[L0.ell + k * L0.size + q] * z[cols[L0.ell + k * L0.size + q]];
+            if (BWD) acc = acc / diags[L0.start + q];
+            z[row] = acc;
+        }
+        __syncthreads();
+        cur = nxt;
+        L0 = L1;
+        L1 = L2;
+    }
+}
+
+// Fused restriction gather + unit-L sweep + U sweep + prolongation, one
+// workgroup per subdomain taken from a queue sorted by subdomain nonzeros.
+// MODE 0: restricted (RAS / global ILU0): y[g] = z (+ coarse term cz[s]) for
+//         the owned rows (src/precond/ras.h:55-68).
+// MODE 1: additive (ASM): the whole local solution goes to zout, summed by
+//         k_asm_gather (src/precond/asm.h:54-58).
 template <typename T, bool ZLDS, int MODE>
 __global__ void __launch_bounds__(kTrsvThreads)
 k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
-                const int* __restrict__ f_lev, const int* __restrict__ f_row,
-                const int* __restrict__ f_gid, const int* __restrict__ f_off,
-                const int* __restrict__ f_K, const int* __restrict__ f_col,
-                const T* __restrict__ f_val, const int* __restrict__ b_lev,
-                const int* __restrict__ b_row, const int* __restrict__ b_out,
-                const int* __restrict__ b_off, const int* __restrict__ b_K,
+                const Level* __restrict__ flev, const int* __restrict__ f_row,
+                const int* __restrict__ f_col, const T* __restrict__ f_val,
+                const Level* __restrict__ blev, const int* __restrict__ b_row,
                 const int* __restrict__ b_col, const T* __restrict__ b_val,
-                const T* __restrict__ b_diag, const double* __restrict__ x,
+                const T* __restrict__ b_diag, const int* __restrict__ dofs,
+                const int* __restrict__ own, const double* __restrict__ x,
                 double* __restrict__ y, T* __restrict__ zglob, T* __restrict__ zout,
                 const T* __restrict__ cz, const int* __restrict__ guard) {
     if (guard && *guard == 0) return;
@@ -297,50 +357,28 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     const int s = order[blockIdx.x];
     const SubInfo si = subs[s];
     T* z = ZLDS ? reinterpret_cast<T*>(lds_raw) : zglob + si.zoff;
-    const int wave = threadIdx.x / kChunk, lane = threadIdx.x % kChunk;
-    const int nw = blockDim.x / kChunk;
+    const int* d = dofs + si.row0;
+    for (int i = threadIdx.x; i < si.ns; i += blockDim.x) z[i] = (T)x[d[i]];  // R_s x
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
-    // forward: z = L^{-1} R_s x
-    for (int l = si.flev0; l < si.flev0 + si.nflev; ++l) {
-        const int c1 = f_lev[l + 1];
-        for (int c = f_lev[l] + wave; c < c1; c += nw) {
-            const int q = c * kChunk + lane;
-            const int row = f_row[q];
-            const int off = f_off[c] + lane, K = f_K[c];
-            if (row >= 0) {
-                T t = (T)x[f_gid[q]];
-                for (int k = 0; k < K; ++k) t -= f_val[off + k * kChunk] * z[f_col[off + k * kChunk]];
-                z[row] = t;
+    sweep<T, false>(z, flev + si.flev0, si.nflev, f_row, f_col, f_val, nullptr);
+    sweep<T, true>(z, blev + si.blev0, si.nblev, b_row, b_col, b_val, b_diag);
+    if (MODE == 0) {
+        const int* ow = own + si.own0;
+        if (cz) {
+            const T add = cz[s];
+            for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
+                const int li = ow[k];
+                y[d[li]] = (double)(z[li] + add);
+            }
+        } else {
+            for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
+                const int li = ow[k];
+                y[d[li]] = (double)z[li];
             }
         }
-        __syncthreads();
-    }
-    const T add = (MODE == 0 && cz) ? cz[s] : (T)0;
-    // backward: z = U^{-1} z, then prolongation
-    for (int l = si.blev0; l < si.blev0 + si.nblev; ++l) {
-        const int c1 = b_lev[l + 1];
-        for (int c = b_lev[l] + wave; c < c1; c += nw) {
-            const int q = c * kChunk + lane;
-            const int row = b_row[q];
-            const int off = b_off[c] + lane, K = b_K[c];
-            if (row >= 0) {
-                T t = z[row];
-                for (int k = 0; k < K; ++k) t -= b_val[off + k * kChunk] * z[b_col[off + k * kChunk]];
-                t = t / b_diag[q];
-                z[row] = t;
-                if (MODE == 0) {
-                    const int g = b_out[q];
-                    if (g >= 0) {
-                        if (cz) y[g] = (double)(t + add);
-                        else y[g] = (double)t;
-                    }
-                } else {
-                    zout[si.row0 + row] = t;
-                }
-            }
-        }
-        __syncthreads();
+    } else {
+        for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = z[i];
     }
 }
 
@@ -364,17 +402,25 @@ __global__ void k_asm_gather(int n, const int* __restrict__ ptr, const int* __re
 }
 
 // Coarse restriction r0[s] = sum_{k in owned_s} c_s x[k]
-// (src/precond/coarse_space.h:63-70), one workgroup per subdomain.
+// (src/precond/coarse_space.h:63-70).  Subdomain s is split over `split`
+// workgroups; partial sums go to part[s * split + j] and are added in fixed
+// order by the coarse solve.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_coarse_restrict(const int* __restrict__ own_ptr, const int* __restrict__ own_idx,
-                  const T* __restrict__ c, const double* __restrict__ x, T* __restrict__ r0,
+k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __restrict__ dofs,
+                  const int* __restrict__ own, const T* __restrict__ c,
+                  const double* __restrict__ x, T* __restrict__ part,
                   const int* __restrict__ guard) {
     if (guard && *guard == 0) return;
-    const int s = blockIdx.x;
+    const int s = blockIdx.x / split, j = blockIdx.x % split;
+    const SubInfo si = subs[s];
+    const int per = (si.nown + split - 1) / split;
+    const int a = j * per, b = min(si.nown, a + per);
     const T cs = c[s];
+    const int* d = dofs + si.row0;
+    const int* ow = own + si.own0;
     T acc = 0;
-    for (int k = own_ptr[s] + threadIdx.x; k < own_ptr[s + 1]; k += 256) acc += cs * (T)x[own_idx[k]];
+    for (int k = a + threadIdx.x; k < b; k += 256) acc += cs * (T)x[d[ow[k]]];
     __shared__ T red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -382,7 +428,7 @@ k_coarse_restrict(const int* __restrict__ own_ptr, const int* __restrict__ own_i
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) r0[s] = red[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // Coarse solve z0 = A0^{-1} r0 and cz[s] = c_s z0[s].
@@ -391,13 +437,18 @@ k_coarse_restrict(const int* __restrict__ own_ptr, const int* __restrict__ own_i
 // mode 1: one dense mat-vec with the precomputed inverse (stored transposed).
 template <typename T>
 __global__ void __launch_bounds__(1024)
-k_coarse_solve(int nc, int mode, const T* __restrict__ LUt, const int* __restrict__ piv,
-               const T* __restrict__ invT, const T* __restrict__ r0, const T* __restrict__ c,
-               T* __restrict__ cz, const int* __restrict__ guard) {
+k_coarse_solve(int nc, int mode, int split, const T* __restrict__ LUt,
+               const int* __restrict__ piv, const T* __restrict__ invT,
+               const T* __restrict__ r0part, const T* __restrict__ c, T* __restrict__ cz,
+               const int* __restrict__ guard) {
     if (guard && *guard == 0) return;
     extern __shared__ unsigned char lds_raw[];
     T* xs = reinterpret_cast<T*>(lds_raw);
-    for (int i = threadIdx.x; i < nc; i += blockDim.x) xs[i] = r0[i];
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        T r = 0;
+        for (int j = 0; j < split; ++j) r += r0part[i * split + j];
+        xs[i] = r;
+    }
     __syncthreads();
     if (mode == 1) {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -553,18 +604,17 @@ static void setup_jacobi(sg_precond* M, const sg_csr* A) {
     M->inv_diag.upload(inv, M->ctx->stream);
 }
 
-static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool restricted, bool additive) {
+static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     sg_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const HostCsr& H = A->host;
-    const int ns_total = M->nparts;
-    std::vector<HostCsr> local(ns_total);
-    std::vector<std::vector<int>> diag(ns_total);
-    std::vector<SubPack> pk(ns_total);
-    std::vector<std::vector<int>> owned_local(ns_total);
+    const int nsub = M->nparts;
     const bool global_ilu = (M->kind == SG_PC_ILU0);
+    std::vector<HostCsr> local(nsub);
+    std::vector<std::vector<int>> diag(nsub), own(nsub);
+    std::vector<SweepPack> fw(nsub), bw(nsub);
 
-    run_parallel(ns_total, [&](int s) {
+    run_parallel(nsub, [&](int s) {
         const std::vector<int>& d = M->dofs[s];
         local[s] = global_ilu ? H : extract_submatrix(H, d);
         HostCsr& L = local[s];
@@ -575,138 +625,117 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool restricted, boo
         diag[s] = diag_positions(L);
         for (int i = 0; i < L.n; ++i)
             if (diag[s][i] < 0) throw std::runtime_error("ILU(0) needs a diagonal entry in every row");
-        std::vector<int> lf = lower_levels(L), lb = upper_levels(L);
-        std::vector<int> gid(L.n), out(L.n, -1);
-        // owned flags: ras.h:34-41 (owned DOFs mapped into subdomain_dofs)
-        std::vector<char> own(L.n, 0);
-        if (global_ilu) {
-            for (int i = 0; i < L.n; ++i) { gid[i] = i; own[i] = 1; }
-        } else {
-            for (int i = 0; i < L.n; ++i) {
-                gid[i] = d[i];
-                own[i] = M->part[d[i]] == s;
-            }
-        }
-        for (int i = 0; i < L.n; ++i) out[i] = own[i] ? gid[i] : -1;
-        SubPack& P = pk[s];
-        pack_sweep(L, diag[s], lf, true, gid, P.f_lev, P.f_row, P.f_gid, P.f_off, P.f_K, P.f_col,
-                   P.f_src, nullptr);
-        std::vector<int> dummy_gid;
-        pack_sweep(L, diag[s], lb, false, out, P.b_lev, P.b_row, P.b_out, P.b_off, P.b_K, P.b_col,
-                   P.b_src, &P.b_dsrc);
+        // owned rows in local numbering (src/precond/ras.h:34-41)
+        for (int i = 0; i < L.n; ++i)
+            if (global_ilu || M->part[d[i]] == s) own[s].push_back(i);
+        pack_levels(L, diag[s], lower_levels(L), true, fw[s]);
+        pack_levels(L, diag[s], upper_levels(L), false, bw[s]);
     });
-    (void)restricted;
 
     // concatenate
     Plan pl;
-    pl.subs.resize(ns_total);
-    int row0 = 0, nnz0 = 0, zoff = 0, fchunk0 = 0, bchunk0 = 0, fell0 = 0, bell0 = 0;
+    pl.subs.resize(nsub);
+    long long row0 = 0, nnz0 = 0, zoff = 0, fpos = 0, bpos = 0, fell = 0, bell = 0;
     pl.lrp.push_back(0);
-    M->sub_nnz.assign(ns_total, 0);
-    for (int s = 0; s < ns_total; ++s) {
-        HostCsr& L = local[s];
-        SubPack& P = pk[s];
+    M->sub_nnz.assign(nsub, 0);
+    M->levels_fwd = M->levels_bwd = 0;
+    M->max_levels = 0;
+    for (int s = 0; s < nsub; ++s) {
+        const HostCsr& L = local[s];
         SubInfo& si = pl.subs[s];
         si.ns = L.n;
-        si.row0 = row0;
-        si.zoff = zoff;
-        si.flev0 = (int)pl.f_lev.size();
-        si.nflev = (int)P.f_lev.size() - 1;
-        for (int l = 0; l < si.nflev; ++l) pl.f_lev.push_back(P.f_lev[l] + fchunk0);
-        si.blev0 = (int)pl.b_lev.size();
-        si.nblev = (int)P.b_lev.size() - 1;
-        for (int l = 0; l < si.nblev; ++l) pl.b_lev.push_back(P.b_lev[l] + bchunk0);
-        // each level list needs its closing boundary: store [start..] and the
-        // end of the last level as the start of the next subdomain's first
-        // level; append explicit terminators instead to keep levels self-contained
-        pl.f_lev.push_back(P.f_lev[si.nflev] + fchunk0);
-        pl.b_lev.push_back(P.b_lev[si.nblev] + bchunk0);
-        for (size_t c = 0; c < P.f_K.size(); ++c) {
-            pl.f_off.push_back(P.f_off[c] + fell0);
-            pl.f_K.push_back(P.f_K[c]);
-        }
-        for (size_t c = 0; c < P.b_K.size(); ++c) {
-            pl.b_off.push_back(P.b_off[c] + bell0);
-            pl.b_K.push_back(P.b_K[c]);
-        }
-        pl.f_row.insert(pl.f_row.end(), P.f_row.begin(), P.f_row.end());
-        pl.f_gid.insert(pl.f_gid.end(), P.f_gid.begin(), P.f_gid.end());
-        pl.b_row.insert(pl.b_row.end(), P.b_row.begin(), P.b_row.end());
-        for (int g : P.b_out) pl.b_out.push_back(additive ? -1 : g);
-        pl.f_col.insert(pl.f_col.end(), P.f_col.begin(), P.f_col.end());
-        pl.b_col.insert(pl.b_col.end(), P.b_col.begin(), P.b_col.end());
-        for (int k : P.f_src) pl.f_src.push_back(k >= 0 ? k + nnz0 : -1);
-        for (int k : P.b_src) pl.b_src.push_back(k >= 0 ? k + nnz0 : -1);
-        for (int k : P.b_dsrc) pl.b_dsrc.push_back(k >= 0 ? k + nnz0 : -1);
+        si.row0 = (int)row0;
+        si.zoff = (int)zoff;
+        si.flev0 = (int)pl.flev.size();
+        si.nflev = (int)fw[s].lev.size();
+        si.blev0 = (int)pl.blev.size();
+        si.nblev = (int)bw[s].lev.size();
+        si.own0 = (int)pl.own.size();
+        si.nown = (int)own[s].size();
+        for (Level lv : fw[s].lev) pl.flev.push_back(Level{lv.start + (int)fpos, lv.size, lv.K, lv.ell + (int)fell});
+        for (Level lv : bw[s].lev) pl.blev.push_back(Level{lv.start + (int)bpos, lv.size, lv.K, lv.ell + (int)bell});
+        pl.f_row.insert(pl.f_row.end(), fw[s].row.begin(), fw[s].row.end());
+        pl.b_row.insert(pl.b_row.end(), bw[s].row.begin(), bw[s].row.end());
+        pl.f_col.insert(pl.f_col.end(), fw[s].col.begin(), fw[s].col.end());
+        pl.b_col.insert(pl.b_col.end(), bw[s].col.begin(), bw[s].col.end());
+        for (int k : fw[s].src) pl.f_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
+        for (int k : bw[s].src) pl.b_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
+        for (int k : bw[s].dsrc) pl.b_dsrc.push_back((int)(k + nnz0));
         for (int i = 0; i < L.n; ++i) {
-            pl.lrp.push_back(L.rp[i + 1] + nnz0);
-            pl.ldiag.push_back(diag[s][i] + nnz0);
+            pl.lrp.push_back((int)(L.rp[i + 1] + nnz0));
+            pl.ldiag.push_back((int)(diag[s][i] + nnz0));
         }
         pl.lci.insert(pl.lci.end(), L.ci.begin(), L.ci.end());
         pl.lval.insert(pl.lval.end(), L.v.begin(), L.v.end());
+        if (global_ilu) {
+            for (int i = 0; i < L.n; ++i) pl.dofs.push_back(i);
+        } else {
+            pl.dofs.insert(pl.dofs.end(), M->dofs[s].begin(), M->dofs[s].end());
+        }
+        pl.own.insert(pl.own.end(), own[s].begin(), own[s].end());
         M->sub_nnz[s] = L.nnz();
+        M->levels_fwd += si.nflev;
+        M->levels_bwd += si.nblev;
+        M->max_levels = std::max(M->max_levels, std::max(si.nflev, si.nblev));
         pl.max_ns = std::max(pl.max_ns, L.n);
         row0 += L.n;
         nnz0 += L.nnz();
         zoff += L.n + 1;
-        fchunk0 += (int)P.f_K.size();
-        bchunk0 += (int)P.b_K.size();
-        fell0 += P.f_col.size();
-        bell0 += P.b_col.size();
-        // the f_lev/b_lev arrays per subdomain have nflev+1 entries
+        fpos += fw[s].row.size();
+        bpos += bw[s].row.size();
+        fell += fw[s].col.size();
+        bell += bw[s].col.size();
     }
-    pl.total_rows = row0;
-    // queue: largest subdomains first
-    pl.order.resize(ns_total);
+    if (nnz0 > INT32_MAX || fell > INT32_MAX || bell > INT32_MAX || zoff > INT32_MAX)
+        fail(SG_E_ARG, "subdomain data exceeds 32-bit indexing");
+    pl.total_rows = (int)row0;
+    pl.order.resize(nsub);
     std::iota(pl.order.begin(), pl.order.end(), 0);
     std::stable_sort(pl.order.begin(), pl.order.end(),
                      [&](int a, int b) { return M->sub_nnz[a] > M->sub_nnz[b]; });
 
-    M->nsub = ns_total;
+    M->nsub = nsub;
     M->lrp_host = pl.lrp;
     M->d_subs.upload(pl.subs, st);
     M->d_order.upload(pl.order, st);
-    M->f_lev.upload(pl.f_lev, st);
+    M->flev.upload(pl.flev, st);
+    M->blev.upload(pl.blev, st);
     M->f_row.upload(pl.f_row, st);
-    M->f_gid.upload(pl.f_gid, st);
-    M->f_off.upload(pl.f_off, st);
-    M->f_K.upload(pl.f_K, st);
     M->f_col.upload(pl.f_col, st);
     M->f_src.upload(pl.f_src, st);
-    M->b_lev.upload(pl.b_lev, st);
     M->b_row.upload(pl.b_row, st);
-    M->b_out.upload(pl.b_out, st);
-    M->b_off.upload(pl.b_off, st);
-    M->b_K.upload(pl.b_K, st);
     M->b_col.upload(pl.b_col, st);
     M->b_src.upload(pl.b_src, st);
     M->b_dsrc.upload(pl.b_dsrc, st);
+    M->d_dofs.upload(pl.dofs, st);
+    M->d_own.upload(pl.own, st);
     M->lrp.upload(pl.lrp, st);
     M->lci.upload(pl.lci, st);
     M->ldiag.upload(pl.ldiag, st);
     M->lval.upload(pl.lval, st);
 
     // device factorisation in FP64
-    hipLaunchKernelGGL(k_ilu0_factor, dim3(ns_total), dim3(kTrsvThreads), 0, st, M->d_subs.p,
-                       M->f_lev.p, M->f_row.p, M->lrp.p, M->lci.p, M->ldiag.p, M->lval.p);
+    hipLaunchKernelGGL(k_ilu0_factor, dim3(nsub), dim3(kTrsvThreads), 0, st, M->d_subs.p,
+                       M->flev.p, M->f_row.p, M->lrp.p, M->lci.p, M->ldiag.p, M->lval.p);
     SG_HIP(hipGetLastError());
     // pack values for the sweeps
     const bool fp32 = M->opts.precision == SG_FP32;
     int nf = (int)pl.f_src.size(), nb = (int)pl.b_src.size(), nd = (int)pl.b_dsrc.size();
+    auto pack = [&](auto* dummy, sg::DevBuf<int>& src, auto& out, int count) {
+        using T = std::remove_pointer_t<decltype(dummy)>;
+        out.alloc(count);
+        if (count)
+            hipLaunchKernelGGL(k_pack<T>, dim3(grid_for(count)), dim3(256), 0, st, count, src.p,
+                               M->lval.p, out.p);
+    };
     if (fp32) {
-        M->f_v32.alloc(nf);
-        M->b_v32.alloc(nb);
-        M->b_d32.alloc(nd);
-        if (nf) hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(nf)), dim3(256), 0, st, nf, M->f_src.p, M->lval.p, M->f_v32.p);
-        if (nb) hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(nb)), dim3(256), 0, st, nb, M->b_src.p, M->lval.p, M->b_v32.p);
-        if (nd) hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(nd)), dim3(256), 0, st, nd, M->b_dsrc.p, M->lval.p, M->b_d32.p);
+        pack((float*)nullptr, M->f_src, M->f_v32, nf);
+        pack((float*)nullptr, M->b_src, M->b_v32, nb);
+        pack((float*)nullptr, M->b_dsrc, M->b_d32, nd);
     } else {
-        M->f_v64.alloc(nf);
-        M->b_v64.alloc(nb);
-        M->b_d64.alloc(nd);
-        if (nf) hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(nf)), dim3(256), 0, st, nf, M->f_src.p, M->lval.p, M->f_v64.p);
-        if (nb) hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(nb)), dim3(256), 0, st, nb, M->b_src.p, M->lval.p, M->b_v64.p);
-        if (nd) hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(nd)), dim3(256), 0, st, nd, M->b_dsrc.p, M->lval.p, M->b_d64.p);
+        pack((double*)nullptr, M->f_src, M->f_v64, nf);
+        pack((double*)nullptr, M->b_src, M->b_v64, nb);
+        pack((double*)nullptr, M->b_dsrc, M->b_d64, nd);
     }
     SG_HIP(hipGetLastError());
 
@@ -736,12 +765,12 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool restricted, boo
         // cover lists: for global g, positions row0_s + local index for every
         // subdomain containing g, ascending s (asm.h:54-58 single-thread order)
         std::vector<int> cnt(M->n + 1, 0);
-        for (int s = 0; s < ns_total; ++s)
+        for (int s = 0; s < nsub; ++s)
             for (int g : M->dofs[s]) cnt[g + 1]++;
         for (int g = 0; g < M->n; ++g) cnt[g + 1] += cnt[g];
         std::vector<int> pos(cnt[M->n]);
         std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
-        for (int s = 0; s < ns_total; ++s)
+        for (int s = 0; s < nsub; ++s)
             for (int i = 0; i < (int)M->dofs[s].size(); ++i)
                 pos[fillp[M->dofs[s][i]]++] = pl.subs[s].row0 + i;
         M->cov_ptr.upload(cnt, st);
@@ -804,15 +833,8 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
                        M->d_piv.p, M->d_inv64.p);
     SG_HIP(hipGetLastError());
     M->d_c64.upload(c, st);
-    // owned lists for the restriction
-    std::vector<int> own_ptr(nc + 1, 0), own_idx;
-    own_idx.reserve(M->n);
-    for (int s = 0; s < nc; ++s) {
-        own_idx.insert(own_idx.end(), M->owned[s].begin(), M->owned[s].end());
-        own_ptr[s + 1] = (int)own_idx.size();
-    }
-    M->d_own_ptr.upload(own_ptr, st);
-    M->d_own_idx.upload(own_idx, st);
+    // the restriction splits each subdomain over enough workgroups to fill the GPU
+    M->restrict_split = std::max(1, std::min(64, (4 * ctx->num_cu + nc - 1) / nc));
     if (M->opts.precision == SG_FP32) {
         int nn = nc * nc;
         M->d_LUt32.alloc(nn);
@@ -824,10 +846,10 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
                            M->d_inv64.p, M->d_inv32.p);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(1), dim3(256), 0, st, nc, M->d_c64.p,
                            M->d_c32.p);
-        M->d_r0_32.alloc(nc);
+        M->d_r0_32.alloc((size_t)nc * M->restrict_split);
         M->d_cz32.alloc(nc);
     } else {
-        M->d_r0_64.alloc(nc);
+        M->d_r0_64.alloc((size_t)nc * M->restrict_split);
         M->d_cz64.alloc(nc);
     }
     SG_HIP(hipGetLastError());
@@ -853,7 +875,7 @@ static void setup(sg_precond* M, const sg_csr* A) {
             M->owned[0].resize(M->n);
             std::iota(M->owned[0].begin(), M->owned[0].end(), 0);
             M->dofs[0] = M->owned[0];
-            setup_ilu_batch(M, A, true, false);
+            setup_ilu_batch(M, A, false);
             break;
         case SG_PC_ASM:
         case SG_PC_RAS:
@@ -867,13 +889,13 @@ static void setup(sg_precond* M, const sg_csr* A) {
                 for (int p : M->part)
                     if (p < 0 || p >= np) fail(SG_E_PARTITION, "partition entry out of range");
             } else {
-                M->part = partition_graph(A->host, np, M->opts.ufactor);
+                M->part = partition_graph(A->host, np, M->opts.ufactor, M->opts.weighted_partition != 0);
             }
             build_sets(A->host, M->part, np, std::max(0, M->opts.overlap), M->owned, M->dofs);
             for (int s = 0; s < np; ++s)
                 if (M->owned[s].empty()) fail(SG_E_PARTITION, "partitioning failed: empty subdomain");
             bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
-            setup_ilu_batch(M, A, !additive, additive);
+            setup_ilu_batch(M, A, additive);
             if (M->kind == SG_PC_TWOLEVEL_ASM || M->kind == SG_PC_TWOLEVEL_RAS) setup_coarse(M, A);
             if (additive) {
                 M->d_part.upload(M->part, ctx->stream);
@@ -892,9 +914,8 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const int* g
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE>), dim3(M->nsub), dim3(kTrsvThreads),
                        ZLDS ? M->lds_bytes : 0, M->ctx->stream, M->d_subs.p, M->d_order.p,
-                       M->f_lev.p, M->f_row.p, M->f_gid.p, M->f_off.p, M->f_K.p, M->f_col.p, fv,
-                       M->b_lev.p, M->b_row.p, M->b_out.p, M->b_off.p, M->b_K.p, M->b_col.p, bv,
-                       bd, x, y, zg, zo, cz, guard);
+                       M->flev.p, M->f_row.p, M->f_col.p, fv, M->blev.p, M->b_row.p, M->b_col.p,
+                       bv, bd, M->d_dofs.p, M->d_own.p, x, y, zg, zo, cz, guard);
 }
 
 template <typename T>
@@ -924,11 +945,11 @@ template <typename T>
 static const T* coarse_launch(sg_precond* M, const double* x, const int* guard, const T* LUt,
                               const T* inv, const T* c, T* r0, T* cz) {
     hipStream_t st = M->ctx->stream;
-    int nc = M->nc;
-    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(nc), dim3(256), 0, st, M->d_own_ptr.p,
-                       M->d_own_idx.p, c, x, r0, guard);
+    int nc = M->nc, split = M->restrict_split;
+    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(nc * split), dim3(256), 0, st, split,
+                       M->d_subs.p, M->d_dofs.p, M->d_own.p, c, x, r0, guard);
     hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), (size_t)nc * sizeof(T), st, nc,
-                       M->opts.coarse_solve, LUt, M->d_piv.p, inv, r0, c, cz, guard);
+                       M->opts.coarse_solve, split, LUt, M->d_piv.p, inv, r0, c, cz, guard);
     SG_HIP(hipGetLastError());
     return cz;
 }
@@ -1097,6 +1118,38 @@ sg_status sg_precond_ilu_values(const sg_precond* M, int s, int* nnz, double* va
             SG_HIP(hipStreamSynchronize(M->ctx->stream));
         }
     });
+}
+
+sg_status sg_partition(int n, const int* row_ptr, const int* col_idx, const double* vals,
+                       int nparts, int ufactor, int weighted, int* part) {
+    if (n < 0 || !row_ptr || !part || (row_ptr[n] > 0 && !col_idx) ||
+        (weighted && row_ptr[n] > 0 && !vals))
+        return SG_E_ARG;
+    return run_guarded(nullptr, [&] {
+        HostCsr H;
+        H.n = n;
+        H.ncols = n;
+        H.rp.assign(row_ptr, row_ptr + n + 1);
+        H.ci.assign(col_idx, col_idx + row_ptr[n]);
+        if (weighted) H.v.assign(vals, vals + row_ptr[n]);
+        std::vector<int> p = partition_graph(H, nparts, ufactor, weighted != 0);
+        std::copy(p.begin(), p.end(), part);
+    });
+}
+
+sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info) {
+    if (!M || !info) return SG_E_ARG;
+    if (!M->ready) return SG_E_STATE;
+    *info = sg_precond_info{};
+    info->nsub = M->nsub;
+    info->z_in_lds = M->z_in_lds ? 1 : 0;
+    info->nc = M->nc;
+    for (size_t s = 0; s < M->dofs.size() && M->nsub; ++s) info->n_ext += (long long)M->dofs[s].size();
+    for (int v : M->sub_nnz) info->nnz_ext += v;
+    info->levels_fwd = M->levels_fwd;
+    info->levels_bwd = M->levels_bwd;
+    info->max_levels = M->max_levels;
+    return SG_OK;
 }
 
 sg_status sg_precond_coarse(const sg_precond* M, int* nc, double* A0, double* LU, int* piv) {

@@ -1,0 +1,88 @@
+"""Host-side setup of libsgamd.so that needs no GPU: the C-ABI library loads,
+exports every declared symbol, and the built-in partitioner (the stand-in for
+the reference's METIS call, src/decomp/domain_decomp.h:23-58) behaves.
+
+Partitions themselves are "parity unpinned": METIS 5.1.0 is absent, so the
+checks are properties (balance within ufactor, determinism, all parts
+non-empty, cut quality on grids with a known optimum)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_bind as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def sg():
+    import __graft_entry__
+    return __graft_entry__.load_package()
+
+
+def test_library_loads_and_exports_declared_symbols(sg):
+    import ctypes
+    hdr = open(os.path.join(ROOT, "include", "sg_capi.h")).read()
+    declared = sorted(set(re.findall(r"\b(sg_[a-z0-9_]+)\s*\(", hdr)))
+    lib = ctypes.CDLL(sg.LIB_PATH)
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(declared) >= 30
+
+
+def _cut(rp, ci, part):
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return int(np.sum(part[rows] != part[ci]) // 2)
+
+
+@pytest.mark.parametrize("n_side,nparts,ideal", [(16, 8, 3 * 256), (32, 64, 9 * 1024)])
+def test_partition_cube_reaches_block_optimum(sg, n_side, nparts, ideal):
+    A = O.poisson3d(n_side, n_side, n_side)
+    part = sg.partition(A.rp, A.ci, nparts)
+    counts = np.bincount(part, minlength=nparts)
+    assert counts.min() == counts.max() == A.n // nparts
+    assert _cut(A.rp, A.ci, part) == ideal
+
+
+def test_partition_properties(sg):
+    A = O.poisson2d(61, 47)
+    for nparts, uf in ((1, 0), (2, 0), (7, 30), (13, 300)):
+        part = sg.partition(A.rp, A.ci, nparts, uf)
+        assert part.min() >= 0 and part.max() == nparts - 1
+        counts = np.bincount(part, minlength=nparts)
+        if nparts == 1:
+            assert np.all(part == 0)
+            continue
+        limit = (1.0 + (uf or 30) / 1000.0) * A.n / nparts + 2
+        assert counts.min() > 0 and counts.max() <= limit, (nparts, counts)
+        assert np.array_equal(part, sg.partition(A.rp, A.ci, nparts, uf))  # deterministic
+
+
+def test_weighted_partition_cuts_weak_couplings(sg):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "p", os.path.join(ROOT, "schwarz-gmres-dashboard_amd", "problems.py"))
+    P = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(P)
+    kappa = P.block_jump_kappa(32, 8, 3)
+    rp, ci, v = P.diffusion3d(kappa)
+    pu = sg.partition(rp, ci, 16, 300)
+    pw = sg.partition(rp, ci, 16, 300, vals=v, weighted=True)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    off = rows != ci
+
+    def wcut(part):
+        return float(np.sum(np.abs(v[off & (part[rows] != part[ci])])))
+
+    assert wcut(pw) < 0.9 * wcut(pu)
+    counts = np.bincount(pw, minlength=16)
+    assert counts.min() > 0 and counts.max() <= 1.3 * len(pw) / 16 + 2
+
+
+def test_partition_errors(sg):
+    A = O.poisson2d(4, 4)
+    with pytest.raises(sg.SgError) as e:
+        sg.partition(A.rp, A.ci, 17)
+    assert e.value.status == 4
