@@ -43,11 +43,11 @@ def load_problems():
 
 CONFIGS = {
     "c3": dict(workload="diffusion3d_jump_160^3", restart=40, nparts=256, overlap=1,
-               ufactor=300, weighted=True),
+               ufactor=300, weighted=True, max_iter=10000, base_max_iter=12000),
     "c2": dict(workload="poisson3d_128^3", restart=50, nparts=64, overlap=2, ufactor=0,
-               weighted=False),
+               weighted=False, max_iter=2000, base_max_iter=4000),
     "c1": dict(workload="poisson2d_256^2", restart=30, nparts=1, overlap=0, ufactor=0,
-               weighted=False),
+               weighted=False, max_iter=2000, base_max_iter=2000),
 }
 
 
@@ -161,8 +161,8 @@ def main():
         M = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
                        precision="fp32", partition=part, coarse_solve=1).setup(A)
         method = "fgmres"
-        solve_kw = dict(method="fgmres", restart=cfg["restart"], max_iter=2000, rtol=1e-8,
-                        ortho="cgs2")
+        solve_kw = dict(method="fgmres", restart=cfg["restart"], max_iter=cfg["max_iter"],
+                        rtol=1e-8, ortho="cgs2")
     ctx.sync()
     t_setup = time.perf_counter() - t0
     info = M.stats()
@@ -196,13 +196,15 @@ def main():
     if not args.no_baseline and args.config != "c1":
         Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
                         precision="fp64", partition=part).setup(A)
-        base_kw = dict(method="gmres", restart=cfg["restart"], max_iter=4000, rtol=1e-8,
-                       ortho="mgs")
+        base_kw = dict(method="gmres", restart=cfg["restart"], max_iter=cfg["base_max_iter"],
+                       rtol=1e-8, ortho="mgs")
         rb = run_solves(sg, ctx, A, Mb, db, dx, 1, **base_kw)[0]
         extra["fp64_ras_gmres"] = {"s": round(rb["wall_s"], 4), "iterations": rb["iterations"],
                                    "converged": rb["converged"],
                                    "rel_true_residual": rb["rel_true_residual"]}
+        # a baseline that hit its iteration cap gives a lower bound on the speedup
         extra["speedup_vs_fp64_ras_gmres"] = round(rb["wall_s"] / np.mean(times), 2)
+        extra["speedup_is_lower_bound"] = not rb["converged"]
         print(f"# baseline gmres+ras fp64: {rb['iterations']} it {rb['wall_s']:.3f} s "
               f"conv={rb['converged']}", file=sys.stderr, flush=True)
         del Mb

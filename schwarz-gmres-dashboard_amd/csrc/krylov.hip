@@ -7,9 +7,13 @@
 // tangent-form Givens rotations (gmres.cpp:11-23), back substitution and the
 // x update in the reference's order, max_iter bounding cycles with the total
 // step count checked after each cycle.  Gram-Schmidt runs either as MGS in the
-// reference's order (one fused kernel per projection) or as CGS2 (three fused
-// passes per step).  All reductions use a fixed tree, so a solve is
-// deterministic run to run.
+// reference's order or as a two-pass CGS2 whose re-orthogonalisation
+// coefficients come from the Gram matrix.
+//
+// Reductions: a grid of blocks, each over one contiguous range of the vector,
+// writes per-block partial sums; a one-block "finish" kernel adds them in a
+// fixed order and runs the scalar epilogue.  No grid-wide atomics or fences,
+// and the result is bitwise reproducible run to run.
 //
 // The host launches a whole restart cycle without reading anything back; each
 // kernel of a step checks a device flag and returns at once when the cycle has
@@ -28,12 +32,13 @@ sg_context* precond_ctx(const sg_precond* M);
 int precond_n(const sg_precond* M);
 bool precond_ready(const sg_precond* M);
 
-constexpr int kT = 256;           // threads per reduction block
-constexpr int kMaxBlocks = 4096;  // partial-sum slots per column
-constexpr int kMaxCols = 128;     // restart + 1
+constexpr int kT = 256;                 // threads per reduction block
+constexpr int kMaxBlocks = 4096;        // partial-sum slots per column
+constexpr int kMaxCols = 128;           // restart + 1
 constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis vector)
-constexpr int kCB = 16;           // basis vectors per register block in the Gram passes
-constexpr int kEo = 2;            // elements per thread per tile in the Gram passes
+constexpr int kCB = 16;                 // basis vectors per register block
+constexpr int kEo = 2;                  // elements per thread per tile
+constexpr int kTile = kT * kEo;
 
 struct KState {
     int active;     // current cycle continues
@@ -59,13 +64,28 @@ struct Work {
     double* sn;    // m
     double* y;     // m
     double* h1;    // kRedCols: projection coefficients h
-    double* h2;    // kRedCols: raw dots V^T w and V^T v_j
+    double* h2;    // kRedCols: reduced dots
     double* G;     // kMaxCols^2: Gram matrix V^T V, row-major, stride kMaxCols
     double* hist;  // residual history
-    double* part;  // kRedCols * kMaxBlocks
-    unsigned* ticket;
+    double* part;  // kRedCols * kMaxBlocks block partial sums, column-major
     KState* st;
 };
+
+// Contiguous element range of block b: [b*chunk, min(n, (b+1)*chunk)).
+struct Split {
+    int n, chunk, nblk;
+};
+
+static Split make_split(int n, int max_blocks) {
+    Split s;
+    s.n = n;
+    int want = std::max(1, (n + kTile - 1) / kTile);
+    s.nblk = std::min(want, std::min(max_blocks, kMaxBlocks));
+    int c = (n + s.nblk - 1) / s.nblk;
+    s.chunk = (c + kTile - 1) / kTile * kTile;
+    s.nblk = std::max(1, (n + s.chunk - 1) / s.chunk);
+    return s;
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -128,186 +148,177 @@ __device__ void step_epilogue(const Work& W, int j, double hn) {
     }
 }
 
-// Block sums of ncols partial values in red[wave][col] -> partials, then the
-// last block reduces the partials of every column in fixed order into out[].
-__device__ bool finish_multi(const Work& W, int ncols, double (*red)[kRedCols], double* out) {
-    const int tid = threadIdx.x;
-    __shared__ bool last;
+// Adds the per-wave values red[wave][c] and stores column c's block partial.
+__device__ inline void store_partials(double (*red)[kRedCols], int ncols, double* part) {
     __syncthreads();
-    for (int i = tid; i < ncols; i += kT) {
+    for (int i = threadIdx.x; i < ncols; i += kT) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kT / 64; ++w) s += red[w][i];
-        W.part[i * kMaxBlocks + blockIdx.x] = s;
+        part[(size_t)i * kMaxBlocks + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(W.ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    if (ncols <= 8) {
-        // few columns: the whole block reduces each one
-        const int lane = tid & 63, wv = tid >> 6;
-        for (int i = 0; i < ncols; ++i) {
-            double s = 0.0;
-            for (int b = tid; b < (int)gridDim.x; b += kT) s += W.part[i * kMaxBlocks + b];
-            s = wave_sum(s);
-            if (lane == 0) red[wv][i] = s;
-        }
-        __syncthreads();
-        for (int i = tid; i < ncols; i += kT) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < kT / 64; ++w) s += red[w][i];
-            out[i] = s;
-        }
-    } else {
-        // many columns: one thread per column adds the block partials in order
-        for (int i = tid; i < ncols; i += kT) {
-            const double* pi = W.part + (size_t)i * kMaxBlocks;
-            double s = 0.0;
-            for (int b = 0; b < (int)gridDim.x; ++b) s += pi[b];
-            out[i] = s;
-        }
-    }
-    if (tid == 0) *W.ticket = 0u;
-    __syncthreads();
-    return true;
 }
 
-// Classical Gram-Schmidt with one re-orthogonalisation whose coefficients come
-// from the Gram matrix G = V^T V (two passes over V per step).
-// Pass 1 (k_gram_dot): p = V^T w and, lagged by one step, g = V^T v_j -- the
-// actual Gram row of the newest basis vector -- in one read of V; then
-// h = (2I - G) p, which equals G^{-1} p to first order in the loss of
-// orthogonality, i.e. the coefficients of projecting twice.
-// Pass 2 (k_gram_update): w -= V h and ||w||, then the step epilogue.
+// One block: out[c] = sum over blocks of part[c][b].  Columns are spread over
+// the wavefronts; a wavefront adds a column's partials lane-strided and then
+// across lanes, always in the same order.  Ends with __syncthreads.
+__device__ void reduce_partials(const double* part, int ncols, int nblk, double* out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = wv; i < ncols; i += nw) {
+        const double* pi = part + (size_t)i * kMaxBlocks;
+        double s = 0.0;
+        for (int b = lane; b < nblk; b += 64) s += pi[b];
+        s = wave_sum(s);
+        if (lane == 0) out[i] = s;
+    }
+    __syncthreads();
+}
+
+// ---- CGS2 with Gram-matrix re-orthogonalisation ----------------------------
+// Pass 1 (k_gram_dot + k_gram_finish): p = V^T w and, lagged by one step,
+// g = V^T v_j -- the actual Gram row of the newest basis vector -- in one
+// read of V; then h = (2I - G) p, which equals G^{-1} p to first order in the
+// loss of orthogonality, i.e. the coefficients of projecting twice.
+// Pass 2 (k_gram_update + k_norm_finish): w -= V h, ||w||, step epilogue.
+// Columns beyond ncols inside a register block are clamped to a valid column
+// and masked, so every load is unconditional and stays in flight.
 __global__ void __launch_bounds__(kT)
-k_gram_dot(int n, int ncols, const double* __restrict__ V, size_t ld,
+k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
            const double* __restrict__ w, Work W, int j) {
     if (!W.st->active) return;
     __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     const double* vj = V + (size_t)j * ld;
-    const int tile = kT * kEo;
     for (int cb = 0; cb < ncols; cb += kCB) {
+        const double* Vc[kCB];
+#pragma unroll
+        for (int c = 0; c < kCB; ++c) Vc[c] = V + (size_t)min(cb + c, ncols - 1) * ld;
         double aw[kCB], av[kCB];
 #pragma unroll
         for (int c = 0; c < kCB; ++c) aw[c] = av[c] = 0.0;
-        for (int base = blockIdx.x * tile + tid; base < n; base += gridDim.x * tile) {
+        for (int base = start + tid; base < end; base += kTile) {
+            int ee[kEo];
             double wr[kEo], vr[kEo];
 #pragma unroll
             for (int r = 0; r < kEo; ++r) {
                 const int e = base + r * kT;
-                wr[r] = e < n ? w[e] : 0.0;
-                vr[r] = e < n ? vj[e] : 0.0;
+                ee[r] = min(e, sp.n - 1);
+                const double msk = e < end ? 1.0 : 0.0;
+                wr[r] = w[ee[r]] * msk;
+                vr[r] = vj[ee[r]] * msk;
             }
 #pragma unroll
             for (int c = 0; c < kCB; ++c) {
-                if (cb + c < ncols) {
-                    const double* Vi = V + (size_t)(cb + c) * ld;
 #pragma unroll
-                    for (int r = 0; r < kEo; ++r) {
-                        const int e = base + r * kT;
-                        const double a = e < n ? Vi[e] : 0.0;
-                        aw[c] += a * wr[r];
-                        av[c] += a * vr[r];
-                    }
+                for (int r = 0; r < kEo; ++r) {
+                    const double a = Vc[c][ee[r]];
+                    aw[c] += a * wr[r];
+                    av[c] += a * vr[r];
                 }
             }
         }
 #pragma unroll
         for (int c = 0; c < kCB; ++c) {
-            if (cb + c < ncols) {
-                const double sw = wave_sum(aw[c]);
-                const double sv = wave_sum(av[c]);
-                if (lane == 0) {
-                    red[wv][cb + c] = sw;
-                    red[wv][ncols + cb + c] = sv;
-                }
+            const double sw = wave_sum(aw[c]);
+            const double sv = wave_sum(av[c]);
+            if (lane == 0 && cb + c < ncols) {
+                red[wv][cb + c] = sw;
+                red[wv][ncols + cb + c] = sv;
             }
         }
     }
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
-    if (!finish_multi(W, 2 * ncols, red, W.h2)) return;
+    store_partials(red, 2 * ncols, W.part);
+}
+
+__global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, int nblk) {
+    if (!W.st->active) return;
+    reduce_partials(W.part, 2 * ncols, nblk, W.h2);
     const int m = W.st->m;
     double* Gm = W.G;
-    for (int i = tid; i < ncols; i += kT) {  // Gram row of v_j
+    for (int i = threadIdx.x; i < ncols; i += kT) {  // Gram row of v_j
         const double g = W.h2[ncols + i];
         Gm[j * kMaxCols + i] = g;
         Gm[i * kMaxCols + j] = g;
     }
     __syncthreads();
-    for (int i = tid; i < ncols; i += kT) {
+    // h = 2p - G p, one wavefront per row
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = wv; i < ncols; i += kT / 64) {
         double gp = 0.0;
-        for (int k = 0; k < ncols; ++k) gp += Gm[i * kMaxCols + k] * W.h2[k];
-        const double h = 2.0 * W.h2[i] - gp;
-        W.h1[i] = h;
-        W.H[i + j * (m + 1)] = h;
+        for (int k = lane; k < ncols; k += 64) gp += Gm[i * kMaxCols + k] * W.h2[k];
+        gp = wave_sum(gp);
+        if (lane == 0) {
+            const double h = 2.0 * W.h2[i] - gp;
+            W.h1[i] = h;
+            W.H[i + j * (m + 1)] = h;
+        }
     }
 }
 
 __global__ void __launch_bounds__(kT)
-k_gram_update(int n, int ncols, const double* __restrict__ V, size_t ld, double* __restrict__ w,
-              Work W, int j) {
+k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
+              double* __restrict__ w, Work W) {
     if (!W.st->active) return;
     __shared__ double red[kT / 64][kRedCols];
-    __shared__ double hs[kMaxCols];
+    __shared__ double hs[kMaxCols + kCB];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < ncols; i += kT) hs[i] = W.h1[i];
+    for (int i = tid; i < kMaxCols + kCB; i += kT) hs[i] = i < ncols ? W.h1[i] : 0.0;
     __syncthreads();
-    const int tile = kT * kEo;
+    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double acc = 0.0;
-    for (int base = blockIdx.x * tile + tid; base < n; base += gridDim.x * tile) {
+    for (int base = start + tid; base < end; base += kTile) {
+        int ee[kEo];
         double wr[kEo];
 #pragma unroll
         for (int r = 0; r < kEo; ++r) {
-            const int e = base + r * kT;
-            wr[r] = e < n ? w[e] : 0.0;
+            ee[r] = min(base + r * kT, sp.n - 1);
+            wr[r] = w[ee[r]];
         }
         for (int cb = 0; cb < ncols; cb += kCB) {
 #pragma unroll
             for (int c = 0; c < kCB; ++c) {
-                if (cb + c < ncols) {
-                    const double hi = hs[cb + c];
-                    const double* Vi = V + (size_t)(cb + c) * ld;
+                const double hi = hs[cb + c];  // 0 beyond ncols: subtracts an exact 0
+                const double* Vi = V + (size_t)min(cb + c, ncols - 1) * ld;
 #pragma unroll
-                    for (int r = 0; r < kEo; ++r) {
-                        const int e = base + r * kT;
-                        if (e < n) wr[r] -= hi * Vi[e];
-                    }
-                }
+                for (int r = 0; r < kEo; ++r) wr[r] -= hi * Vi[ee[r]];
             }
         }
 #pragma unroll
         for (int r = 0; r < kEo; ++r) {
-            const int e = base + r * kT;
-            if (e < n) {
-                w[e] = wr[r];
+            if (base + r * kT < end) {
+                w[ee[r]] = wr[r];
                 acc += wr[r] * wr[r];
             }
         }
     }
     acc = wave_sum(acc);
     if (lane == 0) red[wv][0] = acc;
-    double* out = W.h2 + kRedCols - 1;
-    if (!finish_multi(W, 1, red, out)) return;
-    if (tid == 0) step_epilogue(W, j, sqrt(*out));
+    store_partials(red, 1, W.part);
 }
 
-// MGS projection i of step j (gmres.cpp:93-97): w -= h_{i-1} V_{i-1} (when
-// i > 0), then H[i,j] = w . V_i; with Vi == null the norm and the epilogue.
+__global__ void __launch_bounds__(kT) k_norm_finish(Work W, int j, int nblk) {
+    if (!W.st->active) return;
+    double* out = W.h2 + kRedCols - 1;
+    reduce_partials(W.part, 1, nblk, out);
+    if (threadIdx.x == 0) step_epilogue(W, j, sqrt(*out));
+}
+
+// ---- MGS in the reference's order (gmres.cpp:93-97) -------------------------
+// k_mgs: w -= h_{i-1} V_{i-1} (when i > 0), then partial sums of w . V_i, or
+// of w . w when Vi is null; k_mgs_finish stores H[i,j] or runs the epilogue.
 __global__ void __launch_bounds__(kT)
-k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
+k_mgs(Split sp, const double* __restrict__ Vprev, const double* __restrict__ Vi,
       double* __restrict__ w, Work W, int i, int j) {
     if (!W.st->active) return;
     __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int m = W.st->m;
     const double hprev = Vprev ? W.H[(i - 1) + j * (m + 1)] : 0.0;
+    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double s = 0.0;
-    for (int e = blockIdx.x * kT + tid; e < n; e += gridDim.x * kT) {
+    for (int e = start + tid; e < end; e += kT) {
         double we = w[e];
         if (Vprev) {
             we -= hprev * Vprev[e];
@@ -317,26 +328,31 @@ k_mgs(int n, const double* __restrict__ Vprev, const double* __restrict__ Vi,
     }
     s = wave_sum(s);
     if (lane == 0) red[wv][0] = s;
-    double* out = W.h2 + kRedCols - 1;
-    if (!finish_multi(W, 1, red, out)) return;
-    if (tid == 0) {
-        if (Vi) W.H[i + j * (m + 1)] = *out;
-        else step_epilogue(W, j, sqrt(*out));
-    }
+    store_partials(red, 1, W.part);
 }
 
-// Norms at a restart: u.u and, in true-residual mode, r.r; then the
-// beta test and the start of the cycle (gmres.cpp:59-81, fgmres.cpp:54-71).
+__global__ void __launch_bounds__(kT) k_mgs_finish(Work W, int i, int j, int nblk, int is_norm) {
+    if (!W.st->active) return;
+    double* out = W.h2 + kRedCols - 1;
+    reduce_partials(W.part, 1, nblk, out);
+    if (threadIdx.x != 0) return;
+    if (is_norm) step_epilogue(W, j, sqrt(*out));
+    else W.H[i + j * (W.st->m + 1)] = *out;
+}
+
+// ---- restart ---------------------------------------------------------------
+// Partial sums of u.u and (when r is given) r.r.
 __global__ void __launch_bounds__(kT)
-k_restart(int n, const double* __restrict__ u, const double* __restrict__ r, Work W) {
+k_norms(Split sp, const double* __restrict__ u, const double* __restrict__ r, Work W) {
     __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double su = 0.0, sr = 0.0;
-    for (int e = blockIdx.x * kT + tid; e < n; e += gridDim.x * kT) {
-        double a = u[e];
+    for (int e = start + tid; e < end; e += kT) {
+        const double a = u[e];
         su += a * a;
         if (r) {
-            double b = r[e];
+            const double b = r[e];
             sr += b * b;
         }
     }
@@ -346,12 +362,25 @@ k_restart(int n, const double* __restrict__ u, const double* __restrict__ r, Wor
         red[wv][0] = su;
         red[wv][1] = sr;
     }
+    store_partials(red, 2, W.part);
+}
+
+// ||u|| from k_norms into st->bnorm (also used for the exit residual).
+__global__ void __launch_bounds__(kT) k_bnorm_finish(Work W, int nblk) {
     double* out = W.h2 + kRedCols - 2;
-    if (!finish_multi(W, 2, red, out)) return;
-    if (tid != 0) return;
+    reduce_partials(W.part, 2, nblk, out);
+    if (threadIdx.x == 0) W.st->bnorm = sqrt(out[0]);
+}
+
+// Beta test and start of the cycle (gmres.cpp:59-81, fgmres.cpp:54-71); in
+// true-residual mode the relative residual ||r||/||b|| decides convergence.
+__global__ void __launch_bounds__(kT) k_restart_finish(Work W, int nblk, int has_r) {
+    double* out = W.h2 + kRedCols - 2;
+    reduce_partials(W.part, 2, nblk, out);
+    if (threadIdx.x != 0) return;
     KState* st = W.st;
     const double beta = sqrt(out[0]);
-    const double rn = r ? sqrt(out[1]) : beta;
+    const double rn = has_r ? sqrt(out[1]) : beta;
     st->k = 0;
     if (st->rtol > 0.0) {
         const double rel = rn / st->bnorm;
@@ -392,14 +421,26 @@ __global__ void k_scale_basis(int n, const double* __restrict__ u, double* __res
         v[e] = div ? u[e] / sc : u[e] * sc;
 }
 
-// Back substitution H y = g (gmres.cpp:131-137), one thread.
-__global__ void k_backsolve(Work W) {
+// Back substitution H y = g (gmres.cpp:131-137): the block stages the k x k
+// upper triangle in LDS, one thread solves in the reference's order.
+__global__ void __launch_bounds__(64) k_backsolve(Work W) {
+    extern __shared__ double hl[];  // packed rows: row i holds H[i][i..k-1]
     const KState* st = W.st;
     const int k = st->k, m = st->m;
+    if (k == 0) return;
+    // row i starts at i*k - i*(i-1)/2
+    for (int i = 0; i < k; ++i)
+        for (int l = i + threadIdx.x; l < k; l += blockDim.x)
+            hl[i * k - i * (i - 1) / 2 + (l - i)] = W.H[i + l * (m + 1)];
+    double* ys = hl + k * (k + 1) / 2;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     for (int i = k - 1; i >= 0; --i) {
+        const double* row = hl + i * k - i * (i - 1) / 2;
         double yi = W.g[i];
-        for (int l = i + 1; l < k; ++l) yi -= W.H[i + l * (m + 1)] * W.y[l];
-        W.y[i] = yi / W.H[i + i * (m + 1)];
+        for (int l = i + 1; l < k; ++l) yi -= row[l - i] * ys[l];
+        ys[i] = yi / row[0];
+        W.y[i] = ys[i];
     }
 }
 
@@ -429,26 +470,20 @@ __global__ void k_init_state(KState* st, int method, int m, double tol, double r
     st->m = m;
     st->tol = tol;
     st->rtol = rtol;
+    st->bnorm = 0.0;
     st->rel_true = -1.0;
     st->scale = 1.0;
     st->res = 0.0;
 }
 
-__global__ void k_rel_from_norm(KState* st, const double* rn) {
-    st->rel_true = *rn / st->bnorm;
-}
-
-static int nblocks_for(int n, int per_thread) {
-    long long g = ((long long)n + (long long)kT * per_thread - 1) / ((long long)kT * per_thread);
-    if (g < 1) g = 1;
-    if (g > kMaxBlocks) g = kMaxBlocks;
-    return (int)g;
+static int ew_blocks(int n) {
+    long long g = ((long long)n + 1023) / 1024;
+    return (int)std::max<long long>(1, std::min<long long>(g, 8192));
 }
 
 struct SolveWs {
     DevBuf<double> vecs;  // V, Z, r, z, w, t
     DevBuf<double> small;
-    DevBuf<unsigned> ticket;
     DevBuf<KState> st;
     int* pinned = nullptr;
     ~SolveWs() {
@@ -474,6 +509,7 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         const int method = prm->method == SG_FGMRES ? 1 : 0;
         if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
         if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
+        if (n < 1) fail(SG_E_ARG, "empty system");
         const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
         const int L = prm->lookahead > 0 ? prm->lookahead : 2;
         hipStream_t s = ctx->stream;
@@ -495,9 +531,7 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         size_t nsmall = (size_t)(m + 1) * m + (m + 1) + 3 * m + 2 * kRedCols + hcap +
                         (size_t)kRedCols * kMaxBlocks + (size_t)kMaxCols * kMaxCols;
         ws.small.alloc(nsmall);
-        ws.ticket.alloc(1);
         ws.st.alloc(1);
-        SG_HIP(hipMemsetAsync(ws.ticket.p, 0, sizeof(unsigned), s));
         SG_HIP(hipHostMalloc((void**)&ws.pinned, sizeof(int) * (m + 1), hipHostMallocDefault));
         Work W;
         W.H = ws.small.p;
@@ -510,25 +544,25 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         W.hist = W.h2 + kRedCols;
         W.part = W.hist + hcap;
         W.G = W.part + (size_t)kRedCols * kMaxBlocks;
-        W.ticket = ws.ticket.p;
         W.st = ws.st.p;
         KState* st = ws.st.p;
         const int* active = &st->active;
 
+        const Split sp_red = make_split(n, 4 * ctx->num_cu);
+        const Split sp_gram = make_split(n, 4 * ctx->num_cu);
+        const int gew = ew_blocks(n);
+        const size_t bs_lds = sizeof(double) * ((size_t)m * (m + 1) / 2 + m + 1);
+
+        SG_HIP(hipFuncSetAttribute((const void*)k_backsolve,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bs_lds));
         double tol = prm->rtol > 0.0 ? 0.0 : prm->tol_abs;
         hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, st, method, m, tol, prm->rtol);
-        SG_HIP(hipGetLastError());
         // ||b|| for the relative test and the report
-        dot_launch(ctx, n, d_b, d_b, ctx->d_result);
-        double bb = 0.0;
-        SG_HIP(hipMemcpyAsync(&bb, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
-        SG_HIP(hipStreamSynchronize(s));
-        const double bnorm = std::sqrt(bb);
-        SG_HIP(hipMemcpyAsync(&st->bnorm, &bnorm, sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, d_b,
+                           (const double*)nullptr, W);
+        hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk);
+        SG_HIP(hipGetLastError());
 
-        const int gred = nblocks_for(n, 4);
-        const int gcgs = std::min(nblocks_for(n, kEo), 2 * ctx->num_cu);
-        const int gew = nblocks_for(n, 4);
         std::vector<hipEvent_t> evs(m);
         for (int j = 0; j < m; ++j) evs[j] = ctx->get_event();
         struct EvBack {
@@ -561,8 +595,11 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
             }
             {
                 ProfScope ps(ctx, SG_K_UPDATE);
-                hipLaunchKernelGGL(k_restart, dim3(gred), dim3(kT), 0, s, n, u,
-                                   (rmode && u != r) ? r : nullptr, W);
+                const bool has_r = rmode && u != r;
+                hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, u,
+                                   has_r ? (const double*)r : nullptr, W);
+                hipLaunchKernelGGL(k_restart_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk,
+                                   has_r ? 1 : 0);
                 if (!verify_only)
                     hipLaunchKernelGGL(k_scale_basis, dim3(gew), dim3(kT), 0, s, n, u, V, st);
                 SG_HIP(hipGetLastError());
@@ -592,15 +629,23 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
                 {
                     ProfScope ps(ctx, SG_K_ORTHO);
                     if (cgs2) {
-                        hipLaunchKernelGGL(k_gram_dot, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
-                        hipLaunchKernelGGL(k_gram_update, dim3(gcgs), dim3(kT), 0, s, n, j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_gram_dot, dim3(sp_gram.nblk), dim3(kT), 0, s, sp_gram,
+                                           j + 1, V, ld, w, W, j);
+                        hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kT), 0, s, W, j + 1, j,
+                                           sp_gram.nblk);
+                        hipLaunchKernelGGL(k_gram_update, dim3(sp_gram.nblk), dim3(kT), 0, s,
+                                           sp_gram, j + 1, V, ld, w, W);
+                        hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(kT), 0, s, W, j,
+                                           sp_gram.nblk);
                     } else {
-                        for (int i = 0; i <= j; ++i)
-                            hipLaunchKernelGGL(k_mgs, dim3(gred), dim3(kT), 0, s, n,
-                                               i ? V + (size_t)(i - 1) * ld : nullptr,
-                                               V + (size_t)i * ld, w, W, i, j);
-                        hipLaunchKernelGGL(k_mgs, dim3(gred), dim3(kT), 0, s, n,
-                                           V + (size_t)j * ld, nullptr, w, W, j + 1, j);
+                        for (int i = 0; i <= j + 1; ++i) {
+                            const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
+                            const double* Vi = i <= j ? V + (size_t)i * ld : nullptr;
+                            hipLaunchKernelGGL(k_mgs, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red,
+                                               Vp, Vi, w, W, i, j);
+                            hipLaunchKernelGGL(k_mgs_finish, dim3(1), dim3(kT), 0, s, W, i, j,
+                                               sp_red.nblk, i > j ? 1 : 0);
+                        }
                     }
                     SG_HIP(hipGetLastError());
                 }
@@ -614,7 +659,7 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
             }
             if (!verify_only) {
                 ProfScope ps(ctx, SG_K_UPDATE);
-                hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(1), 0, s, W);
+                hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), bs_lds, s, W);
                 hipLaunchKernelGGL(k_update, dim3(gew), dim3(kT), 0, s, n, d_x,
                                    (method && M) ? Z : V, ld, W);
                 SG_HIP(hipGetLastError());
@@ -645,15 +690,19 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         if (hist)
             for (int i = 0; i < hst.total && i < hist_cap; ++i) hist[i] = h[i];
         // FP64 true residual at exit (reported; inside the solve in rtol mode)
-        if (prm->rtol > 0.0 && hst.converged) {
+        if (rmode && hst.converged) {
             res->rel_true_residual = hst.rel_true;
         } else {
+            const double bn = hst.bnorm;
             spmv_launch(A, -1.0, d_x, 1.0, d_b, r, nullptr, s);
-            dot_launch(ctx, n, r, r, ctx->d_result);
-            double rr = 0.0;
-            SG_HIP(hipMemcpyAsync(&rr, ctx->d_result, sizeof(double), hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, r,
+                               (const double*)nullptr, W);
+            hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk);
+            KState fin{};
+            SG_HIP(hipMemcpyAsync(&fin, st, sizeof(KState), hipMemcpyDeviceToHost, s));
             SG_HIP(hipStreamSynchronize(s));
-            res->rel_true_residual = bnorm > 0.0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+            const double rn = fin.bnorm;  // k_bnorm_finish stored ||r|| in the bnorm slot
+            res->rel_true_residual = bn > 0.0 ? rn / bn : rn;
         }
     });
 }

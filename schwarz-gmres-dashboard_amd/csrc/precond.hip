@@ -62,6 +62,7 @@ struct Plan {
     std::vector<double> lval;
     std::vector<int> dofs;   // global id of every local row, concatenated
     std::vector<int> own;    // owned local rows, per subdomain ascending
+    std::vector<int> own_gid;  // global ids of those rows
     int total_rows = 0;
     int max_ns = 0;
 };
@@ -91,13 +92,14 @@ struct sg_precond {
     // sweep plan (device)
     int nsub = 0;
     bool z_in_lds = false;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;      // dynamic LDS of the sweep with z in LDS
+    size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned)
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
     sg::DevBuf<sg::Level> flev, blev;
     sg::DevBuf<int> f_row, f_col, f_src;
     sg::DevBuf<int> b_row, b_col, b_src, b_dsrc;
-    sg::DevBuf<int> d_dofs, d_own;
+    sg::DevBuf<int> d_dofs, d_own, d_own_gid;
     sg::DevBuf<int> lrp, lci, ldiag;
     sg::DevBuf<double> lval;  // factored FP64 values in local CSR order
     sg::DevBuf<double> f_v64, b_v64, b_d64;
@@ -260,78 +262,103 @@ __global__ void k_pack(int n, const int* __restrict__ src, const double* __restr
     }
 }
 
-// Row data of one thread for one level, prefetched a level ahead.
+// Row data of one thread for one level, loaded two levels ahead.  The raw
+// loaded values are kept as they are; `use` (bit k: entry k is real, bit 31:
+// the thread has a row) is applied only when the level is computed, so no
+// instruction touches a prefetched register before it is needed.
 template <typename T>
 struct RowPre {
     int row;
     int col[kKreg];
     T val[kKreg];
     T diag;
+    unsigned use;
 };
 
+// Branch-free load of thread q's row of level L: indices are clamped into the
+// level (the row / ELL arrays carry kTrsvThreads padding entries at the end),
+// so every load is unconditional and stays in flight across levels.
 template <typename T, bool BWD>
 __device__ __forceinline__ void row_prefetch(RowPre<T>& p, const Level& L, int q,
                                              const int* __restrict__ rows,
                                              const int* __restrict__ cols,
                                              const T* __restrict__ vals,
                                              const T* __restrict__ diags) {
-    p.row = -1;
-    if (q < L.size) {
-        p.row = rows[L.start + q];
+    const bool live = q < L.size;
+    const int qq = live ? q : 0;
+    p.row = rows[L.start + qq];
+    unsigned use = live ? 0x80000000u : 0u;
 #pragma unroll
-        for (int k = 0; k < kKreg; ++k)
-            if (k < L.K) {
-                p.col[k] = cols[L.ell + k * L.size + q];
-                p.val[k] = vals[L.ell + k * L.size + q];
-            }
-        if (BWD) p.diag = diags[L.start + q];
+    for (int k = 0; k < kKreg; ++k) {
+        const int idx = L.ell + (k < L.K ? k : 0) * L.size + qq;
+        p.col[k] = cols[idx];
+        p.val[k] = vals[idx];
+        if (live && k < L.K) use |= 1u << k;
+    }
+    if (BWD) p.diag = diags[L.start + qq];
+    p.use = use;
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void row_compute(T* z, const RowPre<T>& p, const Level& L, int zslot,
+                                            const int* __restrict__ rows,
+                                            const int* __restrict__ cols,
+                                            const T* __restrict__ vals,
+                                            const T* __restrict__ diags) {
+    const int t = threadIdx.x, B = blockDim.x;
+    if (p.use & 0x80000000u) {
+        T acc = z[p.row];
+#pragma unroll
+        for (int k = 0; k < kKreg; ++k) {
+            const bool u = (p.use >> k) & 1u;
+            acc -= (u ? p.val[k] : (T)0) * z[u ? p.col[k] : zslot];  // unused: - 0*0
+        }
+        for (int k = kKreg; k < L.K; ++k)
+            acc -= vals[L.ell + k * L.size + t] * z[cols[L.ell + k * L.size + t]];
+        if (BWD) acc = acc / p.diag;
+        z[p.row] = acc;
+    }
+    for (int q = t + B; q < L.size; q += B) {  // levels wider than the block
+        const int row = rows[L.start + q];
+        T acc = z[row];
+        for (int k = 0; k < L.K; ++k)
+            acc -= vals[L.ell + k * L.size + q] * z[cols[L.ell + k * L.size + q]];
+        if (BWD) acc = acc / diags[L.start + q];
+        z[row] = acc;
     }
 }
 
 // One level-scheduled sweep over a subdomain: forward (unit L) or backward
 // (U, division by the diagonal), z in place.  Each row subtracts its entries
-// in CSR order, as src/precond/ilu0.h:62-76 does.  The next level's row data
-// is loaded while the current level computes; levels wider than the block
-// loop without prefetch.
+// in CSR order, as src/precond/ilu0.h:62-76 does.  Row data is loaded two
+// levels ahead into a ring of three register buffers; the loop is unrolled by
+// three so the ring never needs register moves (a move would wait for the
+// loads in flight).  lev points at the subdomain's level table in LDS.
 template <typename T, bool BWD>
-__device__ __forceinline__ void sweep(T* z, const Level* __restrict__ lev, int nlev,
+__device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslot,
                                       const int* __restrict__ rows,
                                       const int* __restrict__ cols,
                                       const T* __restrict__ vals,
                                       const T* __restrict__ diags) {
     if (nlev == 0) return;
-    const int t = threadIdx.x, B = blockDim.x;
+    const int t = threadIdx.x;
     const Level none{0, 0, 0, 0};
-    Level L0 = lev[0];
-    Level L1 = nlev > 1 ? lev[1] : none;
-    RowPre<T> cur;
-    row_prefetch<T, BWD>(cur, L0, t, rows, cols, vals, diags);
-    for (int l = 0; l < nlev; ++l) {
-        const Level L2 = (l + 2 < nlev) ? lev[l + 2] : none;
-        RowPre<T> nxt;
-        row_prefetch<T, BWD>(nxt, L1, t, rows, cols, vals, diags);
-        if (cur.row >= 0) {
-            T acc = z[cur.row];
-#pragma unroll
-            for (int k = 0; k < kKreg; ++k)
-                if (k < L0.K) acc -= cur.val[k] * z[cur.col[k]];
-            for (int k = kKreg; k < L0.K; ++k)
-                acc -= vals[L0.ell + k * L0.size + t] * z[cols[L0.ell + k * L0.size + t]];
-            if (BWD) acc = acc / cur.diag;
-            z[cur.row] = acc;
-        }
-        for (int q = t + B; q < L0.size; q += B) {
-            const int row = rows[L0.start + q];
-            T acc = z[row];
-            for (int k = 0; k < L0.K; ++k)
-                acc -= vals[L0.ell + k * L0.size + q] * z[cols[L0.ell + k * L0.size + q]];
-            if (BWD) acc = acc / diags[L0.start + q];
-            z[row] = acc;
-        }
+    auto level = [&](int l) { return l < nlev ? lev[l] : none; };
+    RowPre<T> b0, b1, b2;
+    row_prefetch<T, BWD>(b0, level(0), t, rows, cols, vals, diags);
+    row_prefetch<T, BWD>(b1, level(1), t, rows, cols, vals, diags);
+    for (int l = 0; l < nlev; l += 3) {
+        row_prefetch<T, BWD>(b2, level(l + 2), t, rows, cols, vals, diags);
+        row_compute<T, BWD>(z, b0, level(l), zslot, rows, cols, vals, diags);
         __syncthreads();
-        cur = nxt;
-        L0 = L1;
-        L1 = L2;
+        if (l + 1 >= nlev) break;
+        row_prefetch<T, BWD>(b0, level(l + 3), t, rows, cols, vals, diags);
+        row_compute<T, BWD>(z, b1, level(l + 1), zslot, rows, cols, vals, diags);
+        __syncthreads();
+        if (l + 2 >= nlev) break;
+        row_prefetch<T, BWD>(b1, level(l + 4), t, rows, cols, vals, diags);
+        row_compute<T, BWD>(z, b2, level(l + 2), zslot, rows, cols, vals, diags);
+        __syncthreads();
     }
 }
 
@@ -349,33 +376,44 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                 const Level* __restrict__ blev, const int* __restrict__ b_row,
                 const int* __restrict__ b_col, const T* __restrict__ b_val,
                 const T* __restrict__ b_diag, const int* __restrict__ dofs,
-                const int* __restrict__ own, const double* __restrict__ x,
+                const int* __restrict__ own, const int* __restrict__ own_gid,
+                const double* __restrict__ x,
                 double* __restrict__ y, T* __restrict__ zglob, T* __restrict__ zout,
-                const T* __restrict__ cz, const int* __restrict__ guard) {
+                const T* __restrict__ cz, const int* __restrict__ guard, int lds_z_off) {
     if (guard && *guard == 0) return;
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
     const SubInfo si = subs[s];
-    T* z = ZLDS ? reinterpret_cast<T*>(lds_raw) : zglob + si.zoff;
+    // LDS: this subdomain's level tables, then (ZLDS) the solution vector
+    Level* lf = reinterpret_cast<Level*>(lds_raw);
+    Level* lb = lf + si.nflev;
+    for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) lf[l] = flev[si.flev0 + l];
+    for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) lb[l] = blev[si.blev0 + l];
+    T* z = ZLDS ? reinterpret_cast<T*>(lds_raw + lds_z_off) : zglob + si.zoff;
     const int* d = dofs + si.row0;
-    for (int i = threadIdx.x; i < si.ns; i += blockDim.x) z[i] = (T)x[d[i]];  // R_s x
+    // R_s x: four independent index/value loads in flight per thread
+    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 4 * blockDim.x) {
+        int g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * blockDim.x;
+            g[u] = i < si.ns ? d[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (g[u] >= 0) z[i0 + u * blockDim.x] = (T)x[g[u]];
+    }
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
-    sweep<T, false>(z, flev + si.flev0, si.nflev, f_row, f_col, f_val, nullptr);
-    sweep<T, true>(z, blev + si.blev0, si.nblev, b_row, b_col, b_val, b_diag);
+    sweep<T, false>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
+    sweep<T, true>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
     if (MODE == 0) {
         const int* ow = own + si.own0;
-        if (cz) {
-            const T add = cz[s];
-            for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
-                const int li = ow[k];
-                y[d[li]] = (double)(z[li] + add);
-            }
-        } else {
-            for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
-                const int li = ow[k];
-                y[d[li]] = (double)z[li];
-            }
+        const int* og = own_gid + si.own0;
+        const T add = cz ? cz[s] : (T)0;
+        for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
+            const int li = ow[k], g = og[k];
+            y[g] = cz ? (double)(z[li] + add) : (double)z[li];
         }
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = z[i];
@@ -407,9 +445,8 @@ __global__ void k_asm_gather(int n, const int* __restrict__ ptr, const int* __re
 // order by the coarse solve.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __restrict__ dofs,
-                  const int* __restrict__ own, const T* __restrict__ c,
-                  const double* __restrict__ x, T* __restrict__ part,
+k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __restrict__ own_gid,
+                  const T* __restrict__ c, const double* __restrict__ x, T* __restrict__ part,
                   const int* __restrict__ guard) {
     if (guard && *guard == 0) return;
     const int s = blockIdx.x / split, j = blockIdx.x % split;
@@ -417,10 +454,9 @@ k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __rest
     const int per = (si.nown + split - 1) / split;
     const int a = j * per, b = min(si.nown, a + per);
     const T cs = c[s];
-    const int* d = dofs + si.row0;
-    const int* ow = own + si.own0;
+    const int* og = own_gid + si.own0;
     T acc = 0;
-    for (int k = a + threadIdx.x; k < b; k += 256) acc += cs * (T)x[d[ow[k]]];
+    for (int k = a + threadIdx.x; k < b; k += 256) acc += cs * (T)x[og[k]];
     __shared__ T red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -429,6 +465,32 @@ k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Coarse solve with the precomputed inverse (row-major): one wavefront per
+// coarse row; every block first adds the restriction partials of all rows.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_coarse_inv(int nc, int split, const T* __restrict__ inv, const T* __restrict__ r0part,
+             const T* __restrict__ c, T* __restrict__ cz, const int* __restrict__ guard) {
+    if (guard && *guard == 0) return;
+    extern __shared__ unsigned char lds_raw[];
+    T* xs = reinterpret_cast<T*>(lds_raw);
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        T r = 0;
+        for (int j = 0; j < split; ++j) r += r0part[i * split + j];
+        xs[i] = r;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (row >= nc) return;
+    const T* ri = inv + (size_t)row * nc;
+    T acc = 0;
+    for (int j = lane; j < nc; j += 64) acc += ri[j] * xs[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) cz[row] = c[row] * acc;
 }
 
 // Coarse solve z0 = A0^{-1} r0 and cz[s] = c_s z0[s].
@@ -673,6 +735,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
             pl.dofs.insert(pl.dofs.end(), M->dofs[s].begin(), M->dofs[s].end());
         }
         pl.own.insert(pl.own.end(), own[s].begin(), own[s].end());
+        for (int li : own[s]) pl.own_gid.push_back(global_ilu ? li : M->dofs[s][li]);
         M->sub_nnz[s] = L.nnz();
         M->levels_fwd += si.nflev;
         M->levels_bwd += si.nblev;
@@ -688,6 +751,14 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     }
     if (nnz0 > INT32_MAX || fell > INT32_MAX || bell > INT32_MAX || zoff > INT32_MAX)
         fail(SG_E_ARG, "subdomain data exceeds 32-bit indexing");
+    // padding read (never used) by the branch-free prefetch of idle threads
+    pl.f_row.resize(pl.f_row.size() + kTrsvThreads, 0);
+    pl.b_row.resize(pl.b_row.size() + kTrsvThreads, 0);
+    pl.f_col.resize(pl.f_col.size() + kTrsvThreads, 0);
+    pl.b_col.resize(pl.b_col.size() + kTrsvThreads, 0);
+    pl.f_src.resize(pl.f_src.size() + kTrsvThreads, -1);
+    pl.b_src.resize(pl.b_src.size() + kTrsvThreads, -1);
+    pl.b_dsrc.resize(pl.b_dsrc.size() + kTrsvThreads, -1);
     pl.total_rows = (int)row0;
     pl.order.resize(nsub);
     std::iota(pl.order.begin(), pl.order.end(), 0);
@@ -709,6 +780,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     M->b_dsrc.upload(pl.b_dsrc, st);
     M->d_dofs.upload(pl.dofs, st);
     M->d_own.upload(pl.own, st);
+    M->d_own_gid.upload(pl.own_gid, st);
     M->lrp.upload(pl.lrp, st);
     M->lci.upload(pl.lci, st);
     M->ldiag.upload(pl.ldiag, st);
@@ -739,10 +811,16 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     }
     SG_HIP(hipGetLastError());
 
-    // z in LDS when the largest subdomain fits
+    // LDS: level tables of the subdomain with the most levels, then z when the
+    // largest subdomain fits
+    int maxlev = 0;
+    for (const SubInfo& si : pl.subs) maxlev = std::max(maxlev, si.nflev + si.nblev);
+    M->lds_tab_bytes = ((size_t)maxlev * sizeof(Level) + 15) / 16 * 16;
+    if (M->lds_tab_bytes > (size_t)ctx->max_lds)
+        fail(SG_E_ARG, "too many ILU levels in one subdomain for the LDS level table");
     size_t zbytes = (size_t)(pl.max_ns + 1) * (fp32 ? 4 : 8);
-    M->z_in_lds = zbytes <= (size_t)ctx->max_lds;
-    M->lds_bytes = M->z_in_lds ? zbytes : 0;
+    M->z_in_lds = M->lds_tab_bytes + zbytes <= (size_t)ctx->max_lds;
+    M->lds_bytes = M->lds_tab_bytes + zbytes;
     if (!M->z_in_lds) {
         if (fp32) M->z32.alloc((size_t)zoff);
         else M->z64.alloc((size_t)zoff);
@@ -751,14 +829,19 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
         if (fp32) M->zout32.alloc((size_t)pl.total_rows + 1);
         else M->zout64.alloc((size_t)pl.total_rows + 1);
     }
-    if (M->z_in_lds) {
-        const void* fns[4] = {(const void*)k_schwarz_sweep<float, true, 0>,
+    {
+        const void* fns[8] = {(const void*)k_schwarz_sweep<float, true, 0>,
                               (const void*)k_schwarz_sweep<float, true, 1>,
                               (const void*)k_schwarz_sweep<double, true, 0>,
-                              (const void*)k_schwarz_sweep<double, true, 1>};
+                              (const void*)k_schwarz_sweep<double, true, 1>,
+                              (const void*)k_schwarz_sweep<float, false, 0>,
+                              (const void*)k_schwarz_sweep<float, false, 1>,
+                              (const void*)k_schwarz_sweep<double, false, 0>,
+                              (const void*)k_schwarz_sweep<double, false, 1>};
+        const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds,
+                                                std::max(M->lds_bytes, M->lds_tab_bytes));
         for (const void* f : fns)
-            SG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)M->lds_bytes));
+            SG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     }
 
     if (additive) {
@@ -828,9 +911,13 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
     M->d_LUt64.alloc((size_t)nc * nc);
     hipLaunchKernelGGL(k_transpose, dim3(grid_for((long long)nc * nc)), dim3(256), 0, st, nc, dA.p,
                        M->d_LUt64.p);
-    M->d_inv64.alloc((size_t)nc * nc);
+    DevBuf<double> invT;  // column j of A0^{-1} stored contiguously
+    invT.alloc((size_t)nc * nc);
     hipLaunchKernelGGL(k_lu_inverse, dim3(nc), dim3(256), (size_t)nc * 8, st, nc, dA.p,
-                       M->d_piv.p, M->d_inv64.p);
+                       M->d_piv.p, invT.p);
+    M->d_inv64.alloc((size_t)nc * nc);  // row-major A0^{-1}
+    hipLaunchKernelGGL(k_transpose, dim3(grid_for((long long)nc * nc)), dim3(256), 0, st, nc,
+                       invT.p, M->d_inv64.p);
     SG_HIP(hipGetLastError());
     M->d_c64.upload(c, st);
     // the restriction splits each subdomain over enough workgroups to fill the GPU
@@ -913,9 +1000,10 @@ template <typename T, bool ZLDS, int MODE>
 static void launch_sweep(sg_precond* M, const double* x, double* y, const int* guard,
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE>), dim3(M->nsub), dim3(kTrsvThreads),
-                       ZLDS ? M->lds_bytes : 0, M->ctx->stream, M->d_subs.p, M->d_order.p,
-                       M->flev.p, M->f_row.p, M->f_col.p, fv, M->blev.p, M->b_row.p, M->b_col.p,
-                       bv, bd, M->d_dofs.p, M->d_own.p, x, y, zg, zo, cz, guard);
+                       ZLDS ? M->lds_bytes : M->lds_tab_bytes, M->ctx->stream, M->d_subs.p,
+                       M->d_order.p, M->flev.p, M->f_row.p, M->f_col.p, fv, M->blev.p, M->b_row.p,
+                       M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zg, zo,
+                       cz, guard, (int)M->lds_tab_bytes);
 }
 
 template <typename T>
@@ -947,9 +1035,13 @@ static const T* coarse_launch(sg_precond* M, const double* x, const int* guard, 
     hipStream_t st = M->ctx->stream;
     int nc = M->nc, split = M->restrict_split;
     hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(nc * split), dim3(256), 0, st, split,
-                       M->d_subs.p, M->d_dofs.p, M->d_own.p, c, x, r0, guard);
-    hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), (size_t)nc * sizeof(T), st, nc,
-                       M->opts.coarse_solve, split, LUt, M->d_piv.p, inv, r0, c, cz, guard);
+                       M->d_subs.p, M->d_own_gid.p, c, x, r0, guard);
+    if (M->opts.coarse_solve == 1)
+        hipLaunchKernelGGL(k_coarse_inv<T>, dim3((nc + 3) / 4), dim3(256), (size_t)nc * sizeof(T),
+                           st, nc, split, inv, r0, c, cz, guard);
+    else
+        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), (size_t)nc * sizeof(T), st, nc,
+                           0, split, LUt, M->d_piv.p, inv, r0, c, cz, guard);
     SG_HIP(hipGetLastError());
     return cz;
 }
