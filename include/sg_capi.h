@@ -213,6 +213,14 @@ typedef enum {
     SG_K_COUNT = 5
 } sg_kernel_family;
 sg_status sg_context_set_profiling(sg_context* ctx, int enable);
+/* Kernel-shape selection.  mode 1 (default): the first SpMV on a matrix and
+ * the first apply of a preconditioner time each shape (SpMV rows per
+ * workgroup, sweep workgroup size: 256, 512, 1024) and keep the fastest.
+ * mode 0: fixed defaults (SpMV 256, sweep 512).  mode 2, 3, 4: force shape
+ * 256, 512, 1024 for both.  All shapes compute identical values, so the mode
+ * never changes results.  No counterpart in the reference (its kernels have
+ * fixed launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
+sg_status sg_context_set_autotune(sg_context* ctx, int mode);
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
                                   double* total_ms);
 sg_status sg_context_reset_stats(sg_context* ctx);

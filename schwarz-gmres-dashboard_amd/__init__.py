@@ -101,6 +101,7 @@ def lib():
         "sg_partition": ([C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP], C.c_int),
         "sg_precond_stats": ([_VP, C.POINTER(PrecondInfo)], C.c_int),
         "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
+        "sg_context_set_autotune": ([_VP, C.c_int], C.c_int),
         "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double)], C.c_int),
         "sg_context_reset_stats": ([_VP], C.c_int),
@@ -140,6 +141,10 @@ class Context:
 
     def profiling(self, on=True):
         self.check(lib().sg_context_set_profiling(self.h, 1 if on else 0))
+
+    def autotune(self, mode=1):
+        """0 fixed shapes, 1 time shapes on first use, 2/3/4 force 256/512/1024."""
+        self.check(lib().sg_context_set_autotune(self.h, int(mode)))
 
     def kernel_stats(self, family):
         n = C.c_longlong()

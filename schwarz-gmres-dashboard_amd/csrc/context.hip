@@ -318,6 +318,12 @@ sg_status sg_context_set_profiling(sg_context* ctx, int enable) {
     return SG_OK;
 }
 
+sg_status sg_context_set_autotune(sg_context* ctx, int enable) {
+    if (!ctx || enable < 0 || enable > 4) return SG_E_ARG;
+    ctx->autotune = enable;
+    return SG_OK;
+}
+
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
                                   double* total_ms) {
     if (!ctx || family < 0 || family >= SG_K_COUNT) return SG_E_ARG;

@@ -39,6 +39,8 @@ constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis 
 constexpr int kCB = 16;                 // basis vectors per register block
 constexpr int kEo = 2;                  // elements per thread per tile
 constexpr int kTile = kT * kEo;
+constexpr int kTG = 512;                // threads per block of the Gram passes
+constexpr int kTileG = kTG * kEo;
 
 struct KState {
     int active;     // current cycle continues
@@ -76,13 +78,13 @@ struct Split {
     int n, chunk, nblk;
 };
 
-static Split make_split(int n, int max_blocks) {
+static Split make_split(int n, int max_blocks, int tile = kTile) {
     Split s;
     s.n = n;
-    int want = std::max(1, (n + kTile - 1) / kTile);
+    int want = std::max(1, (n + tile - 1) / tile);
     s.nblk = std::min(want, std::min(max_blocks, kMaxBlocks));
     int c = (n + s.nblk - 1) / s.nblk;
-    s.chunk = (c + kTile - 1) / kTile * kTile;
+    s.chunk = (c + tile - 1) / tile * tile;
     s.nblk = std::max(1, (n + s.chunk - 1) / s.chunk);
     return s;
 }
@@ -149,12 +151,13 @@ __device__ void step_epilogue(const Work& W, int j, double hn) {
 }
 
 // Adds the per-wave values red[wave][c] and stores column c's block partial.
+template <int NT = kT>
 __device__ inline void store_partials(double (*red)[kRedCols], int ncols, double* part) {
     __syncthreads();
-    for (int i = threadIdx.x; i < ncols; i += kT) {
+    for (int i = threadIdx.x; i < ncols; i += NT) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kT / 64; ++w) s += red[w][i];
+        for (int w = 0; w < NT / 64; ++w) s += red[w][i];
         part[(size_t)i * kMaxBlocks + blockIdx.x] = s;
     }
 }
@@ -182,11 +185,11 @@ __device__ void reduce_partials(const double* part, int ncols, int nblk, double*
 // Pass 2 (k_gram_update + k_norm_finish): w -= V h, ||w||, step epilogue.
 // Columns beyond ncols inside a register block are clamped to a valid column
 // and masked, so every load is unconditional and stays in flight.
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kTG)
 k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
            const double* __restrict__ w, Work W, int j) {
     if (!W.st->active) return;
-    __shared__ double red[kT / 64][kRedCols];
+    __shared__ double red[kTG / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     const double* vj = V + (size_t)j * ld;
@@ -197,12 +200,12 @@ k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
         double aw[kCB], av[kCB];
 #pragma unroll
         for (int c = 0; c < kCB; ++c) aw[c] = av[c] = 0.0;
-        for (int base = start + tid; base < end; base += kTile) {
+        for (int base = start + tid; base < end; base += kTileG) {
             int ee[kEo];
             double wr[kEo], vr[kEo];
 #pragma unroll
             for (int r = 0; r < kEo; ++r) {
-                const int e = base + r * kT;
+                const int e = base + r * kTG;
                 ee[r] = min(e, sp.n - 1);
                 const double msk = e < end ? 1.0 : 0.0;
                 wr[r] = w[ee[r]] * msk;
@@ -229,7 +232,7 @@ k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
         }
     }
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
-    store_partials(red, 2 * ncols, W.part);
+    store_partials<kTG>(red, 2 * ncols, W.part);
 }
 
 __global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, int nblk) {
@@ -257,23 +260,23 @@ __global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, in
     }
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kTG)
 k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
               double* __restrict__ w, Work W) {
     if (!W.st->active) return;
-    __shared__ double red[kT / 64][kRedCols];
+    __shared__ double red[kTG / 64][kRedCols];
     __shared__ double hs[kMaxCols + kCB];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < kMaxCols + kCB; i += kT) hs[i] = i < ncols ? W.h1[i] : 0.0;
+    for (int i = tid; i < kMaxCols + kCB; i += kTG) hs[i] = i < ncols ? W.h1[i] : 0.0;
     __syncthreads();
     const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double acc = 0.0;
-    for (int base = start + tid; base < end; base += kTile) {
+    for (int base = start + tid; base < end; base += kTileG) {
         int ee[kEo];
         double wr[kEo];
 #pragma unroll
         for (int r = 0; r < kEo; ++r) {
-            ee[r] = min(base + r * kT, sp.n - 1);
+            ee[r] = min(base + r * kTG, sp.n - 1);
             wr[r] = w[ee[r]];
         }
         for (int cb = 0; cb < ncols; cb += kCB) {
@@ -287,7 +290,7 @@ k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
         }
 #pragma unroll
         for (int r = 0; r < kEo; ++r) {
-            if (base + r * kT < end) {
+            if (base + r * kTG < end) {
                 w[ee[r]] = wr[r];
                 acc += wr[r] * wr[r];
             }
@@ -295,7 +298,7 @@ k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
     }
     acc = wave_sum(acc);
     if (lane == 0) red[wv][0] = acc;
-    store_partials(red, 1, W.part);
+    store_partials<kTG>(red, 1, W.part);
 }
 
 __global__ void __launch_bounds__(kT) k_norm_finish(Work W, int j, int nblk) {
@@ -549,7 +552,7 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
         const int* active = &st->active;
 
         const Split sp_red = make_split(n, 4 * ctx->num_cu);
-        const Split sp_gram = make_split(n, 4 * ctx->num_cu);
+        const Split sp_gram = make_split(n, 4 * ctx->num_cu, kTileG);
         const int gew = ew_blocks(n);
         const size_t bs_lds = sizeof(double) * ((size_t)m * (m + 1) / 2 + m + 1);
 
@@ -629,11 +632,11 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
                 {
                     ProfScope ps(ctx, SG_K_ORTHO);
                     if (cgs2) {
-                        hipLaunchKernelGGL(k_gram_dot, dim3(sp_gram.nblk), dim3(kT), 0, s, sp_gram,
+                        hipLaunchKernelGGL(k_gram_dot, dim3(sp_gram.nblk), dim3(kTG), 0, s, sp_gram,
                                            j + 1, V, ld, w, W, j);
                         hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kT), 0, s, W, j + 1, j,
                                            sp_gram.nblk);
-                        hipLaunchKernelGGL(k_gram_update, dim3(sp_gram.nblk), dim3(kT), 0, s,
+                        hipLaunchKernelGGL(k_gram_update, dim3(sp_gram.nblk), dim3(kTG), 0, s,
                                            sp_gram, j + 1, V, ld, w, W);
                         hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(kT), 0, s, W, j,
                                            sp_gram.nblk);

@@ -30,7 +30,9 @@
 
 namespace sg {
 
-constexpr int kTrsvThreads = 512;
+constexpr int kTrsvThreads = 512;   // workgroup of the factorisation
+constexpr int kPadRows = 1024;      // padding of the sweep arrays (largest sweep workgroup)
+constexpr int kSweepVariants[3] = {256, 512, 1024};
 constexpr int kKreg = 4;  // factor entries per row prefetched in registers
 
 struct SubInfo {
@@ -94,6 +96,8 @@ struct sg_precond {
     bool z_in_lds = false;
     size_t lds_bytes = 0;      // dynamic LDS of the sweep with z in LDS
     size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned)
+    int sweep_variant = 1;     // index into kSweepVariants (workgroup size)
+    bool sweep_tuned = false;
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
     sg::DevBuf<sg::Level> flev, blev;
@@ -368,8 +372,8 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
 //         the owned rows (src/precond/ras.h:55-68).
 // MODE 1: additive (ASM): the whole local solution goes to zout, summed by
 //         k_asm_gather (src/precond/asm.h:54-58).
-template <typename T, bool ZLDS, int MODE>
-__global__ void __launch_bounds__(kTrsvThreads)
+template <typename T, bool ZLDS, int MODE, int NT>
+__global__ void __launch_bounds__(NT)
 k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                 const Level* __restrict__ flev, const int* __restrict__ f_row,
                 const int* __restrict__ f_col, const T* __restrict__ f_val,
@@ -752,13 +756,13 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     if (nnz0 > INT32_MAX || fell > INT32_MAX || bell > INT32_MAX || zoff > INT32_MAX)
         fail(SG_E_ARG, "subdomain data exceeds 32-bit indexing");
     // padding read (never used) by the branch-free prefetch of idle threads
-    pl.f_row.resize(pl.f_row.size() + kTrsvThreads, 0);
-    pl.b_row.resize(pl.b_row.size() + kTrsvThreads, 0);
-    pl.f_col.resize(pl.f_col.size() + kTrsvThreads, 0);
-    pl.b_col.resize(pl.b_col.size() + kTrsvThreads, 0);
-    pl.f_src.resize(pl.f_src.size() + kTrsvThreads, -1);
-    pl.b_src.resize(pl.b_src.size() + kTrsvThreads, -1);
-    pl.b_dsrc.resize(pl.b_dsrc.size() + kTrsvThreads, -1);
+    pl.f_row.resize(pl.f_row.size() + kPadRows, 0);
+    pl.b_row.resize(pl.b_row.size() + kPadRows, 0);
+    pl.f_col.resize(pl.f_col.size() + kPadRows, 0);
+    pl.b_col.resize(pl.b_col.size() + kPadRows, 0);
+    pl.f_src.resize(pl.f_src.size() + kPadRows, -1);
+    pl.b_src.resize(pl.b_src.size() + kPadRows, -1);
+    pl.b_dsrc.resize(pl.b_dsrc.size() + kPadRows, -1);
     pl.total_rows = (int)row0;
     pl.order.resize(nsub);
     std::iota(pl.order.begin(), pl.order.end(), 0);
@@ -830,18 +834,23 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
         else M->zout64.alloc((size_t)pl.total_rows + 1);
     }
     {
-        const void* fns[8] = {(const void*)k_schwarz_sweep<float, true, 0>,
-                              (const void*)k_schwarz_sweep<float, true, 1>,
-                              (const void*)k_schwarz_sweep<double, true, 0>,
-                              (const void*)k_schwarz_sweep<double, true, 1>,
-                              (const void*)k_schwarz_sweep<float, false, 0>,
-                              (const void*)k_schwarz_sweep<float, false, 1>,
-                              (const void*)k_schwarz_sweep<double, false, 0>,
-                              (const void*)k_schwarz_sweep<double, false, 1>};
         const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds,
                                                 std::max(M->lds_bytes, M->lds_tab_bytes));
-        for (const void* f : fns)
-            SG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+#define SG_SET_LDS(T, Z, MD)                                                                      \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 256>,                     \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes));             \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 512>,                     \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes));             \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 1024>,                    \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        if (fp32) {
+            SG_SET_LDS(float, true, 0) SG_SET_LDS(float, true, 1)
+            SG_SET_LDS(float, false, 0) SG_SET_LDS(float, false, 1)
+        } else {
+            SG_SET_LDS(double, true, 0) SG_SET_LDS(double, true, 1)
+            SG_SET_LDS(double, false, 0) SG_SET_LDS(double, false, 1)
+        }
+#undef SG_SET_LDS
     }
 
     if (additive) {
@@ -999,11 +1008,68 @@ static void setup(sg_precond* M, const sg_csr* A) {
 template <typename T, bool ZLDS, int MODE>
 static void launch_sweep(sg_precond* M, const double* x, double* y, const int* guard,
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
-    hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE>), dim3(M->nsub), dim3(kTrsvThreads),
-                       ZLDS ? M->lds_bytes : M->lds_tab_bytes, M->ctx->stream, M->d_subs.p,
-                       M->d_order.p, M->flev.p, M->f_row.p, M->f_col.p, fv, M->blev.p, M->b_row.p,
-                       M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zg, zo,
-                       cz, guard, (int)M->lds_tab_bytes);
+    const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;
+#define SG_SWEEP(NT)                                                                              \
+    hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE, NT>), dim3(M->nsub), dim3(NT), lds,       \
+                       M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p, M->f_row.p,        \
+                       M->f_col.p, fv, M->blev.p, M->b_row.p, M->b_col.p, bv, bd, M->d_dofs.p,  \
+                       M->d_own.p, M->d_own_gid.p, x, y, zg, zo, cz, guard,                     \
+                       (int)M->lds_tab_bytes)
+    switch (kSweepVariants[M->sweep_variant]) {
+        case 256: SG_SWEEP(256); break;
+        case 1024: SG_SWEEP(1024); break;
+        default: SG_SWEEP(512); break;
+    }
+#undef SG_SWEEP
+}
+
+template <typename T>
+static void launch_any_sweep(sg_precond* M, const double* x, double* y, const int* guard,
+                             const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+    const bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
+    if (additive) {
+        if (M->z_in_lds) launch_sweep<T, true, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+        else launch_sweep<T, false, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+    } else {
+        if (M->z_in_lds) launch_sweep<T, true, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+        else launch_sweep<T, false, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+    }
+    SG_HIP(hipGetLastError());
+}
+
+// Times each sweep workgroup size (three launches each, on a zero input into
+// scratch) and keeps the fastest; every size computes the same values.
+template <typename T>
+static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd, T* zg, T* zo) {
+    hipStream_t s = M->ctx->stream;
+    DevBuf<double> xin, yout;
+    xin.alloc(std::max(M->n, 1));
+    yout.alloc(std::max(M->n, 1));
+    SG_HIP(hipMemsetAsync(xin.p, 0, sizeof(double) * xin.n, s));
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    float best = 0.f;
+    int bestv = 1;
+    for (int v = 0; v < 3; ++v) {
+        M->sweep_variant = v;
+        launch_any_sweep<T>(M, xin.p, yout.p, nullptr, fv, bv, bd, zg, zo, nullptr);
+        SG_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r)
+            launch_any_sweep<T>(M, xin.p, yout.p, nullptr, fv, bv, bd, zg, zo, nullptr);
+        SG_HIP(hipEventRecord(e1, s));
+        SG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (v == 0 || ms < best * 0.98f) {
+            best = ms;
+            bestv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    M->sweep_variant = bestv;
+    M->sweep_tuned = true;
 }
 
 template <typename T>
@@ -1011,16 +1077,12 @@ static void apply_batch(sg_precond* M, const double* x, double* y, const int* gu
                         const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     hipStream_t st = M->ctx->stream;
     bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
+    if (M->ctx->autotune == 1 && !M->sweep_tuned) sweep_autotune<T>(M, fv, bv, bd, zg, zo);
+    if (M->ctx->autotune >= 2) M->sweep_variant = M->ctx->autotune - 2;
+    else if (M->ctx->autotune == 0) M->sweep_variant = 1;
     {
         ProfScope ps(M->ctx, SG_K_TRSV);
-        if (additive) {
-            if (M->z_in_lds) launch_sweep<T, true, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-            else launch_sweep<T, false, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-        } else {
-            if (M->z_in_lds) launch_sweep<T, true, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-            else launch_sweep<T, false, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-        }
-        SG_HIP(hipGetLastError());
+        launch_any_sweep<T>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
     }
     if (additive) {
         hipLaunchKernelGGL(k_asm_gather<T>, dim3(grid_for(M->n)), dim3(256), 0, st, M->n,

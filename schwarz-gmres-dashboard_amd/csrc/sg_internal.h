@@ -75,6 +75,7 @@ struct sg_context {
     double* d_result = nullptr;  // kMaxRedCols doubles
     double* h_pinned = nullptr;  // pinned staging for small reads
     bool profiling = false;
+    int autotune = 1;  // 0 default shapes, 1 time them on first use, 2+v force shape v
     sg::Prof prof[SG_K_COUNT];
     std::vector<hipEvent_t> event_pool;
 
@@ -88,9 +89,12 @@ struct sg_csr {
     sg::HostCsr host;
     sg::DevBuf<int> rp, ci;
     sg::DevBuf<double> v;
-    // row blocks for the LDS-staged SpMV: block b covers rows [blk[b], blk[b+1])
-    int nblk = 0;
-    sg::DevBuf<int> blk;
+    // row blocks for the LDS-staged SpMV, one table per block shape:
+    // block b covers rows [blk[b], blk[b+1])
+    int nblk_v[3] = {0, 0, 0};
+    sg::DevBuf<int> blk_v[3];
+    int variant = 0;     // selected block shape
+    bool tuned = false;
 };
 
 namespace sg {

@@ -1,29 +1,34 @@
 // FP64 CSR SpMV for CDNA4 and the sg_csr object.
 //
-// LDS-staged "CSR stream": a 256-thread workgroup owns a block of up to 256
-// consecutive rows whose nonzeros (at most kCap) are streamed with fully
-// coalesced loads; each product vals[k]*x[col[k]] is rounded and parked in LDS,
-// then one lane per row adds its products sequentially in CSR order.  The sum
-// therefore equals the reference's serial row loop (src/core/sparse_matrix.h:30-37)
-// bit for bit.  Blocks are remapped so that each of the 8 XCDs walks one
-// contiguous eighth of the rows, which keeps the x neighbourhood of a stencil
-// (+-nx*ny rows) inside that XCD's L2.
+// LDS-staged "CSR stream": a workgroup of NT threads (NT = 256, 512 or 1024)
+// owns a block of up to NT consecutive rows whose nonzeros (at most 8*NT) are
+// streamed with fully coalesced loads; each product vals[k]*x[col[k]] is rounded
+// and parked in LDS, then one lane per row adds its products sequentially in CSR
+// order.  The sum therefore equals the reference's serial row loop
+// (src/core/sparse_matrix.h:30-37) bit for bit, whatever NT is.  Blocks are
+// remapped so that each of the 8 XCDs walks one contiguous eighth of the rows,
+// which keeps the x neighbourhood of a stencil (+-nx*ny rows) inside that XCD's L2.
 #include "sg_internal.h"
 
 #include <algorithm>
 
 namespace sg {
 
-constexpr int kSpmvThreads = 256;
-constexpr int kCap = 2048;  // nonzeros staged per block (16 KiB of LDS)
 constexpr int kXcd = 8;
+constexpr int kNnzPerThread = 8;  // LDS staging capacity: 8 products per thread
 
-__global__ void __launch_bounds__(kSpmvThreads)
+// Block shapes (threads = rows per block); results are identical for all of
+// them, so the choice is a pure performance knob (see spmv_autotune).
+constexpr int kSpmvVariants[3] = {256, 512, 1024};
+
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
               const double* __restrict__ v, const int* __restrict__ blk, int nblk, int per_xcd,
               double alpha, const double* __restrict__ x, double beta,
               const double* __restrict__ y0, double* __restrict__ y,
               const int* __restrict__ guard) {
+    constexpr int kCap = NT * kNnzPerThread;
     if (guard && *guard == 0) return;
     const int b = (blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
     if (b >= nblk) return;
@@ -32,7 +37,7 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
     const int t = threadIdx.x;
     __shared__ double prod[kCap];
     if (k1 - k0 <= kCap) {
-        for (int k = k0 + t; k < k1; k += kSpmvThreads) {
+        for (int k = k0 + t; k < k1; k += NT) {
             const double a = __builtin_nontemporal_load(&v[k]);
             const int c = __builtin_nontemporal_load(&ci[k]);
             prod[k - k0] = a * x[c];
@@ -48,10 +53,10 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
     } else {
         // one long row: strided partial sums and a tree reduction
         double s = 0.0;
-        for (int k = k0 + t; k < k1; k += kSpmvThreads) s += v[k] * x[ci[k]];
+        for (int k = k0 + t; k < k1; k += NT) s += v[k] * x[ci[k]];
         prod[t] = s;
         __syncthreads();
-        for (int w = kSpmvThreads / 2; w > 0; w >>= 1) {
+        for (int w = NT / 2; w > 0; w >>= 1) {
             if (t < w) prod[t] += prod[t + w];
             __syncthreads();
         }
@@ -59,30 +64,94 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
     }
 }
 
-void spmv_prepare(sg_csr* A) {
-    const auto& rp = A->host.rp;
+// Row blocks of at most nt rows and nt*kNnzPerThread nonzeros (a longer row
+// gets a block of its own).
+static std::vector<int> row_blocks(const std::vector<int>& rp, int nrows, int nt) {
+    const int cap = nt * kNnzPerThread;
     std::vector<int> blk;
     blk.push_back(0);
     int r = 0;
-    while (r < A->nrows) {
+    while (r < nrows) {
         int start = r;
-        while (r < A->nrows && r - start < kSpmvThreads && rp[r + 1] - rp[start] <= kCap) ++r;
-        if (r == start) ++r;  // a single row longer than kCap
+        while (r < nrows && r - start < nt && rp[r + 1] - rp[start] <= cap) ++r;
+        if (r == start) ++r;
         blk.push_back(r);
     }
-    A->nblk = (int)blk.size() - 1;
-    A->blk.upload(blk, A->ctx->stream);
+    return blk;
+}
+
+void spmv_prepare(sg_csr* A) {
+    for (int v = 0; v < 3; ++v) {
+        std::vector<int> blk = row_blocks(A->host.rp, A->nrows, kSpmvVariants[v]);
+        A->nblk_v[v] = (int)blk.size() - 1;
+        A->blk_v[v].upload(blk, A->ctx->stream);
+    }
+    A->variant = 0;
+}
+
+static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* x, double beta,
+                                const double* y0, double* y, const int* guard, hipStream_t s) {
+    const int nblk = A->nblk_v[var];
+    const int per = (nblk + kXcd - 1) / kXcd;
+    const int* blk = A->blk_v[var].p;
+    switch (kSpmvVariants[var]) {
+        case 256:
+            hipLaunchKernelGGL(k_spmv_stream<256>, dim3(per * kXcd), dim3(256), 0, s, A->rp.p,
+                               A->ci.p, A->v.p, blk, nblk, per, alpha, x, beta, y0, y, guard);
+            break;
+        case 512:
+            hipLaunchKernelGGL(k_spmv_stream<512>, dim3(per * kXcd), dim3(512), 0, s, A->rp.p,
+                               A->ci.p, A->v.p, blk, nblk, per, alpha, x, beta, y0, y, guard);
+            break;
+        default:
+            hipLaunchKernelGGL(k_spmv_stream<1024>, dim3(per * kXcd), dim3(1024), 0, s, A->rp.p,
+                               A->ci.p, A->v.p, blk, nblk, per, alpha, x, beta, y0, y, guard);
+            break;
+    }
+    SG_HIP(hipGetLastError());
+}
+
+// Times every block shape on this matrix (HIP events, three launches each)
+// and keeps the fastest; ties keep the smaller shape.
+void spmv_autotune(sg_csr* A) {
+    sg_context* ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    DevBuf<double> x, y;
+    x.alloc(std::max(A->ncols, 1));
+    y.alloc(std::max(A->nrows, 1));
+    SG_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * x.n, s));
+    hipEvent_t e0, e1;
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    float best = 0.f;
+    int bestv = 0;
+    for (int v = 0; v < 3; ++v) {
+        spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, nullptr, s);  // warm
+        SG_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, nullptr, s);
+        SG_HIP(hipEventRecord(e1, s));
+        SG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (v == 0 || ms < best * 0.98f) {
+            best = ms;
+            bestv = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A->variant = bestv;
+    A->tuned = true;
 }
 
 void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,
                  double* y, const int* guard, hipStream_t s) {
     if (A->nrows == 0) return;
-    int per = (A->nblk + kXcd - 1) / kXcd;
+    if (A->ctx->autotune == 1 && !A->tuned) spmv_autotune(A);
+    if (A->ctx->autotune >= 2) A->variant = A->ctx->autotune - 2;
+    else if (A->ctx->autotune == 0) A->variant = 0;
     ProfScope ps(A->ctx, SG_K_SPMV);
-    hipLaunchKernelGGL(k_spmv_stream, dim3(per * kXcd), dim3(kSpmvThreads), 0, s, A->rp.p,
-                       A->ci.p, A->v.p, A->blk.p, A->nblk, per, alpha, x, beta,
-                       y0 ? y0 : y, y, guard);
-    SG_HIP(hipGetLastError());
+    spmv_launch_variant(A, A->variant, alpha, x, beta, y0 ? y0 : y, y, guard, s);
 }
 
 }  // namespace sg

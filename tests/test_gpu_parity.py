@@ -187,6 +187,39 @@ def test_fp32_apply_matches_oracle_mixed_mode(sg, ctx, kind):
     assert np.linalg.norm(y.get() - want) <= 1e-5 * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_kernel_shapes_do_not_change_results(sg, precision):
+    """Every SpMV block shape and sweep workgroup size (forced, and the one the
+    autotuner picks) gives bit-identical SpMV and RAS/ASM applies."""
+    A = O.poisson3d(40, 36, 30)
+    rng = np.random.default_rng(11)
+    v = A.v * rng.uniform(0.5, 1.5, A.nnz)
+    x = rng.standard_normal(A.n)
+    outs = []
+    for mode in (0, 1, 2, 3, 4):
+        c = sg.Context(0)
+        c.autotune(mode)
+        dA = sg.Csr(c, A.rp, A.ci, v)
+        dx = dev(sg, c, x)
+        y = dev(sg, c, np.zeros(A.n))
+        dA.spmv(dx, y)
+        res = [y.get()]
+        for kind in ("ras", "asm", "twolevel_ras"):
+            M = sg.Precond(c, kind, nparts=16, overlap=1, precision=precision,
+                           partition=np.arange(A.n, dtype=np.int32) * 16 // A.n).setup(dA)
+            for _ in range(2):  # first call tunes in mode 1
+                z = dev(sg, c, np.zeros(A.n))
+                M.apply(dx, z)
+                res.append(z.get())
+        outs.append(res)
+        del M, dA, dx, y, z
+        c.close()
+    assert np.array_equal(outs[0][0], O.spmv(O.Csr(A.rp, A.ci, v), x))
+    for res in outs[1:]:
+        for a, b in zip(res, outs[0]):
+            assert np.array_equal(a, b)
+
+
 def _rhs(g, A):
     from rng_ref import splitmix_uniform
     if g["rhs"] == "ones":
