@@ -115,6 +115,17 @@ def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, gpu_iters):
                       f"scaled to the {gpu_iters} iterations of the GPU solve"}
 
 
+def copy_probe():
+    import torch
+    a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written":
+                      b.numel() * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +135,13 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
               "multi-GPU path (RCCL over xGMI) is a design in DESIGN.md, not built.",

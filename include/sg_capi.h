@@ -181,8 +181,17 @@ typedef struct {
                         verified with an FP64 SpMV (tol_abs is then ignored)           */
     int ortho;       /* sg_ortho: MGS (reference order) or CGS2                       */
     int lookahead;   /* Arnoldi steps the host may run ahead of the device (0 = 2)     */
-    int flags;       /* reserved, 0                                                   */
+    int flags;       /* sg_solve_flags, or 0                                          */
 } sg_solve_params;
+
+typedef enum {
+    /* sg_solve: capture the restart cycle (restart residual, m Arnoldi steps,
+     * back substitution, x update) once as a hipGraph and replay it every
+     * cycle; steps after convergence are launched and return at once. */
+    SG_SOLVE_GRAPH = 1,
+    /* sg_solve_batch: launch every kernel directly instead of the graph. */
+    SG_SOLVE_NO_GRAPH = 2
+} sg_solve_flags;
 
 typedef struct {
     int iterations;           /* GMRESResult::iterations                              */
@@ -200,6 +209,29 @@ typedef struct {
 sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                    const sg_solve_params* params, sg_solve_result* result, double* hist,
                    int hist_cap);
+
+/* ---- batched solve (many independent systems) -----------------------------
+ * No counterpart in the reference (it solves one system per run,
+ * src/main.cpp:238-243).  The systems are stored as one block-diagonal CSR:
+ * rows [sys_rows[i], sys_rows[i+1]) of system i (sys_rows[0] = 0,
+ * sys_rows[nsys] = n) may only reference columns in that range, with global
+ * column indices.  Every SpMV, preconditioner sweep and reduction covers all
+ * systems in one launch; a preconditioner built on this matrix must not let a
+ * subdomain span two systems (give each system its own parts in
+ * sg_precond_opts.partition) for per-system early exit. */
+sg_status sg_csr_create_batch(sg_context* ctx, int nsys, const int* sys_rows, int nnz,
+                              const int* row_ptr, const int* col_idx, const double* vals,
+                              sg_csr** out);
+sg_status sg_csr_systems(const sg_csr* A, int* nsys);
+/* Solves all systems of a batch matrix together: each system runs its own
+ * GMRES/FGMRES state machine (own Hessenberg, Givens, tolerance, restart and
+ * convergence decisions) with the same arithmetic as sg_solve on that system
+ * alone, so its x and iteration count equal a separate sg_solve bit for bit.
+ * result[nsys]; hist (may be NULL) is nsys rows of hist_cap entries.  The
+ * restart cycle runs as a captured hipGraph unless SG_SOLVE_NO_GRAPH. */
+sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                         const sg_solve_params* params, sg_solve_result* result, double* hist,
+                         int hist_cap);
 
 /* ---- kernel timing (HIP events on the context stream) ------------------------
  * With profiling on, every launch of the listed kernel families is bracketed

@@ -99,6 +99,11 @@ def lib():
         "sg_solve": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult), _VP,
                       C.c_int], C.c_int),
         "sg_partition": ([C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP], C.c_int),
+        "sg_csr_create_batch": ([_VP, C.c_int, _VP, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
+                                C.c_int),
+        "sg_csr_systems": ([_VP, C.POINTER(C.c_int)], C.c_int),
+        "sg_solve_batch": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), _VP, _VP, C.c_int],
+                           C.c_int),
         "sg_precond_stats": ([_VP, C.POINTER(PrecondInfo)], C.c_int),
         "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
         "sg_context_set_autotune": ([_VP, C.c_int], C.c_int),
@@ -165,6 +170,59 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class BatchCsr:
+    """Block-diagonal batch of independent systems (sg_csr_create_batch).
+
+    systems: list of (row_ptr, col_idx, vals) with local indices; they are
+    concatenated with column offsets."""
+
+    def __init__(self, ctx, systems):
+        self.ctx = ctx
+        rps, cis, vs, off = [], [], [], [0]
+        nnz = 0
+        for rp, ci, v in systems:
+            rp = np.asarray(rp, np.int64)
+            rps.append(rp[:-1] + nnz)
+            cis.append(np.asarray(ci, np.int64) + off[-1])
+            vs.append(np.asarray(v, np.float64))
+            nnz += int(rp[-1])
+            off.append(off[-1] + len(rp) - 1)
+        self.rp = np.ascontiguousarray(np.concatenate(rps + [np.array([nnz])]), np.int32)
+        self.ci = np.ascontiguousarray(np.concatenate(cis), np.int32)
+        self.v = np.ascontiguousarray(np.concatenate(vs))
+        self.sys_rows = np.asarray(off, np.int32)
+        self.nsys = len(systems)
+        self.n = int(off[-1])
+        self.h = _VP()
+        ctx.check(lib().sg_csr_create_batch(ctx.h, self.nsys, _p(self.sys_rows), nnz, _p(self.rp),
+                                            _p(self.ci), _p(self.v), C.byref(self.h)))
+
+    @property
+    def nnz(self):
+        return int(self.rp[-1])
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0):
+        self.ctx.check(lib().sg_spmv(self.h, alpha, x.ptr, beta, y.ptr))
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().sg_csr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def batch_partition(systems, nparts, ufactor=0, weighted=False):
+    """Concatenated partition for a batch: system i gets parts
+    [sum(nparts[:i]), sum(nparts[:i+1])) from the host partitioner."""
+    out, base = [], 0
+    for (rp, ci, v), k in zip(systems, nparts):
+        out.append(partition(rp, ci, k, ufactor, vals=v, weighted=weighted) + base)
+        base += k
+    return np.ascontiguousarray(np.concatenate(out), np.int32)
 
 
 class DeviceArray:
@@ -323,19 +381,41 @@ def partition(rp, ci, nparts, ufactor=0, vals=None, weighted=False):
     return part
 
 
-def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10, rtol=0.0,
-          ortho="mgs", lookahead=0, hist_cap=None):
-    """Run sg_solve.  b, x are DeviceArrays (x in/out).  Returns a dict shaped
-    like GMRESResult plus the FP64 true relative residual."""
-    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
-                    1 if ortho == "cgs2" else 0, lookahead, 0)
-    r = SolveResult()
-    cap = hist_cap if hist_cap is not None else max_iter + restart + 2
-    hist = np.zeros(max(cap, 1))
-    A.ctx.check(lib().sg_solve(A.h, M.h if M is not None else None, b.ptr, x.ptr, C.byref(p),
-                               C.byref(r), _p(hist), cap))
+SOLVE_GRAPH = 1
+SOLVE_NO_GRAPH = 2
+
+
+def _result(r, hist, cap):
     return dict(iterations=r.iterations, converged=bool(r.converged),
                 residual_norm=r.residual_norm, rel_true_residual=r.rel_true_residual,
                 time_solve_s=r.time_solve_s, cycles=r.cycles,
                 precond_applies=r.precond_applies,
                 hist=hist[:min(r.iterations, cap)].copy())
+
+
+def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10, rtol=0.0,
+          ortho="mgs", lookahead=0, hist_cap=None, graph=False):
+    """Run sg_solve.  b, x are DeviceArrays (x in/out).  Returns a dict shaped
+    like GMRESResult plus the FP64 true relative residual."""
+    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
+                    1 if ortho == "cgs2" else 0, lookahead, SOLVE_GRAPH if graph else 0)
+    r = SolveResult()
+    cap = hist_cap if hist_cap is not None else max_iter + restart + 2
+    hist = np.zeros(max(cap, 1))
+    A.ctx.check(lib().sg_solve(A.h, M.h if M is not None else None, b.ptr, x.ptr, C.byref(p),
+                               C.byref(r), _p(hist), cap))
+    return _result(r, hist, cap)
+
+
+def solve_batch(A, b, x, M=None, method="fgmres", restart=30, max_iter=1000, tol=1e-10,
+                rtol=0.0, ortho="cgs2", graph=True, hist_cap=None):
+    """Run sg_solve_batch on a BatchCsr: one result dict per system."""
+    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
+                    1 if ortho == "cgs2" else 0, 0, 0 if graph else SOLVE_NO_GRAPH)
+    nsys = A.nsys
+    rs = (SolveResult * nsys)()
+    cap = hist_cap if hist_cap is not None else max_iter + restart + 2
+    hist = np.zeros((nsys, max(cap, 1)))
+    A.ctx.check(lib().sg_solve_batch(A.h, M.h if M is not None else None, b.ptr, x.ptr,
+                                     C.byref(p), C.cast(rs, _VP), _p(hist), hist.shape[1]))
+    return [_result(rs[q], hist[q], cap) for q in range(nsys)]

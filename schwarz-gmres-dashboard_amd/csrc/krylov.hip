@@ -1,5 +1,6 @@
 // Restarted GMRES(m) (left preconditioning) and FGMRES(m) (flexible right
-// preconditioning) with every scalar on the device.
+// preconditioning) with every scalar on the device, for one system or for a
+// batch of independent systems stored as one block-diagonal matrix.
 //
 // Semantics follow src/core/gmres.cpp:34-160 and src/core/fgmres.cpp:28-172:
 // absolute test on beta at each restart and on |g_{j+1}| after each Arnoldi
@@ -10,21 +11,26 @@
 // reference's order or as a two-pass CGS2 whose re-orthogonalisation
 // coefficients come from the Gram matrix.
 //
-// Reductions: a grid of blocks, each over one contiguous range of the vector,
-// writes per-block partial sums; a one-block "finish" kernel adds them in a
-// fixed order and runs the scalar epilogue.  No grid-wide atomics or fences,
-// and the result is bitwise reproducible run to run.
+// Reductions: every vector kernel runs over a table of blocks, each block one
+// contiguous range inside one system; blocks write per-block partial sums and
+// a "finish" kernel (one workgroup per system) adds them in a fixed order and
+// runs that system's scalar epilogue.  No grid-wide atomics or fences.  A
+// system's ranges are the same whether it is solved alone or in a batch, so
+// its iterates are bitwise identical either way.
 //
-// The host launches a whole restart cycle without reading anything back; each
-// kernel of a step checks a device flag and returns at once when the cycle has
-// ended (converged or broke down).  The host may run `lookahead` steps ahead of
-// the device and reads the flag asynchronously to stop launching, and
-// synchronises once per cycle.
+// Control: each system has a device state (KState) whose `active` flag every
+// kernel of a step checks, so a system that converged or broke down drops out
+// of the rest of the cycle.  The host reads the states back once per cycle and
+// decides per system (next cycle, true-residual verification, done).  A cycle
+// launches either directly (with `lookahead`: the host reads the flag of one
+// system asynchronously and stops launching) or as one hipGraph captured
+// before the first cycle and replayed (default for batches).
 #include "sg_internal.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 namespace sg {
 
@@ -33,14 +39,16 @@ int precond_n(const sg_precond* M);
 bool precond_ready(const sg_precond* M);
 
 constexpr int kT = 256;                 // threads per reduction block
-constexpr int kMaxBlocks = 4096;        // partial-sum slots per column
+constexpr int kMaxBlocks = 4096;        // blocks per system and split
 constexpr int kMaxCols = 128;           // restart + 1
 constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis vector)
-constexpr int kCB = 16;                 // basis vectors per register block
+constexpr int kCBD = 8;                 // basis vectors per register block, Gram dots
+constexpr int kCBU = 16;                // basis vectors per register block, Gram update
 constexpr int kEo = 2;                  // elements per thread per tile
 constexpr int kTile = kT * kEo;
 constexpr int kTG = 512;                // threads per block of the Gram passes
 constexpr int kTileG = kTG * kEo;
+constexpr int kTF = 1024;               // threads of the finish kernels
 
 struct KState {
     int active;     // current cycle continues
@@ -51,6 +59,8 @@ struct KState {
     int first;      // no restart done yet
     int method;     // 0 left GMRES, 1 FGMRES
     int m;
+    int has_m;      // a preconditioner is applied
+    int applies;    // preconditioner applications so far
     double tol;     // test threshold on beta and |g|
     double rtol;    // > 0: true-residual mode
     double bnorm;
@@ -59,34 +69,89 @@ struct KState {
     double res;
 };
 
+// Per-system control written by the host before each cycle.
+enum : int { kRun = 0, kVerify = 1, kDone = 2 };
+
 struct Work {
-    double* H;     // (m+1) x m, column-major H[i + j*(m+1)]
-    double* g;     // m+1
-    double* cs;    // m
-    double* sn;    // m
-    double* y;     // m
-    double* h1;    // kRedCols: projection coefficients h
-    double* h2;    // kRedCols: reduced dots
-    double* G;     // kMaxCols^2: Gram matrix V^T V, row-major, stride kMaxCols
-    double* hist;  // residual history
-    double* part;  // kRedCols * kMaxBlocks block partial sums, column-major
+    double* H;      // (m+1) x m, column-major H[i + j*(m+1)]
+    double* g;      // m+1
+    double* cs;     // m
+    double* sn;     // m
+    double* y;      // m
+    double* h1;     // kRedCols: projection coefficients h
+    double* h2;     // kRedCols: reduced dots
+    double* G;      // kMaxCols^2: Gram matrix V^T V, row-major, stride kMaxCols
+    double* hist;   // residual history (hcap)
+    double* part;   // block partial sums, part[c * pstride + block]
     KState* st;
+    const int* mode;
+    int m, hcap, pstride;
 };
 
-// Contiguous element range of block b: [b*chunk, min(n, (b+1)*chunk)).
-struct Split {
-    int n, chunk, nblk;
+// The view of system s (pointers advanced to its slices; part is shared).
+__device__ inline Work sys_work(const Work& W, int s) {
+    Work o = W;
+    const size_t m = W.m;
+    o.H += (size_t)s * (m + 1) * m;
+    o.g += (size_t)s * (m + 1);
+    o.cs += (size_t)s * m;
+    o.sn += (size_t)s * m;
+    o.y += (size_t)s * m;
+    o.h1 += (size_t)s * kRedCols;
+    o.h2 += (size_t)s * kRedCols;
+    o.G += (size_t)s * kMaxCols * kMaxCols;
+    o.hist += (size_t)s * W.hcap;
+    o.st += s;
+    o.mode += s;
+    return o;
+}
+
+// One block of a split: elements [start, end) of system sys.
+struct BlockRange {
+    int sys, start, end, pad;
+};
+struct SysBlocks {
+    int blk0, nblk;
+};
+struct Segs {
+    const BlockRange* blk;
+    const SysBlocks* sys;
 };
 
-static Split make_split(int n, int max_blocks, int tile = kTile) {
-    Split s;
-    s.n = n;
+// Block count and chunk of one system of n elements (the chunk is a multiple
+// of the tile so every block but the last runs whole tiles).
+static void split_shape(int n, int max_blocks, int tile, int& nblk, int& chunk) {
     int want = std::max(1, (n + tile - 1) / tile);
-    s.nblk = std::min(want, std::min(max_blocks, kMaxBlocks));
-    int c = (n + s.nblk - 1) / s.nblk;
-    s.chunk = (c + tile - 1) / tile * tile;
-    s.nblk = std::max(1, (n + s.chunk - 1) / s.chunk);
-    return s;
+    nblk = std::min(want, std::min(max_blocks, kMaxBlocks));
+    int c = (n + nblk - 1) / nblk;
+    chunk = (c + tile - 1) / tile * tile;
+    nblk = std::max(1, (n + chunk - 1) / chunk);
+}
+
+struct SegsHost {
+    DevBuf<BlockRange> blk;
+    DevBuf<SysBlocks> sys;
+    int nblk = 0;
+    Segs dev() const { return Segs{blk.p, sys.p}; }
+};
+
+static void build_segs(const std::vector<int>& off, int max_blocks, int tile, SegsHost& out,
+                       hipStream_t s) {
+    std::vector<BlockRange> br;
+    std::vector<SysBlocks> sb;
+    for (size_t q = 0; q + 1 < off.size(); ++q) {
+        const int n = off[q + 1] - off[q];
+        int nblk, chunk;
+        split_shape(n, max_blocks, tile, nblk, chunk);
+        sb.push_back(SysBlocks{(int)br.size(), nblk});
+        for (int b = 0; b < nblk; ++b)
+            br.push_back(BlockRange{(int)q, off[q] + b * chunk,
+                                    off[q] + std::min(n, (b + 1) * chunk), 0});
+    }
+    out.nblk = (int)br.size();
+    out.blk.upload(br, s);
+    out.sys.upload(sb, s);
+    SG_HIP(hipStreamSynchronize(s));
 }
 
 __device__ inline double wave_sum(double v) {
@@ -118,11 +183,13 @@ __device__ inline void givens_apply(double c, double s, double& h1, double& h2) 
     h2 = t2;
 }
 
-// End of Arnoldi step j once h_{j+1,j} is known (gmres.cpp:99-127).
+// End of Arnoldi step j once h_{j+1,j} is known (gmres.cpp:99-127); W is one
+// system's view.
 __device__ void step_epilogue(const Work& W, int j, double hn) {
     KState* st = W.st;
     const int m = st->m;
     double* H = W.H;
+    st->applies += st->has_m;
     H[(j + 1) + j * (m + 1)] = hn;
     if (hn < 1e-30) {  // breakdown: column j is used unrotated
         st->k = j + 1;
@@ -152,25 +219,35 @@ __device__ void step_epilogue(const Work& W, int j, double hn) {
 
 // Adds the per-wave values red[wave][c] and stores column c's block partial.
 template <int NT = kT>
-__device__ inline void store_partials(double (*red)[kRedCols], int ncols, double* part) {
+__device__ inline void store_partials(double (*red)[kRedCols], int ncols, double* part,
+                                      int pstride) {
     __syncthreads();
     for (int i = threadIdx.x; i < ncols; i += NT) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; ++w) s += red[w][i];
-        part[(size_t)i * kMaxBlocks + blockIdx.x] = s;
+        part[(size_t)i * pstride + blockIdx.x] = s;
     }
 }
 
-// One block: out[c] = sum over blocks of part[c][b].  Columns are spread over
-// the wavefronts; a wavefront adds a column's partials lane-strided and then
-// across lanes, always in the same order.  Ends with __syncthreads.
-__device__ void reduce_partials(const double* part, int ncols, int nblk, double* out) {
+// out[c] = sum over a system's blocks of part[c][blk0 + b].  One wavefront per
+// column: each lane loads its partials (16 loads in flight), adds them in
+// order, then across lanes.  Ends with __syncthreads.
+__device__ void reduce_partials(const Work& W, const SysBlocks& sb, int ncols, double* out) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int i = wv; i < ncols; i += nw) {
-        const double* pi = part + (size_t)i * kMaxBlocks;
+        const double* pi = W.part + (size_t)i * W.pstride + sb.blk0;
         double s = 0.0;
-        for (int b = lane; b < nblk; b += 64) s += pi[b];
+        for (int b0 = 0; b0 < sb.nblk; b0 += 16 * 64) {
+            double v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int b = b0 + q * 64 + lane;
+                v[q] = b < sb.nblk ? pi[b] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) s += v[q];
+        }
         s = wave_sum(s);
         if (lane == 0) out[i] = s;
     }
@@ -184,45 +261,57 @@ __device__ void reduce_partials(const double* part, int ncols, int nblk, double*
 // loss of orthogonality, i.e. the coefficients of projecting twice.
 // Pass 2 (k_gram_update + k_norm_finish): w -= V h, ||w||, step epilogue.
 // Columns beyond ncols inside a register block are clamped to a valid column
-// and masked, so every load is unconditional and stays in flight.
+// and masked, elements beyond the block to its last element; every load of a
+// tile is issued before the scheduling barrier, so a wave keeps a whole
+// register block of loads in flight.
 __global__ void __launch_bounds__(kTG)
-k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
+k_gram_dot(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
            const double* __restrict__ w, Work W, int j) {
-    if (!W.st->active) return;
+    const BlockRange br = sp.blk[blockIdx.x];
+    if (!W.st[br.sys].active) return;
     __shared__ double red[kTG / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
+    const int start = br.start, end = br.end;
     const double* vj = V + (size_t)j * ld;
-    for (int cb = 0; cb < ncols; cb += kCB) {
-        const double* Vc[kCB];
+    for (int cb = 0; cb < ncols; cb += kCBD) {
+        const double* Vc[kCBD];
 #pragma unroll
-        for (int c = 0; c < kCB; ++c) Vc[c] = V + (size_t)min(cb + c, ncols - 1) * ld;
-        double aw[kCB], av[kCB];
+        for (int c = 0; c < kCBD; ++c) Vc[c] = V + (size_t)min(cb + c, ncols - 1) * ld;
+        double aw[kCBD], av[kCBD];
 #pragma unroll
-        for (int c = 0; c < kCB; ++c) aw[c] = av[c] = 0.0;
+        for (int c = 0; c < kCBD; ++c) aw[c] = av[c] = 0.0;
         for (int base = start + tid; base < end; base += kTileG) {
             int ee[kEo];
-            double wr[kEo], vr[kEo];
+            double msk[kEo], wr[kEo], vr[kEo], a[kCBD][kEo];
 #pragma unroll
             for (int r = 0; r < kEo; ++r) {
                 const int e = base + r * kTG;
-                ee[r] = min(e, sp.n - 1);
-                const double msk = e < end ? 1.0 : 0.0;
-                wr[r] = w[ee[r]] * msk;
-                vr[r] = vj[ee[r]] * msk;
+                ee[r] = min(e, end - 1);
+                msk[r] = e < end ? 1.0 : 0.0;
+                wr[r] = w[ee[r]];
+                vr[r] = vj[ee[r]];
             }
 #pragma unroll
-            for (int c = 0; c < kCB; ++c) {
+            for (int c = 0; c < kCBD; ++c)
+#pragma unroll
+                for (int r = 0; r < kEo; ++r) a[c][r] = Vc[c][ee[r]];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < kEo; ++r) {
+                wr[r] *= msk[r];
+                vr[r] *= msk[r];
+            }
+#pragma unroll
+            for (int c = 0; c < kCBD; ++c) {
 #pragma unroll
                 for (int r = 0; r < kEo; ++r) {
-                    const double a = Vc[c][ee[r]];
-                    aw[c] += a * wr[r];
-                    av[c] += a * vr[r];
+                    aw[c] += a[c][r] * wr[r];
+                    av[c] += a[c][r] * vr[r];
                 }
             }
         }
 #pragma unroll
-        for (int c = 0; c < kCB; ++c) {
+        for (int c = 0; c < kCBD; ++c) {
             const double sw = wave_sum(aw[c]);
             const double sv = wave_sum(av[c]);
             if (lane == 0 && cb + c < ncols) {
@@ -232,15 +321,16 @@ k_gram_dot(Split sp, int ncols, const double* __restrict__ V, size_t ld,
         }
     }
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
-    store_partials<kTG>(red, 2 * ncols, W.part);
+    store_partials<kTG>(red, 2 * ncols, W.part, W.pstride);
 }
 
-__global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, int nblk) {
+__global__ void __launch_bounds__(kTF) k_gram_finish(Work W0, Segs sp, int ncols, int j) {
+    const Work W = sys_work(W0, blockIdx.x);
     if (!W.st->active) return;
-    reduce_partials(W.part, 2 * ncols, nblk, W.h2);
+    reduce_partials(W, sp.sys[blockIdx.x], 2 * ncols, W.h2);
     const int m = W.st->m;
     double* Gm = W.G;
-    for (int i = threadIdx.x; i < ncols; i += kT) {  // Gram row of v_j
+    for (int i = threadIdx.x; i < ncols; i += kTF) {  // Gram row of v_j
         const double g = W.h2[ncols + i];
         Gm[j * kMaxCols + i] = g;
         Gm[i * kMaxCols + j] = g;
@@ -248,7 +338,7 @@ __global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, in
     __syncthreads();
     // h = 2p - G p, one wavefront per row
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = wv; i < ncols; i += kT / 64) {
+    for (int i = wv; i < ncols; i += kTF / 64) {
         double gp = 0.0;
         for (int k = lane; k < ncols; k += 64) gp += Gm[i * kMaxCols + k] * W.h2[k];
         gp = wave_sum(gp);
@@ -261,31 +351,40 @@ __global__ void __launch_bounds__(kT) k_gram_finish(Work W, int ncols, int j, in
 }
 
 __global__ void __launch_bounds__(kTG)
-k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
-              double* __restrict__ w, Work W) {
-    if (!W.st->active) return;
+k_gram_update(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
+              double* __restrict__ w, Work W0) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    if (!W0.st[br.sys].active) return;
+    const Work W = sys_work(W0, br.sys);
     __shared__ double red[kTG / 64][kRedCols];
-    __shared__ double hs[kMaxCols + kCB];
+    __shared__ double hs[kMaxCols + kCBU];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < kMaxCols + kCB; i += kTG) hs[i] = i < ncols ? W.h1[i] : 0.0;
+    for (int i = tid; i < kMaxCols + kCBU; i += kTG) hs[i] = i < ncols ? W.h1[i] : 0.0;
     __syncthreads();
-    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
+    const int start = br.start, end = br.end;
     double acc = 0.0;
     for (int base = start + tid; base < end; base += kTileG) {
         int ee[kEo];
         double wr[kEo];
 #pragma unroll
         for (int r = 0; r < kEo; ++r) {
-            ee[r] = min(base + r * kTG, sp.n - 1);
+            ee[r] = min(base + r * kTG, end - 1);
             wr[r] = w[ee[r]];
         }
-        for (int cb = 0; cb < ncols; cb += kCB) {
+        for (int cb = 0; cb < ncols; cb += kCBU) {
+            double a[kCBU][kEo];
 #pragma unroll
-            for (int c = 0; c < kCB; ++c) {
-                const double hi = hs[cb + c];  // 0 beyond ncols: subtracts an exact 0
+            for (int c = 0; c < kCBU; ++c) {
                 const double* Vi = V + (size_t)min(cb + c, ncols - 1) * ld;
 #pragma unroll
-                for (int r = 0; r < kEo; ++r) wr[r] -= hi * Vi[ee[r]];
+                for (int r = 0; r < kEo; ++r) a[c][r] = Vi[ee[r]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < kCBU; ++c) {
+                const double hi = hs[cb + c];  // 0 beyond ncols: subtracts an exact 0
+#pragma unroll
+                for (int r = 0; r < kEo; ++r) wr[r] -= hi * a[c][r];
             }
         }
 #pragma unroll
@@ -298,13 +397,14 @@ k_gram_update(Split sp, int ncols, const double* __restrict__ V, size_t ld,
     }
     acc = wave_sum(acc);
     if (lane == 0) red[wv][0] = acc;
-    store_partials<kTG>(red, 1, W.part);
+    store_partials<kTG>(red, 1, W.part, W.pstride);
 }
 
-__global__ void __launch_bounds__(kT) k_norm_finish(Work W, int j, int nblk) {
+__global__ void __launch_bounds__(kTF) k_norm_finish(Work W0, Segs sp, int j) {
+    const Work W = sys_work(W0, blockIdx.x);
     if (!W.st->active) return;
     double* out = W.h2 + kRedCols - 1;
-    reduce_partials(W.part, 1, nblk, out);
+    reduce_partials(W, sp.sys[blockIdx.x], 1, out);
     if (threadIdx.x == 0) step_epilogue(W, j, sqrt(*out));
 }
 
@@ -312,16 +412,17 @@ __global__ void __launch_bounds__(kT) k_norm_finish(Work W, int j, int nblk) {
 // k_mgs: w -= h_{i-1} V_{i-1} (when i > 0), then partial sums of w . V_i, or
 // of w . w when Vi is null; k_mgs_finish stores H[i,j] or runs the epilogue.
 __global__ void __launch_bounds__(kT)
-k_mgs(Split sp, const double* __restrict__ Vprev, const double* __restrict__ Vi,
-      double* __restrict__ w, Work W, int i, int j) {
-    if (!W.st->active) return;
+k_mgs(Segs sp, const double* __restrict__ Vprev, const double* __restrict__ Vi,
+      double* __restrict__ w, Work W0, int i, int j) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    if (!W0.st[br.sys].active) return;
+    const Work W = sys_work(W0, br.sys);
     __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int m = W.st->m;
     const double hprev = Vprev ? W.H[(i - 1) + j * (m + 1)] : 0.0;
-    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double s = 0.0;
-    for (int e = start + tid; e < end; e += kT) {
+    for (int e = br.start + tid; e < br.end; e += kT) {
         double we = w[e];
         if (Vprev) {
             we -= hprev * Vprev[e];
@@ -331,27 +432,31 @@ k_mgs(Split sp, const double* __restrict__ Vprev, const double* __restrict__ Vi,
     }
     s = wave_sum(s);
     if (lane == 0) red[wv][0] = s;
-    store_partials(red, 1, W.part);
+    store_partials(red, 1, W.part, W.pstride);
 }
 
-__global__ void __launch_bounds__(kT) k_mgs_finish(Work W, int i, int j, int nblk, int is_norm) {
+__global__ void __launch_bounds__(kTF) k_mgs_finish(Work W0, Segs sp, int i, int j, int is_norm) {
+    const Work W = sys_work(W0, blockIdx.x);
     if (!W.st->active) return;
     double* out = W.h2 + kRedCols - 1;
-    reduce_partials(W.part, 1, nblk, out);
+    reduce_partials(W, sp.sys[blockIdx.x], 1, out);
     if (threadIdx.x != 0) return;
     if (is_norm) step_epilogue(W, j, sqrt(*out));
     else W.H[i + j * (W.st->m + 1)] = *out;
 }
 
 // ---- restart ---------------------------------------------------------------
-// Partial sums of u.u and (when r is given) r.r.
+// Partial sums of u.u and (when r is given) r.r; blocks of finished systems
+// skip when skip_done is set.
 __global__ void __launch_bounds__(kT)
-k_norms(Split sp, const double* __restrict__ u, const double* __restrict__ r, Work W) {
+k_norms(Segs sp, const double* __restrict__ u, const double* __restrict__ r, Work W,
+        int skip_done) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    if (skip_done && W.mode[br.sys] == kDone) return;
     __shared__ double red[kT / 64][kRedCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int start = blockIdx.x * sp.chunk, end = min(sp.n, start + sp.chunk);
     double su = 0.0, sr = 0.0;
-    for (int e = start + tid; e < end; e += kT) {
+    for (int e = br.start + tid; e < br.end; e += kT) {
         const double a = u[e];
         su += a * a;
         if (r) {
@@ -365,23 +470,35 @@ k_norms(Split sp, const double* __restrict__ u, const double* __restrict__ r, Wo
         red[wv][0] = su;
         red[wv][1] = sr;
     }
-    store_partials(red, 2, W.part);
+    store_partials(red, 2, W.part, W.pstride);
 }
 
 // ||u|| from k_norms into st->bnorm (also used for the exit residual).
-__global__ void __launch_bounds__(kT) k_bnorm_finish(Work W, int nblk) {
+__global__ void __launch_bounds__(kTF) k_bnorm_finish(Work W0, Segs sp) {
+    const Work W = sys_work(W0, blockIdx.x);
     double* out = W.h2 + kRedCols - 2;
-    reduce_partials(W.part, 2, nblk, out);
+    reduce_partials(W, sp.sys[blockIdx.x], 2, out);
     if (threadIdx.x == 0) W.st->bnorm = sqrt(out[0]);
 }
 
 // Beta test and start of the cycle (gmres.cpp:59-81, fgmres.cpp:54-71); in
 // true-residual mode the relative residual ||r||/||b|| decides convergence.
-__global__ void __launch_bounds__(kT) k_restart_finish(Work W, int nblk, int has_r) {
-    double* out = W.h2 + kRedCols - 2;
-    reduce_partials(W.part, 2, nblk, out);
-    if (threadIdx.x != 0) return;
+// A system in verification mode only checks; a finished one does nothing.
+__global__ void __launch_bounds__(kTF) k_restart_finish(Work W0, Segs sp, int has_r) {
+    const Work W = sys_work(W0, blockIdx.x);
     KState* st = W.st;
+    const int md = *W.mode;
+    if (md == kDone) {
+        if (threadIdx.x == 0) {
+            st->active = 0;
+            st->k = 0;
+        }
+        return;
+    }
+    double* out = W.h2 + kRedCols - 2;
+    reduce_partials(W, sp.sys[blockIdx.x], 2, out);
+    if (threadIdx.x != 0) return;
+    if (st->method == 0) st->applies += st->has_m;  // M^{-1} r of the restart
     const double beta = sqrt(out[0]);
     const double rn = has_r ? sqrt(out[1]) : beta;
     st->k = 0;
@@ -406,6 +523,10 @@ __global__ void __launch_bounds__(kT) k_restart_finish(Work W, int nblk, int has
         return;
     }
     st->first = 0;
+    if (md == kVerify) {
+        st->active = 0;
+        return;
+    }
     st->active = 1;
     const int m = st->m;
     W.g[0] = beta;
@@ -415,19 +536,22 @@ __global__ void __launch_bounds__(kT) k_restart_finish(Work W, int nblk, int has
 
 // v = u / scale (GMRES, gmres.cpp:76-77,107-108) or u * scale (FGMRES,
 // fgmres.cpp:66-67,97-98).
-__global__ void k_scale_basis(int n, const double* __restrict__ u, double* __restrict__ v,
-                              const KState* __restrict__ st) {
+__global__ void __launch_bounds__(kT)
+k_scale_basis(Segs sp, const double* __restrict__ u, double* __restrict__ v, Work W) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    const KState* st = W.st + br.sys;
     if (!st->active) return;
     const double sc = st->scale;
     const bool div = st->method == 0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-        v[e] = div ? u[e] / sc : u[e] * sc;
+    for (int e = br.start + threadIdx.x; e < br.end; e += kT) v[e] = div ? u[e] / sc : u[e] * sc;
 }
 
-// Back substitution H y = g (gmres.cpp:131-137): the block stages the k x k
-// upper triangle in LDS, one thread solves in the reference's order.
-__global__ void __launch_bounds__(64) k_backsolve(Work W) {
+// Back substitution H y = g (gmres.cpp:131-137), one workgroup per system:
+// the k x k upper triangle is staged in LDS, one thread solves in the
+// reference's order.
+__global__ void __launch_bounds__(64) k_backsolve(Work W0) {
     extern __shared__ double hl[];  // packed rows: row i holds H[i][i..k-1]
+    const Work W = sys_work(W0, blockIdx.x);
     const KState* st = W.st;
     const int k = st->k, m = st->m;
     if (k == 0) return;
@@ -448,51 +572,365 @@ __global__ void __launch_bounds__(64) k_backsolve(Work W) {
 }
 
 // x += sum_i y_i B_i in the reference's axpy order (gmres.cpp:140-141).
-__global__ void k_update(int n, double* __restrict__ x, const double* __restrict__ B, size_t ld,
-                         Work W) {
+__global__ void __launch_bounds__(kT)
+k_update(Segs sp, double* __restrict__ x, const double* __restrict__ B, size_t ld, Work W0) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    const Work W = sys_work(W0, br.sys);
     const int k = W.st->k;
     if (k == 0) return;
     __shared__ double ys[kMaxCols];
     for (int i = threadIdx.x; i < k; i += blockDim.x) ys[i] = W.y[i];
     __syncthreads();
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    for (int e = br.start + threadIdx.x; e < br.end; e += kT) {
         double xe = x[e];
         for (int i = 0; i < k; ++i) xe += ys[i] * B[(size_t)i * ld + e];
         x[e] = xe;
     }
 }
 
-__global__ void k_init_state(KState* st, int method, int m, double tol, double rtol) {
-    st->active = 0;
-    st->k = 0;
-    st->converged = 0;
-    st->total = 0;
-    st->est_conv = 0;
-    st->first = 1;
-    st->method = method;
-    st->m = m;
-    st->tol = tol;
-    st->rtol = rtol;
-    st->bnorm = 0.0;
-    st->rel_true = -1.0;
-    st->scale = 1.0;
-    st->res = 0.0;
-}
-
-static int ew_blocks(int n) {
-    long long g = ((long long)n + 1023) / 1024;
-    return (int)std::max<long long>(1, std::min<long long>(g, 8192));
+__global__ void k_init_state(KState* st, int nsys, int method, int m, int has_m, double tol,
+                             double rtol) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    KState k{};
+    k.active = 0;
+    k.k = 0;
+    k.converged = 0;
+    k.total = 0;
+    k.est_conv = 0;
+    k.first = 1;
+    k.method = method;
+    k.m = m;
+    k.has_m = has_m;
+    k.applies = 0;
+    k.tol = tol;
+    k.rtol = rtol;
+    k.bnorm = 0.0;
+    k.rel_true = -1.0;
+    k.scale = 1.0;
+    k.res = 0.0;
+    st[s] = k;
 }
 
 struct SolveWs {
     DevBuf<double> vecs;  // V, Z, r, z, w, t
     DevBuf<double> small;
     DevBuf<KState> st;
-    int* pinned = nullptr;
+    DevBuf<int> mode;
+    int* pinned = nullptr;   // lookahead flags (m)
+    int* h_mode = nullptr;   // per-system control (nsys)
+    KState* h_st = nullptr;  // per-system state read back (nsys)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     ~SolveWs() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (pinned) (void)hipHostFree(pinned);
+        if (h_mode) (void)hipHostFree(h_mode);
+        if (h_st) (void)hipHostFree(h_st);
     }
 };
+
+// Ends a stream capture that an exception interrupted, so the stream stays usable.
+struct CaptureGuard {
+    hipStream_t s;
+    bool on = false;
+    ~CaptureGuard() {
+        if (on) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    }
+};
+
+// Profiling off for the scope (event records cannot join a captured graph).
+struct ProfOff {
+    sg_context* c;
+    bool was;
+    explicit ProfOff(sg_context* ctx) : c(ctx), was(ctx->profiling) { c->profiling = false; }
+    ~ProfOff() { c->profiling = was; }
+};
+
+static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                       const sg_solve_params* prm, sg_solve_result* res, double* hist,
+                       int hist_cap, bool use_graph) {
+    sg_context* ctx = A->ctx;
+    if (A->nrows != A->ncols) fail(SG_E_ARG, "GMRES needs a square matrix");
+    if (M && !precond_ready(M)) fail(SG_E_STATE, "preconditioner used before setup");
+    if (M && precond_n(M) != A->nrows) fail(SG_E_ARG, "preconditioner size mismatch");
+    const int n = A->nrows;
+    const int m = prm->restart;
+    const int method = prm->method == SG_FGMRES ? 1 : 0;
+    if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
+    if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
+    if (n < 1) fail(SG_E_ARG, "empty system");
+    const std::vector<int>& off = A->sys_off;
+    const int nsys = (int)off.size() - 1;
+    if (M && precond_is_callback(M) && (use_graph || nsys > 1))
+        fail(SG_E_ARG, "a host-callback preconditioner runs only in unbatched, uncaptured solves");
+    const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
+    const int L = prm->lookahead > 0 ? prm->lookahead : 2;
+    const bool rmode = prm->rtol > 0.0;
+    hipStream_t s = ctx->stream;
+    SG_HIP(hipSetDevice(ctx->device));
+    if (M && nsys > 1) precond_bind_systems(M, off);
+    auto t0 = std::chrono::steady_clock::now();
+
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    const int nz = (method && M) ? m : 0;
+    const int nvec = (m + 1) + nz + 4;
+    SolveWs ws;
+    ws.vecs.alloc(ld * nvec);
+    double* V = ws.vecs.p;
+    double* Z = nz ? V + ld * (m + 1) : nullptr;
+    double* r = V + ld * ((m + 1) + nz);
+    double* z = r + ld;
+    double* w = z + ld;
+    double* t = w + ld;
+    if (nsys > 1)  // stopped systems of a batch keep computing, on finite values
+        SG_HIP(hipMemsetAsync(V, 0, sizeof(double) * ld * nvec, s));
+
+    SegsHost seg_red, seg_gram;
+    build_segs(off, 4 * ctx->num_cu, kTile, seg_red, s);
+    build_segs(off, 4 * ctx->num_cu, kTileG, seg_gram, s);
+    const Segs sp_red = seg_red.dev(), sp_gram = seg_gram.dev();
+    const int pstride = std::max(seg_red.nblk, seg_gram.nblk);
+
+    const int hcap = prm->max_iter + m + 2;
+    const size_t per_sys = (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m + 2 * kRedCols + hcap +
+                           (size_t)kMaxCols * kMaxCols;
+    ws.small.alloc(per_sys * nsys + (size_t)kRedCols * pstride);
+    ws.st.alloc(nsys);
+    ws.mode.alloc(nsys);
+    SG_HIP(hipHostMalloc((void**)&ws.pinned, sizeof(int) * (m + 1), hipHostMallocDefault));
+    SG_HIP(hipHostMalloc((void**)&ws.h_mode, sizeof(int) * nsys, hipHostMallocDefault));
+    SG_HIP(hipHostMalloc((void**)&ws.h_st, sizeof(KState) * nsys, hipHostMallocDefault));
+    Work W;
+    W.H = ws.small.p;
+    W.g = W.H + (size_t)(m + 1) * m * nsys;
+    W.cs = W.g + (size_t)(m + 1) * nsys;
+    W.sn = W.cs + (size_t)m * nsys;
+    W.y = W.sn + (size_t)m * nsys;
+    W.h1 = W.y + (size_t)m * nsys;
+    W.h2 = W.h1 + (size_t)kRedCols * nsys;
+    W.hist = W.h2 + (size_t)kRedCols * nsys;
+    W.G = W.hist + (size_t)hcap * nsys;
+    W.part = W.G + (size_t)kMaxCols * kMaxCols * nsys;
+    W.st = ws.st.p;
+    W.mode = ws.mode.p;
+    W.m = m;
+    W.hcap = hcap;
+    W.pstride = pstride;
+    KState* st = ws.st.p;
+    int* hmode = ws.h_mode;
+    const Guard step_guard{&st->active, (int)(sizeof(KState) / sizeof(int)), nsys > 1};
+
+    const size_t bs_lds = sizeof(double) * ((size_t)m * (m + 1) / 2 + m + 1);
+    SG_HIP(hipFuncSetAttribute((const void*)k_backsolve,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bs_lds));
+    const double tol = rmode ? 0.0 : prm->tol_abs;
+    hipLaunchKernelGGL(k_init_state, dim3((nsys + 63) / 64), dim3(64), 0, s, st, nsys, method, m,
+                       M ? 1 : 0, tol, prm->rtol);
+    for (int q = 0; q < nsys; ++q) hmode[q] = kRun;
+    SG_HIP(hipMemcpyAsync(ws.mode.p, hmode, sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+    // ||b|| for the relative test and the report
+    hipLaunchKernelGGL(k_norms, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, d_b,
+                       (const double*)nullptr, W, 0);
+    hipLaunchKernelGGL(k_bnorm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red);
+    SG_HIP(hipGetLastError());
+
+    // ---- the pieces of one restart cycle ----
+    auto launch_restart = [&]() {
+        {
+            ProfScope ps(ctx, SG_K_UPDATE);
+            spmv_launch(A, -1.0, d_x, 1.0, d_b, r, Guard{}, s);  // r = b - A x
+        }
+        const double* u = r;
+        if (method == 0 && M) {
+            precond_apply_launch(M, r, z, Guard{});
+            u = z;
+        }
+        ProfScope ps(ctx, SG_K_UPDATE);
+        const bool has_r = rmode && u != r;
+        hipLaunchKernelGGL(k_norms, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, u,
+                           has_r ? (const double*)r : nullptr, W, 1);
+        hipLaunchKernelGGL(k_restart_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red,
+                           has_r ? 1 : 0);
+        hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, u, V, W);
+        SG_HIP(hipGetLastError());
+    };
+    auto launch_step = [&](int j) {
+        double* Vj = V + (size_t)j * ld;
+        if (method == 0) {
+            if (M) {
+                spmv_launch(A, 1.0, Vj, 0.0, nullptr, t, step_guard, s);
+                precond_apply_launch(M, t, w, step_guard);
+            } else {
+                spmv_launch(A, 1.0, Vj, 0.0, nullptr, w, step_guard, s);
+            }
+        } else {
+            double* Zj = M ? Z + (size_t)j * ld : Vj;
+            if (M) precond_apply_launch(M, Vj, Zj, step_guard);
+            spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, step_guard, s);
+        }
+        {
+            ProfScope ps(ctx, SG_K_ORTHO);
+            if (cgs2) {
+                hipLaunchKernelGGL(k_gram_dot, dim3(seg_gram.nblk), dim3(kTG), 0, s, sp_gram,
+                                   j + 1, V, ld, w, W, j);
+                hipLaunchKernelGGL(k_gram_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gram, j + 1,
+                                   j);
+                hipLaunchKernelGGL(k_gram_update, dim3(seg_gram.nblk), dim3(kTG), 0, s, sp_gram,
+                                   j + 1, V, ld, w, W);
+                hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gram, j);
+            } else {
+                for (int i = 0; i <= j + 1; ++i) {
+                    const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
+                    const double* Vi = i <= j ? V + (size_t)i * ld : nullptr;
+                    hipLaunchKernelGGL(k_mgs, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, Vp, Vi,
+                                       w, W, i, j);
+                    hipLaunchKernelGGL(k_mgs_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red, i, j,
+                                       i > j ? 1 : 0);
+                }
+            }
+            SG_HIP(hipGetLastError());
+        }
+        if (j + 1 < m) {
+            ProfScope ps(ctx, SG_K_UPDATE);
+            hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, w,
+                               V + (size_t)(j + 1) * ld, W);
+        }
+    };
+    auto launch_tail = [&]() {
+        ProfScope ps(ctx, SG_K_UPDATE);
+        hipLaunchKernelGGL(k_backsolve, dim3(nsys), dim3(64), bs_lds, s, W);
+        hipLaunchKernelGGL(k_update, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, d_x,
+                           (method && M) ? Z : V, ld, W);
+        SG_HIP(hipGetLastError());
+    };
+
+    if (use_graph) {
+        // kernel shapes are picked before capture (tuning times real launches)
+        spmv_ensure_tuned(A);
+        if (M) precond_ensure_tuned(M);
+        SG_HIP(hipStreamSynchronize(s));
+        ProfOff po(ctx);
+        CaptureGuard cg{s};
+        SG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        cg.on = true;
+        launch_restart();
+        for (int j = 0; j < m; ++j) launch_step(j);
+        launch_tail();
+        cg.on = false;
+        SG_HIP(hipStreamEndCapture(s, &ws.graph));
+        SG_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
+    }
+
+    std::vector<hipEvent_t> evs(m);
+    for (int j = 0; j < m; ++j) evs[j] = ctx->get_event();
+    struct EvBack {
+        sg_context* c;
+        std::vector<hipEvent_t>& e;
+        ~EvBack() {
+            for (auto x : e) c->event_pool.push_back(x);
+        }
+    } evback{ctx, evs};
+
+    std::vector<char> verify(nsys, 0), done(nsys, 0);
+    std::vector<int> cycles(nsys, 0);
+    KState* hst = ws.h_st;
+    std::memset((void*)hst, 0, sizeof(KState) * nsys);
+    for (int cycle = 0;; ++cycle) {
+        // per-system decisions before the cycle (gmres.cpp:57 bounds cycles)
+        bool any = false, any_run = false;
+        for (int q = 0; q < nsys; ++q) {
+            if (!done[q] && !verify[q] && cycle >= prm->max_iter) {
+                if (rmode && hst[q].est_conv) verify[q] = 1;
+                else done[q] = 1;
+            }
+            hmode[q] = done[q] ? kDone : (verify[q] ? kVerify : kRun);
+            any = any || !done[q];
+            any_run = any_run || hmode[q] == kRun;
+        }
+        if (!any) break;
+        SG_HIP(hipMemcpyAsync(ws.mode.p, hmode, sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+        if (use_graph) {
+            SG_HIP(hipGraphLaunch(ws.exec, s));
+        } else {
+            launch_restart();
+            for (int j = 0; j < m && any_run; ++j) {
+                if (nsys == 1 && j >= L) {
+                    SG_HIP(hipEventSynchronize(evs[j - L]));
+                    if (!ws.pinned[j - L]) break;
+                }
+                launch_step(j);
+                if (nsys == 1) {
+                    SG_HIP(hipMemcpyAsync(&ws.pinned[j], &st->active, sizeof(int),
+                                          hipMemcpyDeviceToHost, s));
+                    SG_HIP(hipEventRecord(evs[j], s));
+                }
+            }
+            if (any_run) launch_tail();
+        }
+        SG_HIP(hipMemcpyAsync(hst, st, sizeof(KState) * nsys, hipMemcpyDeviceToHost, s));
+        SG_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < nsys; ++q) {
+            if (hmode[q] == kDone) continue;
+            ++cycles[q];
+            if (hst[q].converged || verify[q]) {
+                done[q] = 1;
+            } else if (rmode && hst[q].est_conv) {
+                if (hst[q].total >= prm->max_iter) verify[q] = 1;  // next restart checks
+            } else if (hst[q].total >= prm->max_iter) {
+                done[q] = 1;  // gmres.cpp:146-147
+            }
+        }
+    }
+    SG_HIP(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+    ctx->collect_profiles();
+
+    const double elapsed = std::chrono::duration<double>(t1 - t0).count();
+    std::vector<double> bn(nsys);
+    bool need_final = false;
+    for (int q = 0; q < nsys; ++q) {
+        sg_solve_result& R = res[q];
+        const KState& h = hst[q];
+        R.iterations = h.total;
+        R.converged = h.converged;
+        R.cycles = cycles[q];
+        R.precond_applies = h.applies;
+        R.time_solve_s = elapsed;
+        bn[q] = h.bnorm;
+        std::vector<double> hv(h.total);
+        if (h.total)
+            SG_HIP(hipMemcpy(hv.data(), W.hist + (size_t)q * hcap, sizeof(double) * h.total,
+                             hipMemcpyDeviceToHost));
+        R.residual_norm = h.total ? hv.back() : 0.0;  // gmres.cpp:150-155
+        if (hist)
+            for (int i = 0; i < h.total && i < hist_cap; ++i)
+                hist[(size_t)q * hist_cap + i] = hv[i];
+        if (rmode && h.converged) R.rel_true_residual = h.rel_true;
+        else need_final = true;
+    }
+    // FP64 true residual at exit (reported; inside the solve in rtol mode)
+    if (need_final) {
+        for (int q = 0; q < nsys; ++q) hmode[q] = kRun;
+        SG_HIP(hipMemcpyAsync(ws.mode.p, hmode, sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+        spmv_launch(A, -1.0, d_x, 1.0, d_b, r, Guard{}, s);
+        hipLaunchKernelGGL(k_norms, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, r,
+                           (const double*)nullptr, W, 0);
+        hipLaunchKernelGGL(k_bnorm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red);
+        SG_HIP(hipMemcpyAsync(hst, st, sizeof(KState) * nsys, hipMemcpyDeviceToHost, s));
+        SG_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < nsys; ++q) {
+            if (rmode && res[q].converged) continue;
+            const double rn = hst[q].bnorm;  // k_bnorm_finish stored ||r|| in the bnorm slot
+            res[q].rel_true_residual = bn[q] > 0.0 ? rn / bn[q] : rn;
+        }
+    }
+}
 
 }  // namespace sg
 
@@ -502,210 +940,19 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
                               const sg_solve_params* prm, sg_solve_result* res, double* hist,
                               int hist_cap) {
     if (!A || !prm || !res || (A->nrows && (!d_b || !d_x))) return SG_E_ARG;
-    sg_context* ctx = A->ctx;
-    return run_guarded(ctx, [&] {
-        if (A->nrows != A->ncols) fail(SG_E_ARG, "GMRES needs a square matrix");
-        if (M && !precond_ready(M)) fail(SG_E_STATE, "preconditioner used before setup");
-        if (M && precond_n(M) != A->nrows) fail(SG_E_ARG, "preconditioner size mismatch");
-        const int n = A->nrows;
-        const int m = prm->restart;
-        const int method = prm->method == SG_FGMRES ? 1 : 0;
-        if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
-        if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
-        if (n < 1) fail(SG_E_ARG, "empty system");
-        const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
-        const int L = prm->lookahead > 0 ? prm->lookahead : 2;
-        hipStream_t s = ctx->stream;
-        SG_HIP(hipSetDevice(ctx->device));
-        auto t0 = std::chrono::steady_clock::now();
+    return run_guarded(A->ctx, [&] {
+        if (A->sys_off.size() != 2)
+            fail(SG_E_ARG, "matrix holds a batch of systems: use sg_solve_batch");
+        solve_impl(A, M, d_b, d_x, prm, res, hist, hist_cap, (prm->flags & SG_SOLVE_GRAPH) != 0);
+    });
+}
 
-        const size_t ld = ((size_t)n + 63) / 64 * 64;
-        const int nz = (method && M) ? m : 0;
-        const int nvec = (m + 1) + nz + 4;
-        SolveWs ws;
-        ws.vecs.alloc(ld * nvec);
-        double* V = ws.vecs.p;
-        double* Z = nz ? V + ld * (m + 1) : nullptr;
-        double* r = V + ld * ((m + 1) + nz);
-        double* z = r + ld;
-        double* w = z + ld;
-        double* t = w + ld;
-        const int hcap = prm->max_iter + m + 2;
-        size_t nsmall = (size_t)(m + 1) * m + (m + 1) + 3 * m + 2 * kRedCols + hcap +
-                        (size_t)kRedCols * kMaxBlocks + (size_t)kMaxCols * kMaxCols;
-        ws.small.alloc(nsmall);
-        ws.st.alloc(1);
-        SG_HIP(hipHostMalloc((void**)&ws.pinned, sizeof(int) * (m + 1), hipHostMallocDefault));
-        Work W;
-        W.H = ws.small.p;
-        W.g = W.H + (size_t)(m + 1) * m;
-        W.cs = W.g + (m + 1);
-        W.sn = W.cs + m;
-        W.y = W.sn + m;
-        W.h1 = W.y + m;
-        W.h2 = W.h1 + kRedCols;
-        W.hist = W.h2 + kRedCols;
-        W.part = W.hist + hcap;
-        W.G = W.part + (size_t)kRedCols * kMaxBlocks;
-        W.st = ws.st.p;
-        KState* st = ws.st.p;
-        const int* active = &st->active;
-
-        const Split sp_red = make_split(n, 4 * ctx->num_cu);
-        const Split sp_gram = make_split(n, 4 * ctx->num_cu, kTileG);
-        const int gew = ew_blocks(n);
-        const size_t bs_lds = sizeof(double) * ((size_t)m * (m + 1) / 2 + m + 1);
-
-        SG_HIP(hipFuncSetAttribute((const void*)k_backsolve,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bs_lds));
-        double tol = prm->rtol > 0.0 ? 0.0 : prm->tol_abs;
-        hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, st, method, m, tol, prm->rtol);
-        // ||b|| for the relative test and the report
-        hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, d_b,
-                           (const double*)nullptr, W);
-        hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk);
-        SG_HIP(hipGetLastError());
-
-        std::vector<hipEvent_t> evs(m);
-        for (int j = 0; j < m; ++j) evs[j] = ctx->get_event();
-        struct EvBack {
-            sg_context* c;
-            std::vector<hipEvent_t>& e;
-            ~EvBack() {
-                for (auto x : e) c->event_pool.push_back(x);
-            }
-        } evback{ctx, evs};
-
-        const bool rmode = prm->rtol > 0.0;
-        KState hst{};
-        int cycles = 0, applies = 0;
-        bool verify_only = false;
-        for (int cycle = 0;; ++cycle) {
-            if (!verify_only && cycle >= prm->max_iter) {
-                if (rmode && hst.est_conv) verify_only = true;
-                else break;
-            }
-            // r = b - A x
-            {
-                ProfScope ps(ctx, SG_K_UPDATE);
-                spmv_launch(A, -1.0, d_x, 1.0, d_b, r, nullptr, s);
-            }
-            const double* u = r;
-            if (method == 0 && M) {
-                precond_apply_launch(M, r, z, nullptr);
-                ++applies;
-                u = z;
-            }
-            {
-                ProfScope ps(ctx, SG_K_UPDATE);
-                const bool has_r = rmode && u != r;
-                hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, u,
-                                   has_r ? (const double*)r : nullptr, W);
-                hipLaunchKernelGGL(k_restart_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk,
-                                   has_r ? 1 : 0);
-                if (!verify_only)
-                    hipLaunchKernelGGL(k_scale_basis, dim3(gew), dim3(kT), 0, s, n, u, V, st);
-                SG_HIP(hipGetLastError());
-            }
-            for (int j = 0; j < m && !verify_only; ++j) {
-                if (j >= L) {
-                    SG_HIP(hipEventSynchronize(evs[j - L]));
-                    if (!ws.pinned[j - L]) break;
-                }
-                double* Vj = V + (size_t)j * ld;
-                if (method == 0) {
-                    if (M) {
-                        spmv_launch(A, 1.0, Vj, 0.0, nullptr, t, active, s);
-                        precond_apply_launch(M, t, w, active);
-                        ++applies;
-                    } else {
-                        spmv_launch(A, 1.0, Vj, 0.0, nullptr, w, active, s);
-                    }
-                } else {
-                    double* Zj = M ? Z + (size_t)j * ld : Vj;
-                    if (M) {
-                        precond_apply_launch(M, Vj, Zj, active);
-                        ++applies;
-                    }
-                    spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, active, s);
-                }
-                {
-                    ProfScope ps(ctx, SG_K_ORTHO);
-                    if (cgs2) {
-                        hipLaunchKernelGGL(k_gram_dot, dim3(sp_gram.nblk), dim3(kTG), 0, s, sp_gram,
-                                           j + 1, V, ld, w, W, j);
-                        hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kT), 0, s, W, j + 1, j,
-                                           sp_gram.nblk);
-                        hipLaunchKernelGGL(k_gram_update, dim3(sp_gram.nblk), dim3(kTG), 0, s,
-                                           sp_gram, j + 1, V, ld, w, W);
-                        hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(kT), 0, s, W, j,
-                                           sp_gram.nblk);
-                    } else {
-                        for (int i = 0; i <= j + 1; ++i) {
-                            const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
-                            const double* Vi = i <= j ? V + (size_t)i * ld : nullptr;
-                            hipLaunchKernelGGL(k_mgs, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red,
-                                               Vp, Vi, w, W, i, j);
-                            hipLaunchKernelGGL(k_mgs_finish, dim3(1), dim3(kT), 0, s, W, i, j,
-                                               sp_red.nblk, i > j ? 1 : 0);
-                        }
-                    }
-                    SG_HIP(hipGetLastError());
-                }
-                if (j + 1 < m) {
-                    ProfScope ps(ctx, SG_K_UPDATE);
-                    hipLaunchKernelGGL(k_scale_basis, dim3(gew), dim3(kT), 0, s, n, w,
-                                       V + (size_t)(j + 1) * ld, st);
-                }
-                SG_HIP(hipMemcpyAsync(&ws.pinned[j], active, sizeof(int), hipMemcpyDeviceToHost, s));
-                SG_HIP(hipEventRecord(evs[j], s));
-            }
-            if (!verify_only) {
-                ProfScope ps(ctx, SG_K_UPDATE);
-                hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), bs_lds, s, W);
-                hipLaunchKernelGGL(k_update, dim3(gew), dim3(kT), 0, s, n, d_x,
-                                   (method && M) ? Z : V, ld, W);
-                SG_HIP(hipGetLastError());
-            }
-            SG_HIP(hipMemcpyAsync(&hst, st, sizeof(KState), hipMemcpyDeviceToHost, s));
-            SG_HIP(hipStreamSynchronize(s));
-            ++cycles;
-            if (hst.converged || verify_only) break;
-            if (rmode && hst.est_conv) {
-                if (hst.total >= prm->max_iter) verify_only = true;
-                continue;  // the next restart checks the true residual
-            }
-            if (hst.total >= prm->max_iter) break;  // gmres.cpp:146-147
-        }
-        SG_HIP(hipStreamSynchronize(s));
-        auto t1 = std::chrono::steady_clock::now();
-        ctx->collect_profiles();
-
-        res->iterations = hst.total;
-        res->converged = hst.converged;
-        res->cycles = cycles;
-        res->precond_applies = applies;
-        res->time_solve_s = std::chrono::duration<double>(t1 - t0).count();
-        std::vector<double> h(hst.total);
-        if (hst.total)
-            SG_HIP(hipMemcpy(h.data(), W.hist, sizeof(double) * hst.total, hipMemcpyDeviceToHost));
-        res->residual_norm = hst.total ? h.back() : 0.0;  // gmres.cpp:150-155
-        if (hist)
-            for (int i = 0; i < hst.total && i < hist_cap; ++i) hist[i] = h[i];
-        // FP64 true residual at exit (reported; inside the solve in rtol mode)
-        if (rmode && hst.converged) {
-            res->rel_true_residual = hst.rel_true;
-        } else {
-            const double bn = hst.bnorm;
-            spmv_launch(A, -1.0, d_x, 1.0, d_b, r, nullptr, s);
-            hipLaunchKernelGGL(k_norms, dim3(sp_red.nblk), dim3(kT), 0, s, sp_red, r,
-                               (const double*)nullptr, W);
-            hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kT), 0, s, W, sp_red.nblk);
-            KState fin{};
-            SG_HIP(hipMemcpyAsync(&fin, st, sizeof(KState), hipMemcpyDeviceToHost, s));
-            SG_HIP(hipStreamSynchronize(s));
-            const double rn = fin.bnorm;  // k_bnorm_finish stored ||r|| in the bnorm slot
-            res->rel_true_residual = bn > 0.0 ? rn / bn : rn;
-        }
+extern "C" sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                                    const sg_solve_params* prm, sg_solve_result* res,
+                                    double* hist, int hist_cap) {
+    if (!A || !prm || !res || (A->nrows && (!d_b || !d_x))) return SG_E_ARG;
+    return run_guarded(A->ctx, [&] {
+        solve_impl(A, M, d_b, d_x, prm, res, hist, hist_cap,
+                   (prm->flags & SG_SOLVE_NO_GRAPH) == 0);
     });
 }

@@ -125,6 +125,9 @@ struct sg_precond {
     sg::DevBuf<int> d_piv;
     int restrict_split = 1;
     sg::DevBuf<int> d_lu_status;
+    // batch binding: subdomain -> system (-1: spans systems) for sys_off
+    std::vector<int> bound_sys;
+    sg::DevBuf<int> d_sub_sys;
 };
 
 namespace sg {
@@ -213,8 +216,8 @@ static void run_parallel(int count, const std::function<void(int)>& f) {
 // ---------------------------------------------------------------------------
 
 __global__ void k_jacobi(int n, const double* __restrict__ inv, const double* __restrict__ x,
-                         double* __restrict__ y, const int* __restrict__ guard) {
-    if (guard && *guard == 0) return;
+                         double* __restrict__ y, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         y[i] = inv[i] * x[i];  // src/precond/jacobi.h:22-25
 }
@@ -383,10 +386,10 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                 const int* __restrict__ own, const int* __restrict__ own_gid,
                 const double* __restrict__ x,
                 double* __restrict__ y, T* __restrict__ zglob, T* __restrict__ zout,
-                const T* __restrict__ cz, const int* __restrict__ guard, int lds_z_off) {
-    if (guard && *guard == 0) return;
+                const T* __restrict__ cz, DevGuard guard, int lds_z_off) {
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
+    if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
     // LDS: this subdomain's level tables, then (ZLDS) the solution vector
     Level* lf = reinterpret_cast<Level*>(lds_raw);
@@ -429,8 +432,8 @@ template <typename T>
 __global__ void k_asm_gather(int n, const int* __restrict__ ptr, const int* __restrict__ pos,
                              const T* __restrict__ zbuf, const int* __restrict__ part,
                              const T* __restrict__ cz, double* __restrict__ y,
-                             const int* __restrict__ guard) {
-    if (guard && *guard == 0) return;
+                             DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
     for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         double acc = 0.0;
         for (int k = ptr[g]; k < ptr[g + 1]; ++k) acc += (double)zbuf[pos[k]];
@@ -451,9 +454,14 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __restrict__ own_gid,
                   const T* __restrict__ c, const double* __restrict__ x, T* __restrict__ part,
-                  const int* __restrict__ guard) {
-    if (guard && *guard == 0) return;
+                  DevGuard guard) {
     const int s = blockIdx.x / split, j = blockIdx.x % split;
+    if (guard_stop(guard, s)) {
+        // a stopped system of a batch contributes an exact zero to the
+        // (block-diagonal) coarse solve; otherwise the whole launch is skipped
+        if (guard.map && threadIdx.x == 0) part[blockIdx.x] = 0;
+        return;
+    }
     const SubInfo si = subs[s];
     const int per = (si.nown + split - 1) / split;
     const int a = j * per, b = min(si.nown, a + per);
@@ -476,8 +484,8 @@ k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __rest
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_coarse_inv(int nc, int split, const T* __restrict__ inv, const T* __restrict__ r0part,
-             const T* __restrict__ c, T* __restrict__ cz, const int* __restrict__ guard) {
-    if (guard && *guard == 0) return;
+             const T* __restrict__ c, T* __restrict__ cz, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
     extern __shared__ unsigned char lds_raw[];
     T* xs = reinterpret_cast<T*>(lds_raw);
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -506,8 +514,8 @@ __global__ void __launch_bounds__(1024)
 k_coarse_solve(int nc, int mode, int split, const T* __restrict__ LUt,
                const int* __restrict__ piv, const T* __restrict__ invT,
                const T* __restrict__ r0part, const T* __restrict__ c, T* __restrict__ cz,
-               const int* __restrict__ guard) {
-    if (guard && *guard == 0) return;
+               DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
     extern __shared__ unsigned char lds_raw[];
     T* xs = reinterpret_cast<T*>(lds_raw);
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -957,6 +965,8 @@ static void setup(sg_precond* M, const sg_csr* A) {
     SG_HIP(hipSetDevice(ctx->device));
     if (A->nrows != A->ncols) fail(SG_E_ARG, "preconditioners need a square matrix");
     M->n = A->nrows;
+    M->bound_sys.clear();
+    M->sweep_tuned = false;
     switch (M->kind) {
         case SG_PC_NONE:
             break;
@@ -1006,7 +1016,7 @@ static void setup(sg_precond* M, const sg_csr* A) {
 }
 
 template <typename T, bool ZLDS, int MODE>
-static void launch_sweep(sg_precond* M, const double* x, double* y, const int* guard,
+static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;
 #define SG_SWEEP(NT)                                                                              \
@@ -1024,7 +1034,7 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const int* g
 }
 
 template <typename T>
-static void launch_any_sweep(sg_precond* M, const double* x, double* y, const int* guard,
+static void launch_any_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
                              const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     const bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
     if (additive) {
@@ -1053,10 +1063,11 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
     int bestv = 1;
     for (int v = 0; v < 3; ++v) {
         M->sweep_variant = v;
-        launch_any_sweep<T>(M, xin.p, yout.p, nullptr, fv, bv, bd, zg, zo, nullptr);
+        const DevGuard none{nullptr, nullptr, 0};
+        launch_any_sweep<T>(M, xin.p, yout.p, none, fv, bv, bd, zg, zo, nullptr);
         SG_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 3; ++r)
-            launch_any_sweep<T>(M, xin.p, yout.p, nullptr, fv, bv, bd, zg, zo, nullptr);
+            launch_any_sweep<T>(M, xin.p, yout.p, none, fv, bv, bd, zg, zo, nullptr);
         SG_HIP(hipEventRecord(e1, s));
         SG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1073,37 +1084,44 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
 }
 
 template <typename T>
-static void apply_batch(sg_precond* M, const double* x, double* y, const int* guard,
-                        const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
-    hipStream_t st = M->ctx->stream;
-    bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
+static void tune_sweep(sg_precond* M, const T* fv, const T* bv, const T* bd, T* zg, T* zo) {
     if (M->ctx->autotune == 1 && !M->sweep_tuned) sweep_autotune<T>(M, fv, bv, bd, zg, zo);
     if (M->ctx->autotune >= 2) M->sweep_variant = M->ctx->autotune - 2;
     else if (M->ctx->autotune == 0) M->sweep_variant = 1;
+}
+
+template <typename T>
+static void apply_batch(sg_precond* M, const double* x, double* y, const Guard& guard,
+                        const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+    hipStream_t st = M->ctx->stream;
+    bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
+    tune_sweep<T>(M, fv, bv, bd, zg, zo);
     {
         ProfScope ps(M->ctx, SG_K_TRSV);
-        launch_any_sweep<T>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+        launch_any_sweep<T>(M, x, y, dev_guard(guard, M->d_sub_sys.p), fv, bv, bd, zg, zo, cz);
     }
     if (additive) {
         hipLaunchKernelGGL(k_asm_gather<T>, dim3(grid_for(M->n)), dim3(256), 0, st, M->n,
-                           M->cov_ptr.p, M->cov_pos.p, zo, M->d_part.p, cz, y, guard);
+                           M->cov_ptr.p, M->cov_pos.p, zo, M->d_part.p, cz, y,
+                           dev_guard(guard, nullptr));
         SG_HIP(hipGetLastError());
     }
 }
 
 template <typename T>
-static const T* coarse_launch(sg_precond* M, const double* x, const int* guard, const T* LUt,
+static const T* coarse_launch(sg_precond* M, const double* x, const Guard& guard, const T* LUt,
                               const T* inv, const T* c, T* r0, T* cz) {
     hipStream_t st = M->ctx->stream;
     int nc = M->nc, split = M->restrict_split;
     hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(nc * split), dim3(256), 0, st, split,
-                       M->d_subs.p, M->d_own_gid.p, c, x, r0, guard);
+                       M->d_subs.p, M->d_own_gid.p, c, x, r0, dev_guard(guard, M->d_sub_sys.p));
+    const DevGuard g0 = dev_guard(guard, nullptr);
     if (M->opts.coarse_solve == 1)
         hipLaunchKernelGGL(k_coarse_inv<T>, dim3((nc + 3) / 4), dim3(256), (size_t)nc * sizeof(T),
-                           st, nc, split, inv, r0, c, cz, guard);
+                           st, nc, split, inv, r0, c, cz, g0);
     else
         hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), (size_t)nc * sizeof(T), st, nc,
-                           0, split, LUt, M->d_piv.p, inv, r0, c, cz, guard);
+                           0, split, LUt, M->d_piv.p, inv, r0, c, cz, g0);
     SG_HIP(hipGetLastError());
     return cz;
 }
@@ -1114,7 +1132,7 @@ using namespace sg;
 
 namespace sg {
 
-void precond_apply_launch(sg_precond* M, const double* x, double* y, const int* guard) {
+void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard) {
     sg_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     ProfScope ps(ctx, SG_K_PRECOND);
@@ -1124,17 +1142,17 @@ void precond_apply_launch(sg_precond* M, const double* x, double* y, const int* 
             return;
         case SG_PC_JACOBI:
             hipLaunchKernelGGL(k_jacobi, dim3(grid_for(M->n)), dim3(256), 0, st, M->n,
-                               M->inv_diag.p, x, y, guard);
+                               M->inv_diag.p, x, y, dev_guard(guard, nullptr));
             SG_HIP(hipGetLastError());
             return;
         default:
             break;
     }
     if (M->fn) {
-        // host callback (guard honoured on the host)
-        if (guard) {
+        // host callback (guard honoured on the host; never batched)
+        if (guard.flag) {
             int g = 0;
-            SG_HIP(hipMemcpyAsync(&g, guard, sizeof(int), hipMemcpyDeviceToHost, st));
+            SG_HIP(hipMemcpyAsync(&g, guard.flag, sizeof(int), hipMemcpyDeviceToHost, st));
             SG_HIP(hipStreamSynchronize(st));
             if (!g) return;
         }
@@ -1165,6 +1183,35 @@ void precond_apply_launch(sg_precond* M, const double* x, double* y, const int* 
         apply_batch<double>(M, x, y, guard, M->f_v64.p, M->b_v64.p, M->b_d64.p, M->z64.p, M->zout64.p, cz);
     }
 }
+
+void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off) {
+    if (M->bound_sys == sys_off) return;
+    std::vector<int> map(std::max<size_t>(M->dofs.size(), 1), -1);
+    for (size_t q = 0; q < M->dofs.size(); ++q) {
+        const auto& d = M->dofs[q];
+        if (d.empty()) continue;
+        // dofs are ascending: the subdomain lies in one system iff its first
+        // and last DOF do
+        const int a = (int)(std::upper_bound(sys_off.begin(), sys_off.end(), d.front()) -
+                            sys_off.begin()) - 1;
+        const int b = (int)(std::upper_bound(sys_off.begin(), sys_off.end(), d.back()) -
+                            sys_off.begin()) - 1;
+        map[q] = a == b ? a : -1;
+    }
+    M->d_sub_sys.upload(map, M->ctx->stream);
+    SG_HIP(hipStreamSynchronize(M->ctx->stream));
+    M->bound_sys = sys_off;
+}
+
+void precond_ensure_tuned(sg_precond* M) {
+    if (M->fn || M->kind == SG_PC_NONE || M->kind == SG_PC_JACOBI || !M->ready) return;
+    if (M->opts.precision == SG_FP32)
+        tune_sweep<float>(M, M->f_v32.p, M->b_v32.p, M->b_d32.p, M->z32.p, M->zout32.p);
+    else
+        tune_sweep<double>(M, M->f_v64.p, M->b_v64.p, M->b_d64.p, M->z64.p, M->zout64.p);
+}
+
+bool precond_is_callback(const sg_precond* M) { return M->fn != nullptr; }
 
 int precond_n(const sg_precond* M) { return M->n; }
 sg_context* precond_ctx(const sg_precond* M) { return M->ctx; }
@@ -1225,7 +1272,7 @@ sg_status sg_precond_apply(sg_precond* M, const double* d_x, double* d_y) {
         return SG_E_STATE;
     }
     if (M->n && (!d_x || !d_y)) return SG_E_ARG;
-    return run_guarded(M->ctx, [&] { precond_apply_launch(M, d_x, d_y, nullptr); });
+    return run_guarded(M->ctx, [&] { precond_apply_launch(M, d_x, d_y, Guard{}); });
 }
 
 sg_status sg_precond_destroy(sg_precond* M) {

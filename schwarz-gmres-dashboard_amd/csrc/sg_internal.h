@@ -55,6 +55,36 @@ struct DevBuf {
     }
 };
 
+// Step guard handed from the solver to the kernels of one Arnoldi step.  A
+// kernel (or the part of it serving one system of a batch) returns at once
+// when its system's flag is 0.  flag == nullptr: always run.  The flags of
+// consecutive systems are `stride` ints apart.  When `batched`, each consumer
+// looks the system up through its own unit -> system map (SpMV row block,
+// subdomain); a consumer without such a map always runs.
+struct Guard {
+    const int* flag = nullptr;
+    int stride = 0;
+    bool batched = false;
+};
+
+struct DevGuard {
+    const int* flag;
+    const int* map;  // unit -> system; entries < 0 always run
+    int stride;
+};
+
+__device__ inline bool guard_stop(const DevGuard& g, int unit) {
+    if (!g.flag) return false;
+    if (!g.map) return g.flag[0] == 0;
+    const int s = g.map[unit];
+    return s >= 0 && g.flag[s * g.stride] == 0;
+}
+
+inline DevGuard dev_guard(const Guard& g, const int* map) {
+    if (g.batched) return DevGuard{map ? g.flag : nullptr, map, g.stride};
+    return DevGuard{g.flag, nullptr, 0};
+}
+
 struct Prof {
     long long launches = 0;
     double ms = 0.0;
@@ -93,6 +123,9 @@ struct sg_csr {
     // block b covers rows [blk[b], blk[b+1])
     int nblk_v[3] = {0, 0, 0};
     sg::DevBuf<int> blk_v[3];
+    sg::DevBuf<int> blk_sys_v[3];  // row block -> system (blocks never straddle systems)
+    // independent systems of a block-diagonal batch: rows [sys_off[i], sys_off[i+1])
+    std::vector<int> sys_off;
     int variant = 0;     // selected block shape
     bool tuned = false;
 };
@@ -109,17 +142,24 @@ struct ProfScope {
 };
 
 // ---- SpMV (spmv.hip) ---------------------------------------------------------
-// y = alpha*A*x + beta*y; guard (device int, may be null): skip when *guard == 0.
+// y = alpha*A*x + beta*y, skipping the rows of stopped systems (see Guard).
 // y0 may be null (then y0 = y); beta == 0 never reads y0.
 void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,
-                 double* y, const int* guard, hipStream_t s);
+                 double* y, const Guard& guard, hipStream_t s);
 void spmv_prepare(sg_csr* A);
+// Runs the block-shape autotuner now if it is due (never inside graph capture).
+void spmv_ensure_tuned(sg_csr* A);
 
 // ---- reductions / BLAS-1 (blas.hip) -------------------------------------------
 // result[0] = dot(a, b) on the device (deterministic tree), stream-ordered.
 void dot_launch(sg_context* ctx, int n, const double* a, const double* b, double* result);
 
 // ---- preconditioner apply (precond.hip) ----------------------------------------
-void precond_apply_launch(sg_precond* M, const double* x, double* y, const int* guard);
+void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard);
+// Maps each subdomain to the system of a block-diagonal batch that holds it
+// (-1 if it spans several) for per-system guards.
+void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off);
+void precond_ensure_tuned(sg_precond* M);
+bool precond_is_callback(const sg_precond* M);
 
 }  // namespace sg

@@ -26,12 +26,10 @@ __global__ void __launch_bounds__(NT)
 k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
               const double* __restrict__ v, const int* __restrict__ blk, int nblk, int per_xcd,
               double alpha, const double* __restrict__ x, double beta,
-              const double* __restrict__ y0, double* __restrict__ y,
-              const int* __restrict__ guard) {
+              const double* __restrict__ y0, double* __restrict__ y, DevGuard guard) {
     constexpr int kCap = NT * kNnzPerThread;
-    if (guard && *guard == 0) return;
     const int b = (blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
-    if (b >= nblk) return;
+    if (b >= nblk || guard_stop(guard, b)) return;
     const int r0 = blk[b], r1 = blk[b + 1];
     const int k0 = rp[r0], k1 = rp[r1];
     const int t = threadIdx.x;
@@ -65,33 +63,42 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
 }
 
 // Row blocks of at most nt rows and nt*kNnzPerThread nonzeros (a longer row
-// gets a block of its own).
-static std::vector<int> row_blocks(const std::vector<int>& rp, int nrows, int nt) {
+// gets a block of its own); no block crosses a system boundary of a batch.
+static void row_blocks(const std::vector<int>& rp, const std::vector<int>& sys_off, int nt,
+                       std::vector<int>& blk, std::vector<int>& blk_sys) {
     const int cap = nt * kNnzPerThread;
-    std::vector<int> blk;
-    blk.push_back(0);
-    int r = 0;
-    while (r < nrows) {
-        int start = r;
-        while (r < nrows && r - start < nt && rp[r + 1] - rp[start] <= cap) ++r;
-        if (r == start) ++r;
-        blk.push_back(r);
+    blk.assign(1, 0);
+    blk_sys.clear();
+    for (size_t q = 0; q + 1 < sys_off.size(); ++q) {
+        int r = sys_off[q];
+        const int rend = sys_off[q + 1];
+        while (r < rend) {
+            int start = r;
+            while (r < rend && r - start < nt && rp[r + 1] - rp[start] <= cap) ++r;
+            if (r == start) ++r;
+            blk.push_back(r);
+            blk_sys.push_back((int)q);
+        }
     }
-    return blk;
 }
 
 void spmv_prepare(sg_csr* A) {
+    if (A->sys_off.empty()) A->sys_off = {0, A->nrows};
+    std::vector<int> blk, bsys;
     for (int v = 0; v < 3; ++v) {
-        std::vector<int> blk = row_blocks(A->host.rp, A->nrows, kSpmvVariants[v]);
+        row_blocks(A->host.rp, A->sys_off, kSpmvVariants[v], blk, bsys);
         A->nblk_v[v] = (int)blk.size() - 1;
         A->blk_v[v].upload(blk, A->ctx->stream);
+        A->blk_sys_v[v].upload(bsys, A->ctx->stream);
+        SG_HIP(hipStreamSynchronize(A->ctx->stream));
     }
     A->variant = 0;
 }
 
 static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* x, double beta,
-                                const double* y0, double* y, const int* guard, hipStream_t s) {
+                                const double* y0, double* y, const Guard& g, hipStream_t s) {
     const int nblk = A->nblk_v[var];
+    const DevGuard guard = dev_guard(g, A->blk_sys_v[var].p);
     const int per = (nblk + kXcd - 1) / kXcd;
     const int* blk = A->blk_v[var].p;
     switch (kSpmvVariants[var]) {
@@ -126,9 +133,9 @@ void spmv_autotune(sg_csr* A) {
     float best = 0.f;
     int bestv = 0;
     for (int v = 0; v < 3; ++v) {
-        spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, nullptr, s);  // warm
+        spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, Guard{}, s);  // warm
         SG_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; ++r) spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, nullptr, s);
+        for (int r = 0; r < 3; ++r) spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, Guard{}, s);
         SG_HIP(hipEventRecord(e1, s));
         SG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -144,12 +151,17 @@ void spmv_autotune(sg_csr* A) {
     A->tuned = true;
 }
 
-void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,
-                 double* y, const int* guard, hipStream_t s) {
+void spmv_ensure_tuned(sg_csr* A) {
     if (A->nrows == 0) return;
     if (A->ctx->autotune == 1 && !A->tuned) spmv_autotune(A);
     if (A->ctx->autotune >= 2) A->variant = A->ctx->autotune - 2;
     else if (A->ctx->autotune == 0) A->variant = 0;
+}
+
+void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,
+                 double* y, const Guard& guard, hipStream_t s) {
+    if (A->nrows == 0) return;
+    spmv_ensure_tuned(A);
     ProfScope ps(A->ctx, SG_K_SPMV);
     spmv_launch_variant(A, A->variant, alpha, x, beta, y0 ? y0 : y, y, guard, s);
 }
@@ -160,8 +172,9 @@ using namespace sg;
 
 extern "C" {
 
-sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const int* row_ptr,
-                        const int* col_idx, const double* vals, sg_csr** out) {
+static sg_status csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const int* row_ptr,
+                            const int* col_idx, const double* vals, const std::vector<int>& sys_off,
+                            sg_csr** out) {
     if (!ctx || !out || nrows < 0 || ncols < 0 || nnz < 0 || !row_ptr ||
         (nnz && (!col_idx || !vals)))
         return SG_E_ARG;
@@ -173,6 +186,17 @@ sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const in
             if (row_ptr[i + 1] < row_ptr[i]) fail(SG_E_ARG, "row_ptr not monotone");
         for (int k = 0; k < nnz; ++k)
             if (col_idx[k] < 0 || col_idx[k] >= ncols) fail(SG_E_ARG, "column index out of range");
+        if (!sys_off.empty()) {
+            for (size_t q = 0; q + 1 < sys_off.size(); ++q) {
+                const int a = sys_off[q], b = sys_off[q + 1];
+                if (b <= a) fail(SG_E_ARG, "batch: every system needs at least one row");
+                for (int k = row_ptr[a]; k < row_ptr[b]; ++k)
+                    if (col_idx[k] < a || col_idx[k] >= b)
+                        fail(SG_E_ARG, "batch: matrix is not block diagonal over the systems");
+            }
+            if (sys_off.front() != 0 || sys_off.back() != nrows || nrows != ncols)
+                fail(SG_E_ARG, "batch: system offsets must cover all rows of a square matrix");
+        }
         SG_HIP(hipSetDevice(ctx->device));
         A->ctx = ctx;
         A->nrows = nrows;
@@ -186,6 +210,7 @@ sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const in
         A->rp.upload(A->host.rp, ctx->stream);
         A->ci.upload(A->host.ci, ctx->stream);
         A->v.upload(A->host.v, ctx->stream);
+        A->sys_off = sys_off;
         spmv_prepare(A);
         SG_HIP(hipStreamSynchronize(ctx->stream));
     });
@@ -194,6 +219,26 @@ sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const in
         return st;
     }
     *out = A;
+    return SG_OK;
+}
+
+sg_status sg_csr_create(sg_context* ctx, int nrows, int ncols, int nnz, const int* row_ptr,
+                        const int* col_idx, const double* vals, sg_csr** out) {
+    return csr_create(ctx, nrows, ncols, nnz, row_ptr, col_idx, vals, {}, out);
+}
+
+sg_status sg_csr_create_batch(sg_context* ctx, int nsys, const int* sys_rows, int nnz,
+                              const int* row_ptr, const int* col_idx, const double* vals,
+                              sg_csr** out) {
+    if (nsys < 1 || !sys_rows) return SG_E_ARG;
+    std::vector<int> off(sys_rows, sys_rows + nsys + 1);
+    const int n = off.back();
+    return csr_create(ctx, n, n, nnz, row_ptr, col_idx, vals, off, out);
+}
+
+sg_status sg_csr_systems(const sg_csr* A, int* nsys) {
+    if (!A || !nsys) return SG_E_ARG;
+    *nsys = (int)A->sys_off.size() - 1;
     return SG_OK;
 }
 
@@ -215,7 +260,7 @@ sg_status sg_csr_info(const sg_csr* A, int* nrows, int* ncols, int* nnz) {
 sg_status sg_spmv(sg_csr* A, double alpha, const double* d_x, double beta, double* d_y) {
     if (!A || (A->nrows && (!d_x || !d_y))) return SG_E_ARG;
     return run_guarded(A->ctx, [&] {
-        spmv_launch(A, alpha, d_x, beta, nullptr, d_y, nullptr, A->ctx->stream);
+        spmv_launch(A, alpha, d_x, beta, nullptr, d_y, Guard{}, A->ctx->stream);
     });
 }
 

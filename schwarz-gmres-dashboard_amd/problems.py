@@ -160,3 +160,26 @@ def grf(n_side, sigma, corr_len, seed):
     w = normal(seed, M * M).reshape(M, M)
     g = np.fft.ifft2(np.sqrt(lam) * np.fft.fft2(w)).real
     return g[:n_side, :n_side]
+
+
+def batch_nparts(n):
+    """Subdomains of one batched system: round(4 + 12 (n - 64^2) / (512^2 - 64^2))
+    clamped to [4, 16]."""
+    return int(min(16, max(4, round(4 + 12 * (n - 64 ** 2) / (512 ** 2 - 64 ** 2)))))
+
+
+def batch_systems(nsys=64, base_seed=4000, side_range=(64, 512), sigma=1.0, corr_len=0.1):
+    """Independent 2D diffusion systems of the batched workload.
+
+    System i (seed s = base_seed + i): grid side = lo + floor(u * (hi - lo + 1))
+    with u the first uniform of SplitMix64(s ^ 0x5EED); kappa = exp(g), g =
+    grf(side, sigma, corr_len, s); b ~ N(0, 1) from SplitMix64(s + 10^6).
+    Returns a list of (rp, ci, v, b)."""
+    lo, hi = side_range
+    out = []
+    for i in range(nsys):
+        s = base_seed + i
+        side = lo + int(uniform01(s ^ 0x5EED, 1)[0] * (hi - lo + 1))
+        rp, ci, v = diffusion2d(np.exp(grf(side, sigma, corr_len, s)))
+        out.append((rp, ci, v, normal(s + 1_000_000, side * side)))
+    return out
