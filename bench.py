@@ -48,6 +48,8 @@ CONFIGS = {
                weighted=False, max_iter=2000, base_max_iter=4000),
     "c1": dict(workload="poisson2d_256^2", restart=30, nparts=1, overlap=0, ufactor=0,
                weighted=False, max_iter=2000, base_max_iter=2000),
+    "c5": dict(workload="diffusion2d_grf_1024^2", restart=30, nparts=256, overlap=2, ufactor=0,
+               weighted=True, max_iter=10000, base_max_iter=20000),
     "c4": dict(workload="batch_64x_diffusion2d_grf", restart=30, nparts=0, overlap=2, ufactor=0,
                weighted=False, max_iter=2000, base_max_iter=4000),
 }
@@ -58,6 +60,10 @@ def make_problem(P, cfg):
         k = P.block_jump_kappa(160, 16, 3)
         rp, ci, v = P.diffusion3d(k)
         b = P.normal(33, len(rp) - 1)
+        xs = None
+    elif cfg == "c5":
+        rp, ci, v = P.diffusion2d(np.exp(P.grf(1024, 1.5, 0.05, 5)))
+        b = P.normal(55, len(rp) - 1)
         xs = None
     elif cfg == "c2":
         rp, ci, v = P.poisson3d(128, 128, 128)
@@ -115,6 +121,29 @@ def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, gpu_iters):
                       f"FP64 two-level RAS on the same matrix and partition "
                       f"({t_iter * 1e3:.1f} ms/iteration, setup {t_setup:.1f} s not included), "
                       f"scaled to the {gpu_iters} iterations of the GPU solve"}
+
+
+def probe(sg, ctx, A, M, db, dx, k, info, n, nnz):
+    """K preconditioner applies and K SpMVs with per-launch HIP-event timing."""
+    dy = sg.DeviceArray(ctx, n)
+    M.apply(db, dy)  # first apply runs the shape autotuner
+    A.spmv(db, dy)
+    ctx.sync()
+    ctx.profiling(True)
+    ctx.reset_stats()
+    for _ in range(k):
+        M.apply(db, dy)
+        A.spmv(dy, dx)
+    ctx.sync()
+    fams = {f: ctx.kernel_stats(f) for f in ("spmv", "trsv", "precond")}
+    ctx.profiling(False)
+    models = traffic_models(n, nnz, info, 0, True)
+    out = {f: {"launches": fams[f][0], "avg_us": round(1e3 * fams[f][1] / max(1, fams[f][0]), 2)}
+           for f in fams}
+    for f in ("spmv", "trsv"):
+        out[f]["GB_s"] = round(models[f] / (out[f]["avg_us"] * 1e-6) / 1e9, 1)
+    out["precond_info"] = info
+    print(json.dumps(out))
 
 
 def copy_probe():
@@ -216,6 +245,12 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="kernel-shape mode (sg_context_set_autotune): 1 tune, 0 defaults, "
+                         "2..6 force a shape")
+    ap.add_argument("--probe", type=int, default=0,
+                    help="profiling aid: after setup run this many preconditioner applies and "
+                         "SpMVs on the workload, print their mean times and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -244,6 +279,7 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = sg.Context(0)
+    ctx.autotune(args.autotune)
     A = sg.Csr(ctx, rp, ci, v)
     db = sg.DeviceArray.from_host(ctx, b)
     dx = sg.DeviceArray(ctx, n)
@@ -268,6 +304,9 @@ def main():
     t_setup = time.perf_counter() - t0
     info = M.stats()
     print(f"# setup {t_setup:.2f} s  {info}", file=sys.stderr, flush=True)
+    if args.probe:
+        probe(sg, ctx, A, M, db, dx, args.probe, info, n, nnz)
+        return
 
     run_solves(sg, ctx, A, M, db, dx, args.warmup, **solve_kw)
     rs = run_solves(sg, ctx, A, M, db, dx, args.steps, **solve_kw)

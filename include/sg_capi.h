@@ -149,6 +149,10 @@ typedef struct {
     int max_levels;
     int z_in_lds;
     int nc;
+    int max_ns;       /* rows of the largest (overlapped) subdomain            */
+    int sweep_shape;  /* sweep kernel shape in use (see sg_context_set_autotune:
+                         0..2 barrier per level with 256/512/1024 threads,
+                         3..4 sync-free with 512/1024), -1 before the first apply */
 } sg_precond_info;
 sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info);
 
@@ -246,12 +250,14 @@ typedef enum {
 } sg_kernel_family;
 sg_status sg_context_set_profiling(sg_context* ctx, int enable);
 /* Kernel-shape selection.  mode 1 (default): the first SpMV on a matrix and
- * the first apply of a preconditioner time each shape (SpMV rows per
- * workgroup, sweep workgroup size: 256, 512, 1024) and keep the fastest.
- * mode 0: fixed defaults (SpMV 256, sweep 512).  mode 2, 3, 4: force shape
- * 256, 512, 1024 for both.  All shapes compute identical values, so the mode
- * never changes results.  No counterpart in the reference (its kernels have
- * fixed launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
+ * the first apply of a preconditioner time each shape and keep the fastest
+ * (SpMV: 256, 512 or 1024 rows per workgroup; subdomain sweeps: 256, 512 or
+ * 1024 threads with a barrier per level, or 512 / 1024 threads sync-free with
+ * per-row flags in LDS).  mode 0: fixed defaults (SpMV 256, sweep 512).
+ * mode 2..6: force sweep shape mode-2 in that order and SpMV shape
+ * min(mode-2, 2).  All shapes compute identical values, so the mode never
+ * changes results.  No counterpart in the reference (its kernels have fixed
+ * launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
 sg_status sg_context_set_autotune(sg_context* ctx, int mode);
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
                                   double* total_ms);

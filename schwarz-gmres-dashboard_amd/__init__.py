@@ -53,7 +53,8 @@ class SolveResult(C.Structure):
 class PrecondInfo(C.Structure):
     _fields_ = [("nsub", C.c_int), ("n_ext", C.c_longlong), ("nnz_ext", C.c_longlong),
                 ("levels_fwd", C.c_longlong), ("levels_bwd", C.c_longlong),
-                ("max_levels", C.c_int), ("z_in_lds", C.c_int), ("nc", C.c_int)]
+                ("max_levels", C.c_int), ("z_in_lds", C.c_int), ("nc", C.c_int),
+                ("max_ns", C.c_int), ("sweep_shape", C.c_int)]
 
 
 _lib = None
@@ -148,7 +149,8 @@ class Context:
         self.check(lib().sg_context_set_profiling(self.h, 1 if on else 0))
 
     def autotune(self, mode=1):
-        """0 fixed shapes, 1 time shapes on first use, 2/3/4 force 256/512/1024."""
+        """0 fixed shapes, 1 time shapes on first use, 2..6 force a shape (see
+        sg_context_set_autotune)."""
         self.check(lib().sg_context_set_autotune(self.h, int(mode)))
 
     def kernel_stats(self, family):

@@ -28,12 +28,26 @@
 #include <stdexcept>
 #include <thread>
 
+#ifndef SG_SF_DEPTH
+#define SG_SF_DEPTH 3
+#endif
+
 namespace sg {
 
 constexpr int kTrsvThreads = 512;   // workgroup of the factorisation
 constexpr int kPadRows = 1024;      // padding of the sweep arrays (largest sweep workgroup)
-constexpr int kSweepVariants[3] = {256, 512, 1024};
+// Sweep kernel shapes, all bit-identical: level-synchronous with 256/512/1024
+// threads, and sync-free (per-row LDS flags, no barrier per level) with
+// 512/1024 threads when z and the flags fit in LDS.
+struct SweepShape {
+    int threads;
+    bool syncfree;
+};
+constexpr int kNumSweepShapes = 5;
+constexpr SweepShape kSweepShapes[kNumSweepShapes] = {
+    {256, false}, {512, false}, {1024, false}, {512, true}, {1024, true}};
 constexpr int kKreg = 4;  // factor entries per row prefetched in registers
+constexpr int kSfDepth = SG_SF_DEPTH;  // items of row data in flight per wave (sync-free sweep)
 
 struct SubInfo {
     int ns;      // rows of the (overlapped) subdomain
@@ -65,6 +79,7 @@ struct Plan {
     std::vector<int> dofs;   // global id of every local row, concatenated
     std::vector<int> own;    // owned local rows, per subdomain ascending
     std::vector<int> own_gid;  // global ids of those rows
+    std::vector<int> f_item, b_item;  // per level: 64-row items of the earlier levels
     int total_rows = 0;
     int max_ns = 0;
 };
@@ -96,8 +111,13 @@ struct sg_precond {
     bool z_in_lds = false;
     size_t lds_bytes = 0;      // dynamic LDS of the sweep with z in LDS
     size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned)
-    int sweep_variant = 1;     // index into kSweepVariants (workgroup size)
+    int sweep_variant = 1;     // index into kSweepShapes
     bool sweep_tuned = false;
+    // sync-free sweep: LDS = level tables | item prefixes | z | row flags
+    bool sf_ok = false;
+    int max_ns = 0;
+    size_t sf_item_off = 0, sf_z_off = 0, sf_flag_off = 0, sf_bytes = 0;
+    sg::DevBuf<int> d_fitem, d_bitem;
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
     sg::DevBuf<sg::Level> flev, blev;
@@ -269,6 +289,20 @@ __global__ void k_pack(int n, const int* __restrict__ src, const double* __restr
     }
 }
 
+// LDS flag bytes with an explicit address space: the accesses stay ds_read /
+// ds_write (a generic pointer would make them flat accesses, whose waits also
+// drain every global load in flight).
+typedef __attribute__((address_space(3))) volatile unsigned char lds_flag_t;
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for all
+// outstanding global loads (vmcnt(0)), which would drain the row-data prefetch
+// at every level.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Row data of one thread for one level, loaded two levels ahead.  The raw
 // loaded values are kept as they are; `use` (bit k: entry k is real, bit 31:
 // the thread has a row) is applied only when the level is computed, so no
@@ -341,7 +375,7 @@ __device__ __forceinline__ void row_compute(T* z, const RowPre<T>& p, const Leve
 // levels ahead into a ring of three register buffers; the loop is unrolled by
 // three so the ring never needs register moves (a move would wait for the
 // loads in flight).  lev points at the subdomain's level table in LDS.
-template <typename T, bool BWD>
+template <typename T, bool BWD, bool ZLDS>
 __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslot,
                                       const int* __restrict__ rows,
                                       const int* __restrict__ cols,
@@ -351,21 +385,25 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
     const int t = threadIdx.x;
     const Level none{0, 0, 0, 0};
     auto level = [&](int l) { return l < nlev ? lev[l] : none; };
+    auto level_barrier = [] {
+        if (ZLDS) lds_barrier();
+        else __syncthreads();
+    };
     RowPre<T> b0, b1, b2;
     row_prefetch<T, BWD>(b0, level(0), t, rows, cols, vals, diags);
     row_prefetch<T, BWD>(b1, level(1), t, rows, cols, vals, diags);
     for (int l = 0; l < nlev; l += 3) {
         row_prefetch<T, BWD>(b2, level(l + 2), t, rows, cols, vals, diags);
         row_compute<T, BWD>(z, b0, level(l), zslot, rows, cols, vals, diags);
-        __syncthreads();
+        level_barrier();
         if (l + 1 >= nlev) break;
         row_prefetch<T, BWD>(b0, level(l + 3), t, rows, cols, vals, diags);
         row_compute<T, BWD>(z, b1, level(l + 1), zslot, rows, cols, vals, diags);
-        __syncthreads();
+        level_barrier();
         if (l + 2 >= nlev) break;
         row_prefetch<T, BWD>(b1, level(l + 4), t, rows, cols, vals, diags);
         row_compute<T, BWD>(z, b2, level(l + 2), zslot, rows, cols, vals, diags);
-        __syncthreads();
+        level_barrier();
     }
 }
 
@@ -412,8 +450,10 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     }
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
-    sweep<T, false>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
-    sweep<T, true>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
+    sweep<T, false, ZLDS>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
+    __syncthreads();
+    sweep<T, true, ZLDS>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
+    __syncthreads();
     if (MODE == 0) {
         const int* ow = own + si.own0;
         const int* og = own_gid + si.own0;
@@ -424,6 +464,171 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         }
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = z[i];
+    }
+}
+
+// ---- sync-free sweep ---------------------------------------------------------
+// The rows of a sweep, in level order, are cut into items of at most 64 rows
+// of one level (item0[l] = items of the levels before l).  Wave w of the
+// workgroup runs items w, w + NW, w + 2 NW, ...; a row first waits until every
+// row it reads has raised its LDS flag to `tgt`, then computes exactly as the
+// level-synchronous sweep does (same entries, same order), stores z and raises
+// its own flag.  An item only reads rows of earlier levels, so the smallest
+// unfinished item can always proceed: no deadlock, and no barrier per level.
+// Row data is prefetched two items ahead per wave.
+struct ItemPos {
+    Level L;
+    int q;
+};
+
+__device__ __forceinline__ ItemPos locate_item(const Level* lev, const int* item0, int nlev,
+                                               int nitems, int it, int& cur) {
+    ItemPos p;
+    if (it >= nitems) {
+        p.L = Level{0, 0, 0, 0};
+        p.q = 0;
+        return p;
+    }
+    while (cur + 1 < nlev && item0[cur + 1] <= it) ++cur;
+    p.L = lev[cur];
+    p.q = (it - item0[cur]) * 64 + (threadIdx.x & 63);
+    return p;
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void row_compute_sf(T* z, lds_flag_t* fl,
+                                               const RowPre<T>& p, const ItemPos& ip,
+                                               int zslot, unsigned char tgt,
+                                               const int* __restrict__ cols,
+                                               const T* __restrict__ vals) {
+    if (!(p.use & 0x80000000u)) return;
+    const Level& L = ip.L;
+    for (;;) {
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < kKreg; ++k)
+            if ((p.use >> k) & 1u) ok = ok && fl[p.col[k]] >= tgt;
+        for (int k = kKreg; k < L.K && ok; ++k) ok = fl[cols[L.ell + k * L.size + ip.q]] >= tgt;
+        if (ok) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    T acc = z[p.row];
+#pragma unroll
+    for (int k = 0; k < kKreg; ++k) {
+        const bool u = (p.use >> k) & 1u;
+        acc -= (u ? p.val[k] : (T)0) * z[u ? p.col[k] : zslot];  // unused: - 0*0
+    }
+    for (int k = kKreg; k < L.K; ++k)
+        acc -= vals[L.ell + k * L.size + ip.q] * z[cols[L.ell + k * L.size + ip.q]];
+    if (BWD) acc = acc / p.diag;
+    z[p.row] = acc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    fl[p.row] = tgt;
+}
+
+template <typename T, bool BWD, int NT, int D>
+__device__ __forceinline__ void sweep_syncfree(T* z, lds_flag_t* fl, const Level* lev,
+                                               const int* item0, int nlev, int zslot,
+                                               unsigned char tgt, const int* __restrict__ rows,
+                                               const int* __restrict__ cols,
+                                               const T* __restrict__ vals,
+                                               const T* __restrict__ diags) {
+    if (nlev == 0) return;
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6;
+    const int nitems = item0[nlev - 1] + (lev[nlev - 1].size + 63) / 64;
+    int cur = 0;
+    // ring of D buffers: item it + d*NW lives in buffer d (mod D); every index
+    // is a compile-time constant after unrolling, so no register moves
+    ItemPos pos[D];
+    RowPre<T> buf[D];
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) {
+        pos[d] = locate_item(lev, item0, nlev, nitems, wv + d * NW, cur);
+        row_prefetch<T, BWD>(buf[d], pos[d].L, pos[d].q, rows, cols, vals, diags);
+    }
+    for (int it = wv; it < nitems; it += D * NW) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int ci = it + d * NW;
+            if (ci >= nitems) break;
+            const int nb = (d + D - 1) % D;
+            pos[nb] = locate_item(lev, item0, nlev, nitems, ci + (D - 1) * NW, cur);
+            row_prefetch<T, BWD>(buf[nb], pos[nb].L, pos[nb].q, rows, cols, vals, diags);
+            row_compute_sf<T, BWD>(z, fl, buf[d], pos[d], zslot, tgt, cols, vals);
+        }
+    }
+}
+
+// The fused gather + sweeps + prolongation of k_schwarz_sweep (z in LDS) with
+// sync-free sweeps.  LDS: level tables | item prefixes | z | row flags.
+template <typename T, int MODE, int NT, int D = kSfDepth>
+__global__ void __launch_bounds__(NT)
+k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order,
+                 const Level* __restrict__ flev, const int* __restrict__ fitem,
+                 const int* __restrict__ f_row, const int* __restrict__ f_col,
+                 const T* __restrict__ f_val, const Level* __restrict__ blev,
+                 const int* __restrict__ bitem, const int* __restrict__ b_row,
+                 const int* __restrict__ b_col, const T* __restrict__ b_val,
+                 const T* __restrict__ b_diag, const int* __restrict__ dofs,
+                 const int* __restrict__ own, const int* __restrict__ own_gid,
+                 const double* __restrict__ x, double* __restrict__ y, T* __restrict__ zout,
+                 const T* __restrict__ cz, DevGuard guard, int item_off, int z_off,
+                 int flag_off) {
+    extern __shared__ unsigned char lds_raw[];
+    const int s = order[blockIdx.x];
+    if (guard_stop(guard, s)) return;
+    const SubInfo si = subs[s];
+    Level* lf = reinterpret_cast<Level*>(lds_raw);
+    Level* lb = lf + si.nflev;
+    int* itf = reinterpret_cast<int*>(lds_raw + item_off);
+    int* itb = itf + si.nflev;
+    T* z = reinterpret_cast<T*>(lds_raw + z_off);
+    lds_flag_t* fl = (lds_flag_t*)(lds_raw + flag_off);
+    for (int l = threadIdx.x; l < si.nflev; l += NT) {
+        lf[l] = flev[si.flev0 + l];
+        itf[l] = fitem[si.flev0 + l];
+    }
+    for (int l = threadIdx.x; l < si.nblev; l += NT) {
+        lb[l] = blev[si.blev0 + l];
+        itb[l] = bitem[si.blev0 + l];
+    }
+    const int* d = dofs + si.row0;
+    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 4 * NT) {
+        int g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * NT;
+            g[u] = i < si.ns ? d[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (g[u] >= 0) {
+                z[i0 + u * NT] = (T)x[g[u]];
+                fl[i0 + u * NT] = 0;
+            }
+    }
+    if (threadIdx.x == 0) {
+        z[si.ns] = (T)0;  // zero slot for padding entries, always ready
+        fl[si.ns] = 3;
+    }
+    __syncthreads();
+    sweep_syncfree<T, false, NT, D>(z, fl, lf, itf, si.nflev, si.ns, 1, f_row, f_col, f_val,
+                                    nullptr);
+    __syncthreads();
+    sweep_syncfree<T, true, NT, D>(z, fl, lb, itb, si.nblev, si.ns, 2, b_row, b_col, b_val, b_diag);
+    __syncthreads();
+    if (MODE == 0) {
+        const int* ow = own + si.own0;
+        const int* og = own_gid + si.own0;
+        const T add = cz ? cz[s] : (T)0;
+        for (int k = threadIdx.x; k < si.nown; k += NT) {
+            const int li = ow[k], g = og[k];
+            y[g] = cz ? (double)(z[li] + add) : (double)z[li];
+        }
+    } else {
+        for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
     }
 }
 
@@ -728,6 +933,15 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
         si.nown = (int)own[s].size();
         for (Level lv : fw[s].lev) pl.flev.push_back(Level{lv.start + (int)fpos, lv.size, lv.K, lv.ell + (int)fell});
         for (Level lv : bw[s].lev) pl.blev.push_back(Level{lv.start + (int)bpos, lv.size, lv.K, lv.ell + (int)bell});
+        for (int dir = 0; dir < 2; ++dir) {
+            const auto& lev = dir ? bw[s].lev : fw[s].lev;
+            auto& item = dir ? pl.b_item : pl.f_item;
+            int acc = 0;
+            for (const Level& lv : lev) {
+                item.push_back(acc);
+                acc += (lv.size + 63) / 64;
+            }
+        }
         pl.f_row.insert(pl.f_row.end(), fw[s].row.begin(), fw[s].row.end());
         pl.b_row.insert(pl.b_row.end(), bw[s].row.begin(), bw[s].row.end());
         pl.f_col.insert(pl.f_col.end(), fw[s].col.begin(), fw[s].col.end());
@@ -778,11 +992,14 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
                      [&](int a, int b) { return M->sub_nnz[a] > M->sub_nnz[b]; });
 
     M->nsub = nsub;
+    M->max_ns = pl.max_ns;
     M->lrp_host = pl.lrp;
     M->d_subs.upload(pl.subs, st);
     M->d_order.upload(pl.order, st);
     M->flev.upload(pl.flev, st);
     M->blev.upload(pl.blev, st);
+    M->d_fitem.upload(pl.f_item, st);
+    M->d_bitem.upload(pl.b_item, st);
     M->f_row.upload(pl.f_row, st);
     M->f_col.upload(pl.f_col, st);
     M->f_src.upload(pl.f_src, st);
@@ -837,6 +1054,14 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
         if (fp32) M->z32.alloc((size_t)zoff);
         else M->z64.alloc((size_t)zoff);
     }
+    {
+        auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+        M->sf_item_off = M->lds_tab_bytes;
+        M->sf_z_off = M->sf_item_off + al((size_t)maxlev * sizeof(int));
+        M->sf_flag_off = M->sf_z_off + al(zbytes);
+        M->sf_bytes = M->sf_flag_off + al((size_t)pl.max_ns + 1);
+        M->sf_ok = M->sf_bytes <= (size_t)ctx->max_lds;
+    }
     if (additive) {
         if (fp32) M->zout32.alloc((size_t)pl.total_rows + 1);
         else M->zout64.alloc((size_t)pl.total_rows + 1);
@@ -859,6 +1084,20 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
             SG_SET_LDS(double, false, 0) SG_SET_LDS(double, false, 1)
         }
 #undef SG_SET_LDS
+        if (M->sf_ok) {
+            const int sfb = (int)M->sf_bytes;
+#define SG_SET_SF(T, MD)                                                                          \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_syncfree<T, MD, 512>,                        \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, sfb));               \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_syncfree<T, MD, 1024>,                       \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, sfb));
+            if (fp32) {
+                SG_SET_SF(float, 0) SG_SET_SF(float, 1)
+            } else {
+                SG_SET_SF(double, 0) SG_SET_SF(double, 1)
+            }
+#undef SG_SET_SF
+        }
     }
 
     if (additive) {
@@ -1018,6 +1257,19 @@ static void setup(sg_precond* M, const sg_csr* A) {
 template <typename T, bool ZLDS, int MODE>
 static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+    const SweepShape sh = kSweepShapes[M->sweep_variant];
+    if (ZLDS && sh.syncfree && M->sf_ok) {
+#define SG_SF(NT)                                                                                 \
+    hipLaunchKernelGGL((k_sweep_syncfree<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->sf_bytes,   \
+                       M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p, M->d_fitem.p,      \
+                       M->f_row.p, M->f_col.p, fv, M->blev.p, M->d_bitem.p, M->b_row.p,         \
+                       M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zo,   \
+                       cz, guard, (int)M->sf_item_off, (int)M->sf_z_off, (int)M->sf_flag_off)
+        if (sh.threads == 1024) SG_SF(1024);
+        else SG_SF(512);
+#undef SG_SF
+        return;
+    }
     const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;
 #define SG_SWEEP(NT)                                                                              \
     hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE, NT>), dim3(M->nsub), dim3(NT), lds,       \
@@ -1025,7 +1277,7 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                        M->f_col.p, fv, M->blev.p, M->b_row.p, M->b_col.p, bv, bd, M->d_dofs.p,  \
                        M->d_own.p, M->d_own_gid.p, x, y, zg, zo, cz, guard,                     \
                        (int)M->lds_tab_bytes)
-    switch (kSweepVariants[M->sweep_variant]) {
+    switch (sh.syncfree ? 512 : sh.threads) {
         case 256: SG_SWEEP(256); break;
         case 1024: SG_SWEEP(1024); break;
         default: SG_SWEEP(512); break;
@@ -1061,7 +1313,8 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
     SG_HIP(hipEventCreate(&e1));
     float best = 0.f;
     int bestv = 1;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < kNumSweepShapes; ++v) {
+        if (kSweepShapes[v].syncfree && !(M->sf_ok && M->z_in_lds)) continue;
         M->sweep_variant = v;
         const DevGuard none{nullptr, nullptr, 0};
         launch_any_sweep<T>(M, xin.p, yout.p, none, fv, bv, bd, zg, zo, nullptr);
@@ -1345,6 +1598,12 @@ sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info) {
     info->nsub = M->nsub;
     info->z_in_lds = M->z_in_lds ? 1 : 0;
     info->nc = M->nc;
+    info->max_ns = M->max_ns;
+    info->sweep_shape = M->nsub && (M->sweep_tuned || M->ctx->autotune != 1)
+                            ? (kSweepShapes[M->sweep_variant].syncfree && !(M->sf_ok && M->z_in_lds)
+                                   ? 1
+                                   : M->sweep_variant)
+                            : -1;
     for (size_t s = 0; s < M->dofs.size() && M->nsub; ++s) info->n_ext += (long long)M->dofs[s].size();
     for (int v : M->sub_nnz) info->nnz_ext += v;
     info->levels_fwd = M->levels_fwd;

@@ -154,7 +154,7 @@ void spmv_autotune(sg_csr* A) {
 void spmv_ensure_tuned(sg_csr* A) {
     if (A->nrows == 0) return;
     if (A->ctx->autotune == 1 && !A->tuned) spmv_autotune(A);
-    if (A->ctx->autotune >= 2) A->variant = A->ctx->autotune - 2;
+    if (A->ctx->autotune >= 2) A->variant = std::min(A->ctx->autotune - 2, 2);
     else if (A->ctx->autotune == 0) A->variant = 0;
 }
 

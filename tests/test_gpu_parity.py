@@ -196,7 +196,7 @@ def test_kernel_shapes_do_not_change_results(sg, precision):
     v = A.v * rng.uniform(0.5, 1.5, A.nnz)
     x = rng.standard_normal(A.n)
     outs = []
-    for mode in (0, 1, 2, 3, 4):
+    for mode in (0, 1, 2, 3, 4, 5, 6):
         c = sg.Context(0)
         c.autotune(mode)
         dA = sg.Csr(c, A.rp, A.ci, v)
