@@ -105,6 +105,21 @@ def traffic_models(n, nnz, info, restart, fp32):
     }
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE, calibrated on a plain copy); None when
+    this workload has not been profiled."""
+    f = os.path.join(ROOT, "profiles", "traffic_c3_diffusion160.json")
+    try:
+        with open(f) as fh:
+            t = json.load(fh)
+    except OSError:
+        return None
+    if t.get("workload") != workload or kernel not in t.get("kernels", {}):
+        return None
+    return float(t["kernels"][kernel]["bytes"])
+
+
 def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, gpu_iters):
     """Oracle FGMRES + two-level RAS (FP64, reference algorithm) on the host."""
     A = O.Csr(rp, ci, v)
@@ -376,7 +391,8 @@ def main():
                    "setup_s": round(t_setup, 3), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": models[dom],
+                     "traffic": measured_traffic(cfg["workload"], dom),
+                     "bytes_per_launch": models[dom],
                      "avg_launch_us": round(avg_s * 1e6, 2)},
         "kernel_ms": {f: round(fams[f][1], 3) for f in fams},
         "cpu_baseline": cpu,

@@ -237,6 +237,29 @@ sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b, double* d_
                          const sg_solve_params* params, sg_solve_result* result, double* hist,
                          int hist_cap);
 
+/* ---- learned stage: residual-to-error network -------------------------------
+ * No counterpart in the reference.  A fixed 4-level U-Net (channels
+ * 16-32-64-64, 3x3 convolutions, ReLU, 2x2 average pooling, nearest
+ * upsampling with skip concatenation, 1x1 head; 302705 parameters packed as
+ * documented in oracle/unet_oracle.cpp) with bfloat16 weights and activations
+ * and fp32 accumulation on the matrix cores.  Inputs [B][2][H][W] fp32
+ * (residual / rms(residual), log kappa), output [B][H][W] fp32; H and W must
+ * be multiples of 8.  Loss: mean over the batch of ||out - t||^2 / ||t||^2. */
+typedef struct sg_unet sg_unet;
+int sg_unet_nparams(void);
+sg_status sg_unet_create(sg_context* ctx, sg_unet** out);
+sg_status sg_unet_destroy(sg_unet* net);
+/* host fp32 parameter vectors (the fp32 master copy; Adam state is reset) */
+sg_status sg_unet_set_params(sg_unet* net, const float* params);
+sg_status sg_unet_get_params(const sg_unet* net, float* params);
+/* device buffers */
+sg_status sg_unet_forward(sg_unet* net, int B, int H, int W, const float* d_in, float* d_out);
+/* forward + backward; loss and grad (host, nparams floats) may be NULL */
+sg_status sg_unet_loss_grad(sg_unet* net, int B, int H, int W, const float* d_in,
+                            const float* d_target, double* loss, float* grad);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
+sg_status sg_unet_adam_step(sg_unet* net, float lr);
+
 /* ---- kernel timing (HIP events on the context stream) ------------------------
  * With profiling on, every launch of the listed kernel families is bracketed
  * by HIP events; totals are collected at each synchronisation. */

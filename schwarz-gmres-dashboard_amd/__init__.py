@@ -100,6 +100,15 @@ def lib():
         "sg_solve": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult), _VP,
                       C.c_int], C.c_int),
         "sg_partition": ([C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP], C.c_int),
+        "sg_unet_nparams": ([], C.c_int),
+        "sg_unet_create": ([_VP, C.POINTER(_VP)], C.c_int),
+        "sg_unet_destroy": ([_VP], C.c_int),
+        "sg_unet_set_params": ([_VP, _VP], C.c_int),
+        "sg_unet_get_params": ([_VP, _VP], C.c_int),
+        "sg_unet_forward": ([_VP, C.c_int, C.c_int, C.c_int, _VP, _VP], C.c_int),
+        "sg_unet_loss_grad": ([_VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
+                               C.POINTER(C.c_double), _VP], C.c_int),
+        "sg_unet_adam_step": ([_VP, C.c_float], C.c_int),
         "sg_csr_create_batch": ([_VP, C.c_int, _VP, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
                                 C.c_int),
         "sg_csr_systems": ([_VP, C.POINTER(C.c_int)], C.c_int),
@@ -212,6 +221,65 @@ class BatchCsr:
         try:
             if self.h and self.ctx.h:
                 lib().sg_csr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class UNet:
+    """The learned stage's residual-to-error network (sg_unet)."""
+
+    def __init__(self, ctx, params=None):
+        self.ctx = ctx
+        self.h = _VP()
+        ctx.check(lib().sg_unet_create(ctx.h, C.byref(self.h)))
+        self.nparams = lib().sg_unet_nparams()
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.nparams
+        self.ctx.check(lib().sg_unet_set_params(self.h, _p(p)))
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx.check(lib().sg_unet_get_params(self.h, _p(p)))
+        return p
+
+    def forward(self, x):
+        """x: host [B, 2, H, W] -> host [B, H, W]."""
+        x = np.ascontiguousarray(x, np.float32)
+        B, _, H, W = x.shape
+        dx = DeviceArray.from_host(self.ctx, x.ravel())
+        dy = DeviceArray(self.ctx, B * H * W, np.float32)
+        self.forward_dev(B, H, W, dx, dy)
+        return dy.get().reshape(B, H, W)
+
+    def forward_dev(self, B, H, W, dx, dy):
+        self.ctx.check(lib().sg_unet_forward(self.h, B, H, W, dx.ptr, dy.ptr))
+
+    def loss_grad_dev(self, B, H, W, dx, dt, want_grad=False):
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.float32) if want_grad else None
+        self.ctx.check(lib().sg_unet_loss_grad(self.h, B, H, W, dx.ptr, dt.ptr, C.byref(loss),
+                                               _p(g)))
+        return (loss.value, g) if want_grad else loss.value
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        B, _, H, W = x.shape
+        dx = DeviceArray.from_host(self.ctx, x.ravel())
+        dt = DeviceArray.from_host(self.ctx, np.ascontiguousarray(t, np.float32).ravel())
+        return self.loss_grad_dev(B, H, W, dx, dt, want_grad=True)
+
+    def adam(self, lr):
+        self.ctx.check(lib().sg_unet_adam_step(self.h, float(lr)))
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().sg_unet_destroy(self.h)
                 self.h = None
         except Exception:
             pass

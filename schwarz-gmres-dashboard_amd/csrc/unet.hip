@@ -1,0 +1,796 @@
+// The learned residual-to-error network of the learned FGMRES stage: forward,
+// backward and Adam on the GPU.  The architecture (a 4-level U-Net, channels
+// 16-32-64-64, 3x3 convolutions, ReLU, 2x2 average pooling, nearest
+// upsampling with skip concatenation, 1x1 head) and the parameter packing are
+// those of the CPU oracle that the tests compare against.
+//
+// Storage: activations and activation gradients in bfloat16, NCHW with the
+// batch outermost; weights as an fp32 master copy plus bfloat16 copies laid out
+// for the kernels; every sum in fp32.
+//
+// 3x3 convolutions (forward, and the data gradient as a convolution with the
+// flipped, transposed weights) are implicit GEMMs on the matrix cores:
+// v_mfma_f32_16x16x16bf16_1k with A = weights (16 output channels x 16 input
+// channels of one tap) and B = input (16 input channels x 16 pixels of one
+// image row), both staged in LDS per 32-channel chunk.  A workgroup computes an
+// 8 x 16 pixel tile for up to 64 output channels; each wavefront two rows.
+// The weight gradient runs on the vector ALUs in fp32 (each thread one
+// (output, input) channel pair, a 3x3 window slid along the tile rows) and
+// writes per-tile partial sums that a second kernel adds in a fixed order.
+#include "sg_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace sg {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+struct UConv {
+    int cin, cout;
+};
+constexpr int kUConvs = 14;
+constexpr UConv kUSpec[kUConvs] = {{2, 16},  {16, 16}, {16, 32}, {32, 32}, {32, 64},
+                                   {64, 64}, {64, 64}, {64, 64}, {128, 64}, {64, 64},
+                                   {96, 32}, {32, 32}, {48, 16}, {16, 16}};
+
+inline int uconv_offset(int i) {
+    int o = 0;
+    for (int k = 0; k < i; ++k) o += kUSpec[k].cout * kUSpec[k].cin * 9 + kUSpec[k].cout;
+    return o;
+}
+inline int unet_nparams_() { return uconv_offset(kUConvs) + 17; }
+inline int cin_pad(int c) { return (c + 15) / 16 * 16; }
+
+// A channel window [c0, c0 + C) of a [B][ctot][H][W] bfloat16 tensor.
+struct View {
+    uint16_t* p;
+    int ctot, c0;
+    __device__ size_t at(int b, int c, int y, int x, int H, int W) const {
+        return (((size_t)b * ctot + c0 + c) * H + y) * W + x;
+    }
+};
+
+// ---- 3x3 convolution on the matrix cores ------------------------------------
+constexpr int kCX = 16, kCY = 8, kCIC = 32;
+constexpr int kPS = kCIC + 4;  // bf16 per staged pixel / weight row (bank spread)
+
+// MODE 0: out = relu(acc + bias); MODE 1: out = acc (data gradient).
+// Weights wt: [9][cout][cinp] bfloat16 (cinp = cin rounded up to 16).
+// Grid: x = B * tilesY * tilesX, y = output-channel blocks of 64.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
+          const float* __restrict__ bias, View out, int cout, int H, int W) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[(kCY + 2) * (kCX + 2) * kPS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * 64 * kPS];
+    const int tilesX = (W + kCX - 1) / kCX, tilesY = (H + kCY - 1) / kCY;
+    int t = blockIdx.x;
+    const int tx = t % tilesX;
+    t /= tilesX;
+    const int ty = t % tilesY;
+    const int b = t / tilesY;
+    const int x0 = tx * kCX, y0 = ty * kCY;
+    const int co0 = blockIdx.y * 64;
+    const int ncot = min(4, (cout - co0) / 16);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v4f acc[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[r][n] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < cinp; c0 += kCIC) {
+        const int cc = min(kCIC, cinp - c0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cc * (kCY + 2) * (kCX + 2); e += 256) {
+            const int px = e % (kCX + 2);
+            const int r = e / (kCX + 2);
+            const int py = r % (kCY + 2);
+            const int ci = r / (kCY + 2);
+            const int gy = y0 + py - 1, gx = x0 + px - 1;
+            uint16_t v = 0;
+            if (c0 + ci < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = in.p[in.at(b, c0 + ci, gy, gx, H, W)];
+            s_in[(py * (kCX + 2) + px) * kPS + ci] = v;
+        }
+        const int q4 = cc / 4;
+        for (int e = threadIdx.x; e < 9 * ncot * 16 * q4; e += 256) {
+            const int c4 = e % q4;
+            const int r = e / q4;
+            const int co = r % (ncot * 16);
+            const int tap = r / (ncot * 16);
+            const uint2 v = *reinterpret_cast<const uint2*>(
+                wt + ((size_t)tap * cout + co0 + co) * cinp + c0 + 4 * c4);
+            *reinterpret_cast<uint2*>(&s_w[(tap * 64 + co) * kPS + 4 * c4]) = v;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            for (int kb = 0; kb < cc; kb += 16) {
+                const int kofs = kb + 4 * (lane >> 4);
+                v4s bfr[2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int py = 2 * wv + r + ky, px = (lane & 15) + kx;
+                    bfr[r] = *reinterpret_cast<const v4s*>(&s_in[(py * (kCX + 2) + px) * kPS + kofs]);
+                }
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    if (n >= ncot) break;
+                    const v4s afr = *reinterpret_cast<const v4s*>(
+                        &s_w[(tap * 64 + n * 16 + (lane & 15)) * kPS + kofs]);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+                        acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[r], acc[r][n],
+                                                                             0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[co][pixel]: lane holds pixel (lane & 15), channels 4*(lane >> 4) + i
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int y = y0 + 2 * wv + r, x = x0 + (lane & 15);
+        if (y >= H || x >= W) continue;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            if (n >= ncot) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = co0 + n * 16 + 4 * (lane >> 4) + i;
+                float v = acc[r][n][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    v = v > 0.f ? v : 0.f;
+                }
+                out.p[out.at(b, co, y, x, H, W)] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ---- weight gradient (fp32 vector ALUs) ---------------------------------------
+// Tile: 8 rows x 32 columns of one image.  Block (co16, ci16, tile): thread
+// (co, ci) = (t >> 4, t & 15) accumulates the 9 taps over the tile; thread
+// ci = 0 also sums the bias gradient.  part[tile][tap][cout][cin] (+ bias part).
+constexpr int kWY = 8, kWX = 32;
+__global__ void __launch_bounds__(256)
+k_conv_dw(View in, int cin, View dp, int cout, int B, int H, int W, float* __restrict__ part,
+          float* __restrict__ part_b) {
+    __shared__ float s_g[16][kWY][kWX];
+    __shared__ float s_x[16][kWY + 2][kWX + 2 + 1];
+    const int tilesX = (W + kWX - 1) / kWX, tilesY = (H + kWY - 1) / kWY;
+    int t = blockIdx.x;
+    const int tile = t;
+    const int tx = t % tilesX;
+    t /= tilesX;
+    const int ty = t % tilesY;
+    const int b = t / tilesY;
+    const int x0 = tx * kWX, y0 = ty * kWY;
+    const int cob = blockIdx.y * 16, cib = blockIdx.z * 16;
+    for (int e = threadIdx.x; e < 16 * kWY * kWX; e += 256) {
+        const int x = e % kWX, y = (e / kWX) % kWY, c = e / (kWX * kWY);
+        const int gy = y0 + y, gx = x0 + x;
+        s_g[c][y][x] = (gy < H && gx < W) ? bf2f(dp.p[dp.at(b, cob + c, gy, gx, H, W)]) : 0.f;
+    }
+    for (int e = threadIdx.x; e < 16 * (kWY + 2) * (kWX + 2); e += 256) {
+        const int x = e % (kWX + 2), y = (e / (kWX + 2)) % (kWY + 2), c = e / ((kWX + 2) * (kWY + 2));
+        const int gy = y0 + y - 1, gx = x0 + x - 1;
+        float v = 0.f;
+        if (cib + c < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
+            v = bf2f(in.p[in.at(b, cib + c, gy, gx, H, W)]);
+        s_x[c][y][x] = v;
+    }
+    __syncthreads();
+    const int co = threadIdx.x >> 4, ci = threadIdx.x & 15;
+    float acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = 0.f;
+    float bsum = 0.f;
+    for (int y = 0; y < kWY; ++y) {
+        float w0[3], w1[3];  // window columns x, x+1 of rows y, y+1, y+2
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            w0[k] = s_x[ci][y + k][0];
+            w1[k] = s_x[ci][y + k][1];
+        }
+        for (int x = 0; x < kWX; ++x) {
+            float w2[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) w2[k] = s_x[ci][y + k][x + 2];
+            const float g = s_g[co][y][x];
+            bsum += g;
+#pragma unroll
+            for (int ky = 0; ky < 3; ++ky) {
+                acc[ky * 3 + 0] += g * w0[ky];
+                acc[ky * 3 + 1] += g * w1[ky];
+                acc[ky * 3 + 2] += g * w2[ky];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                w0[k] = w1[k];
+                w1[k] = w2[k];
+            }
+        }
+    }
+    if (cib + ci < cin) {
+        float* pt = part + (size_t)tile * 9 * cout * cin;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pt[((size_t)k * cout + cob + co) * cin + cib + ci] = acc[k];
+    }
+    if (blockIdx.z == 0 && ci == 0) part_b[(size_t)tile * cout + cob + co] = bsum;
+}
+
+// grad (packed W[co][ci][ky][kx], then b) = sum over tiles, in tile order.
+__global__ void k_dw_reduce(int ntile, int cin, int cout, const float* __restrict__ part,
+                            const float* __restrict__ part_b, float* __restrict__ grad) {
+    const int nw = 9 * cout * cin;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nw + cout; e += gridDim.x * blockDim.x) {
+        float s = 0.f;
+        if (e < nw) {
+            const int tap = e % 9, ci = (e / 9) % cin, co = e / (9 * cin);
+            for (int t = 0; t < ntile; ++t) s += part[((size_t)t * 9 + tap) * cout * cin + (size_t)co * cin + ci];
+        } else {
+            const int co = e - nw;
+            for (int t = 0; t < ntile; ++t) s += part_b[(size_t)t * cout + co];
+        }
+        grad[e] = s;
+    }
+}
+
+// ---- elementwise / layout kernels ---------------------------------------------
+// network input: [B][2][H][W] fp32 -> bf16 [B][16][H][W] (channels 2..15 zero)
+__global__ void k_input(int B, int H, int W, const float* __restrict__ in, View x) {
+    const size_t hw = (size_t)H * W, n = (size_t)B * 16 * hw;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = e % hw;
+        const int c = (int)((e / hw) % 16), b = (int)(e / (16 * hw));
+        x.p[e] = c < 2 ? f2bf(in[((size_t)b * 2 + c) * hw + p]) : (uint16_t)0;
+    }
+}
+
+__global__ void k_pool(int B, int C, int H, int W, View in, View out) {
+    const int h = H / 2, w = W / 2;
+    const size_t n = (size_t)B * C * h * w;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % w), y = (int)((e / w) % h), c = (int)((e / ((size_t)w * h)) % C);
+        const int b = (int)(e / ((size_t)w * h * C));
+        const float s = bf2f(in.p[in.at(b, c, 2 * y, 2 * x, H, W)]) + bf2f(in.p[in.at(b, c, 2 * y, 2 * x + 1, H, W)]) +
+                        bf2f(in.p[in.at(b, c, 2 * y + 1, 2 * x, H, W)]) +
+                        bf2f(in.p[in.at(b, c, 2 * y + 1, 2 * x + 1, H, W)]);
+        out.p[out.at(b, c, y, x, h, w)] = f2bf(0.25f * s);
+    }
+}
+
+// dx (H x W) += 0.25 * dy(y/2, x/2), rounded
+__global__ void k_pool_bwd(int B, int C, int H, int W, View dy, View dx) {
+    const size_t n = (size_t)B * C * H * W;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % W), y = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
+        const int b = (int)(e / ((size_t)W * H * C));
+        const size_t i = dx.at(b, c, y, x, H, W);
+        dx.p[i] = f2bf(bf2f(dx.p[i]) + 0.25f * bf2f(dy.p[dy.at(b, c, y / 2, x / 2, H / 2, W / 2)]));
+    }
+}
+
+// out channels [0, C) at H x W = nearest upsampling of a (H/2 x W/2)
+__global__ void k_up(int B, int C, int H, int W, View a, View out) {
+    const size_t n = (size_t)B * C * H * W;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % W), y = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
+        const int b = (int)(e / ((size_t)W * H * C));
+        out.p[out.at(b, c, y, x, H, W)] = a.p[a.at(b, c, y / 2, x / 2, H / 2, W / 2)];
+    }
+}
+
+// da (H/2 x W/2) = sum of each 2x2 block of dy, rounded
+__global__ void k_up_bwd(int B, int C, int H, int W, View dy, View da) {
+    const int h = H / 2, w = W / 2;
+    const size_t n = (size_t)B * C * h * w;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % w), y = (int)((e / w) % h), c = (int)((e / ((size_t)w * h)) % C);
+        const int b = (int)(e / ((size_t)w * h * C));
+        // row-major order within the block, as the oracle accumulates
+        float s = bf2f(dy.p[dy.at(b, c, 2 * y, 2 * x, H, W)]);
+        s += bf2f(dy.p[dy.at(b, c, 2 * y, 2 * x + 1, H, W)]);
+        s += bf2f(dy.p[dy.at(b, c, 2 * y + 1, 2 * x, H, W)]);
+        s += bf2f(dy.p[dy.at(b, c, 2 * y + 1, 2 * x + 1, H, W)]);
+        da.p[da.at(b, c, y, x, h, w)] = f2bf(s);
+    }
+}
+
+// dy *= (y > 0)
+__global__ void k_relu_mask(int B, int C, int H, int W, View y, View dy) {
+    const size_t n = (size_t)B * C * H * W;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % W), yy = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
+        const int b = (int)(e / ((size_t)W * H * C));
+        if (!(bf2f(y.p[y.at(b, c, yy, x, H, W)]) > 0.f)) dy.p[dy.at(b, c, yy, x, H, W)] = 0;
+    }
+}
+
+// head: out[b][p] = hb + sum_c hw[c] * d0[b][c][p]
+__global__ void k_head(int B, int H, int W, View d0, const float* __restrict__ hw, float* __restrict__ out) {
+    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = e % hwn;
+        const int b = (int)(e / hwn);
+        float s = hw[16];
+        for (int c = 0; c < 16; ++c) s += hw[c] * bf2f(d0.p[((size_t)b * d0.ctot + d0.c0 + c) * hwn + p]);
+        out[e] = s;
+    }
+}
+
+// per-sample sums t.t and (out - t).(out - t): block partials [b][blk][2]
+constexpr int kLB = 64;  // blocks per sample
+__global__ void __launch_bounds__(256)
+k_loss_part(int H, int W, const float* __restrict__ out, const float* __restrict__ tg,
+            double* __restrict__ part) {
+    const int b = blockIdx.y;
+    const size_t hwn = (size_t)H * W;
+    double tt = 0.0, ee = 0.0;
+    for (size_t p = blockIdx.x * 256 + threadIdx.x; p < hwn; p += kLB * 256) {
+        const double t = tg[b * hwn + p], d = (double)out[b * hwn + p] - t;
+        tt += t * t;
+        ee += d * d;
+    }
+    __shared__ double r0[256], r1[256];
+    r0[threadIdx.x] = tt;
+    r1[threadIdx.x] = ee;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            r0[threadIdx.x] += r0[threadIdx.x + s];
+            r1[threadIdx.x] += r1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[((size_t)b * kLB + blockIdx.x) * 2] = r0[0];
+        part[((size_t)b * kLB + blockIdx.x) * 2 + 1] = r1[0];
+    }
+}
+
+// sums[b] = {tt, ee}; loss = mean_b ee/tt
+__global__ void k_loss_fin(int B, const double* __restrict__ part, double* __restrict__ sums,
+                           double* __restrict__ loss) {
+    if (threadIdx.x != 0) return;
+    double l = 0.0;
+    for (int b = 0; b < B; ++b) {
+        double tt = 0.0, ee = 0.0;
+        for (int k = 0; k < kLB; ++k) {
+            tt += part[((size_t)b * kLB + k) * 2];
+            ee += part[((size_t)b * kLB + k) * 2 + 1];
+        }
+        sums[2 * b] = tt;
+        sums[2 * b + 1] = ee;
+        l += ee / tt / B;
+    }
+    *loss = l;
+}
+
+// dout = 2 (out - t) / (tt B); dd0[b][c][p] = bf16(dout * hw[c]); head partial
+// gradients per block: [blk][17] over all samples
+__global__ void __launch_bounds__(256)
+k_head_bwd(int B, int H, int W, const float* __restrict__ out, const float* __restrict__ tg,
+           const double* __restrict__ sums, View d0, const float* __restrict__ hw, View dd0,
+           float* __restrict__ hpart) {
+    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
+    float acc[17];
+    for (int c = 0; c < 17; ++c) acc[c] = 0.f;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = e % hwn;
+        const int b = (int)(e / hwn);
+        const float g = (float)(2.0 * ((double)out[e] - (double)tg[e]) / sums[2 * b] / B);
+        acc[16] += g;
+        for (int c = 0; c < 16; ++c) {
+            const size_t i = ((size_t)b * d0.ctot + d0.c0 + c) * hwn + p;
+            acc[c] += g * bf2f(d0.p[i]);
+            dd0.p[((size_t)b * dd0.ctot + dd0.c0 + c) * hwn + p] = f2bf(g * hw[c]);
+        }
+    }
+    __shared__ float red[256];
+    for (int c = 0; c < 17; ++c) {
+        red[threadIdx.x] = acc[c];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) hpart[(size_t)blockIdx.x * 17 + c] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_head_reduce(int nblk, const float* __restrict__ hpart, float* __restrict__ grad) {
+    const int c = threadIdx.x;
+    if (c >= 17) return;
+    float s = 0.f;
+    for (int k = 0; k < nblk; ++k) s += hpart[(size_t)k * 17 + c];
+    grad[c] = s;
+}
+
+// weights for the kernels from the fp32 master:
+// fwd[tap][co][cinp] = W[co][ci][tap]; bwd[tap][ci][cout] = W[co][ci][8 - tap]
+__global__ void k_pack_weights(int cin, int cinp, int cout, const float* __restrict__ w,
+                               uint16_t* __restrict__ fwd, uint16_t* __restrict__ bwd) {
+    const int n = 9 * cout * cinp;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int ci = e % cinp, co = (e / cinp) % cout, tap = e / (cinp * cout);
+        fwd[e] = ci < cin ? f2bf(w[((size_t)co * cin + ci) * 9 + tap]) : (uint16_t)0;
+    }
+    if (!bwd) return;
+    const int m = 9 * cin * cout;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const int co = e % cout, ci = (e / cout) % cin, tap = e / (cout * cin);
+        bwd[e] = f2bf(w[((size_t)co * cin + ci) * 9 + (8 - tap)]);
+    }
+}
+
+__global__ void k_head_pack(const float* __restrict__ w, float* __restrict__ hw) {
+    const int c = threadIdx.x;
+    if (c < 16) hw[c] = bf2f(f2bf(w[c]));
+    if (c == 16) hw[16] = w[16];
+}
+
+__global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+static int grid1(size_t n) {
+    size_t g = (n + 255) / 256;
+    return (int)std::max<size_t>(1, std::min<size_t>(g, 16384));
+}
+
+}  // namespace sg
+
+using namespace sg;
+
+struct sg_unet {
+    sg_context* ctx = nullptr;
+    int np = 0;
+    DevBuf<float> p, g, m, v;           // master params, gradient, Adam moments
+    DevBuf<uint16_t> wf[kUConvs], wb[kUConvs];
+    DevBuf<float> hw;                   // head weights (bf16-rounded) + bias
+    bool packed = false;
+    long long adam_t = 0;
+    // activations for the current shape
+    int B = 0, H = 0, W = 0;
+    DevBuf<uint16_t> x, a0, cat0, p0, a2, cat1, p1, a4, cat2, p2, a6, a7, a8, a9, a10, a11, a12, a13;
+    DevBuf<uint16_t> gx, ga0, gcat0, gp0, ga2, gcat1, gp1, ga4, gcat2, gp2, ga6, ga7, ga8, ga9, ga10,
+        ga11, ga12, ga13;
+    DevBuf<float> out, dwpart, dwpart_b, hpart;
+    DevBuf<double> lpart, lsums, lloss;
+    bool grads_alloc = false;
+};
+
+namespace sg {
+
+static void unet_pack(sg_unet* U) {
+    if (U->packed) return;
+    hipStream_t s = U->ctx->stream;
+    for (int i = 0; i < kUConvs; ++i) {
+        const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci);
+        if (!U->wf[i].p) U->wf[i].alloc((size_t)9 * co * cp);
+        if (i > 0 && !U->wb[i].p) U->wb[i].alloc((size_t)9 * ci * co);
+        hipLaunchKernelGGL(k_pack_weights, dim3(grid1((size_t)9 * co * cp)), dim3(256), 0, s, ci, cp, co,
+                           U->p.p + uconv_offset(i), U->wf[i].p, i > 0 ? U->wb[i].p : nullptr);
+    }
+    if (!U->hw.p) U->hw.alloc(17);
+    hipLaunchKernelGGL(k_head_pack, dim3(1), dim3(64), 0, s, U->p.p + uconv_offset(kUConvs), U->hw.p);
+    SG_HIP(hipGetLastError());
+    U->packed = true;
+}
+
+static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
+    if (B < 1 || H < 8 || W < 8 || H % 8 || W % 8) fail(SG_E_ARG, "network input needs H, W multiples of 8");
+    const size_t hw0 = (size_t)H * W, hw1 = hw0 / 4, hw2 = hw0 / 16, hw3 = hw0 / 64;
+    if (U->B != B || U->H != H || U->W != W) {
+        U->B = B, U->H = H, U->W = W;
+        U->grads_alloc = false;
+        auto A = [&](DevBuf<uint16_t>& d, int c, size_t hw) { d.alloc((size_t)B * c * hw); };
+        A(U->x, 16, hw0), A(U->a0, 16, hw0), A(U->cat0, 48, hw0), A(U->p0, 16, hw1);
+        A(U->a2, 32, hw1), A(U->cat1, 96, hw1), A(U->p1, 32, hw2), A(U->a4, 64, hw2);
+        A(U->cat2, 128, hw2), A(U->p2, 64, hw3), A(U->a6, 64, hw3), A(U->a7, 64, hw3);
+        A(U->a8, 64, hw2), A(U->a9, 64, hw2), A(U->a10, 32, hw1), A(U->a11, 32, hw1);
+        A(U->a12, 16, hw0), A(U->a13, 16, hw0);
+        U->out.alloc((size_t)B * hw0);
+        U->lpart.alloc((size_t)B * kLB * 2);
+        U->lsums.alloc((size_t)B * 2);
+        U->lloss.alloc(1);
+    }
+    if (grads && !U->grads_alloc) {
+        auto A = [&](DevBuf<uint16_t>& d, int c, size_t hw) { d.alloc((size_t)B * c * hw); };
+        A(U->ga0, 16, hw0), A(U->gcat0, 48, hw0), A(U->gp0, 16, hw1);
+        A(U->ga2, 32, hw1), A(U->gcat1, 96, hw1), A(U->gp1, 32, hw2), A(U->ga4, 64, hw2);
+        A(U->gcat2, 128, hw2), A(U->gp2, 64, hw3), A(U->ga6, 64, hw3), A(U->ga7, 64, hw3);
+        A(U->ga8, 64, hw2), A(U->ga9, 64, hw2), A(U->ga10, 32, hw1), A(U->ga11, 32, hw1);
+        A(U->ga12, 16, hw0), A(U->ga13, 16, hw0);
+        // weight-gradient partials: the largest tiles x 9 cout cin over all convs
+        size_t mx = 0, mxb = 0;
+        int h = H, w = W;
+        const int lvl[kUConvs] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0};
+        for (int i = 0; i < kUConvs; ++i) {
+            h = H >> lvl[i], w = W >> lvl[i];
+            const size_t tiles = (size_t)B * ((h + kWY - 1) / kWY) * ((w + kWX - 1) / kWX);
+            mx = std::max(mx, tiles * 9 * kUSpec[i].cout * kUSpec[i].cin);
+            mxb = std::max(mxb, tiles * kUSpec[i].cout);
+        }
+        U->dwpart.alloc(mx);
+        U->dwpart_b.alloc(mxb);
+        U->hpart.alloc((size_t)1024 * 17);
+        U->grads_alloc = true;
+    }
+}
+
+static View V(DevBuf<uint16_t>& d, int ctot, int c0 = 0) { return View{d.p, ctot, c0}; }
+
+static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W) {
+    const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
+    const int tiles = U->B * ((H + kCY - 1) / kCY) * ((W + kCX - 1) / kCX);
+    hipLaunchKernelGGL(k_conv3x3<0>, dim3(tiles, (co + 63) / 64), dim3(256), 0, U->ctx->stream, in,
+                       ci, cin_pad(ci), U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co,
+                       H, W);
+}
+
+static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
+    const int ci = kUSpec[i].cin, co = kUSpec[i].cout;  // dx has ci channels
+    const int tiles = U->B * ((H + kCY - 1) / kCY) * ((W + kCX - 1) / kCX);
+    hipLaunchKernelGGL(k_conv3x3<1>, dim3(tiles, (ci + 63) / 64), dim3(256), 0, U->ctx->stream, dp,
+                       co, co, U->wb[i].p, (const float*)nullptr, dx, ci, H, W);
+}
+
+static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
+    const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
+    const int tiles = U->B * ((H + kWY - 1) / kWY) * ((W + kWX - 1) / kWX);
+    hipLaunchKernelGGL(k_conv_dw, dim3(tiles, co / 16, (ci + 15) / 16), dim3(256), 0, U->ctx->stream,
+                       in, ci, dp, co, U->B, H, W, U->dwpart.p, U->dwpart_b.p);
+    hipLaunchKernelGGL(k_dw_reduce, dim3(grid1((size_t)9 * co * ci + co)), dim3(256), 0,
+                       U->ctx->stream, tiles, ci, co, U->dwpart.p, U->dwpart_b.p,
+                       U->g.p + uconv_offset(i));
+}
+
+void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads) {
+    unet_pack(U);
+    unet_shape(U, B, H, W, grads);
+    hipStream_t s = U->ctx->stream;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    const size_t hw0 = (size_t)H * W;
+    hipLaunchKernelGGL(k_input, dim3(grid1(B * 16 * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16));
+    conv_fwd(U, 0, V(U->x, 16), V(U->a0, 16), H, W);
+    conv_fwd(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W);  // s0
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 16 * hw0 / 4)), dim3(256), 0, s, B, 16, H, W, V(U->cat0, 48, 32),
+                       V(U->p0, 16));
+    conv_fwd(U, 2, V(U->p0, 16), V(U->a2, 32), H1, W1);
+    conv_fwd(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1);  // s1
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 32 * hw0 / 16)), dim3(256), 0, s, B, 32, H1, W1,
+                       V(U->cat1, 96, 64), V(U->p1, 32));
+    conv_fwd(U, 4, V(U->p1, 32), V(U->a4, 64), H2, W2);
+    conv_fwd(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2);  // s2
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 64 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
+                       V(U->cat2, 128, 64), V(U->p2, 64));
+    conv_fwd(U, 6, V(U->p2, 64), V(U->a6, 64), H3, W3);
+    conv_fwd(U, 7, V(U->a6, 64), V(U->a7, 64), H3, W3);
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 64 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2, V(U->a7, 64),
+                       V(U->cat2, 128, 0));
+    conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2);
+    conv_fwd(U, 9, V(U->a8, 64), V(U->a9, 64), H2, W2);
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 64 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1, V(U->a9, 64),
+                       V(U->cat1, 96, 0));
+    conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1);
+    conv_fwd(U, 11, V(U->a10, 32), V(U->a11, 32), H1, W1);
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 32 * hw0)), dim3(256), 0, s, B, 32, H, W, V(U->a11, 32),
+                       V(U->cat0, 48, 0));
+    conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W);
+    conv_fwd(U, 13, V(U->a12, 16), V(U->a13, 16), H, W);
+    hipLaunchKernelGGL(k_head, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, V(U->a13, 16), U->hw.p,
+                       U->out.p);
+    SG_HIP(hipGetLastError());
+}
+
+void unet_backward_dev(sg_unet* U, const float* d_target) {
+    const int B = U->B, H = U->H, W = U->W;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    const size_t hw0 = (size_t)H * W;
+    hipStream_t s = U->ctx->stream;
+    hipLaunchKernelGGL(k_loss_part, dim3(kLB, B), dim3(256), 0, s, H, W, U->out.p, d_target, U->lpart.p);
+    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, s, B, U->lpart.p, U->lsums.p, U->lloss.p);
+    const int hb = std::min(1024, grid1(B * hw0));
+    hipLaunchKernelGGL(k_head_bwd, dim3(hb), dim3(256), 0, s, B, H, W, U->out.p, d_target, U->lsums.p,
+                       V(U->a13, 16), U->hw.p, V(U->ga13, 16), U->hpart.p);
+    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, s, hb, U->hpart.p, U->g.p + uconv_offset(kUConvs));
+    auto mask = [&](DevBuf<uint16_t>& y, int yc, int yc0, DevBuf<uint16_t>& dy, int dc, int dc0, int C,
+                    int h, int w) {
+        hipLaunchKernelGGL(k_relu_mask, dim3(grid1((size_t)B * C * h * w)), dim3(256), 0, s, B, C, h, w,
+                           V(y, yc, yc0), V(dy, dc, dc0));
+    };
+    // conv i: dy (gradient w.r.t. its ReLU output) is masked in place, then the
+    // weight gradient from its input, then the data gradient into dx
+    // dec0
+    mask(U->a13, 16, 0, U->ga13, 16, 0, 16, H, W);
+    conv_dw(U, 13, V(U->a12, 16), V(U->ga13, 16), H, W);
+    conv_dx(U, 13, V(U->ga13, 16), V(U->ga12, 16), H, W);
+    mask(U->a12, 16, 0, U->ga12, 16, 0, 16, H, W);
+    conv_dw(U, 12, V(U->cat0, 48), V(U->ga12, 16), H, W);
+    conv_dx(U, 12, V(U->ga12, 16), V(U->gcat0, 48), H, W);
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid1(B * 32 * hw0 / 4)), dim3(256), 0, s, B, 32, H, W,
+                       V(U->gcat0, 48, 0), V(U->ga11, 32));
+    // dec1
+    mask(U->a11, 32, 0, U->ga11, 32, 0, 32, H1, W1);
+    conv_dw(U, 11, V(U->a10, 32), V(U->ga11, 32), H1, W1);
+    conv_dx(U, 11, V(U->ga11, 32), V(U->ga10, 32), H1, W1);
+    mask(U->a10, 32, 0, U->ga10, 32, 0, 32, H1, W1);
+    conv_dw(U, 10, V(U->cat1, 96), V(U->ga10, 32), H1, W1);
+    conv_dx(U, 10, V(U->ga10, 32), V(U->gcat1, 96), H1, W1);
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid1(B * 64 * hw0 / 16)), dim3(256), 0, s, B, 64, H1, W1,
+                       V(U->gcat1, 96, 0), V(U->ga9, 64));
+    // dec2
+    mask(U->a9, 64, 0, U->ga9, 64, 0, 64, H2, W2);
+    conv_dw(U, 9, V(U->a8, 64), V(U->ga9, 64), H2, W2);
+    conv_dx(U, 9, V(U->ga9, 64), V(U->ga8, 64), H2, W2);
+    mask(U->a8, 64, 0, U->ga8, 64, 0, 64, H2, W2);
+    conv_dw(U, 8, V(U->cat2, 128), V(U->ga8, 64), H2, W2);
+    conv_dx(U, 8, V(U->ga8, 64), V(U->gcat2, 128), H2, W2);
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid1(B * 64 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
+                       V(U->gcat2, 128, 0), V(U->ga7, 64));
+    // bottleneck
+    mask(U->a7, 64, 0, U->ga7, 64, 0, 64, H3, W3);
+    conv_dw(U, 7, V(U->a6, 64), V(U->ga7, 64), H3, W3);
+    conv_dx(U, 7, V(U->ga7, 64), V(U->ga6, 64), H3, W3);
+    mask(U->a6, 64, 0, U->ga6, 64, 0, 64, H3, W3);
+    conv_dw(U, 6, V(U->p2, 64), V(U->ga6, 64), H3, W3);
+    conv_dx(U, 6, V(U->ga6, 64), V(U->gp2, 64), H3, W3);
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1(B * 64 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2,
+                       V(U->gp2, 64), V(U->gcat2, 128, 64));
+    // enc2 (s2 = cat2[64..128), its gradient gcat2[64..128))
+    mask(U->cat2, 128, 64, U->gcat2, 128, 64, 64, H2, W2);
+    conv_dw(U, 5, V(U->a4, 64), V(U->gcat2, 128, 64), H2, W2);
+    conv_dx(U, 5, V(U->gcat2, 128, 64), V(U->ga4, 64), H2, W2);
+    mask(U->a4, 64, 0, U->ga4, 64, 0, 64, H2, W2);
+    conv_dw(U, 4, V(U->p1, 32), V(U->ga4, 64), H2, W2);
+    conv_dx(U, 4, V(U->ga4, 64), V(U->gp1, 32), H2, W2);
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1(B * 32 * hw0 / 4)), dim3(256), 0, s, B, 32, H1, W1,
+                       V(U->gp1, 32), V(U->gcat1, 96, 64));
+    // enc1
+    mask(U->cat1, 96, 64, U->gcat1, 96, 64, 32, H1, W1);
+    conv_dw(U, 3, V(U->a2, 32), V(U->gcat1, 96, 64), H1, W1);
+    conv_dx(U, 3, V(U->gcat1, 96, 64), V(U->ga2, 32), H1, W1);
+    mask(U->a2, 32, 0, U->ga2, 32, 0, 32, H1, W1);
+    conv_dw(U, 2, V(U->p0, 16), V(U->ga2, 32), H1, W1);
+    conv_dx(U, 2, V(U->ga2, 32), V(U->gp0, 16), H1, W1);
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1(B * 16 * hw0)), dim3(256), 0, s, B, 16, H, W,
+                       V(U->gp0, 16), V(U->gcat0, 48, 32));
+    // enc0
+    mask(U->cat0, 48, 32, U->gcat0, 48, 32, 16, H, W);
+    conv_dw(U, 1, V(U->a0, 16), V(U->gcat0, 48, 32), H, W);
+    conv_dx(U, 1, V(U->gcat0, 48, 32), V(U->ga0, 16), H, W);
+    mask(U->a0, 16, 0, U->ga0, 16, 0, 16, H, W);
+    conv_dw(U, 0, V(U->x, 16), V(U->ga0, 16), H, W);
+    SG_HIP(hipGetLastError());
+}
+
+// the network output of the last forward pass (device, [B][H][W] fp32)
+const float* unet_output(const sg_unet* U) { return U->out.p; }
+sg_context* unet_ctx(const sg_unet* U) { return U->ctx; }
+
+}  // namespace sg
+
+extern "C" {
+
+int sg_unet_nparams(void) { return unet_nparams_(); }
+
+sg_status sg_unet_create(sg_context* ctx, sg_unet** out) {
+    if (!ctx || !out) return SG_E_ARG;
+    *out = nullptr;
+    sg_unet* U = new sg_unet();
+    sg_status st = run_guarded(ctx, [&] {
+        SG_HIP(hipSetDevice(ctx->device));
+        U->ctx = ctx;
+        U->np = unet_nparams_();
+        U->p.alloc(U->np);
+        U->g.alloc(U->np);
+        U->m.alloc(U->np);
+        U->v.alloc(U->np);
+        SG_HIP(hipMemsetAsync(U->p.p, 0, sizeof(float) * U->np, ctx->stream));
+        SG_HIP(hipMemsetAsync(U->g.p, 0, sizeof(float) * U->np, ctx->stream));
+        SG_HIP(hipMemsetAsync(U->m.p, 0, sizeof(float) * U->np, ctx->stream));
+        SG_HIP(hipMemsetAsync(U->v.p, 0, sizeof(float) * U->np, ctx->stream));
+        SG_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (st != SG_OK) {
+        delete U;
+        return st;
+    }
+    *out = U;
+    return SG_OK;
+}
+
+sg_status sg_unet_destroy(sg_unet* U) {
+    if (!U) return SG_E_ARG;
+    (void)hipStreamSynchronize(U->ctx->stream);
+    delete U;
+    return SG_OK;
+}
+
+sg_status sg_unet_set_params(sg_unet* U, const float* params) {
+    if (!U || !params) return SG_E_ARG;
+    return run_guarded(U->ctx, [&] {
+        SG_HIP(hipMemcpyAsync(U->p.p, params, sizeof(float) * U->np, hipMemcpyHostToDevice, U->ctx->stream));
+        SG_HIP(hipMemsetAsync(U->m.p, 0, sizeof(float) * U->np, U->ctx->stream));
+        SG_HIP(hipMemsetAsync(U->v.p, 0, sizeof(float) * U->np, U->ctx->stream));
+        U->adam_t = 0;
+        U->packed = false;
+        SG_HIP(hipStreamSynchronize(U->ctx->stream));
+    });
+}
+
+sg_status sg_unet_get_params(const sg_unet* U, float* params) {
+    if (!U || !params) return SG_E_ARG;
+    return run_guarded(U->ctx, [&] {
+        SG_HIP(hipMemcpyAsync(params, U->p.p, sizeof(float) * U->np, hipMemcpyDeviceToHost, U->ctx->stream));
+        SG_HIP(hipStreamSynchronize(U->ctx->stream));
+    });
+}
+
+sg_status sg_unet_forward(sg_unet* U, int B, int H, int W, const float* d_in, float* d_out) {
+    if (!U || !d_in || !d_out) return SG_E_ARG;
+    return run_guarded(U->ctx, [&] {
+        unet_forward_dev(U, B, H, W, d_in, false);
+        SG_HIP(hipMemcpyAsync(d_out, U->out.p, sizeof(float) * B * H * W, hipMemcpyDeviceToDevice,
+                              U->ctx->stream));
+    });
+}
+
+sg_status sg_unet_loss_grad(sg_unet* U, int B, int H, int W, const float* d_in, const float* d_target,
+                            double* loss, float* grad) {
+    if (!U || !d_in || !d_target) return SG_E_ARG;
+    return run_guarded(U->ctx, [&] {
+        unet_forward_dev(U, B, H, W, d_in, true);
+        unet_backward_dev(U, d_target);
+        if (loss)
+            SG_HIP(hipMemcpyAsync(loss, U->lloss.p, sizeof(double), hipMemcpyDeviceToHost, U->ctx->stream));
+        if (grad)
+            SG_HIP(hipMemcpyAsync(grad, U->g.p, sizeof(float) * U->np, hipMemcpyDeviceToHost, U->ctx->stream));
+        SG_HIP(hipStreamSynchronize(U->ctx->stream));
+    });
+}
+
+sg_status sg_unet_adam_step(sg_unet* U, float lr) {
+    if (!U || !(lr > 0.f)) return SG_E_ARG;
+    return run_guarded(U->ctx, [&] {
+        U->adam_t += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)U->adam_t), c2 = 1.f - std::pow(b2, (float)U->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3(grid1(U->np)), dim3(256), 0, U->ctx->stream, U->np, U->p.p, U->g.p,
+                           U->m.p, U->v.p, lr, b1, b2, 1e-8f, c1, c2);
+        SG_HIP(hipGetLastError());
+        U->packed = false;
+    });
+}
+
+}  // extern "C"

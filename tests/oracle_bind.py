@@ -38,6 +38,13 @@ def lib():
                                      C.c_void_p, C.c_int]
         L.or_precond_new.restype = C.c_void_p
         L.or_precond_apply.argtypes = [C.c_void_p, _D, _D]
+        _F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.or_unet_nparams.restype = C.c_int
+        L.or_unet_param_offset.argtypes = [C.c_int]
+        L.or_unet_param_offset.restype = C.c_int
+        L.or_unet_forward.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F, _F, C.c_int]
+        L.or_unet_loss_grad.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F, _F, C.c_int, _D]
+        L.or_unet_loss_grad.restype = C.c_double
         L.or_precond_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_precond_free.argtypes = [C.c_void_p]
         L.or_lu_factor.argtypes = [C.c_int, _D, _I]
@@ -199,3 +206,32 @@ def metrics_json(**kw):
                           f["iterations"], f["tol"], f["residual_norm"], f["time_setup_s"],
                           f["time_solve_s"], f["speedup"], f["efficiency"], buf, 4096)
     return buf.value.decode()
+
+
+# ---- learned-stage network (oracle/unet_oracle.cpp) ----
+
+def unet_nparams():
+    return lib().or_unet_nparams()
+
+
+def unet_param_offset(i):
+    return lib().or_unet_param_offset(i)
+
+
+def unet_forward(params, x, round_bf16=True):
+    """x: [B, 2, H, W] float32 -> [B, H, W]."""
+    x = np.ascontiguousarray(x, np.float32)
+    B, _, H, W = x.shape
+    out = np.zeros((B, H, W), np.float32)
+    lib().or_unet_forward(np.ascontiguousarray(params, np.float32), B, H, W, x, out,
+                          1 if round_bf16 else 0)
+    return out
+
+
+def unet_loss_grad(params, x, t, round_bf16=True):
+    x = np.ascontiguousarray(x, np.float32)
+    B, _, H, W = x.shape
+    g = np.zeros(unet_nparams())
+    loss = lib().or_unet_loss_grad(np.ascontiguousarray(params, np.float32), B, H, W, x,
+                                   np.ascontiguousarray(t, np.float32), 1 if round_bf16 else 0, g)
+    return loss, g
