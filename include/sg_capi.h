@@ -89,7 +89,8 @@ typedef enum {
     SG_PC_ASM = 3,
     SG_PC_RAS = 4,
     SG_PC_TWOLEVEL_ASM = 5,
-    SG_PC_TWOLEVEL_RAS = 6
+    SG_PC_TWOLEVEL_RAS = 6,
+    SG_PC_LEARNED = 7 /* made by sg_precond_create_learned only */
 } sg_precond_kind;
 
 typedef enum { SG_FP64 = 0, SG_FP32 = 1 } sg_precision;
@@ -259,6 +260,27 @@ sg_status sg_unet_loss_grad(sg_unet* net, int B, int H, int W, const float* d_in
                             const float* d_target, double* loss, float* grad);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
 sg_status sg_unet_adam_step(sg_unet* net, float lr);
+
+/* ---- learned FGMRES stage ------------------------------------------------------
+ * No counterpart in the reference: a flexible preconditioner for FGMRES
+ * (fgmres_flex's FlexApply role, src/core/fgmres.h:37-55) on a 2D grid.  Each
+ * application, on the device:  z_b = base(x);  s = x - A z_b;
+ * e = rms(s) * net(s / rms(s), log kappa);  alpha = (s . A e) / (A e . A e);
+ * z = z_b + alpha e when that removes at least min_gain of ||s||^2, else
+ * z = z_b (the base two-level Schwarz step is the fallback).  Because z
+ * changes from step to step it is valid with SG_FGMRES only.  base and net
+ * are not owned and must outlive the stage; sg_precond_setup(stage, A) needs
+ * base already set up on A. */
+typedef struct {
+    int nx, ny;      /* grid: row y*nx + x; nx and ny multiples of 8            */
+    double min_gain; /* 0 -> 0.01                                               */
+    int every;       /* run the network on every every-th application (0 -> 1) */
+} sg_learned_opts;
+sg_status sg_precond_create_learned(sg_context* ctx, sg_precond* base, sg_unet* net,
+                                    const double* log_kappa /* host, nx*ny */,
+                                    const sg_learned_opts* opts, sg_precond** out);
+/* applications so far, and how many of them kept the network's correction */
+sg_status sg_precond_learned_stats(const sg_precond* M, long long* calls, long long* accepted);
 
 /* ---- kernel timing (HIP events on the context stream) ------------------------
  * With profiling on, every launch of the listed kernel families is bracketed

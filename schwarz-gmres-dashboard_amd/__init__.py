@@ -50,6 +50,10 @@ class SolveResult(C.Structure):
                 ("precond_applies", C.c_int)]
 
 
+class LearnedOpts(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("min_gain", C.c_double), ("every", C.c_int)]
+
+
 class PrecondInfo(C.Structure):
     _fields_ = [("nsub", C.c_int), ("n_ext", C.c_longlong), ("nnz_ext", C.c_longlong),
                 ("levels_fwd", C.c_longlong), ("levels_bwd", C.c_longlong),
@@ -100,6 +104,10 @@ def lib():
         "sg_solve": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult), _VP,
                       C.c_int], C.c_int),
         "sg_partition": ([C.c_int, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP], C.c_int),
+        "sg_precond_create_learned": ([_VP, _VP, _VP, _VP, C.POINTER(LearnedOpts),
+                                       C.POINTER(_VP)], C.c_int),
+        "sg_precond_learned_stats": ([_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+                                     C.c_int),
         "sg_unet_nparams": ([], C.c_int),
         "sg_unet_create": ([_VP, C.POINTER(_VP)], C.c_int),
         "sg_unet_destroy": ([_VP], C.c_int),
@@ -280,6 +288,42 @@ class UNet:
         try:
             if self.h and self.ctx.h:
                 lib().sg_unet_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class LearnedStage:
+    """Learned FGMRES stage (sg_precond_create_learned): base preconditioner +
+    network correction with residual-reduction fallback.  base and net must
+    outlive the stage (references are kept here)."""
+
+    def __init__(self, ctx, base, net, log_kappa, nx, ny, min_gain=0.01, every=1):
+        self.ctx, self.base, self.net = ctx, base, net
+        lk = np.ascontiguousarray(log_kappa, np.float64)
+        assert lk.size == nx * ny
+        o = LearnedOpts(nx, ny, min_gain, every)
+        self.h = _VP()
+        ctx.check(lib().sg_precond_create_learned(ctx.h, base.h, net.h, _p(lk), C.byref(o),
+                                                  C.byref(self.h)))
+        self.n = nx * ny
+
+    def setup(self, A):
+        self.ctx.check(lib().sg_precond_setup(self.h, A.h))
+        return self
+
+    def apply(self, x, y):
+        self.ctx.check(lib().sg_precond_apply(self.h, x.ptr, y.ptr))
+
+    def counts(self):
+        c, a = C.c_longlong(), C.c_longlong()
+        self.ctx.check(lib().sg_precond_learned_stats(self.h, C.byref(c), C.byref(a)))
+        return c.value, a.value
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().sg_precond_destroy(self.h)
                 self.h = None
         except Exception:
             pass

@@ -667,6 +667,8 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     if (n < 1) fail(SG_E_ARG, "empty system");
     const std::vector<int>& off = A->sys_off;
     const int nsys = (int)off.size() - 1;
+    if (M && method == 0 && precond_is_flexible(M))
+        fail(SG_E_ARG, "a flexible (learned) stage needs FGMRES");
     if (M && precond_is_callback(M) && (use_graph || nsys > 1))
         fail(SG_E_ARG, "a host-callback preconditioner runs only in unbatched, uncaptured solves");
     const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;

@@ -148,6 +148,11 @@ struct sg_precond {
     // batch binding: subdomain -> system (-1: spans systems) for sys_off
     std::vector<int> bound_sys;
     sg::DevBuf<int> d_sub_sys;
+    // learned FGMRES stage around a base preconditioner (not owned)
+    sg::LearnedStage* learned = nullptr;
+    ~sg_precond() {
+        if (learned) sg::learned_free(learned);
+    }
 };
 
 namespace sg {
@@ -1386,6 +1391,10 @@ using namespace sg;
 namespace sg {
 
 void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard) {
+    if (M->learned) {
+        learned_apply(M->learned, x, y, guard);
+        return;
+    }
     sg_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     ProfScope ps(ctx, SG_K_PRECOND);
@@ -1438,6 +1447,7 @@ void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard
 }
 
 void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off) {
+    if (M->learned) fail(SG_E_ARG, "the learned stage does not run in batched solves");
     if (M->bound_sys == sys_off) return;
     std::vector<int> map(std::max<size_t>(M->dofs.size(), 1), -1);
     for (size_t q = 0; q < M->dofs.size(); ++q) {
@@ -1457,6 +1467,10 @@ void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off) {
 }
 
 void precond_ensure_tuned(sg_precond* M) {
+    if (M->learned) {
+        if (M->ready) learned_prepare(M->learned);
+        return;
+    }
     if (M->fn || M->kind == SG_PC_NONE || M->kind == SG_PC_JACOBI || !M->ready) return;
     if (M->opts.precision == SG_FP32)
         tune_sweep<float>(M, M->f_v32.p, M->b_v32.p, M->b_d32.p, M->z32.p, M->zout32.p);
@@ -1464,7 +1478,11 @@ void precond_ensure_tuned(sg_precond* M) {
         tune_sweep<double>(M, M->f_v64.p, M->b_v64.p, M->b_d64.p, M->z64.p, M->zout64.p);
 }
 
-bool precond_is_callback(const sg_precond* M) { return M->fn != nullptr; }
+bool precond_is_callback(const sg_precond* M) {
+    return M->fn != nullptr || (M->learned && precond_is_callback(learned_base(M->learned)));
+}
+
+bool precond_is_flexible(const sg_precond* M) { return M->learned != nullptr; }
 
 int precond_n(const sg_precond* M) { return M->n; }
 sg_context* precond_ctx(const sg_precond* M) { return M->ctx; }
@@ -1495,6 +1513,32 @@ sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_prec
     return SG_OK;
 }
 
+sg_status sg_precond_create_learned(sg_context* ctx, sg_precond* base, sg_unet* net,
+                                    const double* log_kappa, const sg_learned_opts* opts,
+                                    sg_precond** out) {
+    if (!ctx || !base || !net || !log_kappa || !opts || !out) return SG_E_ARG;
+    *out = nullptr;
+    if (base->learned) return SG_E_ARG;
+    sg_precond* M = new sg_precond();
+    sg_status st = run_guarded(ctx, [&] {
+        M->ctx = ctx;
+        M->kind = SG_PC_LEARNED;
+        M->learned = learned_new(ctx, base, net, opts->nx, opts->ny, log_kappa, opts);
+        M->n = opts->nx * opts->ny;
+    });
+    if (st != SG_OK) {
+        delete M;
+        return st;
+    }
+    *out = M;
+    return SG_OK;
+}
+
+sg_status sg_precond_learned_stats(const sg_precond* M, long long* calls, long long* accepted) {
+    if (!M || !M->learned) return SG_E_ARG;
+    return run_guarded(M->ctx, [&] { learned_counts(M->learned, calls, accepted); });
+}
+
 sg_status sg_precond_create_callback(sg_context* ctx, int n, sg_apply_fn fn, void* user,
                                      int device_pointers, sg_precond** out) {
     if (!ctx || !out || !fn || n < 0) return SG_E_ARG;
@@ -1513,6 +1557,12 @@ sg_status sg_precond_create_callback(sg_context* ctx, int n, sg_apply_fn fn, voi
 sg_status sg_precond_setup(sg_precond* M, const sg_csr* A) {
     if (!M || !A) return SG_E_ARG;
     if (M->fn) return A->nrows == M->n ? SG_OK : SG_E_ARG;
+    if (M->learned)
+        return run_guarded(M->ctx, [&] {
+            learned_setup(M->learned, A);
+            M->n = A->nrows;
+            M->ready = true;
+        });
     sg_status st = run_guarded(M->ctx, [&] { setup(M, A); });
     M->opts.partition = nullptr;  // the caller's array need not outlive setup
     return st;
@@ -1593,6 +1643,7 @@ sg_status sg_partition(int n, const int* row_ptr, const int* col_idx, const doub
 
 sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info) {
     if (!M || !info) return SG_E_ARG;
+    if (M->learned) return sg_precond_stats(learned_base(M->learned), info);
     if (!M->ready) return SG_E_STATE;
     *info = sg_precond_info{};
     info->nsub = M->nsub;

@@ -154,6 +154,20 @@ void spmv_ensure_tuned(sg_csr* A);
 // result[0] = dot(a, b) on the device (deterministic tree), stream-ordered.
 void dot_launch(sg_context* ctx, int n, const double* a, const double* b, double* result);
 
+// ---- learned stage (unet.hip, learned.hip) ---------------------------------------
+void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads);
+const float* unet_output(const sg_unet* U);
+void unet_prepare(sg_unet* U, int B, int H, int W);  // pack weights, size buffers
+struct LearnedStage;
+LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int nx, int ny,
+                          const double* log_kappa, const sg_learned_opts* opts);
+void learned_free(LearnedStage* L);
+void learned_setup(LearnedStage* L, const sg_csr* A);
+void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& guard);
+void learned_prepare(LearnedStage* L);
+void learned_counts(const LearnedStage* L, long long* calls, long long* accepted);
+sg_precond* learned_base(const LearnedStage* L);
+
 // ---- preconditioner apply (precond.hip) ----------------------------------------
 void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard);
 // Maps each subdomain to the system of a block-diagonal batch that holds it
@@ -161,5 +175,8 @@ void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard
 void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off);
 void precond_ensure_tuned(sg_precond* M);
 bool precond_is_callback(const sg_precond* M);
+bool precond_is_flexible(const sg_precond* M);  // changes per application: FGMRES only
+bool precond_ready(const sg_precond* M);
+int precond_n(const sg_precond* M);
 
 }  // namespace sg

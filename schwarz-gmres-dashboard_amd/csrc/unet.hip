@@ -694,6 +694,11 @@ void unet_backward_dev(sg_unet* U, const float* d_target) {
     SG_HIP(hipGetLastError());
 }
 
+void unet_prepare(sg_unet* U, int B, int H, int W) {
+    unet_pack(U);
+    unet_shape(U, B, H, W, false);
+}
+
 // the network output of the last forward pass (device, [B][H][W] fp32)
 const float* unet_output(const sg_unet* U) { return U->out.p; }
 sg_context* unet_ctx(const sg_unet* U) { return U->ctx; }

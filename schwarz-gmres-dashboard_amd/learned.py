@@ -1,0 +1,184 @@
+"""Training data and training of the learned FGMRES stage's network.
+
+Pairs come from the plain two-level RAS-FGMRES solve itself.  For each seeded
+field (kappa = exp(g), g a Gaussian random field on a side x side grid, b from
+the same seed) the batched solver runs `per_field` restart cycles of
+`restart` steps from x0 = 0, keeping each iterate x_k, then converges to
+||b - A x|| <= 1e-10 ||b|| for x*.  Pair k: r_k = b - A x_k, e_k = x* - x_k
+(A e_k = r_k up to the 1e-10 solve).  Network input (r_k / rms(r_k), g),
+target e_k / rms(r_k).  Every field shares one unweighted partition of the
+grid (`nparts` subdomains, overlap 2) and FP32 ILU(0) with a Nicolaides
+coarse space.
+
+The network is trained with Adam on the relative squared error of e, on
+batches of whole fields, with a cosine learning-rate decay.
+
+    python learned.py --fields 2000 --epochs 8 --out weights.npy
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+CONVS = [(2, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 64), (64, 64),
+         (128, 64), (64, 64), (96, 32), (32, 32), (48, 16), (16, 16)]
+
+
+def init_params(seed):
+    """He-normal weights, small biases, in the packed parameter order."""
+    rng = np.random.default_rng(seed)
+    p = []
+    for cin, cout in CONVS:
+        p.append(rng.standard_normal(cout * cin * 9) * np.sqrt(2.0 / (9 * cin)))
+        p.append(np.zeros(cout))
+    p.append(rng.standard_normal(16) * 0.1)
+    p.append(np.zeros(1))
+    return np.concatenate(p).astype(np.float32)
+
+
+def load_problems():
+    spec = importlib.util.spec_from_file_location("sgamd_problems", os.path.join(HERE, "problems.py"))
+    P = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(P)
+    return P
+
+
+def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10, restart=10,
+                   sigma=1.5, corr=0.05, nparts=16, batch=32, log=None):
+    """Returns X [N, 2, side, side] and T [N, side, side] (float32, N = nfields * per_field)."""
+    n = side * side
+    rp0, ci0, _ = P.poisson2d(side, side)
+    part1 = sg.partition(rp0, ci0, nparts).astype(np.int32)
+    N = nfields * per_field
+    X = np.empty((N, 2, side, side), np.float32)
+    T = np.empty((N, side, side), np.float32)
+    k = 0
+    t0 = time.perf_counter()
+    for f0 in range(0, nfields, batch):
+        seeds = list(range(base_seed + f0, base_seed + min(nfields, f0 + batch)))
+        mats, gs, bs = [], [], []
+        for s in seeds:
+            g = P.grf(side, sigma, corr, s)
+            mats.append(P.diffusion2d(np.exp(g)))
+            gs.append(g)
+            bs.append(P.normal(s + 7, n))
+        ns = len(seeds)
+        B = sg.BatchCsr(ctx, mats)
+        part = np.concatenate([part1 + q * nparts for q in range(ns)]).astype(np.int32)
+        M = sg.Precond(ctx, "twolevel_ras", nparts=nparts * ns, overlap=2, precision="fp32",
+                       partition=part, coarse_solve=1).setup(B)
+        b = np.concatenate(bs)
+        db = sg.DeviceArray.from_host(ctx, b)
+        dx = sg.DeviceArray(ctx, B.n)
+        dr = sg.DeviceArray(ctx, B.n)
+        xs, rs = [], []
+        for _ in range(per_field):
+            sg.solve_batch(B, db, dx, M, method="fgmres", restart=restart, max_iter=1, tol=0.0,
+                           ortho="cgs2")
+            xs.append(dx.get())
+            dr.upload(b)
+            B.spmv(dx, dr, alpha=-1.0, beta=1.0)
+            rs.append(dr.get())
+        res = sg.solve_batch(B, db, dx, M, method="fgmres", restart=30, max_iter=2000, rtol=1e-10,
+                             ortho="cgs2")
+        assert all(r["converged"] for r in res)
+        xstar = dx.get()
+        for q in range(ns):
+            sl = slice(q * n, (q + 1) * n)
+            for j in range(per_field):
+                r = rs[j][sl]
+                rms = np.sqrt(np.mean(r * r))
+                X[k, 0] = (r / rms).reshape(side, side)
+                X[k, 1] = gs[q]
+                T[k] = ((xstar[sl] - xs[j][sl]) / rms).reshape(side, side)
+                k += 1
+        del M, B
+        if log:
+            log(f"# pairs: {k}/{N} ({time.perf_counter() - t0:.1f} s)")
+    return X, T
+
+
+class _Ptr:
+    def __init__(self, p):
+        self.ptr = p
+
+
+def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0.05, log=None):
+    """Adam on device-resident data; returns (params, history)."""
+    import ctypes as C
+    N, _, H, W = X.shape
+    nval = max(1, int(round(val * N))) if N > 1 else 0
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(N)
+    X, T = X[perm], T[perm]
+    dX = sg.DeviceArray.from_host(ctx, X.reshape(-1))
+    dT = sg.DeviceArray.from_host(ctx, T.reshape(-1))
+    net = sg.UNet(ctx, init_params(seed) if params is None else params)
+    xs, ts = 2 * H * W * 4, H * W * 4
+
+    def views(i, b):
+        i = int(i)
+        return (_Ptr(C.c_void_p(dX.ptr.value + i * xs)), _Ptr(C.c_void_p(dT.ptr.value + i * ts)))
+
+    ntr = N - nval
+    nb = max(1, ntr // batch)
+    total = epochs * nb
+    hist = []
+    step = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(nb)
+        tl = 0.0
+        for j in order:
+            vx, vt = views(j * batch, batch)
+            tl += net.loss_grad_dev(batch, H, W, vx, vt)
+            a = lr * 0.5 * (1.0 + np.cos(np.pi * step / total))
+            net.adam(max(a, 1e-3 * lr))
+            step += 1
+        vl = 0.0
+        if nval:
+            for i in range(ntr, N, batch):
+                bb = min(batch, N - i)
+                vx, vt = views(i, bb)
+                vl += net.loss_grad_dev(bb, H, W, vx, vt) * bb
+            vl /= nval
+        hist.append({"epoch": ep + 1, "train_loss": tl / nb, "val_loss": vl,
+                     "s": round(time.perf_counter() - t0, 2)})
+        if log:
+            log(f"# epoch {ep + 1}: train {tl / nb:.4f} val {vl:.4f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    return net.get_params(), hist
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--fields", type=int, default=2000)
+    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--per-field", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(HERE))
+    import __graft_entry__
+    sg = __graft_entry__.load_package()
+    P = load_problems()
+    ctx = sg.Context(0)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field, log=log)
+    tgen = time.perf_counter() - t0
+    params, hist = train(sg, ctx, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    np.save(args.out, params)
+    print(json.dumps({"pairs": int(X.shape[0]), "generate_s": round(tgen, 1), "history": hist}))
+
+
+if __name__ == "__main__":
+    main()

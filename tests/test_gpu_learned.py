@@ -1,0 +1,128 @@
+"""The learned FGMRES stage (sg_precond_create_learned):
+
+* fallback: when no network correction is accepted the stage is exactly the
+  base two-level Schwarz preconditioner, so the FGMRES solve is bitwise the
+  plain one;
+* an accepted correction never increases the base step's residual;
+* it needs the flexible method;
+* a network trained on pairs from the plain solve (tiny run) is used by the
+  solve, which still reaches an FP64-verified 1e-8.
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sg():
+    import __graft_entry__
+    return __graft_entry__.load_package()
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import __graft_entry__
+    out = []
+    for name in ("problems", "learned"):
+        spec = importlib.util.spec_from_file_location(
+            "sgamd_" + name, os.path.join(__graft_entry__.PKG_DIR, name + ".py"))
+        m = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(m)
+        out.append(m)
+    return out
+
+
+@pytest.fixture(scope="module")
+def ctx(sg):
+    c = sg.Context(0)
+    yield c
+    c.close()
+
+
+def dev(sg, ctx, a):
+    return sg.DeviceArray.from_host(ctx, np.ascontiguousarray(a, np.float64))
+
+
+def grf_problem(P, side, seed):
+    g = P.grf(side, 1.5, 0.05, seed)
+    rp, ci, v = P.diffusion2d(np.exp(g))
+    return rp, ci, v, g, P.normal(seed + 7, side * side)
+
+
+def test_fallback_is_the_plain_solve(sg, ctx, mods):
+    P, Lm = mods
+    side = 64
+    rp, ci, v, g, b = grf_problem(P, side, 21)
+    A = sg.Csr(ctx, rp, ci, v)
+    base = sg.Precond(ctx, "twolevel_ras", nparts=8, overlap=2, precision="fp32").setup(A)
+    net = sg.UNet(ctx, Lm.init_params(3))
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, min_gain=1.0001).setup(A)
+    kw = dict(method="fgmres", restart=20, max_iter=500, rtol=1e-8, ortho="cgs2")
+    x1, x2 = dev(sg, ctx, np.zeros(A.n)), dev(sg, ctx, np.zeros(A.n))
+    db = dev(sg, ctx, b)
+    r1 = sg.solve(A, db, x1, base, **kw)
+    r2 = sg.solve(A, db, x2, stage, **kw)
+    assert r1["converged"] and r2["converged"]
+    assert r1["iterations"] == r2["iterations"]
+    assert np.array_equal(x1.get(), x2.get())
+    calls, acc = stage.counts()
+    assert acc == 0 and calls == r2["precond_applies"] > 0
+
+
+def test_accepted_correction_reduces_the_residual(sg, ctx, mods):
+    P, Lm = mods
+    side = 32
+    rp, ci, v, g, _ = grf_problem(P, side, 22)
+    A = sg.Csr(ctx, rp, ci, v)
+    base = sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=2, precision="fp32").setup(A)
+    net = sg.UNet(ctx, Lm.init_params(4))
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, min_gain=1e-12).setup(A)
+    rng = np.random.default_rng(5)
+    from scipy.sparse import csr_matrix
+    As = csr_matrix((v, ci, rp), shape=(A.n, A.n))
+    for trial in range(3):
+        x = rng.standard_normal(A.n)
+        zb, z = dev(sg, ctx, np.zeros(A.n)), dev(sg, ctx, np.zeros(A.n))
+        base.apply(dev(sg, ctx, x), zb)
+        stage.apply(dev(sg, ctx, x), z)
+        rb = np.linalg.norm(x - As @ zb.get())
+        rz = np.linalg.norm(x - As @ z.get())
+        assert rz <= rb * (1 + 1e-12)
+    calls, acc = stage.counts()
+    assert calls == 3 and acc == 3
+
+
+def test_needs_fgmres(sg, ctx, mods):
+    P, Lm = mods
+    rp, ci, v, g, b = grf_problem(P, 16, 23)
+    A = sg.Csr(ctx, rp, ci, v)
+    base = sg.Precond(ctx, "ras", nparts=2, overlap=1).setup(A)
+    net = sg.UNet(ctx, Lm.init_params(5))
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), 16, 16).setup(A)
+    with pytest.raises(sg.SgError):
+        sg.solve(A, dev(sg, ctx, b), dev(sg, ctx, np.zeros(A.n)), stage, method="gmres")
+
+
+def test_tiny_training_run_drives_the_stage(sg, ctx, mods):
+    P, Lm = mods
+    X, T = Lm.generate_pairs(sg, ctx, P, nfields=8, side=32, base_seed=500, per_field=4,
+                             restart=4, nparts=4, batch=4)
+    assert X.shape == (32, 2, 32, 32) and np.isfinite(X).all() and np.isfinite(T).all()
+    params, hist = Lm.train(sg, ctx, X, T, epochs=30, batch=4, lr=2e-3, seed=1)
+    assert hist[-1]["train_loss"] < hist[0]["train_loss"]
+    side = 32
+    rp, ci, v, g, b = grf_problem(P, side, 777)
+    A = sg.Csr(ctx, rp, ci, v)
+    base = sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=2, precision="fp32").setup(A)
+    net = sg.UNet(ctx, params)
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side).setup(A)
+    x = dev(sg, ctx, np.zeros(A.n))
+    r = sg.solve(A, dev(sg, ctx, b), x, stage, method="fgmres", restart=20, max_iter=500,
+                 rtol=1e-8, ortho="cgs2")
+    assert r["converged"] and r["rel_true_residual"] <= 1e-8
+    calls, acc = stage.counts()
+    assert calls == r["precond_applies"]
