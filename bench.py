@@ -251,6 +251,91 @@ def run_batch(args, sg, P, cfg):
     print(json.dumps(out))
 
 
+def run_learned(args, sg, P, cfg):
+    """2D GRF diffusion 1024^2: mixed two-level RAS-FGMRES(30) with the learned
+    stage vs the same solver without it (ablation) vs FP64 RAS-GMRES."""
+    import importlib.util as iu
+    spec = iu.spec_from_file_location("sgamd_learned", os.path.join(__graft_entry__.PKG_DIR,
+                                                                    "learned.py"))
+    Lm = iu.module_from_spec(spec)
+    spec.loader.exec_module(Lm)
+    t0 = time.perf_counter()
+    side = 1024
+    g = P.grf(side, 1.5, 0.05, 5)
+    rp, ci, v = P.diffusion2d(np.exp(g))
+    b = P.normal(55, side * side)
+    n, nnz = len(rp) - 1, int(rp[-1])
+    print(f"# problem {cfg['workload']}: n={n} nnz={nnz} ({time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    ctx = sg.Context(0)
+    ctx.autotune(args.autotune)
+    A = sg.Csr(ctx, rp, ci, v)
+    db = sg.DeviceArray.from_host(ctx, b)
+    dx = sg.DeviceArray(ctx, n)
+    t0 = time.perf_counter()
+    part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v, weighted=cfg["weighted"])
+    base = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                      precision="fp32", partition=part, coarse_solve=1).setup(A)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    kw = dict(method="fgmres", restart=cfg["restart"], max_iter=cfg["max_iter"], rtol=1e-8,
+              ortho="cgs2")
+
+    def timed(M, label):
+        run_solves(sg, ctx, A, M, db, dx, args.warmup, **kw)
+        rs = run_solves(sg, ctx, A, M, db, dx, args.steps, **kw)
+        x = dx.get()
+        rel = np.linalg.norm(b - spmv_np(rp, ci, v, x)) / np.linalg.norm(b)
+        r = rs[-1]
+        assert r["converged"] and rel <= 1e-8 * 1.0001, (label, r, rel)
+        t = float(np.mean([q["wall_s"] for q in rs]))
+        print(f"# {label}: {r['iterations']} it, {t:.4f} s, rel {rel:.2e}", file=sys.stderr,
+              flush=True)
+        return t, r["iterations"], rel
+
+    t_plain, it_plain, _ = timed(base, "plain mixed two-level RAS-FGMRES")
+    wpath = args.weights or os.path.join(__graft_entry__.PKG_DIR, "weights", "unet_grf256.npy")
+    learned = None
+    if os.path.exists(wpath):
+        net = sg.UNet(ctx, np.load(wpath))
+        stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side,
+                                min_gain=args.min_gain, every=args.every).setup(A)
+        t_l, it_l, rel_l = timed(stage, "learned stage FGMRES")
+        calls, acc = stage.counts()
+        learned = {"s": round(t_l, 4), "iterations": it_l, "rel_true_residual": rel_l,
+                   "stage_calls": calls, "stage_accepted": acc, "weights": os.path.basename(wpath)}
+    extra = {"plain_mixed_twolevel_fgmres": {"s": round(t_plain, 4), "iterations": it_plain}}
+    if learned:
+        extra["learned_stage"] = learned
+        extra["learned_speedup_vs_plain"] = round(t_plain / learned["s"], 3)
+    if not args.no_baseline:
+        Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                        precision="fp64", partition=part).setup(A)
+        rb = run_solves(sg, ctx, A, Mb, db, dx, 1, method="gmres", restart=cfg["restart"],
+                        max_iter=cfg["base_max_iter"], rtol=1e-8, ortho="mgs")[0]
+        extra["fp64_ras_gmres"] = {"s": round(rb["wall_s"], 4), "iterations": rb["iterations"],
+                                   "converged": rb["converged"]}
+        best = learned["s"] if learned else t_plain
+        extra["speedup_vs_fp64_ras_gmres"] = round(rb["wall_s"] / best, 2)
+        print(f"# baseline gmres+ras fp64: {rb['iterations']} it {rb['wall_s']:.3f} s",
+              file=sys.stderr, flush=True)
+    value = learned["s"] if learned else t_plain
+    out = {
+        "metric": "time to ||r||/||b||<=1e-8 (s): mixed/learned FGMRES vs FP64 RAS-GMRES, 1 GPU",
+        "value": round(value, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * value, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 outer / f32 preconditioner / bf16 network", "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n": n, "nnz": nnz,
+                   "solver": f"fgmres({cfg['restart']})+twolevel_ras" + ("+learned" if learned else ""),
+                   "nparts": cfg["nparts"], "overlap": cfg["overlap"],
+                   "setup_s": round(t_setup, 3), "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +351,10 @@ def main():
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling aid: after setup run this many preconditioner applies and "
                          "SpMVs on the workload, print their mean times and exit")
+    ap.add_argument("--weights", default=None,
+                    help="c5: network parameters (.npy) of the learned stage")
+    ap.add_argument("--min-gain", type=float, default=0.01)
+    ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -284,6 +373,9 @@ def main():
     cfg = CONFIGS[args.config]
     if args.config == "c4":
         run_batch(args, sg, P, cfg)
+        return
+    if args.config == "c5":
+        run_learned(args, sg, P, cfg)
         return
     t0 = time.perf_counter()
     rp, ci, v, b, xs = make_problem(P, args.config)

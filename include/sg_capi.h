@@ -245,7 +245,12 @@ sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b, double* d_
  * documented in oracle/unet_oracle.cpp) with bfloat16 weights and activations
  * and fp32 accumulation on the matrix cores.  Inputs [B][2][H][W] fp32
  * (residual / rms(residual), log kappa), output [B][H][W] fp32; H and W must
- * be multiples of 8.  Loss: mean over the batch of ||out - t||^2 / ||t||^2. */
+ * be multiples of 8.  Loss kind 0 (default): mean over the batch of
+ * ||out - t||^2 / ||t||^2.  Loss kind 1 (no target): with s = input channel 0
+ * and A the 2D five-point finite-volume diffusion operator of
+ * kappa = exp(input channel 1) (harmonic faces, Dirichlet boundary), the mean
+ * of 1 - (s . A out)^2 / (|s|^2 |A out|^2): the share of |s|^2 that the best
+ * multiple of out fails to remove. */
 typedef struct sg_unet sg_unet;
 int sg_unet_nparams(void);
 sg_status sg_unet_create(sg_context* ctx, sg_unet** out);
@@ -255,7 +260,9 @@ sg_status sg_unet_set_params(sg_unet* net, const float* params);
 sg_status sg_unet_get_params(const sg_unet* net, float* params);
 /* device buffers */
 sg_status sg_unet_forward(sg_unet* net, int B, int H, int W, const float* d_in, float* d_out);
-/* forward + backward; loss and grad (host, nparams floats) may be NULL */
+sg_status sg_unet_set_loss(sg_unet* net, int kind);
+/* forward + backward; loss and grad (host, nparams floats) may be NULL;
+ * d_target may be NULL with loss kind 1 */
 sg_status sg_unet_loss_grad(sg_unet* net, int B, int H, int W, const float* d_in,
                             const float* d_target, double* loss, float* grad);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */

@@ -19,7 +19,12 @@
 // upsampled tensor first.  Parameters are packed conv by conv in the order
 // above as W[Cout][Cin][3][3] then b[Cout], then the head's w[16], b.
 //
-// Loss (per batch): mean over samples of sum_p (out - t)^2 / sum_p t^2.
+// Loss kind 0 (per batch): mean over samples of sum_p (out - t)^2 / sum_p t^2.
+// Loss kind 1 (residual, no target): with s = input channel 0, kappa =
+// exp(input channel 1) and A the 2D five-point finite-volume diffusion
+// operator (harmonic-mean faces 2 k_p k_q / (k_p + k_q), a boundary face adds
+// k_p to the diagonal), u = A out: mean over samples of 1 - (s.u)^2 / (|s|^2 |u|^2),
+// the fraction of |s|^2 that the best multiple of out leaves.
 //
 // With round_bf16 = 1 the oracle rounds to bfloat16 (round to nearest even)
 // exactly where the GPU stores bfloat16: every weight used by a convolution,
@@ -247,6 +252,27 @@ void forward(Net& n, int B, int H, int W, const float* in) {
             }
 }
 
+// u = A v for one sample (A symmetric), kappa from log kappa
+void stencil(const float* logk, const double* v, double* u, int H, int W) {
+    auto k = [&](int y, int x) { return std::exp((double)logk[y * W + x]); };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const double kp = k(y, x), vp = v[y * W + x];
+            double acc = 0.0;
+            const int dy[4] = {-1, 0, 0, 1}, dx[4] = {0, -1, 1, 0};
+            for (int f = 0; f < 4; ++f) {
+                const int yy = y + dy[f], xx = x + dx[f];
+                if (yy < 0 || yy >= H || xx < 0 || xx >= W) {
+                    acc += kp * vp;
+                } else {
+                    const double kq = k(yy, xx);
+                    acc += 2.0 * kp * kq / (kp + kq) * (vp - v[yy * W + xx]);
+                }
+            }
+            u[y * W + x] = acc;
+        }
+}
+
 }  // namespace
 
 extern "C" {
@@ -265,9 +291,17 @@ void or_unet_forward(const float* params, int B, int H, int W, const float* in, 
     for (size_t i = 0; i < n.out.size(); ++i) out[i] = (float)n.out[i];
 }
 
+double or_unet_loss_grad_kind(const float* params, int B, int H, int W, const float* in,
+                              const float* target, int kind, int round_bf16, double* grad);
+
 // Loss and its gradient with respect to the packed parameters.
 double or_unet_loss_grad(const float* params, int B, int H, int W, const float* in,
                          const float* target, int round_bf16, double* grad) {
+    return or_unet_loss_grad_kind(params, B, H, W, in, target, 0, round_bf16, grad);
+}
+
+double or_unet_loss_grad_kind(const float* params, int B, int H, int W, const float* in,
+                              const float* target, int kind, int round_bf16, double* grad) {
     Net n;
     const bool r = round_bf16 != 0;
     load(n, params, r);
@@ -275,16 +309,37 @@ double or_unet_loss_grad(const float* params, int B, int H, int W, const float* 
     const size_t hw = (size_t)H * W;
     double loss = 0.0;
     std::vector<double> dout(n.out.size());
-    for (int b = 0; b < B; ++b) {
-        double tt = 0.0, ee = 0.0;
-        for (size_t i = 0; i < hw; ++i) {
-            const double t = target[b * hw + i], d = n.out[b * hw + i] - t;
-            tt += t * t;
-            ee += d * d;
+    if (kind == 0) {
+        for (int b = 0; b < B; ++b) {
+            double tt = 0.0, ee = 0.0;
+            for (size_t i = 0; i < hw; ++i) {
+                const double t = target[b * hw + i], d = n.out[b * hw + i] - t;
+                tt += t * t;
+                ee += d * d;
+            }
+            loss += ee / tt / B;
+            for (size_t i = 0; i < hw; ++i)
+                dout[b * hw + i] = 2.0 * (n.out[b * hw + i] - target[b * hw + i]) / tt / B;
         }
-        loss += ee / tt / B;
-        for (size_t i = 0; i < hw; ++i)
-            dout[b * hw + i] = 2.0 * (n.out[b * hw + i] - target[b * hw + i]) / tt / B;
+    } else {
+        std::vector<double> u(hw), g(hw), v(hw);
+        for (int b = 0; b < B; ++b) {
+            const float* sb = in + (size_t)b * 2 * hw;
+            const float* lk = sb + hw;
+            for (size_t i = 0; i < hw; ++i) v[i] = n.out[b * hw + i];
+            stencil(lk, v.data(), u.data(), H, W);
+            double ss = 0.0, a = 0.0, c = 0.0;
+            for (size_t i = 0; i < hw; ++i) {
+                ss += (double)sb[i] * sb[i];
+                a += (double)sb[i] * u[i];
+                c += u[i] * u[i];
+            }
+            loss += (1.0 - a * a / (ss * c)) / B;
+            const double f = -2.0 * a / (ss * c) / B, q = a / c;
+            for (size_t i = 0; i < hw; ++i) g[i] = f * ((double)sb[i] - q * u[i]);
+            stencil(lk, g.data(), u.data(), H, W);  // dL/dout = A dL/du
+            for (size_t i = 0; i < hw; ++i) dout[b * hw + i] = u[i];
+        }
     }
     const int np = nparams();
     std::fill(grad, grad + np, 0.0);

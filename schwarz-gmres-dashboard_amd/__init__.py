@@ -117,6 +117,7 @@ def lib():
         "sg_unet_loss_grad": ([_VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                C.POINTER(C.c_double), _VP], C.c_int),
         "sg_unet_adam_step": ([_VP, C.c_float], C.c_int),
+        "sg_unet_set_loss": ([_VP, C.c_int], C.c_int),
         "sg_csr_create_batch": ([_VP, C.c_int, _VP, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
                                 C.c_int),
         "sg_csr_systems": ([_VP, C.POINTER(C.c_int)], C.c_int),
@@ -267,10 +268,15 @@ class UNet:
     def forward_dev(self, B, H, W, dx, dy):
         self.ctx.check(lib().sg_unet_forward(self.h, B, H, W, dx.ptr, dy.ptr))
 
+    def set_loss(self, kind):
+        """0: relative squared error against a target; 1: residual loss (no target)."""
+        self.ctx.check(lib().sg_unet_set_loss(self.h, int(kind)))
+
     def loss_grad_dev(self, B, H, W, dx, dt, want_grad=False):
         loss = C.c_double()
         g = np.zeros(self.nparams, np.float32) if want_grad else None
-        self.ctx.check(lib().sg_unet_loss_grad(self.h, B, H, W, dx.ptr, dt.ptr, C.byref(loss),
+        self.ctx.check(lib().sg_unet_loss_grad(self.h, B, H, W, dx.ptr,
+                                               dt.ptr if dt is not None else None, C.byref(loss),
                                                _p(g)))
         return (loss.value, g) if want_grad else loss.value
 
@@ -278,7 +284,8 @@ class UNet:
         x = np.ascontiguousarray(x, np.float32)
         B, _, H, W = x.shape
         dx = DeviceArray.from_host(self.ctx, x.ravel())
-        dt = DeviceArray.from_host(self.ctx, np.ascontiguousarray(t, np.float32).ravel())
+        dt = (DeviceArray.from_host(self.ctx, np.ascontiguousarray(t, np.float32).ravel())
+              if t is not None else None)
         return self.loss_grad_dev(B, H, W, dx, dt, want_grad=True)
 
     def adam(self, lr):

@@ -385,19 +385,117 @@ __global__ void k_loss_fin(int B, const double* __restrict__ part, double* __res
     *loss = l;
 }
 
-// dout = 2 (out - t) / (tt B); dd0[b][c][p] = bf16(dout * hw[c]); head partial
-// gradients per block: [blk][17] over all samples
+// loss kind 0: dout = 2 (out - t) / (tt B)
+__global__ void k_mse_dout(int B, int H, int W, const float* __restrict__ out,
+                           const float* __restrict__ tg, const double* __restrict__ sums,
+                           float* __restrict__ dout) {
+    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(e / hwn);
+        dout[e] = (float)(2.0 * ((double)out[e] - (double)tg[e]) / sums[2 * b] / B);
+    }
+}
+
+// loss kind 1: u = A v per sample, A the five-point finite-volume diffusion
+// operator of kappa = exp(in[b][1]) (harmonic faces; a boundary face adds k_p)
+__global__ void k_stencil(int B, int H, int W, const float* __restrict__ in,
+                          const float* __restrict__ v, float* __restrict__ u) {
+    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = e % hwn;
+        const int b = (int)(e / hwn);
+        const int y = (int)(p / W), x = (int)(p % W);
+        const float* lk = in + ((size_t)b * 2 + 1) * hwn;
+        const float* vb = v + (size_t)b * hwn;
+        const float kp = expf(lk[p]), vp = vb[p];
+        float acc = 0.f;
+        const int dy[4] = {-1, 0, 0, 1}, dx[4] = {0, -1, 1, 0};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int yy = y + dy[f], xx = x + dx[f];
+            if (yy < 0 || yy >= H || xx < 0 || xx >= W) {
+                acc += kp * vp;
+            } else {
+                const size_t q = (size_t)yy * W + xx;
+                const float kq = expf(lk[q]);
+                acc += 2.f * kp * kq / (kp + kq) * (vp - vb[q]);
+            }
+        }
+        u[e] = acc;
+    }
+}
+
+// per-sample block partials of s.s, s.u, u.u (s = in[b][0])
 __global__ void __launch_bounds__(256)
-k_head_bwd(int B, int H, int W, const float* __restrict__ out, const float* __restrict__ tg,
-           const double* __restrict__ sums, View d0, const float* __restrict__ hw, View dd0,
-           float* __restrict__ hpart) {
+k_rloss_part(int H, int W, const float* __restrict__ in, const float* __restrict__ u,
+             double* __restrict__ part) {
+    const int b = blockIdx.y;
+    const size_t hwn = (size_t)H * W;
+    const float* sb = in + (size_t)b * 2 * hwn;
+    double ss = 0.0, a = 0.0, c = 0.0;
+    for (size_t p = blockIdx.x * 256 + threadIdx.x; p < hwn; p += kLB * 256) {
+        const double sv = sb[p], uv = u[b * hwn + p];
+        ss += sv * sv;
+        a += sv * uv;
+        c += uv * uv;
+    }
+    __shared__ double r[3][256];
+    r[0][threadIdx.x] = ss;
+    r[1][threadIdx.x] = a;
+    r[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int k = 0; k < 3; ++k) r[k][threadIdx.x] += r[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[((size_t)b * kLB + blockIdx.x) * 3 + k] = r[k][0];
+}
+
+// loss = mean_b (1 - a^2/(ss c)); coef[b] = {f, q}: dL/du = f (s - q u)
+__global__ void k_rloss_fin(int B, const double* __restrict__ part, double* __restrict__ coef,
+                            double* __restrict__ loss) {
+    if (threadIdx.x != 0) return;
+    double l = 0.0;
+    for (int b = 0; b < B; ++b) {
+        double ss = 0.0, a = 0.0, c = 0.0;
+        for (int k = 0; k < kLB; ++k) {
+            ss += part[((size_t)b * kLB + k) * 3];
+            a += part[((size_t)b * kLB + k) * 3 + 1];
+            c += part[((size_t)b * kLB + k) * 3 + 2];
+        }
+        const bool ok = ss > 0.0 && c > 0.0;
+        l += ok ? (1.0 - a * a / (ss * c)) / B : 1.0 / B;
+        coef[2 * b] = ok ? -2.0 * a / (ss * c) / B : 0.0;
+        coef[2 * b + 1] = ok ? a / c : 0.0;
+    }
+    *loss = l;
+}
+
+__global__ void k_rloss_g(int B, int H, int W, const float* __restrict__ in,
+                          const float* __restrict__ u, const double* __restrict__ coef,
+                          float* __restrict__ g) {
+    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = e % hwn;
+        const int b = (int)(e / hwn);
+        const double sv = in[(size_t)b * 2 * hwn + p];
+        g[e] = (float)(coef[2 * b] * (sv - coef[2 * b + 1] * (double)u[e]));
+    }
+}
+
+// dd0[b][c][p] = bf16(dout * hw[c]); head partial gradients per block [blk][17]
+__global__ void __launch_bounds__(256)
+k_head_bwd(int B, int H, int W, const float* __restrict__ dout, View d0,
+           const float* __restrict__ hw, View dd0, float* __restrict__ hpart) {
     const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
     float acc[17];
     for (int c = 0; c < 17; ++c) acc[c] = 0.f;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const size_t p = e % hwn;
         const int b = (int)(e / hwn);
-        const float g = (float)(2.0 * ((double)out[e] - (double)tg[e]) / sums[2 * b] / B);
+        const float g = dout[e];
         acc[16] += g;
         for (int c = 0; c < 16; ++c) {
             const size_t i = ((size_t)b * d0.ctot + d0.c0 + c) * hwn + p;
@@ -484,9 +582,10 @@ struct sg_unet {
     DevBuf<uint16_t> x, a0, cat0, p0, a2, cat1, p1, a4, cat2, p2, a6, a7, a8, a9, a10, a11, a12, a13;
     DevBuf<uint16_t> gx, ga0, gcat0, gp0, ga2, gcat1, gp1, ga4, gcat2, gp2, ga6, ga7, ga8, ga9, ga10,
         ga11, ga12, ga13;
-    DevBuf<float> out, dwpart, dwpart_b, hpart;
+    DevBuf<float> out, dwpart, dwpart_b, hpart, dout, lu, lg;
     DevBuf<double> lpart, lsums, lloss;
     bool grads_alloc = false;
+    int loss_kind = 0;
 };
 
 namespace sg {
@@ -520,7 +619,7 @@ static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
         A(U->a8, 64, hw2), A(U->a9, 64, hw2), A(U->a10, 32, hw1), A(U->a11, 32, hw1);
         A(U->a12, 16, hw0), A(U->a13, 16, hw0);
         U->out.alloc((size_t)B * hw0);
-        U->lpart.alloc((size_t)B * kLB * 2);
+        U->lpart.alloc((size_t)B * kLB * 3);
         U->lsums.alloc((size_t)B * 2);
         U->lloss.alloc(1);
     }
@@ -544,6 +643,9 @@ static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
         U->dwpart.alloc(mx);
         U->dwpart_b.alloc(mxb);
         U->hpart.alloc((size_t)1024 * 17);
+        U->dout.alloc((size_t)B * hw0);
+        U->lu.alloc((size_t)B * hw0);
+        U->lg.alloc((size_t)B * hw0);
         U->grads_alloc = true;
     }
 }
@@ -613,16 +715,29 @@ void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool g
     SG_HIP(hipGetLastError());
 }
 
-void unet_backward_dev(sg_unet* U, const float* d_target) {
+void unet_backward_dev(sg_unet* U, const float* d_in, const float* d_target) {
     const int B = U->B, H = U->H, W = U->W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     const size_t hw0 = (size_t)H * W;
     hipStream_t s = U->ctx->stream;
-    hipLaunchKernelGGL(k_loss_part, dim3(kLB, B), dim3(256), 0, s, H, W, U->out.p, d_target, U->lpart.p);
-    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, s, B, U->lpart.p, U->lsums.p, U->lloss.p);
+    const int ge = grid1(B * hw0);
+    if (U->loss_kind == 0) {
+        hipLaunchKernelGGL(k_loss_part, dim3(kLB, B), dim3(256), 0, s, H, W, U->out.p, d_target,
+                           U->lpart.p);
+        hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, s, B, U->lpart.p, U->lsums.p, U->lloss.p);
+        hipLaunchKernelGGL(k_mse_dout, dim3(ge), dim3(256), 0, s, B, H, W, U->out.p, d_target,
+                           U->lsums.p, U->dout.p);
+    } else {
+        hipLaunchKernelGGL(k_stencil, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->out.p, U->lu.p);
+        hipLaunchKernelGGL(k_rloss_part, dim3(kLB, B), dim3(256), 0, s, H, W, d_in, U->lu.p, U->lpart.p);
+        hipLaunchKernelGGL(k_rloss_fin, dim3(1), dim3(64), 0, s, B, U->lpart.p, U->lsums.p, U->lloss.p);
+        hipLaunchKernelGGL(k_rloss_g, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->lu.p, U->lsums.p,
+                           U->lg.p);
+        hipLaunchKernelGGL(k_stencil, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->lg.p, U->dout.p);
+    }
     const int hb = std::min(1024, grid1(B * hw0));
-    hipLaunchKernelGGL(k_head_bwd, dim3(hb), dim3(256), 0, s, B, H, W, U->out.p, d_target, U->lsums.p,
-                       V(U->a13, 16), U->hw.p, V(U->ga13, 16), U->hpart.p);
+    hipLaunchKernelGGL(k_head_bwd, dim3(hb), dim3(256), 0, s, B, H, W, U->dout.p, V(U->a13, 16),
+                       U->hw.p, V(U->ga13, 16), U->hpart.p);
     hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, s, hb, U->hpart.p, U->g.p + uconv_offset(kUConvs));
     auto mask = [&](DevBuf<uint16_t>& y, int yc, int yc0, DevBuf<uint16_t>& dy, int dc, int dc0, int C,
                     int h, int w) {
@@ -773,16 +888,23 @@ sg_status sg_unet_forward(sg_unet* U, int B, int H, int W, const float* d_in, fl
 
 sg_status sg_unet_loss_grad(sg_unet* U, int B, int H, int W, const float* d_in, const float* d_target,
                             double* loss, float* grad) {
-    if (!U || !d_in || !d_target) return SG_E_ARG;
+    if (!U || !d_in) return SG_E_ARG;
     return run_guarded(U->ctx, [&] {
+        if (U->loss_kind == 0 && !d_target) fail(SG_E_ARG, "this loss needs a target");
         unet_forward_dev(U, B, H, W, d_in, true);
-        unet_backward_dev(U, d_target);
+        unet_backward_dev(U, d_in, d_target);
         if (loss)
             SG_HIP(hipMemcpyAsync(loss, U->lloss.p, sizeof(double), hipMemcpyDeviceToHost, U->ctx->stream));
         if (grad)
             SG_HIP(hipMemcpyAsync(grad, U->g.p, sizeof(float) * U->np, hipMemcpyDeviceToHost, U->ctx->stream));
         SG_HIP(hipStreamSynchronize(U->ctx->stream));
     });
+}
+
+sg_status sg_unet_set_loss(sg_unet* U, int kind) {
+    if (!U || kind < 0 || kind > 1) return SG_E_ARG;
+    U->loss_kind = kind;
+    return SG_OK;
 }
 
 sg_status sg_unet_adam_step(sg_unet* U, float lr) {

@@ -1,17 +1,22 @@
 """Training data and training of the learned FGMRES stage's network.
 
-Pairs come from the plain two-level RAS-FGMRES solve itself.  For each seeded
+Data comes from the plain two-level RAS-FGMRES solve itself.  For each seeded
 field (kappa = exp(g), g a Gaussian random field on a side x side grid, b from
 the same seed) the batched solver runs `per_field` restart cycles of
-`restart` steps from x0 = 0, keeping each iterate x_k, then converges to
-||b - A x|| <= 1e-10 ||b|| for x*.  Pair k: r_k = b - A x_k, e_k = x* - x_k
-(A e_k = r_k up to the 1e-10 solve).  Network input (r_k / rms(r_k), g),
-target e_k / rms(r_k).  Every field shares one unweighted partition of the
-grid (`nparts` subdomains, overlap 2) and FP32 ILU(0) with a Nicolaides
-coarse space.
+`restart` steps from x0 = 0, keeping each iterate x_k and its residual
+r_k = b - A x_k.  Every field shares one unweighted partition of the grid
+(`nparts` subdomains, overlap 2) and FP32 ILU(0) with a Nicolaides coarse
+space.
 
-The network is trained with Adam on the relative squared error of e, on
-batches of whole fields, with a cosine learning-rate decay.
+* mode "residual" (default): the input is what the stage sees at run time,
+  s_k = r_k - A M r_k (the residual left by one two-level step), as
+  (s_k / rms(s_k), g); no target: the network is trained with loss kind 1,
+  the share of |s|^2 that the best multiple of its output fails to remove.
+* mode "error": the solve continues to ||b - A x|| <= 1e-10 ||b|| for x*;
+  input (r_k / rms(r_k), g), target e_k / rms(r_k) with e_k = x* - x_k, loss
+  kind 0 (relative squared error).
+
+Training: Adam on batches of whole fields with a cosine learning-rate decay.
 
     python learned.py --fields 2000 --epochs 8 --out weights.npy
 """
@@ -50,14 +55,15 @@ def load_problems():
 
 
 def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10, restart=10,
-                   sigma=1.5, corr=0.05, nparts=16, batch=32, log=None):
-    """Returns X [N, 2, side, side] and T [N, side, side] (float32, N = nfields * per_field)."""
+                   sigma=1.5, corr=0.05, nparts=16, batch=32, log=None, mode="residual"):
+    """Returns X [N, 2, side, side] and T [N, side, side] (float32, N = nfields *
+    per_field); T is None in mode "residual"."""
     n = side * side
     rp0, ci0, _ = P.poisson2d(side, side)
     part1 = sg.partition(rp0, ci0, nparts).astype(np.int32)
     N = nfields * per_field
     X = np.empty((N, 2, side, side), np.float32)
-    T = np.empty((N, side, side), np.float32)
+    T = np.empty((N, side, side), np.float32) if mode == "error" else None
     k = 0
     t0 = time.perf_counter()
     for f0 in range(0, nfields, batch):
@@ -77,18 +83,27 @@ def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10,
         db = sg.DeviceArray.from_host(ctx, b)
         dx = sg.DeviceArray(ctx, B.n)
         dr = sg.DeviceArray(ctx, B.n)
+        dz = sg.DeviceArray(ctx, B.n)
         xs, rs = [], []
         for _ in range(per_field):
             sg.solve_batch(B, db, dx, M, method="fgmres", restart=restart, max_iter=1, tol=0.0,
                            ortho="cgs2")
-            xs.append(dx.get())
             dr.upload(b)
             B.spmv(dx, dr, alpha=-1.0, beta=1.0)
-            rs.append(dr.get())
-        res = sg.solve_batch(B, db, dx, M, method="fgmres", restart=30, max_iter=2000, rtol=1e-10,
-                             ortho="cgs2")
-        assert all(r["converged"] for r in res)
-        xstar = dx.get()
+            r = dr.get()
+            if mode == "error":
+                xs.append(dx.get())
+                rs.append(r)
+            else:  # s = r - A M r
+                M.apply(dr, dz)
+                B.spmv(dz, dr, alpha=-1.0, beta=1.0)
+                rs.append(dr.get())
+        xstar = None
+        if mode == "error":
+            res = sg.solve_batch(B, db, dx, M, method="fgmres", restart=30, max_iter=2000,
+                                 rtol=1e-10, ortho="cgs2")
+            assert all(r["converged"] for r in res)
+            xstar = dx.get()
         for q in range(ns):
             sl = slice(q * n, (q + 1) * n)
             for j in range(per_field):
@@ -96,7 +111,8 @@ def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10,
                 rms = np.sqrt(np.mean(r * r))
                 X[k, 0] = (r / rms).reshape(side, side)
                 X[k, 1] = gs[q]
-                T[k] = ((xstar[sl] - xs[j][sl]) / rms).reshape(side, side)
+                if T is not None:
+                    T[k] = ((xstar[sl] - xs[j][sl]) / rms).reshape(side, side)
                 k += 1
         del M, B
         if log:
@@ -110,21 +126,25 @@ class _Ptr:
 
 
 def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0.05, log=None):
-    """Adam on device-resident data; returns (params, history)."""
+    """Adam on device-resident data (loss kind 1 when T is None); returns
+    (params, history)."""
     import ctypes as C
     N, _, H, W = X.shape
     nval = max(1, int(round(val * N))) if N > 1 else 0
     rng = np.random.default_rng(seed)
     perm = rng.permutation(N)
-    X, T = X[perm], T[perm]
+    X = X[perm]
+    T = T[perm] if T is not None else None
     dX = sg.DeviceArray.from_host(ctx, X.reshape(-1))
-    dT = sg.DeviceArray.from_host(ctx, T.reshape(-1))
+    dT = sg.DeviceArray.from_host(ctx, T.reshape(-1)) if T is not None else None
     net = sg.UNet(ctx, init_params(seed) if params is None else params)
+    net.set_loss(0 if T is not None else 1)
     xs, ts = 2 * H * W * 4, H * W * 4
 
     def views(i, b):
         i = int(i)
-        return (_Ptr(C.c_void_p(dX.ptr.value + i * xs)), _Ptr(C.c_void_p(dT.ptr.value + i * ts)))
+        vt = _Ptr(C.c_void_p(dT.ptr.value + i * ts)) if dT is not None else None
+        return _Ptr(C.c_void_p(dX.ptr.value + i * xs)), vt
 
     ntr = N - nval
     nb = max(1, ntr // batch)
@@ -164,6 +184,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--mode", default="residual", choices=("residual", "error"))
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
@@ -173,7 +194,8 @@ def main():
     ctx = sg.Context(0)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field, log=log)
+    X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field,
+                          log=log, mode=args.mode)
     tgen = time.perf_counter() - t0
     params, hist = train(sg, ctx, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     np.save(args.out, params)

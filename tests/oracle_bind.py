@@ -45,6 +45,9 @@ def lib():
         L.or_unet_forward.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F, _F, C.c_int]
         L.or_unet_loss_grad.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F, _F, C.c_int, _D]
         L.or_unet_loss_grad.restype = C.c_double
+        L.or_unet_loss_grad_kind.argtypes = [_F, C.c_int, C.c_int, C.c_int, _F, _F, C.c_int,
+                                             C.c_int, _D]
+        L.or_unet_loss_grad_kind.restype = C.c_double
         L.or_precond_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_precond_free.argtypes = [C.c_void_p]
         L.or_lu_factor.argtypes = [C.c_int, _D, _I]
@@ -228,10 +231,12 @@ def unet_forward(params, x, round_bf16=True):
     return out
 
 
-def unet_loss_grad(params, x, t, round_bf16=True):
+def unet_loss_grad(params, x, t, round_bf16=True, kind=0):
+    """kind 0: relative squared error against t; kind 1: residual loss (t unused)."""
     x = np.ascontiguousarray(x, np.float32)
     B, _, H, W = x.shape
     g = np.zeros(unet_nparams())
-    loss = lib().or_unet_loss_grad(np.ascontiguousarray(params, np.float32), B, H, W, x,
-                                   np.ascontiguousarray(t, np.float32), 1 if round_bf16 else 0, g)
+    tt = np.ascontiguousarray(t if t is not None else np.zeros((B, H, W)), np.float32)
+    loss = lib().or_unet_loss_grad_kind(np.ascontiguousarray(params, np.float32), B, H, W, x, tt,
+                                        kind, 1 if round_bf16 else 0, g)
     return loss, g

@@ -110,8 +110,13 @@ def test_needs_fgmres(sg, ctx, mods):
 def test_tiny_training_run_drives_the_stage(sg, ctx, mods):
     P, Lm = mods
     X, T = Lm.generate_pairs(sg, ctx, P, nfields=8, side=32, base_seed=500, per_field=4,
-                             restart=4, nparts=4, batch=4)
+                             restart=4, nparts=4, batch=4, mode="error")
     assert X.shape == (32, 2, 32, 32) and np.isfinite(X).all() and np.isfinite(T).all()
+    Xr, Tr = Lm.generate_pairs(sg, ctx, P, nfields=8, side=32, base_seed=500, per_field=4,
+                               restart=4, nparts=4, batch=4, mode="residual")
+    assert Tr is None and np.isfinite(Xr).all()
+    _, hr = Lm.train(sg, ctx, Xr, None, epochs=20, batch=4, lr=2e-3, seed=2)
+    assert hr[-1]["train_loss"] < hr[0]["train_loss"]
     params, hist = Lm.train(sg, ctx, X, T, epochs=30, batch=4, lr=2e-3, seed=1)
     assert hist[-1]["train_loss"] < hist[0]["train_loss"]
     side = 32

@@ -125,3 +125,52 @@ def test_bf16_mode_close_to_exact():
     _, ge = O.unet_loss_grad(params, x, t, round_bf16=False)
     _, gr = O.unet_loss_grad(params, x, t, round_bf16=True)
     assert np.linalg.norm(gr - ge) <= 5e-2 * np.linalg.norm(ge)
+
+
+def torch_stencil(logk, v):
+    """A v for the 2D finite-volume diffusion operator of exp(logk) (torch)."""
+    import torch
+    k = torch.exp(logk)
+    H, W = k.shape[-2:]
+    out = torch.zeros_like(v)
+    for dy, dx in ((-1, 0), (0, -1), (0, 1), (1, 0)):
+        kq = torch.roll(k, shifts=(-dy, -dx), dims=(-2, -1))
+        vq = torch.roll(v, shifts=(-dy, -dx), dims=(-2, -1))
+        f = 2 * k * kq / (k + kq)
+        yy = torch.arange(H).view(H, 1) + dy
+        xx = torch.arange(W).view(1, W) + dx
+        inside = ((yy >= 0) & (yy < H) & (xx >= 0) & (xx < W)).to(v.dtype)
+        out = out + inside * f * (v - vq) + (1 - inside) * k * v
+    return out
+
+
+def test_residual_loss_matches_torch_and_scipy_matrix():
+    torch = pytest.importorskip("torch")
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "sgamd_problems", os.path.join(os.path.dirname(__file__), "..",
+                                       "schwarz-gmres-dashboard_amd", "problems.py"))
+    P = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(P)
+    params = init_params(8)
+    x, _ = sample(9, B=2, H=16, W=8)
+    x[:, 1] *= 0.7
+    # the torch stencil is the problems.py diffusion2d matrix
+    v = np.random.default_rng(1).standard_normal((16, 8))
+    rp, ci, vals = P.diffusion2d(np.exp(x[0, 1].astype(np.float64)))
+    from scipy.sparse import csr_matrix
+    Am = csr_matrix((vals, ci, rp), shape=(128, 128))
+    ts = torch_stencil(torch.tensor(x[0, 1].astype(np.float64)), torch.tensor(v)).numpy()
+    assert np.allclose(ts.ravel(), Am @ v.ravel(), rtol=1e-12, atol=1e-12)
+    p, f = torch_net(params)
+    xt = torch.tensor(x.astype(np.float64))
+    out = f(xt)
+    s = xt[:, 0]
+    u = torch_stencil(xt[:, 1], out)
+    a = (s * u).sum((1, 2))
+    loss = (1 - a * a / ((s * s).sum((1, 2)) * (u * u).sum((1, 2)))).mean()
+    loss.backward()
+    l2, g = O.unet_loss_grad(params, x, None, round_bf16=False, kind=1)
+    assert abs(l2 - loss.item()) <= 1e-10 * abs(loss.item())
+    assert np.linalg.norm(g - p.grad.numpy()) <= 1e-9 * np.linalg.norm(p.grad.numpy())
