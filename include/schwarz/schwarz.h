@@ -145,6 +145,47 @@ GMRESResult fgmres_flex(const CSRMatrix& A, const std::vector<double>& b,
                         std::vector<double>& x, FlexApply prec_apply,
                         const GMRESParams& params = {});
 
+// ---- extensions without a reference counterpart ----------------------------------
+
+// Learned FGMRES stage on an nx x ny grid (row y*nx + x): base(x) plus the
+// network correction of its residual, kept only when it reduces the residual
+// (sg_precond_create_learned).  Valid with fgmres only.  setup(A) also sets up
+// base; base must outlive the stage.
+class LearnedStage : public Preconditioner {
+public:
+    LearnedStage(DevicePreconditioner& base, std::vector<float> params, int nx, int ny,
+                 std::vector<double> log_kappa, double min_gain = 0.01, int every = 1);
+    ~LearnedStage() override;
+    void setup(const CSRMatrix& A) override;
+    void apply(const double* x, double* y, int n) const override;
+    sg_precond* device_handle() const override { return M_; }
+    // applications so far and how many kept the network's correction
+    std::pair<long long, long long> counts() const;
+
+private:
+    DevicePreconditioner& base_;
+    std::vector<float> params_;
+    std::vector<double> logk_;
+    int nx_, ny_, every_;
+    double min_gain_;
+    sg_unet* net_ = nullptr;
+    sg_precond* M_ = nullptr;
+    sg_csr* A_ = nullptr;
+};
+
+// A 1-D little-endian float32 .npy file (the learned stage's weights).
+std::vector<float> load_npy_f32(const std::string& path);
+
+// Many independent systems solved together (sg_solve_batch): one block-diagonal
+// device matrix, `kind` with nparts subdomains per system (built-in
+// partitioner per system), each system with its own GMRES/FGMRES state.
+// x[i] is system i's initial guess and result.
+std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
+                                     const std::vector<std::vector<double>>& b,
+                                     std::vector<std::vector<double>>& x, sg_precond_kind kind,
+                                     int nparts, int overlap, sg_precision precision,
+                                     const GMRESParams& params, bool flexible);
+
 // One JSON object per line, appended (field order and number formats as the
 // reference's logger; extra fields may only be appended after "efficiency").
 struct Metrics {

@@ -1,7 +1,9 @@
 // Drop-in C++ interface (include/schwarz/schwarz.h) over the C ABI.
 #include "../../include/schwarz/schwarz.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
@@ -273,6 +275,172 @@ std::vector<double> ILU0Precond::lu_vals() const {
     std::vector<double> v(nnz);
     check(sg_precond_ilu_values(M_, 0, &nnz, v.data()), default_context());
     return v;
+}
+
+// ---- learned stage ----------------------------------------------------------------
+
+LearnedStage::LearnedStage(DevicePreconditioner& base, std::vector<float> params, int nx, int ny,
+                           std::vector<double> log_kappa, double min_gain, int every)
+    : base_(base), params_(std::move(params)), logk_(std::move(log_kappa)), nx_(nx), ny_(ny),
+      every_(every), min_gain_(min_gain) {
+    if ((int)params_.size() != sg_unet_nparams())
+        throw std::runtime_error("learned stage: wrong number of network parameters");
+    if ((long long)logk_.size() != (long long)nx * ny)
+        throw std::runtime_error("learned stage: log kappa must have nx * ny values");
+}
+
+LearnedStage::~LearnedStage() {
+    if (M_) sg_precond_destroy(M_);
+    if (net_) sg_unet_destroy(net_);
+    if (A_) sg_csr_destroy(A_);
+}
+
+void LearnedStage::setup(const CSRMatrix& A) {
+    sg_context* ctx = default_context();
+    base_.setup(A);
+    if (!net_) {
+        check(sg_unet_create(ctx, &net_), ctx);
+        check(sg_unet_set_params(net_, params_.data()), ctx);
+    }
+    if (M_) sg_precond_destroy(M_), M_ = nullptr;
+    if (A_) sg_csr_destroy(A_), A_ = nullptr;
+    sg_learned_opts o{nx_, ny_, min_gain_, every_};
+    check(sg_precond_create_learned(ctx, base_.device_handle(), net_, logk_.data(), &o, &M_), ctx);
+    check(sg_csr_create(ctx, A.nrows, A.ncols, A.nnz(), A.row_ptr.data(), A.col_idx.data(),
+                        A.vals.data(), &A_),
+          ctx);
+    check(sg_precond_setup(M_, A_), ctx);
+}
+
+void LearnedStage::apply(const double* x, double* y, int n) const {
+    sg_context* ctx = default_context();
+    if (!M_) throw std::runtime_error("preconditioner used before setup");
+    Dev dx(sizeof(double) * std::max(n, 1)), dy(sizeof(double) * std::max(n, 1));
+    check(sg_memcpy_h2d(ctx, dx.p, x, sizeof(double) * n), ctx);
+    check(sg_precond_apply(M_, dx.d(), dy.d()), ctx);
+    check(sg_memcpy_d2h(ctx, y, dy.p, sizeof(double) * n), ctx);
+}
+
+std::pair<long long, long long> LearnedStage::counts() const {
+    long long c = 0, a = 0;
+    if (M_) check(sg_precond_learned_stats(M_, &c, &a), default_context());
+    return {c, a};
+}
+
+std::vector<float> load_npy_f32(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    char magic[6];
+    in.read(magic, 6);
+    if (std::string(magic, 6) != "\x93NUMPY") throw std::runtime_error(path + ": not a .npy file");
+    unsigned char ver[2];
+    in.read(reinterpret_cast<char*>(ver), 2);
+    uint32_t hlen = 0;
+    if (ver[0] == 1) {
+        unsigned char h[2];
+        in.read(reinterpret_cast<char*>(h), 2);
+        hlen = h[0] | (h[1] << 8);
+    } else {
+        unsigned char h[4];
+        in.read(reinterpret_cast<char*>(h), 4);
+        hlen = h[0] | (h[1] << 8) | (h[2] << 16) | ((uint32_t)h[3] << 24);
+    }
+    std::string header(hlen, ' ');
+    in.read(&header[0], hlen);
+    if (header.find("'<f4'") == std::string::npos || header.find("'fortran_order': False") == std::string::npos)
+        throw std::runtime_error(path + ": expected a C-ordered little-endian float32 array");
+    const size_t l = header.find('('), r = header.find(')');
+    long long count = 1;
+    std::string shape = header.substr(l + 1, r - l - 1);
+    std::stringstream ss(shape);
+    std::string tok;
+    while (std::getline(ss, tok, ','))
+        if (tok.find_first_not_of(' ') != std::string::npos) count *= std::stoll(tok);
+    std::vector<float> v((size_t)count);
+    in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(sizeof(float) * count));
+    if (!in) throw std::runtime_error(path + ": truncated");
+    return v;
+}
+
+// ---- batched solve -------------------------------------------------------------------
+
+std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
+                                     const std::vector<std::vector<double>>& b,
+                                     std::vector<std::vector<double>>& x, sg_precond_kind kind,
+                                     int nparts, int overlap, sg_precision precision,
+                                     const GMRESParams& p, bool flexible) {
+    sg_context* ctx = default_context();
+    const int ns = (int)systems.size();
+    if (ns < 1 || (int)b.size() != ns) throw std::runtime_error("solve_batch: one b per system");
+    x.resize(ns);
+    std::vector<int> off(1, 0), rp(1, 0), ci, part;
+    std::vector<double> v, hb, hx;
+    for (int q = 0; q < ns; ++q) {
+        const CSRMatrix& A = systems[q];
+        if (A.nrows != A.ncols || (int)b[q].size() < A.nrows)
+            throw std::runtime_error("solve_batch: square systems with matching b");
+        x[q].resize(A.nrows, 0.0);
+        const int o = off.back(), nz0 = rp.back();
+        for (int i = 0; i < A.nrows; ++i) rp.push_back(nz0 + A.row_ptr[i + 1]);
+        for (int c : A.col_idx) ci.push_back(c + o);
+        v.insert(v.end(), A.vals.begin(), A.vals.end());
+        hb.insert(hb.end(), b[q].begin(), b[q].begin() + A.nrows);
+        hx.insert(hx.end(), x[q].begin(), x[q].end());
+        std::vector<int> pq(A.nrows, 0);
+        if (nparts > 1)
+            check(sg_partition(A.nrows, A.row_ptr.data(), A.col_idx.data(), A.vals.data(), nparts,
+                               0, 0, pq.data()),
+                  ctx);
+        for (int e : pq) part.push_back(e + q * std::max(1, nparts));
+        off.push_back(o + A.nrows);
+    }
+    const int n = off.back();
+    sg_csr* B = nullptr;
+    check(sg_csr_create_batch(ctx, ns, off.data(), rp.back(), rp.data(), ci.data(), v.data(), &B),
+          ctx);
+    struct DropB {
+        sg_csr* b;
+        ~DropB() { sg_csr_destroy(b); }
+    } dropb{B};
+    sg_precond* M = nullptr;
+    if (kind != SG_PC_NONE) {
+        sg_precond_opts o{};
+        o.nparts = ns * std::max(1, nparts);
+        o.overlap = overlap;
+        o.precision = precision;
+        o.partition = part.data();
+        check(sg_precond_create(ctx, kind, &o, &M), ctx);
+    }
+    struct DropM {
+        sg_precond* m;
+        ~DropM() {
+            if (m) sg_precond_destroy(m);
+        }
+    } dropm{M};
+    if (M) check(sg_precond_setup(M, B), ctx);
+    Dev db(sizeof(double) * n), dx(sizeof(double) * n);
+    check(sg_memcpy_h2d(ctx, db.p, hb.data(), sizeof(double) * n), ctx);
+    check(sg_memcpy_h2d(ctx, dx.p, hx.data(), sizeof(double) * n), ctx);
+    sg_solve_params sp{};
+    sp.method = flexible ? SG_FGMRES : SG_GMRES_LEFT;
+    sp.restart = p.restart;
+    sp.max_iter = p.max_iter;
+    sp.tol_abs = p.tol;
+    sp.rtol = p.rtol;
+    sp.ortho = p.ortho;
+    std::vector<sg_solve_result> r(ns);
+    check(sg_solve_batch(B, M, db.d(), dx.d(), &sp, r.data(), nullptr, 0), ctx);
+    check(sg_memcpy_d2h(ctx, hx.data(), dx.p, sizeof(double) * n), ctx);
+    std::vector<GMRESResult> out(ns);
+    for (int q = 0; q < ns; ++q) {
+        std::copy(hx.begin() + off[q], hx.begin() + off[q + 1], x[q].begin());
+        out[q].iterations = r[q].iterations;
+        out[q].converged = r[q].converged != 0;
+        out[q].residual_norm = r[q].residual_norm;
+        out[q].time_solve_s = r[q].time_solve_s;
+        out[q].rel_true_residual = r[q].rel_true_residual;
+    }
+    return out;
 }
 
 // ---- solvers -------------------------------------------------------------------

@@ -3,7 +3,9 @@
 // solve on the GPU.  Extensions: --poisson3d N, --jump3d N (c3 workload),
 // --rhs ones|uniform|normal with --seed, --rtol, --precision fp64|fp32,
 // --ortho mgs|cgs2, --partition FILE, --ufactor U, --weighted-partition,
-// --coarse-inverse.  Exit code 0 if converged, else 1 (src/main.cpp:274).
+// --coarse-inverse, --batch K (K systems with right-hand-side seeds seed..,
+// solved together), --learned WEIGHTS.npy (learned FGMRES stage on the 2D
+// grid).  Exit code 0 if converged, else 1 (src/main.cpp:274).
 #include "../../include/schwarz/schwarz.h"
 
 #include <chrono>
@@ -50,6 +52,9 @@ void usage() {
            "  --rhs <r>                ones (b = A*1, default) | uniform (b = A*x*, x* ~ U[-1,1]) | normal (b ~ N(0,1))\n"
            "  --seed <s>               seed for --rhs uniform|normal (default 1)\n"
            "\nOutput:\n"
+           "  --batch <K>              solve K systems together (right-hand sides from seeds seed..seed+K-1)\n"
+           "  --learned <w.npy>        learned FGMRES stage on top of --precond (2D --poisson grids)\n"
+           "  --learned-every <k>      run the network on every k-th application (default 1)\n"
            "  --gpu                    accepted; every solve runs on the GPU\n"
            "  --output <file.jsonl>    append a metrics record\n";
 }
@@ -134,6 +139,8 @@ int main(int argc, char** argv) {
     int overlap = 1, nparts = 4, poisson = 0, poisson3d = 0, jump3d = 0, ufactor = 0;
     bool use_fgmres = false, weighted = false, coarse_inv = false;
     uint64_t seed = 1;
+    int batch = 0, learned_every = 1;
+    std::string learned;
     for (int i = 1; i < argc; ++i) {
         auto is = [&](const char* f) { return !std::strcmp(argv[i], f); };
         auto val = [&](const char* f) { return !std::strcmp(argv[i], f) && i + 1 < argc; };
@@ -154,6 +161,9 @@ int main(int argc, char** argv) {
         else if (val("--ufactor")) ufactor = std::atoi(argv[++i]);
         else if (val("--ortho")) ortho = argv[++i];
         else if (val("--precision")) precision = argv[++i];
+        else if (val("--batch")) batch = std::atoi(argv[++i]);
+        else if (val("--learned")) learned = argv[++i];
+        else if (val("--learned-every")) learned_every = std::atoi(argv[++i]);
         else if (is("--fgmres")) use_fgmres = true;
         else if (is("--gpu")) {}
         else if (is("--weighted-partition")) weighted = true;
@@ -189,15 +199,69 @@ int main(int argc, char** argv) {
         }
         std::cout << "  n=" << A.nrows << "  nnz=" << A.nnz() << "\n";
         const int n = A.nrows;
-        std::vector<double> b(n), xs(n, 1.0);
-        if (rhs == "normal") {
-            b = normals(seed, n);
-        } else {
-            if (rhs == "uniform") {
-                SplitMix64 r(seed);
-                for (auto& v : xs) v = 2.0 * r.u01() - 1.0;
+        auto make_rhs = [&](uint64_t sd) {
+            std::vector<double> bb(n), xs(n, 1.0);
+            if (rhs == "normal") {
+                bb = normals(sd, n);
+            } else {
+                if (rhs == "uniform") {
+                    SplitMix64 r(sd);
+                    for (auto& v : xs) v = 2.0 * r.u01() - 1.0;
+                }
+                A.spmv(xs, bb);  // src/main.cpp:153-155
             }
-            A.spmv(xs, b);  // src/main.cpp:153-155
+            return bb;
+        };
+        std::vector<double> b = make_rhs(seed);
+        if (batch > 0) {
+            sg_precond_kind kind = SG_PC_NONE;
+            if (precond == "jacobi") kind = SG_PC_JACOBI;
+            else if (precond == "ilu0") kind = SG_PC_ILU0;
+            else if (precond == "asm") kind = SG_PC_ASM;
+            else if (precond == "ras") kind = SG_PC_RAS;
+            else if (precond == "twolevel_asm") kind = SG_PC_TWOLEVEL_ASM;
+            else if (precond == "twolevel_ras") kind = SG_PC_TWOLEVEL_RAS;
+            std::vector<CSRMatrix> sys(batch, A);
+            std::vector<std::vector<double>> bs, xs(batch);
+            for (int q = 0; q < batch; ++q) bs.push_back(make_rhs(seed + q));
+            prm.ortho = ortho == "cgs2" ? SG_ORTHO_CGS2 : SG_ORTHO_MGS;
+            std::cout << "Solving " << batch << " systems together with "
+                      << (use_fgmres ? "FGMRES(" : "GMRES(") << prm.restart << ") + " << precond << "\n";
+            auto t0b = std::chrono::steady_clock::now();
+            std::vector<GMRESResult> rs =
+                solve_batch(sys, bs, xs, kind, nparts, overlap, precision == "fp32" ? SG_FP32 : SG_FP64,
+                            prm, use_fgmres);
+            const double tb = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0b).count();
+            int conv = 0, itmin = 1 << 30, itmax = 0;
+            double worst = 0.0;
+            for (const auto& r : rs) {
+                conv += r.converged ? 1 : 0;
+                itmin = std::min(itmin, r.iterations);
+                itmax = std::max(itmax, r.iterations);
+                worst = std::max(worst, r.rel_true_residual);
+            }
+            std::cout << "  Converged: " << (conv == batch ? "yes" : "no") << " (" << conv << "/" << batch << ")\n"
+                      << "  Iterations: " << itmin << ".." << itmax << "\n"
+                      << "  Residual: " << rs.back().residual_norm << "\n"
+                      << "  Solve time: " << tb << " s\n"
+                      << "  Relative true residual: " << worst << " (worst)\n";
+            if (!output.empty()) {
+                Metrics m;
+                m.matrix = matrix;
+                m.method = std::string("BATCH_") + (use_fgmres ? "FGMRES+" : "GMRES+") + precond;
+                m.n = n * batch;
+                m.nnz = A.nnz() * batch;
+                m.gpus = 1;
+                m.overlap = overlap;
+                m.restart = prm.restart;
+                m.tol = prm.tol;
+                m.iterations = itmax;
+                m.residual_norm = rs.back().residual_norm;
+                m.time_solve_s = tb;
+                m.extra.emplace_back("systems", std::to_string(batch));
+                m.write_jsonl(output);
+            }
+            return conv == batch ? 0 : 1;
         }
 
         std::unique_ptr<Preconditioner> M;
@@ -236,9 +300,24 @@ int main(int argc, char** argv) {
         prm.ortho = ortho == "cgs2" ? SG_ORTHO_CGS2 : SG_ORTHO_MGS;
         const std::string label = use_fgmres ? "FGMRES(" + std::to_string(prm.restart) + ")"
                                              : "GMRES(" + std::to_string(prm.restart) + ")";
-        std::cout << "Solving with " << label << " + " << precond << "\n";
+        std::unique_ptr<LearnedStage> stage;
+        if (!learned.empty()) {
+            if (poisson <= 0 || !D)
+                throw std::runtime_error("--learned needs a 2D --poisson grid and a device --precond");
+            stage.reset(new LearnedStage(*D, load_npy_f32(learned), poisson, poisson,
+                                         std::vector<double>((size_t)n, 0.0), 0.01, learned_every));
+            stage->setup(A);
+            use_fgmres = true;  // the stage changes per application
+        }
+        const std::string label2 = use_fgmres ? "FGMRES(" + std::to_string(prm.restart) + ")" : label;
+        std::cout << "Solving with " << label2 << " + " << precond << (stage ? " + learned stage" : "") << "\n";
         std::vector<double> x(n, 0.0);
-        GMRESResult res = use_fgmres ? fgmres(A, b, x, M.get(), prm) : gmres(A, b, x, M.get(), prm);
+        const Preconditioner* P = stage ? static_cast<const Preconditioner*>(stage.get()) : M.get();
+        GMRESResult res = use_fgmres ? fgmres(A, b, x, P, prm) : gmres(A, b, x, P, prm);
+        if (stage) {
+            auto c = stage->counts();
+            std::cout << "  Learned stage: " << c.second << "/" << c.first << " corrections kept\n";
+        }
         std::cout << "  Converged: " << (res.converged ? "yes" : "no") << "\n"
                   << "  Iterations: " << res.iterations << "\n"
                   << "  Residual: " << res.residual_norm << "\n"
@@ -248,7 +327,7 @@ int main(int argc, char** argv) {
         if (!output.empty()) {
             Metrics m;
             m.matrix = matrix;
-            m.method = (use_fgmres ? "FGMRES+" : "GMRES+") + precond;
+            m.method = (use_fgmres ? "FGMRES+" : "GMRES+") + precond + (stage ? "+learned" : "");
             m.n = n;
             m.nnz = A.nnz();
             m.ranks = 1;

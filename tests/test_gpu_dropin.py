@@ -60,3 +60,41 @@ def test_cli_two_level_fgmres_mixed():
     assert out.returncode == 0, out.stdout + out.stderr
     rel = float(out.stdout.split("Relative true residual: ")[1].split()[0])
     assert rel <= 1e-8
+
+
+def _cli(*args, timeout=300):
+    out = subprocess.run([os.path.join(PKG, "schwarz_solve"), *args], capture_output=True,
+                         text=True, timeout=timeout)
+    lines = dict(l.strip().split(": ", 1) for l in out.stdout.splitlines() if ": " in l)
+    return out, lines
+
+
+def test_cli_batch_equals_single_solves(tmp_path):
+    common = ["--poisson", "48", "--precond", "ras", "--nparts", "4", "--fgmres", "--ortho",
+              "cgs2", "--rtol", "1e-8", "--rhs", "normal"]
+    jl = tmp_path / "b.jsonl"
+    out, lines = _cli(*common, "--seed", "7", "--batch", "3", "--output", str(jl))
+    assert out.returncode == 0, out.stderr
+    assert lines["Converged"].startswith("yes (3/3)")
+    lo, hi = map(int, lines["Iterations"].split(".."))
+    its = []
+    for s in (7, 8, 9):
+        o, l = _cli(*common, "--seed", str(s))
+        assert o.returncode == 0, o.stderr
+        its.append(int(l["Iterations"]))
+    assert (lo, hi) == (min(its), max(its))
+    rec = json.loads(jl.read_text().splitlines()[-1])
+    assert rec["method"] == "BATCH_FGMRES+ras" and rec["systems"] == 3
+
+
+def test_cli_learned_stage():
+    w = os.path.join(PKG, "weights", "unet_grf256.npy")
+    out, lines = _cli("--poisson", "64", "--precond", "twolevel_ras", "--nparts", "8",
+                      "--overlap", "2", "--precision", "fp32", "--rtol", "1e-8", "--ortho", "cgs2",
+                      "--learned", w)
+    assert out.returncode == 0, out.stderr
+    assert "+ learned stage" in out.stdout
+    assert lines["Converged"] == "yes"
+    assert float(lines["Relative true residual"]) <= 1e-8
+    kept, calls = map(int, lines["Learned stage"].split()[0].split("/"))
+    assert calls >= int(lines["Iterations"]) and 0 <= kept <= calls
