@@ -311,6 +311,12 @@ sg_status sg_context_set_profiling(sg_context* ctx, int enable);
  * changes results.  No counterpart in the reference (its kernels have fixed
  * launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
 sg_status sg_context_set_autotune(sg_context* ctx, int mode);
+/* Tuning cache: the autotuner (mode 1) first looks a matrix / preconditioner
+ * up by a key (sizes plus a hash of its sparsity structure) and uses the
+ * stored shape without timing; new choices are appended to the file at path
+ * (one "key shape" per line), which this call loads.  Makes the selection
+ * deterministic across runs.  NULL or "" keeps the cache in memory only. */
+sg_status sg_context_set_tune_cache(sg_context* ctx, const char* path);
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
                                   double* total_ms);
 sg_status sg_context_reset_stats(sg_context* ctx);

@@ -126,6 +126,7 @@ def lib():
         "sg_precond_stats": ([_VP, C.POINTER(PrecondInfo)], C.c_int),
         "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
         "sg_context_set_autotune": ([_VP, C.c_int], C.c_int),
+        "sg_context_set_tune_cache": ([_VP, C.c_char_p], C.c_int),
         "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double)], C.c_int),
         "sg_context_reset_stats": ([_VP], C.c_int),
@@ -170,6 +171,9 @@ class Context:
         """0 fixed shapes, 1 time shapes on first use, 2..6 force a shape (see
         sg_context_set_autotune)."""
         self.check(lib().sg_context_set_autotune(self.h, int(mode)))
+
+    def tune_cache(self, path):
+        self.check(lib().sg_context_set_tune_cache(self.h, str(path).encode() if path else None))
 
     def kernel_stats(self, family):
         n = C.c_longlong()

@@ -4,7 +4,9 @@
 
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <stdexcept>
 
 namespace sg {
@@ -136,6 +138,34 @@ static void nrm_launch(sg_context* ctx, int n, const double* a, double* result) 
 static int ew_grid(int n) {
     int g = (n + 255) / 256;
     return g < 1 ? 1 : (g > 8192 ? 8192 : g);
+}
+
+uint64_t fnv1a(const void* data, size_t bytes, uint64_t h) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+int tune_lookup(sg_context* ctx, const std::string& key) {
+    for (const auto& kv : ctx->tune_kv)
+        if (kv.first == key) return kv.second;
+    return -1;
+}
+
+void tune_store(sg_context* ctx, const std::string& key, int value) {
+    for (auto& kv : ctx->tune_kv)
+        if (kv.first == key) {
+            kv.second = value;
+            return;
+        }
+    ctx->tune_kv.emplace_back(key, value);
+    if (!ctx->tune_path.empty()) {
+        std::ofstream out(ctx->tune_path, std::ios::app);
+        out << key << " " << value << "\n";
+    }
 }
 
 }  // namespace sg
@@ -322,6 +352,27 @@ sg_status sg_context_set_autotune(sg_context* ctx, int enable) {
     if (!ctx || enable < 0 || enable > 6) return SG_E_ARG;
     ctx->autotune = enable;
     return SG_OK;
+}
+
+sg_status sg_context_set_tune_cache(sg_context* ctx, const char* path) {
+    if (!ctx) return SG_E_ARG;
+    return run_guarded(ctx, [&] {
+        ctx->tune_path = path ? path : "";
+        if (ctx->tune_path.empty()) return;
+        std::ifstream in(ctx->tune_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string key;
+            int v;
+            if (ls >> key >> v) {
+                bool found = false;
+                for (auto& kv : ctx->tune_kv)
+                    if (kv.first == key) kv.second = v, found = true;
+                if (!found) ctx->tune_kv.emplace_back(key, v);
+            }
+        }
+    });
 }
 
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,

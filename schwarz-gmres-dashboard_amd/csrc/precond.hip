@@ -20,6 +20,7 @@
 #include "sg_internal.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <atomic>
@@ -1309,6 +1310,18 @@ static void launch_any_sweep(sg_precond* M, const double* x, double* y, const De
 template <typename T>
 static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd, T* zg, T* zo) {
     hipStream_t s = M->ctx->stream;
+    uint64_t h = fnv1a(M->sub_nnz.data(), sizeof(int) * M->sub_nnz.size());
+    h = fnv1a(M->lrp_host.data(), sizeof(int) * M->lrp_host.size(), h);
+    char key[160];
+    std::snprintf(key, sizeof key, "sweep:%d:%d:%d:%lld:%lld:%d:%016llx", (int)M->kind,
+                  (int)M->opts.precision, M->nsub, M->levels_fwd, M->levels_bwd, M->max_ns,
+                  (unsigned long long)h);
+    const int cached = tune_lookup(M->ctx, key);
+    if (cached >= 0 && cached < kNumSweepShapes) {
+        M->sweep_variant = cached;
+        M->sweep_tuned = true;
+        return;
+    }
     DevBuf<double> xin, yout;
     xin.alloc(std::max(M->n, 1));
     yout.alloc(std::max(M->n, 1));
@@ -1339,6 +1352,7 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
     (void)hipEventDestroy(e1);
     M->sweep_variant = bestv;
     M->sweep_tuned = true;
+    tune_store(M->ctx, key, bestv);
 }
 
 template <typename T>

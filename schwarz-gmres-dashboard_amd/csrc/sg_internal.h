@@ -106,6 +106,9 @@ struct sg_context {
     double* h_pinned = nullptr;  // pinned staging for small reads
     bool profiling = false;
     int autotune = 1;  // 0 default shapes, 1 time them on first use, 2+v force shape v
+    // tuning cache: key -> chosen shape, mirrored in a text file when a path is set
+    std::string tune_path;
+    std::vector<std::pair<std::string, int>> tune_kv;
     sg::Prof prof[SG_K_COUNT];
     std::vector<hipEvent_t> event_pool;
 
@@ -140,6 +143,12 @@ struct ProfScope {
     ProfScope(sg_context* c, int f);
     ~ProfScope();
 };
+
+// ---- tuning cache (context.hip) ---------------------------------------------------
+// 64-bit FNV-1a over raw bytes, chained through h.
+uint64_t fnv1a(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull);
+int tune_lookup(sg_context* ctx, const std::string& key);  // -1 when absent
+void tune_store(sg_context* ctx, const std::string& key, int value);
 
 // ---- SpMV (spmv.hip) ---------------------------------------------------------
 // y = alpha*A*x + beta*y, skipping the rows of stopped systems (see Guard).

@@ -2,15 +2,17 @@
 //
 // LDS-staged "CSR stream": a workgroup of NT threads (NT = 256, 512 or 1024)
 // owns a block of up to NT consecutive rows whose nonzeros (at most 8*NT) are
-// streamed with fully coalesced loads; each product vals[k]*x[col[k]] is rounded
-// and parked in LDS, then one lane per row adds its products sequentially in CSR
-// order.  The sum therefore equals the reference's serial row loop
+// streamed with fully coalesced loads (eight values and column indices per
+// thread issued together, then the eight x gathers, and the row bounds loaded
+// before the pass); each product vals[k]*x[col[k]] is rounded and parked in
+// LDS, then one lane per row adds its products sequentially in CSR order.  The sum therefore equals the reference's serial row loop
 // (src/core/sparse_matrix.h:30-37) bit for bit, whatever NT is.  Blocks are
 // remapped so that each of the 8 XCDs walks one contiguous eighth of the rows,
 // which keeps the x neighbourhood of a stencil (+-nx*ny rows) inside that XCD's L2.
 #include "sg_internal.h"
 
 #include <algorithm>
+#include <cstdio>
 
 namespace sg {
 
@@ -34,16 +36,29 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
     const int k0 = rp[r0], k1 = rp[r1];
     const int t = threadIdx.x;
     __shared__ double prod[kCap];
+    const int rr = min(r0 + t, r1 - 1);
+    const int ra = rp[rr], re = rp[rr + 1];  // row bounds, loaded before the staging pass
     if (k1 - k0 <= kCap) {
-        for (int k = k0 + t; k < k1; k += NT) {
-            const double a = __builtin_nontemporal_load(&v[k]);
-            const int c = __builtin_nontemporal_load(&ci[k]);
-            prod[k - k0] = a * x[c];
+        for (int kb = k0 + t; kb < k1; kb += 8 * NT) {
+            double a[8];
+            int c[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = min(kb + u * NT, k1 - 1);
+                a[u] = __builtin_nontemporal_load(&v[k]);
+                c[u] = __builtin_nontemporal_load(&ci[k]);
+            }
+            double xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (kb + u * NT < k1) prod[kb + u * NT - k0] = a[u] * xv[u];
         }
         __syncthreads();
         const int r = r0 + t;
         if (r < r1) {
-            const int a = rp[r] - k0, e = rp[r + 1] - k0;
+            const int a = ra - k0, e = re - k0;
             double s = 0.0;
             for (int k = a; k < e; ++k) s += prod[k];
             y[r] = beta == 0.0 ? alpha * s : alpha * s + beta * y0[r];
@@ -123,6 +138,17 @@ static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* 
 void spmv_autotune(sg_csr* A) {
     sg_context* ctx = A->ctx;
     hipStream_t s = ctx->stream;
+    uint64_t h = fnv1a(A->host.rp.data(), sizeof(int) * A->host.rp.size());
+    h = fnv1a(A->host.ci.data(), sizeof(int) * A->host.ci.size(), h);
+    h = fnv1a(A->sys_off.data(), sizeof(int) * A->sys_off.size(), h);
+    char key[96];
+    std::snprintf(key, sizeof key, "spmv:%d:%d:%016llx", A->nrows, A->nnz, (unsigned long long)h);
+    const int cached = tune_lookup(ctx, key);
+    if (cached >= 0 && cached < 3) {
+        A->variant = cached;
+        A->tuned = true;
+        return;
+    }
     DevBuf<double> x, y;
     x.alloc(std::max(A->ncols, 1));
     y.alloc(std::max(A->nrows, 1));
@@ -149,6 +175,7 @@ void spmv_autotune(sg_csr* A) {
     (void)hipEventDestroy(e1);
     A->variant = bestv;
     A->tuned = true;
+    tune_store(ctx, key, bestv);
 }
 
 void spmv_ensure_tuned(sg_csr* A) {

@@ -342,3 +342,34 @@ def test_errors_are_reported(sg, ctx):
     with pytest.raises(sg.SgError) as e:
         sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=0, partition=part).setup(dZ)
     assert e.value.status == 3
+
+
+def test_tune_cache_makes_selection_deterministic(sg, tmp_path):
+    """The autotuner stores its choices under a structural key; a context that
+    loads the cache uses the stored shapes without timing (here a forced, edited
+    entry), and results do not depend on the shape."""
+    A = O.poisson3d(40, 36, 30)
+    x = np.random.default_rng(21).standard_normal(A.n)
+    path = tmp_path / "tune.txt"
+    out = []
+    for round_ in range(2):
+        c = sg.Context(0)
+        c.tune_cache(path)
+        dA = sg.Csr(c, A.rp, A.ci, A.v)
+        M = sg.Precond(c, "ras", nparts=16, overlap=1, precision="fp32",
+                       partition=np.arange(A.n, dtype=np.int32) * 16 // A.n).setup(dA)
+        dx, dy, dz = dev(sg, c, x), dev(sg, c, np.zeros(A.n)), dev(sg, c, np.zeros(A.n))
+        dA.spmv(dx, dy)
+        M.apply(dx, dz)
+        out.append((M.stats()["sweep_shape"], dy.get(), dz.get()))
+        if round_ == 0:
+            keys = dict(l.split() for l in path.read_text().splitlines())
+            assert len(keys) == 2 and any(k.startswith("spmv:") for k in keys)
+            sweep_key = [k for k in keys if k.startswith("sweep:")][0]
+            forced = 0 if int(keys[sweep_key]) != 0 else 2
+            keys[sweep_key] = str(forced)
+            path.write_text("".join(f"{k} {v}\n" for k, v in keys.items()))
+        del M, dA, dx, dy, dz
+        c.close()
+    assert out[1][0] == forced
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
