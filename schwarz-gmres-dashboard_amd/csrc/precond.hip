@@ -48,6 +48,7 @@ constexpr int kNumSweepShapes = 5;
 constexpr SweepShape kSweepShapes[kNumSweepShapes] = {
     {256, false}, {512, false}, {1024, false}, {512, true}, {1024, true}};
 constexpr int kKreg = 4;  // factor entries per row prefetched in registers
+constexpr int kRestrictSplit = 8;  // workgroups per subdomain in the coarse restriction
 constexpr int kSfDepth = SG_SF_DEPTH;  // items of row data in flight per wave (sync-free sweep)
 
 struct SubInfo {
@@ -1182,8 +1183,10 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
                        invT.p, M->d_inv64.p);
     SG_HIP(hipGetLastError());
     M->d_c64.upload(c, st);
-    // the restriction splits each subdomain over enough workgroups to fill the GPU
-    M->restrict_split = std::max(1, std::min(64, (4 * ctx->num_cu + nc - 1) / nc));
+    // the restriction splits each subdomain over a fixed number of workgroups,
+    // so its summation order does not depend on how many subdomains there are
+    // (a system gives the same coarse right-hand side alone or in a batch)
+    M->restrict_split = kRestrictSplit;
     if (M->opts.precision == SG_FP32) {
         int nn = nc * nc;
         M->d_LUt32.alloc(nn);

@@ -81,6 +81,8 @@ def _batch(sg, ctx, systems, kind, precision, kw, graph=True, nparts=None):
     ("ras", "fp32", "fgmres", "cgs2"),
     ("ras", "fp64", "gmres", "mgs"),
     ("asm", "fp64", "fgmres", "cgs2"),
+    # block-diagonal coarse matrix: the LU coarse solve factors it block by block
+    ("twolevel_ras", "fp64", "fgmres", "cgs2"),
 ])
 def test_batch_equals_single_solves(sg, ctx, P, kind, precision, method, ortho):
     systems = _small_batch(P)
