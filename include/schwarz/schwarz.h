@@ -140,10 +140,78 @@ GMRESResult gmres(const CSRMatrix& A, const std::vector<double>& b, std::vector<
 GMRESResult fgmres(const CSRMatrix& A, const std::vector<double>& b, std::vector<double>& x,
                    const Preconditioner* M, const GMRESParams& params = {});
 
+// Right-preconditioned BiCGSTAB (src/core/bicgstab.h:57-204): params.tol is
+// RELATIVE to ||r0||, max_iter bounds iterations, restart is unused.
+GMRESResult bicgstab(const CSRMatrix& A, const std::vector<double>& b, std::vector<double>& x,
+                     const Preconditioner* M, const GMRESParams& params = {});
+
 using FlexApply = std::function<void(const double*, double*, int)>;
 GMRESResult fgmres_flex(const CSRMatrix& A, const std::vector<double>& b,
                         std::vector<double>& x, FlexApply prec_apply,
                         const GMRESParams& params = {});
+
+// Matrix Market coordinate files (src/io/matrix_market.h:14-103).  The
+// one-argument reader behaves as the reference's (header flags from any '%'
+// line, symmetric mirroring, entries sorted by (row, col), duplicates kept);
+// the options fix its duplicate handling and header parsing (sg_mm_flags).
+// Errors throw std::runtime_error ("Cannot open file: ...", "Premature end of
+// file", or a message naming the file and line).
+struct MatrixMarketOptions {
+    bool sum_duplicates = false;  // merge entries with the same (row, col)
+    bool strict = false;          // banner-only flags, skew sign, blank lines, clean errors
+};
+CSRMatrix read_matrix_market(const std::string& filename);
+CSRMatrix read_matrix_market(const std::string& filename, const MatrixMarketOptions& opts);
+// digits: significant digits of the values (6 = the reference's output; 17 round-trips)
+void write_matrix_market(const std::string& filename, const CSRMatrix& A, int digits = 6);
+
+// ---- convergence analytics (src/analysis/*.h; called under --analyze) ----------
+
+// Per-run residual-history metrics (convergence_analysis.h:15-106).
+struct ConvergenceMetrics {
+    std::vector<double> residuals;
+    std::vector<double> rates;  // (r_k / r_0)^(1/k)
+    double initial_residual = 0.0;
+    double final_residual = 0.0;
+    double asymptotic_rate = 0.0;  // mean of the last 20 rates
+    int stagnation_start = -1;     // -1 = none
+    bool converged = false;
+    int iterations = 0;
+    static constexpr int STAGNATION_WINDOW = 20;
+    static constexpr double STAGNATION_TOL = 1e-3;
+
+    void compute(const std::vector<double>& res_history, bool conv);
+    // appends one "type":"convergence" JSON line
+    void write_jsonl(const std::string& path, const std::string& matrix,
+                     const std::string& precond, int nparts, int overlap) const;
+    void print_summary() const;
+};
+ConvergenceMetrics analyze_convergence(const std::vector<double>& residual_history,
+                                       bool converged);
+
+// Lanczos estimate of kappa(M^{-1}A) (condition_estimator.h:31-211), on the device.
+struct ConditionEstimate {
+    double lambda_min = 0.0;
+    double lambda_max = 0.0;
+    double kappa = 0.0;
+    int lanczos_steps = 0;
+    bool valid = false;
+};
+ConditionEstimate estimate_condition(const CSRMatrix& A, const Preconditioner* M, int k = 80,
+                                     unsigned int seed = 42);
+void print_condition_estimate(const ConditionEstimate& est, const std::string& label = "");
+
+// Power-iteration estimate of rho(I - M^{-1}A) (spectral_analysis.h:19-107), on the device.
+struct SpectralResult {
+    double spectral_radius = 0.0;
+    bool convergence_ok = false;
+    int power_iters = 0;
+    std::vector<double> eigenvector;
+};
+SpectralResult estimate_spectral_radius(const CSRMatrix& A, const Preconditioner* M,
+                                        int max_iter = 200, double tol = 1e-6,
+                                        unsigned int seed = 123);
+void print_spectral_result(const SpectralResult& sr, const std::string& label = "");
 
 // ---- extensions without a reference counterpart ----------------------------------
 
@@ -185,6 +253,12 @@ std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
                                      std::vector<std::vector<double>>& x, sg_precond_kind kind,
                                      int nparts, int overlap, sg_precision precision,
                                      const GMRESParams& params, bool flexible);
+// The same with any sg_method (SG_BICGSTAB included).
+std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
+                                     const std::vector<std::vector<double>>& b,
+                                     std::vector<std::vector<double>>& x, sg_precond_kind kind,
+                                     int nparts, int overlap, sg_precision precision,
+                                     const GMRESParams& params, sg_method method);
 
 // One JSON object per line, appended (field order and number formats as the
 // reference's logger; extra fields may only be appended after "efficiency").

@@ -68,6 +68,31 @@ sg_status sg_csr_info(const sg_csr* A, int* nrows, int* ncols, int* nnz);
  * bit. */
 sg_status sg_spmv(sg_csr* A, double alpha, const double* d_x, double beta, double* d_y);
 
+/* ---- Matrix Market files (host only; no GPU needed) -------------------------
+ * Replaces read_matrix_market / write_matrix_market (src/io/matrix_market.h:14-103).
+ * flags = 0 reads as the reference does: '%' lines whose text contains
+ * "symmetric" / "pattern" set those flags, symmetric files are mirrored,
+ * entries are sorted by (row, col) and duplicates stay separate entries
+ * (in file order).  Inputs on which the reference is undefined (unreadable
+ * sizes or indices, indices out of range) fail with SG_E_IO.  The arrays are
+ * allocated by the library (row_ptr[nrows+1], col_idx[nnz], vals[nnz]) and
+ * released with sg_mm_free. */
+typedef enum {
+    SG_MM_SUM_DUPLICATES = 1, /* merge entries with the same (row, col) by summing  */
+    SG_MM_STRICT = 2          /* header flags from the %%MatrixMarket banner only;
+                                 skew-symmetric mirrored with negation; blank and
+                                 comment lines skipped anywhere; malformed values,
+                                 array / complex files and extra entries rejected   */
+} sg_mm_flags;
+sg_status sg_mm_read(const char* path, int flags, int* nrows, int* ncols, int* nnz,
+                     int** row_ptr, int** col_idx, double** vals);
+void sg_mm_free(void* p);
+/* "general" banner, sizes, one "row col value" line per stored entry (1-based),
+ * values in %g style with `digits` significant digits (6 = the reference's
+ * stream default; 17 round-trips). */
+sg_status sg_mm_write(const char* path, int nrows, int ncols, const int* row_ptr,
+                      const int* col_idx, const double* vals, int digits);
+
 /* ---- BLAS-1 (FP64, device vectors) ----------------------------------------
  * Replace schwarz::dot/norm2/axpy/scale (src/core/vector.h:11-48) and
  * gpu::cublas_dot/nrm2/axpy/scal (src/gpu/cuda_blas.h:10-39).  Reductions use a
@@ -172,7 +197,17 @@ sg_status sg_precond_coarse(const sg_precond* M, int* nc, double* A0, double* LU
  * Replaces gmres (src/core/gmres.h:28-33), fgmres / fgmres_flex
  * (src/core/fgmres.h:42-55) and gpu::gmres_gpu / solve_gpu
  * (src/gpu/gmres_gpu.h:25-72). */
-typedef enum { SG_GMRES_LEFT = 0, SG_FGMRES = 1 } sg_method;
+typedef enum {
+    SG_GMRES_LEFT = 0,
+    SG_FGMRES = 1,
+    /* Right-preconditioned BiCGSTAB, replaces bicgstab (src/core/bicgstab.h:57-204).
+     * tol_abs is then the reference's RELATIVE tolerance (stop when ||s|| or
+     * ||r|| < tol * ||r0||, bicgstab.h:140,180); max_iter bounds iterations;
+     * restart and ortho are ignored; rtol must be 0.  hist receives ||r|| after
+     * every full iteration (and ||s|| at a half-step exit).  Batches, graphs
+     * (SG_SOLVE_GRAPH / SG_SOLVE_NO_GRAPH) and lookahead work as for GMRES. */
+    SG_BICGSTAB = 2
+} sg_method;
 typedef enum { SG_ORTHO_MGS = 0, SG_ORTHO_CGS2 = 1 } sg_ortho;
 
 typedef struct {
@@ -204,8 +239,11 @@ typedef struct {
     double residual_norm;     /* GMRESResult::residual_norm (last estimate)           */
     double rel_true_residual; /* ||b - A x|| / ||b|| in FP64 at exit                  */
     double time_solve_s;      /* GMRESResult::time_solve_s (wall, includes the sync)  */
-    int cycles;               /* restart cycles run                                   */
+    int cycles;               /* restart cycles run (0 for BiCGSTAB)                  */
     int precond_applies;
+    int hist_len;             /* residual-history entries produced (GMRES: iterations;
+                                 BiCGSTAB: full iterations, plus one at a half-step
+                                 exit); hist holds the first min(hist_len, hist_cap)  */
 } sg_solve_result;
 
 /* x (device, in/out) is the initial guess and the solution.  hist (host, may be
@@ -288,6 +326,50 @@ sg_status sg_precond_create_learned(sg_context* ctx, sg_precond* base, sg_unet* 
                                     const sg_learned_opts* opts, sg_precond** out);
 /* applications so far, and how many of them kept the network's correction */
 sg_status sg_precond_learned_stats(const sg_precond* M, long long* calls, long long* accepted);
+
+/* ---- convergence analytics ----------------------------------------------------
+ * Replace src/analysis/condition_estimator.h, spectral_analysis.h and
+ * convergence_analysis.h (called by src/main.cpp:211-220,255-261 under
+ * --analyze). */
+typedef struct {
+    double lambda_min, lambda_max, kappa; /* extreme Ritz values, lambda_max / lambda_min
+                                             (1e30 when |lambda_min| <= 1e-14)        */
+    int lanczos_steps;
+    int valid;                            /* lambda_min > 0 (meaningful for SPD)      */
+} sg_condition_estimate;
+/* ConditionEstimate estimate_condition(A, M, k, seed) (condition_estimator.h:94-191):
+ * k Lanczos steps on M^{-1}A (A when M is NULL) from the reference's LCG start
+ * vector, stopping early on an invariant subspace; the Ritz extremes of the
+ * tridiagonal.  alpha[k] / beta[k-1] (host, may be NULL) receive its
+ * coefficients.  All zero when min(k, n) < 2 or a coefficient is not finite. */
+sg_status sg_estimate_condition(sg_csr* A, sg_precond* M, int k, unsigned seed,
+                                sg_condition_estimate* out, double* alpha, double* beta);
+typedef struct {
+    double spectral_radius; /* |mu| of the step before the stopping one, as the reference */
+    int convergence_ok;     /* spectral_radius < 1                                        */
+    int power_iters;
+} sg_spectral_result;
+/* estimate_spectral_radius(A, M, max_iter, tol, seed) (spectral_analysis.h:33-95):
+ * power iteration on I - M^{-1}A with Rayleigh quotients, stopping when two
+ * successive quotients agree to tol * (1 + |mu|) or ||E v|| < 1e-14.
+ * d_eigvec (device, n, may be NULL) receives the last iterate. */
+sg_status sg_estimate_spectral_radius(sg_csr* A, sg_precond* M, int max_iter, double tol,
+                                      unsigned seed, sg_spectral_result* out, double* d_eigvec);
+typedef struct {
+    double initial_residual, final_residual, asymptotic_rate;
+    int stagnation_start; /* -1 = none */
+    int converged;
+    int iterations;       /* history length */
+} sg_convergence;
+/* ConvergenceMetrics::compute (convergence_analysis.h:29-66); rates (may be
+ * NULL) receives the n per-step rates.  Host only. */
+sg_status sg_convergence_analyze(const double* hist, int n, int converged, sg_convergence* out,
+                                 double* rates);
+/* ConvergenceMetrics::write_jsonl (convergence_analysis.h:69-93): appends one
+ * "type":"convergence" line to path.  Host only. */
+sg_status sg_convergence_write_jsonl(const char* path, const char* matrix, const char* precond,
+                                     int nparts, int overlap, const double* hist, int n,
+                                     int converged);
 
 /* ---- kernel timing (HIP events on the context stream) ------------------------
  * With profiling on, every launch of the listed kernel families is bracketed

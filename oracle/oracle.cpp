@@ -533,6 +533,106 @@ Result fgmres(const Csr& A, const double* b, double* x, const Precond* M, int m,
     return res;
 }
 
+// Right-preconditioned BiCGSTAB: src/core/bicgstab.h:57-204.  Relative tests
+// against ||r0||; breakdown thresholds 1e-300; the half-step exit keeps
+// x += alpha y and records ||s||.
+Result bicgstab(const Csr& A, const double* b, double* x, const Precond* M, int max_iter,
+                double tol) {
+    const int n = A.n;
+    Result res;
+    std::vector<double> r(n), rh(n), p(n, 0.0), v(n, 0.0), y(n), z(n), s(n), t(n), Ax(n);
+    spmv(A, x, Ax.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - Ax[i];  // bicgstab.h:78-79
+    rh = r;
+    const double r0 = norm2(r.data(), n);
+    if (r0 < 1e-14) {  // bicgstab.h:85-90
+        res.residual_norm = r0;
+        res.converged = true;
+        return res;
+    }
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    for (int it = 1; it <= max_iter; ++it) {
+        const double rho_prev = rho;
+        rho = dot(rh.data(), r.data(), n);
+        if (std::abs(rho) < 1e-300) {  // bicgstab.h:102-108
+            res.iterations = it;
+            res.residual_norm = norm2(r.data(), n);
+            res.converged = false;
+            break;
+        }
+        if (it == 1) {
+            p = r;
+        } else {
+            const double beta = (rho / rho_prev) * (alpha / omega);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        if (M) M->apply(p.data(), y.data());
+        else y = p;
+        spmv(A, y.data(), v.data());
+        const double rv = dot(rh.data(), v.data(), n);
+        if (std::abs(rv) < 1e-300) {  // bicgstab.h:127-132
+            res.iterations = it;
+            res.residual_norm = norm2(r.data(), n);
+            res.converged = false;
+            break;
+        }
+        alpha = rho / rv;
+        for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        const double sn = norm2(s.data(), n);
+        if (sn < tol * r0) {  // bicgstab.h:140-148
+            for (int i = 0; i < n; ++i) x[i] += alpha * y[i];
+            res.iterations = it;
+            res.residual_norm = sn;
+            res.converged = true;
+            res.hist.push_back(sn);
+            break;
+        }
+        if (M) M->apply(s.data(), z.data());
+        else z = s;
+        spmv(A, z.data(), t.data());
+        const double tt = dot(t.data(), t.data(), n);
+        if (tt < 1e-300) {  // bicgstab.h:158-165
+            for (int i = 0; i < n; ++i) x[i] += alpha * y[i];
+            res.iterations = it;
+            res.residual_norm = sn;
+            res.converged = sn < tol * r0;
+            break;
+        }
+        omega = dot(t.data(), s.data(), n) / tt;
+        for (int i = 0; i < n; ++i) x[i] += alpha * y[i] + omega * z[i];
+        for (int i = 0; i < n; ++i) r[i] = s[i] - omega * t[i];
+        const double rn = norm2(r.data(), n);
+        res.hist.push_back(rn);
+        res.iterations = it;
+        res.residual_norm = rn;
+        if (rn < tol * r0) {
+            res.converged = true;
+            break;
+        }
+        if (std::abs(omega) < 1e-300) break;  // bicgstab.h:187-193
+    }
+    return res;
+}
+
+// Unit start vector of the analysis routines: a 32-bit LCG (multiplier
+// 1664525, increment 1013904223), each state read as a signed int / 2^31,
+// normalised (src/analysis/condition_estimator.h:99-112,
+// spectral_analysis.h:42-53).
+std::vector<double> lcg_unit(int n, unsigned seed, bool guard_zero) {
+    std::vector<double> v(n);
+    unsigned st = seed;
+    double nn = 0.0;
+    for (int i = 0; i < n; ++i) {
+        st = st * 1664525u + 1013904223u;
+        v[i] = static_cast<double>(static_cast<int>(st)) / 2147483648.0;
+        nn += v[i] * v[i];
+    }
+    nn = std::sqrt(nn);
+    if (guard_zero && nn < 1e-14) nn = 1.0;
+    for (int i = 0; i < n; ++i) v[i] /= nn;
+    return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -708,6 +808,149 @@ int or_metrics_json(const char* matrix, const char* method, int n, int nnz, int 
     if ((int)s.size() + 1 > cap) return -(int)s.size() - 1;
     std::memcpy(out, s.c_str(), s.size() + 1);
     return (int)s.size();
+}
+
+
+// src/core/bicgstab.h:57-204; returns iterations.
+int or_bicgstab(int n, const int* rp, const int* ci, const double* v, const double* b, double* x,
+                void* pc, int max_iter, double tol, double* residual_norm, int* converged,
+                double* hist, int hist_cap, int* hist_len) {
+    Csr a = make_csr(n, rp, ci, v);
+    Result r = bicgstab(a, b, x, static_cast<const Precond*>(pc), max_iter, tol);
+    if (residual_norm) *residual_norm = r.residual_norm;
+    if (converged) *converged = r.converged ? 1 : 0;
+    const int hl = (int)r.hist.size();
+    if (hist) for (int i = 0; i < hl && i < hist_cap; ++i) hist[i] = r.hist[i];
+    if (hist_len) *hist_len = hl;
+    return r.iterations;
+}
+
+// Lanczos recurrence of estimate_condition (src/analysis/condition_estimator.h:94-151):
+// alpha[m], beta[m-1] of the tridiagonal; returns m (0 when k < 2).
+int or_lanczos(int n, const int* rp, const int* ci, const double* v, void* pc, int k,
+               unsigned seed, double* alpha, double* beta) {
+    Csr a = make_csr(n, rp, ci, v);
+    const Precond* M = static_cast<const Precond*>(pc);
+    k = std::min(k, n);
+    if (k < 2) return 0;
+    std::vector<double> vc = lcg_unit(n, seed, true), vp(n, 0.0), w(n), z(n);
+    double bprev = 0.0;
+    int m = 0, nb = 0;
+    for (int j = 0; j < k; ++j) {
+        spmv(a, vc.data(), w.data());
+        if (M) M->apply(w.data(), z.data());
+        else z = w;
+        const double al = dot(vc.data(), z.data(), n);
+        alpha[m++] = al;
+        for (int i = 0; i < n; ++i) z[i] -= al * vc[i] + bprev * vp[i];
+        const double bn = norm2(z.data(), n);
+        if (bn < 1e-10 * (1.0 + std::abs(al))) break;  // invariant subspace
+        if (j < k - 1) {
+            beta[nb++] = bn;
+            vp = vc;
+            bprev = bn;
+            const double inv = 1.0 / bn;
+            for (int i = 0; i < n; ++i) vc[i] = z[i] * inv;
+        }
+    }
+    return m;
+}
+
+// Eigenvalues of the symmetric tridiagonal (d[n], e[n-1]) by implicit QL with
+// Wilkinson shifts, sorted ascending (condition_estimator.h:43-88, the
+// textbook tqli iteration with its at-most-100n sweeps per eigenvalue).
+void or_tridiag_eigs(int n, const double* d_in, const double* e_in, double* out) {
+    std::vector<double> d(d_in, d_in + n), e(n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+    const int cap = 100 * n;
+    for (int l = 0; l < n; ++l) {
+        int m, sweeps = 0;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
+                if (std::abs(e[m]) <= 1e-14 * dd) break;
+            }
+            if (m == l) break;
+            if (++sweeps > cap) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double sn = 1.0, cs = 1.0, p = 0.0;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = sn * e[i], b = cs * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    break;
+                }
+                sn = f / r;
+                cs = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2.0 * cs * b;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = cs * r - b;
+            }
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        } while (m != l);
+    }
+    std::sort(d.begin(), d.end());
+    for (int i = 0; i < n; ++i) out[i] = d[i];
+}
+
+// estimate_condition (condition_estimator.h:94-191): out = {lambda_min,
+// lambda_max, kappa}; returns the Lanczos steps, and *valid.
+int or_condition(int n, const int* rp, const int* ci, const double* v, void* pc, int k,
+                 unsigned seed, double* out, int* valid) {
+    out[0] = out[1] = out[2] = 0.0;
+    *valid = 0;
+    const int kk = std::max(2, std::min(k, n));
+    std::vector<double> al(kk), be(kk);
+    const int m = or_lanczos(n, rp, ci, v, pc, k, seed, al.data(), be.data());
+    if (m < 2) return 0;
+    for (int i = 0; i < m; ++i) if (!std::isfinite(al[i])) return 0;
+    for (int i = 0; i + 1 < m; ++i) if (!std::isfinite(be[i])) return 0;
+    std::vector<double> ev(m), fin;
+    or_tridiag_eigs(m, al.data(), be.data(), ev.data());
+    for (double x : ev) if (std::isfinite(x)) fin.push_back(x);
+    if (fin.empty()) return 0;
+    const double lmin = fin.front(), lmax = fin.back();
+    out[0] = lmin;
+    out[1] = lmax;
+    out[2] = std::abs(lmin) > 1e-14 ? lmax / lmin : 1e30;
+    *valid = lmin > 0.0;
+    return m;
+}
+
+// estimate_spectral_radius (src/analysis/spectral_analysis.h:33-95): power
+// iteration on E = I - M^{-1}A; reports |mu| of the step before the stopping
+// one, as the reference does.  Returns the iterations; *rho the radius.
+int or_spectral(int n, const int* rp, const int* ci, const double* v, void* pc, int max_iter,
+                double tol, unsigned seed, double* rho) {
+    Csr a = make_csr(n, rp, ci, v);
+    const Precond* M = static_cast<const Precond*>(pc);
+    std::vector<double> x = lcg_unit(n, seed, false), w(n), z(n);
+    double mu_prev = 0.0;
+    int iters = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        spmv(a, x.data(), w.data());
+        if (M) M->apply(w.data(), z.data());
+        else z = w;
+        for (int i = 0; i < n; ++i) w[i] = x[i] - z[i];
+        const double mu = dot(x.data(), w.data(), n);
+        const double en = norm2(w.data(), n);
+        iters = it + 1;
+        if (en < 1e-14) break;
+        for (int i = 0; i < n; ++i) x[i] = w[i] / en;
+        if (std::abs(mu - mu_prev) < tol * (1.0 + std::abs(mu))) break;
+        mu_prev = mu;
+    }
+    *rho = std::abs(mu_prev);
+    return iters;
 }
 
 }  // extern "C"

@@ -30,6 +30,15 @@
  *                              (oracle/_ref), known answers 141 (Poisson 100^2)
  *                              and 248 (150^2), identity and SPD exactness.
  *   or_metrics_json            golden strings from reference Metrics::to_json.
+ *   or_bicgstab                golden iteration counts, residual histories and
+ *                              solutions from the reference bicgstab.h
+ *                              (oracle/_ref), bit for bit; identity -> 1 step.
+ *   or_lanczos / or_tridiag_eigs / or_condition / or_spectral
+ *                              golden Lanczos coefficients, Ritz extremes and
+ *                              power-iteration radii from the reference
+ *                              analysis headers (oracle/_ref); tridiagonal
+ *                              eigenvalues against numpy.linalg.eigvalsh;
+ *                              Poisson 1D/2D closed-form spectra.
  */
 #pragma once
 #include <stdint.h>
@@ -77,6 +86,20 @@ void or_lu_solve(int n, const double* LU, const int* piv, const double* b, doubl
 int or_solve(int method, int n, const int* rp, const int* ci, const double* v,
              const double* b, double* x, void* pc, int restart, int max_iter, double tol,
              double* residual_norm, int* converged, double* hist, int hist_cap, int* hist_len);
+
+/* ---- BiCGSTAB (src/core/bicgstab.h:57-204): tol relative to ||r0||.  Returns iterations. */
+int or_bicgstab(int n, const int* rp, const int* ci, const double* v, const double* b, double* x,
+                void* pc, int max_iter, double tol, double* residual_norm, int* converged,
+                double* hist, int hist_cap, int* hist_len);
+
+/* ---- analysis (src/analysis/condition_estimator.h, spectral_analysis.h) --- */
+int or_lanczos(int n, const int* rp, const int* ci, const double* v, void* pc, int k,
+               unsigned seed, double* alpha, double* beta);
+void or_tridiag_eigs(int n, const double* d, const double* e, double* out);
+int or_condition(int n, const int* rp, const int* ci, const double* v, void* pc, int k,
+                 unsigned seed, double* out, int* valid);
+int or_spectral(int n, const int* rp, const int* ci, const double* v, void* pc, int max_iter,
+                double tol, unsigned seed, double* rho);
 
 /* ---- JSONL record (src/io/metrics_logger.h:28-51) ------------------------ */
 int or_metrics_json(const char* matrix, const char* method, int n, int nnz, int ranks,

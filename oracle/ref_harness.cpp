@@ -2,8 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the
 // unmodified reference headers and translation units (core/gmres.cpp,
-// core/fgmres.cpp, precond/ilu0.h, precond/jacobi.h, precond/coarse_solve.h,
-// mesh/poisson.h, io/metrics_logger.h, core/sparse_matrix.h).  The binary goes
+// core/fgmres.cpp, core/bicgstab.h, precond/ilu0.h, precond/jacobi.h,
+// precond/coarse_solve.h, mesh/poisson.h, io/metrics_logger.h,
+// io/matrix_market.h, analysis/*.h, core/sparse_matrix.h).  The binary goes
 // to oracle/_ref/ and is driven by tests/golden/make_golden.py, which writes the
 // committed fixtures.  Nothing under the reference's decomp/ (needs METIS) is
 // included: RAS/ASM goldens are produced from the reference's own
@@ -13,12 +14,18 @@
 #include "core/vector.h"
 #include "core/gmres.h"
 #include "core/fgmres.h"
+#include "core/bicgstab.h"
 #include "precond/ilu0.h"
 #include "precond/jacobi.h"
 #include "precond/coarse_solve.h"
 #include "mesh/poisson.h"
 #include "io/metrics_logger.h"
+#include "io/matrix_market.h"
+#include "analysis/condition_estimator.h"
+#include "analysis/spectral_analysis.h"
+#include "analysis/convergence_analysis.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -57,8 +64,31 @@ void print_ivec(const char* key, const std::vector<int>& v, bool last = false) {
 }
 
 CSRMatrix grid(const std::string& spec) {
-    // "2d:NX:NY" or "3d:NX:NY:NZ"
+    // "2d:NX:NY", "3d:NX:NY:NZ" or "rnd:N:SEED" (N x N, diagonal plus about
+    // three random off-diagonals per row, values uniform in [-1, 1) and the
+    // diagonal 4 + |u|; columns sorted)
     int a = 0, b = 0, c = 0;
+    if (spec.rfind("rnd:", 0) == 0) {
+        unsigned long long seed = 0;
+        std::sscanf(spec.c_str() + 4, "%d:%llu", &a, &seed);
+        SplitMix64 rng(seed);
+        CSRMatrix A;
+        A.nrows = A.ncols = a;
+        A.row_ptr.assign(a + 1, 0);
+        for (int i = 0; i < a; ++i) {
+            std::vector<int> cols{i};
+            for (int t = 0; t < 3; ++t) cols.push_back((int)(rng.next() % (uint64_t)a));
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            for (int j : cols) {
+                A.col_idx.push_back(j);
+                const double u = 2.0 * rng.uniform01() - 1.0;
+                A.vals.push_back(j == i ? 4.0 + std::abs(u) : u);
+            }
+            A.row_ptr[i + 1] = (int)A.col_idx.size();
+        }
+        return A;
+    }
     if (spec.rfind("2d:", 0) == 0) {
         std::sscanf(spec.c_str() + 3, "%d:%d", &a, &b);
         return make_poisson_2d(a, b);
@@ -112,8 +142,9 @@ int cmd_solve(int argc, char** argv) {
         else mb = b;
         prm.tol = -prm.tol * norm2(mb);
     }
-    GMRESResult r = method == "fgmres" ? fgmres(A, b, x, M.get(), prm)
-                                       : gmres(A, b, x, M.get(), prm);
+    GMRESResult r = method == "fgmres"     ? fgmres(A, b, x, M.get(), prm)
+                    : method == "bicgstab" ? bicgstab(A, b, x, M.get(), prm)
+                                           : gmres(A, b, x, M.get(), prm);
     std::vector<double> ax;
     A.spmv(x, ax);
     double rn = 0, bn = 0, en = 0, xn = 0;
@@ -198,6 +229,63 @@ int cmd_lu(int n, uint64_t seed, int singular) {
     return 0;
 }
 
+// analyze PC GRID K POWER_ITERS: estimate_condition on A and on M^{-1}A and
+// estimate_spectral_radius, as src/main.cpp:211-220 calls them.
+int cmd_analyze(int argc, char** argv) {
+    if (argc < 6) return 2;
+    std::string pc = argv[2];
+    CSRMatrix A = grid(argv[3]);
+    const int k = std::atoi(argv[4]), pit = std::atoi(argv[5]);
+    std::unique_ptr<Preconditioner> M;
+    if (pc == "jacobi") M.reset(new JacobiPrecond());
+    else if (pc == "ilu0") M.reset(new ILU0Precond());
+    if (M) M->setup(A);
+    auto pe = [](const char* key, const ConditionEstimate& e) {
+        std::printf("\"%s\":{\"lambda_min\":%.17g,\"lambda_max\":%.17g,\"kappa\":%.17g,"
+                    "\"steps\":%d,\"valid\":%d},",
+                    key, e.lambda_min, e.lambda_max, e.kappa, e.lanczos_steps, e.valid ? 1 : 0);
+    };
+    std::printf("{");
+    pe("cond_A", estimate_condition(A, nullptr, k));
+    pe("cond_MA", estimate_condition(A, M.get(), k));
+    SpectralResult sr = estimate_spectral_radius(A, M.get(), pit);
+    std::printf("\"rho\":%.17g,\"power_iters\":%d,\"ok\":%d}\n", sr.spectral_radius,
+                sr.power_iters, sr.convergence_ok ? 1 : 0);
+    return 0;
+}
+
+// conv OUT CONVERGED: residual history on stdin (count, then values);
+// analyze_convergence + write_jsonl(OUT, "m.mtx", "ras", 8, 2), then
+// print_summary on stdout.
+int cmd_conv(int argc, char** argv) {
+    if (argc < 4) return 2;
+    int cnt;
+    if (std::scanf("%d", &cnt) != 1) return 2;
+    std::vector<double> h(cnt);
+    for (auto& t : h) if (std::scanf("%lf", &t) != 1) return 2;
+    ConvergenceMetrics m = analyze_convergence(h, std::atoi(argv[3]) != 0);
+    m.write_jsonl(argv[2], "m.mtx", "ras", 8, 2);
+    m.print_summary();
+    return 0;
+}
+
+// mm FILE: read_matrix_market, printed as JSON (or the exception message).
+int cmd_mm(const char* path) {
+    CSRMatrix A;
+    try {
+        A = read_matrix_market(path);
+    } catch (const std::exception& e) {
+        std::printf("{\"error\":\"%s\"}\n", e.what());
+        return 0;
+    }
+    std::printf("{\"nrows\":%d,\"ncols\":%d,", A.nrows, A.ncols);
+    print_ivec("row_ptr", A.row_ptr);
+    print_ivec("col_idx", A.col_idx);
+    print_vec("vals", A.vals, true);
+    std::printf("}\n");
+    return 0;
+}
+
 int cmd_metrics() {
     Metrics m;
     m.matrix = "poisson2d_256";
@@ -240,5 +328,18 @@ int main(int argc, char** argv) {
     if (c == "blocks") return cmd_blocks();
     if (c == "lu" && argc >= 5) return cmd_lu(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]));
     if (c == "metrics") return cmd_metrics();
+    if (c == "analyze") return cmd_analyze(argc, argv);
+    if (c == "conv") return cmd_conv(argc, argv);
+    if (c == "mm" && argc >= 3) return cmd_mm(argv[2]);
+    if (c == "mmw" && argc >= 4) {  // mmw GRID OUT: write_matrix_market; the matrix on stdout
+        CSRMatrix A = grid(argv[2]);
+        write_matrix_market(argv[3], A);
+        std::printf("{\"nrows\":%d,\"ncols\":%d,", A.nrows, A.ncols);
+        print_ivec("row_ptr", A.row_ptr);
+        print_ivec("col_idx", A.col_idx);
+        print_vec("vals", A.vals, true);
+        std::printf("}\n");
+        return 0;
+    }
     return 2;
 }

@@ -47,7 +47,23 @@ class SolveResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int),
                 ("residual_norm", C.c_double), ("rel_true_residual", C.c_double),
                 ("time_solve_s", C.c_double), ("cycles", C.c_int),
-                ("precond_applies", C.c_int)]
+                ("precond_applies", C.c_int), ("hist_len", C.c_int)]
+
+
+class ConditionEstimate(C.Structure):
+    _fields_ = [("lambda_min", C.c_double), ("lambda_max", C.c_double), ("kappa", C.c_double),
+                ("lanczos_steps", C.c_int), ("valid", C.c_int)]
+
+
+class SpectralResult(C.Structure):
+    _fields_ = [("spectral_radius", C.c_double), ("convergence_ok", C.c_int),
+                ("power_iters", C.c_int)]
+
+
+class Convergence(C.Structure):
+    _fields_ = [("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("asymptotic_rate", C.c_double), ("stagnation_start", C.c_int),
+                ("converged", C.c_int), ("iterations", C.c_int)]
 
 
 class LearnedOpts(C.Structure):
@@ -130,6 +146,20 @@ def lib():
         "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double)], C.c_int),
         "sg_context_reset_stats": ([_VP], C.c_int),
+        "sg_mm_read": ([C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                        C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int)),
+                        C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_double))],
+                       C.c_int),
+        "sg_mm_free": ([_VP], None),
+        "sg_estimate_condition": ([_VP, _VP, C.c_int, C.c_uint, C.POINTER(ConditionEstimate),
+                                   _VP, _VP], C.c_int),
+        "sg_estimate_spectral_radius": ([_VP, _VP, C.c_int, C.c_double, C.c_uint,
+                                         C.POINTER(SpectralResult), _VP], C.c_int),
+        "sg_convergence_analyze": ([_VP, C.c_int, C.c_int, C.POINTER(Convergence), _VP],
+                                   C.c_int),
+        "sg_convergence_write_jsonl": ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
+                                        _VP, C.c_int, C.c_int], C.c_int),
+        "sg_mm_write": ([C.c_char_p, C.c_int, C.c_int, _VP, _VP, _VP, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -510,19 +540,111 @@ SOLVE_GRAPH = 1
 SOLVE_NO_GRAPH = 2
 
 
+MM_SUM_DUPLICATES = 1
+MM_STRICT = 2
+
+
+def read_matrix_market(path, sum_duplicates=False, strict=False):
+    """sg_mm_read: (nrows, ncols, row_ptr, col_idx, vals) as numpy arrays.
+    Host only; no GPU needed."""
+    L = lib()
+    nr, nc, nz = C.c_int(), C.c_int(), C.c_int()
+    rp, ci = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+    v = C.POINTER(C.c_double)()
+    flags = (MM_SUM_DUPLICATES if sum_duplicates else 0) | (MM_STRICT if strict else 0)
+    st = L.sg_mm_read(os.fsencode(path), flags, C.byref(nr), C.byref(nc), C.byref(nz),
+                      C.byref(rp), C.byref(ci), C.byref(v))
+    if st:
+        raise SgError(st, L.sg_last_error(None).decode())
+    try:
+        n, k = nr.value, nz.value
+        out = (n, nc.value, np.ctypeslib.as_array(rp, (n + 1,)).copy(),
+               np.ctypeslib.as_array(ci, (max(k, 1),))[:k].copy(),
+               np.ctypeslib.as_array(v, (max(k, 1),))[:k].copy())
+    finally:
+        for a in (rp, ci, v):
+            L.sg_mm_free(C.cast(a, _VP))
+    return out
+
+
+def write_matrix_market(path, rp, ci, v, ncols=None, digits=6):
+    """sg_mm_write (digits=6 writes as the reference; 17 round-trips)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    n = len(rp) - 1
+    st = lib().sg_mm_write(os.fsencode(path), n, n if ncols is None else ncols, _p(rp), _p(ci),
+                           _p(v), digits)
+    if st:
+        raise SgError(st, lib().sg_last_error(None).decode())
+
+
+def estimate_condition(A, M=None, k=80, seed=42):
+    """sg_estimate_condition: Lanczos estimate of kappa(M^-1 A) on the device.
+    Returns a dict with lambda_min, lambda_max, kappa, lanczos_steps, valid and
+    the tridiagonal coefficients alpha, beta."""
+    e = ConditionEstimate()
+    kk = max(k, 2)
+    al, be = np.zeros(kk), np.zeros(kk)
+    A.ctx.check(lib().sg_estimate_condition(A.h, M.h if M is not None else None, k, seed,
+                                            C.byref(e), _p(al), _p(be)))
+    m = e.lanczos_steps
+    return dict(lambda_min=e.lambda_min, lambda_max=e.lambda_max, kappa=e.kappa,
+                lanczos_steps=e.lanczos_steps, valid=bool(e.valid), alpha=al[:m].copy(),
+                beta=be[:max(m - 1, 0)].copy())
+
+
+def estimate_spectral_radius(A, M=None, max_iter=200, tol=1e-6, seed=123, eigvec=None):
+    """sg_estimate_spectral_radius: power iteration on I - M^-1 A on the device."""
+    r = SpectralResult()
+    A.ctx.check(lib().sg_estimate_spectral_radius(
+        A.h, M.h if M is not None else None, max_iter, tol, seed, C.byref(r),
+        eigvec.ptr if eigvec is not None else None))
+    return dict(spectral_radius=r.spectral_radius, convergence_ok=bool(r.convergence_ok),
+                power_iters=r.power_iters)
+
+
+def convergence_metrics(hist, converged):
+    """sg_convergence_analyze (host only)."""
+    h = np.ascontiguousarray(hist, np.float64)
+    c = Convergence()
+    rates = np.zeros(max(len(h), 1))
+    st = lib().sg_convergence_analyze(_p(h), len(h), 1 if converged else 0, C.byref(c),
+                                      _p(rates))
+    if st:
+        raise SgError(st, lib().sg_last_error(None).decode())
+    return dict(initial_residual=c.initial_residual, final_residual=c.final_residual,
+                asymptotic_rate=c.asymptotic_rate, stagnation_start=c.stagnation_start,
+                converged=bool(c.converged), iterations=c.iterations, rates=rates[:len(h)])
+
+
+def convergence_write_jsonl(path, matrix, precond, nparts, overlap, hist, converged):
+    """sg_convergence_write_jsonl (host only): appends the "convergence" line."""
+    h = np.ascontiguousarray(hist, np.float64)
+    st = lib().sg_convergence_write_jsonl(os.fsencode(path), matrix.encode(), precond.encode(),
+                                          nparts, overlap, _p(h), len(h),
+                                          1 if converged else 0)
+    if st:
+        raise SgError(st, lib().sg_last_error(None).decode())
+
+
+METHODS = {"gmres": 0, "fgmres": 1, "bicgstab": 2}
+
+
 def _result(r, hist, cap):
     return dict(iterations=r.iterations, converged=bool(r.converged),
                 residual_norm=r.residual_norm, rel_true_residual=r.rel_true_residual,
                 time_solve_s=r.time_solve_s, cycles=r.cycles,
                 precond_applies=r.precond_applies,
-                hist=hist[:min(r.iterations, cap)].copy())
+                hist=hist[:min(r.hist_len, cap)].copy())
 
 
 def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10, rtol=0.0,
           ortho="mgs", lookahead=0, hist_cap=None, graph=False):
     """Run sg_solve.  b, x are DeviceArrays (x in/out).  Returns a dict shaped
-    like GMRESResult plus the FP64 true relative residual."""
-    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
+    like GMRESResult plus the FP64 true relative residual.  method: "gmres"
+    (left), "fgmres" or "bicgstab" (tol relative to ||r0|| there)."""
+    p = SolveParams(METHODS[method], restart, max_iter, tol, rtol,
                     1 if ortho == "cgs2" else 0, lookahead, SOLVE_GRAPH if graph else 0)
     r = SolveResult()
     cap = hist_cap if hist_cap is not None else max_iter + restart + 2
@@ -535,7 +657,7 @@ def solve(A, b, x, M=None, method="gmres", restart=30, max_iter=1000, tol=1e-10,
 def solve_batch(A, b, x, M=None, method="fgmres", restart=30, max_iter=1000, tol=1e-10,
                 rtol=0.0, ortho="cgs2", graph=True, hist_cap=None):
     """Run sg_solve_batch on a BatchCsr: one result dict per system."""
-    p = SolveParams(1 if method == "fgmres" else 0, restart, max_iter, tol, rtol,
+    p = SolveParams(METHODS[method], restart, max_iter, tol, rtol,
                     1 if ortho == "cgs2" else 0, 0, 0 if graph else SOLVE_NO_GRAPH)
     nsys = A.nsys
     rs = (SolveResult * nsys)()

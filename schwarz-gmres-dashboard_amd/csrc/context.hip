@@ -24,6 +24,11 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
 
 void fail(sg_status st, const std::string& msg) { throw Error{st, msg}; }
 
+void set_global_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_err = msg;
+}
+
 // Runs f, converting exceptions into a status and the context's message.
 template <typename F>
 sg_status guarded(sg_context* ctx, F&& f) {

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace sg {
@@ -48,5 +49,13 @@ std::vector<int> diag_positions(const HostCsr& A);
 std::vector<int> lower_levels(const HostCsr& A);
 // Level of each row for the backward (U) sweep (dependencies on columns > row).
 std::vector<int> upper_levels(const HostCsr& A);
+
+// Matrix Market coordinate files (matrix_market.cpp); flags are SG_MM_*.
+void mm_read(const std::string& path, int flags, HostCsr& A);
+void mm_write(const std::string& path, int nrows, int ncols, const int* rp, const int* ci,
+              const double* v, int digits);
+
+// Message returned by sg_last_error(NULL) (context.hip).
+void set_global_error(const std::string& msg);
 
 }  // namespace sg

@@ -25,7 +25,7 @@
 // launches either directly (with `lookahead`: the host reads the flag of one
 // system asynchronously and stops launching) or as one hipGraph captured
 // before the first cycle and replayed (default for batches).
-#include "sg_internal.h"
+#include "segs.h"
 
 #include <algorithm>
 #include <chrono>
@@ -39,7 +39,6 @@ int precond_n(const sg_precond* M);
 bool precond_ready(const sg_precond* M);
 
 constexpr int kT = 256;                 // threads per reduction block
-constexpr int kMaxBlocks = 4096;        // blocks per system and split
 constexpr int kMaxCols = 128;           // restart + 1
 constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis vector)
 constexpr int kCBD = 8;                 // basis vectors per register block, Gram dots
@@ -104,60 +103,6 @@ __device__ inline Work sys_work(const Work& W, int s) {
     o.st += s;
     o.mode += s;
     return o;
-}
-
-// One block of a split: elements [start, end) of system sys.
-struct BlockRange {
-    int sys, start, end, pad;
-};
-struct SysBlocks {
-    int blk0, nblk;
-};
-struct Segs {
-    const BlockRange* blk;
-    const SysBlocks* sys;
-};
-
-// Block count and chunk of one system of n elements (the chunk is a multiple
-// of the tile so every block but the last runs whole tiles).
-static void split_shape(int n, int max_blocks, int tile, int& nblk, int& chunk) {
-    int want = std::max(1, (n + tile - 1) / tile);
-    nblk = std::min(want, std::min(max_blocks, kMaxBlocks));
-    int c = (n + nblk - 1) / nblk;
-    chunk = (c + tile - 1) / tile * tile;
-    nblk = std::max(1, (n + chunk - 1) / chunk);
-}
-
-struct SegsHost {
-    DevBuf<BlockRange> blk;
-    DevBuf<SysBlocks> sys;
-    int nblk = 0;
-    Segs dev() const { return Segs{blk.p, sys.p}; }
-};
-
-static void build_segs(const std::vector<int>& off, int max_blocks, int tile, SegsHost& out,
-                       hipStream_t s) {
-    std::vector<BlockRange> br;
-    std::vector<SysBlocks> sb;
-    for (size_t q = 0; q + 1 < off.size(); ++q) {
-        const int n = off[q + 1] - off[q];
-        int nblk, chunk;
-        split_shape(n, max_blocks, tile, nblk, chunk);
-        sb.push_back(SysBlocks{(int)br.size(), nblk});
-        for (int b = 0; b < nblk; ++b)
-            br.push_back(BlockRange{(int)q, off[q] + b * chunk,
-                                    off[q] + std::min(n, (b + 1) * chunk), 0});
-    }
-    out.nblk = (int)br.size();
-    out.blk.upload(br, s);
-    out.sys.upload(sb, s);
-    SG_HIP(hipStreamSynchronize(s));
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
 }
 
 // Givens rotation, tangent form (src/core/gmres.cpp:11-23, fgmres.cpp:12-20).
@@ -631,27 +576,6 @@ struct SolveWs {
     }
 };
 
-// Ends a stream capture that an exception interrupted, so the stream stays usable.
-struct CaptureGuard {
-    hipStream_t s;
-    bool on = false;
-    ~CaptureGuard() {
-        if (on) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s, &g);
-            if (g) (void)hipGraphDestroy(g);
-        }
-    }
-};
-
-// Profiling off for the scope (event records cannot join a captured graph).
-struct ProfOff {
-    sg_context* c;
-    bool was;
-    explicit ProfOff(sg_context* ctx) : c(ctx), was(ctx->profiling) { c->profiling = false; }
-    ~ProfOff() { c->profiling = was; }
-};
-
 static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                        const sg_solve_params* prm, sg_solve_result* res, double* hist,
                        int hist_cap, bool use_graph) {
@@ -903,6 +827,7 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         R.converged = h.converged;
         R.cycles = cycles[q];
         R.precond_applies = h.applies;
+        R.hist_len = h.total;
         R.time_solve_s = elapsed;
         bn[q] = h.bnorm;
         std::vector<double> hv(h.total);
@@ -945,6 +870,9 @@ extern "C" sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, doubl
     return run_guarded(A->ctx, [&] {
         if (A->sys_off.size() != 2)
             fail(SG_E_ARG, "matrix holds a batch of systems: use sg_solve_batch");
+        if (prm->method == SG_BICGSTAB)
+            return bicgstab_impl(A, M, d_b, d_x, prm, res, hist, hist_cap,
+                                 (prm->flags & SG_SOLVE_GRAPH) != 0);
         solve_impl(A, M, d_b, d_x, prm, res, hist, hist_cap, (prm->flags & SG_SOLVE_GRAPH) != 0);
     });
 }
@@ -954,6 +882,9 @@ extern "C" sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b,
                                     double* hist, int hist_cap) {
     if (!A || !prm || !res || (A->nrows && (!d_b || !d_x))) return SG_E_ARG;
     return run_guarded(A->ctx, [&] {
+        if (prm->method == SG_BICGSTAB)
+            return bicgstab_impl(A, M, d_b, d_x, prm, res, hist, hist_cap,
+                                 (prm->flags & SG_SOLVE_NO_GRAPH) == 0);
         solve_impl(A, M, d_b, d_x, prm, res, hist, hist_cap,
                    (prm->flags & SG_SOLVE_NO_GRAPH) == 0);
     });

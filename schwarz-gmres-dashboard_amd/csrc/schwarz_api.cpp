@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
+#include <iostream>
 #include <mutex>
 #include <numeric>
 #include <sstream>
@@ -80,7 +81,7 @@ GMRESResult run_solve(const CSRMatrix& A, const std::vector<double>& b, std::vec
     out.residual_norm = r.residual_norm;
     out.time_solve_s = r.time_solve_s;
     out.rel_true_residual = r.rel_true_residual;
-    hist.resize(std::min<size_t>(hist.size(), (size_t)r.iterations));
+    hist.resize(std::min<size_t>(hist.size(), (size_t)r.hist_len));
     out.residual_history = std::move(hist);
     return out;
 }
@@ -369,6 +370,15 @@ std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
                                      std::vector<std::vector<double>>& x, sg_precond_kind kind,
                                      int nparts, int overlap, sg_precision precision,
                                      const GMRESParams& p, bool flexible) {
+    return solve_batch(systems, b, x, kind, nparts, overlap, precision, p,
+                       flexible ? SG_FGMRES : SG_GMRES_LEFT);
+}
+
+std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
+                                     const std::vector<std::vector<double>>& b,
+                                     std::vector<std::vector<double>>& x, sg_precond_kind kind,
+                                     int nparts, int overlap, sg_precision precision,
+                                     const GMRESParams& p, sg_method method) {
     sg_context* ctx = default_context();
     const int ns = (int)systems.size();
     if (ns < 1 || (int)b.size() != ns) throw std::runtime_error("solve_batch: one b per system");
@@ -422,7 +432,7 @@ std::vector<GMRESResult> solve_batch(const std::vector<CSRMatrix>& systems,
     check(sg_memcpy_h2d(ctx, db.p, hb.data(), sizeof(double) * n), ctx);
     check(sg_memcpy_h2d(ctx, dx.p, hx.data(), sizeof(double) * n), ctx);
     sg_solve_params sp{};
-    sp.method = flexible ? SG_FGMRES : SG_GMRES_LEFT;
+    sp.method = method;
     sp.restart = p.restart;
     sp.max_iter = p.max_iter;
     sp.tol_abs = p.tol;
@@ -473,6 +483,11 @@ GMRESResult fgmres(const CSRMatrix& A, const std::vector<double>& b, std::vector
     return solve_with(A, b, x, M, SG_FGMRES, params);
 }
 
+GMRESResult bicgstab(const CSRMatrix& A, const std::vector<double>& b, std::vector<double>& x,
+                     const Preconditioner* M, const GMRESParams& params) {
+    return solve_with(A, b, x, M, SG_BICGSTAB, params);
+}
+
 GMRESResult fgmres_flex(const CSRMatrix& A, const std::vector<double>& b,
                         std::vector<double>& x, FlexApply prec_apply,
                         const GMRESParams& params) {
@@ -485,6 +500,163 @@ GMRESResult fgmres_flex(const CSRMatrix& A, const std::vector<double>& b,
         ~Drop() { sg_precond_destroy(p); }
     } drop{cb};
     return run_solve(A, b, x, cb, SG_FGMRES, params);
+}
+
+// ---- Matrix Market ----------------------------------------------------------------
+
+CSRMatrix read_matrix_market(const std::string& filename) {
+    return read_matrix_market(filename, MatrixMarketOptions{});
+}
+
+CSRMatrix read_matrix_market(const std::string& filename, const MatrixMarketOptions& o) {
+    int nr = 0, nc = 0, nz = 0;
+    int *rp = nullptr, *ci = nullptr;
+    double* v = nullptr;
+    const int flags = (o.sum_duplicates ? SG_MM_SUM_DUPLICATES : 0) | (o.strict ? SG_MM_STRICT : 0);
+    if (sg_mm_read(filename.c_str(), flags, &nr, &nc, &nz, &rp, &ci, &v) != SG_OK)
+        throw std::runtime_error(sg_last_error(nullptr));
+    CSRMatrix A;
+    A.nrows = nr;
+    A.ncols = nc;
+    A.row_ptr.assign(rp, rp + nr + 1);
+    A.col_idx.assign(ci, ci + nz);
+    A.vals.assign(v, v + nz);
+    sg_mm_free(rp);
+    sg_mm_free(ci);
+    sg_mm_free(v);
+    return A;
+}
+
+void write_matrix_market(const std::string& filename, const CSRMatrix& A, int digits) {
+    std::vector<int> rp = A.row_ptr;
+    if (rp.empty()) rp.assign(1, 0);
+    if (sg_mm_write(filename.c_str(), A.nrows, A.ncols, rp.data(), A.col_idx.data(),
+                    A.vals.data(), digits) != SG_OK)
+        throw std::runtime_error(sg_last_error(nullptr));
+}
+
+// ---- convergence analytics -------------------------------------------------------
+
+void ConvergenceMetrics::compute(const std::vector<double>& h, bool conv) {
+    residuals = h;
+    sg_convergence c{};
+    rates.assign(h.size(), 0.0);
+    check(sg_convergence_analyze(h.data(), (int)h.size(), conv ? 1 : 0, &c, rates.data()),
+          nullptr);
+    if (h.empty()) rates.clear();
+    initial_residual = c.initial_residual;
+    final_residual = c.final_residual;
+    asymptotic_rate = c.asymptotic_rate;
+    stagnation_start = c.stagnation_start;
+    converged = c.converged != 0;
+    iterations = c.iterations;
+}
+
+void ConvergenceMetrics::write_jsonl(const std::string& path, const std::string& matrix,
+                                     const std::string& precond, int nparts,
+                                     int overlap) const {
+    check(sg_convergence_write_jsonl(path.c_str(), matrix.c_str(), precond.c_str(), nparts,
+                                     overlap, residuals.data(), (int)residuals.size(),
+                                     converged ? 1 : 0),
+          nullptr);
+}
+
+void ConvergenceMetrics::print_summary() const {
+    std::cout << "  Convergence analysis:\n"
+              << "    Initial residual : " << initial_residual << "\n"
+              << "    Final residual   : " << final_residual << "\n"
+              << "    Asymptotic rate  : " << asymptotic_rate << "\n";
+    if (stagnation_start >= 0)
+        std::cout << "    *** Stagnation detected at iter " << stagnation_start << " ***\n";
+    else
+        std::cout << "    No stagnation detected\n";
+}
+
+ConvergenceMetrics analyze_convergence(const std::vector<double>& h, bool converged) {
+    ConvergenceMetrics m;
+    m.compute(h, converged);
+    return m;
+}
+
+namespace {
+
+// Device handle of M for the analysis calls: the library's own preconditioner,
+// or a host-callback wrapper around a user-defined one (destroyed by the guard).
+struct PrecondHandle {
+    sg_precond* h = nullptr;
+    sg_precond* owned = nullptr;
+    PrecondHandle(const Preconditioner* M, int n) {
+        if (!M) return;
+        h = M->device_handle();
+        if (h) return;
+        sg_context* ctx = default_context();
+        check(sg_precond_create_callback(ctx, n, host_callback, const_cast<Preconditioner*>(M),
+                                         0, &owned),
+              ctx);
+        h = owned;
+    }
+    ~PrecondHandle() {
+        if (owned) sg_precond_destroy(owned);
+    }
+};
+
+}  // namespace
+
+ConditionEstimate estimate_condition(const CSRMatrix& A, const Preconditioner* M, int k,
+                                     unsigned int seed) {
+    sg_context* ctx = default_context();
+    DevCsr dA(A);
+    PrecondHandle ph(M, A.nrows);
+    sg_condition_estimate e{};
+    check(sg_estimate_condition(dA.h, ph.h, k, seed, &e, nullptr, nullptr), ctx);
+    ConditionEstimate out;
+    out.lambda_min = e.lambda_min;
+    out.lambda_max = e.lambda_max;
+    out.kappa = e.kappa;
+    out.lanczos_steps = e.lanczos_steps;
+    out.valid = e.valid != 0;
+    return out;
+}
+
+void print_condition_estimate(const ConditionEstimate& est, const std::string& label) {
+    std::cout << "  Condition estimate" << (label.empty() ? "" : " (" + label + ")")
+              << " [Lanczos " << est.lanczos_steps << " steps]:\n";
+    if (!est.valid || est.lambda_min <= 0.0) {
+        std::cout << "    Ritz(min) = " << est.lambda_min << "\n"
+                  << "    Ritz(max) = " << est.lambda_max << "\n"
+                  << "    kappa_est = " << est.kappa << "\n"
+                  << "    (Note: M^{-1}A is non-symmetric; bounds are approximate)\n";
+    } else {
+        std::cout << "    lambda_min = " << est.lambda_min << "\n"
+                  << "    lambda_max = " << est.lambda_max << "\n"
+                  << "    kappa      = " << est.kappa << "\n";
+    }
+}
+
+SpectralResult estimate_spectral_radius(const CSRMatrix& A, const Preconditioner* M,
+                                        int max_iter, double tol, unsigned int seed) {
+    sg_context* ctx = default_context();
+    DevCsr dA(A);
+    PrecondHandle ph(M, A.nrows);
+    const int n = A.nrows;
+    Dev dv(sizeof(double) * std::max(n, 1));
+    sg_spectral_result r{};
+    check(sg_estimate_spectral_radius(dA.h, ph.h, max_iter, tol, seed, &r, dv.d()), ctx);
+    SpectralResult out;
+    out.spectral_radius = r.spectral_radius;
+    out.convergence_ok = r.convergence_ok != 0;
+    out.power_iters = r.power_iters;
+    out.eigenvector.resize(n);
+    check(sg_memcpy_d2h(ctx, out.eigenvector.data(), dv.p, sizeof(double) * n), ctx);
+    return out;
+}
+
+void print_spectral_result(const SpectralResult& sr, const std::string& label) {
+    std::cout << "  Spectral radius" << (label.empty() ? "" : " (" + label + ")")
+              << " [power iter " << sr.power_iters << " steps]:\n"
+              << "    rho(I - M^{-1}A) = " << sr.spectral_radius << "\n"
+              << "    Convergence cond : "
+              << (sr.convergence_ok ? "SATISFIED (rho < 1)" : "VIOLATED (rho >= 1)") << "\n";
 }
 
 // ---- Metrics ---------------------------------------------------------------------

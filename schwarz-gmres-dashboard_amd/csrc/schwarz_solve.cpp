@@ -1,11 +1,14 @@
 // schwarz_solve -- command-line driver, drop-in for the reference's single-rank
-// path (src/main.cpp:63-275): same flags, stdout lines and JSONL record, every
-// solve on the GPU.  Extensions: --poisson3d N, --jump3d N (c3 workload),
+// path (src/main.cpp:63-275): same flags (--matrix / --poisson, --precond,
+// --restart, --tol, --maxiter, --fgmres, --bicgstab, --nparts, --overlap,
+// --gpu, --output, --analyze), stdout lines and JSONL records, every solve and
+// analysis on the GPU.  Extensions: --poisson3d N, --jump3d N (c3 workload),
 // --rhs ones|uniform|normal with --seed, --rtol, --precision fp64|fp32,
 // --ortho mgs|cgs2, --partition FILE, --ufactor U, --weighted-partition,
 // --coarse-inverse, --batch K (K systems with right-hand-side seeds seed..,
 // solved together), --learned WEIGHTS.npy (learned FGMRES stage on the 2D
-// grid).  Exit code 0 if converged, else 1 (src/main.cpp:274).
+// grid), --mm-sum-duplicates / --mm-strict (Matrix Market reader fixes).
+// Exit code 0 if converged, else 1 (src/main.cpp:274).
 #include "../../include/schwarz/schwarz.h"
 
 #include <chrono>
@@ -28,6 +31,9 @@ void usage() {
     std::cerr
         << "Usage: schwarz_solve [options]\n"
            "\nMatrix source (one required):\n"
+           "  --matrix <file.mtx>      Matrix Market file\n"
+           "  --mm-sum-duplicates      merge duplicate entries of the file by summing\n"
+           "  --mm-strict              header flags from the banner only; reject malformed lines\n"
            "  --poisson <N>            Poisson 2D, N x N grid (n = N^2)\n"
            "  --poisson3d <N>          Poisson 3D, N^3 grid\n"
            "  --jump3d <N>             3D finite-volume diffusion, kappa = 10^{-3,0,3} on 16^3 blocks\n"
@@ -40,6 +46,7 @@ void usage() {
            "  --rtol <val>             stop on an FP64-verified ||b-Ax||/||b|| <= val\n"
            "  --maxiter <n>            max iterations (default 1000)\n"
            "  --fgmres                 flexible GMRES (right preconditioning)\n"
+           "  --bicgstab               BiCGSTAB (right preconditioning; --tol relative to ||r0||)\n"
            "  --ortho <o>              mgs (default) | cgs2\n"
            "\nDomain decomposition:\n"
            "  --nparts <n>             subdomains (default 4)\n"
@@ -56,6 +63,8 @@ void usage() {
            "  --learned <w.npy>        learned FGMRES stage on top of --precond (2D --poisson grids)\n"
            "  --learned-every <k>      run the network on every k-th application (default 1)\n"
            "  --gpu                    accepted; every solve runs on the GPU\n"
+           "  --analyze                Lanczos condition estimates, spectral radius and\n"
+           "                           convergence metrics (<output>.conv.jsonl)\n"
            "  --output <file.jsonl>    append a metrics record\n";
 }
 
@@ -137,7 +146,10 @@ int main(int argc, char** argv) {
                 precision = "fp64";
     GMRESParams prm;
     int overlap = 1, nparts = 4, poisson = 0, poisson3d = 0, jump3d = 0, ufactor = 0;
-    bool use_fgmres = false, weighted = false, coarse_inv = false;
+    bool use_fgmres = false, use_bicgstab = false, weighted = false, coarse_inv = false;
+    bool analyze = false;
+    MatrixMarketOptions mm;
+    std::string matrix_file;
     uint64_t seed = 1;
     int batch = 0, learned_every = 1;
     std::string learned;
@@ -164,27 +176,35 @@ int main(int argc, char** argv) {
         else if (val("--batch")) batch = std::atoi(argv[++i]);
         else if (val("--learned")) learned = argv[++i];
         else if (val("--learned-every")) learned_every = std::atoi(argv[++i]);
+        else if (val("--matrix")) matrix_file = argv[++i];
         else if (is("--fgmres")) use_fgmres = true;
+        else if (is("--bicgstab")) use_bicgstab = true;
+        else if (is("--analyze")) analyze = true;
+        else if (is("--mm-sum-duplicates")) mm.sum_duplicates = true;
+        else if (is("--mm-strict")) mm.strict = true;
         else if (is("--gpu")) {}
         else if (is("--weighted-partition")) weighted = true;
         else if (is("--coarse-inverse")) coarse_inv = true;
-        else if (is("--bicgstab") || is("--analyze") || val("--matrix") || val("--drop-tol") ||
-                 val("--fill-factor")) {
-            std::cerr << "schwarz_solve: " << argv[i]
-                      << " (BiCGSTAB, ILUT, analysis, Matrix Market input) is not part of this build\n";
+        else if (val("--drop-tol") || val("--fill-factor")) {
+            std::cerr << "schwarz_solve: " << argv[i] << " (ILUT) is not part of this build\n";
             return 2;
         } else {
             usage();
             return 1;
         }
     }
-    if (poisson <= 0 && poisson3d <= 0 && jump3d <= 0) {
-        usage();
+    if (poisson <= 0 && poisson3d <= 0 && jump3d <= 0 && matrix_file.empty()) {
+        usage();  // src/main.cpp:122-126
         return 1;
     }
     try {
         CSRMatrix A;
-        if (poisson > 0) {
+        if (poisson <= 0 && poisson3d <= 0 && jump3d <= 0) {
+            std::cout << "Loading matrix: " << matrix_file << "\n";  // src/main.cpp:136-139
+            A = read_matrix_market(matrix_file, mm);
+            matrix = matrix_file;
+            if (A.nrows != A.ncols) throw std::runtime_error("the matrix is not square");
+        } else if (poisson > 0) {
             std::cout << "Generating Poisson2D " << poisson << "x" << poisson << "\n";
             A = make_poisson_2d(poisson, poisson);
             matrix = "poisson2d_" + std::to_string(poisson);
@@ -197,8 +217,11 @@ int main(int argc, char** argv) {
             A = make_jump3d(jump3d, 3);
             matrix = "jump3d_" + std::to_string(jump3d);
         }
+        if (!matrix_file.empty()) matrix = matrix_file;  // src/main.cpp:133-135
         std::cout << "  n=" << A.nrows << "  nnz=" << A.nnz() << "\n";
         const int n = A.nrows;
+        if (use_bicgstab && (use_fgmres || !learned.empty()))
+            throw std::runtime_error("--bicgstab runs alone (no --fgmres / --learned)");
         auto make_rhs = [&](uint64_t sd) {
             std::vector<double> bb(n), xs(n, 1.0);
             if (rhs == "normal") {
@@ -225,12 +248,15 @@ int main(int argc, char** argv) {
             std::vector<std::vector<double>> bs, xs(batch);
             for (int q = 0; q < batch; ++q) bs.push_back(make_rhs(seed + q));
             prm.ortho = ortho == "cgs2" ? SG_ORTHO_CGS2 : SG_ORTHO_MGS;
+            const sg_method bm = use_bicgstab ? SG_BICGSTAB : (use_fgmres ? SG_FGMRES : SG_GMRES_LEFT);
             std::cout << "Solving " << batch << " systems together with "
-                      << (use_fgmres ? "FGMRES(" : "GMRES(") << prm.restart << ") + " << precond << "\n";
+                      << (use_bicgstab ? std::string("BiCGSTAB")
+                                       : (use_fgmres ? "FGMRES(" : "GMRES(") + std::to_string(prm.restart) + ")")
+                      << " + " << precond << "\n";
             auto t0b = std::chrono::steady_clock::now();
             std::vector<GMRESResult> rs =
                 solve_batch(sys, bs, xs, kind, nparts, overlap, precision == "fp32" ? SG_FP32 : SG_FP64,
-                            prm, use_fgmres);
+                            prm, bm);
             const double tb = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0b).count();
             int conv = 0, itmin = 1 << 30, itmax = 0;
             double worst = 0.0;
@@ -248,7 +274,8 @@ int main(int argc, char** argv) {
             if (!output.empty()) {
                 Metrics m;
                 m.matrix = matrix;
-                m.method = std::string("BATCH_") + (use_fgmres ? "FGMRES+" : "GMRES+") + precond;
+                m.method = std::string("BATCH_") +
+                           (use_bicgstab ? "BiCGSTAB+" : use_fgmres ? "FGMRES+" : "GMRES+") + precond;
                 m.n = n * batch;
                 m.nnz = A.nnz() * batch;
                 m.gpus = 1;
@@ -297,6 +324,14 @@ int main(int argc, char** argv) {
         auto t1 = std::chrono::steady_clock::now();
         const double t_setup = std::chrono::duration<double>(t1 - t0).count();
 
+        if (analyze && M) {  // src/main.cpp:210-221
+            std::cout << "\n--- Pre-solve analysis ---\n";
+            print_condition_estimate(estimate_condition(A, nullptr, 80), "A (unpreconditioned)");
+            print_condition_estimate(estimate_condition(A, M.get(), 80), "M^{-1}A (preconditioned)");
+            print_spectral_result(estimate_spectral_radius(A, M.get(), 300), precond);
+            std::cout << "--------------------------\n\n";
+        }
+
         prm.ortho = ortho == "cgs2" ? SG_ORTHO_CGS2 : SG_ORTHO_MGS;
         const std::string label = use_fgmres ? "FGMRES(" + std::to_string(prm.restart) + ")"
                                              : "GMRES(" + std::to_string(prm.restart) + ")";
@@ -309,11 +344,15 @@ int main(int argc, char** argv) {
             stage->setup(A);
             use_fgmres = true;  // the stage changes per application
         }
-        const std::string label2 = use_fgmres ? "FGMRES(" + std::to_string(prm.restart) + ")" : label;
+        const std::string label2 = use_bicgstab ? std::string("BiCGSTAB")
+                                   : use_fgmres ? "FGMRES(" + std::to_string(prm.restart) + ")"
+                                                : label;
         std::cout << "Solving with " << label2 << " + " << precond << (stage ? " + learned stage" : "") << "\n";
         std::vector<double> x(n, 0.0);
         const Preconditioner* P = stage ? static_cast<const Preconditioner*>(stage.get()) : M.get();
-        GMRESResult res = use_fgmres ? fgmres(A, b, x, P, prm) : gmres(A, b, x, P, prm);
+        GMRESResult res = use_bicgstab ? bicgstab(A, b, x, P, prm)
+                          : use_fgmres ? fgmres(A, b, x, P, prm)
+                                       : gmres(A, b, x, P, prm);
         if (stage) {
             auto c = stage->counts();
             std::cout << "  Learned stage: " << c.second << "/" << c.first << " corrections kept\n";
@@ -324,10 +363,17 @@ int main(int argc, char** argv) {
                   << "  Setup time: " << t_setup << " s\n"
                   << "  Solve time: " << res.time_solve_s << " s\n"
                   << "  Relative true residual: " << res.rel_true_residual << "\n";
+        if (analyze) {  // src/main.cpp:254-261
+            ConvergenceMetrics ca = analyze_convergence(res.residual_history, res.converged);
+            ca.print_summary();
+            if (!output.empty())
+                ca.write_jsonl(output + ".conv.jsonl", matrix, precond, nparts, overlap);
+        }
         if (!output.empty()) {
             Metrics m;
             m.matrix = matrix;
-            m.method = (use_fgmres ? "FGMRES+" : "GMRES+") + precond + (stage ? "+learned" : "");
+            m.method = (use_bicgstab ? "BiCGSTAB+" : use_fgmres ? "FGMRES+" : "GMRES+") + precond +
+                       (stage ? "+learned" : "");
             m.n = n;
             m.nnz = A.nnz();
             m.ranks = 1;

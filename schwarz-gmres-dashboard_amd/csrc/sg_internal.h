@@ -163,6 +163,11 @@ void spmv_ensure_tuned(sg_csr* A);
 // result[0] = dot(a, b) on the device (deterministic tree), stream-ordered.
 void dot_launch(sg_context* ctx, int n, const double* a, const double* b, double* result);
 
+// ---- BiCGSTAB (bicgstab.hip) ---------------------------------------------------
+void bicgstab_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                   const sg_solve_params* prm, sg_solve_result* res, double* hist, int hist_cap,
+                   bool use_graph);
+
 // ---- learned stage (unet.hip, learned.hip) ---------------------------------------
 void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads);
 const float* unet_output(const sg_unet* U);

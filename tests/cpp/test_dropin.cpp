@@ -4,8 +4,11 @@
 // check and returns the number of failures.
 #include "schwarz/schwarz.h"
 
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <stdexcept>
 #include <string>
 
 using namespace schwarz;
@@ -180,7 +183,52 @@ static void extensions() {
           "Metrics::to_json field order");
 }
 
+static void next_rows() {
+    // BiCGSTAB (src/core/bicgstab.h): relative tolerance, right preconditioning
+    CSRMatrix A = make_poisson_2d(30, 30);
+    std::vector<double> ones(A.nrows, 1.0), b;
+    A.spmv(ones, b);
+    ILU0Precond ilu;
+    ilu.setup(A);
+    std::vector<double> x(A.nrows, 0.0);
+    GMRESResult r = bicgstab(A, b, x, &ilu);
+    CHECK(r.converged && std::abs(r.iterations - 23) <= 1,
+          "BiCGSTAB + ILU0 on 30x30: the reference's 23 iterations");
+    CHECK(l1(x, ones) < 1e-6 * A.nrows, "BiCGSTAB solution");
+    CHECK((int)r.residual_history.size() == r.iterations, "one history entry per iteration");
+    // Matrix Market round trip (src/io/matrix_market.h)
+    const std::string path = "/tmp/sgamd_dropin_" + std::to_string(::getpid()) + ".mtx";
+    write_matrix_market(path, A, 17);
+    CSRMatrix B = read_matrix_market(path);
+    std::remove(path.c_str());
+    CHECK(B.nrows == A.nrows && B.row_ptr == A.row_ptr && B.col_idx == A.col_idx &&
+              B.vals == A.vals,
+          "Matrix Market write/read round trip");
+    bool threw = false;
+    try {
+        read_matrix_market(path);
+    } catch (const std::runtime_error& e) {
+        threw = std::string(e.what()) == "Cannot open file: " + path;
+    }
+    CHECK(threw, "missing file throws the reference's message");
+    // analysis (src/analysis/*.h): Poisson 2D 12x12 closed-form extremes
+    CSRMatrix P = make_poisson_2d(12, 12);
+    ConditionEstimate ce = estimate_condition(P, nullptr, 80);
+    const double lo = 8.0 * std::pow(std::sin(M_PI / 26.0), 2);
+    CHECK(ce.valid && std::abs(ce.lambda_min - lo) < 1e-10 && std::abs(ce.lambda_max - (8 - lo)) < 1e-9,
+          "Lanczos extremes of Poisson 12x12");
+    JacobiPrecond jac;
+    jac.setup(P);
+    SpectralResult sr = estimate_spectral_radius(P, &jac, 300);
+    CHECK(sr.power_iters > 0 && sr.spectral_radius < 1.0 && (int)sr.eigenvector.size() == P.nrows,
+          "power iteration with Jacobi");
+    ConvergenceMetrics cm = analyze_convergence(r.residual_history, r.converged);
+    CHECK(cm.iterations == r.iterations && cm.converged && cm.stagnation_start == -1,
+          "convergence metrics of the BiCGSTAB run");
+}
+
 int main() {
+    next_rows();
     gmres_identity();
     gmres_spd3();
     gmres_poisson5();

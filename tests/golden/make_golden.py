@@ -8,7 +8,8 @@ small JSON fixtures.  Subdomain index sets for the RAS/ASM fixtures are built
 here by a brute-force breadth-first search over the dense adjacency of the
 grid, independent of the oracle.
 
-Usage:  python tests/golden/make_golden.py      (needs /root/reference)
+Usage:  python tests/golden/make_golden.py [bicgstab analysis conv mm]
+        (needs /root/reference; with names, only those fixtures are rewritten)
 """
 import json
 import os
@@ -92,9 +93,131 @@ def blocks_case(spec, part, nparts, overlap, seed):
             "asm": out["asm"]}
 
 
+# Matrix Market inputs for the reader fixtures: (name, text).  They cover the
+# reference's header handling (flags from any '%' line), symmetric mirroring,
+# pattern files, unsorted and duplicate entries, CRLF line ends, missing and
+# unreadable values, number formats, rectangular shapes and its two error
+# messages.
+MM_CASES = [
+    ("general", "%%MatrixMarket matrix coordinate real general\n% a comment\n4 4 7\n"
+                "4 4 2.5\n1 1 4\n2 1 -1\n1 2 -1e-3\n3 3 1.25E+2\n2 2 4\n3 4 .5\n"),
+    ("symmetric", "%%MatrixMarket matrix coordinate real symmetric\n5 5 8\n1 1 2\n2 1 -1\n"
+                  "2 2 2\n3 2 -1\n3 3 2\n5 3 -0.25\n4 4 2\n5 5 2\n"),
+    ("pattern_sym", "%%MatrixMarket matrix coordinate pattern symmetric\n4 4 5\n1 1\n2 1\n"
+                    "3 3\n4 2\n4 4\n"),
+    ("duplicates", "%%MatrixMarket matrix coordinate real general\n3 3 6\n1 1 1\n2 2 2\n"
+                   "1 1 3\n3 3 5\n2 2 7\n1 3 1\n"),
+    ("comment_flag", "%%MatrixMarket matrix coordinate real general\n"
+                     "% this matrix is not symmetric\n3 3 3\n1 1 1\n2 1 5\n3 3 1\n"),
+    ("crlf", "%%MatrixMarket matrix coordinate real general\r\n3 3 3\r\n1 1 1.5\r\n"
+             "2 3 -2\r\n3 3 4\r\n"),
+    ("integer", "%%MatrixMarket matrix coordinate integer general\n2 2 3\n1 1 3\n1 2 -7\n"
+                "2 2 +9\n"),
+    ("missing_value", "%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1\n1 2 abc\n"
+                      "2 2 1e\n"),
+    ("numbers", "%%MatrixMarket matrix coordinate real general\n3 3 6\n1 1 1e400\n"
+                "1 2 -1e400\n2 2 1e-310\n2 3 -0.0\n3 3 123456789012345678901234567890\n"
+                "3 1 0.1234567890123456789 trailing tokens\n"),
+    ("skew", "%%MatrixMarket matrix coordinate real skew-symmetric\n3 3 2\n2 1 3\n3 2 -4\n"),
+    ("rect", "%%MatrixMarket matrix coordinate real general\n   2   5  4  \n2 5 1\n1 4 2\n"
+             "1 1 3\n2 2 4\n"),
+    ("premature", "%%MatrixMarket matrix coordinate real general\n3 3 4\n1 1 1\n2 2 1\n"),
+    ("empty_rows", "%%MatrixMarket matrix coordinate real general\n5 5 2\n4 2 1\n4 1 2\n"),
+]
+
+
+def gen_bicgstab():
+    """BiCGSTAB (src/core/bicgstab.h): relative tol, right preconditioning."""
+    cases = [
+        ("bicgstab", "ilu0", "2d:30:30", "ones", 0, 1000, 1e-10),
+        ("bicgstab", "none", "2d:30:30", "ones", 0, 1000, 1e-10),
+        ("bicgstab", "jacobi", "2d:40:40", "uniform:2", 0, 1000, 1e-10),
+        ("bicgstab", "ilu0", "3d:12:12:12", "uniform:3", 0, 1000, 1e-10),
+        ("bicgstab", "ilu0", "2d:100:100", "ones", 0, 1000, 1e-10),
+        ("bicgstab", "none", "2d:30:30", "uniform:4", 0, 9, 1e-10),    # iteration bound
+        ("bicgstab", "jacobi", "2d:20:20", "ones", 0, 1000, 1e-3),     # loose: half-step exit likely
+        ("bicgstab", "ilu0", "rnd:300:5", "uniform:6", 0, 1000, 1e-10),  # nonsymmetric
+    ]
+    out = []
+    for c in cases:
+        r = run("solve", *c)
+        r.update(dict(method=c[0], pc=c[1], grid=c[2], rhs=c[3], max_iter=c[5], tol_arg=c[6]))
+        out.append(r)
+    dump("bicgstab.json", out)
+
+
+def gen_analysis():
+    """estimate_condition / estimate_spectral_radius (src/analysis/*.h)."""
+    cases = [("none", "2d:20:20", 80, 300), ("jacobi", "2d:20:20", 80, 300),
+             ("ilu0", "2d:30:30", 80, 300), ("ilu0", "3d:8:8:8", 80, 300),
+             ("none", "2d:6:6", 80, 300), ("ilu0", "rnd:200:7", 40, 200),
+             ("jacobi", "2d:3:1", 80, 50)]
+    out = []
+    for pc, spec, k, pit in cases:
+        r = run("analyze", pc, spec, k, pit)
+        r.update(dict(pc=pc, grid=spec, k=k, power_iters_max=pit))
+        out.append(r)
+    dump("analysis.json", out)
+
+
+def gen_conv():
+    """ConvergenceMetrics (src/analysis/convergence_analysis.h): JSONL line and summary."""
+    rng = np.random.default_rng(21)
+    hists = [
+        (np.geomspace(1.0, 1e-9, 40) * (1 + 0.1 * rng.standard_normal(40))).tolist(),
+        ([1.0] + [0.5] * 30 + [0.4]),  # stagnation
+        [3.5],
+        [],
+        (np.geomspace(2e3, 1e-3, 25)).tolist(),
+    ]
+    out = []
+    import tempfile
+    for i, h in enumerate(hists):
+        conv = i % 2
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "c.jsonl")
+            txt = f"{len(h)}\n" + " ".join(repr(float(t)) for t in h) + "\n"
+            so = subprocess.run([HARNESS, "conv", path, str(conv)], input=txt,
+                                capture_output=True, text=True, check=True).stdout
+            line = open(path).read()
+        out.append({"hist": [float(t) for t in h], "converged": conv, "jsonl": line,
+                    "summary": so})
+    dump("conv.json", out)
+
+
+def gen_mm():
+    """read_matrix_market / write_matrix_market (src/io/matrix_market.h)."""
+    import tempfile
+    reads = []
+    with tempfile.TemporaryDirectory() as td:
+        for name, text in MM_CASES:
+            path = os.path.join(td, name + ".mtx")
+            with open(path, "w", newline="") as f:
+                f.write(text)
+            r = run("mm", path)
+            if "error" in r:
+                r["error"] = r["error"].replace(td + os.sep, "")
+            reads.append({"name": name, "text": text, "out": r})
+        writes = []
+        for spec in ("2d:3:3", "rnd:12:3"):
+            path = os.path.join(td, "w.mtx")
+            m = run("mmw", spec, path)
+            writes.append({"grid": spec, "matrix": m, "text": open(path).read()})
+    dump("mm.json", {"reads": reads, "writes": writes})
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -C oracle ref")
+    only = set(sys.argv[1:])
+    extra = {"bicgstab": gen_bicgstab, "analysis": gen_analysis, "conv": gen_conv, "mm": gen_mm}
+    if only:
+        for k in sorted(only):
+            extra[k]()
+        print("fixtures written to", HERE)
+        return
+    for f in extra.values():
+        f()
 
     # ILU(0) factors and one apply (src/precond/ilu0.h).
     ilu = {spec: run("ilu0", spec) for spec in ("2d:8:8", "2d:16:16", "3d:6:6:6", "3d:5:4:3")}

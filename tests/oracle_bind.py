@@ -57,6 +57,15 @@ def lib():
                                C.POINTER(C.c_int), _D, C.c_int, C.POINTER(C.c_int)]
         L.or_metrics_json.argtypes = [C.c_char_p, C.c_char_p] + [C.c_int] * 8 + \
             [C.c_double] * 6 + [C.c_char_p, C.c_int]
+        L.or_bicgstab.argtypes = [C.c_int, _I, _I, _D, _D, _D, C.c_void_p, C.c_int, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int), _D, C.c_int,
+                                  C.POINTER(C.c_int)]
+        L.or_lanczos.argtypes = [C.c_int, _I, _I, _D, C.c_void_p, C.c_int, C.c_uint, _D, _D]
+        L.or_tridiag_eigs.argtypes = [C.c_int, _D, _D, _D]
+        L.or_condition.argtypes = [C.c_int, _I, _I, _D, C.c_void_p, C.c_int, C.c_uint, _D,
+                                   C.POINTER(C.c_int)]
+        L.or_spectral.argtypes = [C.c_int, _I, _I, _D, C.c_void_p, C.c_int, C.c_double,
+                                  C.c_uint, C.POINTER(C.c_double)]
     return _lib
 
 
@@ -103,6 +112,11 @@ def poisson3d(nx, ny, nz):
 
 
 def grid(spec):
+    """"2d:NX:NY", "3d:NX:NY:NZ" or "rnd:N:SEED" (tests/rng_ref.rnd_matrix)."""
+    if spec.startswith("rnd:"):
+        from rng_ref import rnd_matrix
+        n, seed = (int(t) for t in spec.split(":")[1:])
+        return Csr(*rnd_matrix(n, seed))
     dims = [int(t) for t in spec.split(":")[1:]]
     return poisson2d(*dims) if len(dims) == 2 else poisson3d(*dims)
 
@@ -240,3 +254,47 @@ def unet_loss_grad(params, x, t, round_bf16=True, kind=0):
     loss = lib().or_unet_loss_grad_kind(np.ascontiguousarray(params, np.float32), B, H, W, x, tt,
                                         kind, 1 if round_bf16 else 0, g)
     return loss, g
+
+
+def bicgstab(A, b, pc=None, max_iter=1000, tol=1e-10, x0=None):
+    """Reference-semantics BiCGSTAB (relative tol)."""
+    x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
+    rn, conv, hl = C.c_double(), C.c_int(), C.c_int()
+    cap = max(max_iter, 1) + 2
+    hist = np.zeros(cap)
+    it = lib().or_bicgstab(A.n, A.rp, A.ci, A.v, np.ascontiguousarray(b, np.float64), x,
+                           pc.h if pc else None, max_iter, tol, C.byref(rn), C.byref(conv),
+                           hist, cap, C.byref(hl))
+    return dict(iterations=it, residual_norm=rn.value, converged=bool(conv.value),
+                hist=hist[:min(hl.value, cap)].copy(), x=x)
+
+
+def lanczos(A, pc=None, k=80, seed=42):
+    kk = max(2, min(k, A.n))
+    al, be = np.zeros(kk), np.zeros(kk)
+    m = lib().or_lanczos(A.n, A.rp, A.ci, A.v, pc.h if pc else None, k, seed, al, be)
+    return al[:m].copy(), be[:max(m - 1, 0)].copy()
+
+
+def tridiag_eigs(d, e):
+    d = np.ascontiguousarray(d, np.float64)
+    e = np.ascontiguousarray(np.append(e, 0.0), np.float64)
+    out = np.zeros(len(d))
+    lib().or_tridiag_eigs(len(d), d, e, out)
+    return out
+
+
+def condition(A, pc=None, k=80, seed=42):
+    out = np.zeros(3)
+    valid = C.c_int()
+    m = lib().or_condition(A.n, A.rp, A.ci, A.v, pc.h if pc else None, k, seed, out,
+                           C.byref(valid))
+    return dict(lambda_min=out[0], lambda_max=out[1], kappa=out[2], steps=m,
+                valid=int(valid.value))
+
+
+def spectral(A, pc=None, max_iter=200, tol=1e-6, seed=123):
+    rho = C.c_double()
+    it = lib().or_spectral(A.n, A.rp, A.ci, A.v, pc.h if pc else None, max_iter, tol, seed,
+                           C.byref(rho))
+    return dict(rho=rho.value, power_iters=it, ok=int(rho.value < 1.0))

@@ -36,3 +36,31 @@ def splitmix_normal(seed, n):
     out[0::2] = r * np.cos(2.0 * np.pi * u2)
     out[1::2] = r * np.sin(2.0 * np.pi * u2)
     return out[:n]
+
+
+def rnd_matrix(n, seed):
+    """The "rnd:N:SEED" test matrix: per row, the diagonal plus three column
+    draws (u64 % n), sorted and de-duplicated; then one uniform u per kept
+    column in ascending order, giving 4 + |2u - 1| on the diagonal and 2u - 1
+    elsewhere.  Returns (row_ptr, col_idx, vals)."""
+    draws = iter(splitmix_u64(seed, 7 * n).tolist())
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        cols = sorted({i, *(next(draws) % n for _ in range(3))})
+        for j in cols:
+            u = 2.0 * ((next(draws) >> 11) * 2.0 ** -53) - 1.0
+            ci.append(j)
+            v.append(4.0 + abs(u) if j == i else u)
+        rp.append(len(ci))
+    return (np.array(rp, np.int32), np.array(ci, np.int32), np.array(v))
+
+
+def lcg_unit(n, seed):
+    """Start vector of the analysis routines: a 32-bit LCG (multiplier 1664525,
+    increment 1013904223), each state read as a signed int / 2^31, normalised."""
+    v = np.empty(n)
+    st = seed & 0xFFFFFFFF
+    for i in range(n):
+        st = (st * 1664525 + 1013904223) & 0xFFFFFFFF
+        v[i] = (st - (1 << 32) if st >= (1 << 31) else st) / 2147483648.0
+    return v / np.sqrt(np.sum(v * v))

@@ -45,7 +45,8 @@ def test_cli_exit_codes(tmp_path):
     cli = os.path.join(PKG, "schwarz_solve")
     assert subprocess.run([cli], capture_output=True).returncode == 1
     assert subprocess.run([cli, "--bogus"], capture_output=True).returncode == 1
-    assert subprocess.run([cli, "--poisson", "8", "--bicgstab"], capture_output=True).returncode == 2
+    assert subprocess.run([cli, "--poisson", "8", "--drop-tol", "1e-3"],
+                          capture_output=True).returncode == 2  # ILUT
     # not converged -> 1
     r = subprocess.run([cli, "--poisson", "64", "--precond", "none", "--maxiter", "5",
                         "--restart", "5"], capture_output=True, text=True, timeout=120)
@@ -98,3 +99,62 @@ def test_cli_learned_stage():
     assert float(lines["Relative true residual"]) <= 1e-8
     kept, calls = map(int, lines["Learned stage"].split()[0].split("/"))
     assert calls >= int(lines["Iterations"]) and 0 <= kept <= calls
+
+
+def test_cli_bicgstab_matches_reference(tmp_path):
+    """--bicgstab: reference label, iteration count of the reference's
+    bicgstab (golden: Poisson 100^2, ILU(0), tol 1e-10 relative, b = A*1)."""
+    from conftest import golden
+    ref = next(c for c in golden("bicgstab.json") if c["grid"] == "2d:100:100")
+    jl = tmp_path / "b.jsonl"
+    out = subprocess.run([os.path.join(PKG, "schwarz_solve"), "--poisson", "100", "--precond",
+                          "ilu0", "--bicgstab", "--output", str(jl)], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "Solving with BiCGSTAB + ilu0" in out.stdout
+    lines = dict(l.strip().split(": ", 1) for l in out.stdout.splitlines() if ": " in l)
+    # within the summation-order band of test_gpu_bicgstab.py
+    assert abs(int(lines["Iterations"]) - ref["iterations"]) <= max(2, round(0.12 * ref["iterations"]))
+    rec = json.loads(jl.read_text().splitlines()[-1])
+    assert rec["method"] == "BiCGSTAB+ilu0"
+
+
+def test_cli_matrix_market_and_analyze(tmp_path):
+    """--matrix F (label = the path, b = A*1) and --analyze (pre-solve
+    estimates, post-solve summary, <output>.conv.jsonl)."""
+    import numpy as np
+    import oracle_bind as O
+    A = O.poisson2d(40, 40)
+    mtx = tmp_path / "p40.mtx"
+    lines = ["%%MatrixMarket matrix coordinate real symmetric", f"{A.n} {A.n} "]
+    ent = []
+    for i in range(A.n):
+        for k in range(A.rp[i], A.rp[i + 1]):
+            if A.ci[k] <= i:
+                ent.append(f"{i + 1} {A.ci[k] + 1} {A.v[k]:.17g}")
+    lines[1] += str(len(ent))
+    mtx.write_text("\n".join(lines + ent) + "\n")
+    jl = tmp_path / "m.jsonl"
+    cli = os.path.join(PKG, "schwarz_solve")
+    out = subprocess.run([cli, "--matrix", str(mtx), "--precond", "ilu0", "--analyze",
+                          "--output", str(jl)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert f"Loading matrix: {mtx}" in out.stdout and f"n={A.n}  nnz={A.nnz}" in out.stdout
+    assert "--- Pre-solve analysis ---" in out.stdout
+    assert "Condition estimate (A (unpreconditioned)) [Lanczos 80 steps]:" in out.stdout
+    assert "rho(I - M^{-1}A) = " in out.stdout and "Convergence analysis:" in out.stdout
+    rec = json.loads(jl.read_text().splitlines()[-1])
+    assert rec["matrix"] == str(mtx) and rec["n"] == A.n and rec["nnz"] == A.nnz
+    conv = json.loads((tmp_path / "m.jsonl.conv.jsonl").read_text().splitlines()[-1])
+    assert conv["type"] == "convergence" and conv["matrix"] == str(mtx)
+    assert conv["precond"] == "ilu0" and conv["iterations"] == rec["iterations"]
+    # the same system from the generator gives the same iteration count
+    out2 = subprocess.run([cli, "--poisson", "40", "--precond", "ilu0"], capture_output=True,
+                          text=True, timeout=300)
+    it = lambda s: [l for l in s.splitlines() if "Iterations:" in l][0]  # noqa: E731
+    assert it(out.stdout) == it(out2.stdout)
+    # a missing file is an error, not a crash
+    bad = subprocess.run([cli, "--matrix", str(tmp_path / "none.mtx")], capture_output=True,
+                         text=True, timeout=60)
+    assert bad.returncode == 3 and "Cannot open file" in bad.stderr
+    np.testing.assert_equal(rec["iterations"] > 0, True)
