@@ -4,7 +4,10 @@ against the reference's estimate_condition / estimate_spectral_radius
 oracle/_ref) and the CPU oracle's restatement (bit for bit against those
 goldens in test_oracle.py).
 
-Bars: Lanczos step counts equal; Ritz extremes and kappa within 1e-9
+Bars: Lanczos step counts equal, except where the recurrence reaches an
+invariant subspace and the reference's beta at that step is itself roundoff
+(below 1e-7 (1 + |alpha|), e.g. 2.6e-9 on Poisson 6 x 6): then the device may
+stop there, earlier than the reference; Ritz extremes and kappa within 1e-9
 relative (both recurrences are stable to about 1e-13 under a change of
 summation order); power-iteration counts within +-1 and the radius within
 1e-9 relative; the first Lanczos coefficients within 1e-12 of the oracle's.
@@ -36,9 +39,14 @@ def test_matches_reference(sg, ctx, case):
     A = O.grid(case["grid"])
     dA = sg.Csr(ctx, A.rp, A.ci, A.v)
     M = None if case["pc"] == "none" else sg.Precond(ctx, case["pc"]).setup(dA)
-    for key, MM in (("cond_A", None), ("cond_MA", M)):
+    Mo = None if case["pc"] == "none" else O.Precond(case["pc"], A)
+    for key, MM, MMo in (("cond_A", None, None), ("cond_MA", M, Mo)):
         got, ref = sg.estimate_condition(dA, MM, case["k"]), case[key]
-        assert got["lanczos_steps"] == ref["steps"], key
+        if got["lanczos_steps"] != ref["steps"]:
+            al, be = O.lanczos(A, MMo, case["k"])  # the reference's recurrence, bit for bit
+            j = got["lanczos_steps"] - 1
+            assert got["lanczos_steps"] < ref["steps"], key
+            assert be[j] < 1e-7 * (1.0 + abs(al[j])), (key, j, be[j])
         assert got["valid"] == bool(ref["valid"]), key
         for f in ("lambda_min", "lambda_max", "kappa"):
             assert np.isclose(got[f], ref[f], rtol=1e-9, atol=0), (key, f, got[f], ref[f])
