@@ -92,6 +92,20 @@ public:
     std::vector<double> lu_vals() const;
 };
 
+// Dual-threshold ILU (src/precond/ilut.h:21-193): factorisation on the host as
+// the reference's, triangular sweeps on the device.
+class ILUTPrecond : public DevicePreconditioner {
+public:
+    explicit ILUTPrecond(double drop_tol = 1e-4, int fill_factor = 10)
+        : DevicePreconditioner(SG_PC_ILUT, 1, 0) {
+        opts_.drop_tol = drop_tol;
+        opts_.fill_factor = fill_factor;
+    }
+    int n() const { return n_; }
+    int l_nnz() const;
+    int u_nnz() const;
+};
+
 class ASMPrecond : public DevicePreconditioner {
 public:
     ASMPrecond(int nparts, int overlap) : DevicePreconditioner(SG_PC_ASM, nparts, overlap) {}

@@ -115,7 +115,10 @@ typedef enum {
     SG_PC_RAS = 4,
     SG_PC_TWOLEVEL_ASM = 5,
     SG_PC_TWOLEVEL_RAS = 6,
-    SG_PC_LEARNED = 7 /* made by sg_precond_create_learned only */
+    SG_PC_LEARNED = 7, /* made by sg_precond_create_learned only */
+    SG_PC_ILUT = 8     /* ILUTPrecond(drop_tol, fill_factor) (src/precond/ilut.h:21-193):
+                          dual-threshold factorisation on the host, as the reference's;
+                          forward / backward sweeps on the device (FP64 or FP32)      */
 } sg_precond_kind;
 
 typedef enum { SG_FP64 = 0, SG_FP32 = 1 } sg_precision;
@@ -133,6 +136,11 @@ typedef struct {
     int weighted_partition; /* built-in partitioner only: 1 weighs edge (i,j) by |a_ij| so
                               cuts follow weak couplings; 0 = unit weights as the
                               reference's METIS call (domain_decomp.h:48-52)              */
+    double drop_tol;       /* ILUT: relative drop tolerance (entries below drop_tol times
+                              the row's max |a_ij| are dropped)                           */
+    int fill_factor;       /* ILUT: keep at most fill_factor * (nnz / n) entries in each
+                              row of L and of U; drop_tol = fill_factor = 0 selects the
+                              reference's defaults 1e-4 and 10                            */
 } sg_precond_opts;
 
 sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_precond_opts* opts,
@@ -190,6 +198,8 @@ sg_status sg_precond_subdomains(const sg_precond* M, int* total, int* ptr, int* 
 /* ILU(0) factor values of subdomain s in that subdomain's local CSR order
  * (ILU0Precond::lu_vals, ilu0.h:83); s = 0 for the global ILU(0). */
 sg_status sg_precond_ilu_values(const sg_precond* M, int s, int* nnz, double* vals);
+/* ILUTPrecond::l_nnz() / u_nnz() (src/precond/ilut.h:175-176). */
+sg_status sg_precond_ilut_nnz(const sg_precond* M, long long* l_nnz, long long* u_nnz);
 /* Coarse matrix A0 (row-major nc x nc) and its LU factors / pivots. */
 sg_status sg_precond_coarse(const sg_precond* M, int* nc, double* A0, double* LU, int* piv);
 

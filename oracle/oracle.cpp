@@ -247,6 +247,102 @@ struct Ilu0Pc : Precond {
     }
 };
 
+// ILUT: src/precond/ilut.h:28-172, L and U kept as separate row lists as the
+// reference stores them.  Elimination visits the row's sorted pattern as it
+// stood before elimination (the reference's range-for, ilut.h:62); fill joins
+// the list only for the later split.  precision 1 runs both sweeps in FP32.
+struct IlutPc : Precond {
+    int n = 0, precision = 0;
+    std::vector<std::vector<std::pair<int, double>>> L, U;
+    std::vector<double> ud;
+    long long lnz = 0, unz = 0;
+    size_t max_list = 0;  // longest index list of a row (the reference reserves 2 * lim)
+    IlutPc(const Csr& a, double drop, int fill, int prec) : n(a.n), precision(prec) {
+        if (n == 0) return;
+        const int lim = fill * std::max(1, a.rp[n] / n);
+        L.resize(n);
+        U.resize(n);
+        std::vector<double> w(n, 0.0);
+        std::vector<int> idx;
+        using P = std::pair<int, double>;
+        auto trim = [lim](std::vector<P>& v) {
+            if ((int)v.size() <= lim) return;
+            std::partial_sort(v.begin(), v.begin() + lim, v.end(), [](const P& x, const P& y) {
+                return std::abs(x.second) > std::abs(y.second);
+            });
+            v.resize(lim);
+            std::sort(v.begin(), v.end(), [](const P& x, const P& y) { return x.first < y.first; });
+        };
+        for (int i = 0; i < n; ++i) {
+            for (int p = a.rp[i]; p < a.rp[i + 1]; ++p) {
+                if (w[a.ci[p]] == 0.0) idx.push_back(a.ci[p]);
+                w[a.ci[p]] += a.v[p];
+            }
+            double nrm = 0.0;
+            for (int j : idx) nrm = std::max(nrm, std::abs(w[j]));
+            const double t = drop * nrm;
+            std::sort(idx.begin(), idx.end());
+            const std::vector<int> pattern(idx);
+            for (int k : pattern) {
+                if (k >= i) break;
+                if (std::abs(w[k]) < t || std::abs(ud[k]) < 1e-30) {
+                    w[k] = 0.0;
+                    continue;
+                }
+                w[k] = w[k] / ud[k];
+                for (const P& e : U[k]) {
+                    if (e.first <= k) continue;
+                    if (w[e.first] == 0.0) idx.push_back(e.first);
+                    w[e.first] -= w[k] * e.second;
+                }
+            }
+            std::sort(idx.begin(), idx.end());
+            max_list = std::max(max_list, idx.size());
+            std::vector<P> lo, up;
+            for (int j : idx) {
+                const double x = w[j];
+                if (std::abs(x) == 0.0) continue;
+                if (j < i) {
+                    if (std::abs(x) >= t) lo.emplace_back(j, x);
+                } else if (j == i || std::abs(x) >= t) {
+                    up.emplace_back(j, x);
+                }
+            }
+            trim(lo);
+            trim(up);
+            double d = 0.0;
+            for (const P& e : up)
+                if (e.first == i) { d = e.second; break; }
+            ud.push_back(std::abs(d) < 1e-30 ? 1e-30 : d);
+            lnz += (long long)lo.size();
+            unz += (long long)up.size();
+            L[i] = std::move(lo);
+            U[i] = std::move(up);
+            for (int j : idx) w[j] = 0.0;
+            idx.clear();
+        }
+    }
+    template <typename T>
+    void sweep(const T* x, T* y) const {
+        std::vector<T> z(n);
+        for (int i = 0; i < n; ++i) {  // ilut.h:156-161
+            z[i] = x[i];
+            for (const auto& e : L[i]) z[i] -= (T)e.second * z[e.first];
+        }
+        for (int i = n - 1; i >= 0; --i) {  // ilut.h:164-171: the first U entry is skipped
+            y[i] = z[i];
+            for (size_t p = 1; p < U[i].size(); ++p) y[i] -= (T)U[i][p].second * y[U[i][p].first];
+            y[i] /= (T)ud[i];
+        }
+    }
+    void apply(const double* x, double* y) const override {
+        if (precision == 0) { sweep(x, y); return; }
+        std::vector<float> a(x, x + n), b(n);
+        sweep(a.data(), b.data());
+        for (int i = 0; i < n; ++i) y[i] = b[i];
+    }
+};
+
 // One-level Schwarz: src/precond/ras.h:21-70 (restricted) and
 // src/precond/asm.h:20-62 (additive).  ASM accumulates in ascending subdomain
 // order, which is the single-thread order of the reference's critical section.
@@ -748,6 +844,16 @@ void* or_precond_new(int kind, int n, const int* rp, const int* ci, const double
         return nullptr;
     }
 }
+
+void* or_ilut_new(int n, const int* rp, const int* ci, const double* v, double drop_tol,
+                  int fill_factor, int precision, long long* l_nnz, long long* u_nnz) {
+    IlutPc* p = new IlutPc(make_csr(n, rp, ci, v), drop_tol, fill_factor, precision);
+    if (l_nnz) *l_nnz = p->lnz;
+    if (u_nnz) *u_nnz = p->unz;
+    return p;
+}
+
+long long or_ilut_max_list(void* h) { return (long long)static_cast<IlutPc*>(h)->max_list; }
 
 void or_precond_apply(void* h, const double* x, double* y) {
     static_cast<Precond*>(h)->apply(x, y);

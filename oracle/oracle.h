@@ -30,6 +30,10 @@
  *                              (oracle/_ref), known answers 141 (Poisson 100^2)
  *                              and 248 (150^2), identity and SPD exactness.
  *   or_metrics_json            golden strings from reference Metrics::to_json.
+ *   or_ilut_new                golden apply outputs, l_nnz / u_nnz and GMRES
+ *                              iteration counts from the reference ILUTPrecond
+ *                              (oracle/_ref), bit for bit; drop_tol = 0 with
+ *                              unbounded fill on a tridiagonal == exact LU.
  *   or_bicgstab                golden iteration counts, residual histories and
  *                              solutions from the reference bicgstab.h
  *                              (oracle/_ref), bit for bit; identity -> 1 step.
@@ -74,6 +78,13 @@ enum { OR_PC_NONE = 0, OR_PC_JACOBI = 1, OR_PC_ILU0 = 2, OR_PC_ASM = 3, OR_PC_RA
 void* or_precond_new(int kind, int n, const int* rp, const int* ci, const double* v,
                      int nparts, int overlap, const int* part, int precision);
 void or_precond_apply(void* h, const double* x, double* y);
+/* ILUTPrecond(drop_tol, fill_factor) (src/precond/ilut.h:21-193); l_nnz / u_nnz
+ * as its l_nnz() / u_nnz(); freed with or_precond_free. */
+void* or_ilut_new(int n, const int* rp, const int* ci, const double* v, double drop_tol,
+                  int fill_factor, int precision, long long* l_nnz, long long* u_nnz);
+/* Longest per-row index list: the reference's elimination loop is defined only
+ * while it stays within the 2 * fill_factor * (nnz / n) entries it reserves. */
+long long or_ilut_max_list(void* h);
 int or_precond_coarse(void* h, double* A0, double* LU, int* piv);
 void or_precond_free(void* h);
 

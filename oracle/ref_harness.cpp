@@ -17,6 +17,7 @@
 #include "core/bicgstab.h"
 #include "precond/ilu0.h"
 #include "precond/jacobi.h"
+#include "precond/ilut.h"
 #include "precond/coarse_solve.h"
 #include "mesh/poisson.h"
 #include "io/metrics_logger.h"
@@ -114,8 +115,24 @@ int cmd_ilu0(const std::string& spec) {
     return 0;
 }
 
+// ilut GRID DROP FILL: l_nnz, u_nnz and one apply on a SplitMix x.
+int cmd_ilut(const std::string& spec, double drop, int fill) {
+    CSRMatrix A = grid(spec);
+    ILUTPrecond p(drop, fill);
+    p.setup(A);
+    std::vector<double> x(A.nrows), y(A.nrows);
+    SplitMix64 rng(11);
+    for (auto& t : x) t = 2.0 * rng.uniform01() - 1.0;
+    p.apply(x.data(), y.data(), A.nrows);
+    std::printf("{\"l_nnz\":%d,\"u_nnz\":%d,", p.l_nnz(), p.u_nnz());
+    print_vec("x", x);
+    print_vec("y", y, true);
+    std::printf("}\n");
+    return 0;
+}
+
 int cmd_solve(int argc, char** argv) {
-    // solve METHOD PC GRID RHS RESTART MAXITER TOL
+    // solve METHOD PC GRID RHS RESTART MAXITER TOL  (PC "ilut" uses ILUTPrecond())
     if (argc < 9) return 2;
     std::string method = argv[2], pc = argv[3], rhs = argv[5];
     CSRMatrix A = grid(argv[4]);
@@ -133,6 +150,7 @@ int cmd_solve(int argc, char** argv) {
     std::unique_ptr<Preconditioner> M;
     if (pc == "jacobi") M.reset(new JacobiPrecond());
     else if (pc == "ilu0") M.reset(new ILU0Precond());
+    else if (pc == "ilut") M.reset(new ILUTPrecond());
     if (M) M->setup(A);
     // A negative TOL means relative: |TOL| * ||M^-1 b|| for GMRES (the quantity
     // its absolute test measures at x0 = 0) and |TOL| * ||b|| for FGMRES.
@@ -329,6 +347,7 @@ int main(int argc, char** argv) {
     if (c == "lu" && argc >= 5) return cmd_lu(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]));
     if (c == "metrics") return cmd_metrics();
     if (c == "analyze") return cmd_analyze(argc, argv);
+    if (c == "ilut" && argc >= 5) return cmd_ilut(argv[2], std::atof(argv[3]), std::atoi(argv[4]));
     if (c == "conv") return cmd_conv(argc, argv);
     if (c == "mm" && argc >= 3) return cmd_mm(argv[2]);
     if (c == "mmw" && argc >= 4) {  // mmw GRID OUT: write_matrix_market; the matrix on stdout

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsgamd.so")
 
 PC_KIND = {"none": 0, "jacobi": 1, "ilu0": 2, "asm": 3, "ras": 4, "twolevel_asm": 5,
-           "twolevel_ras": 6}
+           "twolevel_ras": 6, "ilut": 8}
 STATUS = {0: "SG_OK", 1: "SG_E_ARG", 2: "SG_E_HIP", 3: "SG_E_SINGULAR", 4: "SG_E_PARTITION",
           5: "SG_E_IO", 6: "SG_E_NOCONV", 7: "SG_E_STATE"}
 K_FAMILY = {"spmv": 0, "precond": 1, "trsv": 2, "ortho": 3, "update": 4}
@@ -34,7 +34,8 @@ class SgError(RuntimeError):
 class PrecondOpts(C.Structure):
     _fields_ = [("nparts", C.c_int), ("overlap", C.c_int), ("precision", C.c_int),
                 ("partition", C.POINTER(C.c_int)), ("ufactor", C.c_int),
-                ("coarse_solve", C.c_int), ("weighted_partition", C.c_int)]
+                ("coarse_solve", C.c_int), ("weighted_partition", C.c_int),
+                ("drop_tol", C.c_double), ("fill_factor", C.c_int)]
 
 
 class SolveParams(C.Structure):
@@ -151,6 +152,8 @@ def lib():
                         C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_double))],
                        C.c_int),
         "sg_mm_free": ([_VP], None),
+        "sg_precond_ilut_nnz": ([_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+                                C.c_int),
         "sg_estimate_condition": ([_VP, _VP, C.c_int, C.c_uint, C.POINTER(ConditionEstimate),
                                    _VP, _VP], C.c_int),
         "sg_estimate_spectral_radius": ([_VP, _VP, C.c_int, C.c_double, C.c_uint,
@@ -456,14 +459,16 @@ class Precond:
     """Device preconditioner (sg_precond): kind in PC_KIND."""
 
     def __init__(self, ctx, kind, nparts=4, overlap=1, precision="fp64", partition=None,
-                 ufactor=0, coarse_solve=0, weighted_partition=False):
+                 ufactor=0, coarse_solve=0, weighted_partition=False, drop_tol=1e-4,
+                 fill_factor=10):
         self.ctx = ctx
         self.kind = kind
         self._part = None if partition is None else np.ascontiguousarray(partition, np.int32)
         self.opts = PrecondOpts(nparts, overlap, 1 if precision == "fp32" else 0,
                                 None if self._part is None else
                                 self._part.ctypes.data_as(C.POINTER(C.c_int)),
-                                ufactor, coarse_solve, 1 if weighted_partition else 0)
+                                ufactor, coarse_solve, 1 if weighted_partition else 0,
+                                drop_tol, fill_factor)
         self.h = _VP()
         ctx.check(lib().sg_precond_create(ctx.h, PC_KIND[kind], C.byref(self.opts),
                                           C.byref(self.h)))
@@ -490,6 +495,12 @@ class Precond:
         idx = np.zeros(tot.value, np.int32)
         self.ctx.check(lib().sg_precond_subdomains(self.h, C.byref(tot), _p(ptr), _p(idx)))
         return [idx[ptr[s]:ptr[s + 1]] for s in range(nparts)]
+
+    def ilut_nnz(self):
+        """(l_nnz, u_nnz) of an ILUT preconditioner, as ILUTPrecond reports them."""
+        a, b = C.c_longlong(), C.c_longlong()
+        self.ctx.check(lib().sg_precond_ilut_nnz(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def ilu_values(self, s=0):
         nnz = C.c_int()

@@ -504,4 +504,117 @@ std::vector<int> upper_levels(const HostCsr& A) {
     return lev;
 }
 
+// ILUT(drop_tol, fill_factor).  Row i: scatter A's row into the dense work
+// row w (an index joins the list when w[j] is still 0.0), tolerance
+// drop_tol * max|w|; sort the list; eliminate with each listed k < i in
+// ascending order -- only the indices listed before elimination starts, as the
+// reference's range-for over the growing list visits (ilut.h:62) -- dropping
+// w[k] below the tolerance, else w[k] /= u_kk and w -= w[k] * (U row k right
+// of k), new fill joining the list; sort again; split into L (j < i, kept if
+// |w_j| >= tol) and U (j >= i, the diagonal always, others if >= tol), zeros
+// skipped; keep the max_fill largest of each by std::partial_sort on |value|
+// and re-sort by column (std::sort); u_ii is the U entry in column i, 1e-30 if
+// absent or smaller.
+void ilut_factor(const HostCsr& A, double drop_tol, int fill_factor, HostCsr& F,
+                 std::vector<int>& diag, long long& l_nnz, long long& u_nnz) {
+    using Ent = std::pair<int, double>;
+    const int n = A.n;
+    F = HostCsr{};
+    F.n = F.ncols = n;
+    F.rp.assign((size_t)n + 1, 0);
+    diag.assign(n, -1);
+    l_nnz = u_nnz = 0;
+    if (n == 0) return;
+    const int avg = std::max(1, A.nnz() / n);
+    const int max_fill = fill_factor * avg;
+    std::vector<std::vector<Ent>> Lr(n), Ur(n);
+    std::vector<double> udiag(n);
+    std::vector<double> w(n, 0.0);
+    std::vector<int> nz;
+    auto by_mag = [](const Ent& a, const Ent& b) { return std::abs(a.second) > std::abs(b.second); };
+    auto by_col = [](const Ent& a, const Ent& b) { return a.first < b.first; };
+    auto keep_largest = [&](std::vector<Ent>& part) {
+        if ((int)part.size() > max_fill) {
+            std::partial_sort(part.begin(), part.begin() + max_fill, part.end(), by_mag);
+            part.resize(max_fill);
+            std::sort(part.begin(), part.end(), by_col);
+        }
+    };
+    for (int i = 0; i < n; ++i) {
+        for (int p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            const int j = A.ci[p];
+            if (w[j] == 0.0) nz.push_back(j);
+            w[j] += A.v[p];
+        }
+        double rmax = 0.0;
+        for (int j : nz) rmax = std::max(rmax, std::abs(w[j]));
+        const double tol = drop_tol * rmax;
+        std::sort(nz.begin(), nz.end());
+        const size_t listed = nz.size();
+        for (size_t t = 0; t < listed; ++t) {
+            const int k = nz[t];
+            if (k >= i) break;
+            if (std::abs(w[k]) < tol) {
+                w[k] = 0.0;
+                continue;
+            }
+            const double ukk = udiag[k];
+            if (std::abs(ukk) < 1e-30) {
+                w[k] = 0.0;
+                continue;
+            }
+            const double lik = w[k] / ukk;
+            w[k] = lik;
+            for (const Ent& e : Ur[k]) {
+                if (e.first <= k) continue;
+                if (w[e.first] == 0.0) nz.push_back(e.first);
+                w[e.first] -= lik * e.second;
+            }
+        }
+        std::sort(nz.begin(), nz.end());
+        std::vector<Ent> lp, up;
+        for (int j : nz) {
+            if (std::abs(w[j]) == 0.0) continue;
+            if (j < i) {
+                if (std::abs(w[j]) >= tol) lp.emplace_back(j, w[j]);
+            } else if (j == i || std::abs(w[j]) >= tol) {
+                up.emplace_back(j, w[j]);
+            }
+        }
+        keep_largest(lp);
+        keep_largest(up);
+        double d = 0.0;
+        for (const Ent& e : up)
+            if (e.first == i) {
+                d = e.second;
+                break;
+            }
+        if (std::abs(d) < 1e-30) d = 1e-30;
+        udiag[i] = d;
+        Lr[i] = std::move(lp);
+        Ur[i] = std::move(up);
+        for (int j : nz) w[j] = 0.0;
+        nz.clear();
+    }
+    for (int i = 0; i < n; ++i) {
+        l_nnz += (long long)Lr[i].size();
+        u_nnz += (long long)Ur[i].size();
+        for (const Ent& e : Lr[i]) {
+            F.ci.push_back(e.first);
+            F.v.push_back(e.second);
+        }
+        diag[i] = (int)F.ci.size();
+        F.ci.push_back(i);
+        F.v.push_back(udiag[i]);
+        for (size_t t = 1; t < Ur[i].size(); ++t) {
+            if (Ur[i][t].first <= i)
+                throw std::runtime_error("ILUT: repeated diagonal entry in a U row");
+            F.ci.push_back(Ur[i][t].first);
+            F.v.push_back(Ur[i][t].second);
+        }
+        if (F.ci.size() > (size_t)INT32_MAX) throw std::runtime_error("ILUT factors exceed 32-bit indexing");
+        F.rp[i + 1] = (int)F.ci.size();
+    }
+}
+
 }  // namespace sg

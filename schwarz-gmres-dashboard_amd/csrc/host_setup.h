@@ -50,6 +50,17 @@ std::vector<int> lower_levels(const HostCsr& A);
 // Level of each row for the backward (U) sweep (dependencies on columns > row).
 std::vector<int> upper_levels(const HostCsr& A);
 
+// Dual-threshold incomplete LU as the reference's ILUTPrecond::setup
+// (src/precond/ilut.h:28-149), returned as one factor matrix F whose row i is
+// [L row i (unit-lower part, ascending columns)] [(i, u_ii)] [U row i without
+// its first entry] with diag[i] the position of (i, u_ii): the reference's
+// apply (ilut.h:152-172) is then a forward sweep over the entries left of
+// diag[i] and a backward sweep over those right of it, divided by F[diag[i]]
+// -- the shape the ILU(0) sweeps consume.  l_nnz / u_nnz are the sizes of the
+// reference's L and U arrays.
+void ilut_factor(const HostCsr& A, double drop_tol, int fill_factor, HostCsr& F,
+                 std::vector<int>& diag, long long& l_nnz, long long& u_nnz);
+
 // Matrix Market coordinate files (matrix_market.cpp); flags are SG_MM_*.
 void mm_read(const std::string& path, int flags, HostCsr& A);
 void mm_write(const std::string& path, int nrows, int ncols, const int* rp, const int* ci,

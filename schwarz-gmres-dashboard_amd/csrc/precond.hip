@@ -112,7 +112,8 @@ struct sg_precond {
     int nsub = 0;
     bool z_in_lds = false;
     size_t lds_bytes = 0;      // dynamic LDS of the sweep with z in LDS
-    size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned)
+    size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned); 0 when in global memory
+    bool tab_in_lds = true;
     int sweep_variant = 1;     // index into kSweepShapes
     bool sweep_tuned = false;
     // sync-free sweep: LDS = level tables | item prefixes | z | row flags
@@ -128,6 +129,7 @@ struct sg_precond {
     sg::DevBuf<int> d_dofs, d_own, d_own_gid;
     sg::DevBuf<int> lrp, lci, ldiag;
     sg::DevBuf<double> lval;  // factored FP64 values in local CSR order
+    long long ilut_l_nnz = 0, ilut_u_nnz = 0;  // sizes of the reference's L / U arrays
     sg::DevBuf<double> f_v64, b_v64, b_d64;
     sg::DevBuf<float> f_v32, b_v32, b_d32;
     sg::DevBuf<double> z64;   // global z scratch when z does not fit in LDS
@@ -436,11 +438,22 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
-    // LDS: this subdomain's level tables, then (ZLDS) the solution vector
-    Level* lf = reinterpret_cast<Level*>(lds_raw);
-    Level* lb = lf + si.nflev;
-    for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) lf[l] = flev[si.flev0 + l];
-    for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) lb[l] = blev[si.blev0 + l];
+    // LDS: this subdomain's level tables, then (ZLDS) the solution vector.
+    // lds_z_off == 0: the tables are too long for LDS and are read from
+    // global memory (uniform loads, two levels ahead).
+    const Level* lf;
+    const Level* lb;
+    if (lds_z_off > 0) {
+        Level* tf = reinterpret_cast<Level*>(lds_raw);
+        Level* tb = tf + si.nflev;
+        for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) tf[l] = flev[si.flev0 + l];
+        for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) tb[l] = blev[si.blev0 + l];
+        lf = tf;
+        lb = tb;
+    } else {
+        lf = flev + si.flev0;
+        lb = blev + si.blev0;
+    }
     T* z = ZLDS ? reinterpret_cast<T*>(lds_raw + lds_z_off) : zglob + si.zoff;
     const int* d = dofs + si.row0;
     // R_s x: four independent index/value loads in flight per thread
@@ -890,27 +903,35 @@ static void setup_jacobi(sg_precond* M, const sg_csr* A) {
     M->inv_diag.upload(inv, M->ctx->stream);
 }
 
-static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
+// pre: an already factored global matrix (ILUT) with its diagonal positions;
+// its sweeps are built the same way and the device factorisation is skipped.
+static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
+                            const HostCsr* pre = nullptr, const std::vector<int>* pre_diag = nullptr) {
     sg_context* ctx = M->ctx;
     hipStream_t st = ctx->stream;
     const HostCsr& H = A->host;
     const int nsub = M->nparts;
-    const bool global_ilu = (M->kind == SG_PC_ILU0);
+    const bool global_ilu = (M->kind == SG_PC_ILU0 || pre);
     std::vector<HostCsr> local(nsub);
     std::vector<std::vector<int>> diag(nsub), own(nsub);
     std::vector<SweepPack> fw(nsub), bw(nsub);
 
     run_parallel(nsub, [&](int s) {
         const std::vector<int>& d = M->dofs[s];
-        local[s] = global_ilu ? H : extract_submatrix(H, d);
         HostCsr& L = local[s];
-        for (int i = 0; i < L.n; ++i)
-            for (int k = L.rp[i] + 1; k < L.rp[i + 1]; ++k)
-                if (L.ci[k] <= L.ci[k - 1])
-                    throw std::runtime_error("ILU(0) needs strictly ascending column indices");
-        diag[s] = diag_positions(L);
-        for (int i = 0; i < L.n; ++i)
-            if (diag[s][i] < 0) throw std::runtime_error("ILU(0) needs a diagonal entry in every row");
+        if (pre) {
+            L = *pre;
+            diag[s] = *pre_diag;
+        } else {
+            L = global_ilu ? H : extract_submatrix(H, d);
+            for (int i = 0; i < L.n; ++i)
+                for (int k = L.rp[i] + 1; k < L.rp[i + 1]; ++k)
+                    if (L.ci[k] <= L.ci[k - 1])
+                        throw std::runtime_error("ILU(0) needs strictly ascending column indices");
+            diag[s] = diag_positions(L);
+            for (int i = 0; i < L.n; ++i)
+                if (diag[s][i] < 0) throw std::runtime_error("ILU(0) needs a diagonal entry in every row");
+        }
         // owned rows in local numbering (src/precond/ras.h:34-41)
         for (int i = 0; i < L.n; ++i)
             if (global_ilu || M->part[d[i]] == s) own[s].push_back(i);
@@ -1023,9 +1044,11 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     M->lval.upload(pl.lval, st);
 
     // device factorisation in FP64
-    hipLaunchKernelGGL(k_ilu0_factor, dim3(nsub), dim3(kTrsvThreads), 0, st, M->d_subs.p,
-                       M->flev.p, M->f_row.p, M->lrp.p, M->lci.p, M->ldiag.p, M->lval.p);
-    SG_HIP(hipGetLastError());
+    if (!pre) {
+        hipLaunchKernelGGL(k_ilu0_factor, dim3(nsub), dim3(kTrsvThreads), 0, st, M->d_subs.p,
+                           M->flev.p, M->f_row.p, M->lrp.p, M->lci.p, M->ldiag.p, M->lval.p);
+        SG_HIP(hipGetLastError());
+    }
     // pack values for the sweeps
     const bool fp32 = M->opts.precision == SG_FP32;
     int nf = (int)pl.f_src.size(), nb = (int)pl.b_src.size(), nd = (int)pl.b_dsrc.size();
@@ -1052,8 +1075,9 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
     int maxlev = 0;
     for (const SubInfo& si : pl.subs) maxlev = std::max(maxlev, si.nflev + si.nblev);
     M->lds_tab_bytes = ((size_t)maxlev * sizeof(Level) + 15) / 16 * 16;
-    if (M->lds_tab_bytes > (size_t)ctx->max_lds)
-        fail(SG_E_ARG, "too many ILU levels in one subdomain for the LDS level table");
+    // level tables that would take more than half the LDS stay in global memory
+    M->tab_in_lds = M->lds_tab_bytes <= (size_t)ctx->max_lds / 2;
+    if (!M->tab_in_lds) M->lds_tab_bytes = 0;
     size_t zbytes = (size_t)(pl.max_ns + 1) * (fp32 ? 4 : 8);
     M->z_in_lds = M->lds_tab_bytes + zbytes <= (size_t)ctx->max_lds;
     M->lds_bytes = M->lds_tab_bytes + zbytes;
@@ -1067,7 +1091,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive) {
         M->sf_z_off = M->sf_item_off + al((size_t)maxlev * sizeof(int));
         M->sf_flag_off = M->sf_z_off + al(zbytes);
         M->sf_bytes = M->sf_flag_off + al((size_t)pl.max_ns + 1);
-        M->sf_ok = M->sf_bytes <= (size_t)ctx->max_lds;
+        M->sf_ok = M->tab_in_lds && M->sf_bytes <= (size_t)ctx->max_lds;
     }
     if (additive) {
         if (fp32) M->zout32.alloc((size_t)pl.total_rows + 1);
@@ -1231,6 +1255,25 @@ static void setup(sg_precond* M, const sg_csr* A) {
             M->dofs[0] = M->owned[0];
             setup_ilu_batch(M, A, false);
             break;
+        case SG_PC_ILUT: {
+            // host factorisation, sequential over rows as the reference's
+            // (src/precond/ilut.h:28-149); device sweeps for the apply
+            if (!(M->opts.drop_tol >= 0.0) || M->opts.fill_factor < 1)
+                fail(SG_E_ARG, "ILUT needs drop_tol >= 0 and fill_factor >= 1");
+            M->nparts = 1;
+            M->part.assign(M->n, 0);
+            M->owned.assign(1, {});
+            M->dofs.assign(1, {});
+            M->owned[0].resize(M->n);
+            std::iota(M->owned[0].begin(), M->owned[0].end(), 0);
+            M->dofs[0] = M->owned[0];
+            HostCsr F;
+            std::vector<int> fd;
+            ilut_factor(A->host, M->opts.drop_tol, M->opts.fill_factor, F, fd, M->ilut_l_nnz,
+                        M->ilut_u_nnz);
+            setup_ilu_batch(M, A, false, &F, &fd);
+            break;
+        }
         case SG_PC_ASM:
         case SG_PC_RAS:
         case SG_PC_TWOLEVEL_ASM:
@@ -1511,7 +1554,8 @@ extern "C" {
 
 sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_precond_opts* opts,
                             sg_precond** out) {
-    if (!ctx || !out || (int)kind < 0 || (int)kind > SG_PC_TWOLEVEL_RAS) return SG_E_ARG;
+    if (!ctx || !out || (int)kind < 0 || (int)kind > SG_PC_ILUT || kind == SG_PC_LEARNED)
+        return SG_E_ARG;
     sg_precond* M = new sg_precond();
     M->ctx = ctx;
     M->kind = kind;
@@ -1519,6 +1563,10 @@ sg_status sg_precond_create(sg_context* ctx, sg_precond_kind kind, const sg_prec
     else {
         M->opts.nparts = 4;
         M->opts.overlap = 1;
+    }
+    if (kind == SG_PC_ILUT && M->opts.fill_factor == 0 && M->opts.drop_tol == 0.0) {
+        M->opts.drop_tol = 1e-4;  // ILUTPrecond() defaults (src/precond/ilut.h:25)
+        M->opts.fill_factor = 10;
     }
     if (M->opts.precision != SG_FP64 && M->opts.precision != SG_FP32) {
         delete M;
@@ -1639,6 +1687,14 @@ sg_status sg_precond_ilu_values(const sg_precond* M, int s, int* nnz, double* va
             SG_HIP(hipStreamSynchronize(M->ctx->stream));
         }
     });
+}
+
+sg_status sg_precond_ilut_nnz(const sg_precond* M, long long* l_nnz, long long* u_nnz) {
+    if (!M || M->kind != SG_PC_ILUT) return SG_E_ARG;
+    if (!M->ready) return SG_E_STATE;
+    if (l_nnz) *l_nnz = M->ilut_l_nnz;
+    if (u_nnz) *u_nnz = M->ilut_u_nnz;
+    return SG_OK;
 }
 
 sg_status sg_partition(int n, const int* row_ptr, const int* col_idx, const double* vals,

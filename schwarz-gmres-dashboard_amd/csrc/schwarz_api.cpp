@@ -270,6 +270,18 @@ std::vector<int> DevicePreconditioner::partition() const {
     return p;
 }
 
+int ILUTPrecond::l_nnz() const {
+    long long l = 0;
+    if (M_) check(sg_precond_ilut_nnz(M_, &l, nullptr), default_context());
+    return (int)l;
+}
+
+int ILUTPrecond::u_nnz() const {
+    long long u = 0;
+    if (M_) check(sg_precond_ilut_nnz(M_, nullptr, &u), default_context());
+    return (int)u;
+}
+
 std::vector<double> ILU0Precond::lu_vals() const {
     int nnz = 0;
     check(sg_precond_ilu_values(M_, 0, &nnz, nullptr), default_context());

@@ -38,7 +38,9 @@ void usage() {
            "  --poisson3d <N>          Poisson 3D, N^3 grid\n"
            "  --jump3d <N>             3D finite-volume diffusion, kappa = 10^{-3,0,3} on 16^3 blocks\n"
            "\nPreconditioner:\n"
-           "  --precond <type>         none|jacobi|ilu0|asm|ras|twolevel_asm|twolevel_ras\n"
+           "  --precond <type>         none|jacobi|ilu0|ilut|asm|ras|twolevel_asm|twolevel_ras\n"
+           "  --drop-tol <t>           ILUT drop tolerance (default 1e-4)\n"
+           "  --fill-factor <f>        ILUT fill factor (default 10)\n"
            "  --precision <p>          fp64 (default) | fp32 (factors and applies)\n"
            "\nSolver:\n"
            "  --restart <m>            restart length (default 30)\n"
@@ -148,6 +150,8 @@ int main(int argc, char** argv) {
     int overlap = 1, nparts = 4, poisson = 0, poisson3d = 0, jump3d = 0, ufactor = 0;
     bool use_fgmres = false, use_bicgstab = false, weighted = false, coarse_inv = false;
     bool analyze = false;
+    double drop_tol = 1e-4;  // src/main.cpp:81-82
+    int fill_factor = 10;
     MatrixMarketOptions mm;
     std::string matrix_file;
     uint64_t seed = 1;
@@ -185,10 +189,9 @@ int main(int argc, char** argv) {
         else if (is("--gpu")) {}
         else if (is("--weighted-partition")) weighted = true;
         else if (is("--coarse-inverse")) coarse_inv = true;
-        else if (val("--drop-tol") || val("--fill-factor")) {
-            std::cerr << "schwarz_solve: " << argv[i] << " (ILUT) is not part of this build\n";
-            return 2;
-        } else {
+        else if (val("--drop-tol")) drop_tol = std::atof(argv[++i]);
+        else if (val("--fill-factor")) fill_factor = std::atoi(argv[++i]);
+        else {
             usage();
             return 1;
         }
@@ -244,6 +247,7 @@ int main(int argc, char** argv) {
             else if (precond == "ras") kind = SG_PC_RAS;
             else if (precond == "twolevel_asm") kind = SG_PC_TWOLEVEL_ASM;
             else if (precond == "twolevel_ras") kind = SG_PC_TWOLEVEL_RAS;
+            else if (precond == "ilut") throw std::runtime_error("--batch does not take ILUT");
             std::vector<CSRMatrix> sys(batch, A);
             std::vector<std::vector<double>> bs, xs(batch);
             for (int q = 0; q < batch; ++q) bs.push_back(make_rhs(seed + q));
@@ -302,10 +306,7 @@ int main(int argc, char** argv) {
             M.reset(D = new TwoLevelSchwarzPrecond(nparts, overlap, FineLevelType::ASM));
         else if (precond == "twolevel_ras")
             M.reset(D = new TwoLevelSchwarzPrecond(nparts, overlap, FineLevelType::RAS));
-        else if (precond == "ilut") {
-            std::cerr << "schwarz_solve: ILUT is not part of this build\n";
-            return 2;
-        }
+        else if (precond == "ilut") M.reset(D = new ILUTPrecond(drop_tol, fill_factor));
         if (D) {
             D->set_precision(precision == "fp32" ? SG_FP32 : SG_FP64);
             D->set_ufactor(ufactor);
@@ -320,6 +321,8 @@ int main(int argc, char** argv) {
                 D->set_partition(std::move(p));
             }
             D->setup(A);
+            if (precond == "ilut")  // src/main.cpp:194-197
+                std::cout << "  ILUT: drop_tol=" << drop_tol << " fill_factor=" << fill_factor << "\n";
         }
         auto t1 = std::chrono::steady_clock::now();
         const double t_setup = std::chrono::duration<double>(t1 - t0).count();

@@ -8,7 +8,7 @@ small JSON fixtures.  Subdomain index sets for the RAS/ASM fixtures are built
 here by a brute-force breadth-first search over the dense adjacency of the
 grid, independent of the oracle.
 
-Usage:  python tests/golden/make_golden.py [bicgstab analysis conv mm]
+Usage:  python tests/golden/make_golden.py [bicgstab analysis conv mm ilut]
         (needs /root/reference; with names, only those fixtures are rewritten)
 """
 import json
@@ -206,11 +206,35 @@ def gen_mm():
     dump("mm.json", {"reads": reads, "writes": writes})
 
 
+def gen_ilut():
+    """ILUTPrecond (src/precond/ilut.h): factor sizes, one apply, GMRES / BiCGSTAB counts."""
+    applies = []
+    # every case keeps each row's index list within the 2 * fill * (nnz / n)
+    # entries the reference reserves: beyond that its elimination loop reads a
+    # reallocated vector (undefined; rnd:150:8 with drop 5e-2, fill 1 crashes)
+    for spec, drop, fill in (("2d:12:12", 1e-4, 10), ("2d:20:20", 1e-2, 2), ("3d:6:6:6", 1e-3, 5),
+                             ("rnd:150:8", 1e-4, 10), ("rnd:150:8", 5e-2, 3), ("2d:30:30", 0.0, 1),
+                             ("3d:8:8:8", 1e-4, 10), ("2d:30:30", 0.0, 3)):
+        r = run("ilut", spec, drop, fill)
+        r.update(dict(grid=spec, drop_tol=drop, fill_factor=fill))
+        applies.append(r)
+    solves = []
+    for c in (("gmres", "ilut", "2d:60:60", "ones", 30, 1000, 1e-10),
+              ("fgmres", "ilut", "3d:12:12:12", "uniform:2", 20, 1000, 1e-10),
+              ("bicgstab", "ilut", "2d:60:60", "uniform:3", 0, 1000, 1e-10)):
+        r = run("solve", *c)
+        r.update(dict(method=c[0], pc=c[1], grid=c[2], rhs=c[3], restart=c[4], max_iter=c[5],
+                      tol_arg=c[6]))
+        solves.append(r)
+    dump("ilut.json", {"applies": applies, "solves": solves})
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -C oracle ref")
     only = set(sys.argv[1:])
-    extra = {"bicgstab": gen_bicgstab, "analysis": gen_analysis, "conv": gen_conv, "mm": gen_mm}
+    extra = {"bicgstab": gen_bicgstab, "analysis": gen_analysis, "conv": gen_conv, "mm": gen_mm,
+             "ilut": gen_ilut}
     if only:
         for k in sorted(only):
             extra[k]()

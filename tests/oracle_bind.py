@@ -60,6 +60,11 @@ def lib():
         L.or_bicgstab.argtypes = [C.c_int, _I, _I, _D, _D, _D, C.c_void_p, C.c_int, C.c_double,
                                   C.POINTER(C.c_double), C.POINTER(C.c_int), _D, C.c_int,
                                   C.POINTER(C.c_int)]
+        L.or_ilut_new.argtypes = [C.c_int, _I, _I, _D, C.c_double, C.c_int, C.c_int,
+                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.or_ilut_new.restype = C.c_void_p
+        L.or_ilut_max_list.argtypes = [C.c_void_p]
+        L.or_ilut_max_list.restype = C.c_longlong
         L.or_lanczos.argtypes = [C.c_int, _I, _I, _D, C.c_void_p, C.c_int, C.c_uint, _D, _D]
         L.or_tridiag_eigs.argtypes = [C.c_int, _D, _D, _D]
         L.or_condition.argtypes = [C.c_int, _I, _I, _D, C.c_void_p, C.c_int, C.c_uint, _D,
@@ -152,8 +157,18 @@ def overlap_sets(A, part, nparts, overlap):
 
 
 class Precond:
-    def __init__(self, kind, A, nparts=1, overlap=0, part=None, precision=0):
+    def __init__(self, kind, A, nparts=1, overlap=0, part=None, precision=0, drop_tol=1e-4,
+                 fill_factor=10):
         self.A = A
+        if kind == "ilut":
+            ln, un = C.c_longlong(), C.c_longlong()
+            self.h = lib().or_ilut_new(A.n, A.rp, A.ci, A.v, drop_tol, fill_factor, precision,
+                                       C.byref(ln), C.byref(un))
+            self.l_nnz, self.u_nnz = ln.value, un.value
+            self.max_list = lib().or_ilut_max_list(self.h)
+            self.nparts = 1
+            self._part = None
+            return
         self._part = None if part is None else np.ascontiguousarray(part, np.int32)
         self.h = lib().or_precond_new(PC[kind], A.n, A.rp, A.ci, A.v, nparts, overlap,
                                       None if self._part is None else self._part.ctypes.data,

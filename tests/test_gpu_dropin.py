@@ -45,8 +45,7 @@ def test_cli_exit_codes(tmp_path):
     cli = os.path.join(PKG, "schwarz_solve")
     assert subprocess.run([cli], capture_output=True).returncode == 1
     assert subprocess.run([cli, "--bogus"], capture_output=True).returncode == 1
-    assert subprocess.run([cli, "--poisson", "8", "--drop-tol", "1e-3"],
-                          capture_output=True).returncode == 2  # ILUT
+    assert subprocess.run([cli, "--poisson"], capture_output=True).returncode == 1
     # not converged -> 1
     r = subprocess.run([cli, "--poisson", "64", "--precond", "none", "--maxiter", "5",
                         "--restart", "5"], capture_output=True, text=True, timeout=120)
@@ -158,3 +157,17 @@ def test_cli_matrix_market_and_analyze(tmp_path):
                          text=True, timeout=60)
     assert bad.returncode == 3 and "Cannot open file" in bad.stderr
     np.testing.assert_equal(rec["iterations"] > 0, True)
+
+
+def test_cli_ilut_matches_reference():
+    """--precond ilut --drop-tol --fill-factor (src/main.cpp:194-197): the
+    reference's GMRES(30) + ILUT() count on Poisson 60^2 (golden)."""
+    from conftest import golden
+    ref = next(c for c in golden("ilut.json")["solves"] if c["method"] == "gmres")
+    out = subprocess.run([os.path.join(PKG, "schwarz_solve"), "--poisson", "60", "--precond",
+                          "ilut", "--drop-tol", "1e-4", "--fill-factor", "10"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "ILUT: drop_tol=0.0001 fill_factor=10" in out.stdout
+    lines = dict(l.strip().split(": ", 1) for l in out.stdout.splitlines() if ": " in l)
+    assert abs(int(lines["Iterations"]) - ref["iterations"]) <= 1

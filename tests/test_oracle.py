@@ -399,3 +399,45 @@ def test_bicgstab_count_depends_on_summation_order():
     assert seq == 67 and pair != seq and ext != seq
     band = max(2, round(0.12 * seq))
     assert abs(pair - seq) <= band and abs(ext - seq) <= band
+
+
+# ---- ILUT ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", __import__("conftest").golden("ilut.json")["applies"],
+                         ids=lambda c: f"{c['grid']}-{c['drop_tol']}-{c['fill_factor']}")
+def test_ilut_apply_bitwise_reference(case):
+    A = O.grid(case["grid"])
+    P = O.Precond("ilut", A, drop_tol=case["drop_tol"], fill_factor=case["fill_factor"])
+    assert (P.l_nnz, P.u_nnz) == (case["l_nnz"], case["u_nnz"])
+    assert np.array_equal(P.apply(np.array(case["x"])), np.array(case["y"]))
+
+
+@pytest.mark.parametrize("case", __import__("conftest").golden("ilut.json")["solves"],
+                         ids=lambda c: c["method"])
+def test_ilut_solves_bitwise_reference(case):
+    A = O.grid(case["grid"])
+    b = _rhs(A, case["rhs"])
+    P = O.Precond("ilut", A)
+    if case["method"] == "bicgstab":
+        r = O.bicgstab(A, b, P, case["max_iter"], case["tol_arg"])
+    else:
+        r = O.solve(case["method"], A, b, P, case["restart"], case["max_iter"], case["tol_arg"])
+    assert r["iterations"] == case["iterations"]
+    assert np.array_equal(r["hist"], np.array(case["hist"]))
+
+
+def test_ilut_exact_on_tridiagonal():
+    """No dropping and enough fill: ILUT of a tridiagonal matrix is its exact LU."""
+    n = 40
+    rng = np.random.default_rng(8)
+    d = 4 + rng.random(n)
+    e = rng.standard_normal(n - 1)
+    f = rng.standard_normal(n - 1)
+    T = np.diag(d) + np.diag(e, -1) + np.diag(f, 1)
+    rows, cols = np.nonzero(T)
+    rp = np.searchsorted(rows, np.arange(n + 1))
+    A = O.Csr(rp, cols, T[rows, cols])
+    P = O.Precond("ilut", A, drop_tol=0.0, fill_factor=5)
+    x = rng.standard_normal(n)
+    assert np.allclose(P.apply(x), np.linalg.solve(T, x), rtol=1e-12, atol=1e-12)
+    assert P.l_nnz == n - 1 and P.u_nnz == 2 * n - 1
