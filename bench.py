@@ -336,6 +336,171 @@ def run_learned(args, sg, P, cfg):
     print(json.dumps(out))
 
 
+def _timed(fn, k):
+    out, ts = None, []
+    for _ in range(k):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return out, float(np.mean(ts))
+
+
+def _families(sg, ctx, fn):
+    ctx.profiling(True)
+    ctx.reset_stats()
+    fn()
+    fams = {f: ctx.kernel_stats(f) for f in ("spmv", "trsv", "precond", "ortho", "update")}
+    ctx.profiling(False)
+    return {f: [int(a), round(float(b), 3)] for f, (a, b) in fams.items()}
+
+
+def run_rows(args, sg, P):
+    """The rows around the headline path -- BiCGSTAB, ILUT, the convergence
+    analytics and the Matrix Market reader -- each on the GPU with the CPU
+    restatement of the reference (or the reference's own reader) timed beside
+    it.  One JSON line per row; times are seconds per call with inputs in HBM."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as O
+    ctx = sg.Context(0)
+    steps = max(1, args.steps)
+
+    def emit(d):
+        print(json.dumps(d), flush=True)
+
+    # ---- c1 matrix: Poisson 256^2, b = A x*, x* ~ U[-1, 1] ----
+    rp, ci, v = P.poisson2d(256, 256)
+    n = len(rp) - 1
+    b = spmv_np(rp, ci, v, P.uniform_pm1(1, n))
+    A = sg.Csr(ctx, rp, ci, v)
+    Ao = O.Csr(rp, ci, v)
+    db, dx = sg.DeviceArray.from_host(ctx, b), sg.DeviceArray(ctx, n)
+    M = sg.Precond(ctx, "ilu0").setup(A)
+    Mo = O.Precond("ilu0", Ao)
+
+    def gsolve(MM, **kw):
+        def f():
+            dx.zero()
+            ctx.sync()
+            r = sg.solve(A, db, dx, MM, **kw)
+            ctx.sync()
+            return r
+        return f
+
+    # BiCGSTAB + ILU(0)
+    kw = dict(method="bicgstab", max_iter=2000, tol=1e-10)
+    gsolve(M, **kw)()
+    r, t = _timed(gsolve(M, **kw), steps)
+    fams = _families(sg, ctx, gsolve(M, **kw))
+    ro, tc = _timed(lambda: O.bicgstab(Ao, b, Mo, 2000, 1e-10), 1)
+    vec_ms = fams["update"][1] + fams["ortho"][1]
+    vec_bytes = 152.0 * n * r["iterations"]  # 19 vector passes of 8 bytes per iteration
+    emit({"row": "bicgstab", "workload": "poisson2d_256^2", "precond": "ilu0", "tol": 1e-10,
+          "gpu_s": round(t, 5), "iterations": r["iterations"],
+          "rel_true_residual": r["rel_true_residual"],
+          "cpu_s": round(tc, 4), "cpu_iterations": ro["iterations"], "cpu_kind": "port",
+          "cpu_cores": 1, "speedup": round(tc / t, 2), "kernel_ms": fams,
+          "vector_kernels_GB_s": round(vec_bytes / (vec_ms * 1e-3) / 1e9, 1) if vec_ms else None})
+
+    # GMRES(30) + ILUT(1e-4, 10)
+    t0 = time.perf_counter()
+    Mt = sg.Precond(ctx, "ilut", drop_tol=1e-4, fill_factor=10).setup(A)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Mto = O.Precond("ilut", Ao, drop_tol=1e-4, fill_factor=10)
+    t_setup_cpu = time.perf_counter() - t0
+    kw = dict(method="gmres", restart=30, max_iter=2000, tol=1e-10, ortho="mgs")
+    gsolve(Mt, **kw)()
+    r, t = _timed(gsolve(Mt, **kw), steps)
+    fams = _families(sg, ctx, gsolve(Mt, **kw))
+    ro, tc = _timed(lambda: O.solve("gmres", Ao, b, Mto, 30, 2000, 1e-10), 1)
+    info = Mt.stats()
+    emit({"row": "ilut", "workload": "poisson2d_256^2", "solver": "gmres(30)", "tol": 1e-10,
+          "l_nnz_u_nnz": list(Mt.ilut_nnz()), "levels": [info["levels_fwd"], info["levels_bwd"]],
+          "setup_s": round(t_setup, 3), "cpu_setup_s": round(t_setup_cpu, 3),
+          "gpu_s": round(t, 5), "iterations": r["iterations"],
+          "rel_true_residual": r["rel_true_residual"], "cpu_s": round(tc, 4),
+          "cpu_iterations": ro["iterations"], "cpu_kind": "port", "cpu_cores": 1,
+          "speedup": round(tc / t, 2), "kernel_ms": fams})
+
+    # condition estimate (80 Lanczos steps) and spectral radius (300 power steps)
+    def gan():
+        c = sg.estimate_condition(A, M, 80)
+        s = sg.estimate_spectral_radius(A, M, 300)
+        ctx.sync()
+        return c, s
+    gan()
+    (c, sr), t = _timed(gan, steps)
+    (co, so), tc = _timed(lambda: (O.condition(Ao, Mo, 80), O.spectral(Ao, Mo, 300)), 1)
+    emit({"row": "analysis", "workload": "poisson2d_256^2", "precond": "ilu0",
+          "gpu_s": round(t, 5), "lanczos_steps": c["lanczos_steps"], "kappa": c["kappa"],
+          "power_iters": sr["power_iters"], "rho": sr["spectral_radius"],
+          "cpu_s": round(tc, 4), "cpu_kappa": co["kappa"], "cpu_rho": so["rho"],
+          "cpu_power_iters": so["power_iters"], "cpu_kind": "port", "cpu_cores": 1,
+          "speedup": round(tc / t, 2)})
+    del M, Mt, A, db, dx
+
+    # ---- c3 matrix: BiCGSTAB with the FP64 two-level RAS (with the FP32 one,
+    # whose rounding makes the operator slightly nonlinear, BiCGSTAB stagnates
+    # near 1e-3 on this 10^6-contrast problem) ----
+    cfg = CONFIGS["c3"]
+    rp, ci, v, b, _ = make_problem(P, "c3")
+    n = len(rp) - 1
+    A = sg.Csr(ctx, rp, ci, v)
+    db, dx = sg.DeviceArray.from_host(ctx, b), sg.DeviceArray(ctx, n)
+    part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v, weighted=True)
+    M = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                   precision="fp64", partition=part, coarse_solve=1).setup(A)
+    kw = dict(method="bicgstab", max_iter=5000, tol=1e-8)  # relative to ||r0|| = ||b||
+    gsolve(M, **kw)()
+    r, t = _timed(gsolve(M, **kw), steps)
+    fams = _families(sg, ctx, gsolve(M, **kw))
+    cpu = None
+    if not args.no_cpu:
+        Ao = O.Csr(rp, ci, v)
+        Mo = O.Precond("twolevel_ras", Ao, cfg["nparts"], cfg["overlap"], part, precision=0)
+        k = 8
+        t0 = time.perf_counter()
+        ro = O.bicgstab(Ao, b, Mo, k, 0.0)
+        per = (time.perf_counter() - t0) / max(1, ro["iterations"])
+        cpu = {"value": round(per * r["iterations"], 2), "unit": "s", "kind": "port",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+               "sample": f"{ro['iterations']} BiCGSTAB iterations of the oracle's FP64 two-level "
+                         f"RAS ({per * 1e3:.0f} ms/iteration), scaled to {r['iterations']}"}
+    emit({"row": "bicgstab", "workload": cfg["workload"], "precond": "twolevel_ras fp64",
+          "tol": 1e-8, "gpu_s": round(t, 4), "iterations": r["iterations"],
+          "rel_true_residual": r["rel_true_residual"], "kernel_ms": fams, "cpu_baseline": cpu})
+    (c, sr), t = _timed(lambda: (sg.estimate_condition(A, M, 80),
+                                 sg.estimate_spectral_radius(A, M, 300)), 1)
+    emit({"row": "analysis", "workload": cfg["workload"], "precond": "twolevel_ras fp64",
+          "gpu_s": round(t, 4), "lanczos_steps": c["lanczos_steps"], "kappa": c["kappa"],
+          "lambda_min": c["lambda_min"], "lambda_max": c["lambda_max"],
+          "power_iters": sr["power_iters"], "rho": sr["spectral_radius"]})
+    del M
+
+    # ---- Matrix Market: the c3 matrix as a 17-digit coordinate file ----
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "c3.mtx")
+        sg.write_matrix_market(path, rp, ci, v, digits=17)
+        size = os.path.getsize(path)
+        out, t = _timed(lambda: sg.read_matrix_market(path), 1)
+        assert np.array_equal(out[3], ci) and np.array_equal(out[4], v)
+        ref = None
+        harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+        if os.path.exists(harness):
+            import subprocess
+            q = subprocess.run([harness, "mmtime", path], capture_output=True, text=True,
+                               timeout=600)
+            if q.returncode == 0:
+                ref = json.loads(q.stdout)
+        emit({"row": "matrix_market", "workload": cfg["workload"], "file_MB": round(size / 1e6, 1),
+              "read_s": round(t, 3), "MB_s": round(size / 1e6 / t, 1),
+              "reference_read_s": ref["seconds"] if ref else None,
+              "reference_kind": "reference (read_matrix_market, 1 core)" if ref else None,
+              "speedup": round(ref["seconds"] / t, 2) if ref else None})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +520,9 @@ def main():
                     help="c5: network parameters (.npy) of the learned stage")
     ap.add_argument("--min-gain", type=float, default=0.01)
     ap.add_argument("--every", type=int, default=1)
+    ap.add_argument("--rows", action="store_true",
+                    help="measure the rows around the headline path (BiCGSTAB, ILUT, "
+                         "analytics, Matrix Market reader), one JSON line each, and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -370,6 +538,9 @@ def main():
 
     sg = __graft_entry__.load_package()
     P = load_problems()
+    if args.rows:
+        run_rows(args, sg, P)
+        return
     cfg = CONFIGS[args.config]
     if args.config == "c4":
         run_batch(args, sg, P, cfg)

@@ -27,6 +27,7 @@
 #include "analysis/convergence_analysis.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -287,6 +288,18 @@ int cmd_conv(int argc, char** argv) {
     return 0;
 }
 
+// mmtime FILE: read_matrix_market wall time (s), sizes and a checksum only.
+int cmd_mmtime(const char* path) {
+    auto t0 = std::chrono::steady_clock::now();
+    CSRMatrix A = read_matrix_market(path);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double sum = 0.0;
+    for (double v : A.vals) sum += v;
+    std::printf("{\"seconds\":%.6f,\"nrows\":%d,\"nnz\":%d,\"sum\":%.17g}\n", s, A.nrows,
+                A.nnz(), sum);
+    return 0;
+}
+
 // mm FILE: read_matrix_market, printed as JSON (or the exception message).
 int cmd_mm(const char* path) {
     CSRMatrix A;
@@ -350,6 +363,7 @@ int main(int argc, char** argv) {
     if (c == "ilut" && argc >= 5) return cmd_ilut(argv[2], std::atof(argv[3]), std::atoi(argv[4]));
     if (c == "conv") return cmd_conv(argc, argv);
     if (c == "mm" && argc >= 3) return cmd_mm(argv[2]);
+    if (c == "mmtime" && argc >= 3) return cmd_mmtime(argv[2]);
     if (c == "mmw" && argc >= 4) {  // mmw GRID OUT: write_matrix_market; the matrix on stdout
         CSRMatrix A = grid(argv[2]);
         write_matrix_market(argv[3], A);

@@ -19,6 +19,8 @@
 // (src/precond/ilu0.h:62-76) and the FP64 sweep reproduces it bit for bit.
 #include "sg_internal.h"
 
+#include <climits>
+
 #include <algorithm>
 #include <cstdio>
 #include <cmath>
@@ -74,7 +76,7 @@ struct Plan {
     std::vector<SubInfo> subs;
     std::vector<int> order;  // subdomains by descending nnz
     std::vector<Level> flev, blev;
-    std::vector<int> f_row, f_col, f_src;
+    std::vector<int> f_row, f_col, f_src, f_bpos;
     std::vector<int> b_row, b_col, b_src, b_dsrc;
     std::vector<int> lrp, lci, ldiag;  // concatenated local CSR
     std::vector<double> lval;
@@ -114,6 +116,11 @@ struct sg_precond {
     size_t lds_bytes = 0;      // dynamic LDS of the sweep with z in LDS
     size_t lds_tab_bytes = 0;  // level tables only (16-byte aligned); 0 when in global memory
     bool tab_in_lds = true;
+    // ring sweep (z too large for LDS): R slots, a full barrier every F levels
+    bool ring = false;
+    int ring_R = 0, ring_F = 0;
+    size_t ring_bytes = 0, zstride = 0;
+    sg::DevBuf<int> f_bpos;
     int sweep_variant = 1;     // index into kSweepShapes
     bool sweep_tuned = false;
     // sync-free sweep: LDS = level tables | item prefixes | z | row flags
@@ -484,6 +491,204 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         }
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = z[i];
+    }
+}
+
+// ---- ring sweep: z too large for LDS --------------------------------------------
+// For subdomains whose solution vector does not fit in LDS, the recently
+// computed values live in an LDS ring of R entries indexed by the row's
+// position in level order (slot = position mod R, slot R holds zero for
+// padding).  Column references were encoded on the host: c >= 0 is a ring
+// slot still live when the row's level runs, c < 0 is ~index into the global
+// copy (forward results in backward order, zb; backward results by local row,
+// zr).  Every value a row reads from global memory was written at least F
+// levels earlier, and a full barrier (global stores complete) closes every
+// F-th level, so those reads see it; the other levels end with an LDS-only
+// barrier.  The own value of a row comes from a level-ordered array (gathered
+// input zf for the forward sweep, zb for the backward one) and is prefetched
+// with the row data, so nothing on the level-to-level chain waits for global
+// memory.  Operation order per row equals the ILU(0) sweep's.
+template <typename T>
+struct RowPreR {
+    int row;  // forward: backward position of the row; backward: the local row
+    int col[kKreg];
+    T val[kKreg];
+    T diag;
+    T own;
+    unsigned use;
+};
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void row_prefetch_r(RowPreR<T>& p, const Level& L, int q, int pos0,
+                                               const int* __restrict__ rows,
+                                               const int* __restrict__ cols,
+                                               const T* __restrict__ vals,
+                                               const T* __restrict__ diags,
+                                               const int* __restrict__ dst,
+                                               const T* __restrict__ ownv) {
+    const bool live = q < L.size;
+    const int qq = live ? q : 0;
+    p.row = BWD ? rows[L.start + qq] : dst[L.start + qq];
+    p.own = ownv[L.start - pos0 + qq];
+    unsigned use = live ? 0x80000000u : 0u;
+#pragma unroll
+    for (int k = 0; k < kKreg; ++k) {
+        const int idx = L.ell + (k < L.K ? k : 0) * L.size + qq;
+        p.col[k] = cols[idx];
+        p.val[k] = vals[idx];
+        if (live && k < L.K) use |= 1u << k;
+    }
+    if (BWD) p.diag = diags[L.start + qq];
+    p.use = use;
+}
+
+template <typename T>
+__device__ __forceinline__ T ring_read(const T* ring, const T* __restrict__ zglob, int c) {
+    return c >= 0 ? ring[c] : zglob[~c];
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void row_compute_r(T* ring, const T* zdep, T* zout,
+                                              const RowPreR<T>& p, const Level& L, int pos0,
+                                              int R, const int* __restrict__ rows,
+                                              const int* __restrict__ cols,
+                                              const T* __restrict__ vals,
+                                              const T* __restrict__ diags,
+                                              const int* __restrict__ dst,
+                                              const T* __restrict__ ownv) {
+    const int t = threadIdx.x, B = blockDim.x;
+    int slot0 = (L.start - pos0) % R;
+    if (p.use & 0x80000000u) {
+        T acc = p.own;
+#pragma unroll
+        for (int k = 0; k < kKreg; ++k) {
+            const bool u = (p.use >> k) & 1u;
+            acc -= (u ? p.val[k] : (T)0) * ring_read(ring, zdep, u ? p.col[k] : R);
+        }
+        for (int k = kKreg; k < L.K; ++k)
+            acc -= vals[L.ell + k * L.size + t] * ring_read(ring, zdep, cols[L.ell + k * L.size + t]);
+        if (BWD) acc = acc / p.diag;
+        int sl = slot0 + t;
+        if (sl >= R) sl -= R;
+        ring[sl] = acc;
+        zout[p.row] = acc;
+    }
+    for (int q = t + B; q < L.size; q += B) {  // levels wider than the block
+        T acc = ownv[L.start - pos0 + q];
+        for (int k = 0; k < L.K; ++k)
+            acc -= vals[L.ell + k * L.size + q] * ring_read(ring, zdep, cols[L.ell + k * L.size + q]);
+        if (BWD) acc = acc / diags[L.start + q];
+        int sl = slot0 + q;
+        while (sl >= R) sl -= R;
+        ring[sl] = acc;
+        zout[BWD ? rows[L.start + q] : dst[L.start + q]] = acc;
+    }
+}
+
+template <typename T, bool BWD>
+__device__ __forceinline__ void sweep_r(T* ring, const T* zdep, T* zout, const Level* lev,
+                                        int nlev, int pos0, int R, int F,
+                                        const int* __restrict__ rows,
+                                        const int* __restrict__ cols,
+                                        const T* __restrict__ vals,
+                                        const T* __restrict__ diags,
+                                        const int* __restrict__ dst,
+                                        const T* __restrict__ ownv) {
+    if (nlev == 0) return;
+    const int t = threadIdx.x;
+    const Level none{0, 0, 0, 0};
+    auto level = [&](int l) { return l < nlev ? lev[l] : none; };
+    auto level_barrier = [&](int l) {
+        if ((l + 1) % F == 0) __syncthreads();
+        else lds_barrier();
+    };
+    RowPreR<T> b0, b1, b2;
+    // prefetches of levels past the end read position pos0 of an empty level
+    auto pre = [&](RowPreR<T>& b, int l) {
+        const Level L = level(l);
+        row_prefetch_r<T, BWD>(b, l < nlev ? L : Level{pos0, 0, 0, 0}, t, pos0, rows, cols,
+                               vals, diags, dst, ownv);
+    };
+    pre(b0, 0);
+    pre(b1, 1);
+    for (int l = 0; l < nlev; l += 3) {
+        pre(b2, l + 2);
+        row_compute_r<T, BWD>(ring, zdep, zout, b0, level(l), pos0, R, rows, cols, vals, diags,
+                              dst, ownv);
+        level_barrier(l);
+        if (l + 1 >= nlev) break;
+        pre(b0, l + 3);
+        row_compute_r<T, BWD>(ring, zdep, zout, b1, level(l + 1), pos0, R, rows, cols, vals,
+                              diags, dst, ownv);
+        level_barrier(l + 1);
+        if (l + 2 >= nlev) break;
+        pre(b1, l + 4);
+        row_compute_r<T, BWD>(ring, zdep, zout, b2, level(l + 2), pos0, R, rows, cols, vals,
+                              diags, dst, ownv);
+        level_barrier(l + 2);
+    }
+}
+
+// Gather into forward level order, forward sweep (results to zb in backward
+// order), backward sweep (results to zr by local row), prolongation as
+// k_schwarz_sweep does.  z3 holds zf | zb | zr, each ns + 1 per subdomain.
+template <typename T, int MODE, int NT>
+__global__ void __launch_bounds__(NT)
+k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
+                     const Level* __restrict__ flev, const int* __restrict__ f_row,
+                     const int* __restrict__ f_col, const T* __restrict__ f_val,
+                     const int* __restrict__ f_bpos, const Level* __restrict__ blev,
+                     const int* __restrict__ b_row, const int* __restrict__ b_col,
+                     const T* __restrict__ b_val, const T* __restrict__ b_diag,
+                     const int* __restrict__ dofs, const int* __restrict__ own,
+                     const int* __restrict__ own_gid, const double* __restrict__ x,
+                     double* __restrict__ y, T* __restrict__ z3, size_t zstride,
+                     T* __restrict__ zout, const T* __restrict__ cz, DevGuard guard,
+                     int lds_ring_off, int R, int F) {
+    extern __shared__ unsigned char lds_raw[];
+    const int s = order[blockIdx.x];
+    if (guard_stop(guard, s)) return;
+    const SubInfo si = subs[s];
+    const Level* lf;
+    const Level* lb;
+    if (lds_ring_off > 0) {
+        Level* tf = reinterpret_cast<Level*>(lds_raw);
+        Level* tb = tf + si.nflev;
+        for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) tf[l] = flev[si.flev0 + l];
+        for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) tb[l] = blev[si.blev0 + l];
+        lf = tf;
+        lb = tb;
+    } else {
+        lf = flev + si.flev0;
+        lb = blev + si.blev0;
+    }
+    T* ring = reinterpret_cast<T*>(lds_raw + lds_ring_off);
+    T* zf = z3 + si.zoff;
+    T* zb = zf + zstride;
+    T* zr = zb + zstride;
+    const int pf0 = si.nflev ? flev[si.flev0].start : 0;
+    const int pb0 = si.nblev ? blev[si.blev0].start : 0;
+    const int* d = dofs + si.row0;
+    // R_s x in forward level order: zf[p] = x[dofs[row(p)]]
+    for (int p = threadIdx.x; p < si.ns; p += blockDim.x) zf[p] = (T)x[d[f_row[pf0 + p]]];
+    if (threadIdx.x == 0) ring[R] = (T)0;
+    __syncthreads();
+    sweep_r<T, false>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val, nullptr,
+                      f_bpos, zf);
+    __syncthreads();
+    sweep_r<T, true>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val, b_diag,
+                     nullptr, zb);
+    __syncthreads();
+    if (MODE == 0) {
+        const int* ow = own + si.own0;
+        const int* og = own_gid + si.own0;
+        const T add = cz ? cz[s] : (T)0;
+        for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
+            const int li = ow[k], g = og[k];
+            y[g] = cz ? (double)(zr[li] + add) : (double)zr[li];
+        }
+    } else {
+        for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = zr[i];
     }
 }
 
@@ -903,6 +1108,60 @@ static void setup_jacobi(sg_precond* M, const sg_csr* A) {
     M->inv_diag.upload(inv, M->ctx->stream);
 }
 
+// Ring encoding of one subdomain's sweeps (see k_schwarz_sweep_ring): column
+// references become live ring slots (level-order position mod R) or ~index
+// into the global copy (forward: the dependency's backward position; backward:
+// its local row); padding becomes R.  bpos_out[p] = backward position of the
+// row at forward position p.  Returns the smallest level distance of a global
+// reference (INT_MAX when none).
+static int ring_encode(const SweepPack& fw, const SweepPack& bw, int ns, int R,
+                       std::vector<int>& fcol, std::vector<int>& bcol,
+                       std::vector<int>& bpos_out) {
+    std::vector<int> fpos(ns), bposv(ns), flv(ns), blv(ns);
+    for (int l = 0; l < (int)fw.lev.size(); ++l)
+        for (int q = 0; q < fw.lev[l].size; ++q) {
+            const int r = fw.row[fw.lev[l].start + q];
+            fpos[r] = fw.lev[l].start + q;
+            flv[r] = l;
+        }
+    for (int l = 0; l < (int)bw.lev.size(); ++l)
+        for (int q = 0; q < bw.lev[l].size; ++q) {
+            const int r = bw.row[bw.lev[l].start + q];
+            bposv[r] = bw.lev[l].start + q;
+            blv[r] = l;
+        }
+    int F = INT_MAX;
+    auto encode = [&](const SweepPack& P, bool fwd, std::vector<int>& out) {
+        out = P.col;
+        const std::vector<int>& pos = fwd ? fpos : bposv;
+        const std::vector<int>& lv = fwd ? flv : blv;
+        for (int l = 0; l < (int)P.lev.size(); ++l) {
+            const Level& L = P.lev[l];
+            for (int k = 0; k < L.K; ++k)
+                for (int q = 0; q < L.size; ++q) {
+                    const size_t idx = (size_t)L.ell + (size_t)k * L.size + q;
+                    const int c = P.col[idx];
+                    if (c == ns) {
+                        out[idx] = R;
+                        continue;
+                    }
+                    const int pc = pos[c];
+                    if (pc >= L.start + L.size - R) {
+                        out[idx] = pc % R;
+                    } else {
+                        out[idx] = ~(fwd ? bposv[c] : c);
+                        F = std::min(F, l - lv[c]);
+                    }
+                }
+        }
+    };
+    encode(fw, true, fcol);
+    encode(bw, false, bcol);
+    bpos_out.resize(ns);
+    for (int p = 0; p < ns; ++p) bpos_out[p] = bposv[fw.row[p]];
+    return F;
+}
+
 // pre: an already factored global matrix (ILUT) with its diagonal positions;
 // its sweeps are built the same way and the device factorisation is skipped.
 static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
@@ -939,6 +1198,33 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         pack_levels(L, diag[s], upper_levels(L), false, bw[s]);
     });
 
+    // LDS plan: level tables (when they take at most half the LDS), then z when
+    // the largest subdomain fits, else a ring of recent values (every level
+    // must fit in it)
+    const bool fp32 = M->opts.precision == SG_FP32;
+    const size_t tsz = fp32 ? 4 : 8;
+    int maxlev = 0, max_ns = 0, max_w = 0;
+    for (int s = 0; s < nsub; ++s) {
+        maxlev = std::max(maxlev, (int)(fw[s].lev.size() + bw[s].lev.size()));
+        max_ns = std::max(max_ns, local[s].n);
+        for (const Level& lv : fw[s].lev) max_w = std::max(max_w, lv.size);
+        for (const Level& lv : bw[s].lev) max_w = std::max(max_w, lv.size);
+    }
+    M->lds_tab_bytes = ((size_t)maxlev * sizeof(Level) + 15) / 16 * 16;
+    M->tab_in_lds = M->lds_tab_bytes <= (size_t)ctx->max_lds / 2;
+    if (!M->tab_in_lds) M->lds_tab_bytes = 0;
+    const size_t zbytes = (size_t)(max_ns + 1) * tsz;
+    M->z_in_lds = M->lds_tab_bytes + zbytes <= (size_t)ctx->max_lds;
+    M->ring = false;
+    M->ring_F = INT_MAX;
+    if (!M->z_in_lds) {
+        const long long R = (long long)(((size_t)ctx->max_lds - M->lds_tab_bytes) / tsz) - 1;
+        if (R >= 64 && max_w <= R) {
+            M->ring = true;
+            M->ring_R = (int)R;
+        }
+    }
+
     // concatenate
     Plan pl;
     pl.subs.resize(nsub);
@@ -972,8 +1258,16 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
         pl.f_row.insert(pl.f_row.end(), fw[s].row.begin(), fw[s].row.end());
         pl.b_row.insert(pl.b_row.end(), bw[s].row.begin(), bw[s].row.end());
-        pl.f_col.insert(pl.f_col.end(), fw[s].col.begin(), fw[s].col.end());
-        pl.b_col.insert(pl.b_col.end(), bw[s].col.begin(), bw[s].col.end());
+        if (M->ring) {
+            std::vector<int> fc, bc, bp;
+            M->ring_F = std::min(M->ring_F, ring_encode(fw[s], bw[s], L.n, M->ring_R, fc, bc, bp));
+            pl.f_col.insert(pl.f_col.end(), fc.begin(), fc.end());
+            pl.b_col.insert(pl.b_col.end(), bc.begin(), bc.end());
+            pl.f_bpos.insert(pl.f_bpos.end(), bp.begin(), bp.end());
+        } else {
+            pl.f_col.insert(pl.f_col.end(), fw[s].col.begin(), fw[s].col.end());
+            pl.b_col.insert(pl.b_col.end(), bw[s].col.begin(), bw[s].col.end());
+        }
         for (int k : fw[s].src) pl.f_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
         for (int k : bw[s].src) pl.b_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
         for (int k : bw[s].dsrc) pl.b_dsrc.push_back((int)(k + nnz0));
@@ -1007,6 +1301,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         fail(SG_E_ARG, "subdomain data exceeds 32-bit indexing");
     // padding read (never used) by the branch-free prefetch of idle threads
     pl.f_row.resize(pl.f_row.size() + kPadRows, 0);
+    if (M->ring) pl.f_bpos.resize(pl.f_row.size(), 0);
     pl.b_row.resize(pl.b_row.size() + kPadRows, 0);
     pl.f_col.resize(pl.f_col.size() + kPadRows, 0);
     pl.b_col.resize(pl.b_col.size() + kPadRows, 0);
@@ -1036,6 +1331,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
     M->b_src.upload(pl.b_src, st);
     M->b_dsrc.upload(pl.b_dsrc, st);
     M->d_dofs.upload(pl.dofs, st);
+    if (M->ring) M->f_bpos.upload(pl.f_bpos, st);
     M->d_own.upload(pl.own, st);
     M->d_own_gid.upload(pl.own_gid, st);
     M->lrp.upload(pl.lrp, st);
@@ -1050,7 +1346,6 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         SG_HIP(hipGetLastError());
     }
     // pack values for the sweeps
-    const bool fp32 = M->opts.precision == SG_FP32;
     int nf = (int)pl.f_src.size(), nb = (int)pl.b_src.size(), nd = (int)pl.b_dsrc.size();
     auto pack = [&](auto* dummy, sg::DevBuf<int>& src, auto& out, int count) {
         using T = std::remove_pointer_t<decltype(dummy)>;
@@ -1070,20 +1365,13 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
     }
     SG_HIP(hipGetLastError());
 
-    // LDS: level tables of the subdomain with the most levels, then z when the
-    // largest subdomain fits
-    int maxlev = 0;
-    for (const SubInfo& si : pl.subs) maxlev = std::max(maxlev, si.nflev + si.nblev);
-    M->lds_tab_bytes = ((size_t)maxlev * sizeof(Level) + 15) / 16 * 16;
-    // level tables that would take more than half the LDS stay in global memory
-    M->tab_in_lds = M->lds_tab_bytes <= (size_t)ctx->max_lds / 2;
-    if (!M->tab_in_lds) M->lds_tab_bytes = 0;
-    size_t zbytes = (size_t)(pl.max_ns + 1) * (fp32 ? 4 : 8);
-    M->z_in_lds = M->lds_tab_bytes + zbytes <= (size_t)ctx->max_lds;
     M->lds_bytes = M->lds_tab_bytes + zbytes;
-    if (!M->z_in_lds) {
-        if (fp32) M->z32.alloc((size_t)zoff);
-        else M->z64.alloc((size_t)zoff);
+    M->ring_bytes = M->lds_tab_bytes + (size_t)(M->ring_R + 1) * tsz;
+    M->zstride = (size_t)zoff;
+    if (!M->z_in_lds) {  // global z: one copy, or zf | zb | zr for the ring sweep
+        const size_t zn = (size_t)zoff * (M->ring ? 3 : 1);
+        if (fp32) M->z32.alloc(zn);
+        else M->z64.alloc(zn);
     }
     {
         auto al = [](size_t b) { return (b + 15) / 16 * 16; };
@@ -1115,6 +1403,22 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             SG_SET_LDS(double, false, 0) SG_SET_LDS(double, false, 1)
         }
 #undef SG_SET_LDS
+        if (M->ring) {
+            const int rb = (int)M->ring_bytes;
+#define SG_SET_RING(T, MD)                                                                        \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 256>,                    \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));                \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 512>,                    \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));                \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 1024>,                   \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));
+            if (fp32) {
+                SG_SET_RING(float, 0) SG_SET_RING(float, 1)
+            } else {
+                SG_SET_RING(double, 0) SG_SET_RING(double, 1)
+            }
+#undef SG_SET_RING
+        }
         if (M->sf_ok) {
             const int sfb = (int)M->sf_bytes;
 #define SG_SET_SF(T, MD)                                                                          \
@@ -1320,6 +1624,22 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
         if (sh.threads == 1024) SG_SF(1024);
         else SG_SF(512);
 #undef SG_SF
+        return;
+    }
+    if (!ZLDS && M->ring) {
+        const int F = M->ring_F == INT_MAX ? INT_MAX : std::max(1, M->ring_F);
+#define SG_RING(NT)                                                                               \
+    hipLaunchKernelGGL((k_schwarz_sweep_ring<T, MODE, NT>), dim3(M->nsub), dim3(NT),             \
+                       M->ring_bytes, M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p,     \
+                       M->f_row.p, M->f_col.p, fv, M->f_bpos.p, M->blev.p, M->b_row.p,         \
+                       M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zg,   \
+                       M->zstride, zo, cz, guard, (int)M->lds_tab_bytes, M->ring_R, F)
+        switch (sh.syncfree ? 512 : sh.threads) {
+            case 256: SG_RING(256); break;
+            case 1024: SG_RING(1024); break;
+            default: SG_RING(512); break;
+        }
+#undef SG_RING
         return;
     }
     const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;

@@ -118,6 +118,52 @@ def test_ilu0_random_values_bitwise(sg, ctx):
     assert np.array_equal(dy.get(), O.ilu0_solve(Ao, lu, diag, x))
 
 
+@pytest.mark.parametrize("spec,kind,nparts", [
+    ("2d:256:256", "ilu0", 1),     # 65536 FP64 rows: window of recent values in LDS
+    ("rnd:60000:3", "ilu0", 1),    # random couplings: many references outside the window
+    ("3d:48:48:48", "ras", 2),     # two subdomains of ~57K rows each
+    ("3d:40:40:40", "asm", 2),
+])
+def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
+    """Subdomains whose solution vector exceeds LDS run the ring sweep: it must
+    equal the oracle bit for bit in FP64 with every workgroup size."""
+    A = O.grid(spec)
+    rng = np.random.default_rng(12)
+    v = A.v * rng.uniform(0.9, 1.1, A.nnz)
+    Ao = O.Csr(A.rp, A.ci, v)
+    dA = sg.Csr(ctx, A.rp, A.ci, v)
+    x = rng.standard_normal(A.n)
+    outs = []
+    for mode in (2, 3, 4):  # barrier sweeps with 256 / 512 / 1024 threads
+        ctx.autotune(mode)
+        M = sg.Precond(ctx, kind, nparts=nparts, overlap=1).setup(dA)
+        info = M.stats()
+        assert not info["z_in_lds"]
+        y = dev(sg, ctx, np.zeros(A.n))
+        M.apply(dev(sg, ctx, x), y)
+        outs.append(y.get())
+        part = M.partition() if nparts > 1 else None
+    ctx.autotune(1)
+    ref = O.Precond(kind, Ao, nparts, 1, part).apply(x)
+    for y in outs:
+        assert np.array_equal(y, ref)
+
+
+def test_ring_sweep_fp32(sg, ctx):
+    A = O.poisson3d(64, 64, 64)
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    M = sg.Precond(ctx, "ras", nparts=4, overlap=1, precision="fp32").setup(dA)
+    assert not M.stats()["z_in_lds"]
+    x = np.random.default_rng(13).standard_normal(A.n)
+    y = dev(sg, ctx, np.zeros(A.n))
+    M.apply(dev(sg, ctx, x), y)
+    ref = O.Precond("ras", A, 4, 1, M.partition(), precision=1).apply(x)
+    assert np.linalg.norm(y.get() - ref) <= 1e-5 * np.linalg.norm(ref)
+    y2 = dev(sg, ctx, np.zeros(A.n))
+    M.apply(dev(sg, ctx, x), y2)
+    assert np.array_equal(y.get(), y2.get())
+
+
 def test_ras_asm_match_reference_blocks(sg, ctx, gold):
     for case in gold("blocks.json"):
         A = O.grid(case["grid"])
