@@ -42,13 +42,17 @@ constexpr int kPadRows = 1024;      // padding of the sweep arrays (largest swee
 // Sweep kernel shapes, all bit-identical: level-synchronous with 256/512/1024
 // threads, and sync-free (per-row LDS flags, no barrier per level) with
 // 512/1024 threads when z and the flags fit in LDS.
+// depth: levels of row data in flight per thread in the barrier sweeps (the
+// row data of level l + depth - 1 is loaded while level l computes)
 struct SweepShape {
     int threads;
     bool syncfree;
+    int depth;
 };
-constexpr int kNumSweepShapes = 5;
+constexpr int kNumSweepShapes = 7;
 constexpr SweepShape kSweepShapes[kNumSweepShapes] = {
-    {256, false}, {512, false}, {1024, false}, {512, true}, {1024, true}};
+    {256, false, 3}, {512, false, 3}, {1024, false, 3}, {512, true, 3}, {1024, true, 3},
+    {256, false, 6}, {512, false, 6}};
 constexpr int kKreg = 4;  // factor entries per row prefetched in registers
 constexpr int kRestrictSplit = 8;  // workgroups per subdomain in the coarse restriction
 constexpr int kSfDepth = SG_SF_DEPTH;  // items of row data in flight per wave (sync-free sweep)
@@ -387,11 +391,11 @@ __device__ __forceinline__ void row_compute(T* z, const RowPre<T>& p, const Leve
 
 // One level-scheduled sweep over a subdomain: forward (unit L) or backward
 // (U, division by the diagonal), z in place.  Each row subtracts its entries
-// in CSR order, as src/precond/ilu0.h:62-76 does.  Row data is loaded two
-// levels ahead into a ring of three register buffers; the loop is unrolled by
-// three so the ring never needs register moves (a move would wait for the
-// loads in flight).  lev points at the subdomain's level table in LDS.
-template <typename T, bool BWD, bool ZLDS>
+// in CSR order, as src/precond/ilu0.h:62-76 does.  Row data is loaded D - 1
+// levels ahead into a ring of D register buffers; the loop is unrolled by D so
+// the ring never needs register moves (a move would wait for the loads in
+// flight).  lev points at the subdomain's level table.
+template <typename T, bool BWD, bool ZLDS, int D>
 __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslot,
                                       const int* __restrict__ rows,
                                       const int* __restrict__ cols,
@@ -405,21 +409,18 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
         if (ZLDS) lds_barrier();
         else __syncthreads();
     };
-    RowPre<T> b0, b1, b2;
-    row_prefetch<T, BWD>(b0, level(0), t, rows, cols, vals, diags);
-    row_prefetch<T, BWD>(b1, level(1), t, rows, cols, vals, diags);
-    for (int l = 0; l < nlev; l += 3) {
-        row_prefetch<T, BWD>(b2, level(l + 2), t, rows, cols, vals, diags);
-        row_compute<T, BWD>(z, b0, level(l), zslot, rows, cols, vals, diags);
-        level_barrier();
-        if (l + 1 >= nlev) break;
-        row_prefetch<T, BWD>(b0, level(l + 3), t, rows, cols, vals, diags);
-        row_compute<T, BWD>(z, b1, level(l + 1), zslot, rows, cols, vals, diags);
-        level_barrier();
-        if (l + 2 >= nlev) break;
-        row_prefetch<T, BWD>(b1, level(l + 4), t, rows, cols, vals, diags);
-        row_compute<T, BWD>(z, b2, level(l + 2), zslot, rows, cols, vals, diags);
-        level_barrier();
+    RowPre<T> b[D];
+#pragma unroll
+    for (int j = 0; j < D - 1; ++j) row_prefetch<T, BWD>(b[j], level(j), t, rows, cols, vals, diags);
+    for (int l = 0; l < nlev; l += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            row_prefetch<T, BWD>(b[(j + D - 1) % D], level(l + j + D - 1), t, rows, cols, vals,
+                                 diags);
+            row_compute<T, BWD>(z, b[j], level(l + j), zslot, rows, cols, vals, diags);
+            level_barrier();
+            if (l + j + 1 >= nlev) return;
+        }
     }
 }
 
@@ -429,7 +430,7 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
 //         the owned rows (src/precond/ras.h:55-68).
 // MODE 1: additive (ASM): the whole local solution goes to zout, summed by
 //         k_asm_gather (src/precond/asm.h:54-58).
-template <typename T, bool ZLDS, int MODE, int NT>
+template <typename T, bool ZLDS, int MODE, int NT, int D, bool TAB>
 __global__ void __launch_bounds__(NT)
 k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                 const Level* __restrict__ flev, const int* __restrict__ f_row,
@@ -445,21 +446,20 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
-    // LDS: this subdomain's level tables, then (ZLDS) the solution vector.
-    // lds_z_off == 0: the tables are too long for LDS and are read from
-    // global memory (uniform loads, two levels ahead).
-    const Level* lf;
-    const Level* lb;
-    if (lds_z_off > 0) {
+    // LDS: this subdomain's level tables (TAB; else they are too long for LDS
+    // and are read from global memory), then (ZLDS) the solution vector.  The
+    // choice is a template parameter so every LDS pointer is known as one and
+    // its accesses stay ds_read / ds_write (a pointer that may be either would
+    // make them flat accesses, whose waits also drain the global prefetch).
+    const Level* lf = flev + si.flev0;
+    const Level* lb = blev + si.blev0;
+    if (TAB) {
         Level* tf = reinterpret_cast<Level*>(lds_raw);
         Level* tb = tf + si.nflev;
         for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) tf[l] = flev[si.flev0 + l];
         for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) tb[l] = blev[si.blev0 + l];
         lf = tf;
         lb = tb;
-    } else {
-        lf = flev + si.flev0;
-        lb = blev + si.blev0;
     }
     T* z = ZLDS ? reinterpret_cast<T*>(lds_raw + lds_z_off) : zglob + si.zoff;
     const int* d = dofs + si.row0;
@@ -477,9 +477,9 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     }
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
-    sweep<T, false, ZLDS>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
+    sweep<T, false, ZLDS, D>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
     __syncthreads();
-    sweep<T, true, ZLDS>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
+    sweep<T, true, ZLDS, D>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
     __syncthreads();
     if (MODE == 0) {
         const int* ow = own + si.own0;
@@ -543,12 +543,19 @@ __device__ __forceinline__ void row_prefetch_r(RowPreR<T>& p, const Level& L, in
 }
 
 template <typename T>
-__device__ __forceinline__ T ring_read(const T* ring, const T* __restrict__ zglob, int c) {
-    return c >= 0 ? ring[c] : zglob[~c];
+using lds_t = __attribute__((address_space(3))) T;
+
+// The two loads stay apart (an LDS-typed and a global pointer), so the common
+// LDS read is a ds_read.  GLOB = false (no reference leaves the ring in this
+// preconditioner) drops the global branch: its wait would drain the prefetch.
+template <typename T, bool GLOB>
+__device__ __forceinline__ T ring_read(const lds_t<T>* ring, const T* __restrict__ zglob, int c) {
+    if (!GLOB || c >= 0) return ring[c];
+    return zglob[~c];
 }
 
-template <typename T, bool BWD>
-__device__ __forceinline__ void row_compute_r(T* ring, const T* zdep, T* zout,
+template <typename T, bool BWD, bool GLOB>
+__device__ __forceinline__ void row_compute_r(lds_t<T>* ring, const T* zdep, T* zout,
                                               const RowPreR<T>& p, const Level& L, int pos0,
                                               int R, const int* __restrict__ rows,
                                               const int* __restrict__ cols,
@@ -563,10 +570,11 @@ __device__ __forceinline__ void row_compute_r(T* ring, const T* zdep, T* zout,
 #pragma unroll
         for (int k = 0; k < kKreg; ++k) {
             const bool u = (p.use >> k) & 1u;
-            acc -= (u ? p.val[k] : (T)0) * ring_read(ring, zdep, u ? p.col[k] : R);
+            acc -= (u ? p.val[k] : (T)0) * ring_read<T, GLOB>(ring, zdep, u ? p.col[k] : R);
         }
         for (int k = kKreg; k < L.K; ++k)
-            acc -= vals[L.ell + k * L.size + t] * ring_read(ring, zdep, cols[L.ell + k * L.size + t]);
+            acc -= vals[L.ell + k * L.size + t] *
+                   ring_read<T, GLOB>(ring, zdep, cols[L.ell + k * L.size + t]);
         if (BWD) acc = acc / p.diag;
         int sl = slot0 + t;
         if (sl >= R) sl -= R;
@@ -576,7 +584,8 @@ __device__ __forceinline__ void row_compute_r(T* ring, const T* zdep, T* zout,
     for (int q = t + B; q < L.size; q += B) {  // levels wider than the block
         T acc = ownv[L.start - pos0 + q];
         for (int k = 0; k < L.K; ++k)
-            acc -= vals[L.ell + k * L.size + q] * ring_read(ring, zdep, cols[L.ell + k * L.size + q]);
+            acc -= vals[L.ell + k * L.size + q] *
+                   ring_read<T, GLOB>(ring, zdep, cols[L.ell + k * L.size + q]);
         if (BWD) acc = acc / diags[L.start + q];
         int sl = slot0 + q;
         while (sl >= R) sl -= R;
@@ -585,8 +594,8 @@ __device__ __forceinline__ void row_compute_r(T* ring, const T* zdep, T* zout,
     }
 }
 
-template <typename T, bool BWD>
-__device__ __forceinline__ void sweep_r(T* ring, const T* zdep, T* zout, const Level* lev,
+template <typename T, bool BWD, int D, bool GLOB>
+__device__ __forceinline__ void sweep_r(lds_t<T>* ring, const T* zdep, T* zout, const Level* lev,
                                         int nlev, int pos0, int R, int F,
                                         const int* __restrict__ rows,
                                         const int* __restrict__ cols,
@@ -596,43 +605,33 @@ __device__ __forceinline__ void sweep_r(T* ring, const T* zdep, T* zout, const L
                                         const T* __restrict__ ownv) {
     if (nlev == 0) return;
     const int t = threadIdx.x;
-    const Level none{0, 0, 0, 0};
-    auto level = [&](int l) { return l < nlev ? lev[l] : none; };
     auto level_barrier = [&](int l) {
-        if ((l + 1) % F == 0) __syncthreads();
+        if (GLOB && (l + 1) % F == 0) __syncthreads();
         else lds_barrier();
     };
-    RowPreR<T> b0, b1, b2;
-    // prefetches of levels past the end read position pos0 of an empty level
-    auto pre = [&](RowPreR<T>& b, int l) {
-        const Level L = level(l);
-        row_prefetch_r<T, BWD>(b, l < nlev ? L : Level{pos0, 0, 0, 0}, t, pos0, rows, cols,
-                               vals, diags, dst, ownv);
-    };
-    pre(b0, 0);
-    pre(b1, 1);
-    for (int l = 0; l < nlev; l += 3) {
-        pre(b2, l + 2);
-        row_compute_r<T, BWD>(ring, zdep, zout, b0, level(l), pos0, R, rows, cols, vals, diags,
-                              dst, ownv);
-        level_barrier(l);
-        if (l + 1 >= nlev) break;
-        pre(b0, l + 3);
-        row_compute_r<T, BWD>(ring, zdep, zout, b1, level(l + 1), pos0, R, rows, cols, vals,
-                              diags, dst, ownv);
-        level_barrier(l + 1);
-        if (l + 2 >= nlev) break;
-        pre(b1, l + 4);
-        row_compute_r<T, BWD>(ring, zdep, zout, b2, level(l + 2), pos0, R, rows, cols, vals,
-                              diags, dst, ownv);
-        level_barrier(l + 2);
+    // prefetches past the last level read position pos0 of an empty level
+    auto level = [&](int l) { return l < nlev ? lev[l] : Level{pos0, 0, 0, 0}; };
+    RowPreR<T> b[D];
+#pragma unroll
+    for (int j = 0; j < D - 1; ++j)
+        row_prefetch_r<T, BWD>(b[j], level(j), t, pos0, rows, cols, vals, diags, dst, ownv);
+    for (int l = 0; l < nlev; l += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            row_prefetch_r<T, BWD>(b[(j + D - 1) % D], level(l + j + D - 1), t, pos0, rows,
+                                   cols, vals, diags, dst, ownv);
+            row_compute_r<T, BWD, GLOB>(ring, zdep, zout, b[j], level(l + j), pos0, R, rows,
+                                        cols, vals, diags, dst, ownv);
+            level_barrier(l + j);
+            if (l + j + 1 >= nlev) return;
+        }
     }
 }
 
 // Gather into forward level order, forward sweep (results to zb in backward
 // order), backward sweep (results to zr by local row), prolongation as
 // k_schwarz_sweep does.  z3 holds zf | zb | zr, each ns + 1 per subdomain.
-template <typename T, int MODE, int NT>
+template <typename T, int MODE, int NT, int D, bool TAB>
 __global__ void __launch_bounds__(NT)
 k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                      const Level* __restrict__ flev, const int* __restrict__ f_row,
@@ -649,43 +648,73 @@ k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ o
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
-    const Level* lf;
-    const Level* lb;
-    if (lds_ring_off > 0) {
+    const Level* lf = flev + si.flev0;
+    const Level* lb = blev + si.blev0;
+    if (TAB) {
         Level* tf = reinterpret_cast<Level*>(lds_raw);
         Level* tb = tf + si.nflev;
         for (int l = threadIdx.x; l < si.nflev; l += blockDim.x) tf[l] = flev[si.flev0 + l];
         for (int l = threadIdx.x; l < si.nblev; l += blockDim.x) tb[l] = blev[si.blev0 + l];
         lf = tf;
         lb = tb;
-    } else {
-        lf = flev + si.flev0;
-        lb = blev + si.blev0;
     }
-    T* ring = reinterpret_cast<T*>(lds_raw + lds_ring_off);
+    lds_t<T>* ring = (lds_t<T>*)(lds_raw + lds_ring_off);
     T* zf = z3 + si.zoff;
     T* zb = zf + zstride;
     T* zr = zb + zstride;
     const int pf0 = si.nflev ? flev[si.flev0].start : 0;
     const int pb0 = si.nblev ? blev[si.blev0].start : 0;
     const int* d = dofs + si.row0;
-    // R_s x in forward level order: zf[p] = x[dofs[row(p)]]
-    for (int p = threadIdx.x; p < si.ns; p += blockDim.x) zf[p] = (T)x[d[f_row[pf0 + p]]];
+    // R_s x in forward level order: zf[p] = x[dofs[row(p)]], eight positions
+    // per thread in flight
+    constexpr int U = 8;
+    for (int p0 = threadIdx.x; p0 < si.ns; p0 += U * NT) {
+        int r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = p0 + u * NT;
+            r[u] = p < si.ns ? f_row[pf0 + p] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] = r[u] >= 0 ? d[r[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (r[u] >= 0) zf[p0 + u * NT] = (T)x[r[u]];
+    }
     if (threadIdx.x == 0) ring[R] = (T)0;
     __syncthreads();
-    sweep_r<T, false>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val, nullptr,
-                      f_bpos, zf);
-    __syncthreads();
-    sweep_r<T, true>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val, b_diag,
-                     nullptr, zb);
+    if (F == INT_MAX) {  // every reference stays in the ring
+        sweep_r<T, false, D, false>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
+                                    nullptr, f_bpos, zf);
+        __syncthreads();
+        sweep_r<T, true, D, false>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
+                                   b_diag, nullptr, zb);
+    } else {
+        sweep_r<T, false, D, true>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
+                                   nullptr, f_bpos, zf);
+        __syncthreads();
+        sweep_r<T, true, D, true>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
+                                  b_diag, nullptr, zb);
+    }
     __syncthreads();
     if (MODE == 0) {
         const int* ow = own + si.own0;
         const int* og = own_gid + si.own0;
         const T add = cz ? cz[s] : (T)0;
-        for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
-            const int li = ow[k], g = og[k];
-            y[g] = cz ? (double)(zr[li] + add) : (double)zr[li];
+        for (int k0 = threadIdx.x; k0 < si.nown; k0 += U * NT) {
+            int li[U], g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * NT;
+                li[u] = k < si.nown ? ow[k] : -1;
+                g[u] = k < si.nown ? og[k] : -1;
+            }
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = li[u] >= 0 ? zr[li[u]] : (T)0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (g[u] >= 0) y[g[u]] = cz ? (double)(v[u] + add) : (double)v[u];
         }
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = zr[i];
@@ -1388,13 +1417,14 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
     {
         const int bytes = (int)std::min<size_t>((size_t)ctx->max_lds,
                                                 std::max(M->lds_bytes, M->lds_tab_bytes));
-#define SG_SET_LDS(T, Z, MD)                                                                      \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 256>,                     \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes));             \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 512>,                     \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes));             \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, 1024>,                    \
+#define SG_SET_LDS1(T, Z, MD, NT, D, TB)                                                          \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep<T, Z, MD, NT, D, TB>,              \
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+#define SG_SET_LDS(T, Z, MD)                                                                      \
+    SG_SET_LDS1(T, Z, MD, 256, 3, true) SG_SET_LDS1(T, Z, MD, 512, 3, true)                     \
+    SG_SET_LDS1(T, Z, MD, 1024, 3, true) SG_SET_LDS1(T, Z, MD, 256, 6, true)                    \
+    SG_SET_LDS1(T, Z, MD, 512, 6, true) SG_SET_LDS1(T, Z, MD, 256, 3, false)                    \
+    SG_SET_LDS1(T, Z, MD, 512, 3, false) SG_SET_LDS1(T, Z, MD, 1024, 3, false)
         if (fp32) {
             SG_SET_LDS(float, true, 0) SG_SET_LDS(float, true, 1)
             SG_SET_LDS(float, false, 0) SG_SET_LDS(float, false, 1)
@@ -1402,21 +1432,24 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             SG_SET_LDS(double, true, 0) SG_SET_LDS(double, true, 1)
             SG_SET_LDS(double, false, 0) SG_SET_LDS(double, false, 1)
         }
+#undef SG_SET_LDS1
 #undef SG_SET_LDS
         if (M->ring) {
             const int rb = (int)M->ring_bytes;
-#define SG_SET_RING(T, MD)                                                                        \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 256>,                    \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));                \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 512>,                    \
-                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));                \
-    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, 1024>,                   \
+#define SG_SET_RING1(T, MD, NT, D, TB)                                                            \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, NT, D, TB>,             \
                                hipFuncAttributeMaxDynamicSharedMemorySize, rb));
+#define SG_SET_RING(T, MD)                                                                        \
+    SG_SET_RING1(T, MD, 256, 3, true) SG_SET_RING1(T, MD, 512, 3, true)                         \
+    SG_SET_RING1(T, MD, 1024, 3, true) SG_SET_RING1(T, MD, 256, 6, true)                        \
+    SG_SET_RING1(T, MD, 512, 6, true) SG_SET_RING1(T, MD, 256, 3, false)                        \
+    SG_SET_RING1(T, MD, 512, 3, false) SG_SET_RING1(T, MD, 1024, 3, false)
             if (fp32) {
                 SG_SET_RING(float, 0) SG_SET_RING(float, 1)
             } else {
                 SG_SET_RING(double, 0) SG_SET_RING(double, 1)
             }
+#undef SG_SET_RING1
 #undef SG_SET_RING
         }
         if (M->sf_ok) {
@@ -1628,32 +1661,42 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
     }
     if (!ZLDS && M->ring) {
         const int F = M->ring_F == INT_MAX ? INT_MAX : std::max(1, M->ring_F);
-#define SG_RING(NT)                                                                               \
-    hipLaunchKernelGGL((k_schwarz_sweep_ring<T, MODE, NT>), dim3(M->nsub), dim3(NT),             \
+#define SG_RING(NT, D, TB)                                                                        \
+    hipLaunchKernelGGL((k_schwarz_sweep_ring<T, MODE, NT, D, TB>), dim3(M->nsub), dim3(NT),      \
                        M->ring_bytes, M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p,     \
                        M->f_row.p, M->f_col.p, fv, M->f_bpos.p, M->blev.p, M->b_row.p,         \
                        M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zg,   \
                        M->zstride, zo, cz, guard, (int)M->lds_tab_bytes, M->ring_R, F)
-        switch (sh.syncfree ? 512 : sh.threads) {
-            case 256: SG_RING(256); break;
-            case 1024: SG_RING(1024); break;
-            default: SG_RING(512); break;
-        }
+        const int nt = sh.syncfree ? 512 : sh.threads;
+        if (!M->tab_in_lds) {
+            if (nt == 256) SG_RING(256, 3, false);
+            else if (nt == 1024) SG_RING(1024, 3, false);
+            else SG_RING(512, 3, false);
+        } else if (sh.depth == 6 && nt == 256) SG_RING(256, 6, true);
+        else if (sh.depth == 6) SG_RING(512, 6, true);
+        else if (nt == 256) SG_RING(256, 3, true);
+        else if (nt == 1024) SG_RING(1024, 3, true);
+        else SG_RING(512, 3, true);
 #undef SG_RING
         return;
     }
     const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;
-#define SG_SWEEP(NT)                                                                              \
-    hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE, NT>), dim3(M->nsub), dim3(NT), lds,       \
+#define SG_SWEEP(NT, D, TB)                                                                       \
+    hipLaunchKernelGGL((k_schwarz_sweep<T, ZLDS, MODE, NT, D, TB>), dim3(M->nsub), dim3(NT), lds, \
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p, M->f_row.p,        \
                        M->f_col.p, fv, M->blev.p, M->b_row.p, M->b_col.p, bv, bd, M->d_dofs.p,  \
                        M->d_own.p, M->d_own_gid.p, x, y, zg, zo, cz, guard,                     \
                        (int)M->lds_tab_bytes)
-    switch (sh.syncfree ? 512 : sh.threads) {
-        case 256: SG_SWEEP(256); break;
-        case 1024: SG_SWEEP(1024); break;
-        default: SG_SWEEP(512); break;
-    }
+    const int nt = sh.syncfree ? 512 : sh.threads;
+    if (!M->tab_in_lds) {
+        if (nt == 256) SG_SWEEP(256, 3, false);
+        else if (nt == 1024) SG_SWEEP(1024, 3, false);
+        else SG_SWEEP(512, 3, false);
+    } else if (sh.depth == 6 && nt == 256) SG_SWEEP(256, 6, true);
+    else if (sh.depth == 6) SG_SWEEP(512, 6, true);
+    else if (nt == 256) SG_SWEEP(256, 3, true);
+    else if (nt == 1024) SG_SWEEP(1024, 3, true);
+    else SG_SWEEP(512, 3, true);
 #undef SG_SWEEP
 }
 

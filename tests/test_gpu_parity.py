@@ -134,7 +134,7 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     dA = sg.Csr(ctx, A.rp, A.ci, v)
     x = rng.standard_normal(A.n)
     outs = []
-    for mode in (2, 3, 4):  # barrier sweeps with 256 / 512 / 1024 threads
+    for mode in (2, 3, 4, 7, 8):  # barrier sweeps: 256 / 512 / 1024 threads; deep prefetch
         ctx.autotune(mode)
         M = sg.Precond(ctx, kind, nparts=nparts, overlap=1).setup(dA)
         info = M.stats()
@@ -242,7 +242,7 @@ def test_kernel_shapes_do_not_change_results(sg, precision):
     v = A.v * rng.uniform(0.5, 1.5, A.nnz)
     x = rng.standard_normal(A.n)
     outs = []
-    for mode in (0, 1, 2, 3, 4, 5, 6):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         c = sg.Context(0)
         c.autotune(mode)
         dA = sg.Csr(c, A.rp, A.ci, v)
