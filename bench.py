@@ -363,10 +363,18 @@ def run_rows(args, sg, P):
     import oracle_bind as O
     ctx = sg.Context(0)
     steps = max(1, args.steps)
+    groups = set(args.row or ["c1", "c3", "mm"])
 
     def emit(d):
         print(json.dumps(d), flush=True)
 
+    if "c1" in groups:
+        rows_c1(args, sg, P, O, ctx, steps, emit)
+    if "c3" in groups or "mm" in groups:
+        rows_c3(args, sg, P, O, ctx, steps, emit, groups)
+
+
+def rows_c1(args, sg, P, O, ctx, steps, emit):
     # ---- c1 matrix: Poisson 256^2, b = A x*, x* ~ U[-1, 1] ----
     rp, ci, v = P.poisson2d(256, 256)
     n = len(rp) - 1
@@ -440,6 +448,17 @@ def run_rows(args, sg, P):
           "speedup": round(tc / t, 2)})
     del M, Mt, A, db, dx
 
+
+def rows_c3(args, sg, P, O, ctx, steps, emit, groups):
+    def gsolve(MM, **kw):
+        def f():
+            dx.zero()
+            ctx.sync()
+            r = sg.solve(A, db, dx, MM, **kw)
+            ctx.sync()
+            return r
+        return f
+
     # ---- c3 matrix: BiCGSTAB with the FP64 two-level RAS (with the FP32 one,
     # whose rounding makes the operator slightly nonlinear, BiCGSTAB stagnates
     # near 1e-3 on this 10^6-contrast problem) ----
@@ -448,6 +467,8 @@ def run_rows(args, sg, P):
     n = len(rp) - 1
     A = sg.Csr(ctx, rp, ci, v)
     db, dx = sg.DeviceArray.from_host(ctx, b), sg.DeviceArray(ctx, n)
+    if "c3" not in groups:
+        return rows_mm(sg, cfg, rp, ci, v, emit)
     part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v, weighted=True)
     M = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
                    precision="fp64", partition=part, coarse_solve=1).setup(A)
@@ -477,7 +498,11 @@ def run_rows(args, sg, P):
           "lambda_min": c["lambda_min"], "lambda_max": c["lambda_max"],
           "power_iters": sr["power_iters"], "rho": sr["spectral_radius"]})
     del M
+    if "mm" in groups:
+        rows_mm(sg, cfg, rp, ci, v, emit)
 
+
+def rows_mm(sg, cfg, rp, ci, v, emit):
     # ---- Matrix Market: the c3 matrix as a 17-digit coordinate file ----
     import tempfile
     with tempfile.TemporaryDirectory() as td:
@@ -523,6 +548,10 @@ def main():
     ap.add_argument("--rows", action="store_true",
                     help="measure the rows around the headline path (BiCGSTAB, ILUT, "
                          "analytics, Matrix Market reader), one JSON line each, and exit")
+    ap.add_argument("--row", action="append", choices=("c1", "c3", "mm"),
+                    help="with --rows: only these groups (c1: BiCGSTAB / ILUT / analytics on "
+                         "Poisson 256^2; c3: BiCGSTAB / analytics on the 160^3 jump case; "
+                         "mm: Matrix Market reader); repeatable")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -626,6 +655,20 @@ def main():
         print(f"# baseline gmres+ras fp64: {rb['iterations']} it {rb['wall_s']:.3f} s "
               f"conv={rb['converged']}", file=sys.stderr, flush=True)
         del Mb
+        # the reference's other Krylov method on the same two-level preconditioner
+        # in FP64 (BiCGSTAB needs a fixed linear operator; the FP32 one stagnates)
+        Mq = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
+                        precision="fp64", partition=part, coarse_solve=1).setup(A)
+        bq_kw = dict(method="bicgstab", max_iter=cfg["max_iter"], tol=1e-8)
+        run_solves(sg, ctx, A, Mq, db, dx, 1, **bq_kw)
+        rq = run_solves(sg, ctx, A, Mq, db, dx, args.steps, **bq_kw)
+        extra["bicgstab_twolevel_fp64"] = {
+            "s": round(float(np.mean([q["wall_s"] for q in rq])), 4),
+            "iterations": rq[-1]["iterations"], "converged": rq[-1]["converged"],
+            "rel_true_residual": rq[-1]["rel_true_residual"]}
+        print(f"# bicgstab+{kind} fp64: {rq[-1]['iterations']} it "
+              f"{extra['bicgstab_twolevel_fp64']['s']:.3f} s", file=sys.stderr, flush=True)
+        del Mq
 
     cpu = None
     if not args.no_cpu and args.config != "c1":

@@ -424,6 +424,30 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
     }
 }
 
+// RAS prolongation of one subdomain: y[og[k]] = z[ow[k]] (+ add) over its
+// owned rows, eight per thread in flight.
+template <int NT, typename T, typename ZP>
+__device__ __forceinline__ void scatter_owned(const int* __restrict__ ow,
+                                              const int* __restrict__ og, int nown, ZP z,
+                                              bool has_cz, T add, double* __restrict__ y) {
+    constexpr int U = 8;
+    for (int k0 = threadIdx.x; k0 < nown; k0 += U * NT) {
+        int li[U], g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * NT;
+            li[u] = k < nown ? ow[k] : -1;
+            g[u] = k < nown ? og[k] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (g[u] >= 0) {
+                const T v = z[li[u]];
+                y[g[u]] = has_cz ? (double)(v + add) : (double)v;
+            }
+    }
+}
+
 // Fused restriction gather + unit-L sweep + U sweep + prolongation, one
 // workgroup per subdomain taken from a queue sorted by subdomain nonzeros.
 // MODE 0: restricted (RAS / global ILU0): y[g] = z (+ coarse term cz[s]) for
@@ -463,17 +487,17 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     }
     T* z = ZLDS ? reinterpret_cast<T*>(lds_raw + lds_z_off) : zglob + si.zoff;
     const int* d = dofs + si.row0;
-    // R_s x: four independent index/value loads in flight per thread
-    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 4 * blockDim.x) {
-        int g[4];
+    // R_s x: eight independent index/value loads in flight per thread
+    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 8 * NT) {
+        int g[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * blockDim.x;
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + u * NT;
             g[u] = i < si.ns ? d[i] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (g[u] >= 0) z[i0 + u * blockDim.x] = (T)x[g[u]];
+        for (int u = 0; u < 8; ++u)
+            if (g[u] >= 0) z[i0 + u * NT] = (T)x[g[u]];
     }
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
@@ -482,13 +506,8 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     sweep<T, true, ZLDS, D>(z, lb, si.nblev, si.ns, b_row, b_col, b_val, b_diag);
     __syncthreads();
     if (MODE == 0) {
-        const int* ow = own + si.own0;
-        const int* og = own_gid + si.own0;
-        const T add = cz ? cz[s] : (T)0;
-        for (int k = threadIdx.x; k < si.nown; k += blockDim.x) {
-            const int li = ow[k], g = og[k];
-            y[g] = cz ? (double)(z[li] + add) : (double)z[li];
-        }
+        scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
+                          cz ? cz[s] : (T)0, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = z[i];
     }
@@ -698,24 +717,8 @@ k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ o
     }
     __syncthreads();
     if (MODE == 0) {
-        const int* ow = own + si.own0;
-        const int* og = own_gid + si.own0;
-        const T add = cz ? cz[s] : (T)0;
-        for (int k0 = threadIdx.x; k0 < si.nown; k0 += U * NT) {
-            int li[U], g[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int k = k0 + u * NT;
-                li[u] = k < si.nown ? ow[k] : -1;
-                g[u] = k < si.nown ? og[k] : -1;
-            }
-            T v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = li[u] >= 0 ? zr[li[u]] : (T)0;
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (g[u] >= 0) y[g[u]] = cz ? (double)(v[u] + add) : (double)v[u];
-        }
+        scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, zr, cz != nullptr,
+                          cz ? cz[s] : (T)0, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += blockDim.x) zout[si.row0 + i] = zr[i];
     }
@@ -849,15 +852,15 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
         itb[l] = bitem[si.blev0 + l];
     }
     const int* d = dofs + si.row0;
-    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 4 * NT) {
-        int g[4];
+    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 8 * NT) {
+        int g[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int i = i0 + u * NT;
             g[u] = i < si.ns ? d[i] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 8; ++u)
             if (g[u] >= 0) {
                 z[i0 + u * NT] = (T)x[g[u]];
                 fl[i0 + u * NT] = 0;
@@ -874,13 +877,8 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
     sweep_syncfree<T, true, NT, D>(z, fl, lb, itb, si.nblev, si.ns, 2, b_row, b_col, b_val, b_diag);
     __syncthreads();
     if (MODE == 0) {
-        const int* ow = own + si.own0;
-        const int* og = own_gid + si.own0;
-        const T add = cz ? cz[s] : (T)0;
-        for (int k = threadIdx.x; k < si.nown; k += NT) {
-            const int li = ow[k], g = og[k];
-            y[g] = cz ? (double)(z[li] + add) : (double)z[li];
-        }
+        scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
+                          cz ? cz[s] : (T)0, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
     }
