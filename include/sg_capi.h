@@ -395,13 +395,15 @@ typedef enum {
 sg_status sg_context_set_profiling(sg_context* ctx, int enable);
 /* Kernel-shape selection.  mode 1 (default): the first SpMV on a matrix and
  * the first apply of a preconditioner time each shape and keep the fastest
- * (SpMV: 256, 512 or 1024 rows per workgroup; subdomain sweeps: 256, 512 or
- * 1024 threads with a barrier per level and row data two levels ahead, 512 /
- * 1024 threads sync-free with per-row flags in LDS, or 256 / 512 threads with a
- * barrier per level and row data five levels ahead).  mode 0: fixed defaults
- * (SpMV 256, sweep 512).  mode 2..8: force sweep shape mode-2 in that order and
- * SpMV shape min(mode-2, 2).  All shapes compute identical values, so the mode never
- * changes results.  No counterpart in the reference (its kernels have fixed
+ * (SpMV: CSR with 256, 512 or 1024 rows per workgroup, or the SELL-64 copy
+ * (slices of 64 rows stored column by column, built when its padding is below
+ * half the nonzeros); subdomain sweeps: 256, 512 or 1024 threads with a
+ * barrier per level and row data two levels ahead, 512 / 1024 threads
+ * sync-free with per-row flags in LDS, or 256 / 512 threads with a barrier per
+ * level and row data five levels ahead).  mode 0: fixed defaults (SpMV CSR
+ * 256, sweep 512).  mode 2..8: force sweep shape mode-2 in that order and SpMV
+ * shape (mode-2) mod 4 (3 = SELL-64, CSR 256 when no SELL copy was built).
+ * All shapes compute identical values, so the mode never changes results.  No counterpart in the reference (its kernels have fixed
  * launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
 sg_status sg_context_set_autotune(sg_context* ctx, int mode);
 /* Tuning cache: the autotuner (mode 1) first looks a matrix / preconditioner

@@ -41,12 +41,14 @@ bool precond_ready(const sg_precond* M);
 constexpr int kT = 256;                 // threads per reduction block
 constexpr int kMaxCols = 128;           // restart + 1
 constexpr int kRedCols = 2 * kMaxCols;  // widest reduction (two dots per basis vector)
-constexpr int kCBD = 8;                 // basis vectors per register block, Gram dots
-constexpr int kCBU = 16;                // basis vectors per register block, Gram update
 constexpr int kEo = 2;                  // elements per thread per tile
 constexpr int kTile = kT * kEo;
-constexpr int kTG = 512;                // threads per block of the Gram passes
-constexpr int kTileG = kTG * kEo;
+constexpr int kCBG = 8;                 // basis vectors per register block of the Gram passes
+constexpr int kEoG = 4;                 // elements per thread per tile of the Gram passes
+constexpr int kTGD = 256;               // threads per block, Gram dots
+constexpr int kTGU = 512;               // threads per block, Gram update
+constexpr int kTileGD = kTGD * kEoG;
+constexpr int kTileGU = kTGU * kEoG;
 constexpr int kTF = 1024;               // threads of the finish kernels
 
 struct KState {
@@ -205,68 +207,85 @@ __device__ void reduce_partials(const Work& W, const SysBlocks& sb, int ncols, d
 // read of V; then h = (2I - G) p, which equals G^{-1} p to first order in the
 // loss of orthogonality, i.e. the coefficients of projecting twice.
 // Pass 2 (k_gram_update + k_norm_finish): w -= V h, ||w||, step epilogue.
-// Columns beyond ncols inside a register block are clamped to a valid column
-// and masked, elements beyond the block to its last element; every load of a
-// tile is issued before the scheduling barrier, so a wave keeps a whole
-// register block of loads in flight.
-__global__ void __launch_bounds__(kTG)
+// Basis vectors go in register blocks of kCBG, the remainder in blocks of 4, 2
+// and 1, so no load is spent on a column beyond ncols.  Elements beyond the
+// block are clamped to its last element and masked; every load of a tile is
+// issued before the scheduling barrier, so a wave keeps a whole register
+// block of loads in flight.
+
+// Dots of columns [cb, cb + CB) with w and v_j over the block's elements.
+template <int CB>
+__device__ __forceinline__ void gram_dot_cols(int start, int end, int cb, int ncols,
+                                              const double* __restrict__ V, size_t ld,
+                                              const double* __restrict__ w,
+                                              const double* __restrict__ vj,
+                                              double (*red)[kRedCols]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double aw[CB], av[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) aw[c] = av[c] = 0.0;
+    for (int base = start + tid; base < end; base += kTileGD) {
+        int ee[kEoG];
+        double msk[kEoG], wr[kEoG], vr[kEoG], a[CB][kEoG];
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) {
+            const int e = base + r * kTGD;
+            ee[r] = min(e, end - 1);
+            msk[r] = e < end ? 1.0 : 0.0;
+            wr[r] = w[ee[r]];
+            vr[r] = vj[ee[r]];
+        }
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) {
+            wr[r] *= msk[r];
+            vr[r] *= msk[r];
+        }
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) {
+                aw[c] += a[c][r] * wr[r];
+                av[c] += a[c][r] * vr[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const double sw = wave_sum(aw[c]);
+        const double sv = wave_sum(av[c]);
+        if (lane == 0) {
+            red[wv][cb + c] = sw;
+            red[wv][ncols + cb + c] = sv;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kTGD)
 k_gram_dot(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
            const double* __restrict__ w, Work W, int j) {
     const BlockRange br = sp.blk[blockIdx.x];
     if (!W.st[br.sys].active) return;
-    __shared__ double red[kTG / 64][kRedCols];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int start = br.start, end = br.end;
+    __shared__ double red[kTGD / 64][kRedCols];
     const double* vj = V + (size_t)j * ld;
-    for (int cb = 0; cb < ncols; cb += kCBD) {
-        const double* Vc[kCBD];
-#pragma unroll
-        for (int c = 0; c < kCBD; ++c) Vc[c] = V + (size_t)min(cb + c, ncols - 1) * ld;
-        double aw[kCBD], av[kCBD];
-#pragma unroll
-        for (int c = 0; c < kCBD; ++c) aw[c] = av[c] = 0.0;
-        for (int base = start + tid; base < end; base += kTileG) {
-            int ee[kEo];
-            double msk[kEo], wr[kEo], vr[kEo], a[kCBD][kEo];
-#pragma unroll
-            for (int r = 0; r < kEo; ++r) {
-                const int e = base + r * kTG;
-                ee[r] = min(e, end - 1);
-                msk[r] = e < end ? 1.0 : 0.0;
-                wr[r] = w[ee[r]];
-                vr[r] = vj[ee[r]];
-            }
-#pragma unroll
-            for (int c = 0; c < kCBD; ++c)
-#pragma unroll
-                for (int r = 0; r < kEo; ++r) a[c][r] = Vc[c][ee[r]];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int r = 0; r < kEo; ++r) {
-                wr[r] *= msk[r];
-                vr[r] *= msk[r];
-            }
-#pragma unroll
-            for (int c = 0; c < kCBD; ++c) {
-#pragma unroll
-                for (int r = 0; r < kEo; ++r) {
-                    aw[c] += a[c][r] * wr[r];
-                    av[c] += a[c][r] * vr[r];
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < kCBD; ++c) {
-            const double sw = wave_sum(aw[c]);
-            const double sv = wave_sum(av[c]);
-            if (lane == 0 && cb + c < ncols) {
-                red[wv][cb + c] = sw;
-                red[wv][ncols + cb + c] = sv;
-            }
-        }
+    int cb = 0;
+    for (; cb + kCBG <= ncols; cb += kCBG)
+        gram_dot_cols<kCBG>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
+    if (ncols - cb >= 4) {
+        gram_dot_cols<4>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
+        cb += 4;
     }
+    if (ncols - cb >= 2) {
+        gram_dot_cols<2>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
+        cb += 2;
+    }
+    if (ncols - cb >= 1) gram_dot_cols<1>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
-    store_partials<kTG>(red, 2 * ncols, W.part, W.pstride);
+    store_partials<kTGD>(red, 2 * ncols, W.part, W.pstride);
 }
 
 __global__ void __launch_bounds__(kTF) k_gram_finish(Work W0, Segs sp, int ncols, int j) {
@@ -295,46 +314,60 @@ __global__ void __launch_bounds__(kTF) k_gram_finish(Work W0, Segs sp, int ncols
     }
 }
 
-__global__ void __launch_bounds__(kTG)
+// wr -= V[:, cb:cb+CB] h[cb:cb+CB] on the thread's elements ee.
+template <int CB>
+__device__ __forceinline__ void gram_update_cols(const double* __restrict__ V, size_t ld, int cb,
+                                                 const int* ee, double* wr, const double* hs) {
+    double a[CB][kEoG];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const double hi = hs[cb + c];
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) wr[r] -= hi * a[c][r];
+    }
+}
+
+__global__ void __launch_bounds__(kTGU)
 k_gram_update(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
               double* __restrict__ w, Work W0) {
     const BlockRange br = sp.blk[blockIdx.x];
     if (!W0.st[br.sys].active) return;
     const Work W = sys_work(W0, br.sys);
-    __shared__ double red[kTG / 64][kRedCols];
-    __shared__ double hs[kMaxCols + kCBU];
+    __shared__ double red[kTGU / 64][kRedCols];
+    __shared__ double hs[kMaxCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < kMaxCols + kCBU; i += kTG) hs[i] = i < ncols ? W.h1[i] : 0.0;
+    for (int i = tid; i < ncols; i += kTGU) hs[i] = W.h1[i];
     __syncthreads();
     const int start = br.start, end = br.end;
+    const int nfull = ncols / kCBG * kCBG;
     double acc = 0.0;
-    for (int base = start + tid; base < end; base += kTileG) {
-        int ee[kEo];
-        double wr[kEo];
+    for (int base = start + tid; base < end; base += kTileGU) {
+        int ee[kEoG];
+        double wr[kEoG];
 #pragma unroll
-        for (int r = 0; r < kEo; ++r) {
-            ee[r] = min(base + r * kTG, end - 1);
+        for (int r = 0; r < kEoG; ++r) {
+            ee[r] = min(base + r * kTGU, end - 1);
             wr[r] = w[ee[r]];
         }
-        for (int cb = 0; cb < ncols; cb += kCBU) {
-            double a[kCBU][kEo];
-#pragma unroll
-            for (int c = 0; c < kCBU; ++c) {
-                const double* Vi = V + (size_t)min(cb + c, ncols - 1) * ld;
-#pragma unroll
-                for (int r = 0; r < kEo; ++r) a[c][r] = Vi[ee[r]];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int c = 0; c < kCBU; ++c) {
-                const double hi = hs[cb + c];  // 0 beyond ncols: subtracts an exact 0
-#pragma unroll
-                for (int r = 0; r < kEo; ++r) wr[r] -= hi * a[c][r];
-            }
+        int cb = 0;
+        for (; cb < nfull; cb += kCBG) gram_update_cols<kCBG>(V, ld, cb, ee, wr, hs);
+        if (ncols - cb >= 4) {
+            gram_update_cols<4>(V, ld, cb, ee, wr, hs);
+            cb += 4;
         }
+        if (ncols - cb >= 2) {
+            gram_update_cols<2>(V, ld, cb, ee, wr, hs);
+            cb += 2;
+        }
+        if (ncols - cb >= 1) gram_update_cols<1>(V, ld, cb, ee, wr, hs);
 #pragma unroll
-        for (int r = 0; r < kEo; ++r) {
-            if (base + r * kTG < end) {
+        for (int r = 0; r < kEoG; ++r) {
+            if (base + r * kTGU < end) {
                 w[ee[r]] = wr[r];
                 acc += wr[r] * wr[r];
             }
@@ -342,7 +375,7 @@ k_gram_update(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
     }
     acc = wave_sum(acc);
     if (lane == 0) red[wv][0] = acc;
-    store_partials<kTG>(red, 1, W.part, W.pstride);
+    store_partials<kTGU>(red, 1, W.part, W.pstride);
 }
 
 __global__ void __launch_bounds__(kTF) k_norm_finish(Work W0, Segs sp, int j) {
@@ -617,11 +650,12 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     if (nsys > 1)  // stopped systems of a batch keep computing, on finite values
         SG_HIP(hipMemsetAsync(V, 0, sizeof(double) * ld * nvec, s));
 
-    SegsHost seg_red, seg_gram;
+    SegsHost seg_red, seg_gd, seg_gu;
     build_segs(off, 4 * ctx->num_cu, kTile, seg_red, s);
-    build_segs(off, 4 * ctx->num_cu, kTileG, seg_gram, s);
-    const Segs sp_red = seg_red.dev(), sp_gram = seg_gram.dev();
-    const int pstride = std::max(seg_red.nblk, seg_gram.nblk);
+    build_segs(off, 4 * ctx->num_cu, kTileGD, seg_gd, s);
+    build_segs(off, 4 * ctx->num_cu, kTileGU, seg_gu, s);
+    const Segs sp_red = seg_red.dev(), sp_gd = seg_gd.dev(), sp_gu = seg_gu.dev();
+    const int pstride = std::max(seg_red.nblk, std::max(seg_gd.nblk, seg_gu.nblk));
 
     const int hcap = prm->max_iter + m + 2;
     const size_t per_sys = (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m + 2 * kRedCols + hcap +
@@ -703,13 +737,13 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         {
             ProfScope ps(ctx, SG_K_ORTHO);
             if (cgs2) {
-                hipLaunchKernelGGL(k_gram_dot, dim3(seg_gram.nblk), dim3(kTG), 0, s, sp_gram,
+                hipLaunchKernelGGL(k_gram_dot, dim3(seg_gd.nblk), dim3(kTGD), 0, s, sp_gd,
                                    j + 1, V, ld, w, W, j);
-                hipLaunchKernelGGL(k_gram_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gram, j + 1,
+                hipLaunchKernelGGL(k_gram_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gd, j + 1,
                                    j);
-                hipLaunchKernelGGL(k_gram_update, dim3(seg_gram.nblk), dim3(kTG), 0, s, sp_gram,
+                hipLaunchKernelGGL(k_gram_update, dim3(seg_gu.nblk), dim3(kTGU), 0, s, sp_gu,
                                    j + 1, V, ld, w, W);
-                hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gram, j);
+                hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
             } else {
                 for (int i = 0; i <= j + 1; ++i) {
                     const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;

@@ -129,7 +129,13 @@ struct sg_csr {
     sg::DevBuf<int> blk_sys_v[3];  // row block -> system (blocks never straddle systems)
     // independent systems of a block-diagonal batch: rows [sys_off[i], sys_off[i+1])
     std::vector<int> sys_off;
-    int variant = 0;     // selected block shape
+    // SELL-64 copy (built when its padding is small): slice s holds rows
+    // [sell_row[s], sell_row[s+1]) (at most 64, never straddling a system), its
+    // entry k of lane l at sell_ptr[s] + 64 k + l; padding entries have column -1
+    int nsl = 0;
+    sg::DevBuf<int> sell_ptr, sell_row, sell_sys, sell_ci;
+    sg::DevBuf<double> sell_v;
+    int variant = 0;     // selected shape: 0..2 row-block CSR, 3 SELL-64
     bool tuned = false;
 };
 

@@ -9,6 +9,16 @@
 // (src/core/sparse_matrix.h:30-37) bit for bit, whatever NT is.  Blocks are
 // remapped so that each of the 8 XCDs walks one contiguous eighth of the rows,
 // which keeps the x neighbourhood of a stencil (+-nx*ny rows) inside that XCD's L2.
+//
+// SELL-64 (sliced ELLPACK, one wavefront per slice of 64 rows): a copy of the
+// matrix with the entries of a slice stored column by column, so lane l owns
+// row l and loads its k-th entry next to lane l+1's.  Value and column loads
+// are fully coalesced, the x gathers of a stencil are nearly contiguous across
+// the wave (consecutive rows read consecutive neighbours), and each lane adds
+// its row's products in CSR order in a register: no LDS staging, no barrier,
+// and the same sum as the serial row loop bit for bit.  Padding entries (column
+// -1) are skipped, not added as zeros, so signed zeros and non-finite x values
+// come out as in the serial loop.
 #include "sg_internal.h"
 
 #include <algorithm>
@@ -22,6 +32,15 @@ constexpr int kNnzPerThread = 8;  // LDS staging capacity: 8 products per thread
 // Block shapes (threads = rows per block); results are identical for all of
 // them, so the choice is a pure performance knob (see spmv_autotune).
 constexpr int kSpmvVariants[3] = {256, 512, 1024};
+constexpr int kSellVariant = 3;   // shape index of the SELL-64 kernel
+constexpr int kNumShapes = 4;
+constexpr int kSellC = 64;        // rows per slice: one wavefront
+constexpr int kSellThreads = 256; // four slices per workgroup
+// The SELL copy is built only when its padding stays below half the nonzeros
+// and no row is longer than the CSR kernels' staging capacity (beyond it they
+// switch to a tree sum, which the SELL kernel would not reproduce).
+constexpr double kSellMaxFill = 1.5;
+constexpr int kSellMaxRow = 256 * kNnzPerThread;
 
 template <int NT>
 __global__ void __launch_bounds__(NT)
@@ -77,6 +96,80 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
     }
 }
 
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_spmv_sell(const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
+            const double* __restrict__ v, int nsl, int per_xcd, double alpha,
+            const double* __restrict__ x, double beta, const double* __restrict__ y0,
+            double* __restrict__ y, DevGuard guard) {
+    constexpr int kSpb = NT / kSellC;
+    const int b = (blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
+    const int sl = b * kSpb + (int)(threadIdx.x / kSellC), lane = threadIdx.x % kSellC;
+    if (sl >= nsl || guard_stop(guard, sl)) return;
+    const int p0 = sp[sl], width = (sp[sl + 1] - p0) / kSellC;
+    const int r = srow[sl] + lane;
+    double sum = 0.0;
+    for (int k0 = 0; k0 < width; k0 += 8) {
+        double a[8];
+        int c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = p0 + min(k0 + u, width - 1) * kSellC + lane;
+            a[u] = __builtin_nontemporal_load(&v[idx]);
+            c[u] = __builtin_nontemporal_load(&ci[idx]);
+        }
+        double xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[max(c[u], 0)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < width && c[u] >= 0) sum += a[u] * xv[u];
+    }
+    if (r < srow[sl + 1]) y[r] = beta == 0.0 ? alpha * sum : alpha * sum + beta * y0[r];
+}
+
+// SELL-64 copy of A, slices cut per system; false (nothing built) when the
+// padding or a row length is beyond the limits above.
+static bool sell_build(sg_csr* A) {
+    const std::vector<int>& rp = A->host.rp;
+    if (A->nnz == 0 || A->ncols == 0) return false;
+    std::vector<int> sp(1, 0), srow, ssys;
+    long long total = 0;
+    for (size_t q = 0; q + 1 < A->sys_off.size(); ++q) {
+        for (int r0 = A->sys_off[q]; r0 < A->sys_off[q + 1]; r0 += kSellC) {
+            const int r1 = std::min(A->sys_off[q + 1], r0 + kSellC);
+            int w = 0;
+            for (int r = r0; r < r1; ++r) w = std::max(w, rp[r + 1] - rp[r]);
+            if (w > kSellMaxRow) return false;
+            total += (long long)kSellC * w;
+            if (total > kSellMaxFill * A->nnz || total > INT32_MAX) return false;
+            sp.push_back((int)total);
+            srow.push_back(r0);
+            ssys.push_back((int)q);
+        }
+    }
+    srow.push_back(A->nrows);
+    const int nsl = (int)ssys.size();
+    std::vector<int> sci((size_t)total, -1);
+    std::vector<double> sv((size_t)total, 0.0);
+    for (int sl = 0; sl < nsl; ++sl)
+        for (int r = srow[sl]; r < srow[sl + 1]; ++r)
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                const size_t idx = (size_t)sp[sl] + (size_t)(k - rp[r]) * kSellC + (r - srow[sl]);
+                sci[idx] = A->host.ci[k];
+                sv[idx] = A->host.v[k];
+            }
+    hipStream_t s = A->ctx->stream;
+    A->nsl = nsl;
+    A->sell_ptr.upload(sp, s);
+    A->sell_row.upload(srow, s);
+    A->sell_sys.upload(ssys, s);
+    A->sell_ci.upload(sci, s);
+    A->sell_v.upload(sv, s);
+    SG_HIP(hipStreamSynchronize(s));  // the host vectors end here
+    return true;
+}
+
 // Row blocks of at most nt rows and nt*kNnzPerThread nonzeros (a longer row
 // gets a block of its own); no block crosses a system boundary of a batch.
 static void row_blocks(const std::vector<int>& rp, const std::vector<int>& sys_off, int nt,
@@ -107,11 +200,25 @@ void spmv_prepare(sg_csr* A) {
         A->blk_sys_v[v].upload(bsys, A->ctx->stream);
         SG_HIP(hipStreamSynchronize(A->ctx->stream));
     }
+    sell_build(A);
     A->variant = 0;
+}
+
+static bool shape_available(const sg_csr* A, int var) {
+    return var >= 0 && var < kNumShapes && (var != kSellVariant || A->nsl > 0);
 }
 
 static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* x, double beta,
                                 const double* y0, double* y, const Guard& g, hipStream_t s) {
+    if (var == kSellVariant) {
+        const int nb = (A->nsl + kSellThreads / kSellC - 1) / (kSellThreads / kSellC);
+        const int per = (nb + kXcd - 1) / kXcd;
+        hipLaunchKernelGGL(k_spmv_sell<kSellThreads>, dim3(per * kXcd), dim3(kSellThreads), 0, s,
+                           A->sell_ptr.p, A->sell_row.p, A->sell_ci.p, A->sell_v.p, A->nsl, per,
+                           alpha, x, beta, y0, y, dev_guard(g, A->sell_sys.p));
+        SG_HIP(hipGetLastError());
+        return;
+    }
     const int nblk = A->nblk_v[var];
     const DevGuard guard = dev_guard(g, A->blk_sys_v[var].p);
     const int per = (nblk + kXcd - 1) / kXcd;
@@ -133,8 +240,8 @@ static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* 
     SG_HIP(hipGetLastError());
 }
 
-// Times every block shape on this matrix (HIP events, three launches each)
-// and keeps the fastest; ties keep the smaller shape.
+// Times every shape on this matrix (HIP events, three launches each) and
+// keeps the fastest; ties keep the earlier shape.
 void spmv_autotune(sg_csr* A) {
     sg_context* ctx = A->ctx;
     hipStream_t s = ctx->stream;
@@ -144,7 +251,7 @@ void spmv_autotune(sg_csr* A) {
     char key[96];
     std::snprintf(key, sizeof key, "spmv:%d:%d:%016llx", A->nrows, A->nnz, (unsigned long long)h);
     const int cached = tune_lookup(ctx, key);
-    if (cached >= 0 && cached < 3) {
+    if (shape_available(A, cached)) {
         A->variant = cached;
         A->tuned = true;
         return;
@@ -158,7 +265,8 @@ void spmv_autotune(sg_csr* A) {
     SG_HIP(hipEventCreate(&e1));
     float best = 0.f;
     int bestv = 0;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < kNumShapes; ++v) {
+        if (!shape_available(A, v)) continue;
         spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, Guard{}, s);  // warm
         SG_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 3; ++r) spmv_launch_variant(A, v, 1.0, x.p, 0.0, nullptr, y.p, Guard{}, s);
@@ -181,8 +289,12 @@ void spmv_autotune(sg_csr* A) {
 void spmv_ensure_tuned(sg_csr* A) {
     if (A->nrows == 0) return;
     if (A->ctx->autotune == 1 && !A->tuned) spmv_autotune(A);
-    if (A->ctx->autotune >= 2) A->variant = std::min(A->ctx->autotune - 2, 2);
-    else if (A->ctx->autotune == 0) A->variant = 0;
+    if (A->ctx->autotune >= 2) {
+        const int v = (A->ctx->autotune - 2) % kNumShapes;
+        A->variant = shape_available(A, v) ? v : 0;
+    } else if (A->ctx->autotune == 0) {
+        A->variant = 0;
+    }
 }
 
 void spmv_launch(sg_csr* A, double alpha, const double* x, double beta, const double* y0,

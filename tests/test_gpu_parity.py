@@ -59,6 +59,63 @@ def test_spmv_bitwise(sg, ctx, spec):
     assert np.array_equal(db.get(), b - O.spmv(Ao, x))
 
 
+def _spmv_cases():
+    rng = np.random.default_rng(7)
+    out = []
+    A = O.grid("3d:40:36:30")
+    out.append(("3d", A.rp, A.ci, A.v * rng.uniform(0.5, 1.5, A.nnz), None))
+    A = O.grid("rnd:3000:5")
+    out.append(("rnd", A.rp, A.ci, A.v, None))
+    # ragged rows (0..12 entries, unsorted lengths) and a batch of systems
+    # whose sizes leave partial 64-row slices at every system end
+    sizes = [70, 1, 130, 64, 257]
+    rows, off = [], 0
+    for m in sizes:
+        for i in range(m):
+            k = int(rng.integers(0, min(m, 12) + 1))
+            rows.append(sorted(off + rng.choice(m, size=k, replace=False)))
+        off += m
+    rp = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    ci = np.concatenate([np.array(r, np.int32) for r in rows])
+    out.append(("ragged_batch", rp, ci, rng.standard_normal(len(ci)), np.cumsum([0] + sizes)))
+    return out
+
+
+@pytest.mark.parametrize("case", _spmv_cases(), ids=lambda c: c[0])
+def test_spmv_every_shape_bitwise(sg, case):
+    """Each SpMV shape (CSR with 256 / 512 / 1024 rows per workgroup and the
+    SELL-64 copy, forced through autotune modes 2..5) equals the serial row loop
+    bit for bit, signed zeros included, in the y = A x and y = b - A x forms."""
+    _, rp, ci, v, sys_rows = case
+    n = len(rp) - 1
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(n)
+    x[::7] = -0.0
+    b = rng.standard_normal(n)
+    Ao = O.Csr(rp, ci, v)
+    ref1, ref2 = O.spmv(Ao, x), b - O.spmv(Ao, x)
+    for mode in (2, 3, 4, 5):
+        c = sg.Context(0)
+        c.autotune(mode)
+        if sys_rows is None:
+            dA = sg.Csr(c, rp, ci, v)
+        else:
+            mats = []
+            for q in range(len(sys_rows) - 1):
+                a, e = sys_rows[q], sys_rows[q + 1]
+                lrp = rp[a:e + 1] - rp[a]
+                mats.append((lrp, ci[rp[a]:rp[e]] - a, v[rp[a]:rp[e]]))
+            dA = sg.BatchCsr(c, mats)
+        dx, dy = dev(sg, c, x), dev(sg, c, np.zeros(n))
+        dA.spmv(dx, dy)
+        assert np.array_equal(dy.get().view(np.int64), ref1.view(np.int64)), mode
+        db = dev(sg, c, b)
+        dA.spmv(dx, db, alpha=-1.0, beta=1.0)
+        assert np.array_equal(db.get().view(np.int64), ref2.view(np.int64)), mode
+        del dA, dx, dy, db
+        c.close()
+
+
 def test_spmv_long_rows(sg, ctx):
     # one dense row longer than the LDS staging capacity + empty rows
     n = 5000
