@@ -537,7 +537,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--autotune", type=int, default=1,
                     help="kernel-shape mode (sg_context_set_autotune): 1 tune, 0 defaults, "
-                         "2..8 force a shape")
+                         "2..11 force a shape")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling aid: after setup run this many preconditioner applies and "
                          "SpMVs on the workload, print their mean times and exit")

@@ -186,7 +186,9 @@ typedef struct {
     int max_ns;       /* rows of the largest (overlapped) subdomain            */
     int sweep_shape;  /* sweep kernel shape in use (see sg_context_set_autotune:
                          0..2 barrier per level with 256/512/1024 threads,
-                         3..4 sync-free with 512/1024), -1 before the first apply */
+                         3..4 sync-free with 512/1024, 5..6 barrier with deeper
+                         prefetch, 7..9 staged through LDS with 1024/512/256),
+                         -1 before the first apply */
 } sg_precond_info;
 sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info);
 
@@ -399,10 +401,12 @@ sg_status sg_context_set_profiling(sg_context* ctx, int enable);
  * (slices of 64 rows stored column by column, built when its padding is below
  * half the nonzeros); subdomain sweeps: 256, 512 or 1024 threads with a
  * barrier per level and row data two levels ahead, 512 / 1024 threads
- * sync-free with per-row flags in LDS, or 256 / 512 threads with a barrier per
- * level and row data five levels ahead).  mode 0: fixed defaults (SpMV CSR
- * 256, sweep 512).  mode 2..8: force sweep shape mode-2 in that order and SpMV
- * shape (mode-2) mod 4 (3 = SELL-64, CSR 256 when no SELL copy was built).
+ * sync-free with per-row flags in LDS, 256 / 512 threads with a barrier per
+ * level and row data five levels ahead, or 1024 / 512 / 256 threads with the
+ * factor data staged through LDS a chunk of levels ahead).  mode 0: fixed
+ * defaults (SpMV CSR 256, sweep 512).  mode 2..11: force sweep shape mode-2 in
+ * that order and SpMV shape (mode-2) mod 4 (3 = SELL-64, CSR 256 when no SELL
+ * copy was built).
  * All shapes compute identical values, so the mode never changes results.  No counterpart in the reference (its kernels have fixed
  * launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
 sg_status sg_context_set_autotune(sg_context* ctx, int mode);

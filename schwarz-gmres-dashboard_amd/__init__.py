@@ -201,7 +201,7 @@ class Context:
         self.check(lib().sg_context_set_profiling(self.h, 1 if on else 0))
 
     def autotune(self, mode=1):
-        """0 fixed shapes, 1 time shapes on first use, 2..8 force a shape (see
+        """0 fixed shapes, 1 time shapes on first use, 2..11 force a shape (see
         sg_context_set_autotune)."""
         self.check(lib().sg_context_set_autotune(self.h, int(mode)))
 

@@ -354,7 +354,7 @@ sg_status sg_context_set_profiling(sg_context* ctx, int enable) {
 }
 
 sg_status sg_context_set_autotune(sg_context* ctx, int enable) {
-    if (!ctx || enable < 0 || enable > 8) return SG_E_ARG;
+    if (!ctx || enable < 0 || enable > 11) return SG_E_ARG;
     ctx->autotune = enable;
     return SG_OK;
 }

@@ -44,15 +44,21 @@ constexpr int kPadRows = 1024;      // padding of the sweep arrays (largest swee
 // 512/1024 threads when z and the flags fit in LDS.
 // depth: levels of row data in flight per thread in the barrier sweeps (the
 // row data of level l + depth - 1 is loaded while level l computes)
+// staged: factor data copied chunk by chunk into LDS ahead of the levels
+// that use them (k_sweep_staged), 1024, 512 or 256 threads.
 struct SweepShape {
     int threads;
     bool syncfree;
     int depth;
+    bool staged;
 };
-constexpr int kNumSweepShapes = 7;
+constexpr int kNumSweepShapes = 10;
 constexpr SweepShape kSweepShapes[kNumSweepShapes] = {
-    {256, false, 3}, {512, false, 3}, {1024, false, 3}, {512, true, 3}, {1024, true, 3},
-    {256, false, 6}, {512, false, 6}};
+    {256, false, 3, false}, {512, false, 3, false}, {1024, false, 3, false},
+    {512, true, 3, false},  {1024, true, 3, false}, {256, false, 6, false},
+    {512, false, 6, false}, {1024, false, 3, true}, {512, false, 3, true},
+    {256, false, 3, true}};
+constexpr int kStageBytes = 32768;  // largest LDS buffer of the staged sweep (two are used)
 constexpr int kKreg = 4;  // factor entries per row prefetched in registers
 constexpr int kRestrictSplit = 8;  // workgroups per subdomain in the coarse restriction
 constexpr int kSfDepth = SG_SF_DEPTH;  // items of row data in flight per wave (sync-free sweep)
@@ -73,6 +79,14 @@ struct SubInfo {
 // list; entry k of the row at position start + q sits at ell + k * size + q.
 struct Level {
     int start, size, K, ell;
+};
+
+// A chunk of consecutive levels of one sweep of one subdomain, staged into LDS
+// as a unit: level-ordered positions [row0, row0 + nrow), ELL entries
+// [ell0, ell0 + nell) (global indices into the concatenated arrays); levels
+// [lev0, lev0 + nlev) of the subdomain's own level table; bwd: backward sweep.
+struct SChunk {
+    int lev0, nlev, row0, nrow, ell0, nell, bwd, img;  // img: offset in the chunk image, 16-byte units
 };
 
 struct Plan {
@@ -132,6 +146,14 @@ struct sg_precond {
     int max_ns = 0;
     size_t sf_item_off = 0, sf_z_off = 0, sf_flag_off = 0, sf_bytes = 0;
     sg::DevBuf<int> d_fitem, d_bitem;
+    // staged sweep: chunks of every subdomain (forward ones first), two LDS
+    // buffers of st_buf bytes after the level tables and z
+    bool st_ok = false;
+    int st_buf = 0;
+    size_t st_buf_off = 0, st_bytes = 0;
+    sg::DevBuf<sg::SChunk> d_chunks;
+    sg::DevBuf<uint4> d_img;   // every chunk in its LDS layout, 16-byte aligned
+    sg::DevBuf<int> d_chunk0;  // per subdomain: first chunk, then the count (2 ints)
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
     sg::DevBuf<sg::Level> flev, blev;
@@ -424,6 +446,25 @@ __device__ __forceinline__ void sweep(T* z, const Level* lev, int nlev, int zslo
     }
 }
 
+// R_s x of one subdomain: z[i] = x[d[i]], i < ns.  Every index and value
+// load is issued unconditionally (indices past the end are clamped to the last
+// row), so each thread keeps eight of each in flight.
+template <int NT, typename T, typename ZP, int U = 8>
+__device__ __forceinline__ void gather_x(const int* __restrict__ d, int ns,
+                                         const double* __restrict__ x, ZP z) {
+    for (int i0 = threadIdx.x; i0 < ns; i0 += U * NT) {
+        int g[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) g[u] = d[min(i0 + u * NT, ns - 1)];
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[g[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i0 + u * NT < ns) z[i0 + u * NT] = (T)xv[u];
+    }
+}
+
 // RAS prolongation of one subdomain: y[og[k]] = z[ow[k]] (+ add) over its
 // owned rows, eight per thread in flight.
 template <int NT, typename T, typename ZP>
@@ -486,19 +527,7 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         lb = tb;
     }
     T* z = ZLDS ? reinterpret_cast<T*>(lds_raw + lds_z_off) : zglob + si.zoff;
-    const int* d = dofs + si.row0;
-    // R_s x: eight independent index/value loads in flight per thread
-    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 8 * NT) {
-        int g[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = i0 + u * NT;
-            g[u] = i < si.ns ? d[i] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (g[u] >= 0) z[i0 + u * NT] = (T)x[g[u]];
-    }
+    gather_x<NT, T>(dofs + si.row0, si.ns, x, z);
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     __syncthreads();
     sweep<T, false, ZLDS, D>(z, lf, si.nflev, si.ns, f_row, f_col, f_val, nullptr);
@@ -685,20 +714,21 @@ k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ o
     const int pb0 = si.nblev ? blev[si.blev0].start : 0;
     const int* d = dofs + si.row0;
     // R_s x in forward level order: zf[p] = x[dofs[row(p)]], eight positions
-    // per thread in flight
+    // per thread in flight (every load issued; positions past the end are
+    // clamped to the last one)
     constexpr int U = 8;
     for (int p0 = threadIdx.x; p0 < si.ns; p0 += U * NT) {
         int r[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int p = p0 + u * NT;
-            r[u] = p < si.ns ? f_row[pf0 + p] : -1;
-        }
+        for (int u = 0; u < U; ++u) r[u] = f_row[pf0 + min(p0 + u * NT, si.ns - 1)];
 #pragma unroll
-        for (int u = 0; u < U; ++u) r[u] = r[u] >= 0 ? d[r[u]] : -1;
+        for (int u = 0; u < U; ++u) r[u] = d[r[u]];
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[r[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (r[u] >= 0) zf[p0 + u * NT] = (T)x[r[u]];
+            if (p0 + u * NT < si.ns) zf[p0 + u * NT] = (T)xv[u];
     }
     if (threadIdx.x == 0) ring[R] = (T)0;
     __syncthreads();
@@ -851,21 +881,8 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
         lb[l] = blev[si.blev0 + l];
         itb[l] = bitem[si.blev0 + l];
     }
-    const int* d = dofs + si.row0;
-    for (int i0 = threadIdx.x; i0 < si.ns; i0 += 8 * NT) {
-        int g[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = i0 + u * NT;
-            g[u] = i < si.ns ? d[i] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (g[u] >= 0) {
-                z[i0 + u * NT] = (T)x[g[u]];
-                fl[i0 + u * NT] = 0;
-            }
-    }
+    gather_x<NT, T>(dofs + si.row0, si.ns, x, z);
+    for (int i = threadIdx.x; i < si.ns; i += NT) fl[i] = 0;
     if (threadIdx.x == 0) {
         z[si.ns] = (T)0;  // zero slot for padding entries, always ready
         fl[si.ns] = 3;
@@ -875,6 +892,285 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
                                     nullptr);
     __syncthreads();
     sweep_syncfree<T, true, NT, D>(z, fl, lb, itb, si.nblev, si.ns, 2, b_row, b_col, b_val, b_diag);
+    __syncthreads();
+    if (MODE == 0) {
+        scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
+                          cz ? cz[s] : (T)0, y);
+    } else {
+        for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
+    }
+}
+
+// ---- staged sweep: factor data through LDS, chunk by chunk ---------------------
+// For subdomains whose solution vector fits in LDS.  The host cuts each sweep
+// into chunks of consecutive levels whose row data fits one LDS buffer; the
+// rows, column indices, values and (backward) diagonals of a chunk are
+// contiguous in the level-ordered arrays.  While the levels of chunk c run
+// entirely out of LDS (row data and z; an LDS-only barrier closes each
+// level), chunk c + 1 is in flight as a few coalesced 32-bit loads per thread,
+// and is written to the other buffer afterwards.  Global memory latency is
+// paid once per chunk instead of on every level's critical path.  Rows and
+// the operations of a row run in the order of the level-synchronous sweep, so
+// the values are the same bit for bit.
+//
+// Buffer layout of a chunk (32-bit words): rows [0, nrow), columns
+// [nrow, nrow + nell), values from stage_val_word() (8-byte aligned for
+// double), then the diagonals (backward sweep).
+template <typename T>
+__host__ __device__ inline int stage_val_word(int nrow, int nell) {
+    const int a = nrow + nell;
+    return sizeof(T) == 8 ? (a + 1) & ~1 : a;
+}
+template <typename T>
+__host__ __device__ inline int stage_words(int nrow, int nell, bool bwd) {
+    constexpr int w = sizeof(T) / 4;
+    return stage_val_word<T>(nrow, nell) + w * nell + (bwd ? w * nrow : 0);
+}
+
+// 16 bytes as clang vector types (valid in every address space, loads and
+// stores of one dwordx4 / ds_read_b128 / ds_write_b128).
+typedef unsigned stage_q __attribute__((ext_vector_type(4)));
+typedef int stage_i4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int NT>
+struct StageRegs {
+    static constexpr int kV = kStageBytes / 16 / NT;  // 16-byte words per thread
+    stage_q v[kV];
+};
+
+// Number of 16-byte words of chunk c in the image.
+template <typename T>
+__host__ __device__ inline int stage_quads(const SChunk& c) {
+    return (stage_words<T>(c.nrow, c.nell, c.bwd != 0) + 3) / 4;
+}
+
+// Issues the 16-byte loads of chunk c from its image (nothing touches the
+// registers until stage_store); indices past the chunk are clamped to its
+// last word, so every load is issued and the waits stay exact.
+template <typename T, int NT>
+__device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
+                                           const stage_q* __restrict__ img) {
+    const __attribute__((address_space(1))) stage_q* src =
+        (const __attribute__((address_space(1))) stage_q*)(img + c.img);
+    const int n = stage_quads<T>(c);
+#pragma unroll
+    for (int i = 0; i < StageRegs<T, NT>::kV; ++i)
+        R.v[i] = src[min((int)threadIdx.x + i * NT, n - 1)];
+}
+
+// Words past the chunk go to a trash word (straight-line code: a branch here
+// would make the compiler wait for every load in flight, the other set's too).
+template <typename T, int NT>
+__device__ __forceinline__ void stage_store(const StageRegs<T, NT>& R, const SChunk& c,
+                                            lds_t<stage_q>* buf, lds_t<stage_q>* trash) {
+    const int n = stage_quads<T>(c);
+#pragma unroll
+    for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
+        const int u = (int)threadIdx.x + i * NT;
+        *(u < n ? buf + u : trash) = R.v[i];
+    }
+}
+
+// Setup: writes every chunk into the image in its LDS layout (one workgroup
+// per chunk); the alignment word before double values is zero.
+template <typename T>
+__global__ void k_stage_image(int nch, const SChunk* __restrict__ chunks,
+                              const int* __restrict__ f_row, const int* __restrict__ f_col,
+                              const T* __restrict__ f_val, const int* __restrict__ b_row,
+                              const int* __restrict__ b_col, const T* __restrict__ b_val,
+                              const T* __restrict__ b_diag, uint4* __restrict__ img) {
+    constexpr int w = sizeof(T) / 4;
+    const SChunk c = chunks[blockIdx.x];
+    const unsigned* rows = (const unsigned*)((c.bwd ? b_row : f_row) + c.row0);
+    const unsigned* cols = (const unsigned*)((c.bwd ? b_col : f_col) + c.ell0);
+    const unsigned* vals = (const unsigned*)((c.bwd ? b_val : f_val) + c.ell0);
+    const unsigned* diag = (const unsigned*)(b_diag + c.row0);
+    const int a1 = c.nrow, a2 = a1 + c.nell;
+    const int vw = stage_val_word<T>(c.nrow, c.nell);
+    const int a3 = vw + w * c.nell;
+    const int W = 4 * stage_quads<T>(c);
+    unsigned* out = (unsigned*)(img + c.img);
+    for (int u = threadIdx.x; u < W; u += blockDim.x) {
+        unsigned v = 0;
+        if (u < a1) v = rows[u];
+        else if (u < a2) v = cols[u - a1];
+        else if (u >= vw && u < a3) v = vals[u - vw];
+        else if (c.bwd && u >= a3 && u < a3 + w * c.nrow) v = diag[u - a3];
+        out[u] = v;
+    }
+}
+
+// One row of a level, read from the chunk buffer: local row, column indices
+// and values of the first kKreg entries, diagonal (backward sweep).
+template <typename T>
+struct StageRow {
+    int row;
+    int col[kKreg];
+    T val[kKreg];
+    T diag;
+};
+
+// Branch-free read of row q (< size, size >= 1) of a level: entries k >= K
+// become column zs (the zero slot) with value 0, so they subtract an exact 0
+// as the padding of the other sweeps does.  Every read is issued (measured:
+// cheaper than skipping the unused ones behind a scalar branch).
+template <typename T, bool BWD>
+__device__ __forceinline__ void stage_fetch(StageRow<T>& r, int q, int size, int K, int rs,
+                                            int es, int zs, const lds_t<int>* rws,
+                                            const lds_t<int>* cls, const lds_t<T>* vls,
+                                            const lds_t<T>* dgs) {
+    r.row = rws[rs + q];
+#pragma unroll
+    for (int k = 0; k < kKreg; ++k) {
+        const int idx = k < K ? es + k * size + q : 0;
+        const int c = cls[idx];
+        const T v = vls[idx];
+        r.col[k] = k < K ? c : zs;
+        r.val[k] = k < K ? v : (T)0;
+    }
+    if (BWD) r.diag = dgs[rs + q];
+}
+
+// The levels of chunk c out of LDS; each level ends with an LDS-only barrier.
+// Thread t takes row t of a level (rows t + NT, ... of a wider level in a
+// second loop).  The next level's descriptor and the thread's row of it are
+// read while the current level computes (they do not depend on z).  A
+// wavefront with no row in a level skips it (a uniform branch): a level then
+// costs the SIMDs only the instructions of the wavefronts that hold its rows.
+// Threads past a level's end inside an active wavefront compute a clamped row
+// and write it to the trash slot zs + 1.
+template <typename T, bool BWD, int NT>
+__device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
+                                             const SChunk& c, const lds_t<unsigned>* buf,
+                                             int zs) {
+    const lds_t<int>* rws = (const lds_t<int>*)buf;
+    const lds_t<int>* cls = rws + c.nrow;
+    const lds_t<T>* vls = (const lds_t<T>*)(buf + stage_val_word<T>(c.nrow, c.nell));
+    const lds_t<T>* dgs = vls + c.nell;
+    const int t = threadIdx.x;
+    const int wbase = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
+    const int lend = c.lev0 + c.nlev;
+    // level descriptors: d (current, uniform), dn (next, uniform), and the raw
+    // LDS values of the one after, read while the current level computes
+    struct D {
+        int size, K, rs, es;
+    };
+    auto uni = [&](int a, int b, int e, int f) {
+        return D{__builtin_amdgcn_readfirstlane(b), __builtin_amdgcn_readfirstlane(e),
+                 __builtin_amdgcn_readfirstlane(a) - c.row0,
+                 __builtin_amdgcn_readfirstlane(f) - c.ell0};
+    };
+    const int l1 = min(c.lev0 + 1, lend - 1);
+    D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1], tab[4 * c.lev0 + 2], tab[4 * c.lev0 + 3]);
+    D dn = uni(tab[4 * l1], tab[4 * l1 + 1], tab[4 * l1 + 2], tab[4 * l1 + 3]);
+    StageRow<T> cur;
+    if (wbase < d.size)
+        stage_fetch<T, BWD>(cur, min(t, d.size - 1), d.size, d.K, d.rs, d.es, zs, rws, cls, vls,
+                            dgs);
+    for (int l = c.lev0; l < lend; ++l) {
+        // this level's z values first: they are the critical path
+        T zk[kKreg], acc = (T)0;
+        if (wbase < d.size) {
+#pragma unroll
+            for (int k = 0; k < kKreg; ++k) zk[k] = z[cur.col[k]];
+            acc = z[cur.row];
+        }
+        // then the next level's rows and the descriptor after it
+        StageRow<T> nxt;
+        if (wbase < dn.size)
+            stage_fetch<T, BWD>(nxt, min(t, dn.size - 1), dn.size, dn.K, dn.rs, dn.es, zs, rws,
+                                cls, vls, dgs);
+        const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
+        if (wbase < d.size) {
+#pragma unroll
+            for (int k = 0; k < kKreg; ++k) acc -= cur.val[k] * zk[k];
+            const int qc = min(t, d.size - 1);
+            for (int k = kKreg; k < d.K; ++k)
+                acc -= vls[d.es + k * d.size + qc] * z[cls[d.es + k * d.size + qc]];
+            if (BWD) acc = acc / cur.diag;
+            z[t < d.size ? cur.row : zs + 1] = acc;
+            if (d.size > NT) {
+                for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
+                    StageRow<T> r;
+                    stage_fetch<T, BWD>(r, q, d.size, d.K, d.rs, d.es, zs, rws, cls, vls, dgs);
+                    T a = z[r.row];
+#pragma unroll
+                    for (int k = 0; k < kKreg; ++k) a -= r.val[k] * z[r.col[k]];
+                    for (int k = kKreg; k < d.K; ++k)
+                        a -= vls[d.es + k * d.size + q] * z[cls[d.es + k * d.size + q]];
+                    if (BWD) a = a / r.diag;
+                    z[r.row] = a;
+                }
+            }
+        }
+        lds_barrier();
+        d = dn;
+        dn = uni(dq.x, dq.y, dq.z, dq.w);
+        cur = nxt;
+    }
+}
+
+// LDS: level tables (forward | backward, four ints per level) | z (ns values,
+// zero slot, trash slot) | buffer 0 | buffer 1 | trash word.
+template <typename T, int MODE, int NT>
+__global__ void __launch_bounds__(NT)
+k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
+               const int* __restrict__ chunk0, const SChunk* __restrict__ chunks,
+               const stage_q* __restrict__ img, const Level* __restrict__ flev,
+               const Level* __restrict__ blev, const int* __restrict__ dofs,
+               const int* __restrict__ own, const int* __restrict__ own_gid,
+               const double* __restrict__ x, double* __restrict__ y, T* __restrict__ zout,
+               const T* __restrict__ cz, DevGuard guard, int z_off, int buf_off,
+               int buf_words) {
+    extern __shared__ unsigned char lds_raw[];
+    const int s = order[blockIdx.x];
+    if (guard_stop(guard, s)) return;
+    const SubInfo si = subs[s];
+    const int c0 = chunk0[2 * s], nch = chunk0[2 * s + 1];
+    lds_t<int>* lf = (lds_t<int>*)lds_raw;
+    lds_t<int>* lb = lf + 4 * si.nflev;
+    lds_t<T>* z = (lds_t<T>*)(lds_raw + z_off);
+    lds_t<unsigned>* bufs = (lds_t<unsigned>*)(lds_raw + buf_off);
+    {
+        const int* tf = (const int*)(flev + si.flev0);
+        const int* tb = (const int*)(blev + si.blev0);
+        for (int l = threadIdx.x; l < 4 * si.nflev; l += NT) lf[l] = tf[l];
+        for (int l = threadIdx.x; l < 4 * si.nblev; l += NT) lb[l] = tb[l];
+    }
+    // two register sets: while chunk c computes, chunks c + 1 and c + 2 are in
+    // flight (set (c + 1) & 1 and set c & 1).  Loads past the last chunk reload
+    // the last one, so every step issues the same loads and the waits stay exact.
+    StageRegs<T, NT> RA, RB;
+    auto chunk_at = [&](int k) { return chunks[c0 + min(k, nch - 1)]; };
+    auto load = [&](StageRegs<T, NT>& R, int k) { stage_load<T, NT>(R, chunk_at(k), img); };
+    load(RA, 0);
+    load(RB, 1);
+    gather_x<NT, T, lds_t<T>*, 24>(dofs + si.row0, si.ns, x, z);
+    if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
+    lds_t<stage_q>* trash = (lds_t<stage_q>*)(bufs + 2 * buf_words);
+    lds_t<stage_q>* q0 = (lds_t<stage_q>*)bufs;
+    lds_t<stage_q>* q1 = (lds_t<stage_q>*)(bufs + buf_words);
+    stage_store<T, NT>(RA, chunk_at(0), q0, trash);
+    load(RA, 2);
+    __syncthreads();
+    auto run = [&](int k, const lds_t<unsigned>* buf) {
+        const SChunk c = chunk_at(k);
+        if (c.bwd) stage_levels<T, true, NT>(z, lb, c, buf, si.ns);
+        else stage_levels<T, false, NT>(z, lf, c, buf, si.ns);
+    };
+    lds_t<unsigned>* buf1 = bufs + buf_words;
+    for (int ci = 0; ci < nch; ci += 2) {
+        run(ci, bufs);
+        if (ci + 1 >= nch) break;
+        stage_store<T, NT>(RB, chunk_at(ci + 1), q1, trash);
+        lds_barrier();
+        load(RB, ci + 3);
+        run(ci + 1, buf1);
+        if (ci + 2 >= nch) break;
+        stage_store<T, NT>(RA, chunk_at(ci + 2), q0, trash);
+        lds_barrier();
+        load(RA, ci + 4);
+    }
     __syncthreads();
     if (MODE == 0) {
         scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
@@ -1408,6 +1704,71 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         M->sf_bytes = M->sf_flag_off + al((size_t)pl.max_ns + 1);
         M->sf_ok = M->tab_in_lds && M->sf_bytes <= (size_t)ctx->max_lds;
     }
+    {
+        // staged sweep: two chunk buffers after the tables and z; every level
+        // must fit in one buffer
+        auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+        M->st_ok = false;
+        M->st_buf_off = al(M->lds_tab_bytes + zbytes + tsz);  // z, zero slot, trash slot
+        const long long room = (long long)ctx->max_lds - (long long)M->st_buf_off;
+        const int cap = (int)std::min<long long>(kStageBytes, (room - 16) / 2 / 16 * 16);
+        if (M->tab_in_lds && M->z_in_lds && cap >= 1024) {
+            const int capw = cap / 4;
+            std::vector<SChunk> ch;
+            std::vector<int> c0(2 * (size_t)nsub);
+            bool ok = true;
+            for (int s = 0; s < nsub && ok; ++s) {
+                const SubInfo& si = pl.subs[s];
+                c0[2 * s] = (int)ch.size();
+                for (int dir = 0; dir < 2 && ok; ++dir) {
+                    const int nl = dir ? si.nblev : si.nflev;
+                    const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
+                    int l = 0;
+                    while (l < nl && ok) {
+                        SChunk c{l, 0, lv[l].start, 0, lv[l].ell, 0, dir, 0};
+                        while (l < nl) {
+                            const int nr = c.nrow + lv[l].size;
+                            const int ne = c.nell + lv[l].K * lv[l].size;
+                            const int words = fp32 ? stage_words<float>(nr, ne, dir)
+                                                   : stage_words<double>(nr, ne, dir);
+                            if ((words + 3) / 4 * 4 > capw) break;
+                            c.nrow = nr;
+                            c.nell = ne;
+                            ++c.nlev;
+                            ++l;
+                        }
+                        if (c.nlev == 0) ok = false;  // one level larger than a buffer
+                        else ch.push_back(c);
+                    }
+                }
+                c0[2 * s + 1] = (int)ch.size() - c0[2 * s];
+            }
+            long long quads = 0;
+            for (SChunk& c : ch) {
+                c.img = (int)quads;
+                quads += fp32 ? stage_quads<float>(c) : stage_quads<double>(c);
+            }
+            if (ok && quads <= INT32_MAX) {
+                M->st_ok = true;
+                M->st_buf = cap;
+                M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
+                M->d_chunks.upload(ch, st);
+                M->d_chunk0.upload(c0, st);
+                M->d_img.alloc((size_t)quads);
+                if (fp32)
+                    hipLaunchKernelGGL(k_stage_image<float>, dim3((int)ch.size()), dim3(256), 0, st,
+                                       (int)ch.size(), M->d_chunks.p, M->f_row.p, M->f_col.p,
+                                       M->f_v32.p, M->b_row.p, M->b_col.p, M->b_v32.p,
+                                       M->b_d32.p, M->d_img.p);
+                else
+                    hipLaunchKernelGGL(k_stage_image<double>, dim3((int)ch.size()), dim3(256), 0,
+                                       st, (int)ch.size(), M->d_chunks.p, M->f_row.p, M->f_col.p,
+                                       M->f_v64.p, M->b_row.p, M->b_col.p, M->b_v64.p,
+                                       M->b_d64.p, M->d_img.p);
+                SG_HIP(hipGetLastError());
+            }
+        }
+    }
     if (additive) {
         if (fp32) M->zout32.alloc((size_t)pl.total_rows + 1);
         else M->zout64.alloc((size_t)pl.total_rows + 1);
@@ -1449,6 +1810,22 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             }
 #undef SG_SET_RING1
 #undef SG_SET_RING
+        }
+        if (M->st_ok) {
+            const int stb = (int)M->st_bytes;
+#define SG_SET_ST(T, MD)                                                                          \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 256>,                          \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, stb));               \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 512>,                          \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, stb));               \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 1024>,                         \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, stb));
+            if (fp32) {
+                SG_SET_ST(float, 0) SG_SET_ST(float, 1)
+            } else {
+                SG_SET_ST(double, 0) SG_SET_ST(double, 1)
+            }
+#undef SG_SET_ST
         }
         if (M->sf_ok) {
             const int sfb = (int)M->sf_bytes;
@@ -1645,6 +2022,19 @@ template <typename T, bool ZLDS, int MODE>
 static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
                          const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
     const SweepShape sh = kSweepShapes[M->sweep_variant];
+    if (ZLDS && sh.staged && M->st_ok) {
+#define SG_ST(NT)                                                                                 \
+    hipLaunchKernelGGL((k_sweep_staged<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->st_bytes,     \
+                       M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
+                       (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p, M->d_own.p, \
+                       M->d_own_gid.p, x, y, zo, cz, guard, (int)M->lds_tab_bytes,              \
+                       (int)M->st_buf_off, M->st_buf / 4)
+        if (sh.threads == 1024) SG_ST(1024);
+        else if (sh.threads == 512) SG_ST(512);
+        else SG_ST(256);
+#undef SG_ST
+        return;
+    }
     if (ZLDS && sh.syncfree && M->sf_ok) {
 #define SG_SF(NT)                                                                                 \
     hipLaunchKernelGGL((k_sweep_syncfree<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->sf_bytes,   \
@@ -1712,8 +2102,17 @@ static void launch_any_sweep(sg_precond* M, const double* x, double* y, const De
     SG_HIP(hipGetLastError());
 }
 
-// Times each sweep workgroup size (three launches each, on a zero input into
-// scratch) and keeps the fastest; every size computes the same values.
+// Whether shape v runs as itself on this preconditioner (a sync-free or staged
+// shape whose LDS plan does not fit falls back to the barrier sweep).
+static bool sweep_shape_available(const sg_precond* M, int v) {
+    const SweepShape& sh = kSweepShapes[v];
+    if (sh.syncfree) return M->sf_ok && M->z_in_lds;
+    if (sh.staged) return M->st_ok && M->z_in_lds;
+    return true;
+}
+
+// Times each sweep shape (three launches each, on a zero input into scratch)
+// and keeps the fastest; every shape computes the same values.
 template <typename T>
 static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd, T* zg, T* zo) {
     hipStream_t s = M->ctx->stream;
@@ -1739,7 +2138,7 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
     float best = 0.f;
     int bestv = 1;
     for (int v = 0; v < kNumSweepShapes; ++v) {
-        if (kSweepShapes[v].syncfree && !(M->sf_ok && M->z_in_lds)) continue;
+        if (!sweep_shape_available(M, v)) continue;
         M->sweep_variant = v;
         const DevGuard none{nullptr, nullptr, 0};
         launch_any_sweep<T>(M, xin.p, yout.p, none, fv, bv, bd, zg, zo, nullptr);
@@ -2084,11 +2483,14 @@ sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info) {
     info->z_in_lds = M->z_in_lds ? 1 : 0;
     info->nc = M->nc;
     info->max_ns = M->max_ns;
-    info->sweep_shape = M->nsub && (M->sweep_tuned || M->ctx->autotune != 1)
-                            ? (kSweepShapes[M->sweep_variant].syncfree && !(M->sf_ok && M->z_in_lds)
-                                   ? 1
-                                   : M->sweep_variant)
-                            : -1;
+    // a shape that falls back runs the barrier sweep of its thread count (512
+    // for the sync-free shapes)
+    int shape = M->sweep_variant;
+    if (!sweep_shape_available(M, shape))
+        shape = kSweepShapes[shape].syncfree || kSweepShapes[shape].threads == 512
+                    ? 1
+                    : (kSweepShapes[shape].threads == 256 ? 0 : 2);
+    info->sweep_shape = M->nsub && (M->sweep_tuned || M->ctx->autotune != 1) ? shape : -1;
     for (size_t s = 0; s < M->dofs.size() && M->nsub; ++s) info->n_ext += (long long)M->dofs[s].size();
     for (int v : M->sub_nnz) info->nnz_ext += v;
     info->levels_fwd = M->levels_fwd;

@@ -206,6 +206,39 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
         assert np.array_equal(y, ref)
 
 
+@pytest.mark.parametrize("spec,kind,nparts,prec", [
+    ("2d:100:100", "ilu0", 1, "fp64"),   # 199 levels per sweep in ~18 LDS chunks
+    ("3d:24:24:24", "ras", 8, "fp64"),
+    ("3d:24:24:24", "asm", 8, "fp64"),
+    ("3d:30:30:30", "ilu0", 1, "fp32"),  # levels of up to 675 rows (> 512 threads)
+])
+def test_staged_sweep_bitwise(sg, ctx, spec, kind, nparts, prec):
+    """The staged sweeps (factor data copied into LDS a chunk of levels ahead;
+    autotune modes 9, 10 and 11) equal the barrier sweep bit for bit, and the
+    oracle bit for bit in FP64."""
+    A = O.grid(spec)
+    rng = np.random.default_rng(14)
+    v = A.v * rng.uniform(0.9, 1.1, A.nnz)
+    Ao = O.Csr(A.rp, A.ci, v)
+    dA = sg.Csr(ctx, A.rp, A.ci, v)
+    x = rng.standard_normal(A.n)
+    outs = {}
+    for mode in (3, 9, 10, 11):
+        ctx.autotune(mode)
+        M = sg.Precond(ctx, kind, nparts=nparts, overlap=1, precision=prec).setup(dA)
+        y = dev(sg, ctx, np.zeros(A.n))
+        M.apply(dev(sg, ctx, x), y)
+        info = M.stats()
+        assert info["z_in_lds"] and info["sweep_shape"] == mode - 2, (mode, info)
+        outs[mode] = y.get()
+        part = M.partition() if nparts > 1 else None
+    ctx.autotune(1)
+    for mode in (9, 10, 11):
+        assert np.array_equal(outs[mode], outs[3]), mode
+    if prec == "fp64":
+        assert np.array_equal(outs[9], O.Precond(kind, Ao, nparts, 1, part).apply(x))
+
+
 def test_ring_sweep_fp32(sg, ctx):
     A = O.poisson3d(64, 64, 64)
     dA = sg.Csr(ctx, A.rp, A.ci, A.v)
@@ -299,7 +332,7 @@ def test_kernel_shapes_do_not_change_results(sg, precision):
     v = A.v * rng.uniform(0.5, 1.5, A.nnz)
     x = rng.standard_normal(A.n)
     outs = []
-    for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
+    for mode in range(12):
         c = sg.Context(0)
         c.autotune(mode)
         dA = sg.Csr(c, A.rp, A.ci, v)
