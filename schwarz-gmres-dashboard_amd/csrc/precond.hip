@@ -902,47 +902,39 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
 }
 
 // ---- staged sweep: factor data through LDS, chunk by chunk ---------------------
-// For subdomains whose solution vector fits in LDS.  The host cuts each sweep
-// into chunks of consecutive levels whose row data fits one LDS buffer; the
-// rows, column indices, values and (backward) diagonals of a chunk are
-// contiguous in the level-ordered arrays.  While the levels of chunk c run
-// entirely out of LDS (row data and z; an LDS-only barrier closes each
-// level), chunk c + 1 is in flight as a few coalesced 32-bit loads per thread,
-// and is written to the other buffer afterwards.  Global memory latency is
-// paid once per chunk instead of on every level's critical path.  Rows and
-// the operations of a row run in the order of the level-synchronous sweep, so
-// the values are the same bit for bit.
-//
-// Buffer layout of a chunk (32-bit words): rows [0, nrow), columns
-// [nrow, nrow + nell), values from stage_val_word() (8-byte aligned for
-// double), then the diagonals (backward sweep).
-template <typename T>
-__host__ __device__ inline int stage_val_word(int nrow, int nell) {
-    const int a = nrow + nell;
-    return sizeof(T) == 8 ? (a + 1) & ~1 : a;
-}
-template <typename T>
-__host__ __device__ inline int stage_words(int nrow, int nell, bool bwd) {
-    constexpr int w = sizeof(T) / 4;
-    return stage_val_word<T>(nrow, nell) + w * nell + (bwd ? w * nrow : 0);
-}
+// For subdomains whose solution vector fits in LDS (at most 65534 rows) and
+// whose levels have at most kKreg entries per row.  Setup packs every row of
+// a sweep into one record in level order (16-bit local row and columns, the
+// values, the diagonal: 32 bytes in FP32, 64 in FP64) and cuts each sweep into
+// chunks of consecutive levels whose records fit one LDS buffer.  While the
+// levels of chunk c run entirely out of LDS (an LDS-only barrier closes each
+// level), chunks c + 1 and c + 2 are in flight as 16-byte loads; chunk c + 1 is
+// written to the other buffer afterwards.  A row's data is two (FP64: four)
+// 16-byte LDS reads, issued a level ahead.  Rows and the operations of a row
+// run in the order of the level-synchronous sweep, so the values are the same
+// bit for bit.
 
 // 16 bytes as clang vector types (valid in every address space, loads and
 // stores of one dwordx4 / ds_read_b128 / ds_write_b128).
 typedef unsigned stage_q __attribute__((ext_vector_type(4)));
 typedef int stage_i4 __attribute__((ext_vector_type(4)));
 
+// 16-byte words per row record: [row | c0<<16, c1 | c2<<16, c3, diag (FP32)]
+// then the four values (FP32: one quad; FP64: two quads, then the diagonal).
+template <typename T>
+__host__ __device__ constexpr int stage_rec_quads() {
+    return sizeof(T) == 4 ? 2 : 4;
+}
+template <typename T>
+__host__ __device__ inline int stage_quads(const SChunk& c) {
+    return c.nrow * stage_rec_quads<T>();
+}
+
 template <typename T, int NT>
 struct StageRegs {
     static constexpr int kV = kStageBytes / 16 / NT;  // 16-byte words per thread
     stage_q v[kV];
 };
-
-// Number of 16-byte words of chunk c in the image.
-template <typename T>
-__host__ __device__ inline int stage_quads(const SChunk& c) {
-    return (stage_words<T>(c.nrow, c.nell, c.bwd != 0) + 3) / 4;
-}
 
 // Issues the 16-byte loads of chunk c from its image (nothing touches the
 // registers until stage_store); indices past the chunk are clamped to its
@@ -971,141 +963,153 @@ __device__ __forceinline__ void stage_store(const StageRegs<T, NT>& R, const SCh
     }
 }
 
-// Setup: writes every chunk into the image in its LDS layout (one workgroup
-// per chunk); the alignment word before double values is zero.
+// Setup: packs the rows of every chunk into records (one workgroup per chunk,
+// one thread per row).  Entries past a row's count are column zs (the zero
+// slot of z) with value 0, as in the other sweeps' padding.
 template <typename T>
-__global__ void k_stage_image(int nch, const SChunk* __restrict__ chunks,
-                              const int* __restrict__ f_row, const int* __restrict__ f_col,
-                              const T* __restrict__ f_val, const int* __restrict__ b_row,
-                              const int* __restrict__ b_col, const T* __restrict__ b_val,
-                              const T* __restrict__ b_diag, uint4* __restrict__ img) {
-    constexpr int w = sizeof(T) / 4;
+__global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* __restrict__ subs,
+                              const int* __restrict__ chunk_sub, const Level* __restrict__ flev,
+                              const Level* __restrict__ blev, const int* __restrict__ f_row,
+                              const int* __restrict__ f_col, const T* __restrict__ f_val,
+                              const int* __restrict__ b_row, const int* __restrict__ b_col,
+                              const T* __restrict__ b_val, const T* __restrict__ b_diag,
+                              uint4* __restrict__ img) {
     const SChunk c = chunks[blockIdx.x];
-    const unsigned* rows = (const unsigned*)((c.bwd ? b_row : f_row) + c.row0);
-    const unsigned* cols = (const unsigned*)((c.bwd ? b_col : f_col) + c.ell0);
-    const unsigned* vals = (const unsigned*)((c.bwd ? b_val : f_val) + c.ell0);
-    const unsigned* diag = (const unsigned*)(b_diag + c.row0);
-    const int a1 = c.nrow, a2 = a1 + c.nell;
-    const int vw = stage_val_word<T>(c.nrow, c.nell);
-    const int a3 = vw + w * c.nell;
-    const int W = 4 * stage_quads<T>(c);
-    unsigned* out = (unsigned*)(img + c.img);
-    for (int u = threadIdx.x; u < W; u += blockDim.x) {
-        unsigned v = 0;
-        if (u < a1) v = rows[u];
-        else if (u < a2) v = cols[u - a1];
-        else if (u >= vw && u < a3) v = vals[u - vw];
-        else if (c.bwd && u >= a3 && u < a3 + w * c.nrow) v = diag[u - a3];
-        out[u] = v;
+    const SubInfo si = subs[chunk_sub[blockIdx.x]];
+    const Level* lev = c.bwd ? blev + si.blev0 : flev + si.flev0;
+    const int* rows = c.bwd ? b_row : f_row;
+    const int* cols = c.bwd ? b_col : f_col;
+    const T* vals = c.bwd ? b_val : f_val;
+    const unsigned zs = (unsigned)si.ns;
+    for (int l = c.lev0; l < c.lev0 + c.nlev; ++l) {
+        const Level L = lev[l];
+        for (int q = threadIdx.x; q < L.size; q += blockDim.x) {
+            unsigned cc[kKreg];
+            T vv[kKreg];
+            for (int k = 0; k < kKreg; ++k) {
+                cc[k] = k < L.K ? (unsigned)cols[L.ell + k * L.size + q] : zs;
+                vv[k] = k < L.K ? vals[L.ell + k * L.size + q] : (T)0;
+            }
+            const T dg = c.bwd ? b_diag[L.start + q] : (T)0;
+            uint4* r = img + c.img + (size_t)(L.start - c.row0 + q) * stage_rec_quads<T>();
+            unsigned dbits[2] = {0u, 0u};
+            __builtin_memcpy(dbits, &dg, sizeof(T));
+            r[0] = make_uint4((unsigned)rows[L.start + q] | (cc[0] << 16), cc[1] | (cc[2] << 16),
+                              cc[3], sizeof(T) == 4 ? dbits[0] : 0u);
+            unsigned vb[2 * kKreg];
+            __builtin_memcpy(vb, vv, sizeof(vv));
+            if (sizeof(T) == 4) {
+                r[1] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+            } else {
+                r[1] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+                r[2] = make_uint4(vb[4], vb[5], vb[6], vb[7]);
+                r[3] = make_uint4(dbits[0], dbits[1], 0u, 0u);
+            }
+        }
     }
 }
 
-// One row of a level, read from the chunk buffer: local row, column indices
-// and values of the first kKreg entries, diagonal (backward sweep).
+// One row of a level as read from the chunk buffer (raw 16-byte words).
 template <typename T>
 struct StageRow {
-    int row;
-    int col[kKreg];
-    T val[kKreg];
-    T diag;
+    stage_q q[stage_rec_quads<T>()];
 };
 
-// Branch-free read of row q (< size, size >= 1) of a level: entries k >= K
-// become column zs (the zero slot) with value 0, so they subtract an exact 0
-// as the padding of the other sweeps does.  Every read is issued (measured:
-// cheaper than skipping the unused ones behind a scalar branch).
-template <typename T, bool BWD>
-__device__ __forceinline__ void stage_fetch(StageRow<T>& r, int q, int size, int K, int rs,
-                                            int es, int zs, const lds_t<int>* rws,
-                                            const lds_t<int>* cls, const lds_t<T>* vls,
-                                            const lds_t<T>* dgs) {
-    r.row = rws[rs + q];
+template <typename T>
+__device__ __forceinline__ void stage_fetch(StageRow<T>& r, const lds_t<stage_q>* buf, int p) {
 #pragma unroll
-    for (int k = 0; k < kKreg; ++k) {
-        const int idx = k < K ? es + k * size + q : 0;
-        const int c = cls[idx];
-        const T v = vls[idx];
-        r.col[k] = k < K ? c : zs;
-        r.val[k] = k < K ? v : (T)0;
+    for (int i = 0; i < stage_rec_quads<T>(); ++i) r.q[i] = buf[p * stage_rec_quads<T>() + i];
+}
+
+__device__ __forceinline__ float stage_bits_f(unsigned a) { return __uint_as_float(a); }
+__device__ __forceinline__ double stage_bits_d(unsigned lo, unsigned hi) {
+    return __hiloint2double((int)hi, (int)lo);
+}
+
+template <typename T>
+__device__ __forceinline__ int stage_row(const StageRow<T>& r) { return r.q[0].x & 0xffffu; }
+template <typename T>
+__device__ __forceinline__ int stage_col(const StageRow<T>& r, int k) {
+    return k == 0 ? (int)(r.q[0].x >> 16)
+         : k == 1 ? (int)(r.q[0].y & 0xffffu)
+         : k == 2 ? (int)(r.q[0].y >> 16)
+                  : (int)(r.q[0].z & 0xffffu);
+}
+template <typename T>
+__device__ __forceinline__ T stage_val(const StageRow<T>& r, int k) {
+    if constexpr (sizeof(T) == 4) {
+        return stage_bits_f(k == 0 ? r.q[1].x : k == 1 ? r.q[1].y : k == 2 ? r.q[1].z : r.q[1].w);
+    } else {
+        const stage_q& w = r.q[1 + k / 2];
+        return k % 2 == 0 ? stage_bits_d(w.x, w.y) : stage_bits_d(w.z, w.w);
     }
-    if (BWD) r.diag = dgs[rs + q];
+}
+template <typename T>
+__device__ __forceinline__ T stage_diag(const StageRow<T>& r) {
+    if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[0].w);
+    else return stage_bits_d(r.q[3].x, r.q[3].y);
 }
 
 // The levels of chunk c out of LDS; each level ends with an LDS-only barrier.
 // Thread t takes row t of a level (rows t + NT, ... of a wider level in a
-// second loop).  The next level's descriptor and the thread's row of it are
-// read while the current level computes (they do not depend on z).  A
+// second loop).  The next level's descriptor and the thread's record of it
+// are read while the current level computes (they do not depend on z).  A
 // wavefront with no row in a level skips it (a uniform branch): a level then
 // costs the SIMDs only the instructions of the wavefronts that hold its rows.
 // Threads past a level's end inside an active wavefront compute a clamped row
 // and write it to the trash slot zs + 1.
 template <typename T, bool BWD, int NT>
 __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
-                                             const SChunk& c, const lds_t<unsigned>* buf,
+                                             const SChunk& c, const lds_t<stage_q>* buf,
                                              int zs) {
-    const lds_t<int>* rws = (const lds_t<int>*)buf;
-    const lds_t<int>* cls = rws + c.nrow;
-    const lds_t<T>* vls = (const lds_t<T>*)(buf + stage_val_word<T>(c.nrow, c.nell));
-    const lds_t<T>* dgs = vls + c.nell;
     const int t = threadIdx.x;
     const int wbase = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
     const int lend = c.lev0 + c.nlev;
     // level descriptors: d (current, uniform), dn (next, uniform), and the raw
     // LDS values of the one after, read while the current level computes
     struct D {
-        int size, K, rs, es;
+        int size, p;  // rows, first record of the level in the chunk
     };
-    auto uni = [&](int a, int b, int e, int f) {
-        return D{__builtin_amdgcn_readfirstlane(b), __builtin_amdgcn_readfirstlane(e),
-                 __builtin_amdgcn_readfirstlane(a) - c.row0,
-                 __builtin_amdgcn_readfirstlane(f) - c.ell0};
+    auto uni = [&](int start, int size) {
+        return D{__builtin_amdgcn_readfirstlane(size), __builtin_amdgcn_readfirstlane(start) - c.row0};
     };
     const int l1 = min(c.lev0 + 1, lend - 1);
-    D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1], tab[4 * c.lev0 + 2], tab[4 * c.lev0 + 3]);
-    D dn = uni(tab[4 * l1], tab[4 * l1 + 1], tab[4 * l1 + 2], tab[4 * l1 + 3]);
+    D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
+    D dn = uni(tab[4 * l1], tab[4 * l1 + 1]);
     StageRow<T> cur;
-    if (wbase < d.size)
-        stage_fetch<T, BWD>(cur, min(t, d.size - 1), d.size, d.K, d.rs, d.es, zs, rws, cls, vls,
-                            dgs);
+    if (wbase < d.size) stage_fetch<T>(cur, buf, d.p + min(t, d.size - 1));
     for (int l = c.lev0; l < lend; ++l) {
         // this level's z values first: they are the critical path
         T zk[kKreg], acc = (T)0;
         if (wbase < d.size) {
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) zk[k] = z[cur.col[k]];
-            acc = z[cur.row];
+            for (int k = 0; k < kKreg; ++k) zk[k] = z[stage_col<T>(cur, k)];
+            acc = z[stage_row<T>(cur)];
         }
-        // then the next level's rows and the descriptor after it
+        // then the next level's records and the descriptor after it
         StageRow<T> nxt;
-        if (wbase < dn.size)
-            stage_fetch<T, BWD>(nxt, min(t, dn.size - 1), dn.size, dn.K, dn.rs, dn.es, zs, rws,
-                                cls, vls, dgs);
+        if (wbase < dn.size) stage_fetch<T>(nxt, buf, dn.p + min(t, dn.size - 1));
         const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
         if (wbase < d.size) {
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) acc -= cur.val[k] * zk[k];
-            const int qc = min(t, d.size - 1);
-            for (int k = kKreg; k < d.K; ++k)
-                acc -= vls[d.es + k * d.size + qc] * z[cls[d.es + k * d.size + qc]];
-            if (BWD) acc = acc / cur.diag;
-            z[t < d.size ? cur.row : zs + 1] = acc;
+            for (int k = 0; k < kKreg; ++k) acc -= stage_val<T>(cur, k) * zk[k];
+            if (BWD) acc = acc / stage_diag<T>(cur);
+            z[t < d.size ? stage_row<T>(cur) : zs + 1] = acc;
             if (d.size > NT) {
                 for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
                     StageRow<T> r;
-                    stage_fetch<T, BWD>(r, q, d.size, d.K, d.rs, d.es, zs, rws, cls, vls, dgs);
-                    T a = z[r.row];
+                    stage_fetch<T>(r, buf, d.p + q);
+                    T a = z[stage_row<T>(r)];
 #pragma unroll
-                    for (int k = 0; k < kKreg; ++k) a -= r.val[k] * z[r.col[k]];
-                    for (int k = kKreg; k < d.K; ++k)
-                        a -= vls[d.es + k * d.size + q] * z[cls[d.es + k * d.size + q]];
-                    if (BWD) a = a / r.diag;
-                    z[r.row] = a;
+                    for (int k = 0; k < kKreg; ++k) a -= stage_val<T>(r, k) * z[stage_col<T>(r, k)];
+                    if (BWD) a = a / stage_diag<T>(r);
+                    z[stage_row<T>(r)] = a;
                 }
             }
         }
         lds_barrier();
         d = dn;
-        dn = uni(dq.x, dq.y, dq.z, dq.w);
+        dn = uni(dq.x, dq.y);
         cur = nxt;
     }
 }
@@ -1153,19 +1157,18 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     stage_store<T, NT>(RA, chunk_at(0), q0, trash);
     load(RA, 2);
     __syncthreads();
-    auto run = [&](int k, const lds_t<unsigned>* buf) {
+    auto run = [&](int k, const lds_t<stage_q>* buf) {
         const SChunk c = chunk_at(k);
         if (c.bwd) stage_levels<T, true, NT>(z, lb, c, buf, si.ns);
         else stage_levels<T, false, NT>(z, lf, c, buf, si.ns);
     };
-    lds_t<unsigned>* buf1 = bufs + buf_words;
     for (int ci = 0; ci < nch; ci += 2) {
-        run(ci, bufs);
+        run(ci, q0);
         if (ci + 1 >= nch) break;
         stage_store<T, NT>(RB, chunk_at(ci + 1), q1, trash);
         lds_barrier();
         load(RB, ci + 3);
-        run(ci + 1, buf1);
+        run(ci + 1, q1);
         if (ci + 2 >= nch) break;
         stage_store<T, NT>(RA, chunk_at(ci + 2), q0, trash);
         lds_barrier();
@@ -1712,11 +1715,18 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         M->st_buf_off = al(M->lds_tab_bytes + zbytes + tsz);  // z, zero slot, trash slot
         const long long room = (long long)ctx->max_lds - (long long)M->st_buf_off;
         const int cap = (int)std::min<long long>(kStageBytes, (room - 16) / 2 / 16 * 16);
-        if (M->tab_in_lds && M->z_in_lds && cap >= 1024) {
-            const int capw = cap / 4;
+        // records need 16-bit local rows and columns (ns + 1, the trash slot,
+        // below 65536) and at most kKreg entries per row in every level
+        bool fits = M->tab_in_lds && M->z_in_lds && cap >= 1024 && pl.max_ns <= 65534;
+        for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
+        for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
+        if (fits) {
+            const int rq = fp32 ? stage_rec_quads<float>() : stage_rec_quads<double>();
+            const int capq = cap / 16;
             std::vector<SChunk> ch;
-            std::vector<int> c0(2 * (size_t)nsub);
+            std::vector<int> c0(2 * (size_t)nsub), csub;
             bool ok = true;
+            long long quads = 0;
             for (int s = 0; s < nsub && ok; ++s) {
                 const SubInfo& si = pl.subs[s];
                 c0[2 * s] = (int)ch.size();
@@ -1725,28 +1735,23 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                     const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
                     int l = 0;
                     while (l < nl && ok) {
-                        SChunk c{l, 0, lv[l].start, 0, lv[l].ell, 0, dir, 0};
-                        while (l < nl) {
-                            const int nr = c.nrow + lv[l].size;
-                            const int ne = c.nell + lv[l].K * lv[l].size;
-                            const int words = fp32 ? stage_words<float>(nr, ne, dir)
-                                                   : stage_words<double>(nr, ne, dir);
-                            if ((words + 3) / 4 * 4 > capw) break;
-                            c.nrow = nr;
-                            c.nell = ne;
+                        SChunk c{l, 0, lv[l].start, 0, lv[l].ell, 0, dir, (int)quads};
+                        while (l < nl && (long long)(c.nrow + lv[l].size) * rq <= capq) {
+                            c.nrow += lv[l].size;
+                            c.nell += lv[l].K * lv[l].size;
                             ++c.nlev;
                             ++l;
                         }
-                        if (c.nlev == 0) ok = false;  // one level larger than a buffer
-                        else ch.push_back(c);
+                        if (c.nlev == 0) {
+                            ok = false;  // one level larger than a buffer
+                        } else {
+                            ch.push_back(c);
+                            csub.push_back(s);
+                            quads += (long long)c.nrow * rq;
+                        }
                     }
                 }
                 c0[2 * s + 1] = (int)ch.size() - c0[2 * s];
-            }
-            long long quads = 0;
-            for (SChunk& c : ch) {
-                c.img = (int)quads;
-                quads += fp32 ? stage_quads<float>(c) : stage_quads<double>(c);
             }
             if (ok && quads <= INT32_MAX) {
                 M->st_ok = true;
@@ -1754,18 +1759,21 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                 M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
                 M->d_chunks.upload(ch, st);
                 M->d_chunk0.upload(c0, st);
+                DevBuf<int> d_csub;
+                d_csub.upload(csub, st);
                 M->d_img.alloc((size_t)quads);
                 if (fp32)
                     hipLaunchKernelGGL(k_stage_image<float>, dim3((int)ch.size()), dim3(256), 0, st,
-                                       (int)ch.size(), M->d_chunks.p, M->f_row.p, M->f_col.p,
-                                       M->f_v32.p, M->b_row.p, M->b_col.p, M->b_v32.p,
-                                       M->b_d32.p, M->d_img.p);
+                                       M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p, M->blev.p,
+                                       M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p, M->b_col.p,
+                                       M->b_v32.p, M->b_d32.p, M->d_img.p);
                 else
                     hipLaunchKernelGGL(k_stage_image<double>, dim3((int)ch.size()), dim3(256), 0,
-                                       st, (int)ch.size(), M->d_chunks.p, M->f_row.p, M->f_col.p,
-                                       M->f_v64.p, M->b_row.p, M->b_col.p, M->b_v64.p,
-                                       M->b_d64.p, M->d_img.p);
+                                       st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                       M->blev.p, M->f_row.p, M->f_col.p, M->f_v64.p, M->b_row.p,
+                                       M->b_col.p, M->b_v64.p, M->b_d64.p, M->d_img.p);
                 SG_HIP(hipGetLastError());
+                SG_HIP(hipStreamSynchronize(st));  // d_csub ends here
             }
         }
     }
