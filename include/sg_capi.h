@@ -399,14 +399,15 @@ sg_status sg_context_set_profiling(sg_context* ctx, int enable);
  * the first apply of a preconditioner time each shape and keep the fastest
  * (SpMV: CSR with 256, 512 or 1024 rows per workgroup, or the SELL-64 copy
  * (slices of 64 rows stored column by column, built when its padding is below
- * half the nonzeros); subdomain sweeps: 256, 512 or 1024 threads with a
+ * half the nonzeros) with 32-bit columns or, when every column is within 32767
+ * of its row, 16-bit offsets; subdomain sweeps: 256, 512 or 1024 threads with a
  * barrier per level and row data two levels ahead, 512 / 1024 threads
  * sync-free with per-row flags in LDS, 256 / 512 threads with a barrier per
  * level and row data five levels ahead, or 1024 / 512 / 256 threads with the
  * factor data staged through LDS a chunk of levels ahead).  mode 0: fixed
  * defaults (SpMV CSR 256, sweep 512).  mode 2..11: force sweep shape mode-2 in
- * that order and SpMV shape (mode-2) mod 4 (3 = SELL-64, CSR 256 when no SELL
- * copy was built).
+ * that order and SpMV shape (mode-2) mod 5 (3 = SELL-64, 4 = SELL-64 with 16-bit
+ * offsets; CSR 256 when that copy was not built).
  * All shapes compute identical values, so the mode never changes results.  No counterpart in the reference (its kernels have fixed
  * launch shapes, src/gpu/cuda_sparse.h, src/gpu/local_solve_kernel.cu). */
 sg_status sg_context_set_autotune(sg_context* ctx, int mode);

@@ -86,6 +86,7 @@ struct Work {
     double* part;   // block partial sums, part[c * pstride + block]
     KState* st;
     const int* mode;
+    int* hflag;     // single system, direct launches: host-mapped active flag per step
     int m, hcap, pstride;
 };
 
@@ -380,10 +381,18 @@ k_gram_update(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
 
 __global__ void __launch_bounds__(kTF) k_norm_finish(Work W0, Segs sp, int j) {
     const Work W = sys_work(W0, blockIdx.x);
-    if (!W.st->active) return;
+    // the host reads hflag[j] after the step's event, whether or not the
+    // system was still active
+    if (!W.st->active) {
+        if (W.hflag && threadIdx.x == 0) W.hflag[j] = 0;
+        return;
+    }
     double* out = W.h2 + kRedCols - 1;
     reduce_partials(W, sp.sys[blockIdx.x], 1, out);
-    if (threadIdx.x == 0) step_epilogue(W, j, sqrt(*out));
+    if (threadIdx.x == 0) {
+        step_epilogue(W, j, sqrt(*out));
+        if (W.hflag) W.hflag[j] = W.st->active;
+    }
 }
 
 // ---- MGS in the reference's order (gmres.cpp:93-97) -------------------------
@@ -682,6 +691,9 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     W.m = m;
     W.hcap = hcap;
     W.pstride = pstride;
+    W.hflag = nullptr;
+    if (nsys == 1 && !use_graph && cgs2)
+        SG_HIP(hipHostGetDevicePointer((void**)&W.hflag, ws.pinned, 0));
     KState* st = ws.st.p;
     int* hmode = ws.h_mode;
     const Guard step_guard{&st->active, (int)(sizeof(KState) / sizeof(int)), nsys > 1};
@@ -826,8 +838,11 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                 }
                 launch_step(j);
                 if (nsys == 1) {
-                    SG_HIP(hipMemcpyAsync(&ws.pinned[j], &st->active, sizeof(int),
-                                          hipMemcpyDeviceToHost, s));
+                    // the CGS2 epilogue stores the flag into host memory itself;
+                    // MGS copies it
+                    if (!W.hflag)
+                        SG_HIP(hipMemcpyAsync(&ws.pinned[j], &st->active, sizeof(int),
+                                              hipMemcpyDeviceToHost, s));
                     SG_HIP(hipEventRecord(evs[j], s));
                 }
             }

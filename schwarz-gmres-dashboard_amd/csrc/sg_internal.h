@@ -135,7 +135,10 @@ struct sg_csr {
     int nsl = 0;
     sg::DevBuf<int> sell_ptr, sell_row, sell_sys, sell_ci;
     sg::DevBuf<double> sell_v;
-    int variant = 0;     // selected shape: 0..2 row-block CSR, 3 SELL-64
+    // the same columns as 16-bit offsets from the row (-32768: padding), when
+    // every offset fits
+    sg::DevBuf<short> sell_cd;
+    int variant = 0;     // selected shape: 0..2 row-block CSR, 3 SELL-64, 4 SELL-64 16-bit
     bool tuned = false;
 };
 

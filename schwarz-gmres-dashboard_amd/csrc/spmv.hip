@@ -33,7 +33,9 @@ constexpr int kNnzPerThread = 8;  // LDS staging capacity: 8 products per thread
 // them, so the choice is a pure performance knob (see spmv_autotune).
 constexpr int kSpmvVariants[3] = {256, 512, 1024};
 constexpr int kSellVariant = 3;   // shape index of the SELL-64 kernel
-constexpr int kNumShapes = 4;
+constexpr int kSell16Variant = 4; // SELL-64 with 16-bit column offsets
+constexpr int kNumShapes = 5;
+constexpr short kSellPad16 = -32768;
 constexpr int kSellC = 64;        // rows per slice: one wavefront
 constexpr int kSellThreads = 256; // four slices per workgroup
 // The SELL copy is built only when its padding stays below half the nonzeros
@@ -128,6 +130,40 @@ k_spmv_sell(const int* __restrict__ sp, const int* __restrict__ srow, const int*
     if (r < srow[sl + 1]) y[r] = beta == 0.0 ? alpha * sum : alpha * sum + beta * y0[r];
 }
 
+// SELL-64 with the columns stored as 16-bit offsets from the row: 10 bytes per
+// entry instead of 12.  Same loads, products and sums as k_spmv_sell.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_spmv_sell16(const int* __restrict__ sp, const int* __restrict__ srow,
+              const short* __restrict__ cd, const double* __restrict__ v, int nsl, int per_xcd,
+              double alpha, const double* __restrict__ x, double beta,
+              const double* __restrict__ y0, double* __restrict__ y, DevGuard guard) {
+    constexpr int kSpb = NT / kSellC;
+    const int b = (blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
+    const int sl = b * kSpb + (int)(threadIdx.x / kSellC), lane = threadIdx.x % kSellC;
+    if (sl >= nsl || guard_stop(guard, sl)) return;
+    const int p0 = sp[sl], width = (sp[sl + 1] - p0) / kSellC;
+    const int r = srow[sl] + lane;
+    double sum = 0.0;
+    for (int k0 = 0; k0 < width; k0 += 8) {
+        double a[8];
+        short d[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = p0 + min(k0 + u, width - 1) * kSellC + lane;
+            a[u] = __builtin_nontemporal_load(&v[idx]);
+            d[u] = __builtin_nontemporal_load(&cd[idx]);
+        }
+        double xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[d[u] == kSellPad16 ? 0 : r + d[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < width && d[u] != kSellPad16) sum += a[u] * xv[u];
+    }
+    if (r < srow[sl + 1]) y[r] = beta == 0.0 ? alpha * sum : alpha * sum + beta * y0[r];
+}
+
 // SELL-64 copy of A, slices cut per system; false (nothing built) when the
 // padding or a row length is beyond the limits above.
 static bool sell_build(sg_csr* A) {
@@ -159,6 +195,22 @@ static bool sell_build(sg_csr* A) {
                 sci[idx] = A->host.ci[k];
                 sv[idx] = A->host.v[k];
             }
+    // 16-bit offsets when every entry's column is within 32767 of its row
+    std::vector<short> scd((size_t)total, kSellPad16);
+    bool fits16 = true;
+    for (int sl = 0; sl < nsl && fits16; ++sl)
+        for (int l = 0; l < kSellC; ++l) {
+            const int r = srow[sl] + l;
+            for (size_t idx = (size_t)sp[sl] + l; idx < (size_t)sp[sl + 1]; idx += kSellC) {
+                if (sci[idx] < 0) continue;
+                const int d = sci[idx] - r;
+                if (d < -32767 || d > 32767) {
+                    fits16 = false;
+                    break;
+                }
+                scd[idx] = (short)d;
+            }
+        }
     hipStream_t s = A->ctx->stream;
     A->nsl = nsl;
     A->sell_ptr.upload(sp, s);
@@ -166,6 +218,7 @@ static bool sell_build(sg_csr* A) {
     A->sell_sys.upload(ssys, s);
     A->sell_ci.upload(sci, s);
     A->sell_v.upload(sv, s);
+    if (fits16) A->sell_cd.upload(scd, s);
     SG_HIP(hipStreamSynchronize(s));  // the host vectors end here
     return true;
 }
@@ -205,17 +258,25 @@ void spmv_prepare(sg_csr* A) {
 }
 
 static bool shape_available(const sg_csr* A, int var) {
-    return var >= 0 && var < kNumShapes && (var != kSellVariant || A->nsl > 0);
+    if (var < 0 || var >= kNumShapes) return false;
+    if (var == kSellVariant) return A->nsl > 0;
+    if (var == kSell16Variant) return A->nsl > 0 && A->sell_cd.p;
+    return true;
 }
 
 static void spmv_launch_variant(sg_csr* A, int var, double alpha, const double* x, double beta,
                                 const double* y0, double* y, const Guard& g, hipStream_t s) {
-    if (var == kSellVariant) {
+    if (var == kSellVariant || var == kSell16Variant) {
         const int nb = (A->nsl + kSellThreads / kSellC - 1) / (kSellThreads / kSellC);
         const int per = (nb + kXcd - 1) / kXcd;
-        hipLaunchKernelGGL(k_spmv_sell<kSellThreads>, dim3(per * kXcd), dim3(kSellThreads), 0, s,
-                           A->sell_ptr.p, A->sell_row.p, A->sell_ci.p, A->sell_v.p, A->nsl, per,
-                           alpha, x, beta, y0, y, dev_guard(g, A->sell_sys.p));
+        if (var == kSellVariant)
+            hipLaunchKernelGGL(k_spmv_sell<kSellThreads>, dim3(per * kXcd), dim3(kSellThreads), 0,
+                               s, A->sell_ptr.p, A->sell_row.p, A->sell_ci.p, A->sell_v.p, A->nsl,
+                               per, alpha, x, beta, y0, y, dev_guard(g, A->sell_sys.p));
+        else
+            hipLaunchKernelGGL(k_spmv_sell16<kSellThreads>, dim3(per * kXcd), dim3(kSellThreads),
+                               0, s, A->sell_ptr.p, A->sell_row.p, A->sell_cd.p, A->sell_v.p,
+                               A->nsl, per, alpha, x, beta, y0, y, dev_guard(g, A->sell_sys.p));
         SG_HIP(hipGetLastError());
         return;
     }

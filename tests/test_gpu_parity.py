@@ -78,14 +78,21 @@ def _spmv_cases():
     rp = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
     ci = np.concatenate([np.array(r, np.int32) for r in rows])
     out.append(("ragged_batch", rp, ci, rng.standard_normal(len(ci)), np.cumsum([0] + sizes)))
+    # couplings 40000 rows away: no 16-bit offset copy, mode 6 runs CSR
+    n = 70000
+    rows = [sorted({i, (i + 40000) % n, (i + 1) % n}) for i in range(n)]
+    rp = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    ci = np.concatenate([np.array(r, np.int32) for r in rows])
+    out.append(("far", rp, ci, rng.standard_normal(len(ci)), None))
     return out
 
 
 @pytest.mark.parametrize("case", _spmv_cases(), ids=lambda c: c[0])
 def test_spmv_every_shape_bitwise(sg, case):
     """Each SpMV shape (CSR with 256 / 512 / 1024 rows per workgroup and the
-    SELL-64 copy, forced through autotune modes 2..5) equals the serial row loop
-    bit for bit, signed zeros included, in the y = A x and y = b - A x forms."""
+    SELL-64 copy with 32-bit columns or 16-bit offsets, forced through autotune
+    modes 2..6) equals the serial row loop bit for bit, signed zeros included,
+    in the y = A x and y = b - A x forms."""
     _, rp, ci, v, sys_rows = case
     n = len(rp) - 1
     rng = np.random.default_rng(8)
@@ -94,7 +101,7 @@ def test_spmv_every_shape_bitwise(sg, case):
     b = rng.standard_normal(n)
     Ao = O.Csr(rp, ci, v)
     ref1, ref2 = O.spmv(Ao, x), b - O.spmv(Ao, x)
-    for mode in (2, 3, 4, 5):
+    for mode in (2, 3, 4, 5, 6):
         c = sg.Context(0)
         c.autotune(mode)
         if sys_rows is None:
