@@ -315,6 +315,38 @@ __global__ void __launch_bounds__(kTF) k_gram_finish(Work W0, Segs sp, int ncols
     }
 }
 
+// Column c of system blockIdx.y: the sum of its block partials in block order
+// (the order of reduce_partials), into h2[c]; columns ncols.. are the Gram row
+// of v_j and also go into G.  One wavefront per column, so the reduction is
+// spread over 2 ncols workgroups instead of one.
+__global__ void __launch_bounds__(64) k_gram_sums(Work W0, Segs sp, int ncols, int j) {
+    const Work W = sys_work(W0, blockIdx.y);
+    if (!W.st->active) return;
+    const SysBlocks sb = sp.sys[blockIdx.y];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const double* pi = W.part + (size_t)c * W.pstride + sb.blk0;
+    double s = 0.0;
+    for (int b0 = 0; b0 < sb.nblk; b0 += 16 * 64) {
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int b = b0 + q * 64 + lane;
+            v[q] = b < sb.nblk ? pi[b] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s += v[q];
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+        W.h2[c] = s;
+        if (c >= ncols) {
+            const int i = c - ncols;
+            W.G[j * kMaxCols + i] = s;
+            W.G[i * kMaxCols + j] = s;
+        }
+    }
+}
+
 // wr -= V[:, cb:cb+CB] h[cb:cb+CB] on the thread's elements ee.
 template <int CB>
 __device__ __forceinline__ void gram_update_cols(const double* __restrict__ V, size_t ld, int cb,
@@ -333,16 +365,30 @@ __device__ __forceinline__ void gram_update_cols(const double* __restrict__ V, s
     }
 }
 
+// h = (2I - G) p is formed by every block from the sums of k_gram_sums (one
+// wavefront per row, lanes over the columns, as k_gram_finish does); the
+// system's first block also stores it as column j of H.
 __global__ void __launch_bounds__(kTGU)
 k_gram_update(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
-              double* __restrict__ w, Work W0) {
+              double* __restrict__ w, Work W0, int j) {
     const BlockRange br = sp.blk[blockIdx.x];
     if (!W0.st[br.sys].active) return;
     const Work W = sys_work(W0, br.sys);
     __shared__ double red[kTGU / 64][kRedCols];
     __shared__ double hs[kMaxCols];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < ncols; i += kTGU) hs[i] = W.h1[i];
+    const bool first = (int)blockIdx.x == sp.sys[br.sys].blk0;
+    const int m = W.m;
+    for (int i = wv; i < ncols; i += kTGU / 64) {
+        double gp = 0.0;
+        for (int k = lane; k < ncols; k += 64) gp += W.G[i * kMaxCols + k] * W.h2[k];
+        gp = wave_sum(gp);
+        if (lane == 0) {
+            const double h = 2.0 * W.h2[i] - gp;
+            hs[i] = h;
+            if (first) W.H[i + j * (m + 1)] = h;
+        }
+    }
     __syncthreads();
     const int start = br.start, end = br.end;
     const int nfull = ncols / kCBG * kCBG;
@@ -393,6 +439,51 @@ __global__ void __launch_bounds__(kTF) k_norm_finish(Work W0, Segs sp, int j) {
         step_epilogue(W, j, sqrt(*out));
         if (W.hflag) W.hflag[j] = W.st->active;
     }
+}
+
+// ||w|| from the update pass's partials, the step epilogue (system's first
+// block) and v_{j+1} = w scaled, in one kernel: every block reduces the same
+// partials in the same order (reduce_partials), so all of them scale by the
+// same value, the one the epilogue records.  Replaces k_norm_finish followed
+// by k_scale_basis for the steps that have a next basis vector.
+__global__ void __launch_bounds__(kT)
+k_norm_scale(Segs sp, Segs sp_norm, const double* __restrict__ u, double* __restrict__ v,
+             Work W0, int j) {
+    const BlockRange br = sp.blk[blockIdx.x];
+    const Work W = sys_work(W0, br.sys);
+    const bool first = (int)blockIdx.x == sp.sys[br.sys].blk0;
+    if (!W.st->active) {
+        if (first && W.hflag && threadIdx.x == 0) W.hflag[j] = 0;
+        return;
+    }
+    __shared__ double nrm;
+    const SysBlocks sb = sp_norm.sys[br.sys];
+    if (threadIdx.x < 64) {
+        const double* pi = W.part + sb.blk0;
+        double s = 0.0;
+        for (int b0 = 0; b0 < sb.nblk; b0 += 16 * 64) {
+            double t[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int b = b0 + q * 64 + (int)threadIdx.x;
+                t[q] = b < sb.nblk ? pi[b] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) s += t[q];
+        }
+        s = wave_sum(s);
+        if (threadIdx.x == 0) nrm = s;
+    }
+    __syncthreads();
+    const double hn = sqrt(nrm);
+    const bool div = W.st->method == 0;
+    if (first && threadIdx.x == 0) {
+        step_epilogue(W, j, hn);
+        if (W.hflag) W.hflag[j] = W.st->active;
+    }
+    if (hn < 1e-30) return;  // breakdown: the epilogue ends the cycle, no v_{j+1}
+    const double sc = div ? hn : 1.0 / hn;
+    for (int e = br.start + threadIdx.x; e < br.end; e += kT) v[e] = div ? u[e] / sc : u[e] * sc;
 }
 
 // ---- MGS in the reference's order (gmres.cpp:93-97) -------------------------
@@ -751,11 +842,12 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
             if (cgs2) {
                 hipLaunchKernelGGL(k_gram_dot, dim3(seg_gd.nblk), dim3(kTGD), 0, s, sp_gd,
                                    j + 1, V, ld, w, W, j);
-                hipLaunchKernelGGL(k_gram_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gd, j + 1,
-                                   j);
+                hipLaunchKernelGGL(k_gram_sums, dim3(2 * (j + 1), nsys), dim3(64), 0, s, W, sp_gd,
+                                   j + 1, j);
                 hipLaunchKernelGGL(k_gram_update, dim3(seg_gu.nblk), dim3(kTGU), 0, s, sp_gu,
-                                   j + 1, V, ld, w, W);
-                hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
+                                   j + 1, V, ld, w, W, j);
+                if (j + 1 == m)
+                    hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
             } else {
                 for (int i = 0; i <= j + 1; ++i) {
                     const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
@@ -770,8 +862,12 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         }
         if (j + 1 < m) {
             ProfScope ps(ctx, SG_K_UPDATE);
-            hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, w,
-                               V + (size_t)(j + 1) * ld, W);
+            if (cgs2)
+                hipLaunchKernelGGL(k_norm_scale, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, sp_gu,
+                                   w, V + (size_t)(j + 1) * ld, W, j);
+            else
+                hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, w,
+                                   V + (size_t)(j + 1) * ld, W);
         }
     };
     auto launch_tail = [&]() {
