@@ -456,6 +456,13 @@ k_norm_scale(Segs sp, Segs sp_norm, const double* __restrict__ u, double* __rest
         if (first && W.hflag && threadIdx.x == 0) W.hflag[j] = 0;
         return;
     }
+    // this block's first elements are loaded before the reduction (they do
+    // not depend on it), then the rest eight per thread at a time
+    constexpr int U = 8;
+    const int tid = threadIdx.x;
+    double pre[U];
+#pragma unroll
+    for (int r = 0; r < U; ++r) pre[r] = u[min(br.start + tid + r * kT, br.end - 1)];
     __shared__ double nrm;
     const SysBlocks sb = sp_norm.sys[br.sys];
     if (threadIdx.x < 64) {
@@ -483,7 +490,19 @@ k_norm_scale(Segs sp, Segs sp_norm, const double* __restrict__ u, double* __rest
     }
     if (hn < 1e-30) return;  // breakdown: the epilogue ends the cycle, no v_{j+1}
     const double sc = div ? hn : 1.0 / hn;
-    for (int e = br.start + threadIdx.x; e < br.end; e += kT) v[e] = div ? u[e] / sc : u[e] * sc;
+#pragma unroll
+    for (int r = 0; r < U; ++r) {
+        const int e = br.start + tid + r * kT;
+        if (e < br.end) v[e] = div ? pre[r] / sc : pre[r] * sc;
+    }
+    for (int e0 = br.start + tid + U * kT; e0 < br.end; e0 += U * kT) {
+        double a[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) a[r] = u[min(e0 + r * kT, br.end - 1)];
+#pragma unroll
+        for (int r = 0; r < U; ++r)
+            if (e0 + r * kT < br.end) v[e0 + r * kT] = div ? a[r] / sc : a[r] * sc;
+    }
 }
 
 // ---- MGS in the reference's order (gmres.cpp:93-97) -------------------------

@@ -1224,7 +1224,19 @@ k_coarse_restrict(int split, const SubInfo* __restrict__ subs, const int* __rest
     const T cs = c[s];
     const int* og = own_gid + si.own0;
     T acc = 0;
-    for (int k = a + threadIdx.x; k < b; k += 256) acc += cs * (T)x[og[k]];
+    // eight index and value loads in flight per thread; the terms are added
+    // in the order of the one-at-a-time loop
+    for (int k0 = a + threadIdx.x; k0 < b; k0 += 8 * 256) {
+        int g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = og[min(k0 + u * 256, b - 1)];
+        double xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[g[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u * 256 < b) acc += cs * (T)xv[u];
+    }
     __shared__ T red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
