@@ -89,6 +89,13 @@ struct SChunk {
     int lev0, nlev, row0, nrow, ell0, nell, bwd, img;  // img: offset in the chunk image, 16-byte units
 };
 
+// A chunk of the staged ring sweep: records at img, the rows' own values at own
+// (both in 16-byte units; own values of a sweep's positions are written there
+// by the gather (forward) or the forward sweep (backward)).
+struct SChunkR {
+    int lev0, nlev, row0, nrow, img, own, bwd, pad;
+};
+
 struct Plan {
     // host-side staging, moved to the device at setup
     std::vector<SubInfo> subs;
@@ -153,6 +160,16 @@ struct sg_precond {
     size_t st_buf_off = 0, st_bytes = 0;
     sg::DevBuf<sg::SChunk> d_chunks;
     sg::DevBuf<uint4> d_img;   // every chunk in its LDS layout, 16-byte aligned
+    // staged ring sweep (vector too large for LDS): ring of R values + two
+    // chunk buffers; chunks per subdomain: forward first/count, backward first/count
+    bool sr_ok = false;
+    int sr_R = 0, sr_buf = 0;
+    size_t sr_ring_off = 0, sr_buf_off = 0, sr_bytes = 0;
+    sg::DevBuf<sg::SChunkR> d_rchunks;
+    sg::DevBuf<int> d_rchunk0;   // 4 ints per subdomain
+    sg::DevBuf<uint4> d_rimg;    // records
+    sg::DevBuf<uint4> d_rown;    // own values, per chunk segment
+    sg::DevBuf<int> d_fslot, d_fgid;  // forward position -> own element index, global row
     sg::DevBuf<int> d_chunk0;  // per subdomain: first chunk, then the count (2 ints)
     sg::DevBuf<sg::SubInfo> d_subs;
     sg::DevBuf<int> d_order;
@@ -1183,6 +1200,307 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     }
 }
 
+// ---- staged ring sweep ------------------------------------------------------------
+// The staged sweep for subdomains whose vector exceeds LDS, built on the ring
+// encoding (every column reference is a live slot of an LDS ring of R recent
+// values: level-order position mod R; padding is slot R, which holds zero).
+// Setup packs each row into a record: destination, four ring references, four
+// values, diagonal (FP32 48 bytes, FP64 64 bytes).  A row's own value comes
+// from a level-ordered array that the gather fills (forward) or the forward
+// sweep writes (backward, at the row's backward position); each chunk's share
+// of it is 16-byte aligned and is staged with the records.  Forward and
+// backward sweeps are two pipelines (the backward own values exist only once
+// the forward sweep has finished).  Operation order per row equals the ring
+// sweep's, so the results are the same bit for bit.
+template <typename T>
+__host__ __device__ constexpr int sring_rec_quads() {
+    return sizeof(T) == 4 ? 3 : 4;
+}
+template <typename T>
+__host__ __device__ inline int sring_own_quads(int nrow) {
+    return (int)((nrow * sizeof(T) + 15) / 16);
+}
+template <typename T>
+__host__ __device__ inline int sring_quads(const SChunkR& c) {
+    return c.nrow * sring_rec_quads<T>() + sring_own_quads<T>(c.nrow);
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void sring_load(StageRegs<T, NT>& R, const SChunkR& c,
+                                           const stage_q* __restrict__ img,
+                                           const stage_q* __restrict__ own) {
+    typedef const __attribute__((address_space(1))) stage_q gq;
+    const int nrec = c.nrow * sring_rec_quads<T>();
+    const int n = nrec + sring_own_quads<T>(c.nrow);
+    const size_t bi = (size_t)(img + c.img), bo = (size_t)(own + c.own) - 16 * (size_t)nrec;
+#pragma unroll
+    for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
+        const int u = min((int)threadIdx.x + i * NT, n - 1);
+        R.v[i] = *(gq*)((u < nrec ? bi : bo) + 16 * (size_t)u);
+    }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void sring_store(const StageRegs<T, NT>& R, const SChunkR& c,
+                                            lds_t<stage_q>* buf, lds_t<stage_q>* trash) {
+    const int n = sring_quads<T>(c);
+#pragma unroll
+    for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
+        const int u = (int)threadIdx.x + i * NT;
+        *(u < n ? buf + u : trash) = R.v[i];
+    }
+}
+
+template <typename T>
+struct SRingRow {
+    stage_q q[sring_rec_quads<T>()];
+};
+
+template <typename T>
+__device__ __forceinline__ T sring_val(const SRingRow<T>& r, int k) {
+    if constexpr (sizeof(T) == 4) {
+        return stage_bits_f(k == 0 ? r.q[2].x : k == 1 ? r.q[2].y : k == 2 ? r.q[2].z : r.q[2].w);
+    } else {
+        const stage_q& w = r.q[2 + k / 2];
+        return k % 2 == 0 ? stage_bits_d(w.x, w.y) : stage_bits_d(w.z, w.w);
+    }
+}
+template <typename T>
+__device__ __forceinline__ int sring_col(const SRingRow<T>& r, int k) {
+    return (int)(k == 0 ? r.q[0].y : k == 1 ? r.q[0].z : k == 2 ? r.q[0].w : r.q[1].x);
+}
+template <typename T>
+__device__ __forceinline__ T sring_diag(const SRingRow<T>& r) {
+    if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[1].y);
+    else return stage_bits_d(r.q[1].y, r.q[1].z);
+}
+
+template <typename T>
+__device__ __forceinline__ void sring_fetch(SRingRow<T>& r, const lds_t<stage_q>* buf, int p) {
+#pragma unroll
+    for (int i = 0; i < sring_rec_quads<T>(); ++i) r.q[i] = buf[p * sring_rec_quads<T>() + i];
+}
+
+// One chunk's levels: row t of a level computes own - sum v_k ring[c_k]
+// (/ diag backward), writes ring[(position - pos0) mod R] and zout[dst].
+template <typename T, bool BWD, int NT>
+__device__ __forceinline__ void sring_levels(lds_t<T>* ring, int R, const lds_t<int>* tab,
+                                             const SChunkR& c, const lds_t<stage_q>* buf,
+                                             int pos0, T* __restrict__ zout) {
+    const int t = threadIdx.x;
+    const int wbase = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
+    const lds_t<T>* own = (const lds_t<T>*)(buf + c.nrow * sring_rec_quads<T>());
+    const int lend = c.lev0 + c.nlev;
+    // a chunk may start or end inside a level (rows of one level are
+    // independent): each level runs its rows inside [row0, row0 + nrow)
+    struct D {
+        int size, p, slot0;  // rows, first record in the chunk, ring slot of the first row
+    };
+    auto uni = [&](int start, int size) {
+        const int st = max(__builtin_amdgcn_readfirstlane(start), c.row0);
+        const int en = min(__builtin_amdgcn_readfirstlane(start) + __builtin_amdgcn_readfirstlane(size),
+                           c.row0 + c.nrow);
+        return D{en - st, st - c.row0, (st - pos0) % R};
+    };
+    const int l1 = min(c.lev0 + 1, lend - 1);
+    D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
+    D dn = uni(tab[4 * l1], tab[4 * l1 + 1]);
+    SRingRow<T> cur;
+    T cown = (T)0;
+    if (wbase < d.size) {
+        sring_fetch<T>(cur, buf, d.p + min(t, d.size - 1));
+        cown = own[d.p + min(t, d.size - 1)];
+    }
+    for (int l = c.lev0; l < lend; ++l) {
+        T zk[kKreg];
+        if (wbase < d.size) {
+#pragma unroll
+            for (int k = 0; k < kKreg; ++k) zk[k] = ring[sring_col<T>(cur, k)];
+        }
+        SRingRow<T> nxt;
+        T nown = (T)0;
+        if (wbase < dn.size) {
+            sring_fetch<T>(nxt, buf, dn.p + min(t, dn.size - 1));
+            nown = own[dn.p + min(t, dn.size - 1)];
+        }
+        const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
+        if (wbase < d.size) {
+            T acc = cown;
+#pragma unroll
+            for (int k = 0; k < kKreg; ++k) acc -= sring_val<T>(cur, k) * zk[k];
+            if (BWD) acc = acc / sring_diag<T>(cur);
+            int sl = d.slot0 + t;
+            if (sl >= R) sl -= R;
+            ring[t < d.size ? sl : R + 1] = acc;
+            if (t < d.size) zout[cur.q[0].x] = acc;
+            if (d.size > NT) {
+                for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
+                    SRingRow<T> r;
+                    sring_fetch<T>(r, buf, d.p + q);
+                    T a = own[d.p + q];
+#pragma unroll
+                    for (int k = 0; k < kKreg; ++k) a -= sring_val<T>(r, k) * ring[sring_col<T>(r, k)];
+                    if (BWD) a = a / sring_diag<T>(r);
+                    int s2 = d.slot0 + q;
+                    while (s2 >= R) s2 -= R;
+                    ring[s2] = a;
+                    zout[r.q[0].x] = a;
+                }
+            }
+        }
+        lds_barrier();
+        d = dn;
+        dn = uni(dq.x, dq.y);
+        cur = nxt;
+        cown = nown;
+    }
+}
+
+// LDS: level tables (forward | backward, four ints per level) | ring (R values,
+// zero slot, trash slot) | buffer 0 | buffer 1 | trash word.
+template <typename T, int MODE, int NT>
+__global__ void __launch_bounds__(NT)
+k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
+                    const int* __restrict__ chunk0, const SChunkR* __restrict__ chunks,
+                    const stage_q* __restrict__ img, stage_q* __restrict__ own,
+                    const int* __restrict__ fslot, const int* __restrict__ fgid,
+                    const Level* __restrict__ flev, const Level* __restrict__ blev,
+                    const int* __restrict__ own_l, const int* __restrict__ own_gid,
+                    const double* __restrict__ x, double* __restrict__ y, T* __restrict__ z3,
+                    size_t zstride, T* __restrict__ zout, const T* __restrict__ cz,
+                    DevGuard guard, int ring_off, int buf_off, int buf_quads, int R) {
+    extern __shared__ unsigned char lds_raw[];
+    const int s = order[blockIdx.x];
+    if (guard_stop(guard, s)) return;
+    const SubInfo si = subs[s];
+    lds_t<int>* lf = (lds_t<int>*)lds_raw;
+    lds_t<int>* lb = lf + 4 * si.nflev;
+    lds_t<T>* ring = (lds_t<T>*)(lds_raw + ring_off);
+    lds_t<stage_q>* q0 = (lds_t<stage_q>*)(lds_raw + buf_off);
+    lds_t<stage_q>* q1 = q0 + buf_quads;
+    lds_t<stage_q>* trash = q0 + 2 * buf_quads;
+    {
+        const int* tf = (const int*)(flev + si.flev0);
+        const int* tb = (const int*)(blev + si.blev0);
+        for (int l = threadIdx.x; l < 4 * si.nflev; l += NT) lf[l] = tf[l];
+        for (int l = threadIdx.x; l < 4 * si.nblev; l += NT) lb[l] = tb[l];
+    }
+    const int pf0 = si.nflev ? flev[si.flev0].start : 0;
+    const int pb0 = si.nblev ? blev[si.blev0].start : 0;
+    T* ownT = (T*)own;
+    T* zr = z3 + 2 * zstride + si.zoff;
+    // forward own values: x gathered in forward level order into the chunks'
+    // own segments, eight positions per thread in flight
+    {
+        constexpr int U = 8;
+        for (int p0 = threadIdx.x; p0 < si.ns; p0 += U * NT) {
+            int g[U], sl[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = pf0 + min(p0 + u * NT, si.ns - 1);
+                g[u] = fgid[p];
+                sl[u] = fslot[p];
+            }
+            double xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[g[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (p0 + u * NT < si.ns) ownT[sl[u]] = (T)xv[u];
+        }
+    }
+    if (threadIdx.x == 0) ring[R] = (T)0;
+    __syncthreads();
+    // one sweep: chunks [c0, c0 + nch) through the two buffers, as k_sweep_staged
+    auto pipeline = [&](int c0, int nch, const lds_t<int>* tab, bool bwd, int pos0, T* out) {
+        StageRegs<T, NT> RA, RB;
+        auto chunk_at = [&](int k) { return chunks[c0 + min(k, nch - 1)]; };
+        auto load = [&](StageRegs<T, NT>& Rg, int k) {
+            sring_load<T, NT>(Rg, chunk_at(k), img, own);
+        };
+        auto run = [&](int k, const lds_t<stage_q>* buf) {
+            const SChunkR c = chunk_at(k);
+            if (bwd) sring_levels<T, true, NT>(ring, R, tab, c, buf, pos0, out);
+            else sring_levels<T, false, NT>(ring, R, tab, c, buf, pos0, out);
+        };
+        load(RA, 0);
+        load(RB, 1);
+        sring_store<T, NT>(RA, chunk_at(0), q0, trash);
+        load(RA, 2);
+        lds_barrier();
+        for (int ci = 0; ci < nch; ci += 2) {
+            run(ci, q0);
+            if (ci + 1 >= nch) break;
+            sring_store<T, NT>(RB, chunk_at(ci + 1), q1, trash);
+            lds_barrier();
+            load(RB, ci + 3);
+            run(ci + 1, q1);
+            if (ci + 2 >= nch) break;
+            sring_store<T, NT>(RA, chunk_at(ci + 2), q0, trash);
+            lds_barrier();
+            load(RA, ci + 4);
+        }
+    };
+    if (si.nflev) pipeline(chunk0[4 * s], chunk0[4 * s + 1], lf, false, pf0, ownT);
+    __syncthreads();  // backward own values (the forward results) are in global memory
+    if (si.nblev) pipeline(chunk0[4 * s + 2], chunk0[4 * s + 3], lb, true, pb0, zr);
+    __syncthreads();
+    if (MODE == 0) {
+        scatter_owned<NT>(own_l + si.own0, own_gid + si.own0, si.nown, zr, cz != nullptr,
+                          cz ? cz[s] : (T)0, y);
+    } else {
+        for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = zr[i];
+    }
+}
+
+// Setup: packs the rows of every staged-ring chunk into records (one workgroup
+// per chunk).  dst: forward, the own-value index of the row's backward
+// position; backward, the local row.
+template <typename T>
+__global__ void k_stage_image_ring(const SChunkR* __restrict__ chunks,
+                                   const SubInfo* __restrict__ subs,
+                                   const int* __restrict__ chunk_sub,
+                                   const Level* __restrict__ flev, const Level* __restrict__ blev,
+                                   const int* __restrict__ fdst, const int* __restrict__ bdst,
+                                   const int* __restrict__ fcol, const int* __restrict__ bcol,
+                                   const T* __restrict__ f_val, const T* __restrict__ b_val,
+                                   const T* __restrict__ b_diag, int R, uint4* __restrict__ img) {
+    const SChunkR c = chunks[blockIdx.x];
+    const SubInfo si = subs[chunk_sub[blockIdx.x]];
+    const Level* lev = c.bwd ? blev + si.blev0 : flev + si.flev0;
+    const int* cols = c.bwd ? bcol : fcol;
+    const T* vals = c.bwd ? b_val : f_val;
+    for (int l = c.lev0; l < c.lev0 + c.nlev; ++l) {
+        const Level L = lev[l];
+        // only the level's rows inside the chunk (a level may span chunks)
+        const int qa = max(c.row0 - L.start, 0);
+        const int qb = min(c.row0 + c.nrow - L.start, L.size);
+        for (int q = qa + (int)threadIdx.x; q < qb; q += blockDim.x) {
+            unsigned cc[kKreg];
+            T vv[kKreg];
+            for (int k = 0; k < kKreg; ++k) {
+                cc[k] = k < L.K ? (unsigned)cols[L.ell + k * L.size + q] : (unsigned)R;
+                vv[k] = k < L.K ? vals[L.ell + k * L.size + q] : (T)0;
+            }
+            const int p = L.start + q;
+            const T dg = c.bwd ? b_diag[p] : (T)0;
+            unsigned db[2] = {0u, 0u};
+            __builtin_memcpy(db, &dg, sizeof(T));
+            unsigned vb[2 * kKreg];
+            __builtin_memcpy(vb, vv, sizeof(vv));
+            uint4* r = img + c.img + (size_t)(p - c.row0) * sring_rec_quads<T>();
+            r[0] = make_uint4((unsigned)(c.bwd ? bdst[p] : fdst[p]), cc[0], cc[1], cc[2]);
+            r[1] = make_uint4(cc[3], db[0], sizeof(T) == 8 ? db[1] : 0u, 0u);
+            if (sizeof(T) == 4) {
+                r[2] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+            } else {
+                r[2] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+                r[3] = make_uint4(vb[4], vb[5], vb[6], vb[7]);
+            }
+        }
+    }
+}
+
 // ASM prolongation: y[g] = sum over subdomains containing g, ascending, of z.
 template <typename T>
 __global__ void k_asm_gather(int n, const int* __restrict__ ptr, const int* __restrict__ pos,
@@ -1789,6 +2107,126 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             }
         }
     }
+    {
+        // staged ring sweep: ring of R values after the tables, then two chunk
+        // buffers; every reference must stay in the ring (no global copies)
+        auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+        M->sr_ok = false;
+        bool fits = M->ring && M->tab_in_lds;
+        for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
+        for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
+        const int cap = kStageBytes;
+        const size_t ring_off = M->lds_tab_bytes;
+        const long long room = (long long)ctx->max_lds - (long long)ring_off - 2LL * cap - 32;
+        const int R = (int)std::min<long long>(room / (long long)tsz - 2, INT32_MAX);
+        std::vector<std::vector<int>> fcs(nsub), bcs(nsub), bps(nsub);
+        std::vector<int> Fs(nsub, 0);
+        if (fits && R >= 64 && max_w <= R)
+            run_parallel(nsub, [&](int s) {
+                Fs[s] = ring_encode(fw[s], bw[s], local[s].n, R, fcs[s], bcs[s], bps[s]);
+            });
+        bool allring = fits && R >= 64 && max_w <= R;
+        for (int f : Fs) allring = allring && f == INT_MAX;
+        if (allring) {
+            const int rq = fp32 ? sring_rec_quads<float>() : sring_rec_quads<double>();
+            auto ownq = [&](int nrow) {
+                return fp32 ? sring_own_quads<float>(nrow) : sring_own_quads<double>(nrow);
+            };
+            const int capq = cap / 16, per_q = (int)(16 / tsz);
+            std::vector<SChunkR> ch;
+            std::vector<int> c0(4 * (size_t)nsub), csub;
+            std::vector<int> fslot(pl.f_row.size(), 0), bslot(pl.b_row.size(), 0);
+            long long qimg = 0, qown = 0;
+            bool ok = true;
+            for (int s = 0; s < nsub && ok; ++s) {
+                const SubInfo& si = pl.subs[s];
+                for (int dir = 0; dir < 2 && ok; ++dir) {
+                    const int nl = dir ? si.nblev : si.nflev;
+                    const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
+                    c0[4 * s + 2 * dir] = (int)ch.size();
+                    // chunks of whole levels while they fit; a level larger than
+                    // a buffer is cut into pieces of rows (its rows are independent)
+                    int maxr = 1;
+                    while ((long long)(maxr + 1) * rq + ownq(maxr + 1) <= capq) ++maxr;
+                    const int pos_beg = nl ? lv[0].start : 0;
+                    const int pos_end = nl ? lv[nl - 1].start + lv[nl - 1].size : 0;
+                    int l = 0, pos = pos_beg;
+                    while (pos < pos_end && ok) {
+                        while (lv[l].start + lv[l].size <= pos) ++l;
+                        SChunkR c{l, 0, pos, 0, (int)qimg, (int)qown, dir, 0};
+                        int end = pos;
+                        int lk = l;
+                        while (lk < nl && lv[lk].start + lv[lk].size - pos <= maxr) {
+                            end = lv[lk].start + lv[lk].size;
+                            ++lk;
+                        }
+                        if (end == pos) end = std::min(pos + maxr, lv[l].start + lv[l].size);
+                        c.nrow = end - pos;
+                        int la = l;
+                        while (la < nl && lv[la].start < end) ++la;
+                        c.nlev = la - l;
+                        pos = end;
+                        std::vector<int>& slot = dir ? bslot : fslot;
+                        for (int p = 0; p < c.nrow; ++p)
+                            slot[(size_t)c.row0 + p] = (int)(qown * per_q + p);
+                        ch.push_back(c);
+                        csub.push_back(s);
+                        qimg += (long long)c.nrow * rq;
+                        qown += ownq(c.nrow);
+                    }
+                    c0[4 * s + 2 * dir + 1] = (int)ch.size() - c0[4 * s + 2 * dir];
+                }
+            }
+            ok = ok && qimg <= INT32_MAX && qown * per_q <= INT32_MAX;
+            if (ok) {
+                std::vector<int> fdst(pl.f_row.size(), 0), fgid(pl.f_row.size(), 0),
+                    bdst(pl.b_row.size(), 0), fc, bc;
+                for (int s = 0; s < nsub; ++s) {
+                    const SubInfo& si = pl.subs[s];
+                    const int pf0 = si.nflev ? pl.flev[si.flev0].start : 0;
+                    const int pb0 = si.nblev ? pl.blev[si.blev0].start : 0;
+                    for (int p = 0; p < si.ns; ++p) {
+                        const int row = fw[s].row[p];
+                        fdst[(size_t)pf0 + p] = bslot[(size_t)pb0 + bps[s][p]];
+                        fgid[(size_t)pf0 + p] = global_ilu ? row : M->dofs[s][row];
+                        bdst[(size_t)pb0 + p] = bw[s].row[p];
+                    }
+                    fc.insert(fc.end(), fcs[s].begin(), fcs[s].end());
+                    bc.insert(bc.end(), bcs[s].begin(), bcs[s].end());
+                }
+                M->sr_ok = true;
+                M->sr_R = R;
+                M->sr_ring_off = ring_off;
+                M->sr_buf_off = al(ring_off + (size_t)(R + 2) * tsz);
+                M->sr_buf = cap;
+                M->sr_bytes = M->sr_buf_off + 2 * (size_t)cap + 16;
+                M->d_rchunks.upload(ch, st);
+                M->d_rchunk0.upload(c0, st);
+                M->d_fslot.upload(fslot, st);
+                M->d_fgid.upload(fgid, st);
+                M->d_rimg.alloc((size_t)qimg);
+                M->d_rown.alloc((size_t)qown);
+                DevBuf<int> d_csub, d_fdst, d_bdst, d_fc, d_bc;
+                d_csub.upload(csub, st);
+                d_fdst.upload(fdst, st);
+                d_bdst.upload(bdst, st);
+                d_fc.upload(fc, st);
+                d_bc.upload(bc, st);
+                if (fp32)
+                    hipLaunchKernelGGL(k_stage_image_ring<float>, dim3((int)ch.size()), dim3(256),
+                                       0, st, M->d_rchunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                       M->blev.p, d_fdst.p, d_bdst.p, d_fc.p, d_bc.p, M->f_v32.p,
+                                       M->b_v32.p, M->b_d32.p, R, M->d_rimg.p);
+                else
+                    hipLaunchKernelGGL(k_stage_image_ring<double>, dim3((int)ch.size()), dim3(256),
+                                       0, st, M->d_rchunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                       M->blev.p, d_fdst.p, d_bdst.p, d_fc.p, d_bc.p, M->f_v64.p,
+                                       M->b_v64.p, M->b_d64.p, R, M->d_rimg.p);
+                SG_HIP(hipGetLastError());
+                SG_HIP(hipStreamSynchronize(st));  // the temporaries end here
+            }
+        }
+    }
     if (additive) {
         if (fp32) M->zout32.alloc((size_t)pl.total_rows + 1);
         else M->zout64.alloc((size_t)pl.total_rows + 1);
@@ -1830,6 +2268,22 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             }
 #undef SG_SET_RING1
 #undef SG_SET_RING
+        }
+        if (M->sr_ok) {
+            const int srb = (int)M->sr_bytes;
+#define SG_SET_SR(T, MD)                                                                          \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 256>,                     \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, srb));               \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 512>,                     \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, srb));               \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 1024>,                    \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, srb));
+            if (fp32) {
+                SG_SET_SR(float, 0) SG_SET_SR(float, 1)
+            } else {
+                SG_SET_SR(double, 0) SG_SET_SR(double, 1)
+            }
+#undef SG_SET_SR
         }
         if (M->st_ok) {
             const int stb = (int)M->st_bytes;
@@ -2067,6 +2521,20 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
 #undef SG_SF
         return;
     }
+    if (!ZLDS && sh.staged && M->sr_ok) {
+#define SG_SR(NT)                                                                                 \
+    hipLaunchKernelGGL((k_sweep_staged_ring<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->sr_bytes, \
+                       M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_rchunk0.p,               \
+                       M->d_rchunks.p, (const stage_q*)M->d_rimg.p, (stage_q*)M->d_rown.p,      \
+                       M->d_fslot.p, M->d_fgid.p, M->flev.p, M->blev.p, M->d_own.p,             \
+                       M->d_own_gid.p, x, y, zg, M->zstride, zo, cz, guard, (int)M->sr_ring_off, \
+                       (int)M->sr_buf_off, M->sr_buf / 16, M->sr_R)
+        if (sh.threads == 1024) SG_SR(1024);
+        else if (sh.threads == 512) SG_SR(512);
+        else SG_SR(256);
+#undef SG_SR
+        return;
+    }
     if (!ZLDS && M->ring) {
         const int F = M->ring_F == INT_MAX ? INT_MAX : std::max(1, M->ring_F);
 #define SG_RING(NT, D, TB)                                                                        \
@@ -2127,7 +2595,7 @@ static void launch_any_sweep(sg_precond* M, const double* x, double* y, const De
 static bool sweep_shape_available(const sg_precond* M, int v) {
     const SweepShape& sh = kSweepShapes[v];
     if (sh.syncfree) return M->sf_ok && M->z_in_lds;
-    if (sh.staged) return M->st_ok && M->z_in_lds;
+    if (sh.staged) return M->z_in_lds ? M->st_ok : M->sr_ok;
     return true;
 }
 

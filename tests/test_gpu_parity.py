@@ -189,8 +189,11 @@ def test_ilu0_random_values_bitwise(sg, ctx):
     ("3d:40:40:40", "asm", 2),
 ])
 def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
-    """Subdomains whose solution vector exceeds LDS run the ring sweep: it must
-    equal the oracle bit for bit in FP64 with every workgroup size."""
+    """Subdomains whose solution vector exceeds LDS run the ring sweep, or its
+    staged form (records and own values streamed through LDS, modes 9..11) when
+    every level has at most four entries per row and every reference stays in
+    the ring: both must equal the oracle bit for bit in FP64 with every
+    workgroup size."""
     A = O.grid(spec)
     rng = np.random.default_rng(12)
     v = A.v * rng.uniform(0.9, 1.1, A.nnz)
@@ -198,11 +201,15 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     dA = sg.Csr(ctx, A.rp, A.ci, v)
     x = rng.standard_normal(A.n)
     outs = []
-    for mode in (2, 3, 4, 7, 8):  # barrier sweeps: 256 / 512 / 1024 threads; deep prefetch
+    staged = []
+    for mode in (2, 3, 4, 7, 8, 9, 10, 11):  # 256 / 512 / 1024 threads; deep prefetch; staged
         ctx.autotune(mode)
         M = sg.Precond(ctx, kind, nparts=nparts, overlap=1).setup(dA)
         info = M.stats()
         assert not info["z_in_lds"]
+        y0 = dev(sg, ctx, np.zeros(A.n))
+        M.apply(dev(sg, ctx, x), y0)
+        staged.append(M.stats()["sweep_shape"] == mode - 2)
         y = dev(sg, ctx, np.zeros(A.n))
         M.apply(dev(sg, ctx, x), y)
         outs.append(y.get())
@@ -211,6 +218,9 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     ref = O.Precond(kind, Ao, nparts, 1, part).apply(x)
     for y in outs:
         assert np.array_equal(y, ref)
+    # the stencil cases stream through the staged ring sweep (the random one
+    # has rows with more than four entries per level and falls back)
+    assert all(staged[5:]) == (spec != "rnd:60000:3"), staged
 
 
 @pytest.mark.parametrize("spec,kind,nparts,prec", [
@@ -259,6 +269,16 @@ def test_ring_sweep_fp32(sg, ctx):
     y2 = dev(sg, ctx, np.zeros(A.n))
     M.apply(dev(sg, ctx, x), y2)
     assert np.array_equal(y.get(), y2.get())
+    # the staged ring sweep gives the same values bit for bit
+    for mode in (9, 10, 11):
+        ctx.autotune(mode)
+        Ms = sg.Precond(ctx, "ras", nparts=4, overlap=1, precision="fp32",
+                        partition=M.partition()).setup(dA)
+        ys = dev(sg, ctx, np.zeros(A.n))
+        Ms.apply(dev(sg, ctx, x), ys)
+        assert Ms.stats()["sweep_shape"] == mode - 2
+        assert np.array_equal(ys.get(), y.get()), mode
+    ctx.autotune(1)
 
 
 def test_ras_asm_match_reference_blocks(sg, ctx, gold):
