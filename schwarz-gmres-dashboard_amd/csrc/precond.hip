@@ -81,12 +81,12 @@ struct Level {
     int start, size, K, ell;
 };
 
-// A chunk of consecutive levels of one sweep of one subdomain, staged into LDS
-// as a unit: level-ordered positions [row0, row0 + nrow), ELL entries
-// [ell0, ell0 + nell) (global indices into the concatenated arrays); levels
-// [lev0, lev0 + nlev) of the subdomain's own level table; bwd: backward sweep.
+// A chunk of one sweep of one subdomain, staged into LDS as a unit: the
+// level-ordered positions [row0, row0 + nrow), which cover levels [lev0, lev0 +
+// nlev) of the subdomain's own level table (the first and last of them possibly
+// in part: a level wider than a buffer spans chunks); bwd: backward sweep.
 struct SChunk {
-    int lev0, nlev, row0, nrow, ell0, nell, bwd, img;  // img: offset in the chunk image, 16-byte units
+    int lev0, nlev, row0, nrow, bwd, img;  // img: offset in the chunk image, 16-byte units
 };
 
 // A chunk of the staged ring sweep: records at img, the rows' own values at own
@@ -1000,7 +1000,10 @@ __global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* 
     const unsigned zs = (unsigned)si.ns;
     for (int l = c.lev0; l < c.lev0 + c.nlev; ++l) {
         const Level L = lev[l];
-        for (int q = threadIdx.x; q < L.size; q += blockDim.x) {
+        // only the level's rows inside the chunk (a level may span chunks)
+        const int qa = max(c.row0 - L.start, 0);
+        const int qb = min(c.row0 + c.nrow - L.start, L.size);
+        for (int q = qa + (int)threadIdx.x; q < qb; q += blockDim.x) {
             unsigned cc[kKreg];
             T vv[kKreg];
             for (int k = 0; k < kKreg; ++k) {
@@ -1084,11 +1087,16 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
     const int lend = c.lev0 + c.nlev;
     // level descriptors: d (current, uniform), dn (next, uniform), and the raw
     // LDS values of the one after, read while the current level computes
+    // a chunk may start or end inside a level (rows of one level are
+    // independent): each level runs its rows inside [row0, row0 + nrow)
     struct D {
         int size, p;  // rows, first record of the level in the chunk
     };
     auto uni = [&](int start, int size) {
-        return D{__builtin_amdgcn_readfirstlane(size), __builtin_amdgcn_readfirstlane(start) - c.row0};
+        const int a = __builtin_amdgcn_readfirstlane(start);
+        const int st = max(a, c.row0);
+        const int en = min(a + __builtin_amdgcn_readfirstlane(size), c.row0 + c.nrow);
+        return D{en - st, st - c.row0};
     };
     const int l1 = min(c.lev0 + 1, lend - 1);
     D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
@@ -2038,8 +2046,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         M->sf_ok = M->tab_in_lds && M->sf_bytes <= (size_t)ctx->max_lds;
     }
     {
-        // staged sweep: two chunk buffers after the tables and z; every level
-        // must fit in one buffer
+        // staged sweep: two chunk buffers after the tables and z
         auto al = [](size_t b) { return (b + 15) / 16 * 16; };
         M->st_ok = false;
         M->st_buf_off = al(M->lds_tab_bytes + zbytes + tsz);  // z, zero slot, trash slot
@@ -2063,22 +2070,23 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                 for (int dir = 0; dir < 2 && ok; ++dir) {
                     const int nl = dir ? si.nblev : si.nflev;
                     const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
-                    int l = 0;
-                    while (l < nl && ok) {
-                        SChunk c{l, 0, lv[l].start, 0, lv[l].ell, 0, dir, (int)quads};
-                        while (l < nl && (long long)(c.nrow + lv[l].size) * rq <= capq) {
-                            c.nrow += lv[l].size;
-                            c.nell += lv[l].K * lv[l].size;
-                            ++c.nlev;
-                            ++l;
-                        }
-                        if (c.nlev == 0) {
-                            ok = false;  // one level larger than a buffer
-                        } else {
-                            ch.push_back(c);
-                            csub.push_back(s);
-                            quads += (long long)c.nrow * rq;
-                        }
+                    // chunks of whole levels while they fit; a level larger than
+                    // a buffer is cut into pieces of rows (its rows are independent)
+                    const int maxr = capq / rq;
+                    const int pos_end = nl ? lv[nl - 1].start + lv[nl - 1].size : 0;
+                    int l = 0, pos = nl ? lv[0].start : 0;
+                    while (pos < pos_end) {
+                        while (lv[l].start + lv[l].size <= pos) ++l;
+                        int end = pos, lk = l;
+                        while (lk < nl && lv[lk].start + lv[lk].size - pos <= maxr)
+                            end = lv[lk].start + lv[lk].size, ++lk;
+                        if (end == pos) end = std::min(pos + maxr, lv[l].start + lv[l].size);
+                        int la = l;
+                        while (la < nl && lv[la].start < end) ++la;
+                        ch.push_back(SChunk{l, la - l, pos, end - pos, dir, (int)quads});
+                        csub.push_back(s);
+                        quads += (long long)(end - pos) * rq;
+                        pos = end;
                     }
                 }
                 c0[2 * s + 1] = (int)ch.size() - c0[2 * s];

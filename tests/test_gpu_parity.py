@@ -228,6 +228,9 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     ("3d:24:24:24", "ras", 8, "fp64"),
     ("3d:24:24:24", "asm", 8, "fp64"),
     ("3d:30:30:30", "ilu0", 1, "fp32"),  # levels of up to 675 rows (> 512 threads)
+    # z takes 108 KiB of LDS, leaving two 24 KiB buffers of 384 FP64 records:
+    # the widest levels (432 rows) are split across chunks
+    ("3d:24:24:24", "ilu0", 1, "fp64"),
 ])
 def test_staged_sweep_bitwise(sg, ctx, spec, kind, nparts, prec):
     """The staged sweeps (factor data copied into LDS a chunk of levels ahead;
