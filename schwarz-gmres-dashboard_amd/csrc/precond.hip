@@ -87,6 +87,7 @@ struct Level {
 // in part: a level wider than a buffer spans chunks); bwd: backward sweep.
 struct SChunk {
     int lev0, nlev, row0, nrow, bwd, img;  // img: offset in the chunk image, 16-byte units
+    int pad[2];                            // 32 bytes: two 16-byte LDS words
 };
 
 // A chunk of the staged ring sweep: records at img, the rows' own values at own
@@ -153,11 +154,11 @@ struct sg_precond {
     int max_ns = 0;
     size_t sf_item_off = 0, sf_z_off = 0, sf_flag_off = 0, sf_bytes = 0;
     sg::DevBuf<int> d_fitem, d_bitem;
-    // staged sweep: chunks of every subdomain (forward ones first), two LDS
-    // buffers of st_buf bytes after the level tables and z
+    // staged sweep: chunks of every subdomain (forward ones first); LDS =
+    // level tables | z | chunk descriptors | two buffers of st_buf bytes
     bool st_ok = false;
     int st_buf = 0;
-    size_t st_buf_off = 0, st_bytes = 0;
+    size_t st_ct_off = 0, st_buf_off = 0, st_bytes = 0;
     sg::DevBuf<sg::SChunk> d_chunks;
     sg::DevBuf<uint4> d_img;   // every chunk in its LDS layout, 16-byte aligned
     // staged ring sweep (vector too large for LDS): ring of R values + two
@@ -1140,7 +1141,7 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
 }
 
 // LDS: level tables (forward | backward, four ints per level) | z (ns values,
-// zero slot, trash slot) | buffer 0 | buffer 1 | trash word.
+// zero slot, trash slot) | chunk descriptors | buffer 0 | buffer 1 | trash word.
 template <typename T, int MODE, int NT>
 __global__ void __launch_bounds__(NT)
 k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
@@ -1149,7 +1150,7 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                const Level* __restrict__ blev, const int* __restrict__ dofs,
                const int* __restrict__ own, const int* __restrict__ own_gid,
                const double* __restrict__ x, double* __restrict__ y, T* __restrict__ zout,
-               const T* __restrict__ cz, DevGuard guard, int z_off, int buf_off,
+               const T* __restrict__ cz, DevGuard guard, int z_off, int ct_off, int buf_off,
                int buf_words) {
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
@@ -1159,45 +1160,70 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     lds_t<int>* lf = (lds_t<int>*)lds_raw;
     lds_t<int>* lb = lf + 4 * si.nflev;
     lds_t<T>* z = (lds_t<T>*)(lds_raw + z_off);
+    lds_t<stage_i4>* ct = (lds_t<stage_i4>*)(lds_raw + ct_off);
     lds_t<unsigned>* bufs = (lds_t<unsigned>*)(lds_raw + buf_off);
     {
         const int* tf = (const int*)(flev + si.flev0);
         const int* tb = (const int*)(blev + si.blev0);
         for (int l = threadIdx.x; l < 4 * si.nflev; l += NT) lf[l] = tf[l];
         for (int l = threadIdx.x; l < 4 * si.nblev; l += NT) lb[l] = tb[l];
+        const stage_i4* tc = (const stage_i4*)(chunks + c0);
+        for (int l = threadIdx.x; l < 2 * nch; l += NT) ct[l] = tc[l];
     }
     // two register sets: while chunk c computes, chunks c + 1 and c + 2 are in
     // flight (set (c + 1) & 1 and set c & 1).  Loads past the last chunk reload
     // the last one, so every step issues the same loads and the waits stay exact.
+    // The first three descriptors come from global memory, the others from the
+    // LDS copy (a scalar load at every chunk step would stall on its latency).
     StageRegs<T, NT> RA, RB;
-    auto chunk_at = [&](int k) { return chunks[c0 + min(k, nch - 1)]; };
-    auto load = [&](StageRegs<T, NT>& R, int k) { stage_load<T, NT>(R, chunk_at(k), img); };
-    load(RA, 0);
-    load(RB, 1);
+    auto chunk_g = [&](int k) { return chunks[c0 + min(k, nch - 1)]; };
+    auto chunk_at = [&](int k) {
+        const int kk = min(k, nch - 1);
+        const stage_i4 a = ct[2 * kk], b = ct[2 * kk + 1];
+        SChunk c;
+        c.lev0 = __builtin_amdgcn_readfirstlane(a.x);
+        c.nlev = __builtin_amdgcn_readfirstlane(a.y);
+        c.row0 = __builtin_amdgcn_readfirstlane(a.z);
+        c.nrow = __builtin_amdgcn_readfirstlane(a.w);
+        c.bwd = __builtin_amdgcn_readfirstlane(b.x);
+        c.img = __builtin_amdgcn_readfirstlane(b.y);
+        return c;
+    };
+    // descriptors of the chunks held in RA and RB, and in buffers q0 and q1
+    SChunk dA = chunk_g(0), dB = chunk_g(1);
+    stage_load<T, NT>(RA, dA, img);
+    stage_load<T, NT>(RB, dB, img);
     gather_x<NT, T, lds_t<T>*, 24>(dofs + si.row0, si.ns, x, z);
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     lds_t<stage_q>* trash = (lds_t<stage_q>*)(bufs + 2 * buf_words);
     lds_t<stage_q>* q0 = (lds_t<stage_q>*)bufs;
     lds_t<stage_q>* q1 = (lds_t<stage_q>*)(bufs + buf_words);
-    stage_store<T, NT>(RA, chunk_at(0), q0, trash);
-    load(RA, 2);
+    SChunk c0q = dA, c1q;
+    stage_store<T, NT>(RA, dA, q0, trash);
+    dA = chunk_g(2);
+    stage_load<T, NT>(RA, dA, img);
     __syncthreads();
-    auto run = [&](int k, const lds_t<stage_q>* buf) {
-        const SChunk c = chunk_at(k);
+    auto run = [&](const SChunk& c, const lds_t<stage_q>* buf) {
         if (c.bwd) stage_levels<T, true, NT>(z, lb, c, buf, si.ns);
         else stage_levels<T, false, NT>(z, lf, c, buf, si.ns);
     };
     for (int ci = 0; ci < nch; ci += 2) {
-        run(ci, q0);
+        run(c0q, q0);
         if (ci + 1 >= nch) break;
-        stage_store<T, NT>(RB, chunk_at(ci + 1), q1, trash);
+        const SChunk nb = chunk_at(ci + 3);  // read before the store's waits
+        stage_store<T, NT>(RB, dB, q1, trash);
+        c1q = dB;
         lds_barrier();
-        load(RB, ci + 3);
-        run(ci + 1, q1);
+        stage_load<T, NT>(RB, nb, img);
+        dB = nb;
+        run(c1q, q1);
         if (ci + 2 >= nch) break;
-        stage_store<T, NT>(RA, chunk_at(ci + 2), q0, trash);
+        const SChunk na = chunk_at(ci + 4);
+        stage_store<T, NT>(RA, dA, q0, trash);
+        c0q = dA;
         lds_barrier();
-        load(RA, ci + 4);
+        stage_load<T, NT>(RA, na, img);
+        dA = na;
     }
     __syncthreads();
     if (MODE == 0) {
@@ -1364,8 +1390,9 @@ __device__ __forceinline__ void sring_levels(lds_t<T>* ring, int R, const lds_t<
     }
 }
 
-// LDS: level tables (forward | backward, four ints per level) | ring (R values,
-// zero slot, trash slot) | buffer 0 | buffer 1 | trash word.
+// LDS: level tables (forward | backward, four ints per level) | chunk
+// descriptors | ring (R values, zero slot, trash slot) | buffer 0 | buffer 1 |
+// trash word.
 template <typename T, int MODE, int NT>
 __global__ void __launch_bounds__(NT)
 k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
@@ -1376,22 +1403,28 @@ k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ or
                     const int* __restrict__ own_l, const int* __restrict__ own_gid,
                     const double* __restrict__ x, double* __restrict__ y, T* __restrict__ z3,
                     size_t zstride, T* __restrict__ zout, const T* __restrict__ cz,
-                    DevGuard guard, int ring_off, int buf_off, int buf_quads, int R) {
+                    DevGuard guard, int ct_off, int ring_off, int buf_off, int buf_quads,
+                    int R) {
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
     lds_t<int>* lf = (lds_t<int>*)lds_raw;
     lds_t<int>* lb = lf + 4 * si.nflev;
+    lds_t<stage_i4>* ct = (lds_t<stage_i4>*)(lds_raw + ct_off);
     lds_t<T>* ring = (lds_t<T>*)(lds_raw + ring_off);
     lds_t<stage_q>* q0 = (lds_t<stage_q>*)(lds_raw + buf_off);
     lds_t<stage_q>* q1 = q0 + buf_quads;
     lds_t<stage_q>* trash = q0 + 2 * buf_quads;
+    // this subdomain's chunks: forward ones, then backward ones
+    const int nfc = chunk0[4 * s + 1], nbc = chunk0[4 * s + 3];
     {
         const int* tf = (const int*)(flev + si.flev0);
         const int* tb = (const int*)(blev + si.blev0);
         for (int l = threadIdx.x; l < 4 * si.nflev; l += NT) lf[l] = tf[l];
         for (int l = threadIdx.x; l < 4 * si.nblev; l += NT) lb[l] = tb[l];
+        const stage_i4* tc = (const stage_i4*)(chunks + chunk0[4 * s]);
+        for (int l = threadIdx.x; l < 2 * (nfc + nbc); l += NT) ct[l] = tc[l];
     }
     const int pf0 = si.nflev ? flev[si.flev0].start : 0;
     const int pb0 = si.nblev ? blev[si.blev0].start : 0;
@@ -1419,39 +1452,59 @@ k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ or
     }
     if (threadIdx.x == 0) ring[R] = (T)0;
     __syncthreads();
-    // one sweep: chunks [c0, c0 + nch) through the two buffers, as k_sweep_staged
-    auto pipeline = [&](int c0, int nch, const lds_t<int>* tab, bool bwd, int pos0, T* out) {
+    // one sweep: chunks [lc0, lc0 + nch) of the LDS descriptor table through
+    // the two buffers, as k_sweep_staged
+    auto pipeline = [&](int lc0, int nch, const lds_t<int>* tab, bool bwd, int pos0, T* out) {
         StageRegs<T, NT> RA, RB;
-        auto chunk_at = [&](int k) { return chunks[c0 + min(k, nch - 1)]; };
-        auto load = [&](StageRegs<T, NT>& Rg, int k) {
-            sring_load<T, NT>(Rg, chunk_at(k), img, own);
+        auto chunk_at = [&](int k) {
+            const int kk = lc0 + min(k, nch - 1);
+            const stage_i4 a = ct[2 * kk], b = ct[2 * kk + 1];
+            SChunkR c;
+            c.lev0 = __builtin_amdgcn_readfirstlane(a.x);
+            c.nlev = __builtin_amdgcn_readfirstlane(a.y);
+            c.row0 = __builtin_amdgcn_readfirstlane(a.z);
+            c.nrow = __builtin_amdgcn_readfirstlane(a.w);
+            c.img = __builtin_amdgcn_readfirstlane(b.x);
+            c.own = __builtin_amdgcn_readfirstlane(b.y);
+            c.bwd = __builtin_amdgcn_readfirstlane(b.z);
+            c.pad = 0;
+            return c;
         };
-        auto run = [&](int k, const lds_t<stage_q>* buf) {
-            const SChunkR c = chunk_at(k);
+        auto run = [&](const SChunkR& c, const lds_t<stage_q>* buf) {
             if (bwd) sring_levels<T, true, NT>(ring, R, tab, c, buf, pos0, out);
             else sring_levels<T, false, NT>(ring, R, tab, c, buf, pos0, out);
         };
-        load(RA, 0);
-        load(RB, 1);
-        sring_store<T, NT>(RA, chunk_at(0), q0, trash);
-        load(RA, 2);
+        // descriptors of the chunks held in RA and RB, and in buffers q0 and q1
+        SChunkR dA = chunk_at(0), dB = chunk_at(1);
+        sring_load<T, NT>(RA, dA, img, own);
+        sring_load<T, NT>(RB, dB, img, own);
+        SChunkR c0q = dA, c1q;
+        sring_store<T, NT>(RA, dA, q0, trash);
+        dA = chunk_at(2);
+        sring_load<T, NT>(RA, dA, img, own);
         lds_barrier();
         for (int ci = 0; ci < nch; ci += 2) {
-            run(ci, q0);
+            run(c0q, q0);
             if (ci + 1 >= nch) break;
-            sring_store<T, NT>(RB, chunk_at(ci + 1), q1, trash);
+            const SChunkR nb = chunk_at(ci + 3);  // read before the store's waits
+            sring_store<T, NT>(RB, dB, q1, trash);
+            c1q = dB;
             lds_barrier();
-            load(RB, ci + 3);
-            run(ci + 1, q1);
+            sring_load<T, NT>(RB, nb, img, own);
+            dB = nb;
+            run(c1q, q1);
             if (ci + 2 >= nch) break;
-            sring_store<T, NT>(RA, chunk_at(ci + 2), q0, trash);
+            const SChunkR na = chunk_at(ci + 4);
+            sring_store<T, NT>(RA, dA, q0, trash);
+            c0q = dA;
             lds_barrier();
-            load(RA, ci + 4);
+            sring_load<T, NT>(RA, na, img, own);
+            dA = na;
         }
     };
-    if (si.nflev) pipeline(chunk0[4 * s], chunk0[4 * s + 1], lf, false, pf0, ownT);
+    if (si.nflev) pipeline(0, nfc, lf, false, pf0, ownT);
     __syncthreads();  // backward own values (the forward results) are in global memory
-    if (si.nblev) pipeline(chunk0[4 * s + 2], chunk0[4 * s + 3], lb, true, pb0, zr);
+    if (si.nblev) pipeline(nfc, nbc, lb, true, pb0, zr);
     __syncthreads();
     if (MODE == 0) {
         scatter_owned<NT>(own_l + si.own0, own_gid + si.own0, si.nown, zr, cz != nullptr,
@@ -2046,28 +2099,37 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         M->sf_ok = M->tab_in_lds && M->sf_bytes <= (size_t)ctx->max_lds;
     }
     {
-        // staged sweep: two chunk buffers after the tables and z
+        // staged sweep: chunk descriptors and two chunk buffers after the tables and z
         auto al = [](size_t b) { return (b + 15) / 16 * 16; };
         M->st_ok = false;
-        M->st_buf_off = al(M->lds_tab_bytes + zbytes + tsz);  // z, zero slot, trash slot
-        const long long room = (long long)ctx->max_lds - (long long)M->st_buf_off;
-        const int cap = (int)std::min<long long>(kStageBytes, (room - 16) / 2 / 16 * 16);
+        M->st_ct_off = al(M->lds_tab_bytes + zbytes + tsz);  // z, zero slot, trash slot
         // records need 16-bit local rows and columns (ns + 1, the trash slot,
         // below 65536) and at most kKreg entries per row in every level
-        bool fits = M->tab_in_lds && M->z_in_lds && cap >= 1024 && pl.max_ns <= 65534;
+        bool fits = M->tab_in_lds && M->z_in_lds && pl.max_ns <= 65534;
         for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
         for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
-        if (fits) {
-            const int rq = fp32 ? stage_rec_quads<float>() : stage_rec_quads<double>();
+        const int rq = fp32 ? stage_rec_quads<float>() : stage_rec_quads<double>();
+        std::vector<SChunk> ch;
+        std::vector<int> c0(2 * (size_t)nsub), csub;
+        long long quads = 0;
+        int cap = 0;
+        // the descriptor table's size depends on the buffer size and the
+        // other way round: grow the table's share until the plan fits it
+        size_t ct_bytes = 0;
+        bool planned = false;
+        for (int pass = 0; pass < 8 && fits && !planned; ++pass) {
+            const long long room = (long long)ctx->max_lds - (long long)(M->st_ct_off + ct_bytes);
+            cap = (int)std::min<long long>(kStageBytes, (room - 16) / 2 / 16 * 16);
+            if (cap < 1024) break;
             const int capq = cap / 16;
-            std::vector<SChunk> ch;
-            std::vector<int> c0(2 * (size_t)nsub), csub;
-            bool ok = true;
-            long long quads = 0;
-            for (int s = 0; s < nsub && ok; ++s) {
+            ch.clear();
+            csub.clear();
+            quads = 0;
+            size_t max_nch = 0;
+            for (int s = 0; s < nsub; ++s) {
                 const SubInfo& si = pl.subs[s];
                 c0[2 * s] = (int)ch.size();
-                for (int dir = 0; dir < 2 && ok; ++dir) {
+                for (int dir = 0; dir < 2; ++dir) {
                     const int nl = dir ? si.nblev : si.nflev;
                     const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
                     // chunks of whole levels while they fit; a level larger than
@@ -2083,59 +2145,54 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                         if (end == pos) end = std::min(pos + maxr, lv[l].start + lv[l].size);
                         int la = l;
                         while (la < nl && lv[la].start < end) ++la;
-                        ch.push_back(SChunk{l, la - l, pos, end - pos, dir, (int)quads});
+                        ch.push_back(SChunk{l, la - l, pos, end - pos, dir, (int)quads, {0, 0}});
                         csub.push_back(s);
                         quads += (long long)(end - pos) * rq;
                         pos = end;
                     }
                 }
                 c0[2 * s + 1] = (int)ch.size() - c0[2 * s];
+                max_nch = std::max(max_nch, (size_t)c0[2 * s + 1]);
             }
-            if (ok && quads <= INT32_MAX) {
-                M->st_ok = true;
-                M->st_buf = cap;
-                M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
-                M->d_chunks.upload(ch, st);
-                M->d_chunk0.upload(c0, st);
-                DevBuf<int> d_csub;
-                d_csub.upload(csub, st);
-                M->d_img.alloc((size_t)quads);
-                if (fp32)
-                    hipLaunchKernelGGL(k_stage_image<float>, dim3((int)ch.size()), dim3(256), 0, st,
-                                       M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p, M->blev.p,
-                                       M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p, M->b_col.p,
-                                       M->b_v32.p, M->b_d32.p, M->d_img.p);
-                else
-                    hipLaunchKernelGGL(k_stage_image<double>, dim3((int)ch.size()), dim3(256), 0,
-                                       st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
-                                       M->blev.p, M->f_row.p, M->f_col.p, M->f_v64.p, M->b_row.p,
-                                       M->b_col.p, M->b_v64.p, M->b_d64.p, M->d_img.p);
-                SG_HIP(hipGetLastError());
-                SG_HIP(hipStreamSynchronize(st));  // d_csub ends here
-            }
+            const size_t need = al(max_nch * sizeof(SChunk));
+            if (need <= ct_bytes) planned = true;
+            else ct_bytes = need;
+        }
+        if (planned && quads <= INT32_MAX) {
+            M->st_ok = true;
+            M->st_buf = cap;
+            M->st_buf_off = M->st_ct_off + ct_bytes;
+            M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
+            M->d_chunks.upload(ch, st);
+            M->d_chunk0.upload(c0, st);
+            DevBuf<int> d_csub;
+            d_csub.upload(csub, st);
+            M->d_img.alloc((size_t)quads);
+            if (fp32)
+                hipLaunchKernelGGL(k_stage_image<float>, dim3((int)ch.size()), dim3(256), 0, st,
+                                   M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p, M->blev.p,
+                                   M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p, M->b_col.p,
+                                   M->b_v32.p, M->b_d32.p, M->d_img.p);
+            else
+                hipLaunchKernelGGL(k_stage_image<double>, dim3((int)ch.size()), dim3(256), 0,
+                                   st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                   M->blev.p, M->f_row.p, M->f_col.p, M->f_v64.p, M->b_row.p,
+                                   M->b_col.p, M->b_v64.p, M->b_d64.p, M->d_img.p);
+            SG_HIP(hipGetLastError());
+            SG_HIP(hipStreamSynchronize(st));  // d_csub ends here
         }
     }
     {
-        // staged ring sweep: ring of R values after the tables, then two chunk
-        // buffers; every reference must stay in the ring (no global copies)
+        // staged ring sweep: the chunk descriptors and a ring of R values after
+        // the tables, then two chunk buffers; every reference must stay in the
+        // ring (no global copies)
         auto al = [](size_t b) { return (b + 15) / 16 * 16; };
         M->sr_ok = false;
         bool fits = M->ring && M->tab_in_lds;
         for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
         for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
         const int cap = kStageBytes;
-        const size_t ring_off = M->lds_tab_bytes;
-        const long long room = (long long)ctx->max_lds - (long long)ring_off - 2LL * cap - 32;
-        const int R = (int)std::min<long long>(room / (long long)tsz - 2, INT32_MAX);
-        std::vector<std::vector<int>> fcs(nsub), bcs(nsub), bps(nsub);
-        std::vector<int> Fs(nsub, 0);
-        if (fits && R >= 64 && max_w <= R)
-            run_parallel(nsub, [&](int s) {
-                Fs[s] = ring_encode(fw[s], bw[s], local[s].n, R, fcs[s], bcs[s], bps[s]);
-            });
-        bool allring = fits && R >= 64 && max_w <= R;
-        for (int f : Fs) allring = allring && f == INT_MAX;
-        if (allring) {
+        if (fits) {
             const int rq = fp32 ? sring_rec_quads<float>() : sring_rec_quads<double>();
             auto ownq = [&](int nrow) {
                 return fp32 ? sring_own_quads<float>(nrow) : sring_own_quads<double>(nrow);
@@ -2186,6 +2243,21 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                 }
             }
             ok = ok && qimg <= INT32_MAX && qown * per_q <= INT32_MAX;
+            size_t max_nch = 0;
+            for (int s = 0; s < nsub; ++s)
+                max_nch = std::max(max_nch, (size_t)c0[4 * s + 1] + (size_t)c0[4 * s + 3]);
+            const size_t ct_bytes = al(max_nch * sizeof(SChunkR));
+            const size_t ring_off = M->lds_tab_bytes + ct_bytes;
+            const long long room = (long long)ctx->max_lds - (long long)ring_off - 2LL * cap - 32;
+            const int R = (int)std::min<long long>(room / (long long)tsz - 2, INT32_MAX);
+            std::vector<std::vector<int>> fcs(nsub), bcs(nsub), bps(nsub);
+            std::vector<int> Fs(nsub, 0);
+            ok = ok && R >= 64 && max_w <= R;
+            if (ok)
+                run_parallel(nsub, [&](int s) {
+                    Fs[s] = ring_encode(fw[s], bw[s], local[s].n, R, fcs[s], bcs[s], bps[s]);
+                });
+            for (int f : Fs) ok = ok && f == INT_MAX;
             if (ok) {
                 std::vector<int> fdst(pl.f_row.size(), 0), fgid(pl.f_row.size(), 0),
                     bdst(pl.b_row.size(), 0), fc, bc;
@@ -2510,7 +2582,7 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
                        (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p, M->d_own.p, \
                        M->d_own_gid.p, x, y, zo, cz, guard, (int)M->lds_tab_bytes,              \
-                       (int)M->st_buf_off, M->st_buf / 4)
+                       (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
         if (sh.threads == 1024) SG_ST(1024);
         else if (sh.threads == 512) SG_ST(512);
         else SG_ST(256);
@@ -2535,8 +2607,9 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_rchunk0.p,               \
                        M->d_rchunks.p, (const stage_q*)M->d_rimg.p, (stage_q*)M->d_rown.p,      \
                        M->d_fslot.p, M->d_fgid.p, M->flev.p, M->blev.p, M->d_own.p,             \
-                       M->d_own_gid.p, x, y, zg, M->zstride, zo, cz, guard, (int)M->sr_ring_off, \
-                       (int)M->sr_buf_off, M->sr_buf / 16, M->sr_R)
+                       M->d_own_gid.p, x, y, zg, M->zstride, zo, cz, guard,                    \
+                       (int)M->lds_tab_bytes, (int)M->sr_ring_off, (int)M->sr_buf_off,          \
+                       M->sr_buf / 16, M->sr_R)
         if (sh.threads == 1024) SG_SR(1024);
         else if (sh.threads == 512) SG_SR(512);
         else SG_SR(256);
