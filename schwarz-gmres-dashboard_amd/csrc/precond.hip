@@ -1207,24 +1207,27 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         if (c.bwd) stage_levels<T, true, NT>(z, lb, c, buf, si.ns);
         else stage_levels<T, false, NT>(z, lf, c, buf, si.ns);
     };
-    for (int ci = 0; ci < nch; ci += 2) {
-        run(c0q, q0);
-        if (ci + 1 >= nch) break;
-        const SChunk nb = chunk_at(ci + 3);  // read before the store's waits
+    // each step stores the next chunk into the other buffer (free since the
+    // barrier that ended the previous step) and reloads its register set
+    // before computing: the LDS writes drain while the levels run, and the
+    // levels' last barrier publishes them.  Pairs of steps with no exit
+    // inside, so the compiler's waits stay exact (set B's loads, then set A's)
+    int ci = 0;
+    for (; ci + 1 < nch; ci += 2) {
+        const SChunk nb = chunk_at(ci + 3);
         stage_store<T, NT>(RB, dB, q1, trash);
         c1q = dB;
-        lds_barrier();
         stage_load<T, NT>(RB, nb, img);
         dB = nb;
-        run(c1q, q1);
-        if (ci + 2 >= nch) break;
+        run(c0q, q0);
         const SChunk na = chunk_at(ci + 4);
         stage_store<T, NT>(RA, dA, q0, trash);
         c0q = dA;
-        lds_barrier();
         stage_load<T, NT>(RA, na, img);
         dA = na;
+        run(c1q, q1);
     }
+    if (ci < nch) run(c0q, q0);  // odd count: the last chunk is in buffer 0
     __syncthreads();
     if (MODE == 0) {
         scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
@@ -1483,24 +1486,22 @@ k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ or
         dA = chunk_at(2);
         sring_load<T, NT>(RA, dA, img, own);
         lds_barrier();
-        for (int ci = 0; ci < nch; ci += 2) {
-            run(c0q, q0);
-            if (ci + 1 >= nch) break;
-            const SChunkR nb = chunk_at(ci + 3);  // read before the store's waits
+        int ci = 0;
+        for (; ci + 1 < nch; ci += 2) {
+            const SChunkR nb = chunk_at(ci + 3);
             sring_store<T, NT>(RB, dB, q1, trash);
             c1q = dB;
-            lds_barrier();
             sring_load<T, NT>(RB, nb, img, own);
             dB = nb;
-            run(c1q, q1);
-            if (ci + 2 >= nch) break;
+            run(c0q, q0);
             const SChunkR na = chunk_at(ci + 4);
             sring_store<T, NT>(RA, dA, q0, trash);
             c0q = dA;
-            lds_barrier();
             sring_load<T, NT>(RA, na, img, own);
             dA = na;
+            run(c1q, q1);
         }
+        if (ci < nch) run(c0q, q0);
     };
     if (si.nflev) pipeline(0, nfc, lf, false, pf0, ownT);
     __syncthreads();  // backward own values (the forward results) are in global memory
