@@ -178,6 +178,7 @@ struct sg_precond {
     sg::DevBuf<int> f_row, f_col, f_src;
     sg::DevBuf<int> b_row, b_col, b_src, b_dsrc;
     sg::DevBuf<int> d_dofs, d_own, d_own_gid;
+    sg::DevBuf<int> d_own2;  // (local row, global id) pairs of the owned rows (staged sweeps)
     sg::DevBuf<int> lrp, lci, ldiag;
     sg::DevBuf<double> lval;  // factored FP64 values in local CSR order
     long long ilut_l_nnz = 0, ilut_u_nnz = 0;  // sizes of the reference's L / U arrays
@@ -503,6 +504,32 @@ __device__ __forceinline__ void scatter_owned(const int* __restrict__ ow,
             if (g[u] >= 0) {
                 const T v = z[li[u]];
                 y[g[u]] = has_cz ? (double)(v + add) : (double)v;
+            }
+    }
+}
+
+// The same for the staged sweeps, from (local row, global id) pairs, U per
+// thread in flight.  The caller loads the first batch (own_batch) before the
+// last chunk's levels run, so that its latency is hidden; entries past nown
+// (nown >= 1) repeat the last pair and are not stored.
+typedef int own_i2 __attribute__((ext_vector_type(2)));
+template <int NT, int U>
+__device__ __forceinline__ void own_batch(own_i2 (&b)[U], const own_i2* __restrict__ o2, int nown,
+                                          int k0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[u] = o2[min(k0 + u * NT, nown - 1)];
+}
+template <int NT, int U, typename T, typename ZP>
+__device__ __forceinline__ void scatter_owned2(own_i2 (&b)[U], const own_i2* __restrict__ o2,
+                                               int nown, ZP z, bool has_cz, T add,
+                                               double* __restrict__ y) {
+    for (int k0 = threadIdx.x; k0 < nown; k0 += U * NT) {
+        if (k0 != (int)threadIdx.x) own_batch<NT, U>(b, o2, nown, k0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u * NT < nown) {
+                const T v = z[b[u].x];
+                y[b[u].y] = has_cz ? (double)(v + add) : (double)v;
             }
     }
 }
@@ -1148,10 +1175,9 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                const int* __restrict__ chunk0, const SChunk* __restrict__ chunks,
                const stage_q* __restrict__ img, const Level* __restrict__ flev,
                const Level* __restrict__ blev, const int* __restrict__ dofs,
-               const int* __restrict__ own, const int* __restrict__ own_gid,
-               const double* __restrict__ x, double* __restrict__ y, T* __restrict__ zout,
-               const T* __restrict__ cz, DevGuard guard, int z_off, int ct_off, int buf_off,
-               int buf_words) {
+               const own_i2* __restrict__ own2, const double* __restrict__ x,
+               double* __restrict__ y, T* __restrict__ zout, const T* __restrict__ cz,
+               DevGuard guard, int z_off, int ct_off, int buf_off, int buf_words) {
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
@@ -1227,11 +1253,13 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         dA = na;
         run(c1q, q1);
     }
+    constexpr int US = NT >= 1024 ? 16 : 32;
+    own_i2 ob[US];
+    if (MODE == 0 && si.nown > 0) own_batch<NT, US>(ob, own2 + si.own0, si.nown, threadIdx.x);
     if (ci < nch) run(c0q, q0);  // odd count: the last chunk is in buffer 0
     __syncthreads();
     if (MODE == 0) {
-        scatter_owned<NT>(own + si.own0, own_gid + si.own0, si.nown, z, cz != nullptr,
-                          cz ? cz[s] : (T)0, y);
+        scatter_owned2<NT, US>(ob, own2 + si.own0, si.nown, z, cz != nullptr, cz ? cz[s] : (T)0, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
     }
@@ -2052,6 +2080,14 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
     if (M->ring) M->f_bpos.upload(pl.f_bpos, st);
     M->d_own.upload(pl.own, st);
     M->d_own_gid.upload(pl.own_gid, st);
+    {
+        std::vector<int> o2(2 * pl.own.size());
+        for (size_t k = 0; k < pl.own.size(); ++k) {
+            o2[2 * k] = pl.own[k];
+            o2[2 * k + 1] = pl.own_gid[k];
+        }
+        M->d_own2.upload(o2, st);
+    }
     M->lrp.upload(pl.lrp, st);
     M->lci.upload(pl.lci, st);
     M->ldiag.upload(pl.ldiag, st);
@@ -2581,8 +2617,8 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
 #define SG_ST(NT)                                                                                 \
     hipLaunchKernelGGL((k_sweep_staged<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->st_bytes,     \
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
-                       (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p, M->d_own.p, \
-                       M->d_own_gid.p, x, y, zo, cz, guard, (int)M->lds_tab_bytes,              \
+                       (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p,          \
+                       (const own_i2*)M->d_own2.p, x, y, zo, cz, guard, (int)M->lds_tab_bytes,  \
                        (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
         if (sh.threads == 1024) SG_ST(1024);
         else if (sh.threads == 512) SG_ST(512);
