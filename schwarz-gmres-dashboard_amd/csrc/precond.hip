@@ -1167,6 +1167,54 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
     }
 }
 
+// The coarse term of one subdomain folded into its staged sweep (two-level
+// RAS with the precomputed coarse inverse): cz_s = c_s (row s of the inverse
+// . r0), r0[t] the sum of the restriction's split partials of coarse row t.
+// The operations and their order are those of k_coarse_inv for row s, so the
+// value is the same bit for bit; every wavefront computes it from loads issued
+// before the gather (nc <= kFoldRows, split == kRestrictSplit).
+constexpr int kFoldRows = 256;
+template <typename T>
+struct CoarseFold {
+    const T* inv;  // nullptr: no fold
+    const T* r0;
+    const T* c;
+    int nc;
+};
+template <typename T>
+struct FoldRegs {
+    T r[kFoldRows / 64][kRestrictSplit], i[kFoldRows / 64], c;
+};
+template <typename T>
+__device__ __forceinline__ void fold_load(FoldRegs<T>& f, const CoarseFold<T>& cf, int s) {
+    const int lane = threadIdx.x & 63;
+    const T* ri = cf.inv + (size_t)s * cf.nc;
+#pragma unroll
+    for (int k = 0; k < kFoldRows / 64; ++k) {
+        const int j = min(lane + 64 * k, cf.nc - 1);
+        f.i[k] = ri[j];
+#pragma unroll
+        for (int q = 0; q < kRestrictSplit; ++q) f.r[k][q] = cf.r0[j * kRestrictSplit + q];
+    }
+    f.c = cf.c[s];
+}
+template <typename T>
+__device__ __forceinline__ T fold_value(const FoldRegs<T>& f, int nc) {
+    const int lane = threadIdx.x & 63;
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < kFoldRows / 64; ++k)
+        if (lane + 64 * k < nc) {
+            T r = 0;
+#pragma unroll
+            for (int q = 0; q < kRestrictSplit; ++q) r += f.r[k][q];
+            acc += f.i[k] * r;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    return f.c * acc;
+}
+
 // LDS: level tables (forward | backward, four ints per level) | z (ns values,
 // zero slot, trash slot) | chunk descriptors | buffer 0 | buffer 1 | trash word.
 template <typename T, int MODE, int NT>
@@ -1177,7 +1225,8 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                const Level* __restrict__ blev, const int* __restrict__ dofs,
                const own_i2* __restrict__ own2, const double* __restrict__ x,
                double* __restrict__ y, T* __restrict__ zout, const T* __restrict__ cz,
-               DevGuard guard, int z_off, int ct_off, int buf_off, int buf_words) {
+               CoarseFold<T> cf, DevGuard guard, int z_off, int ct_off, int buf_off,
+               int buf_words) {
     extern __shared__ unsigned char lds_raw[];
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
@@ -1215,12 +1264,21 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         c.img = __builtin_amdgcn_readfirstlane(b.y);
         return c;
     };
+    // the coarse term's loads go first: the gather's waits cover them (the
+    // host folds only into shapes of at most 512 threads)
+    constexpr bool kFold = MODE == 0 && NT <= 512;
+    FoldRegs<T> fr;
+    if constexpr (kFold)
+        if (cf.inv) fold_load<T>(fr, cf, s);
     // descriptors of the chunks held in RA and RB, and in buffers q0 and q1
     SChunk dA = chunk_g(0), dB = chunk_g(1);
     stage_load<T, NT>(RA, dA, img);
     stage_load<T, NT>(RB, dB, img);
     gather_x<NT, T, lds_t<T>*, 24>(dofs + si.row0, si.ns, x, z);
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
+    T cfold = (T)0;
+    if constexpr (kFold)
+        if (cf.inv) cfold = fold_value<T>(fr, cf.nc);
     lds_t<stage_q>* trash = (lds_t<stage_q>*)(bufs + 2 * buf_words);
     lds_t<stage_q>* q0 = (lds_t<stage_q>*)bufs;
     lds_t<stage_q>* q1 = (lds_t<stage_q>*)(bufs + buf_words);
@@ -1259,7 +1317,9 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     if (ci < nch) run(c0q, q0);  // odd count: the last chunk is in buffer 0
     __syncthreads();
     if (MODE == 0) {
-        scatter_owned2<NT, US>(ob, own2 + si.own0, si.nown, z, cz != nullptr, cz ? cz[s] : (T)0, y);
+        const bool has_c = cf.inv != nullptr || cz != nullptr;
+        const T add = cf.inv ? cfold : cz ? cz[s] : (T)0;
+        scatter_owned2<NT, US>(ob, own2 + si.own0, si.nown, z, has_c, add, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
     }
@@ -2611,14 +2671,15 @@ static void setup(sg_precond* M, const sg_csr* A) {
 
 template <typename T, bool ZLDS, int MODE>
 static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
-                         const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+                         const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz,
+                         const CoarseFold<T>& fold) {
     const SweepShape sh = kSweepShapes[M->sweep_variant];
     if (ZLDS && sh.staged && M->st_ok) {
 #define SG_ST(NT)                                                                                 \
     hipLaunchKernelGGL((k_sweep_staged<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->st_bytes,     \
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
                        (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p,          \
-                       (const own_i2*)M->d_own2.p, x, y, zo, cz, guard, (int)M->lds_tab_bytes,  \
+                       (const own_i2*)M->d_own2.p, x, y, zo, cz, fold, guard, (int)M->lds_tab_bytes, \
                        (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
         if (sh.threads == 1024) SG_ST(1024);
         else if (sh.threads == 512) SG_ST(512);
@@ -2696,14 +2757,15 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
 
 template <typename T>
 static void launch_any_sweep(sg_precond* M, const double* x, double* y, const DevGuard& guard,
-                             const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+                             const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz,
+                             const CoarseFold<T>& fold = CoarseFold<T>{}) {
     const bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
     if (additive) {
-        if (M->z_in_lds) launch_sweep<T, true, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-        else launch_sweep<T, false, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+        if (M->z_in_lds) launch_sweep<T, true, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz, fold);
+        else launch_sweep<T, false, 1>(M, x, y, guard, fv, bv, bd, zg, zo, cz, fold);
     } else {
-        if (M->z_in_lds) launch_sweep<T, true, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
-        else launch_sweep<T, false, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz);
+        if (M->z_in_lds) launch_sweep<T, true, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz, fold);
+        else launch_sweep<T, false, 0>(M, x, y, guard, fv, bv, bd, zg, zo, cz, fold);
     }
     SG_HIP(hipGetLastError());
 }
@@ -2776,13 +2838,15 @@ static void tune_sweep(sg_precond* M, const T* fv, const T* bv, const T* bd, T* 
 
 template <typename T>
 static void apply_batch(sg_precond* M, const double* x, double* y, const Guard& guard,
-                        const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz) {
+                        const T* fv, const T* bv, const T* bd, T* zg, T* zo, const T* cz,
+                        const CoarseFold<T>& fold = CoarseFold<T>{}) {
     hipStream_t st = M->ctx->stream;
     bool additive = (M->kind == SG_PC_ASM || M->kind == SG_PC_TWOLEVEL_ASM);
     tune_sweep<T>(M, fv, bv, bd, zg, zo);
     {
         ProfScope ps(M->ctx, SG_K_TRSV);
-        launch_any_sweep<T>(M, x, y, dev_guard(guard, M->d_sub_sys.p), fv, bv, bd, zg, zo, cz);
+        launch_any_sweep<T>(M, x, y, dev_guard(guard, M->d_sub_sys.p), fv, bv, bd, zg, zo, cz,
+                            fold);
     }
     if (additive) {
         hipLaunchKernelGGL(k_asm_gather<T>, dim3(grid_for(M->n)), dim3(256), 0, st, M->n,
@@ -2792,15 +2856,27 @@ static void apply_batch(sg_precond* M, const double* x, double* y, const Guard& 
     }
 }
 
+// Whether the staged sweep forms the coarse term itself (k_coarse_inv is
+// skipped): two-level RAS with the coarse inverse, staged sweep of at most 512
+// threads (the fold's registers), at most kFoldRows coarse rows.
+static bool coarse_fold(const sg_precond* M) {
+    const SweepShape& sh = kSweepShapes[M->sweep_variant];
+    return M->kind == SG_PC_TWOLEVEL_RAS && M->opts.coarse_solve == 1 && M->z_in_lds &&
+           M->st_ok && sh.staged && sh.threads <= 512 && M->nc <= kFoldRows &&
+           M->restrict_split == kRestrictSplit;
+}
+
 template <typename T>
 static const T* coarse_launch(sg_precond* M, const double* x, const Guard& guard, const T* LUt,
-                              const T* inv, const T* c, T* r0, T* cz) {
+                              const T* inv, const T* c, T* r0, T* cz, bool fold) {
     hipStream_t st = M->ctx->stream;
     int nc = M->nc, split = M->restrict_split;
     hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(nc * split), dim3(256), 0, st, split,
                        M->d_subs.p, M->d_own_gid.p, c, x, r0, dev_guard(guard, M->d_sub_sys.p));
     const DevGuard g0 = dev_guard(guard, nullptr);
-    if (M->opts.coarse_solve == 1)
+    if (fold) {
+        // the staged sweep forms cz itself
+    } else if (M->opts.coarse_solve == 1)
         hipLaunchKernelGGL(k_coarse_inv<T>, dim3((nc + 3) / 4), dim3(256), (size_t)nc * sizeof(T),
                            st, nc, split, inv, r0, c, cz, g0);
     else
@@ -2859,16 +2935,25 @@ void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard
         return;
     }
     const bool two = (M->kind == SG_PC_TWOLEVEL_ASM || M->kind == SG_PC_TWOLEVEL_RAS);
+    // the sweep shape is settled first: it decides whether the coarse term is folded
+    precond_ensure_tuned(M);
+    const bool fold = two && coarse_fold(M);
     if (M->opts.precision == SG_FP32) {
         const float* cz = two ? coarse_launch<float>(M, x, guard, M->d_LUt32.p, M->d_inv32.p,
-                                                      M->d_c32.p, M->d_r0_32.p, M->d_cz32.p)
+                                                      M->d_c32.p, M->d_r0_32.p, M->d_cz32.p, fold)
                               : nullptr;
-        apply_batch<float>(M, x, y, guard, M->f_v32.p, M->b_v32.p, M->b_d32.p, M->z32.p, M->zout32.p, cz);
+        const CoarseFold<float> cf =
+            fold ? CoarseFold<float>{M->d_inv32.p, M->d_r0_32.p, M->d_c32.p, M->nc} : CoarseFold<float>{};
+        apply_batch<float>(M, x, y, guard, M->f_v32.p, M->b_v32.p, M->b_d32.p, M->z32.p,
+                           M->zout32.p, fold ? nullptr : cz, cf);
     } else {
         const double* cz = two ? coarse_launch<double>(M, x, guard, M->d_LUt64.p, M->d_inv64.p,
-                                                        M->d_c64.p, M->d_r0_64.p, M->d_cz64.p)
+                                                        M->d_c64.p, M->d_r0_64.p, M->d_cz64.p, fold)
                                : nullptr;
-        apply_batch<double>(M, x, y, guard, M->f_v64.p, M->b_v64.p, M->b_d64.p, M->z64.p, M->zout64.p, cz);
+        const CoarseFold<double> cf =
+            fold ? CoarseFold<double>{M->d_inv64.p, M->d_r0_64.p, M->d_c64.p, M->nc} : CoarseFold<double>{};
+        apply_batch<double>(M, x, y, guard, M->f_v64.p, M->b_v64.p, M->b_d64.p, M->z64.p,
+                            M->zout64.p, fold ? nullptr : cz, cf);
     }
 }
 

@@ -356,7 +356,7 @@ def test_fp32_apply_matches_oracle_mixed_mode(sg, ctx, kind):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_kernel_shapes_do_not_change_results(sg, precision):
     """Every SpMV block shape and sweep workgroup size (forced, and the one the
-    autotuner picks) gives bit-identical SpMV and RAS/ASM applies."""
+    autotuner picks) gives bit-identical SpMV and RAS/ASM/two-level applies."""
     A = O.poisson3d(40, 36, 30)
     rng = np.random.default_rng(11)
     v = A.v * rng.uniform(0.5, 1.5, A.nnz)
@@ -370,9 +370,12 @@ def test_kernel_shapes_do_not_change_results(sg, precision):
         y = dev(sg, c, np.zeros(A.n))
         dA.spmv(dx, y)
         res = [y.get()]
-        for kind in ("ras", "asm", "twolevel_ras"):
+        # coarse_solve=1: the staged shapes of at most 512 threads form the
+        # coarse term inside the sweep, the others in k_coarse_inv
+        for kind, kw in (("ras", {}), ("asm", {}), ("twolevel_ras", {}),
+                         ("twolevel_ras", {"coarse_solve": 1})):
             M = sg.Precond(c, kind, nparts=16, overlap=1, precision=precision,
-                           partition=np.arange(A.n, dtype=np.int32) * 16 // A.n).setup(dA)
+                           partition=np.arange(A.n, dtype=np.int32) * 16 // A.n, **kw).setup(dA)
             for _ in range(2):  # first call tunes in mode 1
                 z = dev(sg, c, np.zeros(A.n))
                 M.apply(dx, z)
