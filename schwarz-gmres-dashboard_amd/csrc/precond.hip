@@ -1232,6 +1232,7 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
     const int c0 = chunk0[2 * s], nch = chunk0[2 * s + 1];
+    if (nch == 0) return;  // an empty subdomain: no rows, nothing owned
     lds_t<int>* lf = (lds_t<int>*)lds_raw;
     lds_t<int>* lb = lf + 4 * si.nflev;
     lds_t<T>* z = (lds_t<T>*)(lds_raw + z_off);
