@@ -1312,7 +1312,9 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         dA = na;
         run(c1q, q1);
     }
-    constexpr int US = NT >= 1024 ? 16 : 32;
+    // owned rows per thread in flight: a whole subdomain of up to 24K owned rows
+    // in one batch (1024 threads: 16K, within the register budget)
+    constexpr int US = NT >= 1024 ? 16 : NT >= 512 ? 48 : 64;
     own_i2 ob[US];
     if (MODE == 0 && si.nown > 0) own_batch<NT, US>(ob, own2 + si.own0, si.nown, threadIdx.x);
     if (ci < nch) run(c0q, q0);  // odd count: the last chunk is in buffer 0
