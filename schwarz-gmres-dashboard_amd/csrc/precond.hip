@@ -28,6 +28,7 @@
 #include <atomic>
 #include <exception>
 #include <numeric>
+#include <type_traits>
 #include <stdexcept>
 #include <thread>
 
@@ -157,6 +158,7 @@ struct sg_precond {
     // staged sweep: chunks of every subdomain (forward ones first); LDS =
     // level tables | z | chunk descriptors | two buffers of st_buf bytes
     bool st_ok = false;
+    bool st_k3 = false;  // FP32 records of three entries per row (every level has K <= 3)
     int st_buf = 0;
     size_t st_ct_off = 0, st_buf_off = 0, st_bytes = 0;
     sg::DevBuf<sg::SChunk> d_chunks;
@@ -963,6 +965,7 @@ k_sweep_syncfree(const SubInfo* __restrict__ subs, const int* __restrict__ order
 // stores of one dwordx4 / ds_read_b128 / ds_write_b128).
 typedef unsigned stage_q __attribute__((ext_vector_type(4)));
 typedef int stage_i4 __attribute__((ext_vector_type(4)));
+typedef unsigned stage_d __attribute__((ext_vector_type(2)));
 
 // 16-byte words per row record: [row | c0<<16, c1 | c2<<16, c3, diag (FP32)]
 // then the four values (FP32: one quad; FP64: two quads, then the diagonal).
@@ -970,9 +973,15 @@ template <typename T>
 __host__ __device__ constexpr int stage_rec_quads() {
     return sizeof(T) == 4 ? 2 : 4;
 }
-template <typename T>
-__host__ __device__ inline int stage_quads(const SChunk& c) {
-    return c.nrow * stage_rec_quads<T>();
+// Record formats (KR).  KR = 4, any T: the records above.  KR = 3, FP32 only,
+// when every level of the plan has at most three entries per row: per chunk an
+// index block of 8 bytes per row [row | c0<<16, c1 | c2<<16] (padded to 16
+// bytes), then a value block of 16 bytes per row [v0, v1, v2, diag]: 24 instead
+// of 32 bytes per row, and one z read and one product fewer (the dropped
+// padding term subtracts +0, so the values do not change).
+template <typename T, int KR>
+__host__ __device__ inline int stage_quads_n(int nrow) {
+    return KR == 3 ? (nrow + 1) / 2 + nrow : nrow * stage_rec_quads<T>();
 }
 
 template <typename T, int NT>
@@ -984,12 +993,12 @@ struct StageRegs {
 // Issues the 16-byte loads of chunk c from its image (nothing touches the
 // registers until stage_store); indices past the chunk are clamped to its
 // last word, so every load is issued and the waits stay exact.
-template <typename T, int NT>
+template <typename T, int NT, int KR>
 __device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
                                            const stage_q* __restrict__ img) {
     const __attribute__((address_space(1))) stage_q* src =
         (const __attribute__((address_space(1))) stage_q*)(img + c.img);
-    const int n = stage_quads<T>(c);
+    const int n = stage_quads_n<T, KR>(c.nrow);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i)
         R.v[i] = src[min((int)threadIdx.x + i * NT, n - 1)];
@@ -997,10 +1006,10 @@ __device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
 
 // Words past the chunk go to a trash word (straight-line code: a branch here
 // would make the compiler wait for every load in flight, the other set's too).
-template <typename T, int NT>
+template <typename T, int NT, int KR>
 __device__ __forceinline__ void stage_store(const StageRegs<T, NT>& R, const SChunk& c,
                                             lds_t<stage_q>* buf, lds_t<stage_q>* trash) {
-    const int n = stage_quads<T>(c);
+    const int n = stage_quads_n<T, KR>(c.nrow);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
         const int u = (int)threadIdx.x + i * NT;
@@ -1011,7 +1020,7 @@ __device__ __forceinline__ void stage_store(const StageRegs<T, NT>& R, const SCh
 // Setup: packs the rows of every chunk into records (one workgroup per chunk,
 // one thread per row).  Entries past a row's count are column zs (the zero
 // slot of z) with value 0, as in the other sweeps' padding.
-template <typename T>
+template <typename T, int KR>
 __global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* __restrict__ subs,
                               const int* __restrict__ chunk_sub, const Level* __restrict__ flev,
                               const Level* __restrict__ blev, const int* __restrict__ f_row,
@@ -1039,9 +1048,19 @@ __global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* 
                 vv[k] = k < L.K ? vals[L.ell + k * L.size + q] : (T)0;
             }
             const T dg = c.bwd ? b_diag[L.start + q] : (T)0;
-            uint4* r = img + c.img + (size_t)(L.start - c.row0 + q) * stage_rec_quads<T>();
             unsigned dbits[2] = {0u, 0u};
             __builtin_memcpy(dbits, &dg, sizeof(T));
+            if constexpr (KR == 3) {
+                const int pr = L.start - c.row0 + q;  // row within the chunk
+                unsigned* ib = (unsigned*)(img + c.img);
+                ib[2 * pr] = (unsigned)rows[L.start + q] | (cc[0] << 16);
+                ib[2 * pr + 1] = cc[1] | (cc[2] << 16);
+                unsigned vb[kKreg];
+                __builtin_memcpy(vb, vv, sizeof(vv));
+                img[c.img + (c.nrow + 1) / 2 + pr] = make_uint4(vb[0], vb[1], vb[2], dbits[0]);
+                continue;
+            }
+            uint4* r = img + c.img + (size_t)(L.start - c.row0 + q) * stage_rec_quads<T>();
             r[0] = make_uint4((unsigned)rows[L.start + q] | (cc[0] << 16), cc[1] | (cc[2] << 16),
                               cc[3], sizeof(T) == 4 ? dbits[0] : 0u);
             unsigned vb[2 * kKreg];
@@ -1058,15 +1077,27 @@ __global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* 
 }
 
 // One row of a level as read from the chunk buffer (raw 16-byte words).
-template <typename T>
+template <typename T, int KR>
 struct StageRow {
     stage_q q[stage_rec_quads<T>()];
 };
+template <>
+struct StageRow<float, 3> {
+    stage_d ix;  // row | c0<<16, c1 | c2<<16
+    stage_q v;   // v0, v1, v2, diag
+};
 
-template <typename T>
-__device__ __forceinline__ void stage_fetch(StageRow<T>& r, const lds_t<stage_q>* buf, int p) {
+// vb: first 16-byte word of the value block (KR = 3)
+template <typename T, int KR>
+__device__ __forceinline__ void stage_fetch(StageRow<T, KR>& r, const lds_t<stage_q>* buf, int p,
+                                            int vb) {
+    if constexpr (KR == 3) {
+        r.ix = ((const lds_t<stage_d>*)buf)[p];
+        r.v = buf[vb + p];
+    } else {
 #pragma unroll
-    for (int i = 0; i < stage_rec_quads<T>(); ++i) r.q[i] = buf[p * stage_rec_quads<T>() + i];
+        for (int i = 0; i < stage_rec_quads<T>(); ++i) r.q[i] = buf[p * stage_rec_quads<T>() + i];
+    }
 }
 
 __device__ __forceinline__ float stage_bits_f(unsigned a) { return __uint_as_float(a); }
@@ -1074,27 +1105,36 @@ __device__ __forceinline__ double stage_bits_d(unsigned lo, unsigned hi) {
     return __hiloint2double((int)hi, (int)lo);
 }
 
-template <typename T>
-__device__ __forceinline__ int stage_row(const StageRow<T>& r) { return r.q[0].x & 0xffffu; }
-template <typename T>
-__device__ __forceinline__ int stage_col(const StageRow<T>& r, int k) {
-    return k == 0 ? (int)(r.q[0].x >> 16)
-         : k == 1 ? (int)(r.q[0].y & 0xffffu)
-         : k == 2 ? (int)(r.q[0].y >> 16)
-                  : (int)(r.q[0].z & 0xffffu);
+template <typename T, int KR>
+__device__ __forceinline__ int stage_row(const StageRow<T, KR>& r) {
+    if constexpr (KR == 3) return r.ix.x & 0xffffu;
+    else return r.q[0].x & 0xffffu;
 }
-template <typename T>
-__device__ __forceinline__ T stage_val(const StageRow<T>& r, int k) {
-    if constexpr (sizeof(T) == 4) {
+template <typename T, int KR>
+__device__ __forceinline__ int stage_col(const StageRow<T, KR>& r, int k) {
+    if constexpr (KR == 3)
+        return k == 0 ? (int)(r.ix.x >> 16) : k == 1 ? (int)(r.ix.y & 0xffffu) : (int)(r.ix.y >> 16);
+    else
+        return k == 0 ? (int)(r.q[0].x >> 16)
+             : k == 1 ? (int)(r.q[0].y & 0xffffu)
+             : k == 2 ? (int)(r.q[0].y >> 16)
+                      : (int)(r.q[0].z & 0xffffu);
+}
+template <typename T, int KR>
+__device__ __forceinline__ T stage_val(const StageRow<T, KR>& r, int k) {
+    if constexpr (KR == 3) {
+        return stage_bits_f(k == 0 ? r.v.x : k == 1 ? r.v.y : r.v.z);
+    } else if constexpr (sizeof(T) == 4) {
         return stage_bits_f(k == 0 ? r.q[1].x : k == 1 ? r.q[1].y : k == 2 ? r.q[1].z : r.q[1].w);
     } else {
         const stage_q& w = r.q[1 + k / 2];
         return k % 2 == 0 ? stage_bits_d(w.x, w.y) : stage_bits_d(w.z, w.w);
     }
 }
-template <typename T>
-__device__ __forceinline__ T stage_diag(const StageRow<T>& r) {
-    if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[0].w);
+template <typename T, int KR>
+__device__ __forceinline__ T stage_diag(const StageRow<T, KR>& r) {
+    if constexpr (KR == 3) return stage_bits_f(r.v.w);
+    else if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[0].w);
     else return stage_bits_d(r.q[3].x, r.q[3].y);
 }
 
@@ -1106,10 +1146,11 @@ __device__ __forceinline__ T stage_diag(const StageRow<T>& r) {
 // costs the SIMDs only the instructions of the wavefronts that hold its rows.
 // Threads past a level's end inside an active wavefront compute a clamped row
 // and write it to the trash slot zs + 1.
-template <typename T, bool BWD, int NT>
+template <typename T, bool BWD, int NT, int KR>
 __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
                                              const SChunk& c, const lds_t<stage_q>* buf,
                                              int zs) {
+    const int vb = (c.nrow + 1) / 2;  // value block (KR = 3)
     const int t = threadIdx.x;
     const int wbase = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
     const int lend = c.lev0 + c.nlev;
@@ -1129,34 +1170,34 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
     const int l1 = min(c.lev0 + 1, lend - 1);
     D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
     D dn = uni(tab[4 * l1], tab[4 * l1 + 1]);
-    StageRow<T> cur;
-    if (wbase < d.size) stage_fetch<T>(cur, buf, d.p + min(t, d.size - 1));
+    StageRow<T, KR> cur;
+    if (wbase < d.size) stage_fetch<T, KR>(cur, buf, d.p + min(t, d.size - 1), vb);
     for (int l = c.lev0; l < lend; ++l) {
         // this level's z values first: they are the critical path
-        T zk[kKreg], acc = (T)0;
+        T zk[KR], acc = (T)0;
         if (wbase < d.size) {
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) zk[k] = z[stage_col<T>(cur, k)];
-            acc = z[stage_row<T>(cur)];
+            for (int k = 0; k < KR; ++k) zk[k] = z[stage_col<T, KR>(cur, k)];
+            acc = z[stage_row<T, KR>(cur)];
         }
         // then the next level's records and the descriptor after it
-        StageRow<T> nxt;
-        if (wbase < dn.size) stage_fetch<T>(nxt, buf, dn.p + min(t, dn.size - 1));
+        StageRow<T, KR> nxt;
+        if (wbase < dn.size) stage_fetch<T, KR>(nxt, buf, dn.p + min(t, dn.size - 1), vb);
         const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
         if (wbase < d.size) {
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) acc -= stage_val<T>(cur, k) * zk[k];
-            if (BWD) acc = acc / stage_diag<T>(cur);
-            z[t < d.size ? stage_row<T>(cur) : zs + 1] = acc;
+            for (int k = 0; k < KR; ++k) acc -= stage_val<T, KR>(cur, k) * zk[k];
+            if (BWD) acc = acc / stage_diag<T, KR>(cur);
+            z[t < d.size ? stage_row<T, KR>(cur) : zs + 1] = acc;
             if (d.size > NT) {
                 for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
-                    StageRow<T> r;
-                    stage_fetch<T>(r, buf, d.p + q);
-                    T a = z[stage_row<T>(r)];
+                    StageRow<T, KR> r;
+                    stage_fetch<T, KR>(r, buf, d.p + q, vb);
+                    T a = z[stage_row<T, KR>(r)];
 #pragma unroll
-                    for (int k = 0; k < kKreg; ++k) a -= stage_val<T>(r, k) * z[stage_col<T>(r, k)];
-                    if (BWD) a = a / stage_diag<T>(r);
-                    z[stage_row<T>(r)] = a;
+                    for (int k = 0; k < KR; ++k) a -= stage_val<T, KR>(r, k) * z[stage_col<T, KR>(r, k)];
+                    if (BWD) a = a / stage_diag<T, KR>(r);
+                    z[stage_row<T, KR>(r)] = a;
                 }
             }
         }
@@ -1217,7 +1258,7 @@ __device__ __forceinline__ T fold_value(const FoldRegs<T>& f, int nc) {
 
 // LDS: level tables (forward | backward, four ints per level) | z (ns values,
 // zero slot, trash slot) | chunk descriptors | buffer 0 | buffer 1 | trash word.
-template <typename T, int MODE, int NT>
+template <typename T, int MODE, int NT, int KR>
 __global__ void __launch_bounds__(NT)
 k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                const int* __restrict__ chunk0, const SChunk* __restrict__ chunks,
@@ -1273,8 +1314,8 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
         if (cf.inv) fold_load<T>(fr, cf, s);
     // descriptors of the chunks held in RA and RB, and in buffers q0 and q1
     SChunk dA = chunk_g(0), dB = chunk_g(1);
-    stage_load<T, NT>(RA, dA, img);
-    stage_load<T, NT>(RB, dB, img);
+    stage_load<T, NT, KR>(RA, dA, img);
+    stage_load<T, NT, KR>(RB, dB, img);
     gather_x<NT, T, lds_t<T>*, 24>(dofs + si.row0, si.ns, x, z);
     if (threadIdx.x == 0) z[si.ns] = (T)0;  // zero slot for padding entries
     T cfold = (T)0;
@@ -1284,13 +1325,13 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     lds_t<stage_q>* q0 = (lds_t<stage_q>*)bufs;
     lds_t<stage_q>* q1 = (lds_t<stage_q>*)(bufs + buf_words);
     SChunk c0q = dA, c1q;
-    stage_store<T, NT>(RA, dA, q0, trash);
+    stage_store<T, NT, KR>(RA, dA, q0, trash);
     dA = chunk_g(2);
-    stage_load<T, NT>(RA, dA, img);
+    stage_load<T, NT, KR>(RA, dA, img);
     __syncthreads();
     auto run = [&](const SChunk& c, const lds_t<stage_q>* buf) {
-        if (c.bwd) stage_levels<T, true, NT>(z, lb, c, buf, si.ns);
-        else stage_levels<T, false, NT>(z, lf, c, buf, si.ns);
+        if (c.bwd) stage_levels<T, true, NT, KR>(z, lb, c, buf, si.ns);
+        else stage_levels<T, false, NT, KR>(z, lf, c, buf, si.ns);
     };
     // each step stores the next chunk into the other buffer (free since the
     // barrier that ended the previous step) and reloads its register set
@@ -1300,15 +1341,15 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     int ci = 0;
     for (; ci + 1 < nch; ci += 2) {
         const SChunk nb = chunk_at(ci + 3);
-        stage_store<T, NT>(RB, dB, q1, trash);
+        stage_store<T, NT, KR>(RB, dB, q1, trash);
         c1q = dB;
-        stage_load<T, NT>(RB, nb, img);
+        stage_load<T, NT, KR>(RB, nb, img);
         dB = nb;
         run(c0q, q0);
         const SChunk na = chunk_at(ci + 4);
-        stage_store<T, NT>(RA, dA, q0, trash);
+        stage_store<T, NT, KR>(RA, dA, q0, trash);
         c0q = dA;
-        stage_load<T, NT>(RA, na, img);
+        stage_load<T, NT, KR>(RA, na, img);
         dA = na;
         run(c1q, q1);
     }
@@ -2209,6 +2250,12 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
         for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
         const int rq = fp32 ? stage_rec_quads<float>() : stage_rec_quads<double>();
+        bool k3 = fp32;
+        for (const Level& lv : pl.flev) k3 = k3 && lv.K <= 3;
+        for (const Level& lv : pl.blev) k3 = k3 && lv.K <= 3;
+        auto chunk_quads = [&](int nrow) {
+            return k3 ? (long long)stage_quads_n<float, 3>(nrow) : (long long)nrow * rq;
+        };
         std::vector<SChunk> ch;
         std::vector<int> c0(2 * (size_t)nsub), csub;
         long long quads = 0;
@@ -2234,7 +2281,8 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                     const Level* lv = dir ? &pl.blev[si.blev0] : &pl.flev[si.flev0];
                     // chunks of whole levels while they fit; a level larger than
                     // a buffer is cut into pieces of rows (its rows are independent)
-                    const int maxr = capq / rq;
+                    int maxr = capq / rq;
+                    while (chunk_quads(maxr + 1) <= capq) ++maxr;
                     const int pos_end = nl ? lv[nl - 1].start + lv[nl - 1].size : 0;
                     int l = 0, pos = nl ? lv[0].start : 0;
                     while (pos < pos_end) {
@@ -2247,7 +2295,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                         while (la < nl && lv[la].start < end) ++la;
                         ch.push_back(SChunk{l, la - l, pos, end - pos, dir, (int)quads, {0, 0}});
                         csub.push_back(s);
-                        quads += (long long)(end - pos) * rq;
+                        quads += chunk_quads(end - pos);
                         pos = end;
                     }
                 }
@@ -2260,6 +2308,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
         if (planned && quads <= INT32_MAX) {
             M->st_ok = true;
+            M->st_k3 = k3;
             M->st_buf = cap;
             M->st_buf_off = M->st_ct_off + ct_bytes;
             M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
@@ -2268,13 +2317,18 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             DevBuf<int> d_csub;
             d_csub.upload(csub, st);
             M->d_img.alloc((size_t)quads);
-            if (fp32)
-                hipLaunchKernelGGL(k_stage_image<float>, dim3((int)ch.size()), dim3(256), 0, st,
-                                   M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p, M->blev.p,
-                                   M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p, M->b_col.p,
-                                   M->b_v32.p, M->b_d32.p, M->d_img.p);
+            if (k3)
+                hipLaunchKernelGGL((k_stage_image<float, 3>), dim3((int)ch.size()), dim3(256), 0,
+                                   st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                   M->blev.p, M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p,
+                                   M->b_col.p, M->b_v32.p, M->b_d32.p, M->d_img.p);
+            else if (fp32)
+                hipLaunchKernelGGL((k_stage_image<float, 4>), dim3((int)ch.size()), dim3(256), 0,
+                                   st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                   M->blev.p, M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p,
+                                   M->b_col.p, M->b_v32.p, M->b_d32.p, M->d_img.p);
             else
-                hipLaunchKernelGGL(k_stage_image<double>, dim3((int)ch.size()), dim3(256), 0,
+                hipLaunchKernelGGL((k_stage_image<double, 4>), dim3((int)ch.size()), dim3(256), 0,
                                    st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
                                    M->blev.p, M->f_row.p, M->f_col.p, M->f_v64.p, M->b_row.p,
                                    M->b_col.p, M->b_v64.p, M->b_d64.p, M->d_img.p);
@@ -2467,17 +2521,19 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
         if (M->st_ok) {
             const int stb = (int)M->st_bytes;
-#define SG_SET_ST(T, MD)                                                                          \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 256>,                          \
+#define SG_SET_ST(T, MD, KR)                                                                      \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 256, KR>,                      \
                                hipFuncAttributeMaxDynamicSharedMemorySize, stb));               \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 512>,                          \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 512, KR>,                      \
                                hipFuncAttributeMaxDynamicSharedMemorySize, stb));               \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 1024>,                         \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 1024, KR>,                     \
                                hipFuncAttributeMaxDynamicSharedMemorySize, stb));
-            if (fp32) {
-                SG_SET_ST(float, 0) SG_SET_ST(float, 1)
+            if (M->st_k3) {
+                SG_SET_ST(float, 0, 3) SG_SET_ST(float, 1, 3)
+            } else if (fp32) {
+                SG_SET_ST(float, 0, 4) SG_SET_ST(float, 1, 4)
             } else {
-                SG_SET_ST(double, 0) SG_SET_ST(double, 1)
+                SG_SET_ST(double, 0, 4) SG_SET_ST(double, 1, 4)
             }
 #undef SG_SET_ST
         }
@@ -2678,15 +2734,23 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                          const CoarseFold<T>& fold) {
     const SweepShape sh = kSweepShapes[M->sweep_variant];
     if (ZLDS && sh.staged && M->st_ok) {
-#define SG_ST(NT)                                                                                 \
-    hipLaunchKernelGGL((k_sweep_staged<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->st_bytes,     \
+#define SG_ST(NT, KR)                                                                             \
+    hipLaunchKernelGGL((k_sweep_staged<T, MODE, NT, KR>), dim3(M->nsub), dim3(NT), M->st_bytes, \
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
                        (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p,          \
                        (const own_i2*)M->d_own2.p, x, y, zo, cz, fold, guard, (int)M->lds_tab_bytes, \
                        (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
-        if (sh.threads == 1024) SG_ST(1024);
-        else if (sh.threads == 512) SG_ST(512);
-        else SG_ST(256);
+        bool k3 = false;
+        if constexpr (std::is_same<T, float>::value) k3 = M->st_k3;
+        if (k3) {
+            if constexpr (std::is_same<T, float>::value) {
+                if (sh.threads == 1024) SG_ST(1024, 3);
+                else if (sh.threads == 512) SG_ST(512, 3);
+                else SG_ST(256, 3);
+            }
+        } else if (sh.threads == 1024) SG_ST(1024, 4);
+        else if (sh.threads == 512) SG_ST(512, 4);
+        else SG_ST(256, 4);
 #undef SG_ST
         return;
     }

@@ -223,6 +223,24 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     assert all(staged[5:]) == (spec != "rnd:60000:3"), staged
 
 
+def _stencil9(nx, ny):
+    """2D 9-point stencil (four lower and four upper neighbours per row)."""
+    rows = []
+    for j in range(ny):
+        for i in range(nx):
+            r = []
+            for dj in (-1, 0, 1):
+                for di in (-1, 0, 1):
+                    a, b = i + di, j + dj
+                    if 0 <= a < nx and 0 <= b < ny:
+                        r.append((b * nx + a, 8.0 if (di, dj) == (0, 0) else -1.0))
+            rows.append(r)
+    rp = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    ci = np.array([c for r in rows for c, _ in r], np.int32)
+    v = np.array([x for r in rows for _, x in r])
+    return O.Csr(rp, ci, v)
+
+
 @pytest.mark.parametrize("spec,kind,nparts,prec", [
     ("2d:100:100", "ilu0", 1, "fp64"),   # 199 levels per sweep in ~18 LDS chunks
     ("3d:24:24:24", "ras", 8, "fp64"),
@@ -231,12 +249,16 @@ def test_ring_sweep_bitwise(sg, ctx, spec, kind, nparts):
     # z takes 108 KiB of LDS, leaving two 24 KiB buffers of 384 FP64 records:
     # the widest levels (432 rows) are split across chunks
     ("3d:24:24:24", "ilu0", 1, "fp64"),
+    # FP32 with at most three entries per row (the stencils above): 24-byte
+    # records; a 9-point stencil has four, so it keeps the 32-byte records
+    ("9pt:90:90", "ilu0", 1, "fp32"),
+    ("9pt:90:90", "ras", 4, "fp32"),
 ])
 def test_staged_sweep_bitwise(sg, ctx, spec, kind, nparts, prec):
     """The staged sweeps (factor data copied into LDS a chunk of levels ahead;
     autotune modes 9, 10 and 11) equal the barrier sweep bit for bit, and the
     oracle bit for bit in FP64."""
-    A = O.grid(spec)
+    A = _stencil9(*(int(t) for t in spec.split(":")[1:])) if spec.startswith("9pt") else O.grid(spec)
     rng = np.random.default_rng(14)
     v = A.v * rng.uniform(0.9, 1.1, A.nnz)
     Ao = O.Csr(A.rp, A.ci, v)
