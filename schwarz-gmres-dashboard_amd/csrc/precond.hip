@@ -166,6 +166,7 @@ struct sg_precond {
     // staged ring sweep (vector too large for LDS): ring of R values + two
     // chunk buffers; chunks per subdomain: forward first/count, backward first/count
     bool sr_ok = false;
+    bool sr_cr = false;  // compact two-word records (sring_k entries per row at most)
     int sr_R = 0, sr_buf = 0;
     size_t sr_ring_off = 0, sr_buf_off = 0, sr_bytes = 0;
     sg::DevBuf<sg::SChunkR> d_rchunks;
@@ -1381,25 +1382,34 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
 // backward sweeps are two pipelines (the backward own values exist only once
 // the forward sweep has finished).  Operation order per row equals the ring
 // sweep's, so the results are the same bit for bit.
-template <typename T>
+// Compact records (CR), when every level has at most sring_k entries per row
+// and the ring slots fit 16 bits: two 16-byte words per row.  FP64 (two
+// entries): [dst, c0 | c1<<16, diag lo, diag hi], [v0, v1]; FP32 (three):
+// [dst, c0 | c1<<16, c2, diag], [v0, v1, v2, 0].  Dropped padding terms
+// subtract +0, so the values do not change.
+template <typename T, bool CR>
+__host__ __device__ constexpr int sring_k() {
+    return CR ? (sizeof(T) == 8 ? 2 : 3) : kKreg;
+}
+template <typename T, bool CR = false>
 __host__ __device__ constexpr int sring_rec_quads() {
-    return sizeof(T) == 4 ? 3 : 4;
+    return CR ? 2 : sizeof(T) == 4 ? 3 : 4;
 }
 template <typename T>
 __host__ __device__ inline int sring_own_quads(int nrow) {
     return (int)((nrow * sizeof(T) + 15) / 16);
 }
-template <typename T>
+template <typename T, bool CR>
 __host__ __device__ inline int sring_quads(const SChunkR& c) {
-    return c.nrow * sring_rec_quads<T>() + sring_own_quads<T>(c.nrow);
+    return c.nrow * sring_rec_quads<T, CR>() + sring_own_quads<T>(c.nrow);
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool CR>
 __device__ __forceinline__ void sring_load(StageRegs<T, NT>& R, const SChunkR& c,
                                            const stage_q* __restrict__ img,
                                            const stage_q* __restrict__ own) {
     typedef const __attribute__((address_space(1))) stage_q gq;
-    const int nrec = c.nrow * sring_rec_quads<T>();
+    const int nrec = c.nrow * sring_rec_quads<T, CR>();
     const int n = nrec + sring_own_quads<T>(c.nrow);
     const size_t bi = (size_t)(img + c.img), bo = (size_t)(own + c.own) - 16 * (size_t)nrec;
 #pragma unroll
@@ -1409,10 +1419,10 @@ __device__ __forceinline__ void sring_load(StageRegs<T, NT>& R, const SChunkR& c
     }
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool CR>
 __device__ __forceinline__ void sring_store(const StageRegs<T, NT>& R, const SChunkR& c,
                                             lds_t<stage_q>* buf, lds_t<stage_q>* trash) {
-    const int n = sring_quads<T>(c);
+    const int n = sring_quads<T, CR>(c);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
         const int u = (int)threadIdx.x + i * NT;
@@ -1420,45 +1430,55 @@ __device__ __forceinline__ void sring_store(const StageRegs<T, NT>& R, const SCh
     }
 }
 
-template <typename T>
+template <typename T, bool CR>
 struct SRingRow {
-    stage_q q[sring_rec_quads<T>()];
+    stage_q q[sring_rec_quads<T, CR>()];
 };
 
-template <typename T>
-__device__ __forceinline__ T sring_val(const SRingRow<T>& r, int k) {
-    if constexpr (sizeof(T) == 4) {
+template <typename T, bool CR>
+__device__ __forceinline__ T sring_val(const SRingRow<T, CR>& r, int k) {
+    if constexpr (CR && sizeof(T) == 4) {
+        return stage_bits_f(k == 0 ? r.q[1].x : k == 1 ? r.q[1].y : r.q[1].z);
+    } else if constexpr (CR) {
+        return k == 0 ? stage_bits_d(r.q[1].x, r.q[1].y) : stage_bits_d(r.q[1].z, r.q[1].w);
+    } else if constexpr (sizeof(T) == 4) {
         return stage_bits_f(k == 0 ? r.q[2].x : k == 1 ? r.q[2].y : k == 2 ? r.q[2].z : r.q[2].w);
     } else {
         const stage_q& w = r.q[2 + k / 2];
         return k % 2 == 0 ? stage_bits_d(w.x, w.y) : stage_bits_d(w.z, w.w);
     }
 }
-template <typename T>
-__device__ __forceinline__ int sring_col(const SRingRow<T>& r, int k) {
-    return (int)(k == 0 ? r.q[0].y : k == 1 ? r.q[0].z : k == 2 ? r.q[0].w : r.q[1].x);
+template <typename T, bool CR>
+__device__ __forceinline__ int sring_col(const SRingRow<T, CR>& r, int k) {
+    if constexpr (CR)
+        return (int)(k == 0 ? r.q[0].y & 0xffffu : k == 1 ? r.q[0].y >> 16 : r.q[0].z);
+    else
+        return (int)(k == 0 ? r.q[0].y : k == 1 ? r.q[0].z : k == 2 ? r.q[0].w : r.q[1].x);
 }
-template <typename T>
-__device__ __forceinline__ T sring_diag(const SRingRow<T>& r) {
-    if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[1].y);
+template <typename T, bool CR>
+__device__ __forceinline__ T sring_diag(const SRingRow<T, CR>& r) {
+    if constexpr (CR && sizeof(T) == 4) return stage_bits_f(r.q[0].w);
+    else if constexpr (CR) return stage_bits_d(r.q[0].z, r.q[0].w);
+    else if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[1].y);
     else return stage_bits_d(r.q[1].y, r.q[1].z);
 }
 
-template <typename T>
-__device__ __forceinline__ void sring_fetch(SRingRow<T>& r, const lds_t<stage_q>* buf, int p) {
+template <typename T, bool CR>
+__device__ __forceinline__ void sring_fetch(SRingRow<T, CR>& r, const lds_t<stage_q>* buf, int p) {
 #pragma unroll
-    for (int i = 0; i < sring_rec_quads<T>(); ++i) r.q[i] = buf[p * sring_rec_quads<T>() + i];
+    for (int i = 0; i < sring_rec_quads<T, CR>(); ++i)
+        r.q[i] = buf[p * sring_rec_quads<T, CR>() + i];
 }
 
 // One chunk's levels: row t of a level computes own - sum v_k ring[c_k]
 // (/ diag backward), writes ring[(position - pos0) mod R] and zout[dst].
-template <typename T, bool BWD, int NT>
+template <typename T, bool BWD, int NT, bool CR>
 __device__ __forceinline__ void sring_levels(lds_t<T>* ring, int R, const lds_t<int>* tab,
                                              const SChunkR& c, const lds_t<stage_q>* buf,
                                              int pos0, T* __restrict__ zout) {
     const int t = threadIdx.x;
     const int wbase = __builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63;
-    const lds_t<T>* own = (const lds_t<T>*)(buf + c.nrow * sring_rec_quads<T>());
+    const lds_t<T>* own = (const lds_t<T>*)(buf + c.nrow * sring_rec_quads<T, CR>());
     const int lend = c.lev0 + c.nlev;
     // a chunk may start or end inside a level (rows of one level are
     // independent): each level runs its rows inside [row0, row0 + nrow)
@@ -1474,42 +1494,42 @@ __device__ __forceinline__ void sring_levels(lds_t<T>* ring, int R, const lds_t<
     const int l1 = min(c.lev0 + 1, lend - 1);
     D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
     D dn = uni(tab[4 * l1], tab[4 * l1 + 1]);
-    SRingRow<T> cur;
+    SRingRow<T, CR> cur;
     T cown = (T)0;
     if (wbase < d.size) {
-        sring_fetch<T>(cur, buf, d.p + min(t, d.size - 1));
+        sring_fetch<T, CR>(cur, buf, d.p + min(t, d.size - 1));
         cown = own[d.p + min(t, d.size - 1)];
     }
     for (int l = c.lev0; l < lend; ++l) {
-        T zk[kKreg];
+        T zk[sring_k<T, CR>()];
         if (wbase < d.size) {
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) zk[k] = ring[sring_col<T>(cur, k)];
+            for (int k = 0; k < sring_k<T, CR>(); ++k) zk[k] = ring[sring_col<T, CR>(cur, k)];
         }
-        SRingRow<T> nxt;
+        SRingRow<T, CR> nxt;
         T nown = (T)0;
         if (wbase < dn.size) {
-            sring_fetch<T>(nxt, buf, dn.p + min(t, dn.size - 1));
+            sring_fetch<T, CR>(nxt, buf, dn.p + min(t, dn.size - 1));
             nown = own[dn.p + min(t, dn.size - 1)];
         }
         const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
         if (wbase < d.size) {
             T acc = cown;
 #pragma unroll
-            for (int k = 0; k < kKreg; ++k) acc -= sring_val<T>(cur, k) * zk[k];
-            if (BWD) acc = acc / sring_diag<T>(cur);
+            for (int k = 0; k < sring_k<T, CR>(); ++k) acc -= sring_val<T, CR>(cur, k) * zk[k];
+            if (BWD) acc = acc / sring_diag<T, CR>(cur);
             int sl = d.slot0 + t;
             if (sl >= R) sl -= R;
             ring[t < d.size ? sl : R + 1] = acc;
             if (t < d.size) zout[cur.q[0].x] = acc;
             if (d.size > NT) {
                 for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
-                    SRingRow<T> r;
-                    sring_fetch<T>(r, buf, d.p + q);
+                    SRingRow<T, CR> r;
+                    sring_fetch<T, CR>(r, buf, d.p + q);
                     T a = own[d.p + q];
 #pragma unroll
-                    for (int k = 0; k < kKreg; ++k) a -= sring_val<T>(r, k) * ring[sring_col<T>(r, k)];
-                    if (BWD) a = a / sring_diag<T>(r);
+                    for (int k = 0; k < sring_k<T, CR>(); ++k) a -= sring_val<T, CR>(r, k) * ring[sring_col<T, CR>(r, k)];
+                    if (BWD) a = a / sring_diag<T, CR>(r);
                     int s2 = d.slot0 + q;
                     while (s2 >= R) s2 -= R;
                     ring[s2] = a;
@@ -1528,7 +1548,7 @@ __device__ __forceinline__ void sring_levels(lds_t<T>* ring, int R, const lds_t<
 // LDS: level tables (forward | backward, four ints per level) | chunk
 // descriptors | ring (R values, zero slot, trash slot) | buffer 0 | buffer 1 |
 // trash word.
-template <typename T, int MODE, int NT>
+template <typename T, int MODE, int NT, bool CR>
 __global__ void __launch_bounds__(NT)
 k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                     const int* __restrict__ chunk0, const SChunkR* __restrict__ chunks,
@@ -1606,30 +1626,30 @@ k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ or
             return c;
         };
         auto run = [&](const SChunkR& c, const lds_t<stage_q>* buf) {
-            if (bwd) sring_levels<T, true, NT>(ring, R, tab, c, buf, pos0, out);
-            else sring_levels<T, false, NT>(ring, R, tab, c, buf, pos0, out);
+            if (bwd) sring_levels<T, true, NT, CR>(ring, R, tab, c, buf, pos0, out);
+            else sring_levels<T, false, NT, CR>(ring, R, tab, c, buf, pos0, out);
         };
         // descriptors of the chunks held in RA and RB, and in buffers q0 and q1
         SChunkR dA = chunk_at(0), dB = chunk_at(1);
-        sring_load<T, NT>(RA, dA, img, own);
-        sring_load<T, NT>(RB, dB, img, own);
+        sring_load<T, NT, CR>(RA, dA, img, own);
+        sring_load<T, NT, CR>(RB, dB, img, own);
         SChunkR c0q = dA, c1q;
-        sring_store<T, NT>(RA, dA, q0, trash);
+        sring_store<T, NT, CR>(RA, dA, q0, trash);
         dA = chunk_at(2);
-        sring_load<T, NT>(RA, dA, img, own);
+        sring_load<T, NT, CR>(RA, dA, img, own);
         lds_barrier();
         int ci = 0;
         for (; ci + 1 < nch; ci += 2) {
             const SChunkR nb = chunk_at(ci + 3);
-            sring_store<T, NT>(RB, dB, q1, trash);
+            sring_store<T, NT, CR>(RB, dB, q1, trash);
             c1q = dB;
-            sring_load<T, NT>(RB, nb, img, own);
+            sring_load<T, NT, CR>(RB, nb, img, own);
             dB = nb;
             run(c0q, q0);
             const SChunkR na = chunk_at(ci + 4);
-            sring_store<T, NT>(RA, dA, q0, trash);
+            sring_store<T, NT, CR>(RA, dA, q0, trash);
             c0q = dA;
-            sring_load<T, NT>(RA, na, img, own);
+            sring_load<T, NT, CR>(RA, na, img, own);
             dA = na;
             run(c1q, q1);
         }
@@ -1650,7 +1670,7 @@ k_sweep_staged_ring(const SubInfo* __restrict__ subs, const int* __restrict__ or
 // Setup: packs the rows of every staged-ring chunk into records (one workgroup
 // per chunk).  dst: forward, the own-value index of the row's backward
 // position; backward, the local row.
-template <typename T>
+template <typename T, bool CR>
 __global__ void k_stage_image_ring(const SChunkR* __restrict__ chunks,
                                    const SubInfo* __restrict__ subs,
                                    const int* __restrict__ chunk_sub,
@@ -1682,8 +1702,19 @@ __global__ void k_stage_image_ring(const SChunkR* __restrict__ chunks,
             __builtin_memcpy(db, &dg, sizeof(T));
             unsigned vb[2 * kKreg];
             __builtin_memcpy(vb, vv, sizeof(vv));
-            uint4* r = img + c.img + (size_t)(p - c.row0) * sring_rec_quads<T>();
-            r[0] = make_uint4((unsigned)(c.bwd ? bdst[p] : fdst[p]), cc[0], cc[1], cc[2]);
+            uint4* r = img + c.img + (size_t)(p - c.row0) * sring_rec_quads<T, CR>();
+            const unsigned dst = (unsigned)(c.bwd ? bdst[p] : fdst[p]);
+            if constexpr (CR) {
+                if (sizeof(T) == 4) {
+                    r[0] = make_uint4(dst, cc[0] | (cc[1] << 16), cc[2], db[0]);
+                    r[1] = make_uint4(vb[0], vb[1], vb[2], 0u);
+                } else {
+                    r[0] = make_uint4(dst, cc[0] | (cc[1] << 16), db[0], db[1]);
+                    r[1] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
+                }
+                continue;
+            }
+            r[0] = make_uint4(dst, cc[0], cc[1], cc[2]);
             r[1] = make_uint4(cc[3], db[0], sizeof(T) == 8 ? db[1] : 0u, 0u);
             if (sizeof(T) == 4) {
                 r[2] = make_uint4(vb[0], vb[1], vb[2], vb[3]);
@@ -2347,7 +2378,10 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
         const int cap = kStageBytes;
         if (fits) {
-            const int rq = fp32 ? sring_rec_quads<float>() : sring_rec_quads<double>();
+            bool cr = true;
+            for (const Level& lv : pl.flev) cr = cr && lv.K <= (fp32 ? 3 : 2);
+            for (const Level& lv : pl.blev) cr = cr && lv.K <= (fp32 ? 3 : 2);
+            const int rq = cr ? 2 : fp32 ? sring_rec_quads<float>() : sring_rec_quads<double>();
             auto ownq = [&](int nrow) {
                 return fp32 ? sring_own_quads<float>(nrow) : sring_own_quads<double>(nrow);
             };
@@ -2403,7 +2437,8 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             const size_t ct_bytes = al(max_nch * sizeof(SChunkR));
             const size_t ring_off = M->lds_tab_bytes + ct_bytes;
             const long long room = (long long)ctx->max_lds - (long long)ring_off - 2LL * cap - 32;
-            const int R = (int)std::min<long long>(room / (long long)tsz - 2, INT32_MAX);
+            // compact records keep ring slots (and the zero slot R) in 16 bits
+            const int R = (int)std::min<long long>(room / (long long)tsz - 2, cr ? 65534 : INT32_MAX);
             std::vector<std::vector<int>> fcs(nsub), bcs(nsub), bps(nsub);
             std::vector<int> Fs(nsub, 0);
             ok = ok && R >= 64 && max_w <= R;
@@ -2429,6 +2464,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                     bc.insert(bc.end(), bcs[s].begin(), bcs[s].end());
                 }
                 M->sr_ok = true;
+                M->sr_cr = cr;
                 M->sr_R = R;
                 M->sr_ring_off = ring_off;
                 M->sr_buf_off = al(ring_off + (size_t)(R + 2) * tsz);
@@ -2446,16 +2482,20 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                 d_bdst.upload(bdst, st);
                 d_fc.upload(fc, st);
                 d_bc.upload(bc, st);
-                if (fp32)
-                    hipLaunchKernelGGL(k_stage_image_ring<float>, dim3((int)ch.size()), dim3(256),
-                                       0, st, M->d_rchunks.p, M->d_subs.p, d_csub.p, M->flev.p,
-                                       M->blev.p, d_fdst.p, d_bdst.p, d_fc.p, d_bc.p, M->f_v32.p,
-                                       M->b_v32.p, M->b_d32.p, R, M->d_rimg.p);
-                else
-                    hipLaunchKernelGGL(k_stage_image_ring<double>, dim3((int)ch.size()), dim3(256),
-                                       0, st, M->d_rchunks.p, M->d_subs.p, d_csub.p, M->flev.p,
-                                       M->blev.p, d_fdst.p, d_bdst.p, d_fc.p, d_bc.p, M->f_v64.p,
-                                       M->b_v64.p, M->b_d64.p, R, M->d_rimg.p);
+#define SG_IMG_RING(T, CR, V, B, D)                                                               \
+    hipLaunchKernelGGL((k_stage_image_ring<T, CR>), dim3((int)ch.size()), dim3(256), 0, st,      \
+                       M->d_rchunks.p, M->d_subs.p, d_csub.p, M->flev.p, M->blev.p, d_fdst.p,    \
+                       d_bdst.p, d_fc.p, d_bc.p, V, B, D, R, M->d_rimg.p)
+                if (fp32 && cr) {
+                    SG_IMG_RING(float, true, M->f_v32.p, M->b_v32.p, M->b_d32.p);
+                } else if (fp32) {
+                    SG_IMG_RING(float, false, M->f_v32.p, M->b_v32.p, M->b_d32.p);
+                } else if (cr) {
+                    SG_IMG_RING(double, true, M->f_v64.p, M->b_v64.p, M->b_d64.p);
+                } else {
+                    SG_IMG_RING(double, false, M->f_v64.p, M->b_v64.p, M->b_d64.p);
+                }
+#undef SG_IMG_RING
                 SG_HIP(hipGetLastError());
                 SG_HIP(hipStreamSynchronize(st));  // the temporaries end here
             }
@@ -2505,17 +2545,19 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
         if (M->sr_ok) {
             const int srb = (int)M->sr_bytes;
-#define SG_SET_SR(T, MD)                                                                          \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 256>,                     \
+#define SG_SET_SR(T, MD, CR)                                                                      \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 256, CR>,                 \
                                hipFuncAttributeMaxDynamicSharedMemorySize, srb));               \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 512>,                     \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 512, CR>,                 \
                                hipFuncAttributeMaxDynamicSharedMemorySize, srb));               \
-    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 1024>,                    \
+    SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged_ring<T, MD, 1024, CR>,                \
                                hipFuncAttributeMaxDynamicSharedMemorySize, srb));
             if (fp32) {
-                SG_SET_SR(float, 0) SG_SET_SR(float, 1)
+                if (M->sr_cr) { SG_SET_SR(float, 0, true) SG_SET_SR(float, 1, true) }
+                else { SG_SET_SR(float, 0, false) SG_SET_SR(float, 1, false) }
             } else {
-                SG_SET_SR(double, 0) SG_SET_SR(double, 1)
+                if (M->sr_cr) { SG_SET_SR(double, 0, true) SG_SET_SR(double, 1, true) }
+                else { SG_SET_SR(double, 0, false) SG_SET_SR(double, 1, false) }
             }
 #undef SG_SET_SR
         }
@@ -2767,17 +2809,21 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
         return;
     }
     if (!ZLDS && sh.staged && M->sr_ok) {
-#define SG_SR(NT)                                                                                 \
-    hipLaunchKernelGGL((k_sweep_staged_ring<T, MODE, NT>), dim3(M->nsub), dim3(NT), M->sr_bytes, \
+#define SG_SR(NT, CR)                                                                             \
+    hipLaunchKernelGGL((k_sweep_staged_ring<T, MODE, NT, CR>), dim3(M->nsub), dim3(NT), M->sr_bytes, \
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_rchunk0.p,               \
                        M->d_rchunks.p, (const stage_q*)M->d_rimg.p, (stage_q*)M->d_rown.p,      \
                        M->d_fslot.p, M->d_fgid.p, M->flev.p, M->blev.p, M->d_own.p,             \
                        M->d_own_gid.p, x, y, zg, M->zstride, zo, cz, guard,                    \
                        (int)M->lds_tab_bytes, (int)M->sr_ring_off, (int)M->sr_buf_off,          \
                        M->sr_buf / 16, M->sr_R)
-        if (sh.threads == 1024) SG_SR(1024);
-        else if (sh.threads == 512) SG_SR(512);
-        else SG_SR(256);
+        if (M->sr_cr) {
+            if (sh.threads == 1024) SG_SR(1024, true);
+            else if (sh.threads == 512) SG_SR(512, true);
+            else SG_SR(256, true);
+        } else if (sh.threads == 1024) SG_SR(1024, false);
+        else if (sh.threads == 512) SG_SR(512, false);
+        else SG_SR(256, false);
 #undef SG_SR
         return;
     }
