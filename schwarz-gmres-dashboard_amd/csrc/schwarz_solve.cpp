@@ -2,7 +2,8 @@
 // path (src/main.cpp:63-275): same flags (--matrix / --poisson, --precond,
 // --restart, --tol, --maxiter, --fgmres, --bicgstab, --nparts, --overlap,
 // --gpu, --output, --analyze), stdout lines and JSONL records, every solve and
-// analysis on the GPU.  Extensions: --poisson3d N, --jump3d N (c3 workload),
+// analysis on the GPU.  Extensions: --poisson3d N, --jump3d N (3D diffusion, N^3,
+// coefficient jumps between 16^3 blocks),
 // --rhs ones|uniform|normal with --seed, --rtol, --precision fp64|fp32,
 // --ortho mgs|cgs2, --partition FILE, --ufactor U, --weighted-partition,
 // --coarse-inverse, --batch K (K systems with right-hand-side seeds seed..,
