@@ -158,7 +158,7 @@ struct sg_precond {
     // staged sweep: chunks of every subdomain (forward ones first); LDS =
     // level tables | z | chunk descriptors | two buffers of st_buf bytes
     bool st_ok = false;
-    bool st_k3 = false;  // FP32 records of three entries per row (every level has K <= 3)
+    int st_kr = 4;  // record format: 4, or (FP32) 3 / 2 when every level has K <= 3 / 2
     int st_buf = 0;
     size_t st_ct_off = 0, st_buf_off = 0, st_bytes = 0;
     sg::DevBuf<sg::SChunk> d_chunks;
@@ -980,9 +980,14 @@ __host__ __device__ constexpr int stage_rec_quads() {
 // bytes), then a value block of 16 bytes per row [v0, v1, v2, diag]: 24 instead
 // of 32 bytes per row, and one z read and one product fewer (the dropped
 // padding term subtracts +0, so the values do not change).
+// KR = 2 (FP32, at most two entries per row): forward rows are one 16-byte
+// word [row | c0<<16, c1, v0, v1]; backward chunks use the KR = 3 blocks with
+// values [v0, v1, 0, diag].
 template <typename T, int KR>
-__host__ __device__ inline int stage_quads_n(int nrow) {
-    return KR == 3 ? (nrow + 1) / 2 + nrow : nrow * stage_rec_quads<T>();
+__host__ __device__ inline int stage_quads_n(int nrow, bool bwd) {
+    return KR == 3 || (KR == 2 && bwd) ? (nrow + 1) / 2 + nrow
+         : KR == 2                     ? nrow
+                                       : nrow * stage_rec_quads<T>();
 }
 
 template <typename T, int NT>
@@ -999,7 +1004,7 @@ __device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
                                            const stage_q* __restrict__ img) {
     const __attribute__((address_space(1))) stage_q* src =
         (const __attribute__((address_space(1))) stage_q*)(img + c.img);
-    const int n = stage_quads_n<T, KR>(c.nrow);
+    const int n = stage_quads_n<T, KR>(c.nrow, c.bwd);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i)
         R.v[i] = src[min((int)threadIdx.x + i * NT, n - 1)];
@@ -1010,7 +1015,7 @@ __device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
 template <typename T, int NT, int KR>
 __device__ __forceinline__ void stage_store(const StageRegs<T, NT>& R, const SChunk& c,
                                             lds_t<stage_q>* buf, lds_t<stage_q>* trash) {
-    const int n = stage_quads_n<T, KR>(c.nrow);
+    const int n = stage_quads_n<T, KR>(c.nrow, c.bwd);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
         const int u = (int)threadIdx.x + i * NT;
@@ -1051,14 +1056,20 @@ __global__ void k_stage_image(const SChunk* __restrict__ chunks, const SubInfo* 
             const T dg = c.bwd ? b_diag[L.start + q] : (T)0;
             unsigned dbits[2] = {0u, 0u};
             __builtin_memcpy(dbits, &dg, sizeof(T));
-            if constexpr (KR == 3) {
+            if constexpr (KR <= 3) {
                 const int pr = L.start - c.row0 + q;  // row within the chunk
-                unsigned* ib = (unsigned*)(img + c.img);
-                ib[2 * pr] = (unsigned)rows[L.start + q] | (cc[0] << 16);
-                ib[2 * pr + 1] = cc[1] | (cc[2] << 16);
                 unsigned vb[kKreg];
                 __builtin_memcpy(vb, vv, sizeof(vv));
-                img[c.img + (c.nrow + 1) / 2 + pr] = make_uint4(vb[0], vb[1], vb[2], dbits[0]);
+                const unsigned r0 = (unsigned)rows[L.start + q] | (cc[0] << 16);
+                if (KR == 2 && !c.bwd) {
+                    img[c.img + pr] = make_uint4(r0, cc[1], vb[0], vb[1]);
+                    continue;
+                }
+                unsigned* ib = (unsigned*)(img + c.img);
+                ib[2 * pr] = r0;
+                ib[2 * pr + 1] = KR == 2 ? cc[1] : cc[1] | (cc[2] << 16);
+                img[c.img + (c.nrow + 1) / 2 + pr] =
+                    make_uint4(vb[0], vb[1], KR == 2 ? 0u : vb[2], dbits[0]);
                 continue;
             }
             uint4* r = img + c.img + (size_t)(L.start - c.row0 + q) * stage_rec_quads<T>();
@@ -1087,12 +1098,21 @@ struct StageRow<float, 3> {
     stage_d ix;  // row | c0<<16, c1 | c2<<16
     stage_q v;   // v0, v1, v2, diag
 };
+template <>
+struct StageRow<float, 2> {
+    stage_d ix;  // row | c0<<16, c1
+    stage_q v;   // v0, v1, -, diag
+};
 
-// vb: first 16-byte word of the value block (KR = 3)
-template <typename T, int KR>
+// vb: first 16-byte word of the value block (KR = 3, backward KR = 2)
+template <typename T, int KR, bool BWD>
 __device__ __forceinline__ void stage_fetch(StageRow<T, KR>& r, const lds_t<stage_q>* buf, int p,
                                             int vb) {
-    if constexpr (KR == 3) {
+    if constexpr (KR == 2 && !BWD) {
+        const stage_q w = buf[p];
+        r.ix = stage_d{w.x, w.y};
+        r.v = stage_q{w.z, w.w, 0u, 0u};
+    } else if constexpr (KR <= 3) {
         r.ix = ((const lds_t<stage_d>*)buf)[p];
         r.v = buf[vb + p];
     } else {
@@ -1108,12 +1128,12 @@ __device__ __forceinline__ double stage_bits_d(unsigned lo, unsigned hi) {
 
 template <typename T, int KR>
 __device__ __forceinline__ int stage_row(const StageRow<T, KR>& r) {
-    if constexpr (KR == 3) return r.ix.x & 0xffffu;
+    if constexpr (KR <= 3) return r.ix.x & 0xffffu;
     else return r.q[0].x & 0xffffu;
 }
 template <typename T, int KR>
 __device__ __forceinline__ int stage_col(const StageRow<T, KR>& r, int k) {
-    if constexpr (KR == 3)
+    if constexpr (KR <= 3)
         return k == 0 ? (int)(r.ix.x >> 16) : k == 1 ? (int)(r.ix.y & 0xffffu) : (int)(r.ix.y >> 16);
     else
         return k == 0 ? (int)(r.q[0].x >> 16)
@@ -1123,7 +1143,7 @@ __device__ __forceinline__ int stage_col(const StageRow<T, KR>& r, int k) {
 }
 template <typename T, int KR>
 __device__ __forceinline__ T stage_val(const StageRow<T, KR>& r, int k) {
-    if constexpr (KR == 3) {
+    if constexpr (KR <= 3) {
         return stage_bits_f(k == 0 ? r.v.x : k == 1 ? r.v.y : r.v.z);
     } else if constexpr (sizeof(T) == 4) {
         return stage_bits_f(k == 0 ? r.q[1].x : k == 1 ? r.q[1].y : k == 2 ? r.q[1].z : r.q[1].w);
@@ -1134,7 +1154,7 @@ __device__ __forceinline__ T stage_val(const StageRow<T, KR>& r, int k) {
 }
 template <typename T, int KR>
 __device__ __forceinline__ T stage_diag(const StageRow<T, KR>& r) {
-    if constexpr (KR == 3) return stage_bits_f(r.v.w);
+    if constexpr (KR <= 3) return stage_bits_f(r.v.w);
     else if constexpr (sizeof(T) == 4) return stage_bits_f(r.q[0].w);
     else return stage_bits_d(r.q[3].x, r.q[3].y);
 }
@@ -1172,7 +1192,7 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
     D d = uni(tab[4 * c.lev0], tab[4 * c.lev0 + 1]);
     D dn = uni(tab[4 * l1], tab[4 * l1 + 1]);
     StageRow<T, KR> cur;
-    if (wbase < d.size) stage_fetch<T, KR>(cur, buf, d.p + min(t, d.size - 1), vb);
+    if (wbase < d.size) stage_fetch<T, KR, BWD>(cur, buf, d.p + min(t, d.size - 1), vb);
     for (int l = c.lev0; l < lend; ++l) {
         // this level's z values first: they are the critical path
         T zk[KR], acc = (T)0;
@@ -1183,7 +1203,7 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
         }
         // then the next level's records and the descriptor after it
         StageRow<T, KR> nxt;
-        if (wbase < dn.size) stage_fetch<T, KR>(nxt, buf, dn.p + min(t, dn.size - 1), vb);
+        if (wbase < dn.size) stage_fetch<T, KR, BWD>(nxt, buf, dn.p + min(t, dn.size - 1), vb);
         const stage_i4 dq = ((const lds_t<stage_i4>*)tab)[min(l + 2, lend - 1)];
         if (wbase < d.size) {
 #pragma unroll
@@ -1193,7 +1213,7 @@ __device__ __forceinline__ void stage_levels(lds_t<T>* z, const lds_t<int>* tab,
             if (d.size > NT) {
                 for (int q = t + NT; q < d.size; q += NT) {  // levels wider than the workgroup
                     StageRow<T, KR> r;
-                    stage_fetch<T, KR>(r, buf, d.p + q, vb);
+                    stage_fetch<T, KR, BWD>(r, buf, d.p + q, vb);
                     T a = z[stage_row<T, KR>(r)];
 #pragma unroll
                     for (int k = 0; k < KR; ++k) a -= stage_val<T, KR>(r, k) * z[stage_col<T, KR>(r, k)];
@@ -2281,11 +2301,15 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         for (const Level& lv : pl.flev) fits = fits && lv.K <= kKreg;
         for (const Level& lv : pl.blev) fits = fits && lv.K <= kKreg;
         const int rq = fp32 ? stage_rec_quads<float>() : stage_rec_quads<double>();
-        bool k3 = fp32;
-        for (const Level& lv : pl.flev) k3 = k3 && lv.K <= 3;
-        for (const Level& lv : pl.blev) k3 = k3 && lv.K <= 3;
-        auto chunk_quads = [&](int nrow) {
-            return k3 ? (long long)stage_quads_n<float, 3>(nrow) : (long long)nrow * rq;
+        int kmax = 0;
+        for (const Level& lv : pl.flev) kmax = std::max(kmax, lv.K);
+        for (const Level& lv : pl.blev) kmax = std::max(kmax, lv.K);
+        const int kr = !fp32 ? 4 : kmax <= 2 ? 2 : kmax <= 3 ? 3 : 4;
+        const bool k3 = kr == 3;
+        auto chunk_quads = [&](int nrow, int dir) {
+            return kr == 2   ? (long long)stage_quads_n<float, 2>(nrow, dir != 0)
+                 : kr == 3 ? (long long)stage_quads_n<float, 3>(nrow, dir != 0)
+                           : (long long)nrow * rq;
         };
         std::vector<SChunk> ch;
         std::vector<int> c0(2 * (size_t)nsub), csub;
@@ -2313,7 +2337,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                     // chunks of whole levels while they fit; a level larger than
                     // a buffer is cut into pieces of rows (its rows are independent)
                     int maxr = capq / rq;
-                    while (chunk_quads(maxr + 1) <= capq) ++maxr;
+                    while (chunk_quads(maxr + 1, dir) <= capq) ++maxr;
                     const int pos_end = nl ? lv[nl - 1].start + lv[nl - 1].size : 0;
                     int l = 0, pos = nl ? lv[0].start : 0;
                     while (pos < pos_end) {
@@ -2326,7 +2350,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                         while (la < nl && lv[la].start < end) ++la;
                         ch.push_back(SChunk{l, la - l, pos, end - pos, dir, (int)quads, {0, 0}});
                         csub.push_back(s);
-                        quads += chunk_quads(end - pos);
+                        quads += chunk_quads(end - pos, dir);
                         pos = end;
                     }
                 }
@@ -2339,7 +2363,7 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
         if (planned && quads <= INT32_MAX) {
             M->st_ok = true;
-            M->st_k3 = k3;
+            M->st_kr = kr;
             M->st_buf = cap;
             M->st_buf_off = M->st_ct_off + ct_bytes;
             M->st_bytes = M->st_buf_off + 2 * (size_t)cap + 16;  // + trash word
@@ -2348,7 +2372,12 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
             DevBuf<int> d_csub;
             d_csub.upload(csub, st);
             M->d_img.alloc((size_t)quads);
-            if (k3)
+            if (kr == 2)
+                hipLaunchKernelGGL((k_stage_image<float, 2>), dim3((int)ch.size()), dim3(256), 0,
+                                   st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
+                                   M->blev.p, M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p,
+                                   M->b_col.p, M->b_v32.p, M->b_d32.p, M->d_img.p);
+            else if (k3)
                 hipLaunchKernelGGL((k_stage_image<float, 3>), dim3((int)ch.size()), dim3(256), 0,
                                    st, M->d_chunks.p, M->d_subs.p, d_csub.p, M->flev.p,
                                    M->blev.p, M->f_row.p, M->f_col.p, M->f_v32.p, M->b_row.p,
@@ -2570,7 +2599,9 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
                                hipFuncAttributeMaxDynamicSharedMemorySize, stb));               \
     SG_HIP(hipFuncSetAttribute((const void*)k_sweep_staged<T, MD, 1024, KR>,                     \
                                hipFuncAttributeMaxDynamicSharedMemorySize, stb));
-            if (M->st_k3) {
+            if (M->st_kr == 2) {
+                SG_SET_ST(float, 0, 2) SG_SET_ST(float, 1, 2)
+            } else if (M->st_kr == 3) {
                 SG_SET_ST(float, 0, 3) SG_SET_ST(float, 1, 3)
             } else if (fp32) {
                 SG_SET_ST(float, 0, 4) SG_SET_ST(float, 1, 4)
@@ -2782,9 +2813,15 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                        (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p,          \
                        (const own_i2*)M->d_own2.p, x, y, zo, cz, fold, guard, (int)M->lds_tab_bytes, \
                        (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
-        bool k3 = false;
-        if constexpr (std::is_same<T, float>::value) k3 = M->st_k3;
-        if (k3) {
+        int kr = 4;
+        if constexpr (std::is_same<T, float>::value) kr = M->st_kr;
+        if (kr == 2) {
+            if constexpr (std::is_same<T, float>::value) {
+                if (sh.threads == 1024) SG_ST(1024, 2);
+                else if (sh.threads == 512) SG_ST(512, 2);
+                else SG_ST(256, 2);
+            }
+        } else if (kr == 3) {
             if constexpr (std::is_same<T, float>::value) {
                 if (sh.threads == 1024) SG_ST(1024, 3);
                 else if (sh.threads == 512) SG_ST(512, 3);

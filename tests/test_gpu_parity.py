@@ -253,6 +253,9 @@ def _stencil9(nx, ny):
     # records; a 9-point stencil has four, so it keeps the 32-byte records
     ("9pt:90:90", "ilu0", 1, "fp32"),
     ("9pt:90:90", "ras", 4, "fp32"),
+    # FP32 with at most two entries per row (5-point): one-word forward records
+    ("2d:100:100", "ilu0", 1, "fp32"),
+    ("2d:120:120", "ras", 4, "fp32"),
 ])
 def test_staged_sweep_bitwise(sg, ctx, spec, kind, nparts, prec):
     """The staged sweeps (factor data copied into LDS a chunk of levels ahead;
