@@ -94,7 +94,12 @@ def run_solves(sg, ctx, A, M, db, dx, k, **kw):
     return out
 
 
+KERNEL_NAMES = {"spmv": "k_spmv_*", "trsv": "k_sweep_*", "gram_dot": "k_gram_dot",
+                "gram_update": "k_gram_update"}
+
+
 def traffic_models(n, nnz, info, restart, fp32):
+    """Algorithmic bytes per launch of the SpMV and the fused RAS sweep."""
     v = 4 if fp32 else 8
     return {
         # SpMV (FP64 CSR): vals+cols, row_ptr, x, y
@@ -120,22 +125,59 @@ def measured_traffic(workload, kernel):
     return float(t["kernels"][kernel]["bytes"])
 
 
-def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, gpu_iters):
-    """Oracle FGMRES + two-level RAS (FP64, reference algorithm) on the host."""
+def kernel_rooflines(ctx, workload, n, nnz, info, fp32):
+    """Roofline entry of each hot kernel of the last profiled solve: HIP-event
+    time per launch on the solver stream against its algorithmic bytes per
+    launch (Gram passes: 8 n (j + 2) and 8 n (j + 3) for the j of each launch,
+    from the columns the library recorded)."""
+    models = traffic_models(n, nnz, info, 0, fp32)
+    out = {}
+    for f in ("spmv", "trsv", "gram_dot", "gram_update"):
+        launches, ms = ctx.kernel_stats(f)
+        if not launches:
+            continue
+        if f == "gram_dot":
+            total = 8.0 * n * (ctx.kernel_units(f) + launches)
+        elif f == "gram_update":
+            total = 8.0 * n * (ctx.kernel_units(f) + 2 * launches)
+        else:
+            total = models[f] * launches
+        avg_s = ms * 1e-3 / launches
+        bpl = total / launches
+        ach = bpl / avg_s / 1e9
+        out[f] = {"bound": "hbm", "kernel": KERNEL_NAMES[f], "achieved": round(ach, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                  "traffic": measured_traffic(workload, f), "bytes_per_launch": round(bpl),
+                  "avg_launch_us": round(avg_s * 1e6, 2), "launches": launches,
+                  "total_ms": round(ms, 3)}
+    return out
+
+
+def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, est_iters, est_label):
+    """The oracle's FP64 FGMRES + two-level RAS (the reference algorithm, OpenMP
+    over subdomains as in src/precond/ras.h:48) on the host cores: setup timed
+    once, then the median over 3 runs of the time per iteration of `iters`
+    iterations; value = setup + that rate x est_iters (an extrapolation to the
+    iteration count of the same FP64 method, est_label)."""
     A = O.Csr(rp, ci, v)
     t0 = time.perf_counter()
     pc = O.Precond("twolevel_ras", A, cfg["nparts"], cfg["overlap"], part, precision=0)
     t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    r = O.solve("fgmres", A, b, pc, cfg["restart"], iters, 0.0)
-    t_iter = (time.perf_counter() - t0) / max(1, r["iterations"])
+    rates = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = O.solve("fgmres", A, b, pc, cfg["restart"], iters, 0.0)
+        rates.append((time.perf_counter() - t0) / max(1, r["iterations"]))
+    t_iter = float(np.median(rates))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(t_setup * 0 + t_iter * gpu_iters, 3), "unit": "s",
+    return {"value": round(t_setup + t_iter * est_iters, 3), "unit": "s",
             "cores": threads, "kind": "port",
-            "sample": f"{r['iterations']} FGMRES({cfg['restart']}) iterations of the oracle's "
+            "setup_s": round(t_setup, 3), "s_per_iteration": round(t_iter, 5),
+            "sample": f"measured: setup {t_setup:.2f} s and the median of 3 runs of "
+                      f"{r['iterations']} FGMRES({cfg['restart']}) iterations of the oracle's "
                       f"FP64 two-level RAS on the same matrix and partition "
-                      f"({t_iter * 1e3:.1f} ms/iteration, setup {t_setup:.1f} s not included), "
-                      f"scaled to the {gpu_iters} iterations of the GPU solve"}
+                      f"({t_iter * 1e3:.1f} ms/iteration); extrapolated: value = setup + "
+                      f"{est_iters} iterations ({est_label}) at that rate"}
 
 
 def probe(sg, ctx, A, M, db, dx, k, info, n, nnz):
@@ -534,7 +576,7 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-iters", type=int, default=15)
     ap.add_argument("--autotune", type=int, default=1,
                     help="kernel-shape mode (sg_context_set_autotune): 1 tune, 0 defaults, "
                          "2..11 force a shape")
@@ -552,6 +594,8 @@ def main():
                     help="with --rows: only these groups (c1: BiCGSTAB / ILUT / analytics on "
                          "Poisson 256^2; c3: BiCGSTAB / analytics on the 160^3 jump case; "
                          "mm: Matrix Market reader); repeatable")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="single systems: 1 = capture the restart cycle as a hipGraph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -577,6 +621,13 @@ def main():
     if args.config == "c5":
         run_learned(args, sg, P, cfg)
         return
+    run_single(args, sg, P, cfg)
+
+
+def run_single(args, sg, P, cfg):
+    """One system (c3 by default; c2, c1): the measured solver, the legs that
+    separate its gains, the reference method on the same GPU, per-kernel
+    rooflines and the CPU oracle."""
     t0 = time.perf_counter()
     rp, ci, v, b, xs = make_problem(P, args.config)
     if b is None:
@@ -590,27 +641,34 @@ def main():
     A = sg.Csr(ctx, rp, ci, v)
     db = sg.DeviceArray.from_host(ctx, b)
     dx = sg.DeviceArray(ctx, n)
+    graph = bool(args.graph)
+
+    def true_rel(x):
+        return float(np.linalg.norm(b - spmv_np(rp, ci, v, x)) / np.linalg.norm(b))
 
     t0 = time.perf_counter()
+    t_part = 0.0
     if args.config == "c1":
         part = None
         M = sg.Precond(ctx, "ilu0").setup(A)
         method, kind = "gmres", "ilu0"
         solve_kw = dict(method="gmres", restart=cfg["restart"], max_iter=2000, rtol=1e-8,
-                        ortho="cgs2")
+                        ortho="cgs2", graph=graph)
     else:
         part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v,
                             weighted=cfg["weighted"])
+        t_part = time.perf_counter() - t0
         kind = "twolevel_ras"
         M = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
                        precision="fp32", partition=part, coarse_solve=1).setup(A)
         method = "fgmres"
         solve_kw = dict(method="fgmres", restart=cfg["restart"], max_iter=cfg["max_iter"],
-                        rtol=1e-8, ortho="cgs2")
+                        rtol=1e-8, ortho="cgs2", graph=graph)
     ctx.sync()
     t_setup = time.perf_counter() - t0
     info = M.stats()
-    print(f"# setup {t_setup:.2f} s  {info}", file=sys.stderr, flush=True)
+    print(f"# setup {t_setup:.2f} s (partition {t_part:.2f} s)  {info}", file=sys.stderr,
+          flush=True)
     if args.probe:
         probe(sg, ctx, A, M, db, dx, args.probe, info, n, nnz)
         return
@@ -620,35 +678,80 @@ def main():
     times = [r["wall_s"] for r in rs]
     r = rs[-1]
     x = dx.get()
-    rel = np.linalg.norm(b - spmv_np(rp, ci, v, x)) / np.linalg.norm(b)
+    rel = true_rel(x)
     assert r["converged"] and rel <= 1e-8 * 1.0001, (r, rel)
     print(f"# {method}+{kind}: {r['iterations']} it, {np.mean(times):.4f} s/solve, "
           f"rel {rel:.2e}", file=sys.stderr, flush=True)
 
-    # per-family kernel time (HIP events on the solver stream), one profiled solve
+    # per-kernel time (HIP events on the solver stream), one profiled solve
     ctx.profiling(True)
     ctx.reset_stats()
     dx.zero()
-    rp_ = sg.solve(A, db, dx, M, **solve_kw)
+    sg.solve(A, db, dx, M, **solve_kw)
     fams = {f: ctx.kernel_stats(f) for f in ("spmv", "trsv", "precond", "ortho", "update")}
+    roofs = kernel_rooflines(ctx, cfg["workload"], n, nnz, info, True)
     ctx.profiling(False)
-    models = traffic_models(n, nnz, info, cfg["restart"], True)
-    dom = max(("spmv", "trsv"), key=lambda f: fams[f][1])
-    launches, ms = fams[dom]
-    avg_s = ms * 1e-3 / max(1, launches)
-    achieved = models[dom] / avg_s / 1e9
+    dom = max(roofs, key=lambda f: roofs[f]["total_ms"])
     print(f"# kernel families (launches, ms): {fams}", file=sys.stderr, flush=True)
+    print(f"# kernels: " + ", ".join(f"{f} {q['avg_launch_us']} us {q['frac']}"
+                                     for f, q in roofs.items()), file=sys.stderr, flush=True)
 
     extra = {}
+    legs = {}
+    x_err = None
+    if args.config == "c2":
+        x_err = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
     if not args.no_baseline and args.config != "c1":
+        # the same solver with an FP64 preconditioner: isolates the mixed-precision factor
+        M64 = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
+                         precision="fp64", partition=part, coarse_solve=1).setup(A)
+        run_solves(sg, ctx, A, M64, db, dx, 1, **solve_kw)
+        r64 = run_solves(sg, ctx, A, M64, db, dx, args.steps, **solve_kw)
+        legs["fp64_twolevel_fgmres"] = {
+            "s": round(float(np.mean([q["wall_s"] for q in r64])), 4),
+            "iterations": r64[-1]["iterations"], "converged": r64[-1]["converged"],
+            "rel_true_residual": r64[-1]["rel_true_residual"],
+            "what": "same solver, partition and CGS2 with the FP64 preconditioner"}
+        print(f"# fp64 {method}+{kind}: {r64[-1]['iterations']} it "
+              f"{legs['fp64_twolevel_fgmres']['s']:.3f} s", file=sys.stderr, flush=True)
+        if xs is None:
+            # reference solution: FP64 two-level FGMRES to ||b - A x|| / ||b|| <= 1e-12
+            ref_kw = dict(solve_kw, rtol=1e-12, max_iter=4 * cfg["max_iter"])
+            rr = run_solves(sg, ctx, A, M64, db, dx, 1, **ref_kw)[0]
+            xref = dx.get()
+            legs["x_reference"] = {"iterations": rr["iterations"], "converged": rr["converged"],
+                                   "rel_true_residual": true_rel(xref)}
+            x_err = float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+        del M64
+        # the reference's partition semantics (unit edge weights, as its METIS call)
+        if cfg["weighted"]:
+            t0 = time.perf_counter()
+            part_u = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v, weighted=False)
+            t_pu = time.perf_counter() - t0
+            Mu = sg.Precond(ctx, kind, nparts=cfg["nparts"], overlap=cfg["overlap"],
+                            precision="fp32", partition=part_u, coarse_solve=1).setup(A)
+            ru = run_solves(sg, ctx, A, Mu, db, dx, 1, **solve_kw)[0]
+            legs["unit_weight_partition"] = {
+                "s": round(ru["wall_s"], 4), "iterations": ru["iterations"],
+                "converged": ru["converged"], "rel_true_residual": ru["rel_true_residual"],
+                "partition_s": round(t_pu, 3),
+                "what": "the measured mixed solver on a unit-edge-weight partition"}
+            print(f"# unit-weight partition: {ru['iterations']} it {ru['wall_s']:.3f} s "
+                  f"conv={ru['converged']}", file=sys.stderr, flush=True)
+            del Mu
+        else:
+            part_u = part
+        # the reference's method: FP64 one-level RAS-GMRES with MGS, on the
+        # reference's (unit-weight) partition
         Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
-                        precision="fp64", partition=part).setup(A)
+                        precision="fp64", partition=part_u).setup(A)
         base_kw = dict(method="gmres", restart=cfg["restart"], max_iter=cfg["base_max_iter"],
                        rtol=1e-8, ortho="mgs")
         rb = run_solves(sg, ctx, A, Mb, db, dx, 1, **base_kw)[0]
         extra["fp64_ras_gmres"] = {"s": round(rb["wall_s"], 4), "iterations": rb["iterations"],
                                    "converged": rb["converged"],
-                                   "rel_true_residual": rb["rel_true_residual"]}
+                                   "rel_true_residual": rb["rel_true_residual"],
+                                   "partition": "unit-weight"}
         # a baseline that hit its iteration cap gives a lower bound on the speedup
         extra["speedup_vs_fp64_ras_gmres"] = round(rb["wall_s"] / np.mean(times), 2)
         extra["speedup_is_lower_bound"] = not rb["converged"]
@@ -662,19 +765,24 @@ def main():
         bq_kw = dict(method="bicgstab", max_iter=cfg["max_iter"], tol=1e-8)
         run_solves(sg, ctx, A, Mq, db, dx, 1, **bq_kw)
         rq = run_solves(sg, ctx, A, Mq, db, dx, args.steps, **bq_kw)
-        extra["bicgstab_twolevel_fp64"] = {
+        legs["bicgstab_twolevel_fp64"] = {
             "s": round(float(np.mean([q["wall_s"] for q in rq])), 4),
             "iterations": rq[-1]["iterations"], "converged": rq[-1]["converged"],
             "rel_true_residual": rq[-1]["rel_true_residual"]}
         print(f"# bicgstab+{kind} fp64: {rq[-1]['iterations']} it "
-              f"{extra['bicgstab_twolevel_fp64']['s']:.3f} s", file=sys.stderr, flush=True)
+              f"{legs['bicgstab_twolevel_fp64']['s']:.3f} s", file=sys.stderr, flush=True)
         del Mq
 
     cpu = None
     if not args.no_cpu and args.config != "c1":
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_bind as O
-        cpu = cpu_baseline(O, rp, ci, v, b, part, cfg, args.cpu_iters, r["iterations"])
+        if "fp64_twolevel_fgmres" in legs:
+            est, lab = legs["fp64_twolevel_fgmres"]["iterations"], \
+                "those of the FP64 two-level FGMRES leg on the GPU"
+        else:
+            est, lab = r["iterations"], "those of the measured GPU solve"
+        cpu = cpu_baseline(O, rp, ci, v, b, part, cfg, args.cpu_iters, est, lab)
 
     out = {
         "metric": "time to ||r||/||b||<=1e-8 (s): mixed/learned FGMRES vs FP64 RAS-GMRES, 1 GPU",
@@ -693,16 +801,22 @@ def main():
                    "solver": f"{method}({cfg['restart']})+{kind}",
                    "nparts": cfg["nparts"], "overlap": cfg["overlap"],
                    "partition": "coefficient-weighted" if cfg["weighted"] else "unit-weight",
+                   "ufactor": cfg["ufactor"],
+                   "coarse": "FP64 A0 and LU on the device at setup; applied as the FP32 "
+                             "explicit inverse (one dense mat-vec)" if kind != "ilu0" else None,
+                   "ortho": "cgs2", "graph": graph,
                    "iterations": r["iterations"], "rel_true_residual": rel,
-                   "setup_s": round(t_setup, 3), "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(cfg["workload"], dom),
-                     "bytes_per_launch": models[dom],
-                     "avg_launch_us": round(avg_s * 1e6, 2)},
+                   "x_rel_error": x_err,
+                   "x_rel_error_vs": ("x*" if xs is not None else
+                                      "FP64 two-level FGMRES solution at 1e-12"),
+                   "setup_s": round(t_setup, 3), "partition_s": round(t_part, 3),
+                   "parallelism": "single GPU"},
+        "roofline": dict(roofs[dom], kernel=KERNEL_NAMES[dom], family=dom),
+        "kernel_rooflines": roofs,
         "kernel_ms": {f: round(fams[f][1], 3) for f in fams},
         "cpu_baseline": cpu,
     }
+    out.update(legs)
     out.update(extra)
     print(json.dumps(out))
 

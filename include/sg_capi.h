@@ -265,6 +265,21 @@ sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                    const sg_solve_params* params, sg_solve_result* result, double* hist,
                    int hist_cap);
 
+/* Verification hook, no counterpart in the reference: the Krylov basis of the
+ * last restart cycle that ran Arnoldi steps in the last single-system
+ * GMRES/FGMRES sg_solve on this context (the workspace is kept by the context
+ * between solves; the record is replaced by the next such solve).  n, m
+ * (restart), k (steps of that cycle) and kv (basis vectors formed: k + 1, or k
+ * when the cycle ended at its m-th step or broke down) are always written; V
+ * (host, n x kv, column i at V + i*n), Z (host, n x k, FGMRES with a
+ * preconditioner only) and Hbar (host, (m+1) x m column-major, the Hessenberg
+ * columns as Gram-Schmidt produced them, before the Givens rotations) may be
+ * NULL.  With them, FGMRES satisfies A Z_j = V_{j+1} Hbar_j and left GMRES
+ * M^-1 A V_j = V_{j+1} Hbar_j for j < kv (the Arnoldi loops of
+ * src/core/fgmres.cpp:77-107 and src/core/gmres.cpp:84-128). */
+sg_status sg_solve_last_basis(sg_context* ctx, int* n, int* m, int* k, int* kv, double* V,
+                              double* Z, double* Hbar);
+
 /* ---- batched solve (many independent systems) -----------------------------
  * No counterpart in the reference (it solves one system per run,
  * src/main.cpp:238-243).  The systems are stored as one block-diagonal CSR:
@@ -392,7 +407,9 @@ typedef enum {
     SG_K_TRSV = 2,     /* fused gather + ILU(0) sweeps + scatter    */
     SG_K_ORTHO = 3,    /* Gram-Schmidt + norm + Givens              */
     SG_K_UPDATE = 4,   /* restart residual, basis scaling, x update */
-    SG_K_COUNT = 5
+    SG_K_GRAM_DOT = 5,    /* CGS2 pass 1 alone (V^T w, V^T v_j); also in ORTHO */
+    SG_K_GRAM_UPDATE = 6, /* CGS2 pass 2 alone (w -= V h, ||w||^2); also in ORTHO */
+    SG_K_COUNT = 7
 } sg_kernel_family;
 sg_status sg_context_set_profiling(sg_context* ctx, int enable);
 /* Kernel-shape selection.  mode 1 (default): the first SpMV on a matrix and
@@ -420,6 +437,11 @@ sg_status sg_context_set_tune_cache(sg_context* ctx, const char* path);
 sg_status sg_context_kernel_stats(sg_context* ctx, int family, long long* launches,
                                   double* total_ms);
 sg_status sg_context_reset_stats(sg_context* ctx);
+/* Work units of a family's timed launches since the last reset: for
+ * SG_K_GRAM_DOT / SG_K_GRAM_UPDATE the basis columns j + 1 of each launch
+ * (so algorithmic bytes are 8 n (units + launches) and 8 n (units + 2 launches)),
+ * one per launch for the other families.  Host only. */
+sg_status sg_context_kernel_units(sg_context* ctx, int family, double* units);
 
 #ifdef __cplusplus
 }

@@ -27,8 +27,9 @@
  *                              (oracle/_ref), scipy.linalg.lu_factor.
  *   or_gmres / or_fgmres       golden iteration counts and residual histories
  *                              from the reference gmres.cpp / fgmres.cpp
- *                              (oracle/_ref), known answers 141 (Poisson 100^2)
- *                              and 248 (150^2), identity and SPD exactness.
+ *                              (oracle/_ref), known answers 201 (Poisson 100^2)
+ *                              and 301 (150^2) of the compiled reference
+ *                              (tol 1e-10, b = A*1), identity and SPD exactness.
  *   or_metrics_json            golden strings from reference Metrics::to_json.
  *   or_ilut_new                golden apply outputs, l_nnz / u_nnz and GMRES
  *                              iteration counts from the reference ILUTPrecond

@@ -22,7 +22,8 @@ PC_KIND = {"none": 0, "jacobi": 1, "ilu0": 2, "asm": 3, "ras": 4, "twolevel_asm"
            "twolevel_ras": 6, "ilut": 8}
 STATUS = {0: "SG_OK", 1: "SG_E_ARG", 2: "SG_E_HIP", 3: "SG_E_SINGULAR", 4: "SG_E_PARTITION",
           5: "SG_E_IO", 6: "SG_E_NOCONV", 7: "SG_E_STATE"}
-K_FAMILY = {"spmv": 0, "precond": 1, "trsv": 2, "ortho": 3, "update": 4}
+K_FAMILY = {"spmv": 0, "precond": 1, "trsv": 2, "ortho": 3, "update": 4, "gram_dot": 5,
+            "gram_update": 6}
 
 
 class SgError(RuntimeError):
@@ -147,6 +148,7 @@ def lib():
         "sg_context_kernel_stats": ([_VP, C.c_int, C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double)], C.c_int),
         "sg_context_reset_stats": ([_VP], C.c_int),
+        "sg_context_kernel_units": ([_VP, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "sg_mm_read": ([C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                         C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int)),
                         C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_double))],
@@ -163,6 +165,8 @@ def lib():
         "sg_convergence_write_jsonl": ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
                                         _VP, C.c_int, C.c_int], C.c_int),
         "sg_mm_write": ([C.c_char_p, C.c_int, C.c_int, _VP, _VP, _VP, C.c_int], C.c_int),
+        "sg_solve_last_basis": ([_VP, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), _VP, _VP, _VP], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -213,6 +217,11 @@ class Context:
         ms = C.c_double()
         self.check(lib().sg_context_kernel_stats(self.h, K_FAMILY[family], C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def kernel_units(self, family):
+        u = C.c_double()
+        self.check(lib().sg_context_kernel_units(self.h, K_FAMILY[family], C.byref(u)))
+        return u.value
 
     def reset_stats(self):
         self.check(lib().sg_context_reset_stats(self.h))
@@ -677,3 +686,20 @@ def solve_batch(A, b, x, M=None, method="fgmres", restart=30, max_iter=1000, tol
     A.ctx.check(lib().sg_solve_batch(A.h, M.h if M is not None else None, b.ptr, x.ptr,
                                      C.byref(p), C.cast(rs, _VP), _p(hist), hist.shape[1]))
     return [_result(rs[q], hist[q], cap) for q in range(nsys)]
+
+
+def last_basis(ctx):
+    """The Krylov basis of the last cycle of the last single-system GMRES/FGMRES
+    solve on ctx (sg_solve_last_basis): dict with V (n x kv), Z (n x k, or None),
+    Hbar ((m+1) x m, before the Givens rotations), k and kv."""
+    n, m, k, kv = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    L = lib()
+    ctx.check(L.sg_solve_last_basis(ctx.h, C.byref(n), C.byref(m), C.byref(k), C.byref(kv),
+                                    None, None, None))
+    V = np.zeros((kv.value, n.value))
+    Z = np.zeros((k.value, n.value))
+    H = np.zeros((m.value, m.value + 1))  # column-major (m+1) x m
+    ctx.check(L.sg_solve_last_basis(ctx.h, C.byref(n), C.byref(m), C.byref(k), C.byref(kv),
+                                    _p(V), _p(Z), _p(H)))
+    return dict(V=V.T.copy(), Z=Z.T.copy(), Hbar=H.T.copy(), k=k.value, kv=kv.value,
+                m=m.value, n=n.value)

@@ -56,7 +56,7 @@ sg_status guarded(sg_context* ctx, F&& f) {
     }
 }
 
-ProfScope::ProfScope(sg_context* c, int f) : ctx(c), fam(f) {
+ProfScope::ProfScope(sg_context* c, int f, double u) : ctx(c), fam(f), units(u) {
     if (!ctx->profiling) return;
     a = ctx->get_event();
     SG_HIP(hipEventRecord(a, ctx->stream));
@@ -68,6 +68,7 @@ ProfScope::~ProfScope() {
     (void)hipEventRecord(b, ctx->stream);
     ctx->prof[fam].pending.emplace_back(a, b);
     ctx->prof[fam].launches++;
+    ctx->prof[fam].units += units;
 }
 
 // ---- deterministic reduction kernels ----------------------------------------
@@ -241,6 +242,7 @@ sg_status sg_context_destroy(sg_context* ctx) {
             (void)hipEventDestroy(ab.second);
         }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    sg::krylov_cache_free(ctx->krylov);
     if (ctx->d_partials) (void)hipFree(ctx->d_partials);
     if (ctx->d_ticket) (void)hipFree(ctx->d_ticket);
     if (ctx->d_result) (void)hipFree(ctx->d_result);
@@ -399,8 +401,15 @@ sg_status sg_context_reset_stats(sg_context* ctx) {
         for (auto& p : ctx->prof) {
             p.launches = 0;
             p.ms = 0.0;
+            p.units = 0.0;
         }
     });
+}
+
+sg_status sg_context_kernel_units(sg_context* ctx, int family, double* units) {
+    if (!ctx || !units || family < 0 || family >= SG_K_COUNT) return SG_E_ARG;
+    *units = ctx->prof[family].units;
+    return SG_OK;
 }
 
 }  // extern "C"

@@ -86,6 +86,7 @@ enum : int { kRun = 0, kVerify = 1, kDone = 2 };
 
 struct Work {
     double* H;      // (m+1) x m, column-major H[i + j*(m+1)]
+    double* Hr;     // the same columns before the Givens rotations (H-bar of the cycle)
     double* g;      // m+1
     double* cs;     // m
     double* sn;     // m
@@ -106,6 +107,7 @@ __device__ inline Work sys_work(const Work& W, int s) {
     Work o = W;
     const size_t m = W.m;
     o.H += (size_t)s * (m + 1) * m;
+    o.Hr += (size_t)s * (m + 1) * m;
     o.g += (size_t)s * (m + 1);
     o.cs += (size_t)s * m;
     o.sn += (size_t)s * m;
@@ -150,6 +152,7 @@ __device__ void step_epilogue(const Work& W, int j, double hn) {
     double* H = W.H;
     st->applies += st->has_m;
     H[(j + 1) + j * (m + 1)] = hn;
+    for (int i = 0; i <= j + 1; ++i) W.Hr[i + j * (m + 1)] = H[i + j * (m + 1)];
     if (hn < 1e-30) {  // breakdown: column j is used unrotated
         st->k = j + 1;
         st->active = 0;
@@ -225,54 +228,53 @@ __device__ void reduce_partials(const Work& W, const SysBlocks& sb, int ncols, d
 // issued before the scheduling barrier, so a wave keeps a whole register
 // block of loads in flight.
 
-// Dots of columns [cb, cb + CB) with w and v_j over the block's elements.
+// Pass 1 holds a thread's elements of w and v_j in registers (up to
+// kTilesGD tiles of kEoG elements, i.e. a block range of up to 4096 elements
+// in one register pass) and streams the other basis columns past them, so w
+// and v_j are read once per step and column j is not read a second time:
+// 8 n (j + 2) bytes.  Within a thread the products are added tile by tile, as
+// the order of a tile loop over the block range would add them.
+constexpr int kTilesGD = 4;
+constexpr int kPassGD = kTileGD * kTilesGD;
+
+// Dots of columns [cb, cb + CB) with w and v_j over the resident elements;
+// lane 0 of each wave stores (first pass) or adds its wave's sums into redw.
 template <int CB>
-__device__ __forceinline__ void gram_dot_cols(int start, int end, int cb, int ncols,
+__device__ __forceinline__ void gram_dot_cols(int cb, int ncols, int ntile, bool first,
                                               const double* __restrict__ V, size_t ld,
-                                              const double* __restrict__ w,
-                                              const double* __restrict__ vj,
-                                              double (*red)[kRedCols]) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+                                              const int (&ee)[kTilesGD][kEoG],
+                                              const double (&wr)[kTilesGD][kEoG],
+                                              const double (&vr)[kTilesGD][kEoG],
+                                              double* redw) {
     double aw[CB], av[CB];
 #pragma unroll
     for (int c = 0; c < CB; ++c) aw[c] = av[c] = 0.0;
-    for (int base = start + tid; base < end; base += kTileGD) {
-        int ee[kEoG];
-        double msk[kEoG], wr[kEoG], vr[kEoG], a[CB][kEoG];
 #pragma unroll
-        for (int r = 0; r < kEoG; ++r) {
-            const int e = base + r * kTGD;
-            ee[r] = min(e, end - 1);
-            msk[r] = e < end ? 1.0 : 0.0;
-            wr[r] = w[ee[r]];
-            vr[r] = vj[ee[r]];
-        }
+    for (int t = 0; t < kTilesGD; ++t) {
+        if (t >= ntile) break;  // uniform over the block
+        double a[CB][kEoG];
 #pragma unroll
         for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[t][r]];
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < kEoG; ++r) {
-            wr[r] *= msk[r];
-            vr[r] *= msk[r];
-        }
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
 #pragma unroll
             for (int r = 0; r < kEoG; ++r) {
-                aw[c] += a[c][r] * wr[r];
-                av[c] += a[c][r] * vr[r];
+                aw[c] += a[c][r] * wr[t][r];
+                av[c] += a[c][r] * vr[t][r];
             }
         }
     }
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
         const double sw = wave_sum(aw[c]);
         const double sv = wave_sum(av[c]);
         if (lane == 0) {
-            red[wv][cb + c] = sw;
-            red[wv][ncols + cb + c] = sv;
+            redw[cb + c] = first ? sw : redw[cb + c] + sw;
+            redw[ncols + cb + c] = first ? sv : redw[ncols + cb + c] + sv;
         }
     }
 }
@@ -283,19 +285,73 @@ k_gram_dot(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
     const BlockRange br = sp.blk[blockIdx.x];
     if (!W.st[br.sys].active) return;
     __shared__ double red[kTGD / 64][kRedCols];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double* redw = red[wv];  // only lane 0 of wave wv touches it
     const double* vj = V + (size_t)j * ld;
-    int cb = 0;
-    for (; cb + kCBG <= ncols; cb += kCBG)
-        gram_dot_cols<kCBG>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
-    if (ncols - cb >= 4) {
-        gram_dot_cols<4>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
-        cb += 4;
+    const int nv = ncols - 1;  // columns 0..j-1 are streamed; column j is v_j itself
+    for (int p0 = br.start; p0 < br.end; p0 += kPassGD) {
+        const int p1 = min(br.end, p0 + kPassGD);
+        const int ntile = (p1 - p0 + kTileGD - 1) / kTileGD;
+        const bool first = p0 == br.start;
+        int ee[kTilesGD][kEoG];
+        double wr[kTilesGD][kEoG], vr[kTilesGD][kEoG], msk[kTilesGD][kEoG];
+#pragma unroll
+        for (int t = 0; t < kTilesGD; ++t)
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) {
+                const int e = p0 + t * kTileGD + r * kTGD + tid;
+                ee[t][r] = min(e, p1 - 1);
+                msk[t][r] = e < p1 ? 1.0 : 0.0;
+            }
+#pragma unroll
+        for (int t = 0; t < kTilesGD; ++t) {
+            if (t >= ntile) break;
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) {
+                wr[t][r] = w[ee[t][r]];
+                vr[t][r] = vj[ee[t][r]];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < kTilesGD; ++t) {
+            if (t >= ntile) break;
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) {
+                wr[t][r] *= msk[t][r];
+                vr[t][r] *= msk[t][r];
+            }
+        }
+        int cb = 0;
+        for (; cb + kCBG <= nv; cb += kCBG)
+            gram_dot_cols<kCBG>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+        if (nv - cb >= 4) {
+            gram_dot_cols<4>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+            cb += 4;
+        }
+        if (nv - cb >= 2) {
+            gram_dot_cols<2>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+            cb += 2;
+        }
+        if (nv - cb >= 1) gram_dot_cols<1>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+        // column j: v_j . w and v_j . v_j from the resident values
+        double sw = 0.0, sv = 0.0;
+#pragma unroll
+        for (int t = 0; t < kTilesGD; ++t) {
+            if (t >= ntile) break;
+#pragma unroll
+            for (int r = 0; r < kEoG; ++r) {
+                sw += vr[t][r] * wr[t][r];
+                sv += vr[t][r] * vr[t][r];
+            }
+        }
+        sw = wave_sum(sw);
+        sv = wave_sum(sv);
+        if (lane == 0) {
+            redw[j] = first ? sw : redw[j] + sw;
+            redw[ncols + j] = first ? sv : redw[ncols + j] + sv;
+        }
     }
-    if (ncols - cb >= 2) {
-        gram_dot_cols<2>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
-        cb += 2;
-    }
-    if (ncols - cb >= 1) gram_dot_cols<1>(br.start, br.end, cb, ncols, V, ld, w, vj, red);
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
     store_partials<kTGD>(red, 2 * ncols, W.part, W.pstride);
 }
@@ -611,7 +667,8 @@ __global__ void __launch_bounds__(kTF) k_restart_finish(Work W0, Segs sp, int ha
     const double rn = has_r ? sqrt(out[1]) : beta;
     st->k = 0;
     if (st->rtol > 0.0) {
-        const double rel = rn / st->bnorm;
+        // b = 0: the absolute residual decides (x = 0 is exact), as at exit
+        const double rel = st->bnorm > 0.0 ? rn / st->bnorm : rn;
         st->rel_true = rel;
         if (rel <= st->rtol || !(beta > 0.0)) {
             st->converged = rel <= st->rtol;
@@ -619,7 +676,8 @@ __global__ void __launch_bounds__(kTF) k_restart_finish(Work W0, Segs sp, int ha
             return;
         }
         if (st->first) {
-            st->tol = st->method == 0 ? st->rtol * beta : st->rtol * st->bnorm;
+            st->tol = st->method == 0 ? st->rtol * beta
+                                      : st->rtol * (st->bnorm > 0.0 ? st->bnorm : 1.0);
         } else if (st->est_conv) {
             double f = 0.5 * st->rtol / rel;
             st->tol *= f < 1e-3 ? 1e-3 : (f > 0.9 ? 0.9 : f);
@@ -720,7 +778,10 @@ __global__ void k_init_state(KState* st, int nsys, int method, int m, int has_m,
     st[s] = k;
 }
 
-struct SolveWs {
+// Device and pinned buffers of sg_solve, kept by the context and grown on
+// demand, so a repeated solve allocates nothing; also the record of the last
+// single-system solve's basis for sg_solve_last_basis.
+struct KrylovCache {
     DevBuf<double> vecs;  // V, Z, r, z, w, t
     DevBuf<double> small;
     DevBuf<KState> st;
@@ -728,14 +789,75 @@ struct SolveWs {
     int* pinned = nullptr;   // lookahead flags (m)
     int* h_mode = nullptr;   // per-system control (nsys)
     KState* h_st = nullptr;  // per-system state read back (nsys)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    ~SolveWs() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+    size_t n_pinned = 0, n_hmode = 0, n_hst = 0;
+    // reduction tables, rebuilt when the system sizes change
+    std::vector<int> seg_off;
+    int seg_cu = 0;
+    SegsHost seg_red, seg_gd, seg_gu;
+    bool busy = false;
+    // last single-system solve (valid until the next solve on the context)
+    bool basis_valid = false;
+    int b_n = 0, b_m = 0, b_k = 0, b_kv = 0, b_hasz = 0;
+    size_t b_ld = 0;
+    const double* b_V = nullptr;
+    const double* b_Z = nullptr;
+    const double* b_Hr = nullptr;
+    ~KrylovCache() {
         if (pinned) (void)hipHostFree(pinned);
         if (h_mode) (void)hipHostFree(h_mode);
         if (h_st) (void)hipHostFree(h_st);
+    }
+};
+
+void krylov_cache_free(KrylovCache* c) { delete c; }
+
+template <typename T>
+static void grow(DevBuf<T>& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+}
+
+template <typename T>
+static T* grow_pinned(T*& p, size_t& cap, size_t n) {
+    if (cap < n) {
+        if (p) SG_HIP(hipHostFree(p));
+        p = nullptr;
+        cap = 0;
+        SG_HIP(hipHostMalloc((void**)&p, sizeof(T) * n, hipHostMallocDefault));
+        cap = n;
+    }
+    return p;
+}
+
+// A solve nested inside another one on the same context (e.g. from a host
+// callback preconditioner) gets a workspace of its own.
+struct CacheLease {
+    KrylovCache* c;
+    bool own;
+    explicit CacheLease(sg_context* ctx) {
+        if (!ctx->krylov) ctx->krylov = new KrylovCache();
+        if (ctx->krylov->busy) {
+            c = new KrylovCache();
+            own = true;
+        } else {
+            c = ctx->krylov;
+            own = false;
+        }
+        c->busy = true;
+        c->basis_valid = false;
+    }
+    ~CacheLease() {
+        c->busy = false;
+        if (own) delete c;
+    }
+};
+
+// Graph objects of one solve.
+struct GraphHold {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphHold() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
@@ -769,8 +891,10 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     const size_t ld = basis_ld(n);
     const int nz = (method && M) ? m : 0;
     const int nvec = (m + 1) + nz + 4;
-    SolveWs ws;
-    ws.vecs.alloc(ld * nvec);
+    CacheLease lease(ctx);
+    KrylovCache& ws = *lease.c;
+    GraphHold gh;
+    grow(ws.vecs, ld * nvec);
     double* V = ws.vecs.p;
     double* Z = nz ? V + ld * (m + 1) : nullptr;
     double* r = V + ld * ((m + 1) + nz);
@@ -780,25 +904,30 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     if (nsys > 1)  // stopped systems of a batch keep computing, on finite values
         SG_HIP(hipMemsetAsync(V, 0, sizeof(double) * ld * nvec, s));
 
-    SegsHost seg_red, seg_gd, seg_gu;
-    build_segs(off, 4 * ctx->num_cu, kTile, seg_red, s);
-    build_segs(off, 4 * ctx->num_cu, kTileGD, seg_gd, s);
-    build_segs(off, 4 * ctx->num_cu, kTileGU, seg_gu, s);
+    if (ws.seg_off != off || ws.seg_cu != ctx->num_cu) {
+        build_segs(off, 4 * ctx->num_cu, kTile, ws.seg_red, s);
+        build_segs(off, 4 * ctx->num_cu, kTileGD, ws.seg_gd, s, kPassGD);
+        build_segs(off, 4 * ctx->num_cu, kTileGU, ws.seg_gu, s);
+        ws.seg_off = off;
+        ws.seg_cu = ctx->num_cu;
+    }
+    const SegsHost &seg_red = ws.seg_red, &seg_gd = ws.seg_gd, &seg_gu = ws.seg_gu;
     const Segs sp_red = seg_red.dev(), sp_gd = seg_gd.dev(), sp_gu = seg_gu.dev();
     const int pstride = std::max(seg_red.nblk, std::max(seg_gd.nblk, seg_gu.nblk));
 
     const int hcap = prm->max_iter + m + 2;
-    const size_t per_sys = (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m + 2 * kRedCols + hcap +
-                           (size_t)kMaxCols * kMaxCols;
-    ws.small.alloc(per_sys * nsys + (size_t)kRedCols * pstride);
-    ws.st.alloc(nsys);
-    ws.mode.alloc(nsys);
-    SG_HIP(hipHostMalloc((void**)&ws.pinned, sizeof(int) * (m + 1), hipHostMallocDefault));
-    SG_HIP(hipHostMalloc((void**)&ws.h_mode, sizeof(int) * nsys, hipHostMallocDefault));
-    SG_HIP(hipHostMalloc((void**)&ws.h_st, sizeof(KState) * nsys, hipHostMallocDefault));
+    const size_t per_sys = 2 * (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m + 2 * kRedCols +
+                           hcap + (size_t)kMaxCols * kMaxCols;
+    grow(ws.small, per_sys * nsys + (size_t)kRedCols * pstride);
+    grow(ws.st, nsys);
+    grow(ws.mode, nsys);
+    grow_pinned(ws.pinned, ws.n_pinned, (size_t)m + 1);
+    grow_pinned(ws.h_mode, ws.n_hmode, (size_t)nsys);
+    grow_pinned(ws.h_st, ws.n_hst, (size_t)nsys);
     Work W;
     W.H = ws.small.p;
-    W.g = W.H + (size_t)(m + 1) * m * nsys;
+    W.Hr = W.H + (size_t)(m + 1) * m * nsys;
+    W.g = W.Hr + (size_t)(m + 1) * m * nsys;
     W.cs = W.g + (size_t)(m + 1) * nsys;
     W.sn = W.cs + (size_t)m * nsys;
     W.y = W.sn + (size_t)m * nsys;
@@ -870,12 +999,18 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         {
             ProfScope ps(ctx, SG_K_ORTHO);
             if (cgs2) {
-                hipLaunchKernelGGL(k_gram_dot, dim3(seg_gd.nblk), dim3(kTGD), 0, s, sp_gd,
-                                   j + 1, V, ld, w, W, j);
+                {
+                    ProfScope pd(ctx, SG_K_GRAM_DOT, j + 1);
+                    hipLaunchKernelGGL(k_gram_dot, dim3(seg_gd.nblk), dim3(kTGD), 0, s, sp_gd,
+                                       j + 1, V, ld, w, W, j);
+                }
                 hipLaunchKernelGGL(k_gram_sums, dim3(2 * (j + 1), nsys), dim3(64), 0, s, W, sp_gd,
                                    j + 1, j);
-                hipLaunchKernelGGL(k_gram_update, dim3(seg_gu.nblk), dim3(kTGU), 0, s, sp_gu,
-                                   j + 1, V, ld, w, W, j);
+                {
+                    ProfScope pu(ctx, SG_K_GRAM_UPDATE, j + 1);
+                    hipLaunchKernelGGL(k_gram_update, dim3(seg_gu.nblk), dim3(kTGU), 0, s, sp_gu,
+                                       j + 1, V, ld, w, W, j);
+                }
                 if (j + 1 == m)
                     hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
             } else {
@@ -921,8 +1056,8 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         for (int j = 0; j < m; ++j) launch_step(j);
         launch_tail();
         cg.on = false;
-        SG_HIP(hipStreamEndCapture(s, &ws.graph));
-        SG_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
+        SG_HIP(hipStreamEndCapture(s, &gh.graph));
+        SG_HIP(hipGraphInstantiate(&gh.exec, gh.graph, nullptr, nullptr, 0));
     }
 
     std::vector<hipEvent_t> evs(m);
@@ -939,6 +1074,7 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     std::vector<int> cycles(nsys, 0);
     KState* hst = ws.h_st;
     std::memset((void*)hst, 0, sizeof(KState) * nsys);
+    int last_k = 0;  // columns of the last cycle that ran Arnoldi steps (single system)
     for (int cycle = 0;; ++cycle) {
         // per-system decisions before the cycle (gmres.cpp:57 bounds cycles)
         bool any = false, any_run = false;
@@ -954,7 +1090,7 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         if (!any) break;
         SG_HIP(hipMemcpyAsync(ws.mode.p, hmode, sizeof(int) * nsys, hipMemcpyHostToDevice, s));
         if (use_graph) {
-            SG_HIP(hipGraphLaunch(ws.exec, s));
+            SG_HIP(hipGraphLaunch(gh.exec, s));
         } else {
             launch_restart();
             for (int j = 0; j < m && any_run; ++j) {
@@ -976,6 +1112,7 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
         }
         SG_HIP(hipMemcpyAsync(hst, st, sizeof(KState) * nsys, hipMemcpyDeviceToHost, s));
         SG_HIP(hipStreamSynchronize(s));
+        if (nsys == 1 && hmode[0] == kRun && hst[0].k > 0) last_k = hst[0].k;
         for (int q = 0; q < nsys; ++q) {
             if (hmode[q] == kDone) continue;
             ++cycles[q];
@@ -991,6 +1128,26 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     SG_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     ctx->collect_profiles();
+    if (nsys == 1) {
+        ws.basis_valid = true;
+        ws.b_n = n;
+        ws.b_m = m;
+        ws.b_k = last_k;
+        ws.b_ld = ld;
+        ws.b_V = V;
+        ws.b_Z = Z;
+        ws.b_hasz = Z ? 1 : 0;
+        ws.b_Hr = W.Hr;
+        // v_k exists when the last step was not the cycle's m-th and did not
+        // break down (k_norm_scale forms v_{j+1} only then)
+        ws.b_kv = last_k;
+        if (last_k > 0 && last_k < m) {
+            double hk = 0.0;
+            SG_HIP(hipMemcpy(&hk, W.Hr + (size_t)last_k + (size_t)(last_k - 1) * (m + 1),
+                             sizeof(double), hipMemcpyDeviceToHost));
+            if (hk >= 1e-30) ws.b_kv = last_k + 1;
+        }
+    }
 
     const double elapsed = std::chrono::duration<double>(t1 - t0).count();
     std::vector<double> bn(nsys);
@@ -1062,5 +1219,33 @@ extern "C" sg_status sg_solve_batch(sg_csr* A, sg_precond* M, const double* d_b,
                                  (prm->flags & SG_SOLVE_NO_GRAPH) == 0);
         solve_impl(A, M, d_b, d_x, prm, res, hist, hist_cap,
                    (prm->flags & SG_SOLVE_NO_GRAPH) == 0);
+    });
+}
+
+extern "C" sg_status sg_solve_last_basis(sg_context* ctx, int* n, int* m, int* k, int* kv,
+                                         double* V, double* Z, double* Hbar) {
+    if (!ctx || !n || !m || !k || !kv) return SG_E_ARG;
+    return run_guarded(ctx, [&] {
+        KrylovCache* c = ctx->krylov;
+        if (!c || !c->basis_valid || c->busy)
+            fail(SG_E_STATE, "no single-system GMRES/FGMRES solve on this context yet");
+        *n = c->b_n;
+        *m = c->b_m;
+        *k = c->b_k;
+        *kv = c->b_kv;
+        SG_HIP(hipSetDevice(ctx->device));
+        SG_HIP(hipStreamSynchronize(ctx->stream));
+        const size_t nb = sizeof(double) * (size_t)c->b_n;
+        if (V)
+            for (int i = 0; i < c->b_kv; ++i)
+                SG_HIP(hipMemcpy(V + (size_t)i * c->b_n, c->b_V + (size_t)i * c->b_ld, nb,
+                                 hipMemcpyDeviceToHost));
+        if (Z && c->b_hasz)
+            for (int i = 0; i < c->b_k; ++i)
+                SG_HIP(hipMemcpy(Z + (size_t)i * c->b_n, c->b_Z + (size_t)i * c->b_ld, nb,
+                                 hipMemcpyDeviceToHost));
+        if (Hbar)
+            SG_HIP(hipMemcpy(Hbar, c->b_Hr, sizeof(double) * (size_t)(c->b_m + 1) * c->b_m,
+                             hipMemcpyDeviceToHost));
     });
 }

@@ -46,14 +46,18 @@ struct SegsHost {
     Segs dev() const { return Segs{blk.p, sys.p}; }
 };
 
+// pass > 0: a system of n elements may use up to ceil(n / pass) blocks when
+// that exceeds max_blocks, so its blocks stay within `pass` elements (up to
+// kMaxBlocks blocks).  The table of a system depends only on its size.
 inline void build_segs(const std::vector<int>& off, int max_blocks, int tile, SegsHost& out,
-                       hipStream_t s) {
+                       hipStream_t s, int pass = 0) {
     std::vector<BlockRange> br;
     std::vector<SysBlocks> sb;
     for (size_t q = 0; q + 1 < off.size(); ++q) {
         const int n = off[q + 1] - off[q];
         int nblk, chunk;
-        split_shape(n, max_blocks, tile, nblk, chunk);
+        const int mb = pass > 0 ? std::max(max_blocks, (n + pass - 1) / pass) : max_blocks;
+        split_shape(n, mb, tile, nblk, chunk);
         sb.push_back(SysBlocks{(int)br.size(), nblk});
         for (int b = 0; b < nblk; ++b)
             br.push_back(BlockRange{(int)q, off[q] + b * chunk,

@@ -88,8 +88,13 @@ inline DevGuard dev_guard(const Guard& g, const int* map) {
 struct Prof {
     long long launches = 0;
     double ms = 0.0;
+    double units = 0.0;  // work units of the timed launches (see ProfScope)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
+
+// GMRES/FGMRES workspace kept by a context between solves (krylov.hip).
+struct KrylovCache;
+void krylov_cache_free(KrylovCache* c);
 
 }  // namespace sg
 
@@ -111,6 +116,7 @@ struct sg_context {
     std::vector<std::pair<std::string, int>> tune_kv;
     sg::Prof prof[SG_K_COUNT];
     std::vector<hipEvent_t> event_pool;
+    sg::KrylovCache* krylov = nullptr;  // reused by every sg_solve on this context
 
     hipEvent_t get_event();
     void collect_profiles();  // stream must be synchronised
@@ -145,11 +151,14 @@ struct sg_csr {
 namespace sg {
 
 // RAII timing of one launch family on the context stream.
+// `units` is added to the family's work-unit count when profiling is on (the
+// Gram passes record their basis columns j + 1; other families 1 per launch).
 struct ProfScope {
     sg_context* ctx;
     int fam;
+    double units;
     hipEvent_t a = nullptr;
-    ProfScope(sg_context* c, int f);
+    ProfScope(sg_context* c, int f, double u = 1.0);
     ~ProfScope();
 };
 

@@ -39,8 +39,8 @@ constexpr short kSellPad16 = -32768;
 constexpr int kSellC = 64;        // rows per slice: one wavefront
 constexpr int kSellThreads = 256; // four slices per workgroup
 // The SELL copy is built only when its padding stays below half the nonzeros
-// and no row is longer than the CSR kernels' staging capacity (beyond it they
-// switch to a tree sum, which the SELL kernel would not reproduce).
+// and no row is longer than 2048 entries (a wavefront would spend most of a
+// slice on one row's padding).
 constexpr double kSellMaxFill = 1.5;
 constexpr int kSellMaxRow = 256 * kNnzPerThread;
 
@@ -85,16 +85,20 @@ k_spmv_stream(const int* __restrict__ rp, const int* __restrict__ ci,
             y[r] = beta == 0.0 ? alpha * s : alpha * s + beta * y0[r];
         }
     } else {
-        // one long row: strided partial sums and a tree reduction
+        // one row longer than the staging capacity (row_blocks gives it a block
+        // of its own): its products are staged kCap at a time and lane 0 adds
+        // each chunk in CSR order, so the sum is the serial row loop's whatever
+        // NT is
         double s = 0.0;
-        for (int k = k0 + t; k < k1; k += NT) s += v[k] * x[ci[k]];
-        prod[t] = s;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if (t < w) prod[t] += prod[t + w];
+        for (int c0 = k0; c0 < k1; c0 += kCap) {
+            const int c1 = min(k1, c0 + kCap);
+            for (int k = c0 + t; k < c1; k += NT) prod[k - c0] = v[k] * x[ci[k]];
+            __syncthreads();
+            if (t == 0)
+                for (int k = 0; k < c1 - c0; ++k) s += prod[k];
             __syncthreads();
         }
-        if (t == 0) y[r0] = beta == 0.0 ? alpha * prod[0] : alpha * prod[0] + beta * y0[r0];
+        if (t == 0) y[r0] = beta == 0.0 ? alpha * s : alpha * s + beta * y0[r0];
     }
 }
 
@@ -387,15 +391,18 @@ static sg_status csr_create(sg_context* ctx, int nrows, int ncols, int nnz, cons
         for (int k = 0; k < nnz; ++k)
             if (col_idx[k] < 0 || col_idx[k] >= ncols) fail(SG_E_ARG, "column index out of range");
         if (!sys_off.empty()) {
+            // offsets are validated before any of them indexes row_ptr
+            if (sys_off.front() != 0 || sys_off.back() != nrows || nrows != ncols)
+                fail(SG_E_ARG, "batch: system offsets must cover all rows of a square matrix");
+            for (size_t q = 0; q + 1 < sys_off.size(); ++q)
+                if (sys_off[q] < 0 || sys_off[q] >= sys_off[q + 1] || sys_off[q + 1] > nrows)
+                    fail(SG_E_ARG, "batch: every system needs at least one row");
             for (size_t q = 0; q + 1 < sys_off.size(); ++q) {
                 const int a = sys_off[q], b = sys_off[q + 1];
-                if (b <= a) fail(SG_E_ARG, "batch: every system needs at least one row");
                 for (int k = row_ptr[a]; k < row_ptr[b]; ++k)
                     if (col_idx[k] < a || col_idx[k] >= b)
                         fail(SG_E_ARG, "batch: matrix is not block diagonal over the systems");
             }
-            if (sys_off.front() != 0 || sys_off.back() != nrows || nrows != ncols)
-                fail(SG_E_ARG, "batch: system offsets must cover all rows of a square matrix");
         }
         SG_HIP(hipSetDevice(ctx->device));
         A->ctx = ctx;

@@ -150,3 +150,38 @@ def test_batch_errors(sg, ctx, P):
     bad_ci[-1] = n  # last row of system 0 points into system 1
     with pytest.raises(sg.SgError):
         sg.BatchCsr(ctx, [(rp, bad_ci, v), mats[1]])
+
+
+def test_batch_zero_rhs_system(sg, ctx, P):
+    """A system of the batch with b = 0: in true-residual mode it is converged
+    at the first restart (x0 = 0 is exact) instead of cycling to max_iter, and
+    the other systems still equal their single solves."""
+    systems = _small_batch(P, nsys=3, seed=13)
+    rp, ci, v, b = systems[1]
+    systems[1] = (rp, ci, v, np.zeros_like(b))
+    kw = dict(method="fgmres", restart=10, max_iter=300, rtol=1e-8, ortho="cgs2")
+    res, xs, parts = _batch(sg, ctx, systems, "ras", "fp32", kw)
+    single = _solve_each(sg, ctx, systems, parts, "ras", "fp32", kw)
+    assert res[1]["converged"] and res[1]["iterations"] == 0 and res[1]["cycles"] == 1
+    assert not np.any(xs[1])
+    for q, ((r1, x1), rb, xb) in enumerate(zip(single, res, xs)):
+        assert rb["converged"] and r1["converged"], q
+        assert rb["iterations"] == r1["iterations"], q
+        assert np.array_equal(xb, x1), q
+
+
+def test_batch_offsets_validated_before_use(sg, ctx):
+    """sg_csr_create_batch rejects system offsets that do not start at 0, run
+    past the rows or decrease, before it reads row_ptr at them."""
+    import ctypes as C
+    L = sg.lib()
+    rp = np.array([0, 1, 2, 3, 4], np.int32)
+    ci = np.array([0, 1, 2, 3], np.int32)
+    v = np.ones(4)
+    for off in ([-1000000, 2, 4], [0, 9999999, 4], [0, 3, 2, 4], [1, 2, 4]):
+        o = np.array(off, np.int32)
+        h = C.c_void_p()
+        st = L.sg_csr_create_batch(ctx.h, len(off) - 1, o.ctypes.data_as(C.c_void_p), 4,
+                                   rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                   v.ctypes.data_as(C.c_void_p), C.byref(h))
+        assert st == 1 and not h.value, off  # SG_E_ARG

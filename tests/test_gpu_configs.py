@@ -1,0 +1,209 @@
+"""The benchmark workloads at their full sizes, through the C ABI.
+
+Each test solves its workload on the HIP path, checks the FP64 true residual
+||b - A x|| / ||b|| <= 1e-8 on the host (scipy, independent of the device), and
+pins the pieces the oracle can check at that size:
+
+* 128^3 Poisson (n = 2.1M), RAS(64, overlap 2): one FP64 RAS apply bit for bit
+  against the oracle on the same partition; FP64 GMRES(50) and mixed two-level
+  FGMRES(50) converge, x within 1e-5 of x*.
+* 160^3 jump diffusion (n = 4.1M), two-level RAS(256, ufactor 300, overlap 1):
+  the coarse matrix A0, its LU factors and pivots at Nc = 256 bit for bit
+  against the oracle (src/precond/coarse_space.h:43-60, coarse_solve.h:15-47);
+  one FP32 two-level apply within 1e-5 of the oracle's FP32 mode; the mixed
+  two-level FGMRES(40) converges.
+* 64 GRF systems (sides 64..512, 6.4M rows): the batched graph solve converges
+  for every system, and three sampled systems equal their single solves bit
+  for bit (iterations and x).
+* 1024^2 GRF diffusion: the plain mixed two-level RAS-FGMRES(30) converges.
+"""
+import importlib.util
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_bind as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sg():
+    import __graft_entry__
+    return __graft_entry__.load_package()
+
+
+@pytest.fixture(scope="module")
+def P():
+    import __graft_entry__
+    spec = importlib.util.spec_from_file_location(
+        "sgamd_problems", os.path.join(__graft_entry__.PKG_DIR, "problems.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def say(msg):
+    print(f"[configs] {msg}", file=sys.stderr, flush=True)
+
+
+def true_rel(rp, ci, v, b, x):
+    n = len(rp) - 1
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+
+
+def dev(sg, ctx, a):
+    return sg.DeviceArray.from_host(ctx, np.ascontiguousarray(a, np.float64))
+
+
+def test_poisson128_ras64(sg, P):
+    t0 = time.perf_counter()
+    rp, ci, v = P.poisson3d(128, 128, 128)
+    n = len(rp) - 1
+    xs = P.uniform_pm1(2, n)
+    b = O.spmv(O.Csr(rp, ci, v), xs)
+    part = sg.partition(rp, ci, 64, 0)
+    say(f"poisson 128^3 n={n}: problem + partition {time.perf_counter() - t0:.1f} s")
+    ctx = sg.Context(0)
+    A = sg.Csr(ctx, rp, ci, v)
+    M = sg.Precond(ctx, "ras", nparts=64, overlap=2, precision="fp64", partition=part).setup(A)
+    # one FP64 RAS apply, bit for bit against the oracle on the same partition
+    rng = np.random.default_rng(20)
+    x = rng.standard_normal(n)
+    y = sg.DeviceArray(ctx, n)
+    M.apply(dev(sg, ctx, x), y)
+    say("poisson 128^3: oracle RAS(64, 2) setup + apply")
+    ref = O.Precond("ras", O.Csr(rp, ci, v), 64, 2, part).apply(x)
+    assert np.array_equal(y.get().view(np.int64), ref.view(np.int64))
+    db, dx = dev(sg, ctx, b), sg.DeviceArray(ctx, n)
+    for label, MM, kw in (
+            ("fp64 gmres(50)+ras", M, dict(method="gmres", restart=50, max_iter=2000,
+                                           rtol=1e-8, ortho="mgs")),
+            ("mixed fgmres(50)+twolevel_ras",
+             sg.Precond(ctx, "twolevel_ras", nparts=64, overlap=2, precision="fp32",
+                        partition=part, coarse_solve=1).setup(A),
+             dict(method="fgmres", restart=50, max_iter=2000, rtol=1e-8, ortho="cgs2"))):
+        dx.zero()
+        r = sg.solve(A, db, dx, MM, **kw)
+        xh = dx.get()
+        rel = true_rel(rp, ci, v, b, xh)
+        err = np.linalg.norm(xh - xs) / np.linalg.norm(xs)
+        say(f"poisson 128^3 {label}: {r['iterations']} it, rel {rel:.2e}, x err {err:.2e}")
+        assert r["converged"] and rel <= 1e-8
+        assert abs(rel - r["rel_true_residual"]) <= 1e-3 * rel
+        assert err <= 1e-5
+    ctx.close()
+
+
+def test_jump_diffusion160_two_level(sg, P):
+    t0 = time.perf_counter()
+    rp, ci, v = P.diffusion3d(P.block_jump_kappa(160, 16, 3))
+    n = len(rp) - 1
+    b = P.normal(33, n)
+    part = sg.partition(rp, ci, 256, 300, vals=v, weighted=True)
+    say(f"diffusion 160^3 n={n}: problem + partition {time.perf_counter() - t0:.1f} s")
+    ctx = sg.Context(0)
+    A = sg.Csr(ctx, rp, ci, v)
+    M64 = sg.Precond(ctx, "twolevel_ras", nparts=256, overlap=1, precision="fp64",
+                     partition=part).setup(A)
+    A0, LU, piv = M64.coarse()
+    assert A0.shape == (256, 256)
+    del M64
+    M = sg.Precond(ctx, "twolevel_ras", nparts=256, overlap=1, precision="fp32",
+                   partition=part, coarse_solve=1).setup(A)
+    x = np.random.default_rng(21).standard_normal(n)
+    y = sg.DeviceArray(ctx, n)
+    M.apply(dev(sg, ctx, x), y)
+    y32 = y.get()
+    db, dx = dev(sg, ctx, b), sg.DeviceArray(ctx, n)
+    dx.zero()
+    r = sg.solve(A, db, dx, M, method="fgmres", restart=40, max_iter=10000, rtol=1e-8,
+                 ortho="cgs2")
+    rel = true_rel(rp, ci, v, b, dx.get())
+    say(f"diffusion 160^3 mixed fgmres(40)+twolevel_ras: {r['iterations']} it, rel {rel:.2e}")
+    assert r["converged"] and rel <= 1e-8
+    ctx.close()
+    # the oracle on the same partition (OpenMP over subdomains)
+    Ao = O.Csr(rp, ci, v)
+    say("diffusion 160^3: oracle two-level FP64 setup")
+    ref64 = O.Precond("twolevel_ras", Ao, 256, 1, part, precision=0)
+    rA0, rLU, rpiv = ref64.coarse()
+    assert np.array_equal(A0, rA0)
+    assert np.array_equal(piv, rpiv)
+    assert np.array_equal(LU, rLU)
+    del ref64
+    say("diffusion 160^3: oracle two-level FP32 setup + apply")
+    want = O.Precond("twolevel_ras", Ao, 256, 1, part, precision=1).apply(x)
+    assert np.linalg.norm(y32 - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_batch64_grf(sg, P):
+    t0 = time.perf_counter()
+    systems = P.batch_systems(64, base_seed=4000)
+    mats = [(rp, ci, v) for rp, ci, v, _ in systems]
+    nparts = [P.batch_nparts(len(rp) - 1) for rp, _, _ in mats]
+    part = sg.batch_partition(mats, nparts)
+    b = np.concatenate([s[3] for s in systems])
+    say(f"batch 64: {len(b)} rows, problems + partitions {time.perf_counter() - t0:.1f} s")
+    ctx = sg.Context(0)
+    B = sg.BatchCsr(ctx, mats)
+    M = sg.Precond(ctx, "ras", nparts=int(sum(nparts)), overlap=2, precision="fp32",
+                   partition=part).setup(B)
+    kw = dict(method="fgmres", restart=30, max_iter=2000, rtol=1e-8, ortho="cgs2")
+    dx = sg.DeviceArray(ctx, B.n)
+    dx.zero()
+    res = sg.solve_batch(B, dev(sg, ctx, b), dx, M, graph=True, **kw)
+    xb = dx.get()
+    its = [q["iterations"] for q in res]
+    say(f"batch 64: iterations {min(its)}..{max(its)}")
+    for q, (rp, ci, v, bq) in enumerate(systems):
+        xq = xb[B.sys_rows[q]:B.sys_rows[q + 1]]
+        assert res[q]["converged"], q
+        assert true_rel(rp, ci, v, bq, xq) <= 1e-8, q
+    del M, B
+    # three sampled systems alone: bit for bit the batch's iterates
+    for q in (0, 31, 63):
+        rp, ci, v, bq = systems[q]
+        pq = part[B_rows(systems, q)]
+        A = sg.Csr(ctx, rp, ci, v)
+        Mq = sg.Precond(ctx, "ras", nparts=nparts[q], overlap=2,
+                        precision="fp32", partition=pq - pq.min()).setup(A)
+        xq = sg.DeviceArray(ctx, len(bq))
+        xq.zero()
+        r1 = sg.solve(A, dev(sg, ctx, bq), xq, Mq, **kw)
+        lo, hi = B_rows(systems, q).start, B_rows(systems, q).stop
+        assert r1["iterations"] == res[q]["iterations"], q
+        assert np.array_equal(xq.get(), xb[lo:hi]), q
+        del Mq, A
+    ctx.close()
+
+
+def B_rows(systems, q):
+    off = np.cumsum([0] + [len(s[3]) for s in systems])
+    return slice(int(off[q]), int(off[q + 1]))
+
+
+def test_grf1024_two_level(sg, P):
+    t0 = time.perf_counter()
+    rp, ci, v = P.diffusion2d(np.exp(P.grf(1024, 1.5, 0.05, 5)))
+    n = len(rp) - 1
+    b = P.normal(55, n)
+    part = sg.partition(rp, ci, 256, 0, vals=v, weighted=True)
+    say(f"grf 1024^2: problem + partition {time.perf_counter() - t0:.1f} s")
+    ctx = sg.Context(0)
+    A = sg.Csr(ctx, rp, ci, v)
+    M = sg.Precond(ctx, "twolevel_ras", nparts=256, overlap=2, precision="fp32",
+                   partition=part, coarse_solve=1).setup(A)
+    dx = sg.DeviceArray(ctx, n)
+    dx.zero()
+    r = sg.solve(A, dev(sg, ctx, b), dx, M, method="fgmres", restart=30, max_iter=10000,
+                 rtol=1e-8, ortho="cgs2")
+    rel = true_rel(rp, ci, v, b, dx.get())
+    say(f"grf 1024^2 mixed fgmres(30)+twolevel_ras: {r['iterations']} it, rel {rel:.2e}")
+    assert r["converged"] and rel <= 1e-8
+    ctx.close()
