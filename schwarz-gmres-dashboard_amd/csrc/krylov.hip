@@ -861,6 +861,108 @@ struct GraphHold {
     }
 };
 
+// The launches of one restart cycle over a laid-out workspace: restart
+// residual and v_0, Arnoldi step j, back substitution and x update.
+struct Cycle {
+    sg_context* ctx;
+    sg_csr* A;
+    sg_precond* M;
+    const double* d_b;
+    double* d_x;
+    int method, m, nsys;
+    bool cgs2, rmode;
+    double *V, *Z, *r, *z, *w, *t;
+    size_t ld;
+    Work W;
+    Segs sp_red, sp_gd, sp_gu;
+    int nb_red, nb_gd, nb_gu;
+    Guard step_guard;
+    size_t bs_lds;
+    hipStream_t s;
+
+    void restart() const {
+        {
+            ProfScope ps(ctx, SG_K_UPDATE);
+            spmv_launch(A, -1.0, d_x, 1.0, d_b, r, Guard{}, s);  // r = b - A x
+        }
+        const double* u = r;
+        if (method == 0 && M) {
+            precond_apply_launch(M, r, z, Guard{});
+            u = z;
+        }
+        ProfScope ps(ctx, SG_K_UPDATE);
+        const bool has_r = rmode && u != r;
+        hipLaunchKernelGGL(k_norms, dim3(nb_red), dim3(kT), 0, s, sp_red, u,
+                           has_r ? (const double*)r : nullptr, W, 1);
+        hipLaunchKernelGGL(k_restart_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red,
+                           has_r ? 1 : 0);
+        hipLaunchKernelGGL(k_scale_basis, dim3(nb_red), dim3(kT), 0, s, sp_red, u, V, W);
+        SG_HIP(hipGetLastError());
+    }
+
+    void step(int j) const {
+        double* Vj = V + (size_t)j * ld;
+        if (method == 0) {
+            if (M) {
+                spmv_launch(A, 1.0, Vj, 0.0, nullptr, t, step_guard, s);
+                precond_apply_launch(M, t, w, step_guard);
+            } else {
+                spmv_launch(A, 1.0, Vj, 0.0, nullptr, w, step_guard, s);
+            }
+        } else {
+            double* Zj = M ? Z + (size_t)j * ld : Vj;
+            if (M) precond_apply_launch(M, Vj, Zj, step_guard);
+            spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, step_guard, s);
+        }
+        {
+            ProfScope ps(ctx, SG_K_ORTHO);
+            if (cgs2) {
+                {
+                    ProfScope pd(ctx, SG_K_GRAM_DOT, j + 1);
+                    hipLaunchKernelGGL(k_gram_dot, dim3(nb_gd), dim3(kTGD), 0, s, sp_gd, j + 1, V,
+                                       ld, w, W, j);
+                }
+                hipLaunchKernelGGL(k_gram_sums, dim3(2 * (j + 1), nsys), dim3(64), 0, s, W, sp_gd,
+                                   j + 1, j);
+                {
+                    ProfScope pu(ctx, SG_K_GRAM_UPDATE, j + 1);
+                    hipLaunchKernelGGL(k_gram_update, dim3(nb_gu), dim3(kTGU), 0, s, sp_gu, j + 1,
+                                       V, ld, w, W, j);
+                }
+                if (j + 1 == m)
+                    hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
+            } else {
+                for (int i = 0; i <= j + 1; ++i) {
+                    const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
+                    const double* Vi = i <= j ? V + (size_t)i * ld : nullptr;
+                    hipLaunchKernelGGL(k_mgs, dim3(nb_red), dim3(kT), 0, s, sp_red, Vp, Vi, w, W,
+                                       i, j);
+                    hipLaunchKernelGGL(k_mgs_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red, i, j,
+                                       i > j ? 1 : 0);
+                }
+            }
+            SG_HIP(hipGetLastError());
+        }
+        if (j + 1 < m) {
+            ProfScope ps(ctx, SG_K_UPDATE);
+            if (cgs2)
+                hipLaunchKernelGGL(k_norm_scale, dim3(nb_red), dim3(kT), 0, s, sp_red, sp_gu, w,
+                                   V + (size_t)(j + 1) * ld, W, j);
+            else
+                hipLaunchKernelGGL(k_scale_basis, dim3(nb_red), dim3(kT), 0, s, sp_red, w,
+                                   V + (size_t)(j + 1) * ld, W);
+        }
+    }
+
+    void tail() const {
+        ProfScope ps(ctx, SG_K_UPDATE);
+        hipLaunchKernelGGL(k_backsolve, dim3(nsys), dim3(64), bs_lds, s, W);
+        hipLaunchKernelGGL(k_update, dim3(nb_red), dim3(kT), 0, s, sp_red, d_x,
+                           (method && M) ? Z : V, ld, W);
+        SG_HIP(hipGetLastError());
+    }
+};
+
 static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                        const sg_solve_params* prm, sg_solve_result* res, double* hist,
                        int hist_cap, bool use_graph) {
@@ -962,86 +1064,12 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     hipLaunchKernelGGL(k_bnorm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red);
     SG_HIP(hipGetLastError());
 
-    // ---- the pieces of one restart cycle ----
-    auto launch_restart = [&]() {
-        {
-            ProfScope ps(ctx, SG_K_UPDATE);
-            spmv_launch(A, -1.0, d_x, 1.0, d_b, r, Guard{}, s);  // r = b - A x
-        }
-        const double* u = r;
-        if (method == 0 && M) {
-            precond_apply_launch(M, r, z, Guard{});
-            u = z;
-        }
-        ProfScope ps(ctx, SG_K_UPDATE);
-        const bool has_r = rmode && u != r;
-        hipLaunchKernelGGL(k_norms, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, u,
-                           has_r ? (const double*)r : nullptr, W, 1);
-        hipLaunchKernelGGL(k_restart_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red,
-                           has_r ? 1 : 0);
-        hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, u, V, W);
-        SG_HIP(hipGetLastError());
-    };
-    auto launch_step = [&](int j) {
-        double* Vj = V + (size_t)j * ld;
-        if (method == 0) {
-            if (M) {
-                spmv_launch(A, 1.0, Vj, 0.0, nullptr, t, step_guard, s);
-                precond_apply_launch(M, t, w, step_guard);
-            } else {
-                spmv_launch(A, 1.0, Vj, 0.0, nullptr, w, step_guard, s);
-            }
-        } else {
-            double* Zj = M ? Z + (size_t)j * ld : Vj;
-            if (M) precond_apply_launch(M, Vj, Zj, step_guard);
-            spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, step_guard, s);
-        }
-        {
-            ProfScope ps(ctx, SG_K_ORTHO);
-            if (cgs2) {
-                {
-                    ProfScope pd(ctx, SG_K_GRAM_DOT, j + 1);
-                    hipLaunchKernelGGL(k_gram_dot, dim3(seg_gd.nblk), dim3(kTGD), 0, s, sp_gd,
-                                       j + 1, V, ld, w, W, j);
-                }
-                hipLaunchKernelGGL(k_gram_sums, dim3(2 * (j + 1), nsys), dim3(64), 0, s, W, sp_gd,
-                                   j + 1, j);
-                {
-                    ProfScope pu(ctx, SG_K_GRAM_UPDATE, j + 1);
-                    hipLaunchKernelGGL(k_gram_update, dim3(seg_gu.nblk), dim3(kTGU), 0, s, sp_gu,
-                                       j + 1, V, ld, w, W, j);
-                }
-                if (j + 1 == m)
-                    hipLaunchKernelGGL(k_norm_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_gu, j);
-            } else {
-                for (int i = 0; i <= j + 1; ++i) {
-                    const double* Vp = i ? V + (size_t)(i - 1) * ld : nullptr;
-                    const double* Vi = i <= j ? V + (size_t)i * ld : nullptr;
-                    hipLaunchKernelGGL(k_mgs, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, Vp, Vi,
-                                       w, W, i, j);
-                    hipLaunchKernelGGL(k_mgs_finish, dim3(nsys), dim3(kTF), 0, s, W, sp_red, i, j,
-                                       i > j ? 1 : 0);
-                }
-            }
-            SG_HIP(hipGetLastError());
-        }
-        if (j + 1 < m) {
-            ProfScope ps(ctx, SG_K_UPDATE);
-            if (cgs2)
-                hipLaunchKernelGGL(k_norm_scale, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, sp_gu,
-                                   w, V + (size_t)(j + 1) * ld, W, j);
-            else
-                hipLaunchKernelGGL(k_scale_basis, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, w,
-                                   V + (size_t)(j + 1) * ld, W);
-        }
-    };
-    auto launch_tail = [&]() {
-        ProfScope ps(ctx, SG_K_UPDATE);
-        hipLaunchKernelGGL(k_backsolve, dim3(nsys), dim3(64), bs_lds, s, W);
-        hipLaunchKernelGGL(k_update, dim3(seg_red.nblk), dim3(kT), 0, s, sp_red, d_x,
-                           (method && M) ? Z : V, ld, W);
-        SG_HIP(hipGetLastError());
-    };
+    const Cycle cyc{ctx, A, M, d_b, d_x, method, m, nsys, cgs2, rmode, V, Z, r, z, w, t, ld, W,
+                    sp_red, sp_gd, sp_gu, seg_red.nblk, seg_gd.nblk, seg_gu.nblk, step_guard,
+                    bs_lds, s};
+    auto launch_restart = [&]() { cyc.restart(); };
+    auto launch_step = [&](int j) { cyc.step(j); };
+    auto launch_tail = [&]() { cyc.tail(); };
 
     if (use_graph) {
         // kernel shapes are picked before capture (tuning times real launches)
