@@ -228,53 +228,58 @@ __device__ void reduce_partials(const Work& W, const SysBlocks& sb, int ncols, d
 // issued before the scheduling barrier, so a wave keeps a whole register
 // block of loads in flight.
 
-// Pass 1 holds a thread's elements of w and v_j in registers (up to
-// kTilesGD tiles of kEoG elements, i.e. a block range of up to 4096 elements
-// in one register pass) and streams the other basis columns past them, so w
-// and v_j are read once per step and column j is not read a second time:
-// 8 n (j + 2) bytes.  Within a thread the products are added tile by tile, as
-// the order of a tile loop over the block range would add them.
-constexpr int kTilesGD = 4;
+// Pass 1 stages a block's range of w and v_j (up to kPassGD elements) in LDS
+// once and streams the other basis columns past it, so w and v_j are read
+// from memory once per step and column j is not read a second time:
+// 8 n (j + 2) bytes.  Each thread adds its products tile by tile (kEoG
+// elements of a kTileGD tile per thread), the order a tile loop over the range
+// gives, whatever the column blocking.
+constexpr int kTilesGD = 2;
 constexpr int kPassGD = kTileGD * kTilesGD;
 
-// Dots of columns [cb, cb + CB) with w and v_j over the resident elements;
-// lane 0 of each wave stores (first pass) or adds its wave's sums into redw.
+// Dots of columns [cb, cb + CB) with w and v_j over the staged range; lane 0
+// of each wave stores (first pass) or adds its wave's sums into redw.
 template <int CB>
-__device__ __forceinline__ void gram_dot_cols(int cb, int ncols, int ntile, bool first,
+__device__ __forceinline__ void gram_dot_cols(int cb, int ncols, int p0, int p1, bool first,
                                               const double* __restrict__ V, size_t ld,
-                                              const int (&ee)[kTilesGD][kEoG],
-                                              const double (&wr)[kTilesGD][kEoG],
-                                              const double (&vr)[kTilesGD][kEoG],
-                                              double* redw) {
+                                              const double* sw, const double* sv, double* redw) {
+    const int tid = threadIdx.x;
     double aw[CB], av[CB];
 #pragma unroll
     for (int c = 0; c < CB; ++c) aw[c] = av[c] = 0.0;
-#pragma unroll
-    for (int t = 0; t < kTilesGD; ++t) {
-        if (t >= ntile) break;  // uniform over the block
+    for (int base = p0 + tid; base < p1; base += kTileGD) {
+        int ee[kEoG];
         double a[CB][kEoG];
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) ee[r] = min(base + r * kTGD, p1 - 1);
 #pragma unroll
         for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[t][r]];
+            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+        double wr[kEoG], vr[kEoG];
+#pragma unroll
+        for (int r = 0; r < kEoG; ++r) {
+            wr[r] = sw[base + r * kTGD - p0];
+            vr[r] = sv[base + r * kTGD - p0];
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
 #pragma unroll
             for (int r = 0; r < kEoG; ++r) {
-                aw[c] += a[c][r] * wr[t][r];
-                av[c] += a[c][r] * vr[t][r];
+                aw[c] += a[c][r] * wr[r];
+                av[c] += a[c][r] * vr[r];
             }
         }
     }
-    const int lane = threadIdx.x & 63;
+    const int lane = tid & 63;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
-        const double sw = wave_sum(aw[c]);
-        const double sv = wave_sum(av[c]);
+        const double sw_ = wave_sum(aw[c]);
+        const double sv_ = wave_sum(av[c]);
         if (lane == 0) {
-            redw[cb + c] = first ? sw : redw[cb + c] + sw;
-            redw[ncols + cb + c] = first ? sv : redw[ncols + cb + c] + sv;
+            redw[cb + c] = first ? sw_ : redw[cb + c] + sw_;
+            redw[ncols + cb + c] = first ? sv_ : redw[ncols + cb + c] + sv_;
         }
     }
 }
@@ -285,71 +290,63 @@ k_gram_dot(Segs sp, int ncols, const double* __restrict__ V, size_t ld,
     const BlockRange br = sp.blk[blockIdx.x];
     if (!W.st[br.sys].active) return;
     __shared__ double red[kTGD / 64][kRedCols];
+    // slots past the range (up to the end of its last tile) hold a clamped
+    // copy scaled by zero, as a multiply by a 0/1 mask would leave them
+    __shared__ double sw[kPassGD], sv[kPassGD];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     double* redw = red[wv];  // only lane 0 of wave wv touches it
     const double* vj = V + (size_t)j * ld;
     const int nv = ncols - 1;  // columns 0..j-1 are streamed; column j is v_j itself
     for (int p0 = br.start; p0 < br.end; p0 += kPassGD) {
         const int p1 = min(br.end, p0 + kPassGD);
-        const int ntile = (p1 - p0 + kTileGD - 1) / kTileGD;
         const bool first = p0 == br.start;
-        int ee[kTilesGD][kEoG];
-        double wr[kTilesGD][kEoG], vr[kTilesGD][kEoG], msk[kTilesGD][kEoG];
+        if (!first) __syncthreads();  // the previous pass is done with sw / sv
+        const int ntile = (p1 - p0 + kTileGD - 1) / kTileGD;
+        for (int t = 0; t < ntile; ++t) {
+            double a[kEoG], b[kEoG];
+            int ee[kEoG];
 #pragma unroll
-        for (int t = 0; t < kTilesGD; ++t)
+            for (int r = 0; r < kEoG; ++r) {
+                ee[r] = min(p0 + t * kTileGD + r * kTGD + tid, p1 - 1);
+                a[r] = w[ee[r]];
+                b[r] = vj[ee[r]];
+            }
 #pragma unroll
             for (int r = 0; r < kEoG; ++r) {
                 const int e = p0 + t * kTileGD + r * kTGD + tid;
-                ee[t][r] = min(e, p1 - 1);
-                msk[t][r] = e < p1 ? 1.0 : 0.0;
-            }
-#pragma unroll
-        for (int t = 0; t < kTilesGD; ++t) {
-            if (t >= ntile) break;
-#pragma unroll
-            for (int r = 0; r < kEoG; ++r) {
-                wr[t][r] = w[ee[t][r]];
-                vr[t][r] = vj[ee[t][r]];
+                sw[e - p0] = e < p1 ? a[r] : a[r] * 0.0;
+                sv[e - p0] = e < p1 ? b[r] : b[r] * 0.0;
             }
         }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int t = 0; t < kTilesGD; ++t) {
-            if (t >= ntile) break;
-#pragma unroll
-            for (int r = 0; r < kEoG; ++r) {
-                wr[t][r] *= msk[t][r];
-                vr[t][r] *= msk[t][r];
-            }
-        }
+        __syncthreads();
         int cb = 0;
         for (; cb + kCBG <= nv; cb += kCBG)
-            gram_dot_cols<kCBG>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+            gram_dot_cols<kCBG>(cb, ncols, p0, p1, first, V, ld, sw, sv, redw);
         if (nv - cb >= 4) {
-            gram_dot_cols<4>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+            gram_dot_cols<4>(cb, ncols, p0, p1, first, V, ld, sw, sv, redw);
             cb += 4;
         }
         if (nv - cb >= 2) {
-            gram_dot_cols<2>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
+            gram_dot_cols<2>(cb, ncols, p0, p1, first, V, ld, sw, sv, redw);
             cb += 2;
         }
-        if (nv - cb >= 1) gram_dot_cols<1>(cb, ncols, ntile, first, V, ld, ee, wr, vr, redw);
-        // column j: v_j . w and v_j . v_j from the resident values
-        double sw = 0.0, sv = 0.0;
-#pragma unroll
-        for (int t = 0; t < kTilesGD; ++t) {
-            if (t >= ntile) break;
+        if (nv - cb >= 1) gram_dot_cols<1>(cb, ncols, p0, p1, first, V, ld, sw, sv, redw);
+        // column j: v_j . w and v_j . v_j from the staged values, in the same
+        // per-thread order
+        double s1 = 0.0, s2 = 0.0;
+        for (int base = p0 + tid; base < p1; base += kTileGD) {
 #pragma unroll
             for (int r = 0; r < kEoG; ++r) {
-                sw += vr[t][r] * wr[t][r];
-                sv += vr[t][r] * vr[t][r];
+                const double wv_ = sw[base + r * kTGD - p0], vv = sv[base + r * kTGD - p0];
+                s1 += vv * wv_;
+                s2 += vv * vv;
             }
         }
-        sw = wave_sum(sw);
-        sv = wave_sum(sv);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
         if (lane == 0) {
-            redw[j] = first ? sw : redw[j] + sw;
-            redw[ncols + j] = first ? sv : redw[ncols + j] + sv;
+            redw[j] = first ? s1 : redw[j] + s1;
+            redw[ncols + j] = first ? s2 : redw[ncols + j] + s2;
         }
     }
     // columns [0, ncols) hold V^T w, [ncols, 2 ncols) hold V^T v_j
