@@ -265,6 +265,27 @@ sg_status sg_solve(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                    const sg_solve_params* params, sg_solve_result* result, double* hist,
                    int hist_cap);
 
+/* No-allocation variant of sg_solve.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or pinned memory per call (tight
+ * serving loops, harnesses that forbid it).  Same arguments, arithmetic and
+ * results as sg_solve on a single system (iterations, x and history equal bit
+ * for bit), with these limits: GMRES or FGMRES on a single-system matrix; a
+ * device preconditioner (not a host callback or the learned stage); no graph
+ * (SG_SOLVE_GRAPH is rejected) and no lookahead (every step of a cycle is
+ * launched, finished systems' kernels return at once); kernel shapes are not
+ * timed (the shapes in place are used; every shape computes identical values).
+ * Its whole workspace -- state, reduction tables, Hessenberg and Gram storage,
+ * history, partial sums and the Krylov basis, about 8 n (2m + 6) bytes for
+ * FGMRES with a preconditioner, 8 n (m + 6) otherwise, plus 8 (max_iter + m)
+ * -- lives in one static device array of 64 MiB on each GPU the library runs
+ * on, held until the process ends; a larger system fails with SG_E_ARG before
+ * anything is launched.  Calls that overlap on one GPU (from any contexts or
+ * threads) race on that array; each call starts from a state it initialises
+ * itself, so the contents left by an earlier call do not matter. */
+sg_status sg_solve_noalloc(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                           const sg_solve_params* params, sg_solve_result* result, double* hist,
+                           int hist_cap);
+
 /* Verification hook, no counterpart in the reference: the Krylov basis of the
  * last restart cycle that ran Arnoldi steps in the last single-system
  * GMRES/FGMRES sg_solve on this context (the workspace is kept by the context

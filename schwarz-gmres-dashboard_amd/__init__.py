@@ -165,6 +165,8 @@ def lib():
         "sg_convergence_write_jsonl": ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int,
                                         _VP, C.c_int, C.c_int], C.c_int),
         "sg_mm_write": ([C.c_char_p, C.c_int, C.c_int, _VP, _VP, _VP, C.c_int], C.c_int),
+        "sg_solve_noalloc": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), C.POINTER(SolveResult),
+                              _VP, C.c_int], C.c_int),
         "sg_solve_last_basis": ([_VP, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), _VP, _VP, _VP], C.c_int),
     }

@@ -223,6 +223,7 @@ sg_status sg_context_create(int device, sg_context** out) {
         SG_HIP(hipMemset(ctx->d_ticket, 0, sizeof(unsigned) * 64));
         SG_HIP(hipMalloc(&ctx->d_result, sizeof(double) * kMaxRedCols));
         SG_HIP(hipHostMalloc(&ctx->h_pinned, sizeof(double) * 256, hipHostMallocDefault));
+        sg::noalloc_init_device(device);
     });
     if (st != SG_OK) {
         delete ctx;

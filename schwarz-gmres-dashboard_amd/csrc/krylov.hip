@@ -31,6 +31,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 namespace sg {
 
@@ -1216,6 +1218,244 @@ static void solve_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
     }
 }
 
+// ---- no-allocation solve ------------------------------------------------------
+// sg_solve_noalloc keeps its whole device workspace (state, control word,
+// reduction tables, Hessenberg and Gram storage, partial sums, Krylov basis)
+// in one static device array per GPU, so a call makes no device or pinned
+// allocation.  Host code never passes `scratch` itself to a kernel: it uses
+// the device address that hipGetSymbolAddress returns for the current device
+// (g_scratch[device], looked up once per device at context creation).
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per GPU
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxScratchDevices];
+static std::string g_scratch_err[kMaxScratchDevices];
+
+void noalloc_init_device(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch[device] = static_cast<double*>(p);
+}
+
+static double* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        fail(SG_E_ARG, "sg_solve_noalloc: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device])
+        fail(SG_E_HIP, "sg_solve_noalloc: scratch array unavailable on this device (" +
+                           g_scratch_err[device] + ")");
+    return g_scratch[device];
+}
+
+// Block table of one system of n elements into a caller array (build_segs for
+// one system, without allocating); returns the block count.
+static int segs_into(int n, int max_blocks, int tile, int pass, BlockRange* out) {
+    int nblk, chunk;
+    const int mb = pass > 0 ? std::max(max_blocks, (n + pass - 1) / pass) : max_blocks;
+    split_shape(n, mb, tile, nblk, chunk);
+    for (int b = 0; b < nblk; ++b) out[b] = BlockRange{0, b * chunk, std::min(n, (b + 1) * chunk), 0};
+    return nblk;
+}
+
+static void solve_noalloc_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                               const sg_solve_params* prm, sg_solve_result* res, double* hist,
+                               int hist_cap) {
+    sg_context* ctx = A->ctx;
+    if (A->sys_off.size() != 2) fail(SG_E_ARG, "sg_solve_noalloc: single systems only");
+    if (A->nrows != A->ncols) fail(SG_E_ARG, "GMRES needs a square matrix");
+    if (prm->method != SG_GMRES_LEFT && prm->method != SG_FGMRES)
+        fail(SG_E_ARG, "sg_solve_noalloc: GMRES or FGMRES only");
+    if (prm->flags & SG_SOLVE_GRAPH) fail(SG_E_ARG, "sg_solve_noalloc: no graph capture");
+    if (M && !precond_ready(M)) fail(SG_E_STATE, "preconditioner used before setup");
+    if (M && precond_n(M) != A->nrows) fail(SG_E_ARG, "preconditioner size mismatch");
+    if (M && (precond_is_callback(M) || precond_is_flexible(M)))
+        fail(SG_E_ARG, "sg_solve_noalloc: device preconditioners only (no callback or learned stage)");
+    const int n = A->nrows;
+    const int m = prm->restart;
+    const int method = prm->method == SG_FGMRES ? 1 : 0;
+    if (m < 1 || m + 1 > kMaxCols) fail(SG_E_ARG, "restart must be in [1, 127]");
+    if (prm->max_iter < 0) fail(SG_E_ARG, "max_iter must be >= 0");
+    if (n < 1) fail(SG_E_ARG, "empty system");
+    const bool cgs2 = prm->ortho == SG_ORTHO_CGS2;
+    const bool rmode = prm->rtol > 0.0;
+    hipStream_t s = ctx->stream;
+    SG_HIP(hipSetDevice(ctx->device));
+    double* base = scratch_ptr(ctx->device);
+
+    // reduction tables on the host (no heap), then into the scratch array
+    thread_local BlockRange hb[3][kMaxBlocks];
+    const int nb_red = segs_into(n, 4 * ctx->num_cu, kTile, 0, hb[0]);
+    const int nb_gd = segs_into(n, 4 * ctx->num_cu, kTileGD, kPassGD, hb[1]);
+    const int nb_gu = segs_into(n, 4 * ctx->num_cu, kTileGU, 0, hb[2]);
+    const int pstride = std::max(nb_red, std::max(nb_gd, nb_gu));
+    const int nbs[3] = {nb_red, nb_gd, nb_gu};
+
+    // layout (byte offsets, 256-aligned): state | mode | 3 x (blocks, system) |
+    // small | basis
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t ld = basis_ld(n);
+    const int nz = (method && M) ? m : 0;
+    const int nvec = (m + 1) + nz + 4;
+    const size_t hcap = (size_t)prm->max_iter + m + 2;
+    const size_t small_n = 2 * (size_t)(m + 1) * m + (m + 1) + 3 * (size_t)m + 2 * kRedCols +
+                           hcap + (size_t)kMaxCols * kMaxCols + (size_t)kRedCols * pstride;
+    size_t off = 0;
+    const size_t o_st = off;
+    off += al(sizeof(KState));
+    const size_t o_mode = off;
+    off += al(sizeof(int));
+    size_t o_blk[3], o_sys[3];
+    for (int q = 0; q < 3; ++q) {
+        o_blk[q] = off;
+        off += al(sizeof(BlockRange) * nbs[q]);
+        o_sys[q] = off;
+        off += al(sizeof(SysBlocks));
+    }
+    const size_t o_small = off;
+    off += al(sizeof(double) * small_n);
+    const size_t o_vecs = off;
+    const double need = (double)off + (double)sizeof(double) * (double)ld * nvec;
+    if (need > (double)kScratchBytes)
+        fail(SG_E_ARG, "sg_solve_noalloc: the system needs " +
+                           std::to_string((long long)need) + " bytes of workspace; the static "
+                           "scratch array holds " + std::to_string(kScratchBytes));
+    char* cb = reinterpret_cast<char*>(base);
+    KState* st = reinterpret_cast<KState*>(cb + o_st);
+    int* dmode = reinterpret_cast<int*>(cb + o_mode);
+    Segs sp[3];
+    for (int q = 0; q < 3; ++q) {
+        BlockRange* db = reinterpret_cast<BlockRange*>(cb + o_blk[q]);
+        SysBlocks* ds = reinterpret_cast<SysBlocks*>(cb + o_sys[q]);
+        const SysBlocks hs{0, nbs[q]};
+        SG_HIP(hipMemcpyAsync(db, hb[q], sizeof(BlockRange) * nbs[q], hipMemcpyHostToDevice, s));
+        SG_HIP(hipMemcpyAsync(ds, &hs, sizeof(SysBlocks), hipMemcpyHostToDevice, s));
+        sp[q] = Segs{db, ds};
+    }
+    SG_HIP(hipStreamSynchronize(s));  // the host tables are reused by the next call
+    double* small = reinterpret_cast<double*>(cb + o_small);
+    double* V = reinterpret_cast<double*>(cb + o_vecs);
+    double* Z = nz ? V + ld * (m + 1) : nullptr;
+    double* r = V + ld * ((m + 1) + nz);
+    double* z = r + ld;
+    double* w = z + ld;
+    double* t = w + ld;
+    Work W;
+    W.H = small;
+    W.Hr = W.H + (size_t)(m + 1) * m;
+    W.g = W.Hr + (size_t)(m + 1) * m;
+    W.cs = W.g + (m + 1);
+    W.sn = W.cs + m;
+    W.y = W.sn + m;
+    W.h1 = W.y + m;
+    W.h2 = W.h1 + kRedCols;
+    W.hist = W.h2 + kRedCols;
+    W.G = W.hist + hcap;
+    W.part = W.G + (size_t)kMaxCols * kMaxCols;
+    W.st = st;
+    W.mode = dmode;
+    W.m = m;
+    W.hcap = (int)hcap;
+    W.pstride = pstride;
+    W.hflag = nullptr;
+
+    // no kernel-shape timing and no profiling events inside this call
+    struct NoTune {
+        sg_context* c;
+        bool was;
+        ~NoTune() { c->no_tune = was; }
+    } nt{ctx, ctx->no_tune};
+    ctx->no_tune = true;
+    ProfOff po(ctx);
+
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t bs_lds = sizeof(double) * ((size_t)m * (m + 1) / 2 + m + 1);
+    SG_HIP(hipFuncSetAttribute((const void*)k_backsolve,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bs_lds));
+    const double tol = rmode ? 0.0 : prm->tol_abs;
+    // every value read before it is written in this call starts here: the
+    // state (k_init_state) and the control word; the rest of the array keeps
+    // the previous call's contents and is overwritten before use
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, st, 1, method, m, M ? 1 : 0, tol,
+                       prm->rtol);
+    int hmode = kRun;
+    SG_HIP(hipMemcpyAsync(dmode, &hmode, sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_norms, dim3(nb_red), dim3(kT), 0, s, sp[0], d_b, (const double*)nullptr,
+                       W, 0);
+    hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kTF), 0, s, W, sp[0]);
+    SG_HIP(hipGetLastError());
+
+    const Guard step_guard{&st->active, (int)(sizeof(KState) / sizeof(int)), false};
+    const Cycle cyc{ctx, A, M, d_b, d_x, method, m, 1, cgs2, rmode, V, Z, r, z, w, t, ld, W,
+                    sp[0], sp[1], sp[2], nb_red, nb_gd, nb_gu, step_guard, bs_lds, s};
+    KState hst;
+    std::memset((void*)&hst, 0, sizeof(hst));
+    bool verify = false, done = false;
+    int cycles = 0;
+    for (int cycle = 0;; ++cycle) {
+        if (!done && !verify && cycle >= prm->max_iter) {
+            if (rmode && hst.est_conv) verify = true;
+            else done = true;
+        }
+        if (done) break;
+        hmode = verify ? kVerify : kRun;
+        SG_HIP(hipMemcpyAsync(dmode, &hmode, sizeof(int), hipMemcpyHostToDevice, s));
+        cyc.restart();
+        if (hmode == kRun) {
+            // every step is launched; after convergence its kernels return at once
+            for (int j = 0; j < m; ++j) cyc.step(j);
+            cyc.tail();
+        }
+        SG_HIP(hipMemcpyAsync(&hst, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+        SG_HIP(hipStreamSynchronize(s));
+        ++cycles;
+        if (hst.converged || verify) {
+            done = true;
+        } else if (rmode && hst.est_conv) {
+            if (hst.total >= prm->max_iter) verify = true;  // next restart checks
+        } else if (hst.total >= prm->max_iter) {
+            done = true;  // gmres.cpp:146-147
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    sg_solve_result& R = *res;
+    R.iterations = hst.total;
+    R.converged = hst.converged;
+    R.cycles = cycles;
+    R.precond_applies = hst.applies;
+    R.hist_len = hst.total;
+    R.time_solve_s = std::chrono::duration<double>(t1 - t0).count();
+    R.residual_norm = 0.0;  // gmres.cpp:150-155: the last estimate
+    if (hst.total) {
+        SG_HIP(hipMemcpy(&R.residual_norm, W.hist + (hst.total - 1), sizeof(double),
+                         hipMemcpyDeviceToHost));
+        if (hist && hist_cap > 0)
+            SG_HIP(hipMemcpy(hist, W.hist, sizeof(double) * std::min(hst.total, hist_cap),
+                             hipMemcpyDeviceToHost));
+    }
+    const double bn = hst.bnorm;
+    if (rmode && hst.converged) {
+        R.rel_true_residual = hst.rel_true;
+    } else {
+        // FP64 true residual at exit (reported)
+        spmv_launch(A, -1.0, d_x, 1.0, d_b, r, Guard{}, s);
+        hipLaunchKernelGGL(k_norms, dim3(nb_red), dim3(kT), 0, s, sp[0], r,
+                           (const double*)nullptr, W, 0);
+        hipLaunchKernelGGL(k_bnorm_finish, dim3(1), dim3(kTF), 0, s, W, sp[0]);
+        SG_HIP(hipMemcpyAsync(&hst, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+        SG_HIP(hipStreamSynchronize(s));
+        R.rel_true_residual = bn > 0.0 ? hst.bnorm / bn : hst.bnorm;
+    }
+}
+
 }  // namespace sg
 
 using namespace sg;
@@ -1272,5 +1512,14 @@ extern "C" sg_status sg_solve_last_basis(sg_context* ctx, int* n, int* m, int* k
         if (Hbar)
             SG_HIP(hipMemcpy(Hbar, c->b_Hr, sizeof(double) * (size_t)(c->b_m + 1) * c->b_m,
                              hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" sg_status sg_solve_noalloc(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
+                                      const sg_solve_params* prm, sg_solve_result* res,
+                                      double* hist, int hist_cap) {
+    if (!A || !prm || !res || (A->nrows && (!d_b || !d_x))) return SG_E_ARG;
+    return run_guarded(A->ctx, [&] {
+        solve_noalloc_impl(A, M, d_b, d_x, prm, res, hist, hist_cap);
     });
 }

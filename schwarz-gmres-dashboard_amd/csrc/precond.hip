@@ -2981,7 +2981,8 @@ static void sweep_autotune(sg_precond* M, const T* fv, const T* bv, const T* bd,
 
 template <typename T>
 static void tune_sweep(sg_precond* M, const T* fv, const T* bv, const T* bd, T* zg, T* zo) {
-    if (M->ctx->autotune == 1 && !M->sweep_tuned) sweep_autotune<T>(M, fv, bv, bd, zg, zo);
+    if (M->ctx->autotune == 1 && !M->sweep_tuned && !M->ctx->no_tune)
+        sweep_autotune<T>(M, fv, bv, bd, zg, zo);
     if (M->ctx->autotune >= 2) M->sweep_variant = M->ctx->autotune - 2;
     else if (M->ctx->autotune == 0) M->sweep_variant = 1;
 }

@@ -95,6 +95,9 @@ struct Prof {
 // GMRES/FGMRES workspace kept by a context between solves (krylov.hip).
 struct KrylovCache;
 void krylov_cache_free(KrylovCache* c);
+// Looks up this device's copy of the no-allocation solve's static scratch
+// array (krylov.hip); called once per context with the device current.
+void noalloc_init_device(int device);
 
 }  // namespace sg
 
@@ -117,6 +120,7 @@ struct sg_context {
     sg::Prof prof[SG_K_COUNT];
     std::vector<hipEvent_t> event_pool;
     sg::KrylovCache* krylov = nullptr;  // reused by every sg_solve on this context
+    bool no_tune = false;  // set during sg_solve_noalloc: kernel shapes are not timed
 
     hipEvent_t get_event();
     void collect_profiles();  // stream must be synchronised

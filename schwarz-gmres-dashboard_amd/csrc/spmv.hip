@@ -353,7 +353,7 @@ void spmv_autotune(sg_csr* A) {
 
 void spmv_ensure_tuned(sg_csr* A) {
     if (A->nrows == 0) return;
-    if (A->ctx->autotune == 1 && !A->tuned) spmv_autotune(A);
+    if (A->ctx->autotune == 1 && !A->tuned && !A->ctx->no_tune) spmv_autotune(A);
     if (A->ctx->autotune >= 2) {
         const int v = (A->ctx->autotune - 2) % kNumShapes;
         A->variant = shape_available(A, v) ? v : 0;
