@@ -378,6 +378,43 @@ def run_learned(args, sg, P, cfg):
     print(json.dumps(out))
 
 
+def unet_probe(args, sg, P, k):
+    """K forward passes of the learned stage's network on the 1024^2 GRF
+    workload's input (normalised random residual, log kappa) with the shipped
+    weights; prints the mean time per forward and its fraction of the bf16
+    dense MFMA peak (75.2 GFLOP per forward at 1024^2)."""
+    side = 1024
+    g = P.grf(side, 1.5, 0.05, 5)
+    r = P.normal(77, side * side)
+    x = np.stack([r / np.sqrt(np.mean(r * r)), g.ravel()]).astype(np.float32)
+    ctx = sg.Context(0)
+    wpath = args.weights or os.path.join(__graft_entry__.PKG_DIR, "weights", "unet_grf256.npy")
+    net = sg.UNet(ctx, np.load(wpath))
+    dx = sg.DeviceArray.from_host(ctx, x.ravel())
+    dy = sg.DeviceArray(ctx, side * side, np.float32)
+    for _ in range(3):
+        net.forward_dev(1, side, side, dx, dy)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        net.forward_dev(1, side, side, dx, dy)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / k
+    flops = unet_flops(side, side)
+    print(json.dumps({"unet_forward_ms": round(t * 1e3, 4), "forwards": k, "gflop": round(flops / 1e9, 2),
+                      "tflop_s": round(flops / t / 1e12, 1),
+                      "frac_bf16_peak": round(flops / t / 2.5e15, 4)}))
+
+
+def unet_flops(H, W):
+    """Multiply-add flops of one forward: 2 * 9 * cin * cout per pixel of each
+    3x3 convolution at its level (the 1x1 head and pooling are left out)."""
+    spec = [(2, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+            (64, 64, 3), (64, 64, 3), (128, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1),
+            (48, 16, 0), (16, 16, 0)]
+    return sum(2.0 * 9 * ci * co * (H >> lv) * (W >> lv) for ci, co, lv in spec)
+
+
 def _timed(fn, k):
     out, ts = None, []
     for _ in range(k):
@@ -594,6 +631,8 @@ def main():
                     help="with --rows: only these groups (c1: BiCGSTAB / ILUT / analytics on "
                          "Poisson 256^2; c3: BiCGSTAB / analytics on the 160^3 jump case; "
                          "mm: Matrix Market reader); repeatable")
+    ap.add_argument("--unet-probe", type=int, default=0,
+                    help="with --config c5: time this many network forwards at 1024^2 and exit")
     ap.add_argument("--graph", type=int, default=0,
                     help="single systems: 1 = capture the restart cycle as a hipGraph")
     ap.add_argument("--copy-probe", action="store_true",
@@ -619,6 +658,9 @@ def main():
         run_batch(args, sg, P, cfg)
         return
     if args.config == "c5":
+        if args.unet_probe:
+            unet_probe(args, sg, P, args.unet_probe)
+            return
         run_learned(args, sg, P, cfg)
         return
     run_single(args, sg, P, cfg)

@@ -27,10 +27,36 @@ struct Graph {
     }
 };
 
+// Threads this thread may use: the hardware's (at most 16) at the top; the
+// recursive bisection hands half of its share to each half it runs concurrently.
+int hw_threads() {
+    return (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+thread_local int tl_share = 0;
+
+// Runs f(lo, hi) over [0, n) in contiguous chunks on this thread's share of the
+// cores (results must not depend on the chunking).
+void parallel_rows(int n, const std::function<void(int, int)>& f) {
+    const int nt = tl_share > 0 ? tl_share : hw_threads();
+    if (nt == 1 || n < 200000) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+        th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
+
 Graph graph_from_csr(const HostCsr& A, bool weighted) {
     // symmetrise the pattern, drop self loops and duplicates; with weighted,
     // edge (i,j) weighs max(|a_ij|, |a_ji|) in units of the smallest nonzero
-    // off-diagonal magnitude (at least 1)
+    // off-diagonal magnitude (at least 1).  Vertex i's neighbours are the
+    // union of row i and column i of A, ascending, each with the largest
+    // weight of its occurrences: row i (sorted) merged with the transpose's
+    // row i, which a counting pass builds in ascending order.
     int n = A.n;
     double amin = 0.0;
     if (weighted) {
@@ -40,37 +66,88 @@ Graph graph_from_csr(const HostCsr& A, bool weighted) {
                 if (A.ci[k] != i && A.v[k] != 0.0) amin = std::min(amin, std::fabs(A.v[k]));
         if (!(amin < 1e300)) weighted = false;
     }
-    struct E {
-        int a, b;
-        long long w;
-    };
-    std::vector<E> e;
-    e.reserve((size_t)A.nnz() * 2);
+    std::vector<long long> wk;
+    if (weighted) {
+        wk.resize(A.rp[n]);
+        parallel_rows(n, [&](int lo, int hi) {
+            for (int k = A.rp[lo]; k < A.rp[hi]; ++k) {
+                const double q = std::fabs(A.v[k]) / amin;
+                wk[k] = q >= 1e12 ? 1000000000000LL : std::max(1LL, (long long)std::llround(q));
+            }
+        });
+    }
+    auto wt = [&](int k) -> long long { return weighted ? wk[k] : 1; };
+    auto valid = [&](int i, int j) { return j != i && j >= 0 && j < n; };
+    // transpose pattern: tr row j lists every i with a valid entry (i, j), ascending i
+    std::vector<int> tp(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (valid(i, A.ci[k])) tp[A.ci[k] + 1]++;
+    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    std::vector<int> ti(tp[n]), fill(tp.begin(), tp.end() - 1);
+    std::vector<long long> tw(tp[n]);
     for (int i = 0; i < n; ++i)
         for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            int j = A.ci[k];
-            if (j == i || j < 0 || j >= n) continue;
-            long long w = 1;
-            if (weighted) {
-                double q = std::fabs(A.v[k]) / amin;
-                w = q >= 1e12 ? 1000000000000LL : std::max(1LL, (long long)std::llround(q));
-            }
-            e.push_back({i, j, w});
-            e.push_back({j, i, w});
+            const int j = A.ci[k];
+            if (!valid(i, j)) continue;
+            ti[fill[j]] = i;
+            tw[fill[j]++] = wt(k);
         }
-    std::sort(e.begin(), e.end(), [](const E& x, const E& y) {
-        return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.w > y.w);
-    });
+    // row i's own entries, sorted by column with the largest weight first
+    auto own_row = [&](int i, std::vector<std::pair<int, long long>>& r) {
+        r.clear();
+        bool sorted = true;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (valid(i, A.ci[k])) {
+                if (!r.empty() && r.back().first >= A.ci[k]) sorted = false;
+                r.push_back({A.ci[k], wt(k)});
+            }
+        if (sorted) return;  // strictly ascending already: nothing to order or merge
+        std::sort(r.begin(), r.end(), [](const std::pair<int, long long>& x,
+                                         const std::pair<int, long long>& y) {
+            return x.first != y.first ? x.first < y.first : x.second > y.second;
+        });
+    };
+    // merged neighbour list of i (emit(b, w) in ascending b, one per neighbour)
+    auto merge = [&](int i, std::vector<std::pair<int, long long>>& r, auto&& emit) {
+        own_row(i, r);
+        size_t a = 0;
+        int b = tp[i];
+        const int be = tp[i + 1];
+        while (a < r.size() || b < be) {
+            int c;
+            long long w = -1;
+            if (b >= be || (a < r.size() && r[a].first <= ti[b])) c = r[a].first;
+            else c = ti[b];
+            while (a < r.size() && r[a].first == c) w = std::max(w, r[a++].second);
+            while (b < be && ti[b] == c) w = std::max(w, tw[b++]);
+            emit(c, w);
+        }
+    };
     Graph g;
     g.nv = n;
     g.xadj.assign(n + 1, 0);
-    for (size_t k = 0; k < e.size(); ++k) {
-        if (k > 0 && e[k].a == e[k - 1].a && e[k].b == e[k - 1].b) continue;  // keep the max weight
-        g.adj.push_back(e[k].b);
-        g.ew.push_back(e[k].w);
-        g.xadj[e[k].a + 1]++;
-    }
+    parallel_rows(n, [&](int lo, int hi) {
+        std::vector<std::pair<int, long long>> r;
+        for (int i = lo; i < hi; ++i) {
+            int cnt = 0;
+            merge(i, r, [&](int, long long) { ++cnt; });
+            g.xadj[i + 1] = cnt;
+        }
+    });
     for (int i = 0; i < n; ++i) g.xadj[i + 1] += g.xadj[i];
+    g.adj.resize(g.xadj[n]);
+    g.ew.resize(g.xadj[n]);
+    parallel_rows(n, [&](int lo, int hi) {
+        std::vector<std::pair<int, long long>> r;
+        for (int i = lo; i < hi; ++i) {
+            int pos = g.xadj[i];
+            merge(i, r, [&](int c, long long w) {
+                g.adj[pos] = c;
+                g.ew[pos++] = w;
+            });
+        }
+    });
     g.vw.assign(n, 1);
     return g;
 }
@@ -80,11 +157,13 @@ bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
     int n = g.nv;
     std::vector<int> match(n, -1);
     // visit in order of increasing degree (ties by index): deterministic
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return (g.xadj[a + 1] - g.xadj[a]) < (g.xadj[b + 1] - g.xadj[b]);
-    });
+    // (a stable counting sort on the degree: ties keep index order)
+    int maxdeg = 0;
+    for (int v = 0; v < n; ++v) maxdeg = std::max(maxdeg, g.xadj[v + 1] - g.xadj[v]);
+    std::vector<int> start(maxdeg + 2, 0), order(n);
+    for (int v = 0; v < n; ++v) start[g.xadj[v + 1] - g.xadj[v] + 1]++;
+    for (int d = 0; d <= maxdeg; ++d) start[d + 1] += start[d];
+    for (int v = 0; v < n; ++v) order[start[g.xadj[v + 1] - g.xadj[v]]++] = v;
     long long maxvw = std::max<long long>(1, g.total_vw() / 100);
     for (int v : order) {
         if (match[v] >= 0) continue;
@@ -117,12 +196,15 @@ bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
         int cv = cmap[v];
         if (first[cv] < 0) first[cv] = v; else second[cv] = v;
     }
+    // coarse adjacency: the members' neighbours in order of first appearance,
+    // weights summed
     c.xadj.assign(nc + 1, 0);
     c.adj.clear();
     c.ew.clear();
+    c.adj.reserve(g.adj.size() / 2);
+    c.ew.reserve(g.adj.size() / 2);
     std::vector<int> mark(nc, -1), slot(nc, 0);
     for (int cv = 0; cv < nc; ++cv) {
-        int start = (int)c.adj.size();
         for (int m = 0; m < 2; ++m) {
             int v = m == 0 ? first[cv] : second[cv];
             if (v < 0) continue;
@@ -139,7 +221,6 @@ bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
                 }
             }
         }
-        (void)start;
         c.xadj[cv + 1] = (int)c.adj.size();
     }
     return true;
@@ -172,14 +253,19 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
             if (side[g.adj[k]] != side[v]) return true;
         return false;
     };
+    std::vector<char> bnd(n);
     for (int pass = 0; pass < passes; ++pass) {
         std::set<std::pair<long long, int>> q[2];  // (-gain, v)
         std::fill(locked.begin(), locked.end(), 0);
-        for (int v = 0; v < n; ++v) {
-            if (!boundary(v)) continue;
-            gain[v] = compute_gain(v);
-            q[side[v]].insert({-gain[v], v});
-        }
+        // boundary flags and gains on all threads, then the queues in index order
+        parallel_rows(n, [&](int lo, int hi) {
+            for (int v = lo; v < hi; ++v) {
+                bnd[v] = boundary(v);
+                if (bnd[v]) gain[v] = compute_gain(v);
+            }
+        });
+        for (int v = 0; v < n; ++v)
+            if (bnd[v]) q[side[v]].insert({-gain[v], v});
         std::vector<int> moved;
         long long cur = 0, best = 0;
         size_t best_len = 0;
@@ -320,15 +406,19 @@ std::vector<int> initial_bisect(const Graph& g, long long target, long long lo, 
 std::vector<int> bisect(const Graph& g, double frac, double eps) {
     long long tot = g.total_vw();
     long long target = (long long)std::llround(frac * (double)tot);
-    std::vector<Graph> levels;
+    // level 0 is g itself (not copied); coarser levels are owned here
+    std::vector<Graph> coarse;
     std::vector<std::vector<int>> maps;
-    levels.push_back(g);
-    while (levels.back().nv > 160) {
+    coarse.reserve(64);
+    auto level = [&](int l) -> const Graph& { return l == 0 ? g : coarse[l - 1]; };
+    int nlev = 1;
+    while (level(nlev - 1).nv > 160) {
         Graph c;
         std::vector<int> cm;
-        if (!coarsen(levels.back(), c, cm)) break;
+        if (!coarsen(level(nlev - 1), c, cm)) break;
         maps.push_back(std::move(cm));
-        levels.push_back(std::move(c));
+        coarse.push_back(std::move(c));
+        ++nlev;
     }
     // side s may exceed its target share t_s by eps * t_s (and by one coarse
     // vertex while the graph is coarse)
@@ -340,33 +430,48 @@ std::vector<int> bisect(const Graph& g, double frac, double eps) {
         long long up1 = std::max<long long>((long long)(eps * t1), maxv);
         return std::make_pair(target - up1, target + up0);
     };
-    auto w = window(levels.back());
-    std::vector<int> side = initial_bisect(levels.back(), target, w.first, w.second);
-    for (int l = (int)levels.size() - 2; l >= 0; --l) {
-        std::vector<int> fine(levels[l].nv);
-        for (int v = 0; v < levels[l].nv; ++v) fine[v] = side[maps[l][v]];
+    auto w = window(level(nlev - 1));
+    std::vector<int> side = initial_bisect(level(nlev - 1), target, w.first, w.second);
+    for (int l = nlev - 2; l >= 0; --l) {
+        const Graph& gl = level(l);
+        std::vector<int> fine(gl.nv);
+        for (int v = 0; v < gl.nv; ++v) fine[v] = side[maps[l][v]];
         side.swap(fine);
-        auto wl = window(levels[l]);
-        fm_refine(levels[l], side, wl.first, wl.second, l == 0 ? 6 : 3);
+        auto wl = window(gl);
+        fm_refine(gl, side, wl.first, wl.second, l == 0 ? 6 : 3);
     }
     return side;
 }
 
 Graph induced(const Graph& g, const std::vector<int>& verts, std::vector<int>& local) {
-    // verts ascending; local[v] = new id or -1
+    // verts ascending; local[v] = new id or -1; rows counted, then filled, in parallel
     Graph s;
     s.nv = (int)verts.size();
     s.xadj.assign(s.nv + 1, 0);
     s.vw.resize(s.nv);
-    for (int i = 0; i < s.nv; ++i) {
-        int v = verts[i];
-        s.vw[i] = g.vw[v];
-        for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
-            int u = local[g.adj[k]];
-            if (u >= 0) { s.adj.push_back(u); s.ew.push_back(g.ew[k]); }
+    parallel_rows(s.nv, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            int v = verts[i], cnt = 0;
+            s.vw[i] = g.vw[v];
+            for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) cnt += local[g.adj[k]] >= 0;
+            s.xadj[i + 1] = cnt;
         }
-        s.xadj[i + 1] = (int)s.adj.size();
-    }
+    });
+    for (int i = 0; i < s.nv; ++i) s.xadj[i + 1] += s.xadj[i];
+    s.adj.resize(s.xadj[s.nv]);
+    s.ew.resize(s.xadj[s.nv]);
+    parallel_rows(s.nv, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            int v = verts[i], pos = s.xadj[i];
+            for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+                int u = local[g.adj[k]];
+                if (u >= 0) {
+                    s.adj[pos] = u;
+                    s.ew[pos++] = g.ew[k];
+                }
+            }
+        }
+    });
     return s;
 }
 
@@ -389,8 +494,16 @@ void recurse(const Graph& g, const std::vector<int>& ids, int nparts, int first,
     for (size_t i = 0; i < v1.size(); ++i) ids1[i] = ids[v1[i]];
     // the two halves are independent: recurse concurrently near the top
     if (depth < 3 && g.nv > 20000) {
-        std::thread t([&] { recurse(g0, ids0, k0, first, eps, part, depth + 1); });
+        const int share = tl_share > 0 ? tl_share : hw_threads();
+        const int half = std::max(1, share / 2);
+        std::thread t([&, half] {
+            tl_share = half;
+            recurse(g0, ids0, k0, first, eps, part, depth + 1);
+        });
+        const int mine = tl_share;
+        tl_share = std::max(1, share - half);
         recurse(g1, ids1, k1, first + k0, eps, part, depth + 1);
+        tl_share = mine;
         t.join();
     } else {
         recurse(g0, ids0, k0, first, eps, part, depth + 1);
