@@ -4,16 +4,20 @@
 // upsampling with skip concatenation, 1x1 head) and the parameter packing are
 // those of the CPU oracle that the tests compare against.
 //
-// Storage: activations and activation gradients in bfloat16, NCHW with the
-// batch outermost; weights as an fp32 master copy plus bfloat16 copies laid out
-// for the kernels; every sum in fp32.
+// Storage: activations and activation gradients in bfloat16, NHWC (the
+// channels of a pixel contiguous, so a 16-byte load carries 8 channels); a
+// decoder's concatenation buffer holds the upsampled and the skip channels of
+// each pixel side by side.  Weights as an fp32 master copy plus bfloat16
+// copies laid out for the kernels; every sum in fp32.
 //
 // 3x3 convolutions (forward, and the data gradient as a convolution with the
-// flipped, transposed weights) are implicit GEMMs on the matrix cores:
-// v_mfma_f32_16x16x16bf16_1k with A = weights (16 output channels x 16 input
-// channels of one tap) and B = input (16 input channels x 16 pixels of one
-// image row), both staged in LDS per 32-channel chunk.  A workgroup computes an
-// 8 x 16 pixel tile for up to 64 output channels; each wavefront two rows.
+// flipped, transposed weights) are implicit GEMMs on the matrix cores with
+// v_mfma_f32_16x16x32_bf16: A = weights (16 output channels x 32 of K), B =
+// input (32 of K x 16 pixels of one image row), K running over (tap, input
+// channel) of a staged chunk.  A workgroup computes an 8 x 32 pixel tile for
+// up to 64 output channels (each wavefront two rows: 2 x 2 pixel blocks x 4
+// channel blocks of accumulators); input chunks of 32 channels (with the
+// one-pixel halo) and their weights are staged in LDS with 16-byte loads.
 // The weight gradient runs on the vector ALUs in fp32 (each thread one
 // (output, input) channel pair, a 3x3 window slid along the tile rows) and
 // writes per-tile partial sums that a second kernel adds in a fixed order.
@@ -24,7 +28,6 @@
 
 namespace sg {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -58,111 +61,152 @@ inline int uconv_offset(int i) {
 inline int unet_nparams_() { return uconv_offset(kUConvs) + 17; }
 inline int cin_pad(int c) { return (c + 15) / 16 * 16; }
 
-// A channel window [c0, c0 + C) of a [B][ctot][H][W] bfloat16 tensor.
+// A channel window [c0, c0 + C) of a [B][H][W][ctot] bfloat16 tensor (NHWC;
+// ctot and c0 are multiples of 8).
 struct View {
     uint16_t* p;
     int ctot, c0;
     __device__ size_t at(int b, int c, int y, int x, int H, int W) const {
-        return (((size_t)b * ctot + c0 + c) * H + y) * W + x;
+        return (((size_t)b * H + y) * W + x) * ctot + c0 + c;
     }
 };
 
 // ---- 3x3 convolution on the matrix cores ------------------------------------
-constexpr int kCX = 16, kCY = 8, kCIC = 32;
-constexpr int kPS = kCIC + 4;  // bf16 per staged pixel / weight row (bank spread)
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int kTX = 32, kTY = 8;          // output tile: 8 rows x 32 pixels
+constexpr int kCC = 32;                   // input channels per staged chunk
+constexpr int kHX = kTX + 2, kHY = kTY + 2;
+
+__device__ __forceinline__ v8bf lds_v8(const uint16_t* p) {
+    return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(p));
+}
+
+// LDS of one workgroup: the staged input chunk (pixel stride sp = C + 8) and
+// the chunk's weights for cob output channels (row stride swp).
+struct ConvLds {
+    int sp, swp, bytes;
+};
+__host__ __device__ inline ConvLds conv_lds(int cinp, int cob) {
+    const int C = cinp < kCC ? cinp : kCC;
+    const int sp = C + 8, swp = (9 * C + 31) / 32 * 32 + 8;
+    return ConvLds{sp, swp, 2 * (kHY * kHX * sp + cob * swp)};
+}
 
 // MODE 0: out = relu(acc + bias); MODE 1: out = acc (data gradient).
-// Weights wt: [9][cout][cinp] bfloat16 (cinp = cin rounded up to 16).
-// Grid: x = B * tilesY * tilesX, y = output-channel blocks of 64.
+// Weights wt: [9][cout][cinp] bfloat16 (cinp = cin rounded up to 16; the
+// input view holds at least cinp channels, zero beyond cin).
+// Grid: x = B * tilesY * tilesX, y = output-channel blocks of cob (16, 32 or
+// 64); dynamic LDS conv_lds(cinp, cob).bytes.
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
-          const float* __restrict__ bias, View out, int cout, int H, int W) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_in[(kCY + 2) * (kCX + 2) * kPS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * 64 * kPS];
-    const int tilesX = (W + kCX - 1) / kCX, tilesY = (H + kCY - 1) / kCY;
+          const float* __restrict__ bias, View out, int cout, int H, int W, int cob) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
+    const ConvLds L = conv_lds(cinp, cob);
+    const int kSP = L.sp, kSWP = L.swp;
+    uint16_t* s_in = lds_u16;
+    uint16_t* s_w = lds_u16 + kHY * kHX * kSP;
+    const int tilesX = (W + kTX - 1) / kTX, tilesY = (H + kTY - 1) / kTY;
     int t = blockIdx.x;
     const int tx = t % tilesX;
     t /= tilesX;
     const int ty = t % tilesY;
     const int b = t / tilesY;
-    const int x0 = tx * kCX, y0 = ty * kCY;
-    const int co0 = blockIdx.y * 64;
-    const int ncot = min(4, (cout - co0) / 16);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v4f acc[2][4];
+    const int x0 = tx * kTX, y0 = ty * kTY;
+    const int co0 = blockIdx.y * cob;
+    const int ncot = min(cob, cout - co0) / 16;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    v4f acc[2][2][4];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int n = 0; n < 4; ++n) acc[r][n] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int c0 = 0; c0 < cinp; c0 += kCIC) {
-        const int cc = min(kCIC, cinp - c0);
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) acc[r][p][n] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < cinp; c0 += kCC) {
+        const int C = min(kCC, cinp - c0);  // 16 or 32
+        const int oct = C / 8;
+        const int KC = (9 * C + 31) / 32 * 32;
         __syncthreads();
-        for (int e = threadIdx.x; e < cc * (kCY + 2) * (kCX + 2); e += 256) {
-            const int px = e % (kCX + 2);
-            const int r = e / (kCX + 2);
-            const int py = r % (kCY + 2);
-            const int ci = r / (kCY + 2);
+        // input chunk with its halo: 16-byte loads of 8 channels
+        for (int e = tid; e < kHY * kHX * oct; e += 256) {
+            const int q = e % oct, pix = e / oct;
+            const int py = pix / kHX, px = pix % kHX;
             const int gy = y0 + py - 1, gx = x0 + px - 1;
-            uint16_t v = 0;
-            if (c0 + ci < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = in.p[in.at(b, c0 + ci, gy, gx, H, W)];
-            s_in[(py * (kCX + 2) + px) * kPS + ci] = v;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (c0 + 8 * q < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
+            *reinterpret_cast<uint4*>(&s_in[pix * kSP + 8 * q]) = v;
         }
-        const int q4 = cc / 4;
-        for (int e = threadIdx.x; e < 9 * ncot * 16 * q4; e += 256) {
-            const int c4 = e % q4;
-            const int r = e / q4;
-            const int co = r % (ncot * 16);
-            const int tap = r / (ncot * 16);
-            const uint2 v = *reinterpret_cast<const uint2*>(
-                wt + ((size_t)tap * cout + co0 + co) * cinp + c0 + 4 * c4);
-            *reinterpret_cast<uint2*>(&s_w[(tap * 64 + co) * kPS + 4 * c4]) = v;
+        // weights of the chunk: s_w[co][tap * C + ch]; K beyond 9 C is zero
+        for (int e = tid; e < ncot * 16 * 9 * oct; e += 256) {
+            const int q = e % oct, r = e / oct;
+            const int tap = r % 9, co = r / 9;
+            const uint4 v = *reinterpret_cast<const uint4*>(
+                wt + ((size_t)tap * cout + co0 + co) * cinp + c0 + 8 * q);
+            *reinterpret_cast<uint4*>(&s_w[co * kSWP + tap * C + 8 * q]) = v;
+        }
+        for (int e = tid; e < ncot * 16 * ((KC - 9 * C) / 8); e += 256) {
+            const int per = (KC - 9 * C) / 8;
+            *reinterpret_cast<uint4*>(&s_w[(e / per) * kSWP + 9 * C + 8 * (e % per)]) =
+                uint4{0u, 0u, 0u, 0u};
         }
         __syncthreads();
-        for (int tap = 0; tap < 9; ++tap) {
-            const int ky = tap / 3, kx = tap % 3;
-            for (int kb = 0; kb < cc; kb += 16) {
-                const int kofs = kb + 4 * (lane >> 4);
-                v4s bfr[2];
+        for (int ks = 0; ks < KC; ks += 32) {
+            // this lane's 8 K values: one tap, channels ch..ch+7 (C is a multiple of 8)
+            const int k = ks + 8 * g;
+            const int tap = k / C, ch = k - tap * C;
+            const bool live = tap < 9;  // the zero-padded tail of K
+            const int tc = live ? tap : 8;
+            const int ky = tc / 3, kx = tc % 3;
+            v8bf bfr[2][2];
 #pragma unroll
-                for (int r = 0; r < 2; ++r) {
-                    const int py = 2 * wv + r + ky, px = (lane & 15) + kx;
-                    bfr[r] = *reinterpret_cast<const v4s*>(&s_in[(py * (kCX + 2) + px) * kPS + kofs]);
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const int py = 2 * wv + r + ky, px = 16 * p + li + kx;
+                    const v8bf v = lds_v8(&s_in[(py * kHX + px) * kSP + (live ? ch : 0)]);
+                    bfr[r][p] = live ? v : v8bf{};
                 }
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    if (n >= ncot) break;
-                    const v4s afr = *reinterpret_cast<const v4s*>(
-                        &s_w[(tap * 64 + n * 16 + (lane & 15)) * kPS + kofs]);
+            for (int n = 0; n < 4; ++n) {
+                if (n >= ncot) break;
+                const v8bf afr = lds_v8(&s_w[(n * 16 + li) * kSWP + k]);
 #pragma unroll
-                    for (int r = 0; r < 2; ++r)
-                        acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[r], acc[r][n],
-                                                                             0, 0, 0);
-                }
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+                        acc[r][p][n] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[r][p], acc[r][p][n], 0, 0, 0);
             }
         }
     }
-    // D[co][pixel]: lane holds pixel (lane & 15), channels 4*(lane >> 4) + i
+    // D[co][pixel]: lane holds pixel li, channels 4 g + i of each block
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        const int y = y0 + 2 * wv + r, x = x0 + (lane & 15);
-        if (y >= H || x >= W) continue;
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            if (n >= ncot) break;
+        for (int p = 0; p < 2; ++p) {
+            const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
+            if (y >= H || x >= W) continue;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int co = co0 + n * 16 + 4 * (lane >> 4) + i;
-                float v = acc[r][n][i];
-                if (MODE == 0) {
-                    v += bias[co];
-                    v = v > 0.f ? v : 0.f;
+            for (int n = 0; n < 4; ++n) {
+                if (n >= ncot) break;
+                const int co = co0 + n * 16 + 4 * g;
+                uint16_t h[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float v = acc[r][p][n][i];
+                    if (MODE == 0) {
+                        v += bias[co + i];
+                        v = v > 0.f ? v : 0.f;
+                    }
+                    h[i] = f2bf(v);
                 }
-                out.p[out.at(b, co, y, x, H, W)] = f2bf(v);
+                *reinterpret_cast<uint2*>(out.p + out.at(b, co, y, x, H, W)) =
+                    uint2{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16)};
             }
         }
-    }
 }
 
 // ---- weight gradient (fp32 vector ALUs) ---------------------------------------
@@ -185,12 +229,12 @@ k_conv_dw(View in, int cin, View dp, int cout, int B, int H, int W, float* __res
     const int x0 = tx * kWX, y0 = ty * kWY;
     const int cob = blockIdx.y * 16, cib = blockIdx.z * 16;
     for (int e = threadIdx.x; e < 16 * kWY * kWX; e += 256) {
-        const int x = e % kWX, y = (e / kWX) % kWY, c = e / (kWX * kWY);
+        const int c = e % 16, x = (e / 16) % kWX, y = e / (16 * kWX);
         const int gy = y0 + y, gx = x0 + x;
         s_g[c][y][x] = (gy < H && gx < W) ? bf2f(dp.p[dp.at(b, cob + c, gy, gx, H, W)]) : 0.f;
     }
     for (int e = threadIdx.x; e < 16 * (kWY + 2) * (kWX + 2); e += 256) {
-        const int x = e % (kWX + 2), y = (e / (kWX + 2)) % (kWY + 2), c = e / ((kWX + 2) * (kWY + 2));
+        const int c = e % 16, x = (e / 16) % (kWX + 2), y = e / (16 * (kWX + 2));
         const int gy = y0 + y - 1, gx = x0 + x - 1;
         float v = 0.f;
         if (cib + c < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
@@ -255,26 +299,43 @@ __global__ void k_dw_reduce(int ntile, int cin, int cout, const float* __restric
 }
 
 // ---- elementwise / layout kernels ---------------------------------------------
-// network input: [B][2][H][W] fp32 -> bf16 [B][16][H][W] (channels 2..15 zero)
+// network input: [B][2][H][W] fp32 -> bf16 [B][H][W][16] (channels 2..15
+// zero), one pixel (two 16-byte words) per thread
 __global__ void k_input(int B, int H, int W, const float* __restrict__ in, View x) {
-    const size_t hw = (size_t)H * W, n = (size_t)B * 16 * hw;
+    const size_t hw = (size_t)H * W, n = (size_t)B * hw;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const size_t p = e % hw;
-        const int c = (int)((e / hw) % 16), b = (int)(e / (16 * hw));
-        x.p[e] = c < 2 ? f2bf(in[((size_t)b * 2 + c) * hw + p]) : (uint16_t)0;
+        const int b = (int)(e / hw);
+        const unsigned c0 = f2bf(in[(size_t)b * 2 * hw + p]), c1 = f2bf(in[((size_t)b * 2 + 1) * hw + p]);
+        uint4* o = reinterpret_cast<uint4*>(x.p + e * 16);
+        o[0] = uint4{c0 | (c1 << 16), 0u, 0u, 0u};
+        o[1] = uint4{0u, 0u, 0u, 0u};
     }
 }
 
+// 2x2 average pooling, eight channels (one 16-byte word) per thread
 __global__ void k_pool(int B, int C, int H, int W, View in, View out) {
-    const int h = H / 2, w = W / 2;
-    const size_t n = (size_t)B * C * h * w;
+    const int h = H / 2, w = W / 2, oc = C / 8;
+    const size_t n = (size_t)B * oc * h * w;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(e % w), y = (int)((e / w) % h), c = (int)((e / ((size_t)w * h)) % C);
-        const int b = (int)(e / ((size_t)w * h * C));
-        const float s = bf2f(in.p[in.at(b, c, 2 * y, 2 * x, H, W)]) + bf2f(in.p[in.at(b, c, 2 * y, 2 * x + 1, H, W)]) +
-                        bf2f(in.p[in.at(b, c, 2 * y + 1, 2 * x, H, W)]) +
-                        bf2f(in.p[in.at(b, c, 2 * y + 1, 2 * x + 1, H, W)]);
-        out.p[out.at(b, c, y, x, h, w)] = f2bf(0.25f * s);
+        const int q = (int)(e % oc), x = (int)((e / oc) % w), y = (int)((e / ((size_t)oc * w)) % h);
+        const int b = (int)(e / ((size_t)w * h * oc));
+        const uint4 a = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y, 2 * x, H, W));
+        const uint4 c = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y, 2 * x + 1, H, W));
+        const uint4 d = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y + 1, 2 * x, H, W));
+        const uint4 f = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y + 1, 2 * x + 1, H, W));
+        const uint16_t* pa = reinterpret_cast<const uint16_t*>(&a);
+        const uint16_t* pc = reinterpret_cast<const uint16_t*>(&c);
+        const uint16_t* pd = reinterpret_cast<const uint16_t*>(&d);
+        const uint16_t* pf = reinterpret_cast<const uint16_t*>(&f);
+        uint4 o;
+        uint16_t* po = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float sum = bf2f(pa[k]) + bf2f(pc[k]) + bf2f(pd[k]) + bf2f(pf[k]);
+            po[k] = f2bf(0.25f * sum);
+        }
+        *reinterpret_cast<uint4*>(out.p + out.at(b, 8 * q, y, x, h, w)) = o;
     }
 }
 
@@ -282,20 +343,23 @@ __global__ void k_pool(int B, int C, int H, int W, View in, View out) {
 __global__ void k_pool_bwd(int B, int C, int H, int W, View dy, View dx) {
     const size_t n = (size_t)B * C * H * W;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(e % W), y = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
+        const int c = (int)(e % C), x = (int)((e / C) % W), y = (int)((e / ((size_t)C * W)) % H);
         const int b = (int)(e / ((size_t)W * H * C));
         const size_t i = dx.at(b, c, y, x, H, W);
         dx.p[i] = f2bf(bf2f(dx.p[i]) + 0.25f * bf2f(dy.p[dy.at(b, c, y / 2, x / 2, H / 2, W / 2)]));
     }
 }
 
-// out channels [0, C) at H x W = nearest upsampling of a (H/2 x W/2)
+// out channels [0, C) at H x W = nearest upsampling of a (H/2 x W/2), eight
+// channels (one 16-byte word) per thread
 __global__ void k_up(int B, int C, int H, int W, View a, View out) {
-    const size_t n = (size_t)B * C * H * W;
+    const int oc = C / 8;
+    const size_t n = (size_t)B * oc * H * W;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(e % W), y = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
-        const int b = (int)(e / ((size_t)W * H * C));
-        out.p[out.at(b, c, y, x, H, W)] = a.p[a.at(b, c, y / 2, x / 2, H / 2, W / 2)];
+        const int q = (int)(e % oc), x = (int)((e / oc) % W), y = (int)((e / ((size_t)oc * W)) % H);
+        const int b = (int)(e / ((size_t)W * H * oc));
+        *reinterpret_cast<uint4*>(out.p + out.at(b, 8 * q, y, x, H, W)) =
+            *reinterpret_cast<const uint4*>(a.p + a.at(b, 8 * q, y / 2, x / 2, H / 2, W / 2));
     }
 }
 
@@ -304,7 +368,7 @@ __global__ void k_up_bwd(int B, int C, int H, int W, View dy, View da) {
     const int h = H / 2, w = W / 2;
     const size_t n = (size_t)B * C * h * w;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(e % w), y = (int)((e / w) % h), c = (int)((e / ((size_t)w * h)) % C);
+        const int c = (int)(e % C), x = (int)((e / C) % w), y = (int)((e / ((size_t)C * w)) % h);
         const int b = (int)(e / ((size_t)w * h * C));
         // row-major order within the block, as the oracle accumulates
         float s = bf2f(dy.p[dy.at(b, c, 2 * y, 2 * x, H, W)]);
@@ -319,7 +383,7 @@ __global__ void k_up_bwd(int B, int C, int H, int W, View dy, View da) {
 __global__ void k_relu_mask(int B, int C, int H, int W, View y, View dy) {
     const size_t n = (size_t)B * C * H * W;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(e % W), yy = (int)((e / W) % H), c = (int)((e / ((size_t)W * H)) % C);
+        const int c = (int)(e % C), x = (int)((e / C) % W), yy = (int)((e / ((size_t)C * W)) % H);
         const int b = (int)(e / ((size_t)W * H * C));
         if (!(bf2f(y.p[y.at(b, c, yy, x, H, W)]) > 0.f)) dy.p[dy.at(b, c, yy, x, H, W)] = 0;
     }
@@ -332,7 +396,8 @@ __global__ void k_head(int B, int H, int W, View d0, const float* __restrict__ h
         const size_t p = e % hwn;
         const int b = (int)(e / hwn);
         float s = hw[16];
-        for (int c = 0; c < 16; ++c) s += hw[c] * bf2f(d0.p[((size_t)b * d0.ctot + d0.c0 + c) * hwn + p]);
+        const uint16_t* px = d0.p + (((size_t)b * hwn + p) * d0.ctot + d0.c0);
+        for (int c = 0; c < 16; ++c) s += hw[c] * bf2f(px[c]);
         out[e] = s;
     }
 }
@@ -497,10 +562,11 @@ k_head_bwd(int B, int H, int W, const float* __restrict__ dout, View d0,
         const int b = (int)(e / hwn);
         const float g = dout[e];
         acc[16] += g;
+        const uint16_t* px = d0.p + (((size_t)b * hwn + p) * d0.ctot + d0.c0);
+        uint16_t* qx = dd0.p + (((size_t)b * hwn + p) * dd0.ctot + dd0.c0);
         for (int c = 0; c < 16; ++c) {
-            const size_t i = ((size_t)b * d0.ctot + d0.c0 + c) * hwn + p;
-            acc[c] += g * bf2f(d0.p[i]);
-            dd0.p[((size_t)b * dd0.ctot + dd0.c0 + c) * hwn + p] = f2bf(g * hw[c]);
+            acc[c] += g * bf2f(px[c]);
+            qx[c] = f2bf(g * hw[c]);
         }
     }
     __shared__ float red[256];
@@ -652,19 +718,30 @@ static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
 
 static View V(DevBuf<uint16_t>& d, int ctot, int c0 = 0) { return View{d.p, ctot, c0}; }
 
+// Output channels per workgroup: 64, halved (down to 16) while the grid has
+// fewer than 2048 workgroups, so the small deep layers still fill the CUs.
+static int conv_cob(int tiles, int cout) {
+    int cob = cout >= 64 ? 64 : (cout >= 32 ? 32 : 16);
+    while (cob > 16 && (long long)tiles * ((cout + cob - 1) / cob) < 2048) cob /= 2;
+    return cob;
+}
+
 static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
-    const int tiles = U->B * ((H + kCY - 1) / kCY) * ((W + kCX - 1) / kCX);
-    hipLaunchKernelGGL(k_conv3x3<0>, dim3(tiles, (co + 63) / 64), dim3(256), 0, U->ctx->stream, in,
-                       ci, cin_pad(ci), U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co,
-                       H, W);
+    const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
+    const int cob = conv_cob(tiles, co);
+    hipLaunchKernelGGL(k_conv3x3<0>, dim3(tiles, (co + cob - 1) / cob), dim3(256),
+                       conv_lds(cin_pad(ci), cob).bytes, U->ctx->stream, in, ci, cin_pad(ci),
+                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob);
 }
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout;  // dx has ci channels
-    const int tiles = U->B * ((H + kCY - 1) / kCY) * ((W + kCX - 1) / kCX);
-    hipLaunchKernelGGL(k_conv3x3<1>, dim3(tiles, (ci + 63) / 64), dim3(256), 0, U->ctx->stream, dp,
-                       co, co, U->wb[i].p, (const float*)nullptr, dx, ci, H, W);
+    const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
+    const int cob = conv_cob(tiles, ci);
+    hipLaunchKernelGGL(k_conv3x3<1>, dim3(tiles, (ci + cob - 1) / cob), dim3(256),
+                       conv_lds(co, cob).bytes, U->ctx->stream, dp, co, co, U->wb[i].p,
+                       (const float*)nullptr, dx, ci, H, W, cob);
 }
 
 static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
@@ -683,30 +760,30 @@ void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool g
     hipStream_t s = U->ctx->stream;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     const size_t hw0 = (size_t)H * W;
-    hipLaunchKernelGGL(k_input, dim3(grid1(B * 16 * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16));
+    hipLaunchKernelGGL(k_input, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16));
     conv_fwd(U, 0, V(U->x, 16), V(U->a0, 16), H, W);
     conv_fwd(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W);  // s0
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 16 * hw0 / 4)), dim3(256), 0, s, B, 16, H, W, V(U->cat0, 48, 32),
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 2 * hw0 / 4)), dim3(256), 0, s, B, 16, H, W, V(U->cat0, 48, 32),
                        V(U->p0, 16));
     conv_fwd(U, 2, V(U->p0, 16), V(U->a2, 32), H1, W1);
     conv_fwd(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1);  // s1
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 32 * hw0 / 16)), dim3(256), 0, s, B, 32, H1, W1,
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 4 * hw0 / 16)), dim3(256), 0, s, B, 32, H1, W1,
                        V(U->cat1, 96, 64), V(U->p1, 32));
     conv_fwd(U, 4, V(U->p1, 32), V(U->a4, 64), H2, W2);
     conv_fwd(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2);  // s2
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 64 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
+    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 8 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
                        V(U->cat2, 128, 64), V(U->p2, 64));
     conv_fwd(U, 6, V(U->p2, 64), V(U->a6, 64), H3, W3);
     conv_fwd(U, 7, V(U->a6, 64), V(U->a7, 64), H3, W3);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 64 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2, V(U->a7, 64),
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2, V(U->a7, 64),
                        V(U->cat2, 128, 0));
     conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2);
     conv_fwd(U, 9, V(U->a8, 64), V(U->a9, 64), H2, W2);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 64 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1, V(U->a9, 64),
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1, V(U->a9, 64),
                        V(U->cat1, 96, 0));
     conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1);
     conv_fwd(U, 11, V(U->a10, 32), V(U->a11, 32), H1, W1);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 32 * hw0)), dim3(256), 0, s, B, 32, H, W, V(U->a11, 32),
+    hipLaunchKernelGGL(k_up, dim3(grid1(B * 4 * hw0)), dim3(256), 0, s, B, 32, H, W, V(U->a11, 32),
                        V(U->cat0, 48, 0));
     conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W);
     conv_fwd(U, 13, V(U->a12, 16), V(U->a13, 16), H, W);

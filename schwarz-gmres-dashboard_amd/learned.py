@@ -82,6 +82,7 @@ def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10,
         b = np.concatenate(bs)
         db = sg.DeviceArray.from_host(ctx, b)
         dx = sg.DeviceArray(ctx, B.n)
+        dx.zero()  # x0 = 0: device allocations are not initialised
         dr = sg.DeviceArray(ctx, B.n)
         dz = sg.DeviceArray(ctx, B.n)
         xs, rs = [], []

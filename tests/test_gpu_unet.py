@@ -29,7 +29,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8), (2, 32, 16), (1, 24, 40)])
+@pytest.mark.parametrize("shape", [(1, 8, 8), (2, 32, 16), (1, 24, 40), (1, 128, 96)])
 def test_forward_matches_oracle(sg, ctx, shape):
     B, H, W = shape
     params = init_params(11)
