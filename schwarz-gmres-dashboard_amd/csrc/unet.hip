@@ -129,23 +129,47 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
         const int oct = C / 8;
         const int KC = (9 * C + 31) / 32 * 32;
         __syncthreads();
-        // input chunk with its halo: 16-byte loads of 8 channels
-        for (int e = tid; e < kHY * kHX * oct; e += 256) {
-            const int q = e % oct, pix = e / oct;
-            const int py = pix / kHX, px = pix % kHX;
-            const int gy = y0 + py - 1, gx = x0 + px - 1;
+        // input chunk with its halo and the chunk's weights: every 16-byte
+        // load of the thread is issued before the first LDS store
+        constexpr int kJI = (kHY * kHX * (kCC / 8) + 255) / 256;
+        constexpr int kJW = (64 * 9 * (kCC / 8) + 255) / 256;
+        uint4 ri[kJI], rw[kJW];
+#pragma unroll
+        for (int jj = 0; jj < kJI; ++jj) {
+            const int e = tid + 256 * jj;
             uint4 v = {0u, 0u, 0u, 0u};
-            if (c0 + 8 * q < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
-            *reinterpret_cast<uint4*>(&s_in[pix * kSP + 8 * q]) = v;
+            if (e < kHY * kHX * oct) {
+                const int q = e % oct, pix = e / oct;
+                const int gy = y0 + pix / kHX - 1, gx = x0 + pix % kHX - 1;
+                if (c0 + 8 * q < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    v = *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
+            }
+            ri[jj] = v;
         }
-        // weights of the chunk: s_w[co][tap * C + ch]; K beyond 9 C is zero
-        for (int e = tid; e < ncot * 16 * 9 * oct; e += 256) {
-            const int q = e % oct, r = e / oct;
-            const int tap = r % 9, co = r / 9;
-            const uint4 v = *reinterpret_cast<const uint4*>(
-                wt + ((size_t)tap * cout + co0 + co) * cinp + c0 + 8 * q);
-            *reinterpret_cast<uint4*>(&s_w[co * kSWP + tap * C + 8 * q]) = v;
+#pragma unroll
+        for (int jj = 0; jj < kJW; ++jj) {
+            const int e = tid + 256 * jj;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (e < ncot * 16 * 9 * oct) {
+                const int q = e % oct, r = e / oct;
+                v = *reinterpret_cast<const uint4*>(wt + ((size_t)(r % 9) * cout + co0 + r / 9) * cinp + c0 + 8 * q);
+            }
+            rw[jj] = v;
+        }
+#pragma unroll
+        for (int jj = 0; jj < kJI; ++jj) {
+            const int e = tid + 256 * jj;
+            if (e < kHY * kHX * oct)
+                *reinterpret_cast<uint4*>(&s_in[(e / oct) * kSP + 8 * (e % oct)]) = ri[jj];
+        }
+        // weights: s_w[co][tap * C + ch]; K beyond 9 C is zero
+#pragma unroll
+        for (int jj = 0; jj < kJW; ++jj) {
+            const int e = tid + 256 * jj;
+            if (e < ncot * 16 * 9 * oct) {
+                const int q = e % oct, r = e / oct;
+                *reinterpret_cast<uint4*>(&s_w[(r / 9) * kSWP + (r % 9) * C + 8 * q]) = rw[jj];
+            }
         }
         for (int e = tid; e < ncot * 16 * ((KC - 9 * C) / 8); e += 256) {
             const int per = (KC - 9 * C) / 8;

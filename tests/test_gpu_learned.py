@@ -131,3 +131,46 @@ def test_tiny_training_run_drives_the_stage(sg, ctx, mods):
     assert r["converged"] and r["rel_true_residual"] <= 1e-8
     calls, acc = stage.counts()
     assert calls == r["precond_applies"]
+
+
+def test_shipped_network_gains_on_held_out_fields(sg, ctx, mods):
+    """Ablation with the shipped weights (trained on fields 10000..11999) on
+    held-out 256^2 fields (seeds 20000..20003, sigma 1.5, l 0.05): mixed
+    two-level RAS(16, overlap 2)-FGMRES(30) to an FP64-verified 1e-8 with and
+    without the learned stage.  The stage must cut the total iteration count;
+    the gain and the time ratio are printed (the network costs more than the
+    steps it saves at this size)."""
+    import time
+    P, Lm = mods
+    import __graft_entry__
+    params = np.load(os.path.join(__graft_entry__.PKG_DIR, "weights", "unet_grf256.npy"))
+    side = 256
+    net = sg.UNet(ctx, params)
+    kw = dict(method="fgmres", restart=30, max_iter=3000, rtol=1e-8, ortho="cgs2")
+    it_plain = it_learn = 0
+    t_plain = t_learn = 0.0
+    for seed in range(20000, 20004):
+        rp, ci, v, g, b = grf_problem(P, side, seed)
+        A = sg.Csr(ctx, rp, ci, v)
+        part = sg.partition(rp, ci, 16, 0, vals=v, weighted=True)
+        base = sg.Precond(ctx, "twolevel_ras", nparts=16, overlap=2, precision="fp32",
+                          partition=part, coarse_solve=1).setup(A)
+        stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side).setup(A)
+        db = dev(sg, ctx, b)
+        for M, which in ((base, "plain"), (stage, "learned")):
+            x = dev(sg, ctx, np.zeros(A.n))
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = sg.solve(A, db, x, M, **kw)
+            dt = time.perf_counter() - t0
+            assert r["converged"] and r["rel_true_residual"] <= 1e-8, (seed, which, r)
+            if which == "plain":
+                it_plain += r["iterations"]
+                t_plain += dt
+            else:
+                it_learn += r["iterations"]
+                t_learn += dt
+    gain = 1.0 - it_learn / it_plain
+    print(f"held-out 256^2 fields: iterations {it_plain} plain, {it_learn} with the learned stage "
+          f"({100 * gain:.1f}% fewer); time {t_plain:.3f} s vs {t_learn:.3f} s")
+    assert it_learn < it_plain
