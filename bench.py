@@ -341,7 +341,8 @@ def run_learned(args, sg, P, cfg):
     if os.path.exists(wpath):
         net = sg.UNet(ctx, np.load(wpath))
         stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side,
-                                min_gain=args.min_gain, every=args.every).setup(A)
+                                min_gain=args.min_gain, every=args.every,
+                                coarsen=args.coarsen).setup(A)
         t_l, it_l, rel_l = timed(stage, "learned stage FGMRES")
         calls, acc = stage.counts()
         learned = {"s": round(t_l, 4), "iterations": it_l, "rel_true_residual": rel_l,
@@ -624,6 +625,8 @@ def main():
                     help="c5: network parameters (.npy) of the learned stage")
     ap.add_argument("--min-gain", type=float, default=0.01)
     ap.add_argument("--every", type=int, default=1)
+    ap.add_argument("--coarsen", type=int, default=1,
+                    help="c5: the network sees c x c block averages (1, 2 or 4)")
     ap.add_argument("--rows", action="store_true",
                     help="measure the rows around the headline path (BiCGSTAB, ILUT, "
                          "analytics, Matrix Market reader), one JSON line each, and exit")

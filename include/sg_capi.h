@@ -368,6 +368,10 @@ typedef struct {
     int nx, ny;      /* grid: row y*nx + x; nx and ny multiples of 8            */
     double min_gain; /* 0 -> 0.01                                               */
     int every;       /* run the network on every every-th application (0 -> 1) */
+    int coarsen;     /* 0 or 1: the network sees the nx x ny grid; 2 or 4: it sees
+                        the c x c block averages of s and log kappa (nx/c and ny/c
+                        multiples of 8) and its output is interpolated bilinearly
+                        back to the grid before the A-scaling above              */
 } sg_learned_opts;
 sg_status sg_precond_create_learned(sg_context* ctx, sg_precond* base, sg_unet* net,
                                     const double* log_kappa /* host, nx*ny */,

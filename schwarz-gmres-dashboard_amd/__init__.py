@@ -69,7 +69,8 @@ class Convergence(C.Structure):
 
 
 class LearnedOpts(C.Structure):
-    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("min_gain", C.c_double), ("every", C.c_int)]
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("min_gain", C.c_double), ("every", C.c_int),
+                ("coarsen", C.c_int)]
 
 
 class PrecondInfo(C.Structure):
@@ -353,11 +354,11 @@ class LearnedStage:
     network correction with residual-reduction fallback.  base and net must
     outlive the stage (references are kept here)."""
 
-    def __init__(self, ctx, base, net, log_kappa, nx, ny, min_gain=0.01, every=1):
+    def __init__(self, ctx, base, net, log_kappa, nx, ny, min_gain=0.01, every=1, coarsen=1):
         self.ctx, self.base, self.net = ctx, base, net
         lk = np.ascontiguousarray(log_kappa, np.float64)
         assert lk.size == nx * ny
-        o = LearnedOpts(nx, ny, min_gain, every)
+        o = LearnedOpts(nx, ny, min_gain, every, coarsen)
         self.h = _VP()
         ctx.check(lib().sg_precond_create_learned(ctx.h, base.h, net.h, _p(lk), C.byref(o),
                                                   C.byref(self.h)))

@@ -73,45 +73,39 @@ ProfScope::~ProfScope() {
 
 // ---- deterministic reduction kernels ----------------------------------------
 
-// Block partial sums of a*b (or a*a) over a grid-stride loop; the last block to
-// finish adds the kRedBlocks partials in a fixed order.
+// Block partial sums of a*b (or a*a) over a grid-stride loop (k_dot_part),
+// then one workgroup adds the partials in a fixed order (k_dot_fin): two
+// launches instead of a last-block ticket, whose device-scope fences cost more
+// than the reduction on this part.
 template <bool SQUARE>
 __global__ void __launch_bounds__(kRedThreads)
-k_dot(int n, const double* __restrict__ a, const double* __restrict__ b,
-      double* __restrict__ partials, unsigned* __restrict__ ticket, double* __restrict__ out,
-      int take_sqrt) {
+k_dot_part(int n, const double* __restrict__ a, const double* __restrict__ b,
+           double* __restrict__ partials) {
     double s = 0.0;
     for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += gridDim.x * kRedThreads)
         s += SQUARE ? a[i] * a[i] : a[i] * b[i];
     __shared__ double red[kRedThreads];
-    __shared__ bool last;
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = kRedThreads / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = red[0];
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_dot_fin(int nblk, const double* __restrict__ partials, double* __restrict__ out, int take_sqrt) {
+    __shared__ double red[kRedThreads];
     double t = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kRedThreads)
-        t += __builtin_nontemporal_load(&partials[i]);
+    for (int i = threadIdx.x; i < nblk; i += kRedThreads) t += partials[i];
     red[threadIdx.x] = t;
     __syncthreads();
     for (int w = kRedThreads / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        *out = take_sqrt ? sqrt(red[0]) : red[0];
-        *ticket = 0u;
-    }
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(red[0]) : red[0];
 }
 
 __global__ void k_axpy(int n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
@@ -130,14 +124,20 @@ static int red_grid(int n) {
 }
 
 void dot_launch(sg_context* ctx, int n, const double* a, const double* b, double* result) {
-    hipLaunchKernelGGL(k_dot<false>, dim3(red_grid(n)), dim3(kRedThreads), 0, ctx->stream, n, a,
-                       b, ctx->d_partials, ctx->d_ticket, result, 0);
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_dot_part<false>, dim3(g), dim3(kRedThreads), 0, ctx->stream, n, a, b,
+                       ctx->d_partials);
+    hipLaunchKernelGGL(k_dot_fin, dim3(1), dim3(kRedThreads), 0, ctx->stream, g, ctx->d_partials,
+                       result, 0);
     SG_HIP(hipGetLastError());
 }
 
 static void nrm_launch(sg_context* ctx, int n, const double* a, double* result) {
-    hipLaunchKernelGGL(k_dot<true>, dim3(red_grid(n)), dim3(kRedThreads), 0, ctx->stream, n, a,
-                       a, ctx->d_partials, ctx->d_ticket, result, 1);
+    const int g = red_grid(n);
+    hipLaunchKernelGGL(k_dot_part<true>, dim3(g), dim3(kRedThreads), 0, ctx->stream, n, a, a,
+                       ctx->d_partials);
+    hipLaunchKernelGGL(k_dot_fin, dim3(1), dim3(kRedThreads), 0, ctx->stream, g, ctx->d_partials,
+                       result, 1);
     SG_HIP(hipGetLastError());
 }
 
@@ -219,8 +219,6 @@ sg_status sg_context_create(int device, sg_context** out) {
         SG_HIP(hipDeviceGetAttribute(&ctx->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                      device));
         SG_HIP(hipMalloc(&ctx->d_partials, sizeof(double) * kRedBlocks * kMaxRedCols));
-        SG_HIP(hipMalloc(&ctx->d_ticket, sizeof(unsigned) * 64));
-        SG_HIP(hipMemset(ctx->d_ticket, 0, sizeof(unsigned) * 64));
         SG_HIP(hipMalloc(&ctx->d_result, sizeof(double) * kMaxRedCols));
         SG_HIP(hipHostMalloc(&ctx->h_pinned, sizeof(double) * 256, hipHostMallocDefault));
         sg::noalloc_init_device(device);
@@ -245,7 +243,6 @@ sg_status sg_context_destroy(sg_context* ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     sg::krylov_cache_free(ctx->krylov);
     if (ctx->d_partials) (void)hipFree(ctx->d_partials);
-    if (ctx->d_ticket) (void)hipFree(ctx->d_ticket);
     if (ctx->d_result) (void)hipFree(ctx->d_result);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -107,9 +107,8 @@ struct sg_context {
     std::string err;
     int num_cu = 256;
     int max_lds = 65536;  // bytes of LDS one workgroup may allocate
-    // deterministic reduction scratch: partials[kMaxRedCols][kRedBlocks], ticket, result
+    // deterministic reduction scratch: partials[kMaxRedCols][kRedBlocks], result
     double* d_partials = nullptr;
-    unsigned* d_ticket = nullptr;
     double* d_result = nullptr;  // kMaxRedCols doubles
     double* h_pinned = nullptr;  // pinned staging for small reads
     bool profiling = false;
