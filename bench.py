@@ -203,6 +203,57 @@ def probe(sg, ctx, A, M, db, dx, k, info, n, nnz):
     print(json.dumps(out))
 
 
+def sweep_clock(sg, ctx, M, db, n, k, out_path=None):
+    """K preconditioner applies with the staged sweep's workgroup clock armed.
+
+    Per apply: span = last workgroup end - first workgroup start; busy = the
+    sum of workgroup spans / (256 CUs x span), so 1 - busy is the CU time the
+    sweep leaves idle; balanced = sum of spans / 256, the span if the same
+    work were spread evenly.  A least-squares fit span_s ~ a rows_s + b
+    levels_s + c over the subdomains says which size the span follows."""
+    dy = sg.DeviceArray(ctx, n)
+    M.apply(db, dy)  # first apply runs the shape autotuner
+    ctx.sync()
+    recs = [M.sweep_clock(db, dy) for _ in range(k)]
+    cus = 256
+    rows = []
+    for r in recs:
+        d = r["end_us"] - r["start_us"]
+        span = float(r["end_us"].max())
+        rows.append((span, float(d.sum() / (cus * span)), float(d.sum() / cus), float(d.max()),
+                     float(np.median(d)), float(r["start_us"].max())))
+    a = np.array(rows)
+    med = np.median(a, axis=0)
+    r = recs[int(np.argsort(a[:, 0])[len(a) // 2])]  # the median apply
+    d = r["end_us"] - r["start_us"]
+    lev = (r["flev"] + r["blev"]).astype(float)
+    X = np.stack([r["rows"].astype(float), lev, np.ones_like(lev)], 1)
+    coef, *_ = np.linalg.lstsq(X, d, rcond=None)
+    fit = X @ coef
+    r2 = 1 - float(np.sum((d - fit) ** 2) / np.sum((d - d.mean()) ** 2))
+    per_cu = np.unique(r["hwid"], return_counts=True)[1]
+    t = int(np.argmax(d))
+    ph = {"gather": r["gathered_us"] - r["start_us"], "sweeps": r["swept_us"] - r["gathered_us"],
+          "scatter": r["end_us"] - r["swept_us"]}
+    out = {"sweep_clock": {
+        "applies": k, "workgroups": int(len(d)), "span_us": round(med[0], 2),
+        "cu_busy_frac": round(med[1], 4), "balanced_us": round(med[2], 2),
+        "longest_us": round(med[3], 2), "median_wg_us": round(med[4], 2),
+        "last_start_us": round(med[5], 2),
+        "distinct_cus": int(len(per_cu)), "max_wg_per_cu": int(per_cu.max()),
+        "longest": {"sub": int(r["sub"][t]), "rows": int(r["rows"][t]),
+                    "levels": int(lev[t]), "chunks": int(r["chunks"][t]),
+                    "phases_us": {k2: round(float(v2[t]), 2) for k2, v2 in ph.items()}},
+        "median_phases_us": {k2: round(float(np.median(v2)), 2) for k2, v2 in ph.items()},
+        "max_phases_us": {k2: round(float(v2.max()), 2) for k2, v2 in ph.items()},
+        "fit_us": {"per_krow": round(1e3 * coef[0], 3), "per_level": round(coef[1], 4),
+                   "const": round(coef[2], 2), "r2": round(r2, 3)},
+        "rate_khz": r["rate_khz"]}}
+    print(json.dumps(out))
+    if out_path:
+        np.savez(out_path, **{k2: v2 for k2, v2 in r.items() if k2 != "rate_khz"})
+
+
 def copy_probe():
     import torch
     a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
@@ -621,6 +672,11 @@ def main():
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling aid: after setup run this many preconditioner applies and "
                          "SpMVs on the workload, print their mean times and exit")
+    ap.add_argument("--sweep-clock", type=int, default=0,
+                    help="profiling aid: after setup run this many preconditioner applies "
+                         "with the staged sweep's workgroup clock, print CU busy time and exit")
+    ap.add_argument("--clock-out", default=None,
+                    help="with --sweep-clock: also save the median apply's records (.npz)")
     ap.add_argument("--weights", default=None,
                     help="c5: network parameters (.npy) of the learned stage")
     ap.add_argument("--min-gain", type=float, default=0.01)
@@ -714,6 +770,9 @@ def run_single(args, sg, P, cfg):
     info = M.stats()
     print(f"# setup {t_setup:.2f} s (partition {t_part:.2f} s)  {info}", file=sys.stderr,
           flush=True)
+    if args.sweep_clock:
+        sweep_clock(sg, ctx, M, db, n, args.sweep_clock, args.clock_out)
+        return
     if args.probe:
         probe(sg, ctx, A, M, db, dx, args.probe, info, n, nnz)
         return

@@ -192,6 +192,19 @@ typedef struct {
 } sg_precond_info;
 sg_status sg_precond_stats(const sg_precond* M, sg_precond_info* info);
 
+/* Diagnostic with no counterpart in the reference: one apply d_y = M^{-1} d_x
+ * with the clock of the staged subdomain sweep armed.  Every workgroup of that
+ * sweep writes 12 values to rec (cap >= 12 * nsub entries), at its launch slot:
+ * ticks of the device wall clock at its start, when the right-hand side is
+ * gathered, when both sweeps are done and at its end; subdomain, hardware id
+ * (HIP's __smid: XCC, shader engine and CU ids packed), overlapped rows,
+ * forward levels, backward levels, staged chunks, owned rows, 0.  *nrec
+ * counts the workgroups that recorded (0: the sweep in use is not the staged
+ * kernel; empty or guarded-off subdomains leave zeros).  *rate_khz is the
+ * wall clock's rate. */
+sg_status sg_precond_sweep_clock(sg_precond* M, const double* d_x, double* d_y,
+                                 long long* rec, int cap, int* nrec, double* rate_khz);
+
 /* Introspection for parity tests and tooling (all host outputs). */
 sg_status sg_precond_partition(const sg_precond* M, int* part /* int[nrows] */);
 /* Total overlapped size sum_s |subdomain_dofs[s]|, then the sets (sorted global

@@ -143,6 +143,8 @@ def lib():
         "sg_solve_batch": ([_VP, _VP, _VP, _VP, C.POINTER(SolveParams), _VP, _VP, C.c_int],
                            C.c_int),
         "sg_precond_stats": ([_VP, C.POINTER(PrecondInfo)], C.c_int),
+        "sg_precond_sweep_clock": ([_VP, _VP, _VP, _VP, C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)], C.c_int),
         "sg_context_set_profiling": ([_VP, C.c_int], C.c_int),
         "sg_context_set_autotune": ([_VP, C.c_int], C.c_int),
         "sg_context_set_tune_cache": ([_VP, C.c_char_p], C.c_int),
@@ -525,6 +527,32 @@ class Precond:
         info = PrecondInfo()
         self.ctx.check(lib().sg_precond_stats(self.h, C.byref(info)))
         return {k: getattr(info, k) for k, _ in PrecondInfo._fields_}
+
+    def sweep_clock(self, x, y):
+        """One apply y = M^{-1} x with the staged sweep's workgroup clock armed.
+
+        Returns a dict of per-workgroup arrays (launch order): start, gathered
+        (right-hand side in LDS), swept (both triangular sweeps done) and end
+        in microseconds from the first start, subdomain, hardware id (HIP's
+        __smid: XCC, SE and CU ids packed), rows, forward and backward levels,
+        chunks, owned rows;
+        only the workgroups that recorded.  Raises SgError when the sweep in
+        use is not the staged kernel."""
+        nsub = self.stats()["nsub"]
+        rec = np.zeros(12 * nsub, np.int64)
+        nrec, khz = C.c_int(), C.c_double()
+        self.ctx.check(lib().sg_precond_sweep_clock(self.h, x.ptr, y.ptr, _p(rec), len(rec),
+                                                    C.byref(nrec), C.byref(khz)))
+        if nrec.value == 0:
+            raise SgError(7, "sweep clock: the staged sweep is not the one in use")
+        r = rec.reshape(nsub, 12)
+        r = r[r[:, 3] != 0]
+        t0 = r[:, 0].min()
+        us = 1e3 / khz.value
+        return dict(start_us=(r[:, 0] - t0) * us, gathered_us=(r[:, 1] - t0) * us,
+                    swept_us=(r[:, 2] - t0) * us, end_us=(r[:, 3] - t0) * us, sub=r[:, 4],
+                    hwid=r[:, 5], rows=r[:, 6], flev=r[:, 7], blev=r[:, 8], chunks=r[:, 9],
+                    owned=r[:, 10], rate_khz=khz.value)
 
     def coarse(self):
         nc = C.c_int()

@@ -163,6 +163,9 @@ struct sg_precond {
     size_t st_ct_off = 0, st_buf_off = 0, st_bytes = 0;
     sg::DevBuf<sg::SChunk> d_chunks;
     sg::DevBuf<uint4> d_img;   // every chunk in its LDS layout, 16-byte aligned
+    // diagnostic clock of the staged sweep: 12 values per workgroup while armed
+    sg::DevBuf<long long> d_clk;
+    bool clk_on = false;
     // staged ring sweep (vector too large for LDS): ring of R values + two
     // chunk buffers; chunks per subdomain: forward first/count, backward first/count
     bool sr_ok = false;
@@ -1277,6 +1280,11 @@ __device__ __forceinline__ T fold_value(const FoldRegs<T>& f, int nc) {
     return f.c * acc;
 }
 
+// Diagnostic clock of the staged sweep, per workgroup: start, gather done,
+// sweeps done, end (device wall clock), subdomain, hardware id, rows, forward
+// levels, backward levels, chunks, owned rows, 0.
+constexpr int kClockWords = 12;
+
 // LDS: level tables (forward | backward, four ints per level) | z (ns values,
 // zero slot, trash slot) | chunk descriptors | buffer 0 | buffer 1 | trash word.
 template <typename T, int MODE, int NT, int KR>
@@ -1288,8 +1296,9 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                const own_i2* __restrict__ own2, const double* __restrict__ x,
                double* __restrict__ y, T* __restrict__ zout, const T* __restrict__ cz,
                CoarseFold<T> cf, DevGuard guard, int z_off, int ct_off, int buf_off,
-               int buf_words) {
+               int buf_words, long long* __restrict__ clk) {
     extern __shared__ unsigned char lds_raw[];
+    const long long t_start = clk ? wall_clock64() : 0;
     const int s = order[blockIdx.x];
     if (guard_stop(guard, s)) return;
     const SubInfo si = subs[s];
@@ -1350,6 +1359,7 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     dA = chunk_g(2);
     stage_load<T, NT, KR>(RA, dA, img);
     __syncthreads();
+    const long long t_gathered = clk ? wall_clock64() : 0;
     auto run = [&](const SChunk& c, const lds_t<stage_q>* buf) {
         if (c.bwd) stage_levels<T, true, NT, KR>(z, lb, c, buf, si.ns);
         else stage_levels<T, false, NT, KR>(z, lf, c, buf, si.ns);
@@ -1381,12 +1391,31 @@ k_sweep_staged(const SubInfo* __restrict__ subs, const int* __restrict__ order,
     if (MODE == 0 && si.nown > 0) own_batch<NT, US>(ob, own2 + si.own0, si.nown, threadIdx.x);
     if (ci < nch) run(c0q, q0);  // odd count: the last chunk is in buffer 0
     __syncthreads();
+    const long long t_swept = clk ? wall_clock64() : 0;
     if (MODE == 0) {
         const bool has_c = cf.inv != nullptr || cz != nullptr;
         const T add = cf.inv ? cfold : cz ? cz[s] : (T)0;
         scatter_owned2<NT, US>(ob, own2 + si.own0, si.nown, z, has_c, add, y);
     } else {
         for (int i = threadIdx.x; i < si.ns; i += NT) zout[si.row0 + i] = z[i];
+    }
+    if (clk) {  // diagnostic clock: the workgroup's span and what it solved
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long* r = clk + kClockWords * (size_t)blockIdx.x;
+            r[0] = t_start;
+            r[1] = t_gathered;
+            r[2] = t_swept;
+            r[3] = wall_clock64();
+            r[4] = s;
+            r[5] = (long long)__smid();
+            r[6] = si.ns;
+            r[7] = si.nflev;
+            r[8] = si.nblev;
+            r[9] = nch;
+            r[10] = si.nown;
+            r[11] = 0;
+        }
     }
 }
 
@@ -2812,7 +2841,8 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
                        M->ctx->stream, M->d_subs.p, M->d_order.p, M->d_chunk0.p, M->d_chunks.p, \
                        (const stage_q*)M->d_img.p, M->flev.p, M->blev.p, M->d_dofs.p,          \
                        (const own_i2*)M->d_own2.p, x, y, zo, cz, fold, guard, (int)M->lds_tab_bytes, \
-                       (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4)
+                       (int)M->st_ct_off, (int)M->st_buf_off, M->st_buf / 4,              \
+                       M->clk_on ? M->d_clk.p : nullptr)
         int kr = 4;
         if constexpr (std::is_same<T, float>::value) kr = M->st_kr;
         if (kr == 2) {
@@ -3243,6 +3273,38 @@ sg_status sg_precond_apply(sg_precond* M, const double* d_x, double* d_y) {
     }
     if (M->n && (!d_x || !d_y)) return SG_E_ARG;
     return run_guarded(M->ctx, [&] { precond_apply_launch(M, d_x, d_y, Guard{}); });
+}
+
+sg_status sg_precond_sweep_clock(sg_precond* M, const double* d_x, double* d_y,
+                                 long long* rec, int cap, int* nrec, double* rate_khz) {
+    if (!M || !rec || !nrec || !rate_khz) return SG_E_ARG;
+    if (M->learned || !M->ready || M->nsub == 0) {
+        if (M->ctx) M->ctx->err = "sweep clock: needs a set-up subdomain preconditioner";
+        return SG_E_STATE;
+    }
+    if (!d_x || !d_y || cap < kClockWords * M->nsub) return SG_E_ARG;
+    return run_guarded(M->ctx, [&] {
+        const size_t words = kClockWords * (size_t)M->nsub;
+        if (M->d_clk.n < words) M->d_clk.alloc(words);
+        SG_HIP(hipMemsetAsync(M->d_clk.p, 0, words * sizeof(long long), M->ctx->stream));
+        M->clk_on = true;
+        try {
+            precond_apply_launch(M, d_x, d_y, Guard{});
+        } catch (...) {
+            M->clk_on = false;
+            throw;
+        }
+        M->clk_on = false;
+        SG_HIP(hipMemcpyAsync(rec, M->d_clk.p, words * sizeof(long long), hipMemcpyDeviceToHost,
+                              M->ctx->stream));
+        SG_HIP(hipStreamSynchronize(M->ctx->stream));
+        int cnt = 0;
+        for (int b = 0; b < M->nsub; ++b) cnt += rec[kClockWords * (size_t)b + 3] != 0;
+        *nrec = cnt;
+        int khz = 0;
+        SG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, M->ctx->device));
+        *rate_khz = khz;
+    });
 }
 
 sg_status sg_precond_destroy(sg_precond* M) {

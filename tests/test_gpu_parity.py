@@ -291,6 +291,39 @@ def test_staged_sweep_bitwise(sg, ctx, spec, kind, nparts, prec):
         assert np.array_equal(outs[9], O.Precond(kind, Ao, nparts, 1, part).apply(x))
 
 
+def test_sweep_clock(sg, ctx):
+    """The staged sweep's workgroup clock (sg_precond_sweep_clock): the clocked
+    apply gives the plain apply's values bit for bit, and every subdomain's
+    workgroup records once, with its own size; a sweep that is not the staged
+    kernel is refused."""
+    A = O.grid("2d:120:120")
+    dA = sg.Csr(ctx, A.rp, A.ci, A.v)
+    x = dev(sg, ctx, np.random.default_rng(15).standard_normal(A.n))
+    ctx.autotune(10)
+    M = sg.Precond(ctx, "twolevel_ras", nparts=16, overlap=1, precision="fp32",
+                   coarse_solve=1).setup(dA)
+    y, yc = dev(sg, ctx, np.zeros(A.n)), dev(sg, ctx, np.zeros(A.n))
+    M.apply(x, y)
+    assert M.stats()["sweep_shape"] == 8
+    r = M.sweep_clock(x, yc)
+    assert np.array_equal(yc.get(), y.get())
+    assert sorted(r["sub"].tolist()) == list(range(16))
+    sizes = [len(d) for d in M.subdomains()]
+    assert [int(r["rows"][k]) for k in np.argsort(r["sub"])] == sizes
+    assert r["start_us"].min() == 0.0
+    assert np.all(r["start_us"] <= r["gathered_us"]) and np.all(r["gathered_us"] < r["swept_us"])
+    assert np.all(r["swept_us"] <= r["end_us"])
+    assert r["owned"].sum() == A.n and np.all(r["owned"] <= r["rows"])
+    assert np.all(r["flev"] > 0) and np.all(r["blev"] > 0) and np.all(r["chunks"] > 0)
+    assert r["rate_khz"] > 0
+    ctx.autotune(3)
+    Mb = sg.Precond(ctx, "ras", nparts=16, overlap=1, precision="fp32",
+                    partition=M.partition()).setup(dA)
+    with pytest.raises(sg.SgError):
+        Mb.sweep_clock(x, yc)
+    ctx.autotune(1)
+
+
 def test_ring_sweep_fp32(sg, ctx):
     A = O.poisson3d(64, 64, 64)
     dA = sg.Csr(ctx, A.rp, A.ci, A.v)
