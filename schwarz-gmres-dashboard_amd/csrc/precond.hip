@@ -146,6 +146,7 @@ struct sg_precond {
     // ring sweep (z too large for LDS): R slots, a full barrier every F levels
     bool ring = false;
     int ring_R = 0, ring_F = 0;
+    int ring_kp = sg::kKreg;      // entries prefetched per row: 8 when some level has K > 4
     size_t ring_bytes = 0, zstride = 0;
     sg::DevBuf<int> f_bpos;
     int sweep_variant = 1;     // index into kSweepShapes
@@ -607,18 +608,18 @@ k_schwarz_sweep(const SubInfo* __restrict__ subs, const int* __restrict__ order,
 // input zf for the forward sweep, zb for the backward one) and is prefetched
 // with the row data, so nothing on the level-to-level chain waits for global
 // memory.  Operation order per row equals the ILU(0) sweep's.
-template <typename T>
+template <typename T, int KP>
 struct RowPreR {
     int row;  // forward: backward position of the row; backward: the local row
-    int col[kKreg];
-    T val[kKreg];
+    int col[KP];
+    T val[KP];
     T diag;
     T own;
     unsigned use;
 };
 
-template <typename T, bool BWD>
-__device__ __forceinline__ void row_prefetch_r(RowPreR<T>& p, const Level& L, int q, int pos0,
+template <typename T, bool BWD, int KP>
+__device__ __forceinline__ void row_prefetch_r(RowPreR<T, KP>& p, const Level& L, int q, int pos0,
                                                const int* __restrict__ rows,
                                                const int* __restrict__ cols,
                                                const T* __restrict__ vals,
@@ -631,7 +632,7 @@ __device__ __forceinline__ void row_prefetch_r(RowPreR<T>& p, const Level& L, in
     p.own = ownv[L.start - pos0 + qq];
     unsigned use = live ? 0x80000000u : 0u;
 #pragma unroll
-    for (int k = 0; k < kKreg; ++k) {
+    for (int k = 0; k < KP; ++k) {
         const int idx = L.ell + (k < L.K ? k : 0) * L.size + qq;
         p.col[k] = cols[idx];
         p.val[k] = vals[idx];
@@ -653,9 +654,9 @@ __device__ __forceinline__ T ring_read(const lds_t<T>* ring, const T* __restrict
     return zglob[~c];
 }
 
-template <typename T, bool BWD, bool GLOB>
+template <typename T, bool BWD, bool GLOB, int KP>
 __device__ __forceinline__ void row_compute_r(lds_t<T>* ring, const T* zdep, T* zout,
-                                              const RowPreR<T>& p, const Level& L, int pos0,
+                                              const RowPreR<T, KP>& p, const Level& L, int pos0,
                                               int R, const int* __restrict__ rows,
                                               const int* __restrict__ cols,
                                               const T* __restrict__ vals,
@@ -667,11 +668,11 @@ __device__ __forceinline__ void row_compute_r(lds_t<T>* ring, const T* zdep, T* 
     if (p.use & 0x80000000u) {
         T acc = p.own;
 #pragma unroll
-        for (int k = 0; k < kKreg; ++k) {
+        for (int k = 0; k < KP; ++k) {
             const bool u = (p.use >> k) & 1u;
             acc -= (u ? p.val[k] : (T)0) * ring_read<T, GLOB>(ring, zdep, u ? p.col[k] : R);
         }
-        for (int k = kKreg; k < L.K; ++k)
+        for (int k = KP; k < L.K; ++k)
             acc -= vals[L.ell + k * L.size + t] *
                    ring_read<T, GLOB>(ring, zdep, cols[L.ell + k * L.size + t]);
         if (BWD) acc = acc / p.diag;
@@ -693,7 +694,7 @@ __device__ __forceinline__ void row_compute_r(lds_t<T>* ring, const T* zdep, T* 
     }
 }
 
-template <typename T, bool BWD, int D, bool GLOB>
+template <typename T, bool BWD, int D, bool GLOB, int KP>
 __device__ __forceinline__ void sweep_r(lds_t<T>* ring, const T* zdep, T* zout, const Level* lev,
                                         int nlev, int pos0, int R, int F,
                                         const int* __restrict__ rows,
@@ -710,16 +711,16 @@ __device__ __forceinline__ void sweep_r(lds_t<T>* ring, const T* zdep, T* zout, 
     };
     // prefetches past the last level read position pos0 of an empty level
     auto level = [&](int l) { return l < nlev ? lev[l] : Level{pos0, 0, 0, 0}; };
-    RowPreR<T> b[D];
+    RowPreR<T, KP> b[D];
 #pragma unroll
     for (int j = 0; j < D - 1; ++j)
-        row_prefetch_r<T, BWD>(b[j], level(j), t, pos0, rows, cols, vals, diags, dst, ownv);
+        row_prefetch_r<T, BWD, KP>(b[j], level(j), t, pos0, rows, cols, vals, diags, dst, ownv);
     for (int l = 0; l < nlev; l += D) {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            row_prefetch_r<T, BWD>(b[(j + D - 1) % D], level(l + j + D - 1), t, pos0, rows,
+            row_prefetch_r<T, BWD, KP>(b[(j + D - 1) % D], level(l + j + D - 1), t, pos0, rows,
                                    cols, vals, diags, dst, ownv);
-            row_compute_r<T, BWD, GLOB>(ring, zdep, zout, b[j], level(l + j), pos0, R, rows,
+            row_compute_r<T, BWD, GLOB, KP>(ring, zdep, zout, b[j], level(l + j), pos0, R, rows,
                                         cols, vals, diags, dst, ownv);
             level_barrier(l + j);
             if (l + j + 1 >= nlev) return;
@@ -730,7 +731,9 @@ __device__ __forceinline__ void sweep_r(lds_t<T>* ring, const T* zdep, T* zout, 
 // Gather into forward level order, forward sweep (results to zb in backward
 // order), backward sweep (results to zr by local row), prolongation as
 // k_schwarz_sweep does.  z3 holds zf | zb | zr, each ns + 1 per subdomain.
-template <typename T, int MODE, int NT, int D, bool TAB>
+// KP: factor entries per row prefetched with the row (4, or 8 for ILUT
+// factors); entries past KP are read inside their level.
+template <typename T, int MODE, int NT, int D, bool TAB, int KP = kKreg>
 __global__ void __launch_bounds__(NT)
 k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ order,
                      const Level* __restrict__ flev, const int* __restrict__ f_row,
@@ -784,16 +787,16 @@ k_schwarz_sweep_ring(const SubInfo* __restrict__ subs, const int* __restrict__ o
     if (threadIdx.x == 0) ring[R] = (T)0;
     __syncthreads();
     if (F == INT_MAX) {  // every reference stays in the ring
-        sweep_r<T, false, D, false>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
+        sweep_r<T, false, D, false, KP>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
                                     nullptr, f_bpos, zf);
         __syncthreads();
-        sweep_r<T, true, D, false>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
+        sweep_r<T, true, D, false, KP>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
                                    b_diag, nullptr, zb);
     } else {
-        sweep_r<T, false, D, true>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
+        sweep_r<T, false, D, true, KP>(ring, zb, zb, lf, si.nflev, pf0, R, F, f_row, f_col, f_val,
                                    nullptr, f_bpos, zf);
         __syncthreads();
-        sweep_r<T, true, D, true>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
+        sweep_r<T, true, D, true, KP>(ring, zr, zr, lb, si.nblev, pb0, R, F, b_row, b_col, b_val,
                                   b_diag, nullptr, zb);
     }
     __syncthreads();
@@ -2247,6 +2250,12 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
     M->nsub = nsub;
     M->max_ns = pl.max_ns;
     M->lrp_host = pl.lrp;
+    {
+        int kmax = 0;
+        for (const Level& lv : pl.flev) kmax = std::max(kmax, lv.K);
+        for (const Level& lv : pl.blev) kmax = std::max(kmax, lv.K);
+        M->ring_kp = kmax > kKreg ? 2 * kKreg : kKreg;
+    }
     M->d_subs.upload(pl.subs, st);
     M->d_order.upload(pl.order, st);
     M->flev.upload(pl.flev, st);
@@ -2588,17 +2597,23 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
 #define SG_SET_RING1(T, MD, NT, D, TB)                                                            \
     SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, NT, D, TB>,             \
                                hipFuncAttributeMaxDynamicSharedMemorySize, rb));
+#define SG_SET_RING8(T, MD, NT, D)                                                                \
+    SG_HIP(hipFuncSetAttribute((const void*)k_schwarz_sweep_ring<T, MD, NT, D, true, 8>,        \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, rb));
 #define SG_SET_RING(T, MD)                                                                        \
     SG_SET_RING1(T, MD, 256, 3, true) SG_SET_RING1(T, MD, 512, 3, true)                         \
     SG_SET_RING1(T, MD, 1024, 3, true) SG_SET_RING1(T, MD, 256, 6, true)                        \
     SG_SET_RING1(T, MD, 512, 6, true) SG_SET_RING1(T, MD, 256, 3, false)                        \
-    SG_SET_RING1(T, MD, 512, 3, false) SG_SET_RING1(T, MD, 1024, 3, false)
+    SG_SET_RING1(T, MD, 512, 3, false) SG_SET_RING1(T, MD, 1024, 3, false)                      \
+    SG_SET_RING8(T, MD, 256, 3) SG_SET_RING8(T, MD, 512, 3) SG_SET_RING8(T, MD, 1024, 3)        \
+    SG_SET_RING8(T, MD, 256, 6) SG_SET_RING8(T, MD, 512, 6)
             if (fp32) {
                 SG_SET_RING(float, 0) SG_SET_RING(float, 1)
             } else {
                 SG_SET_RING(double, 0) SG_SET_RING(double, 1)
             }
 #undef SG_SET_RING1
+#undef SG_SET_RING8
 #undef SG_SET_RING
         }
         if (M->sr_ok) {
@@ -2896,12 +2911,22 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
     }
     if (!ZLDS && M->ring) {
         const int F = M->ring_F == INT_MAX ? INT_MAX : std::max(1, M->ring_F);
-#define SG_RING(NT, D, TB)                                                                        \
-    hipLaunchKernelGGL((k_schwarz_sweep_ring<T, MODE, NT, D, TB>), dim3(M->nsub), dim3(NT),      \
+#define SG_RING_KP(NT, D, TB, KP)                                                                 \
+    hipLaunchKernelGGL((k_schwarz_sweep_ring<T, MODE, NT, D, TB, KP>), dim3(M->nsub), dim3(NT),  \
                        M->ring_bytes, M->ctx->stream, M->d_subs.p, M->d_order.p, M->flev.p,     \
                        M->f_row.p, M->f_col.p, fv, M->f_bpos.p, M->blev.p, M->b_row.p,         \
                        M->b_col.p, bv, bd, M->d_dofs.p, M->d_own.p, M->d_own_gid.p, x, y, zg,   \
                        M->zstride, zo, cz, guard, (int)M->lds_tab_bytes, M->ring_R, F)
+// levels with more than four entries per row (ILUT factors) prefetch eight
+#define SG_RING(NT, D, TB)                                                                        \
+    do {                                                                                          \
+        if constexpr (TB) {                                                                       \
+            if (M->ring_kp > kKreg) SG_RING_KP(NT, D, true, 8);                                   \
+            else SG_RING_KP(NT, D, true, kKreg);                                                  \
+        } else {                                                                                  \
+            SG_RING_KP(NT, D, false, kKreg);                                                      \
+        }                                                                                         \
+    } while (0)
         const int nt = sh.syncfree ? 512 : sh.threads;
         if (!M->tab_in_lds) {
             if (nt == 256) SG_RING(256, 3, false);
@@ -2913,6 +2938,7 @@ static void launch_sweep(sg_precond* M, const double* x, double* y, const DevGua
         else if (nt == 1024) SG_RING(1024, 3, true);
         else SG_RING(512, 3, true);
 #undef SG_RING
+#undef SG_RING_KP
         return;
     }
     const size_t lds = ZLDS ? M->lds_bytes : M->lds_tab_bytes;
