@@ -11,7 +11,8 @@ pins the pieces the oracle can check at that size:
   the coarse matrix A0, its LU factors and pivots at Nc = 256 bit for bit
   against the oracle (src/precond/coarse_space.h:43-60, coarse_solve.h:15-47);
   one FP32 two-level apply within 1e-5 of the oracle's FP32 mode; the mixed
-  two-level FGMRES(40) converges.
+  two-level FGMRES(40) converges, and its hipGraph replay gives the same
+  iterates bit for bit.
 * 64 GRF systems (sides 64..512, 6.4M rows): the batched graph solve converges
   for every system, and three sampled systems equal their single solves bit
   for bit (iterations and x).
@@ -127,6 +128,15 @@ def test_jump_diffusion160_two_level(sg, P):
     rel = true_rel(rp, ci, v, b, dx.get())
     say(f"diffusion 160^3 mixed fgmres(40)+twolevel_ras: {r['iterations']} it, rel {rel:.2e}")
     assert r["converged"] and rel <= 1e-8
+    # the restart cycle captured once and replayed as a hipGraph (the bench's
+    # default): the same iterates bit for bit
+    xg = sg.DeviceArray(ctx, n)
+    xg.zero()
+    rg = sg.solve(A, db, xg, M, method="fgmres", restart=40, max_iter=10000, rtol=1e-8,
+                  ortho="cgs2", graph=True)
+    say(f"diffusion 160^3 graph replay: {rg['iterations']} it")
+    assert rg["iterations"] == r["iterations"]
+    assert np.array_equal(xg.get(), dx.get())
     ctx.close()
     # the oracle on the same partition (OpenMP over subdomains)
     Ao = O.Csr(rp, ci, v)
