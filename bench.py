@@ -180,6 +180,49 @@ def cpu_baseline(O, rp, ci, v, b, part, cfg, iters, est_iters, est_label):
                       f"{est_iters} iterations ({est_label}) at that rate"}
 
 
+def cpu_full(args, sg, P, cfg):
+    """The oracle's FP64 two-level RAS-FGMRES (the reference algorithm, OpenMP
+    over subdomains) run to ||r|| <= 1e-8 ||b|| on the host cores, with no GPU:
+    the measured CPU time-to-solution beside the bench's per-iteration
+    extrapolation.  Runs in pieces of ten restart cycles, each continuing from
+    the last x (a restart happens there anyway, so the iterates are those of
+    one run), printing a progress line per piece."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as O
+    rp, ci, v, b, xs = make_problem(P, args.config)
+    if b is None:
+        b = spmv_np(rp, ci, v, xs)
+    t0 = time.perf_counter()
+    part = sg.partition(rp, ci, cfg["nparts"], cfg["ufactor"], vals=v, weighted=cfg["weighted"])
+    t_part = time.perf_counter() - t0
+    A = O.Csr(rp, ci, v)
+    t0 = time.perf_counter()
+    pc = O.Precond("twolevel_ras", A, cfg["nparts"], cfg["overlap"], part, precision=0)
+    t_setup = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print(f"# cpu full: partition {t_part:.1f} s, oracle setup {t_setup:.1f} s, "
+          f"{threads} threads", file=sys.stderr, flush=True)
+    tol = 1e-8 * float(np.linalg.norm(b))
+    x = np.zeros(len(b))
+    its, t_solve, conv = 0, 0.0, False
+    piece = 10 * cfg["restart"]
+    while its < cfg["base_max_iter"] and not conv:
+        t0 = time.perf_counter()
+        r = O.solve("fgmres", A, b, pc, cfg["restart"], piece, tol, x0=x)
+        t_solve += time.perf_counter() - t0
+        x, its, conv = r["x"], its + r["iterations"], r["converged"]
+        print(f"# cpu full: {its} it, {t_solve:.0f} s, |r| {r['residual_norm']:.3e} "
+              f"(target {tol:.3e})", file=sys.stderr, flush=True)
+    rel = float(np.linalg.norm(b - spmv_np(rp, ci, v, x)) / np.linalg.norm(b))
+    print(json.dumps({"cpu_full": {
+        "workload": cfg["workload"], "kind": "port", "cores": threads,
+        "solver": f"fgmres({cfg['restart']})+twolevel_ras fp64",
+        "setup_s": round(t_setup, 3), "partition_s": round(t_part, 3),
+        "solve_s": round(t_solve, 3), "value": round(t_setup + t_solve, 3), "unit": "s",
+        "iterations": its, "converged": conv, "rel_true_residual": rel,
+        "s_per_iteration": round(t_solve / max(1, its), 5)}}))
+
+
 def probe(sg, ctx, A, M, db, dx, k, info, n, nnz):
     """K preconditioner applies and K SpMVs with per-launch HIP-event timing."""
     dy = sg.DeviceArray(ctx, n)
@@ -666,6 +709,9 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=15)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only run the CPU oracle to convergence on the host cores (no GPU; "
+                         "minutes to tens of minutes at c3), print one JSON line and exit")
     ap.add_argument("--autotune", type=int, default=1,
                     help="kernel-shape mode (sg_context_set_autotune): 1 tune, 0 defaults, "
                          "2..11 force a shape")
@@ -710,6 +756,9 @@ def main():
 
     sg = __graft_entry__.load_package()
     P = load_problems()
+    if args.cpu_full:
+        cpu_full(args, sg, P, CONFIGS[args.config])
+        return
     if args.rows:
         run_rows(args, sg, P)
         return
