@@ -4,14 +4,22 @@
 // without host synchronisation (and can sit inside a captured solve):
 //
 //   z_b = base(x);  s = x - A z_b;  ss = s.s
-//   e   = sqrt(ss/n) * net(s / sqrt(ss/n), log kappa)
-//   a = s.(A e), c = (A e).(A e)            (||s - alpha A e||^2 = ss - a^2 / c)
-//   z = z_b + (a / c) e  if a^2 >= min_gain * c * ss,  else  z = z_b
+//   sc  = c x c block averages of s (sc = s when c = 1); q = sc.sc
+//   o   = net(sc / sqrt(q/cn), <log kappa>_c)        (cn = cells of the net grid)
+//   e   = sqrt(q/cn) * P o                            (P: cell-centred bilinear
+//                                                      interpolation, identity at c = 1)
+//   a = s.(A e), cc = (A e).(A e)          (||s - alpha A e||^2 = ss - a^2 / cc)
+//   z = z_b + (a / cc) e  if a^2 >= min_gain * cc * ss,  else  z = z_b
 //
-// With coarsen = c > 1 the network sees the c x c block averages of s and of
-// log kappa (normalised by the averaged residual's own rms) and its output is
-// interpolated bilinearly (cell-centred) back to the grid; the A-scaling and
-// the acceptance test are the same.
+// The grid work outside the network is four kernels: the residual with its
+// block averages and the partial sums of ss and q; the network input (the
+// block-averaged log kappa channel is written once at setup); A e with the
+// partial sums of a and cc, e interpolated on the fly from the network output
+// for every column the row touches (e is never stored); and the update, which
+// interpolates e again.  Every reduction is per-block partials summed in one
+// fixed order by each reader, so applications are reproducible bit for bit.
+// Every kernel, the network's included, returns at once when the solver's
+// step guard says the system has stopped.
 //
 // The stage changes from call to call, which FGMRES allows
 // (src/core/fgmres.cpp:28-172, FlexApply src/core/fgmres.h:37).
@@ -31,82 +39,189 @@ struct LearnedStage {
     int every = 1;
     int coarsen = 1, cnx = 0, cny = 0, cn = 0;  // the network's grid
     long long calls_host = 0;
-    DevBuf<double> s, e, Ae, scal;  // scal: ss, a, c, ss of the averaged residual
-    DevBuf<double> sc;              // block averages of s (coarsen > 1)
-    DevBuf<float> lk, lkc, nin;
-    DevBuf<int> cnt;                // applications, accepted corrections
+    int nblk_r = 0, nblk_e = 0;      // blocks of the residual / A e kernels
+    DevBuf<double> s, sc, scal;      // scal: ss, a, cc, q
+    DevBuf<double> part_r, part_e;   // block partials {ss, q}, {a, cc}
+    DevBuf<float> nin;               // network input [2][cny][cnx]
+    DevBuf<int> cnt;                 // applications, accepted corrections
 };
 
-__global__ void k_stage_in(int n, const double* __restrict__ s, const double* __restrict__ ss,
-                           const float* __restrict__ lk, float* __restrict__ nin) {
-    const double r = *ss > 0.0 ? 1.0 / sqrt(*ss / n) : 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        nin[i] = (float)(s[i] * r);
-        nin[n + i] = lk[i];
+constexpr int kRT = 256;  // threads of the residual kernel: 64 columns x 4 net-grid rows
+constexpr int kET = 256;  // threads of the A e and update kernels
+
+// Deterministic block sum of two doubles (a, b) over kRT threads; thread 0
+// gets the totals.
+__device__ inline void block_sum2(double& a, double& b) {
+    __shared__ double ra[kRT / kWave], rb[kRT / kWave];
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        ra[w] = a;
+        rb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = ra[0];
+        b = rb[0];
+        for (int k = 1; k < (int)(blockDim.x / kWave); ++k) {
+            a += ra[k];
+            b += rb[k];
+        }
     }
 }
 
-__global__ void k_stage_out(int n, const float* __restrict__ o, const double* __restrict__ ss,
-                            double* __restrict__ e) {
-    const double r = sqrt(*ss / n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        e[i] = (double)o[i] * r;
-}
-
-// c x c block averages: sc (double) of s, lkc of log kappa (coarse grid cnx x cny)
-__global__ void k_stage_pool(int cnx, int cny, int c, int nx, const double* __restrict__ s,
-                             const float* __restrict__ lk, double* __restrict__ sc,
-                             float* __restrict__ lkc) {
-    const int cn = cnx * cny;
-    const double w = 1.0 / ((double)c * c);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += gridDim.x * blockDim.x) {
-        const int I = i % cnx, J = i / cnx;
-        double a = 0.0, l = 0.0;
-        for (int dy = 0; dy < c; ++dy)
-            for (int dx = 0; dx < c; ++dx) {
-                const int f = (J * c + dy) * nx + I * c + dx;
-                a += s[f];
-                l += lk[f];
-            }
-        sc[i] = a * w;
-        lkc[i] = (float)(l * w);
+// Sum of part[2 k + j] over k < nblk in one fixed order (every block that
+// calls it gets the same bits); result valid in thread 0.
+__device__ inline void sum_partials(const double* __restrict__ part, int nblk, double& s0,
+                                    double& s1) {
+    s0 = 0.0;
+    s1 = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+        s0 += part[2 * k];
+        s1 += part[2 * k + 1];
     }
+    block_sum2(s0, s1);
 }
 
-// e = rms * bilinear interpolation (cell-centred, clamped at the edges) of the
-// coarse network output o (cnx x cny) onto the nx x ny grid
-__global__ void k_stage_up(int nx, int ny, int c, int cnx, int cny, const float* __restrict__ o,
-                           const double* __restrict__ ss, double* __restrict__ e) {
-    const int n = nx * ny;
-    const double r = sqrt(*ss / ((double)cnx * cny));
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int x = i % nx, y = i / nx;
-        const double fx = (x + 0.5) / c - 0.5, fy = (y + 0.5) / c - 0.5;
-        int x0 = (int)floor(fx), y0 = (int)floor(fy);
-        const double tx = fx - x0, ty = fy - y0;
-        const int x1 = min(max(x0 + 1, 0), cnx - 1), y1 = min(max(y0 + 1, 0), cny - 1);
-        x0 = min(max(x0, 0), cnx - 1);
-        y0 = min(max(y0, 0), cny - 1);
-        const double v = (1.0 - ty) * ((1.0 - tx) * o[y0 * cnx + x0] + tx * o[y0 * cnx + x1]) +
-                         ty * ((1.0 - tx) * o[y1 * cnx + x0] + tx * o[y1 * cnx + x1]);
-        e[i] = v * r;
-    }
-}
-
-__global__ void k_stage_final(int n, const double* __restrict__ sc, const double* __restrict__ e,
-                              double* __restrict__ y, double min_gain, int* __restrict__ cnt,
-                              DevGuard guard) {
+// s = x - A z for the rows of a 64-column x (4 C)-row tile, the C x C block
+// averages sc of s, block partials {s.s, sc.sc}.  Thread (tx, ty): fine
+// column bx * 64 + tx, net-grid row by * 4 + ty (its C fine rows); the C
+// lanes of one net-grid cell add their column sums with shuffles.
+template <int C>
+__global__ void __launch_bounds__(kRT)
+k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ ci,
+            const double* __restrict__ v, const double* __restrict__ x,
+            const double* __restrict__ z, double* __restrict__ s, double* __restrict__ sc,
+            double* __restrict__ part, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
-    const double ss = sc[0], a = sc[1], c = sc[2];
-    const bool ok = c > 0.0 && ss > 0.0 && a * a >= min_gain * c * ss;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt[0] += 1;
-        cnt[1] += ok ? 1 : 0;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int X = blockIdx.x * 64 + tx, Y = blockIdx.y * 4 + ty;
+    const int cnx = nx / C, cny = ny / C;
+    const bool live = X < nx && Y < cny;
+    double col = 0.0, ss = 0.0;
+    if (live) {
+#pragma unroll
+        for (int dy = 0; dy < C; ++dy) {
+            const int i = (Y * C + dy) * nx + X;
+            const int k0 = rp[i], k1 = rp[i + 1];
+            double acc = 0.0;
+            for (int k = k0; k < k1; ++k) acc += v[k] * z[ci[k]];
+            const double si = x[i] - acc;
+            s[i] = si;
+            ss += si * si;
+            col += si;
+        }
     }
-    if (!ok) return;
-    const double al = a / c;
+#pragma unroll
+    for (int o = 1; o < C; o <<= 1) col += __shfl_xor(col, o);
+    double q = 0.0;
+    if (live && (tx % C) == 0) {
+        const double a = col * (1.0 / (C * C));
+        sc[Y * cnx + X / C] = a;
+        q = a * a;
+    }
+    block_sum2(ss, q);
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        part[2 * b] = ss;
+        part[2 * b + 1] = q;
+    }
+}
+
+// nin[0][.] = sc / sqrt(q / cn); block 0 also stores ss and q in scal[0], scal[3]
+__global__ void __launch_bounds__(kRT)
+k_stage_in(int cn, int nblk, const double* __restrict__ part, const double* __restrict__ sc,
+           double* __restrict__ scal, float* __restrict__ nin, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
+    __shared__ double sh_r;
+    double ss, q;
+    sum_partials(part, nblk, ss, q);
+    if (threadIdx.x == 0) {
+        sh_r = q > 0.0 ? 1.0 / sqrt(q / cn) : 0.0;
+        if (blockIdx.x == 0) {
+            scal[0] = ss;
+            scal[3] = q;
+        }
+    }
+    __syncthreads();
+    const double r = sh_r;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += gridDim.x * blockDim.x)
+        nin[i] = (float)(sc[i] * r);
+}
+
+// P o at fine cell j: cell-centred bilinear interpolation from the cnx x cny
+// net grid, clamped at the edges (C = 1: o itself)
+template <int C>
+__device__ inline double interp(const float* __restrict__ o, int j, int nx, int cnx, int cny) {
+    if (C == 1) return (double)o[j];
+    const int x = j % nx, y = j / nx;
+    const double fx = (x + 0.5) / C - 0.5, fy = (y + 0.5) / C - 0.5;
+    int x0 = (int)floor(fx), y0 = (int)floor(fy);
+    const double tx = fx - x0, ty = fy - y0;
+    const int x1 = min(max(x0 + 1, 0), cnx - 1), y1 = min(max(y0 + 1, 0), cny - 1);
+    x0 = min(max(x0, 0), cnx - 1);
+    y0 = min(max(y0, 0), cny - 1);
+    return (1.0 - ty) * ((1.0 - tx) * o[y0 * cnx + x0] + tx * o[y0 * cnx + x1]) +
+           ty * ((1.0 - tx) * o[y1 * cnx + x0] + tx * o[y1 * cnx + x1]);
+}
+
+// block partials {s.(A e), (A e).(A e)} with e = sqrt(q/cn) P o, row i per thread
+template <int C>
+__global__ void __launch_bounds__(kET)
+k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ ci,
+           const double* __restrict__ v, const float* __restrict__ o,
+           const double* __restrict__ scal, const double* __restrict__ s,
+           double* __restrict__ part, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
+    const int n = nx * ny, cnx = nx / C, cny = ny / C;
+    const double r = sqrt(scal[3] / ((double)cnx * cny));
+    double a = 0.0, c = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k0 = rp[i], k1 = rp[i + 1];
+        double ae = 0.0;
+        for (int k = k0; k < k1; ++k) ae += v[k] * (r * interp<C>(o, ci[k], nx, cnx, cny));
+        a += s[i] * ae;
+        c += ae * ae;
+    }
+    block_sum2(a, c);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = c;
+    }
+}
+
+// the acceptance test and z = z_b + (a / cc) e
+template <int C>
+__global__ void __launch_bounds__(kET)
+k_stage_final(int nx, int ny, int nblk, const double* __restrict__ part,
+              double* __restrict__ scal, const float* __restrict__ o, double* __restrict__ y,
+              double min_gain, int* __restrict__ cnt, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
+    __shared__ double sh_al;
+    __shared__ int sh_ok;
+    double a, c;
+    sum_partials(part, nblk, a, c);
+    if (threadIdx.x == 0) {
+        const double ss = scal[0];
+        const bool ok = c > 0.0 && ss > 0.0 && a * a >= min_gain * c * ss;
+        sh_ok = ok ? 1 : 0;
+        sh_al = ok ? a / c : 0.0;
+        if (blockIdx.x == 0) {
+            scal[1] = a;
+            scal[2] = c;
+            cnt[0] += 1;
+            cnt[1] += ok ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    if (!sh_ok) return;
+    const int n = nx * ny, cnx = nx / C, cny = ny / C;
+    const double al = sh_al * sqrt(scal[3] / ((double)cnx * cny));
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        y[i] += al * e[i];
+        y[i] += al * interp<C>(o, i, nx, cnx, cny);
 }
 
 __global__ void k_count_base(int* __restrict__ cnt, DevGuard guard) {
@@ -114,7 +229,7 @@ __global__ void k_count_base(int* __restrict__ cnt, DevGuard guard) {
     cnt[0] += 1;
 }
 
-static int grid_n(int n) { return std::max(1, std::min((n + 255) / 256, 8192)); }
+static int grid_n(int n) { return std::max(1, std::min((n + kET - 1) / kET, 2048)); }
 
 LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int nx, int ny,
                           const double* log_kappa, const sg_learned_opts* o) {
@@ -144,17 +259,23 @@ LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int n
     L->cnx = nx / c;
     L->cny = ny / c;
     L->cn = L->cnx * L->cny;
-    std::vector<float> lk(log_kappa, log_kappa + L->n);
-    L->lk.upload(lk, ctx->stream);
+    // the network's second input channel: c x c block averages of log kappa
+    std::vector<float> nin_h((size_t)2 * L->cn, 0.f);
+    for (int J = 0; J < L->cny; ++J)
+        for (int I = 0; I < L->cnx; ++I) {
+            double a = 0.0;
+            for (int dy = 0; dy < c; ++dy)
+                for (int dx = 0; dx < c; ++dx) a += log_kappa[(size_t)(J * c + dy) * nx + I * c + dx];
+            nin_h[(size_t)L->cn + (size_t)J * L->cnx + I] = (float)(a / ((double)c * c));
+        }
+    L->nin.upload(nin_h, ctx->stream);
+    L->nblk_r = ((nx + 63) / 64) * ((L->cny + 3) / 4);
+    L->nblk_e = grid_n(L->n);
     L->s.alloc(L->n);
-    L->e.alloc(L->n);
-    L->Ae.alloc(L->n);
+    L->sc.alloc(L->cn);
     L->scal.alloc(4);
-    L->nin.alloc((size_t)2 * L->cn);
-    if (c > 1) {
-        L->sc.alloc(L->cn);
-        L->lkc.alloc(L->cn);
-    }
+    L->part_r.alloc((size_t)2 * L->nblk_r);
+    L->part_e.alloc((size_t)2 * L->nblk_e);
     L->cnt.alloc(2);
     SG_HIP(hipMemsetAsync(L->cnt.p, 0, sizeof(int) * 2, ctx->stream));
     SG_HIP(hipStreamSynchronize(ctx->stream));
@@ -175,45 +296,40 @@ void learned_setup(LearnedStage* L, const sg_csr* A) {
 
 void learned_prepare(LearnedStage* L) {
     precond_ensure_tuned(L->base);
-    spmv_ensure_tuned(const_cast<sg_csr*>(L->A));
     unet_prepare(L->net, 1, L->cny, L->cnx);
+}
+
+template <int C>
+static void stage_launch(LearnedStage* L, const double* x, double* y, const DevGuard& dg) {
+    hipStream_t st = L->ctx->stream;
+    const sg_csr* A = L->A;
+    hipLaunchKernelGGL(k_stage_res<C>, dim3((L->nx + 63) / 64, (L->cny + 3) / 4), dim3(kRT), 0, st,
+                       L->nx, L->ny, A->rp.p, A->ci.p, A->v.p, x, y, L->s.p, L->sc.p, L->part_r.p, dg);
+    hipLaunchKernelGGL(k_stage_in, dim3(std::max(1, std::min((L->cn + kRT - 1) / kRT, 256))),
+                       dim3(kRT), 0, st, L->cn, L->nblk_r, L->part_r.p, L->sc.p, L->scal.p,
+                       L->nin.p, dg);
+    unet_forward_dev(L->net, 1, L->cny, L->cnx, L->nin.p, false, dg.flag);
+    hipLaunchKernelGGL(k_stage_ae<C>, dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny, A->rp.p,
+                       A->ci.p, A->v.p, unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
+    hipLaunchKernelGGL(k_stage_final<C>, dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
+                       L->nblk_e, L->part_e.p, L->scal.p, unet_output(L->net), y, L->min_gain,
+                       L->cnt.p, dg);
 }
 
 void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& g) {
     hipStream_t st = L->ctx->stream;
-    sg_csr* A = const_cast<sg_csr*>(L->A);
-    const int n = L->n;
     precond_apply_launch(L->base, x, y, g);
+    const DevGuard dg = dev_guard(g, nullptr);
     const bool run_net = (L->calls_host++ % L->every) == 0;
     if (!run_net) {
-        hipLaunchKernelGGL(k_count_base, dim3(1), dim3(64), 0, st, L->cnt.p, dev_guard(g, nullptr));
-        SG_HIP(hipGetLastError());
-        return;
-    }
-    spmv_launch(A, -1.0, y, 1.0, x, L->s.p, g, st);  // s = x - A z_b
-    dot_launch(L->ctx, n, L->s.p, L->s.p, L->scal.p);
-    if (L->coarsen == 1) {
-        hipLaunchKernelGGL(k_stage_in, dim3(grid_n(n)), dim3(256), 0, st, n, L->s.p, L->scal.p,
-                           L->lk.p, L->nin.p);
-        unet_forward_dev(L->net, 1, L->ny, L->nx, L->nin.p, false);
-        hipLaunchKernelGGL(k_stage_out, dim3(grid_n(n)), dim3(256), 0, st, n, unet_output(L->net),
-                           L->scal.p, L->e.p);
+        hipLaunchKernelGGL(k_count_base, dim3(1), dim3(64), 0, st, L->cnt.p, dg);
+    } else if (L->coarsen == 4) {
+        stage_launch<4>(L, x, y, dg);
+    } else if (L->coarsen == 2) {
+        stage_launch<2>(L, x, y, dg);
     } else {
-        const int cn = L->cn;
-        hipLaunchKernelGGL(k_stage_pool, dim3(grid_n(cn)), dim3(256), 0, st, L->cnx, L->cny,
-                           L->coarsen, L->nx, L->s.p, L->lk.p, L->sc.p, L->lkc.p);
-        dot_launch(L->ctx, cn, L->sc.p, L->sc.p, L->scal.p + 3);
-        hipLaunchKernelGGL(k_stage_in, dim3(grid_n(cn)), dim3(256), 0, st, cn, L->sc.p,
-                           L->scal.p + 3, L->lkc.p, L->nin.p);
-        unet_forward_dev(L->net, 1, L->cny, L->cnx, L->nin.p, false);
-        hipLaunchKernelGGL(k_stage_up, dim3(grid_n(n)), dim3(256), 0, st, L->nx, L->ny, L->coarsen,
-                           L->cnx, L->cny, unet_output(L->net), L->scal.p + 3, L->e.p);
+        stage_launch<1>(L, x, y, dg);
     }
-    spmv_launch(A, 1.0, L->e.p, 0.0, nullptr, L->Ae.p, g, st);
-    dot_launch(L->ctx, n, L->s.p, L->Ae.p, L->scal.p + 1);
-    dot_launch(L->ctx, n, L->Ae.p, L->Ae.p, L->scal.p + 2);
-    hipLaunchKernelGGL(k_stage_final, dim3(grid_n(n)), dim3(256), 0, st, n, L->scal.p, L->e.p, y,
-                       L->min_gain, L->cnt.p, dev_guard(g, nullptr));
     SG_HIP(hipGetLastError());
 }
 

@@ -190,7 +190,10 @@ void bicgstab_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
                    bool use_graph);
 
 // ---- learned stage (unet.hip, learned.hip) ---------------------------------------
-void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads);
+// stop: the forward's kernels return at once while *stop == 0 (a solver's step
+// flag); null: always run
+void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads,
+                      const int* stop);
 const float* unet_output(const sg_unet* U);
 void unet_prepare(sg_unet* U, int B, int H, int W);  // pack weights, size buffers
 struct LearnedStage;

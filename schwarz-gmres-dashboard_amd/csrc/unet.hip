@@ -100,7 +100,9 @@ __host__ __device__ inline ConvLds conv_lds(int cinp, int cob) {
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
-          const float* __restrict__ bias, View out, int cout, int H, int W, int cob) {
+          const float* __restrict__ bias, View out, int cout, int H, int W, int cob,
+          const int* __restrict__ stop) {
+    if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
     const ConvLds L = conv_lds(cinp, cob);
     const int kSP = L.sp, kSWP = L.swp;
@@ -325,7 +327,9 @@ __global__ void k_dw_reduce(int ntile, int cin, int cout, const float* __restric
 // ---- elementwise / layout kernels ---------------------------------------------
 // network input: [B][2][H][W] fp32 -> bf16 [B][H][W][16] (channels 2..15
 // zero), one pixel (two 16-byte words) per thread
-__global__ void k_input(int B, int H, int W, const float* __restrict__ in, View x) {
+__global__ void k_input(int B, int H, int W, const float* __restrict__ in, View x,
+                        const int* __restrict__ stop) {
+    if (stop && *stop == 0) return;
     const size_t hw = (size_t)H * W, n = (size_t)B * hw;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const size_t p = e % hw;
@@ -338,7 +342,8 @@ __global__ void k_input(int B, int H, int W, const float* __restrict__ in, View 
 }
 
 // 2x2 average pooling, eight channels (one 16-byte word) per thread
-__global__ void k_pool(int B, int C, int H, int W, View in, View out) {
+__global__ void k_pool(int B, int C, int H, int W, View in, View out, const int* __restrict__ stop) {
+    if (stop && *stop == 0) return;
     const int h = H / 2, w = W / 2, oc = C / 8;
     const size_t n = (size_t)B * oc * h * w;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -376,7 +381,8 @@ __global__ void k_pool_bwd(int B, int C, int H, int W, View dy, View dx) {
 
 // out channels [0, C) at H x W = nearest upsampling of a (H/2 x W/2), eight
 // channels (one 16-byte word) per thread
-__global__ void k_up(int B, int C, int H, int W, View a, View out) {
+__global__ void k_up(int B, int C, int H, int W, View a, View out, const int* __restrict__ stop) {
+    if (stop && *stop == 0) return;
     const int oc = C / 8;
     const size_t n = (size_t)B * oc * H * W;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -414,7 +420,9 @@ __global__ void k_relu_mask(int B, int C, int H, int W, View y, View dy) {
 }
 
 // head: out[b][p] = hb + sum_c hw[c] * d0[b][c][p]
-__global__ void k_head(int B, int H, int W, View d0, const float* __restrict__ hw, float* __restrict__ out) {
+__global__ void k_head(int B, int H, int W, View d0, const float* __restrict__ hw, float* __restrict__ out,
+                       const int* __restrict__ stop) {
+    if (stop && *stop == 0) return;
     const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const size_t p = e % hwn;
@@ -750,13 +758,13 @@ static int conv_cob(int tiles, int cout) {
     return cob;
 }
 
-static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W) {
+static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
     const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
     const int cob = conv_cob(tiles, co);
     hipLaunchKernelGGL(k_conv3x3<0>, dim3(tiles, (co + cob - 1) / cob), dim3(256),
                        conv_lds(cin_pad(ci), cob).bytes, U->ctx->stream, in, ci, cin_pad(ci),
-                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob);
+                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob, stop);
 }
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
@@ -765,7 +773,7 @@ static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int cob = conv_cob(tiles, ci);
     hipLaunchKernelGGL(k_conv3x3<1>, dim3(tiles, (ci + cob - 1) / cob), dim3(256),
                        conv_lds(co, cob).bytes, U->ctx->stream, dp, co, co, U->wb[i].p,
-                       (const float*)nullptr, dx, ci, H, W, cob);
+                       (const float*)nullptr, dx, ci, H, W, cob, (const int*)nullptr);
 }
 
 static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
@@ -778,41 +786,42 @@ static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
                        U->g.p + uconv_offset(i));
 }
 
-void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads) {
+void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads,
+                      const int* stop) {
     unet_pack(U);
     unet_shape(U, B, H, W, grads);
     hipStream_t s = U->ctx->stream;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     const size_t hw0 = (size_t)H * W;
-    hipLaunchKernelGGL(k_input, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16));
-    conv_fwd(U, 0, V(U->x, 16), V(U->a0, 16), H, W);
-    conv_fwd(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W);  // s0
+    hipLaunchKernelGGL(k_input, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16), stop);
+    conv_fwd(U, 0, V(U->x, 16), V(U->a0, 16), H, W, stop);
+    conv_fwd(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W, stop);  // s0
     hipLaunchKernelGGL(k_pool, dim3(grid1(B * 2 * hw0 / 4)), dim3(256), 0, s, B, 16, H, W, V(U->cat0, 48, 32),
-                       V(U->p0, 16));
-    conv_fwd(U, 2, V(U->p0, 16), V(U->a2, 32), H1, W1);
-    conv_fwd(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1);  // s1
+                       V(U->p0, 16), stop);
+    conv_fwd(U, 2, V(U->p0, 16), V(U->a2, 32), H1, W1, stop);
+    conv_fwd(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1, stop);  // s1
     hipLaunchKernelGGL(k_pool, dim3(grid1(B * 4 * hw0 / 16)), dim3(256), 0, s, B, 32, H1, W1,
-                       V(U->cat1, 96, 64), V(U->p1, 32));
-    conv_fwd(U, 4, V(U->p1, 32), V(U->a4, 64), H2, W2);
-    conv_fwd(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2);  // s2
+                       V(U->cat1, 96, 64), V(U->p1, 32), stop);
+    conv_fwd(U, 4, V(U->p1, 32), V(U->a4, 64), H2, W2, stop);
+    conv_fwd(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2, stop);  // s2
     hipLaunchKernelGGL(k_pool, dim3(grid1(B * 8 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
-                       V(U->cat2, 128, 64), V(U->p2, 64));
-    conv_fwd(U, 6, V(U->p2, 64), V(U->a6, 64), H3, W3);
-    conv_fwd(U, 7, V(U->a6, 64), V(U->a7, 64), H3, W3);
+                       V(U->cat2, 128, 64), V(U->p2, 64), stop);
+    conv_fwd(U, 6, V(U->p2, 64), V(U->a6, 64), H3, W3, stop);
+    conv_fwd(U, 7, V(U->a6, 64), V(U->a7, 64), H3, W3, stop);
     hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2, V(U->a7, 64),
-                       V(U->cat2, 128, 0));
-    conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2);
-    conv_fwd(U, 9, V(U->a8, 64), V(U->a9, 64), H2, W2);
+                       V(U->cat2, 128, 0), stop);
+    conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2, stop);
+    conv_fwd(U, 9, V(U->a8, 64), V(U->a9, 64), H2, W2, stop);
     hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1, V(U->a9, 64),
-                       V(U->cat1, 96, 0));
-    conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1);
-    conv_fwd(U, 11, V(U->a10, 32), V(U->a11, 32), H1, W1);
+                       V(U->cat1, 96, 0), stop);
+    conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1, stop);
+    conv_fwd(U, 11, V(U->a10, 32), V(U->a11, 32), H1, W1, stop);
     hipLaunchKernelGGL(k_up, dim3(grid1(B * 4 * hw0)), dim3(256), 0, s, B, 32, H, W, V(U->a11, 32),
-                       V(U->cat0, 48, 0));
-    conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W);
-    conv_fwd(U, 13, V(U->a12, 16), V(U->a13, 16), H, W);
+                       V(U->cat0, 48, 0), stop);
+    conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W, stop);
+    conv_fwd(U, 13, V(U->a12, 16), V(U->a13, 16), H, W, stop);
     hipLaunchKernelGGL(k_head, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, V(U->a13, 16), U->hw.p,
-                       U->out.p);
+                       U->out.p, stop);
     SG_HIP(hipGetLastError());
 }
 
@@ -981,7 +990,7 @@ sg_status sg_unet_get_params(const sg_unet* U, float* params) {
 sg_status sg_unet_forward(sg_unet* U, int B, int H, int W, const float* d_in, float* d_out) {
     if (!U || !d_in || !d_out) return SG_E_ARG;
     return run_guarded(U->ctx, [&] {
-        unet_forward_dev(U, B, H, W, d_in, false);
+        unet_forward_dev(U, B, H, W, d_in, false, nullptr);
         SG_HIP(hipMemcpyAsync(d_out, U->out.p, sizeof(float) * B * H * W, hipMemcpyDeviceToDevice,
                               U->ctx->stream));
     });
@@ -992,7 +1001,7 @@ sg_status sg_unet_loss_grad(sg_unet* U, int B, int H, int W, const float* d_in, 
     if (!U || !d_in) return SG_E_ARG;
     return run_guarded(U->ctx, [&] {
         if (U->loss_kind == 0 && !d_target) fail(SG_E_ARG, "this loss needs a target");
-        unet_forward_dev(U, B, H, W, d_in, true);
+        unet_forward_dev(U, B, H, W, d_in, true, nullptr);
         unet_backward_dev(U, d_in, d_target);
         if (loss)
             SG_HIP(hipMemcpyAsync(loss, U->lloss.p, sizeof(double), hipMemcpyDeviceToHost, U->ctx->stream));

@@ -96,6 +96,68 @@ def test_accepted_correction_reduces_the_residual(sg, ctx, mods):
     assert calls == 3 and acc == 3
 
 
+def _pool(a, c):
+    h, w = a.shape
+    return a.reshape(h // c, c, w // c, c).mean(axis=(1, 3))
+
+
+def _interp(o, c):
+    """Cell-centred bilinear interpolation of the H x W grid o onto cH x cW,
+    clamped at the edges (the identity for c = 1)."""
+    H, W = o.shape
+
+    def axis(m, k):
+        f = (np.arange(c * m) + 0.5) / c - 0.5
+        i0 = np.floor(f).astype(int)
+        t = f - i0
+        return np.clip(i0, 0, m - 1), np.clip(i0 + 1, 0, m - 1), t
+
+    x0, x1, tx = axis(W, c)
+    y0, y1, ty = axis(H, c)
+    top = (1 - tx)[None, :] * o[y0][:, x0] + tx[None, :] * o[y0][:, x1]
+    bot = (1 - tx)[None, :] * o[y1][:, x0] + tx[None, :] * o[y1][:, x1]
+    return (1 - ty)[:, None] * top + ty[:, None] * bot
+
+
+@pytest.mark.parametrize("coarsen", [1, 2, 4])
+def test_stage_matches_its_definition(sg, ctx, mods, coarsen):
+    """One application against a numpy restatement of the stage: s = x - A z_b,
+    its c x c block averages sc normalised by their rms (plus the averaged
+    log kappa) into the network, e = rms * bilinear(net output), z = z_b +
+    (s.Ae / Ae.Ae) e.  The network output is taken from the same network's
+    own forward on the restated input."""
+    from scipy.sparse import csr_matrix
+    P, Lm = mods
+    side = 64
+    rp, ci, v, g, _ = grf_problem(P, side, 24)
+    A = sg.Csr(ctx, rp, ci, v)
+    As = csr_matrix((v, ci, rp), shape=(A.n, A.n))
+    base = sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=2, precision="fp32").setup(A)
+    net = sg.UNet(ctx, Lm.init_params(6))
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, min_gain=1e-12,
+                            coarsen=coarsen).setup(A)
+    rng = np.random.default_rng(11 + coarsen)
+    for trial in range(2):
+        x = rng.standard_normal(A.n)
+        zb, z = dev(sg, ctx, np.zeros(A.n)), dev(sg, ctx, np.zeros(A.n))
+        base.apply(dev(sg, ctx, x), zb)
+        stage.apply(dev(sg, ctx, x), z)
+        zb = zb.get()
+        s = x - As @ zb
+        sc = _pool(s.reshape(side, side), coarsen)
+        q = float((sc * sc).sum())
+        rms = np.sqrt(q / sc.size)
+        nin = np.stack([sc / rms, _pool(g, coarsen)])[None].astype(np.float32)
+        o = net.forward(nin)[0].astype(np.float64)
+        e = rms * _interp(o, coarsen).ravel()
+        Ae = As @ e
+        alpha = (s @ Ae) / (Ae @ Ae)
+        corr = z.get() - zb
+        assert np.linalg.norm(corr - alpha * e) <= 1e-4 * np.linalg.norm(alpha * e), trial
+    calls, acc = stage.counts()
+    assert calls == 2 and acc == 2
+
+
 def test_needs_fgmres(sg, ctx, mods):
     P, Lm = mods
     rp, ci, v, g, b = grf_problem(P, 16, 23)
