@@ -235,8 +235,10 @@ void print_spectral_result(const SpectralResult& sr, const std::string& label = 
 // base; base must outlive the stage.
 class LearnedStage : public Preconditioner {
 public:
+    // coarsen 1, 2 or 4: the network runs on c x c block averages (sg_learned_opts)
     LearnedStage(DevicePreconditioner& base, std::vector<float> params, int nx, int ny,
-                 std::vector<double> log_kappa, double min_gain = 0.01, int every = 1);
+                 std::vector<double> log_kappa, double min_gain = 0.01, int every = 1,
+                 int coarsen = 1);
     ~LearnedStage() override;
     void setup(const CSRMatrix& A) override;
     void apply(const double* x, double* y, int n) const override;
@@ -248,7 +250,7 @@ private:
     DevicePreconditioner& base_;
     std::vector<float> params_;
     std::vector<double> logk_;
-    int nx_, ny_, every_;
+    int nx_, ny_, every_, coarsen_;
     double min_gain_;
     sg_unet* net_ = nullptr;
     sg_precond* M_ = nullptr;

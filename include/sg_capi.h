@@ -364,8 +364,12 @@ sg_status sg_unet_set_loss(sg_unet* net, int kind);
  * d_target may be NULL with loss kind 1 */
 sg_status sg_unet_loss_grad(sg_unet* net, int B, int H, int W, const float* d_in,
                             const float* d_target, double* loss, float* grad);
-/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient; with a
+ * weight decay wd > 0 the step also shrinks every parameter by lr * wd * p
+ * (decoupled, AdamW) */
 sg_status sg_unet_adam_step(sg_unet* net, float lr);
+/* decoupled weight decay of the Adam step (default 0) */
+sg_status sg_unet_set_weight_decay(sg_unet* net, float wd);
 
 /* ---- learned FGMRES stage ------------------------------------------------------
  * No counterpart in the reference: a flexible preconditioner for FGMRES

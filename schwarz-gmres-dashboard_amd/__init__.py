@@ -136,6 +136,7 @@ def lib():
         "sg_unet_loss_grad": ([_VP, C.c_int, C.c_int, C.c_int, _VP, _VP,
                                C.POINTER(C.c_double), _VP], C.c_int),
         "sg_unet_adam_step": ([_VP, C.c_float], C.c_int),
+        "sg_unet_set_weight_decay": ([_VP, C.c_float], C.c_int),
         "sg_unet_set_loss": ([_VP, C.c_int], C.c_int),
         "sg_csr_create_batch": ([_VP, C.c_int, _VP, C.c_int, _VP, _VP, _VP, C.POINTER(_VP)],
                                 C.c_int),
@@ -341,6 +342,10 @@ class UNet:
 
     def adam(self, lr):
         self.ctx.check(lib().sg_unet_adam_step(self.h, float(lr)))
+
+    def set_weight_decay(self, wd):
+        """Decoupled (AdamW) weight decay of the following Adam steps."""
+        self.ctx.check(lib().sg_unet_set_weight_decay(self.h, float(wd)))
 
     def __del__(self):
         try:
@@ -719,18 +724,20 @@ def solve_batch(A, b, x, M=None, method="fgmres", restart=30, max_iter=1000, tol
     return [_result(rs[q], hist[q], cap) for q in range(nsys)]
 
 
-def last_basis(ctx):
+def last_basis(ctx, with_z=True):
     """The Krylov basis of the last cycle of the last single-system GMRES/FGMRES
     solve on ctx (sg_solve_last_basis): dict with V (n x kv), Z (n x k, or None),
-    Hbar ((m+1) x m, before the Givens rotations), k and kv."""
+    Hbar ((m+1) x m, before the Givens rotations), k and kv.  with_z=False
+    copies V only (a transposed view; Z and Hbar None)."""
     n, m, k, kv = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     L = lib()
     ctx.check(L.sg_solve_last_basis(ctx.h, C.byref(n), C.byref(m), C.byref(k), C.byref(kv),
                                     None, None, None))
     V = np.zeros((kv.value, n.value))
-    Z = np.zeros((k.value, n.value))
-    H = np.zeros((m.value, m.value + 1))  # column-major (m+1) x m
+    Z = np.zeros((k.value, n.value)) if with_z else None
+    H = np.zeros((m.value, m.value + 1)) if with_z else None  # column-major (m+1) x m
     ctx.check(L.sg_solve_last_basis(ctx.h, C.byref(n), C.byref(m), C.byref(k), C.byref(kv),
-                                    _p(V), _p(Z), _p(H)))
-    return dict(V=V.T.copy(), Z=Z.T.copy(), Hbar=H.T.copy(), k=k.value, kv=kv.value,
+                                    _p(V), _p(Z) if with_z else None, _p(H) if with_z else None))
+    return dict(V=V.T.copy() if with_z else V.T, Z=Z.T.copy() if with_z else None,
+                Hbar=H.T.copy() if with_z else None, k=k.value, kv=kv.value,
                 m=m.value, n=n.value)

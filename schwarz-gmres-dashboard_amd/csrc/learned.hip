@@ -12,8 +12,9 @@
 //   z = z_b + (a / cc) e  if a^2 >= min_gain * cc * ss,  else  z = z_b
 //
 // The grid work outside the network is four kernels: the residual with its
-// block averages and the partial sums of ss and q; the network input (the
-// block-averaged log kappa channel is written once at setup); A e with the
+// block averages and the partial sums of ss and q; the network input, written
+// in bf16 straight into the network's input tensor (the block averages of
+// log kappa are formed once at setup); A e with the
 // partial sums of a and cc, e interpolated on the fly from the network output
 // for every column the row touches (e is never stored); and the update, which
 // interpolates e again.  Every reduction is per-block partials summed in one
@@ -42,9 +43,18 @@ struct LearnedStage {
     int nblk_r = 0, nblk_e = 0;      // blocks of the residual / A e kernels
     DevBuf<double> s, sc, scal;      // scal: ss, a, cc, q
     DevBuf<double> part_r, part_e;   // block partials {ss, q}, {a, cc}
-    DevBuf<float> nin;               // network input [2][cny][cnx]
+    DevBuf<float> lk;                // c x c block averages of log kappa [cny][cnx]
     DevBuf<int> cnt;                 // applications, accepted corrections
 };
+
+// float -> bfloat16 bits, round to nearest even (NaN/Inf kept), as the
+// network's own input conversion
+__device__ inline unsigned bf16_bits(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
 
 constexpr int kRT = 256;  // threads of the residual kernel: 64 columns x 4 net-grid rows
 constexpr int kET = 256;  // threads of the A e and update kernels
@@ -86,14 +96,49 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nblk, d
     block_sum2(s0, s1);
 }
 
+// sum_k a_ik u(col_k) of row i in CSR order.  SELL: from the SELL-64 copy of
+// A (slice i / 64, entry k of lane i % 64 at sp[slice] + 64 k + lane, padding
+// column -1; the same entries in the same order), eight entries in flight.
+template <bool SELL, typename U>
+__device__ inline double row_sum(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const double* __restrict__ v, const int* __restrict__ sp,
+                                 const int* __restrict__ sci, const double* __restrict__ sv,
+                                 U&& u) {
+    double acc = 0.0;
+    if (SELL) {
+        const int sl = i >> 6, lane = i & 63;
+        const int p0 = sp[sl], width = (sp[sl + 1] - p0) >> 6;
+        for (int k0 = 0; k0 < width; k0 += 8) {
+            double a[8], xv[8];
+            int c[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = p0 + min(k0 + q, width - 1) * 64 + lane;
+                a[q] = sv[idx];
+                c[q] = sci[idx];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) xv[q] = (k0 + q < width && c[q] >= 0) ? u(c[q]) : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (k0 + q < width && c[q] >= 0) acc += a[q] * xv[q];
+        }
+    } else {
+        const int k0 = rp[i], k1 = rp[i + 1];
+        for (int k = k0; k < k1; ++k) acc += v[k] * u(ci[k]);
+    }
+    return acc;
+}
+
 // s = x - A z for the rows of a 64-column x (4 C)-row tile, the C x C block
 // averages sc of s, block partials {s.s, sc.sc}.  Thread (tx, ty): fine
 // column bx * 64 + tx, net-grid row by * 4 + ty (its C fine rows); the C
 // lanes of one net-grid cell add their column sums with shuffles.
-template <int C>
+template <int C, bool SELL>
 __global__ void __launch_bounds__(kRT)
 k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ ci,
-            const double* __restrict__ v, const double* __restrict__ x,
+            const double* __restrict__ v, const int* __restrict__ sp, const int* __restrict__ sci,
+            const double* __restrict__ sv, const double* __restrict__ x,
             const double* __restrict__ z, double* __restrict__ s, double* __restrict__ sc,
             double* __restrict__ part, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
@@ -106,9 +151,7 @@ k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ 
 #pragma unroll
         for (int dy = 0; dy < C; ++dy) {
             const int i = (Y * C + dy) * nx + X;
-            const int k0 = rp[i], k1 = rp[i + 1];
-            double acc = 0.0;
-            for (int k = k0; k < k1; ++k) acc += v[k] * z[ci[k]];
+            const double acc = row_sum<SELL>(i, rp, ci, v, sp, sci, sv, [&](int j) { return z[j]; });
             const double si = x[i] - acc;
             s[i] = si;
             ss += si * si;
@@ -131,10 +174,13 @@ k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ 
     }
 }
 
-// nin[0][.] = sc / sqrt(q / cn); block 0 also stores ss and q in scal[0], scal[3]
+// The network input, bf16 NHWC [cny][cnx][16]: channel 0 = sc / sqrt(q / cn),
+// channel 1 = the averaged log kappa (lk), the rest zero; block 0 also stores
+// ss and q in scal[0], scal[3]
 __global__ void __launch_bounds__(kRT)
 k_stage_in(int cn, int nblk, const double* __restrict__ part, const double* __restrict__ sc,
-           double* __restrict__ scal, float* __restrict__ nin, DevGuard guard) {
+           const float* __restrict__ lk, double* __restrict__ scal, uint16_t* __restrict__ x,
+           DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     __shared__ double sh_r;
     double ss, q;
@@ -148,8 +194,12 @@ k_stage_in(int cn, int nblk, const double* __restrict__ part, const double* __re
     }
     __syncthreads();
     const double r = sh_r;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += gridDim.x * blockDim.x)
-        nin[i] = (float)(sc[i] * r);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += gridDim.x * blockDim.x) {
+        const unsigned c0 = bf16_bits((float)(sc[i] * r)), c1 = bf16_bits(lk[i]);
+        uint4* o = reinterpret_cast<uint4*>(x + (size_t)i * 16);
+        o[0] = uint4{c0 | (c1 << 16), 0u, 0u, 0u};
+        o[1] = uint4{0u, 0u, 0u, 0u};
+    }
 }
 
 // P o at fine cell j: cell-centred bilinear interpolation from the cnx x cny
@@ -169,10 +219,11 @@ __device__ inline double interp(const float* __restrict__ o, int j, int nx, int 
 }
 
 // block partials {s.(A e), (A e).(A e)} with e = sqrt(q/cn) P o, row i per thread
-template <int C>
+template <int C, bool SELL>
 __global__ void __launch_bounds__(kET)
 k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ ci,
-           const double* __restrict__ v, const float* __restrict__ o,
+           const double* __restrict__ v, const int* __restrict__ sp, const int* __restrict__ sci,
+           const double* __restrict__ sv, const float* __restrict__ o,
            const double* __restrict__ scal, const double* __restrict__ s,
            double* __restrict__ part, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
@@ -180,9 +231,9 @@ k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ c
     const double r = sqrt(scal[3] / ((double)cnx * cny));
     double a = 0.0, c = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int k0 = rp[i], k1 = rp[i + 1];
-        double ae = 0.0;
-        for (int k = k0; k < k1; ++k) ae += v[k] * (r * interp<C>(o, ci[k], nx, cnx, cny));
+        const double ae = row_sum<SELL>(i, rp, ci, v, sp, sci, sv, [&](int j) {
+            return r * interp<C>(o, j, nx, cnx, cny);
+        });
         a += s[i] * ae;
         c += ae * ae;
     }
@@ -229,7 +280,10 @@ __global__ void k_count_base(int* __restrict__ cnt, DevGuard guard) {
     cnt[0] += 1;
 }
 
-static int grid_n(int n) { return std::max(1, std::min((n + kET - 1) / kET, 2048)); }
+// blocks of the A e kernel (each leaves a partial pair) and of the update
+// kernel (each block adds all of those partials itself)
+static int grid_ae(int n) { return std::max(1, std::min((n + kET - 1) / kET, 1024)); }
+static int grid_final(int n) { return std::max(1, std::min((n + kET - 1) / kET, 256)); }
 
 LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int nx, int ny,
                           const double* log_kappa, const sg_learned_opts* o) {
@@ -260,17 +314,17 @@ LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int n
     L->cny = ny / c;
     L->cn = L->cnx * L->cny;
     // the network's second input channel: c x c block averages of log kappa
-    std::vector<float> nin_h((size_t)2 * L->cn, 0.f);
+    std::vector<float> lk_h((size_t)L->cn, 0.f);
     for (int J = 0; J < L->cny; ++J)
         for (int I = 0; I < L->cnx; ++I) {
             double a = 0.0;
             for (int dy = 0; dy < c; ++dy)
                 for (int dx = 0; dx < c; ++dx) a += log_kappa[(size_t)(J * c + dy) * nx + I * c + dx];
-            nin_h[(size_t)L->cn + (size_t)J * L->cnx + I] = (float)(a / ((double)c * c));
+            lk_h[(size_t)J * L->cnx + I] = (float)(a / ((double)c * c));
         }
-    L->nin.upload(nin_h, ctx->stream);
+    L->lk.upload(lk_h, ctx->stream);
     L->nblk_r = ((nx + 63) / 64) * ((L->cny + 3) / 4);
-    L->nblk_e = grid_n(L->n);
+    L->nblk_e = grid_ae(L->n);
     L->s.alloc(L->n);
     L->sc.alloc(L->cn);
     L->scal.alloc(4);
@@ -299,19 +353,21 @@ void learned_prepare(LearnedStage* L) {
     unet_prepare(L->net, 1, L->cny, L->cnx);
 }
 
-template <int C>
+template <int C, bool SELL>
 static void stage_launch(LearnedStage* L, const double* x, double* y, const DevGuard& dg) {
     hipStream_t st = L->ctx->stream;
     const sg_csr* A = L->A;
-    hipLaunchKernelGGL(k_stage_res<C>, dim3((L->nx + 63) / 64, (L->cny + 3) / 4), dim3(kRT), 0, st,
-                       L->nx, L->ny, A->rp.p, A->ci.p, A->v.p, x, y, L->s.p, L->sc.p, L->part_r.p, dg);
+    hipLaunchKernelGGL((k_stage_res<C, SELL>), dim3((L->nx + 63) / 64, (L->cny + 3) / 4), dim3(kRT),
+                       0, st, L->nx, L->ny, A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p,
+                       A->sell_v.p, x, y, L->s.p, L->sc.p, L->part_r.p, dg);
     hipLaunchKernelGGL(k_stage_in, dim3(std::max(1, std::min((L->cn + kRT - 1) / kRT, 256))),
-                       dim3(kRT), 0, st, L->cn, L->nblk_r, L->part_r.p, L->sc.p, L->scal.p,
-                       L->nin.p, dg);
-    unet_forward_dev(L->net, 1, L->cny, L->cnx, L->nin.p, false, dg.flag);
-    hipLaunchKernelGGL(k_stage_ae<C>, dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny, A->rp.p,
-                       A->ci.p, A->v.p, unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
-    hipLaunchKernelGGL(k_stage_final<C>, dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
+                       dim3(kRT), 0, st, L->cn, L->nblk_r, L->part_r.p, L->sc.p, L->lk.p,
+                       L->scal.p, unet_input_buffer(L->net, 1, L->cny, L->cnx), dg);
+    unet_forward_dev(L->net, 1, L->cny, L->cnx, nullptr, false, dg.flag);
+    hipLaunchKernelGGL((k_stage_ae<C, SELL>), dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
+                       A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p,
+                       unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
+    hipLaunchKernelGGL(k_stage_final<C>, dim3(grid_final(L->n)), dim3(kET), 0, st, L->nx, L->ny,
                        L->nblk_e, L->part_e.p, L->scal.p, unet_output(L->net), y, L->min_gain,
                        L->cnt.p, dg);
 }
@@ -323,12 +379,17 @@ void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& g) 
     const bool run_net = (L->calls_host++ % L->every) == 0;
     if (!run_net) {
         hipLaunchKernelGGL(k_count_base, dim3(1), dim3(64), 0, st, L->cnt.p, dg);
-    } else if (L->coarsen == 4) {
-        stage_launch<4>(L, x, y, dg);
-    } else if (L->coarsen == 2) {
-        stage_launch<2>(L, x, y, dg);
     } else {
-        stage_launch<1>(L, x, y, dg);
+        // the SELL-64 copy serves whole wavefronts of 64 grid columns when its
+        // slices are the aligned 64-row blocks of one system
+        const sg_csr* A = L->A;
+        const bool sell = A->nsl > 0 && A->sys_off.size() == 2 && L->nx % 64 == 0;
+        if (L->coarsen == 4)
+            sell ? stage_launch<4, true>(L, x, y, dg) : stage_launch<4, false>(L, x, y, dg);
+        else if (L->coarsen == 2)
+            sell ? stage_launch<2, true>(L, x, y, dg) : stage_launch<2, false>(L, x, y, dg);
+        else
+            sell ? stage_launch<1, true>(L, x, y, dg) : stage_launch<1, false>(L, x, y, dg);
     }
     SG_HIP(hipGetLastError());
 }

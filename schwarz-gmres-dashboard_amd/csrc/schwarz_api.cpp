@@ -293,9 +293,9 @@ std::vector<double> ILU0Precond::lu_vals() const {
 // ---- learned stage ----------------------------------------------------------------
 
 LearnedStage::LearnedStage(DevicePreconditioner& base, std::vector<float> params, int nx, int ny,
-                           std::vector<double> log_kappa, double min_gain, int every)
+                           std::vector<double> log_kappa, double min_gain, int every, int coarsen)
     : base_(base), params_(std::move(params)), logk_(std::move(log_kappa)), nx_(nx), ny_(ny),
-      every_(every), min_gain_(min_gain) {
+      every_(every), coarsen_(coarsen), min_gain_(min_gain) {
     if ((int)params_.size() != sg_unet_nparams())
         throw std::runtime_error("learned stage: wrong number of network parameters");
     if ((long long)logk_.size() != (long long)nx * ny)
@@ -317,7 +317,7 @@ void LearnedStage::setup(const CSRMatrix& A) {
     }
     if (M_) sg_precond_destroy(M_), M_ = nullptr;
     if (A_) sg_csr_destroy(A_), A_ = nullptr;
-    sg_learned_opts o{nx_, ny_, min_gain_, every_};
+    sg_learned_opts o{nx_, ny_, min_gain_, every_, coarsen_};
     check(sg_precond_create_learned(ctx, base_.device_handle(), net_, logk_.data(), &o, &M_), ctx);
     check(sg_csr_create(ctx, A.nrows, A.ncols, A.nnz(), A.row_ptr.data(), A.col_idx.data(),
                         A.vals.data(), &A_),

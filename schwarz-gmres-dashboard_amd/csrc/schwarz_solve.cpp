@@ -65,6 +65,7 @@ void usage() {
            "  --batch <K>              solve K systems together (right-hand sides from seeds seed..seed+K-1)\n"
            "  --learned <w.npy>        learned FGMRES stage on top of --precond (2D --poisson grids)\n"
            "  --learned-every <k>      run the network on every k-th application (default 1)\n"
+           "  --learned-coarsen <c>    the network sees c x c block averages: 1, 2 or 4 (default 1)\n"
            "  --gpu                    accepted; every solve runs on the GPU\n"
            "  --analyze                Lanczos condition estimates, spectral radius and\n"
            "                           convergence metrics (<output>.conv.jsonl)\n"
@@ -156,7 +157,7 @@ int main(int argc, char** argv) {
     MatrixMarketOptions mm;
     std::string matrix_file;
     uint64_t seed = 1;
-    int batch = 0, learned_every = 1;
+    int batch = 0, learned_every = 1, learned_coarsen = 1;
     std::string learned;
     for (int i = 1; i < argc; ++i) {
         auto is = [&](const char* f) { return !std::strcmp(argv[i], f); };
@@ -181,6 +182,7 @@ int main(int argc, char** argv) {
         else if (val("--batch")) batch = std::atoi(argv[++i]);
         else if (val("--learned")) learned = argv[++i];
         else if (val("--learned-every")) learned_every = std::atoi(argv[++i]);
+        else if (val("--learned-coarsen")) learned_coarsen = std::atoi(argv[++i]);
         else if (val("--matrix")) matrix_file = argv[++i];
         else if (is("--fgmres")) use_fgmres = true;
         else if (is("--bicgstab")) use_bicgstab = true;
@@ -344,7 +346,8 @@ int main(int argc, char** argv) {
             if (poisson <= 0 || !D)
                 throw std::runtime_error("--learned needs a 2D --poisson grid and a device --precond");
             stage.reset(new LearnedStage(*D, load_npy_f32(learned), poisson, poisson,
-                                         std::vector<double>((size_t)n, 0.0), 0.01, learned_every));
+                                         std::vector<double>((size_t)n, 0.0), 0.01, learned_every,
+                                         learned_coarsen));
             stage->setup(A);
             use_fgmres = true;  // the stage changes per application
         }

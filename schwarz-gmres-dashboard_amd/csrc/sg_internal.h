@@ -195,6 +195,8 @@ void bicgstab_impl(sg_csr* A, sg_precond* M, const double* d_b, double* d_x,
 void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads,
                       const int* stop);
 const float* unet_output(const sg_unet* U);
+// the bf16 NHWC input [B][H][W][16] that unet_forward_dev(.., d_in = nullptr, ..) reads
+uint16_t* unet_input_buffer(sg_unet* U, int B, int H, int W);
 void unet_prepare(sg_unet* U, int B, int H, int W);  // pack weights, size buffers
 struct LearnedStage;
 LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int nx, int ny,

@@ -97,11 +97,19 @@ __host__ __device__ inline ConvLds conv_lds(int cinp, int cob) {
 // input view holds at least cinp channels, zero beyond cin).
 // Grid: x = B * tilesY * tilesX, y = output-channel blocks of cob (16, 32 or
 // 64); dynamic LDS conv_lds(cinp, cob).bytes.
-template <int MODE>
+//
+// Fusions (forward, MODE 0): channels [0, cup) of the input are read from the
+// half-resolution view `up` at (y/2, x/2) (nearest upsampling of a decoder
+// input, cup a multiple of the 32-channel chunk); EPI 1 also writes the 2x2
+// average pooling of the (bf16-rounded) output to `pout`; EPI 2 also writes
+// the 1x1 head hout = hw[16] + sum_c hw[c] out[c] over the bf16-rounded
+// outputs (cout = 16).  The pooling adds the four outputs of a 2x2 block in
+// row-major order and scales by 1/4 before rounding to bf16.
+template <int MODE, int EPI>
 __global__ void __launch_bounds__(256)
-k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
-          const float* __restrict__ bias, View out, int cout, int H, int W, int cob,
-          const int* __restrict__ stop) {
+k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restrict__ wt,
+          const float* __restrict__ bias, View out, int cout, int H, int W, int cob, View pout,
+          const float* __restrict__ hw, float* __restrict__ hout, const int* __restrict__ stop) {
     if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
     const ConvLds L = conv_lds(cinp, cob);
@@ -126,16 +134,16 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int n = 0; n < 4; ++n) acc[r][p][n] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int c0 = 0; c0 < cinp; c0 += kCC) {
-        const int C = min(kCC, cinp - c0);  // 16 or 32
-        const int oct = C / 8;
-        const int KC = (9 * C + 31) / 32 * 32;
-        __syncthreads();
-        // input chunk with its halo and the chunk's weights: every 16-byte
-        // load of the thread is issued before the first LDS store
-        constexpr int kJI = (kHY * kHX * (kCC / 8) + 255) / 256;
-        constexpr int kJW = (64 * 9 * (kCC / 8) + 255) / 256;
-        uint4 ri[kJI], rw[kJW];
+    // input chunk c0 with its halo and the chunk's weights in registers: every
+    // 16-byte load is issued before the first use; the next chunk's loads
+    // are issued before the current chunk's MFMAs, so they are in flight
+    // while those run
+    constexpr int kJI = (kHY * kHX * (kCC / 8) + 255) / 256;
+    constexpr int kJW = (64 * 9 * (kCC / 8) + 255) / 256;
+    uint4 ri[kJI], rw[kJW];
+    auto load_chunk = [&](int c0) {
+        const int oct = min(kCC, cinp - c0) / 8;
+        const bool from_up = c0 < cup;  // the whole chunk comes from the upsampled view
 #pragma unroll
         for (int jj = 0; jj < kJI; ++jj) {
             const int e = tid + 256 * jj;
@@ -144,7 +152,9 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
                 const int q = e % oct, pix = e / oct;
                 const int gy = y0 + pix / kHX - 1, gx = x0 + pix % kHX - 1;
                 if (c0 + 8 * q < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    v = *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
+                    v = from_up ? *reinterpret_cast<const uint4*>(
+                                      up.p + up.at(b, c0 + 8 * q, gy >> 1, gx >> 1, H >> 1, W >> 1))
+                                : *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
             }
             ri[jj] = v;
         }
@@ -158,6 +168,13 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
             }
             rw[jj] = v;
         }
+    };
+    load_chunk(0);
+    for (int c0 = 0; c0 < cinp; c0 += kCC) {
+        const int C = min(kCC, cinp - c0);  // 16 or 32
+        const int oct = C / 8;
+        const int KC = (9 * C + 31) / 32 * 32;
+        __syncthreads();  // the previous chunk's MFMAs are done with the LDS
 #pragma unroll
         for (int jj = 0; jj < kJI; ++jj) {
             const int e = tid + 256 * jj;
@@ -179,6 +196,7 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
                 uint4{0u, 0u, 0u, 0u};
         }
         __syncthreads();
+        if (c0 + kCC < cinp) load_chunk(c0 + kCC);
         for (int ks = 0; ks < KC; ks += 32) {
             // this lane's 8 K values: one tap, channels ch..ch+7 (C is a multiple of 8)
             const int k = ks + 8 * g;
@@ -209,12 +227,11 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
         }
     }
     // D[co][pixel]: lane holds pixel li, channels 4 g + i of each block
+    uint32_t hv[2][2][4][2];  // bf16 outputs, two per word: [r][p][n][i / 2]
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
-            if (y >= H || x >= W) continue;
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
                 if (n >= ncot) break;
@@ -229,10 +246,59 @@ k_conv3x3(View in, int cin, int cinp, const uint16_t* __restrict__ wt,
                     }
                     h[i] = f2bf(v);
                 }
-                *reinterpret_cast<uint2*>(out.p + out.at(b, co, y, x, H, W)) =
-                    uint2{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16)};
+                hv[r][p][n][0] = (unsigned)h[0] | ((unsigned)h[1] << 16);
+                hv[r][p][n][1] = (unsigned)h[2] | ((unsigned)h[3] << 16);
+                const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
+                if (y < H && x < W)
+                    *reinterpret_cast<uint2*>(out.p + out.at(b, co, y, x, H, W)) =
+                        uint2{hv[r][p][n][0], hv[r][p][n][1]};
             }
-        }
+    auto unpack = [](uint32_t w, int k) { return bf2f((uint16_t)(k ? (w >> 16) : (w & 0xffffu))); };
+    if (EPI == 1) {
+        // rows 2 wv, 2 wv + 1 are in this lane, columns x, x + 1 in lanes li, li ^ 1
+        const int y = y0 + 2 * wv;  // even
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                if (n >= ncot) break;
+                uint16_t o[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float a = unpack(hv[0][p][n][i / 2], i & 1);
+                    const float d = unpack(hv[1][p][n][i / 2], i & 1);
+                    const float c = __shfl_xor(a, 1), f = __shfl_xor(d, 1);
+                    o[i] = f2bf(0.25f * (a + c + d + f));
+                }
+                const int x = x0 + 16 * p + li;
+                if ((li & 1) == 0 && y + 1 < H && x + 1 < W)
+                    *reinterpret_cast<uint2*>(pout.p + pout.at(b, co0 + n * 16 + 4 * g, y >> 1, x >> 1,
+                                                              H >> 1, W >> 1)) =
+                        uint2{(unsigned)o[0] | ((unsigned)o[1] << 16),
+                              (unsigned)o[2] | ((unsigned)o[3] << 16)};
+            }
+    }
+    if (EPI == 2) {
+        // cout = 16: channels 4 g + i of pixel li sit in lane li + 16 g
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                float ch[16];
+#pragma unroll
+                for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        ch[4 * gg + i] = __shfl(unpack(hv[r][p][0][i / 2], i & 1), li + 16 * gg);
+                const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
+                if (g == 0 && y < H && x < W) {
+                    float sum = hw[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) sum += hw[k] * ch[k];
+                    hout[((size_t)b * H + y) * W + x] = sum;
+                }
+            }
+    }
 }
 
 // ---- weight gradient (fp32 vector ALUs) ---------------------------------------
@@ -341,33 +407,6 @@ __global__ void k_input(int B, int H, int W, const float* __restrict__ in, View 
     }
 }
 
-// 2x2 average pooling, eight channels (one 16-byte word) per thread
-__global__ void k_pool(int B, int C, int H, int W, View in, View out, const int* __restrict__ stop) {
-    if (stop && *stop == 0) return;
-    const int h = H / 2, w = W / 2, oc = C / 8;
-    const size_t n = (size_t)B * oc * h * w;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const int q = (int)(e % oc), x = (int)((e / oc) % w), y = (int)((e / ((size_t)oc * w)) % h);
-        const int b = (int)(e / ((size_t)w * h * oc));
-        const uint4 a = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y, 2 * x, H, W));
-        const uint4 c = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y, 2 * x + 1, H, W));
-        const uint4 d = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y + 1, 2 * x, H, W));
-        const uint4 f = *reinterpret_cast<const uint4*>(in.p + in.at(b, 8 * q, 2 * y + 1, 2 * x + 1, H, W));
-        const uint16_t* pa = reinterpret_cast<const uint16_t*>(&a);
-        const uint16_t* pc = reinterpret_cast<const uint16_t*>(&c);
-        const uint16_t* pd = reinterpret_cast<const uint16_t*>(&d);
-        const uint16_t* pf = reinterpret_cast<const uint16_t*>(&f);
-        uint4 o;
-        uint16_t* po = reinterpret_cast<uint16_t*>(&o);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float sum = bf2f(pa[k]) + bf2f(pc[k]) + bf2f(pd[k]) + bf2f(pf[k]);
-            po[k] = f2bf(0.25f * sum);
-        }
-        *reinterpret_cast<uint4*>(out.p + out.at(b, 8 * q, y, x, h, w)) = o;
-    }
-}
-
 // dx (H x W) += 0.25 * dy(y/2, x/2), rounded
 __global__ void k_pool_bwd(int B, int C, int H, int W, View dy, View dx) {
     const size_t n = (size_t)B * C * H * W;
@@ -416,21 +455,6 @@ __global__ void k_relu_mask(int B, int C, int H, int W, View y, View dy) {
         const int c = (int)(e % C), x = (int)((e / C) % W), yy = (int)((e / ((size_t)C * W)) % H);
         const int b = (int)(e / ((size_t)W * H * C));
         if (!(bf2f(y.p[y.at(b, c, yy, x, H, W)]) > 0.f)) dy.p[dy.at(b, c, yy, x, H, W)] = 0;
-    }
-}
-
-// head: out[b][p] = hb + sum_c hw[c] * d0[b][c][p]
-__global__ void k_head(int B, int H, int W, View d0, const float* __restrict__ hw, float* __restrict__ out,
-                       const int* __restrict__ stop) {
-    if (stop && *stop == 0) return;
-    const size_t hwn = (size_t)H * W, n = (size_t)B * hwn;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = e % hwn;
-        const int b = (int)(e / hwn);
-        float s = hw[16];
-        const uint16_t* px = d0.p + (((size_t)b * hwn + p) * d0.ctot + d0.c0);
-        for (int c = 0; c < 16; ++c) s += hw[c] * bf2f(px[c]);
-        out[e] = s;
     }
 }
 
@@ -647,14 +671,15 @@ __global__ void k_head_pack(const float* __restrict__ w, float* __restrict__ hw)
 
 __global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, float lr, float b1, float b2, float eps, float c1,
-                       float c2) {
+                       float c2, float wd) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float gi = g[i];
         const float mi = b1 * m[i] + (1.f - b1) * gi;
         const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
-        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        const float pi = wd > 0.f ? p[i] - lr * wd * p[i] : p[i];
+        p[i] = pi - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     }
 }
 
@@ -684,6 +709,7 @@ struct sg_unet {
     DevBuf<double> lpart, lsums, lloss;
     bool grads_alloc = false;
     int loss_kind = 0;
+    float wd = 0.f;  // decoupled weight decay of the Adam step
 };
 
 namespace sg {
@@ -758,22 +784,29 @@ static int conv_cob(int tiles, int cout) {
     return cob;
 }
 
-static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop) {
+// Forward convolution i; `up`/`cup`: the input's first cup channels read from
+// the half-resolution view up; EPI 1 with pooled output pout, EPI 2 with the head.
+template <int EPI = 0>
+static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop,
+                     View up = View{nullptr, 0, 0}, int cup = 0, View pout = View{nullptr, 0, 0}) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
     const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
     const int cob = conv_cob(tiles, co);
-    hipLaunchKernelGGL(k_conv3x3<0>, dim3(tiles, (co + cob - 1) / cob), dim3(256),
-                       conv_lds(cin_pad(ci), cob).bytes, U->ctx->stream, in, ci, cin_pad(ci),
-                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob, stop);
+    hipLaunchKernelGGL((k_conv3x3<0, EPI>), dim3(tiles, (co + cob - 1) / cob), dim3(256),
+                       conv_lds(cin_pad(ci), cob).bytes, U->ctx->stream, in, up, cup, ci, cin_pad(ci),
+                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob, pout,
+                       (const float*)U->hw.p, U->out.p, stop);
 }
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout;  // dx has ci channels
     const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
     const int cob = conv_cob(tiles, ci);
-    hipLaunchKernelGGL(k_conv3x3<1>, dim3(tiles, (ci + cob - 1) / cob), dim3(256),
-                       conv_lds(co, cob).bytes, U->ctx->stream, dp, co, co, U->wb[i].p,
-                       (const float*)nullptr, dx, ci, H, W, cob, (const int*)nullptr);
+    const View none{nullptr, 0, 0};
+    hipLaunchKernelGGL((k_conv3x3<1, 0>), dim3(tiles, (ci + cob - 1) / cob), dim3(256),
+                       conv_lds(co, cob).bytes, U->ctx->stream, dp, none, 0, co, co, U->wb[i].p,
+                       (const float*)nullptr, dx, ci, H, W, cob, none, (const float*)nullptr,
+                       (float*)nullptr, (const int*)nullptr);
 }
 
 static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
@@ -786,6 +819,11 @@ static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
                        U->g.p + uconv_offset(i));
 }
 
+// d_in == nullptr: the caller has written the bf16 input (unet_input_buffer).
+// Pooling and the head run in the epilogues of the convolutions that produce
+// their inputs; without gradients the decoders read their upsampled channels
+// straight from the half-resolution tensors (training keeps the concatenation
+// buffers, which the backward pass reads).
 void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool grads,
                       const int* stop) {
     unet_pack(U);
@@ -793,36 +831,50 @@ void unet_forward_dev(sg_unet* U, int B, int H, int W, const float* d_in, bool g
     hipStream_t s = U->ctx->stream;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     const size_t hw0 = (size_t)H * W;
-    hipLaunchKernelGGL(k_input, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16), stop);
+    if (d_in)
+        hipLaunchKernelGGL(k_input, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, d_in, V(U->x, 16),
+                           stop);
     conv_fwd(U, 0, V(U->x, 16), V(U->a0, 16), H, W, stop);
-    conv_fwd(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W, stop);  // s0
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 2 * hw0 / 4)), dim3(256), 0, s, B, 16, H, W, V(U->cat0, 48, 32),
-                       V(U->p0, 16), stop);
+    conv_fwd<1>(U, 1, V(U->a0, 16), V(U->cat0, 48, 32), H, W, stop, {}, 0, V(U->p0, 16));  // s0
     conv_fwd(U, 2, V(U->p0, 16), V(U->a2, 32), H1, W1, stop);
-    conv_fwd(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1, stop);  // s1
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 4 * hw0 / 16)), dim3(256), 0, s, B, 32, H1, W1,
-                       V(U->cat1, 96, 64), V(U->p1, 32), stop);
+    conv_fwd<1>(U, 3, V(U->a2, 32), V(U->cat1, 96, 64), H1, W1, stop, {}, 0, V(U->p1, 32));  // s1
     conv_fwd(U, 4, V(U->p1, 32), V(U->a4, 64), H2, W2, stop);
-    conv_fwd(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2, stop);  // s2
-    hipLaunchKernelGGL(k_pool, dim3(grid1(B * 8 * hw0 / 64)), dim3(256), 0, s, B, 64, H2, W2,
-                       V(U->cat2, 128, 64), V(U->p2, 64), stop);
+    conv_fwd<1>(U, 5, V(U->a4, 64), V(U->cat2, 128, 64), H2, W2, stop, {}, 0, V(U->p2, 64));  // s2
     conv_fwd(U, 6, V(U->p2, 64), V(U->a6, 64), H3, W3, stop);
     conv_fwd(U, 7, V(U->a6, 64), V(U->a7, 64), H3, W3, stop);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2, V(U->a7, 64),
-                       V(U->cat2, 128, 0), stop);
-    conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2, stop);
+    if (grads) {
+        hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 16)), dim3(256), 0, s, B, 64, H2, W2,
+                           V(U->a7, 64), V(U->cat2, 128, 0), stop);
+        conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2, stop);
+    } else {
+        conv_fwd(U, 8, V(U->cat2, 128), V(U->a8, 64), H2, W2, stop, V(U->a7, 64), 64);
+    }
     conv_fwd(U, 9, V(U->a8, 64), V(U->a9, 64), H2, W2, stop);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1, V(U->a9, 64),
-                       V(U->cat1, 96, 0), stop);
-    conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1, stop);
+    if (grads) {
+        hipLaunchKernelGGL(k_up, dim3(grid1(B * 8 * hw0 / 4)), dim3(256), 0, s, B, 64, H1, W1,
+                           V(U->a9, 64), V(U->cat1, 96, 0), stop);
+        conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1, stop);
+    } else {
+        conv_fwd(U, 10, V(U->cat1, 96), V(U->a10, 32), H1, W1, stop, V(U->a9, 64), 64);
+    }
     conv_fwd(U, 11, V(U->a10, 32), V(U->a11, 32), H1, W1, stop);
-    hipLaunchKernelGGL(k_up, dim3(grid1(B * 4 * hw0)), dim3(256), 0, s, B, 32, H, W, V(U->a11, 32),
-                       V(U->cat0, 48, 0), stop);
-    conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W, stop);
-    conv_fwd(U, 13, V(U->a12, 16), V(U->a13, 16), H, W, stop);
-    hipLaunchKernelGGL(k_head, dim3(grid1(B * hw0)), dim3(256), 0, s, B, H, W, V(U->a13, 16), U->hw.p,
-                       U->out.p, stop);
+    if (grads) {
+        hipLaunchKernelGGL(k_up, dim3(grid1(B * 4 * hw0)), dim3(256), 0, s, B, 32, H, W,
+                           V(U->a11, 32), V(U->cat0, 48, 0), stop);
+        conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W, stop);
+    } else {
+        conv_fwd(U, 12, V(U->cat0, 48), V(U->a12, 16), H, W, stop, V(U->a11, 32), 32);
+    }
+    conv_fwd<2>(U, 13, V(U->a12, 16), V(U->a13, 16), H, W, stop);  // + head -> out
     SG_HIP(hipGetLastError());
+}
+
+// The bf16 NHWC input tensor [B][H][W][16] of the next forward at this shape
+// (channels 0, 1 the network's two inputs, the rest zero).
+uint16_t* unet_input_buffer(sg_unet* U, int B, int H, int W) {
+    unet_pack(U);
+    unet_shape(U, B, H, W, false);
+    return U->x.p;
 }
 
 void unet_backward_dev(sg_unet* U, const float* d_in, const float* d_target) {
@@ -1011,6 +1063,12 @@ sg_status sg_unet_loss_grad(sg_unet* U, int B, int H, int W, const float* d_in, 
     });
 }
 
+sg_status sg_unet_set_weight_decay(sg_unet* U, float wd) {
+    if (!U || !(wd >= 0.f)) return SG_E_ARG;
+    U->wd = wd;
+    return SG_OK;
+}
+
 sg_status sg_unet_set_loss(sg_unet* U, int kind) {
     if (!U || kind < 0 || kind > 1) return SG_E_ARG;
     U->loss_kind = kind;
@@ -1024,7 +1082,7 @@ sg_status sg_unet_adam_step(sg_unet* U, float lr) {
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, (float)U->adam_t), c2 = 1.f - std::pow(b2, (float)U->adam_t);
         hipLaunchKernelGGL(k_adam, dim3(grid1(U->np)), dim3(256), 0, U->ctx->stream, U->np, U->p.p, U->g.p,
-                           U->m.p, U->v.p, lr, b1, b2, 1e-8f, c1, c2);
+                           U->m.p, U->v.p, lr, b1, b2, 1e-8f, c1, c2, U->wd);
         SG_HIP(hipGetLastError());
         U->packed = false;
     });

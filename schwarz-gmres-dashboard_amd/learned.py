@@ -1,24 +1,30 @@
 """Training data and training of the learned FGMRES stage's network.
 
-Data comes from the plain two-level RAS-FGMRES solve itself.  For each seeded
-field (kappa = exp(g), g a Gaussian random field on a side x side grid, b from
-the same seed) the batched solver runs `per_field` restart cycles of
-`restart` steps from x0 = 0, keeping each iterate x_k and its residual
-r_k = b - A x_k.  Every field shares one unweighted partition of the grid
-(`nparts` subdomains, overlap 2) and FP32 ILU(0) with a Nicolaides coarse
-space.
+Data comes from the plain two-level RAS-FGMRES solve itself.
 
-* mode "residual" (default): the input is what the stage sees at run time,
-  s_k = r_k - A M r_k (the residual left by one two-level step), as
-  (s_k / rms(s_k), g); no target: the network is trained with loss kind 1,
-  the share of |s|^2 that the best multiple of its output fails to remove.
+* mode "krylov" (default, what the shipped weights use): the network runs on
+  the c x c block averages of what the stage sees at run time.  For each
+  seeded field (kappa = exp(g), g a Gaussian random field on a side x side
+  grid, sigma 1.5, correlation length 0.05; b ~ N(0, 1) from the seed) a
+  coefficient-weighted partition into `nparts` subdomains (overlap 2, FP32
+  ILU(0), Nicolaides coarse space) drives FGMRES(30) from x0 = 0 for
+  `cycles` restart cycles.  After the cycles listed in `sample_cycles` the
+  stage's inputs are sampled: Arnoldi vectors v_j of the cycle (j in
+  `columns`) and the residual r = b - A x it ended with; for each such vector
+  u the input is s = u - A M u (the residual a two-level step leaves), as
+  (<s>_c / rms(<s>_c), <g>_c) with <.>_c the c x c block average.  No target:
+  loss kind 1 (the share of |s|^2 that the best multiple of A e fails to
+  remove, A rebuilt from the averaged log kappa channel).
+* mode "residual": the same without averaging, from restart-cycle residuals
+  of the batched solve on a shared unweighted partition.
 * mode "error": the solve continues to ||b - A x|| <= 1e-10 ||b|| for x*;
   input (r_k / rms(r_k), g), target e_k / rms(r_k) with e_k = x* - x_k, loss
   kind 0 (relative squared error).
 
-Training: Adam on batches of whole fields with a cosine learning-rate decay.
+Training: Adam on batches of samples with a cosine learning-rate decay; the
+parameters of the epoch with the lowest validation loss are returned.
 
-    python learned.py --fields 2000 --epochs 8 --out weights.npy
+    python learned.py --fields 1000 --epochs 10 --out weights.npy
 """
 import argparse
 import importlib.util
@@ -26,6 +32,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -121,14 +128,84 @@ def generate_pairs(sg, ctx, P, nfields, side=256, base_seed=10000, per_field=10,
     return X, T
 
 
+def block_average(a, c):
+    """c x c block averages of a 2-D array (sides multiples of c)."""
+    h, w = a.shape
+    return a.reshape(h // c, c, w // c, c).mean(axis=(1, 3)) if c > 1 else a
+
+
+def generate_krylov_pairs(sg, ctx, P, nfields, side=1024, coarsen=4, base_seed=30000,
+                          cycles=6, sample_cycles=(0, 2, 5), columns=(2, 12, 25), restart=30,
+                          sigma=1.5, corr=0.05, nparts=256, log=None, threads=8):
+    """Returns X [N, 2, side/coarsen, side/coarsen] float32 (mode "krylov")."""
+    n = side * side
+    cs = side // coarsen
+
+    def field(seed):
+        g = P.grf(side, sigma, corr, seed)
+        rp, ci, v = P.diffusion2d(np.exp(g))
+        part = sg.partition(rp, ci, nparts, 0, vals=v, weighted=True).astype(np.int32)
+        return g, (rp, ci, v), part, P.normal(seed + 7, n)
+
+    ds = sg.DeviceArray(ctx, n)
+    dz = sg.DeviceArray(ctx, n)
+    out = []
+
+    def sample(A, M, u, lkc):
+        ds.upload(u)
+        M.apply(ds, dz)
+        A.spmv(dz, ds, alpha=-1.0, beta=1.0)  # s = u - A M u
+        sc = block_average(ds.get().reshape(side, side), coarsen)
+        out.append(np.stack([sc / np.sqrt(np.mean(sc * sc)), lkc]).astype(np.float32))
+
+    t0 = time.perf_counter()
+    seeds = list(range(base_seed, base_seed + nfields))
+    with ThreadPoolExecutor(threads) as ex:
+        futs = [ex.submit(field, s) for s in seeds[:threads]]
+        for i in range(nfields):
+            g, mat, part, b = futs[i].result()
+            futs[i] = None
+            if i + threads < nfields:
+                futs.append(ex.submit(field, seeds[i + threads]))
+            A = sg.Csr(ctx, *mat)
+            M = sg.Precond(ctx, "twolevel_ras", nparts=nparts, overlap=2, precision="fp32",
+                           partition=part, coarse_solve=1).setup(A)
+            db = sg.DeviceArray.from_host(ctx, b)
+            dx = sg.DeviceArray(ctx, n)
+            dx.zero()  # x0 = 0: device allocations are not initialised
+            lkc = block_average(g, coarsen).astype(np.float32)
+            for cyc in range(cycles):
+                if cyc in sample_cycles:  # the residual this cycle starts from
+                    dr = sg.DeviceArray.from_host(ctx, b)
+                    A.spmv(dx, dr, alpha=-1.0, beta=1.0)
+                    sample(A, M, dr.get(), lkc)
+                sg.solve(A, db, dx, M, method="fgmres", restart=restart, max_iter=1, tol=0.0,
+                         ortho="cgs2")
+                if cyc in sample_cycles:
+                    V = sg.last_basis(ctx, with_z=False)["V"]
+                    for j in columns:
+                        if j < V.shape[1]:
+                            sample(A, M, V[:, j], lkc)
+                    del V
+            del M, A
+            if log and (i % 50 == 0 or i == nfields - 1):
+                log(f"# fields {i + 1}/{nfields}: {len(out)} samples "
+                    f"({time.perf_counter() - t0:.1f} s)")
+    return np.stack(out)
+
+
 class _Ptr:
     def __init__(self, p):
         self.ptr = p
 
 
-def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0.05, log=None):
+def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0.05, log=None,
+          wd=0.0, keep_best=True, on_epoch=None):
     """Adam on device-resident data (loss kind 1 when T is None); returns
-    (params, history)."""
+    (params, history): the parameters after the epoch with the lowest
+    validation loss (keep_best, and a validation set) or after the last epoch.
+    wd: decoupled weight decay of the Adam steps; on_epoch(epoch, net) runs
+    after each epoch."""
     import ctypes as C
     N, _, H, W = X.shape
     nval = max(1, int(round(val * N))) if N > 1 else 0
@@ -140,6 +217,7 @@ def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0
     dT = sg.DeviceArray.from_host(ctx, T.reshape(-1)) if T is not None else None
     net = sg.UNet(ctx, init_params(seed) if params is None else params)
     net.set_loss(0 if T is not None else 1)
+    net.set_weight_decay(wd)
     xs, ts = 2 * H * W * 4, H * W * 4
 
     def views(i, b):
@@ -152,6 +230,7 @@ def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0
     total = epochs * nb
     hist = []
     step = 0
+    best, best_val = None, np.inf
     t0 = time.perf_counter()
     for ep in range(epochs):
         order = rng.permutation(nb)
@@ -171,21 +250,26 @@ def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0
             vl /= nval
         hist.append({"epoch": ep + 1, "train_loss": tl / nb, "val_loss": vl,
                      "s": round(time.perf_counter() - t0, 2)})
+        if keep_best and nval and vl < best_val:
+            best, best_val = net.get_params(), vl
+        if on_epoch:
+            on_epoch(ep + 1, net)
         if log:
             log(f"# epoch {ep + 1}: train {tl / nb:.4f} val {vl:.4f} "
                 f"({time.perf_counter() - t0:.1f} s)")
-    return net.get_params(), hist
+    return (best if best is not None else net.get_params()), hist
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--fields", type=int, default=2000)
-    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--fields", type=int, default=1000)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--coarsen", type=int, default=4)
     ap.add_argument("--per-field", type=int, default=10)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--mode", default="residual", choices=("residual", "error"))
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--mode", default="krylov", choices=("krylov", "residual", "error"))
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
@@ -195,8 +279,12 @@ def main():
     ctx = sg.Context(0)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field,
-                          log=log, mode=args.mode)
+    if args.mode == "krylov":
+        X, T = generate_krylov_pairs(sg, ctx, P, args.fields, side=args.side,
+                                     coarsen=args.coarsen, log=log), None
+    else:
+        X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field,
+                              log=log, mode=args.mode)
     tgen = time.perf_counter() - t0
     params, hist = train(sg, ctx, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     np.save(args.out, params)

@@ -88,10 +88,10 @@ def test_cli_batch_equals_single_solves(tmp_path):
 
 
 def test_cli_learned_stage():
-    w = os.path.join(PKG, "weights", "unet_grf256.npy")
+    w = os.path.join(PKG, "weights", "unet_grf1024_c4.npy")
     out, lines = _cli("--poisson", "64", "--precond", "twolevel_ras", "--nparts", "8",
                       "--overlap", "2", "--precision", "fp32", "--rtol", "1e-8", "--ortho", "cgs2",
-                      "--learned", w)
+                      "--learned", w, "--learned-coarsen", "4")
     assert out.returncode == 0, out.stderr
     assert "+ learned stage" in out.stdout
     assert lines["Converged"] == "yes"

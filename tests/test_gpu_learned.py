@@ -196,35 +196,39 @@ def test_tiny_training_run_drives_the_stage(sg, ctx, mods):
 
 
 def test_shipped_network_gains_on_held_out_fields(sg, ctx, mods):
-    """Ablation with the shipped weights (trained on fields 10000..11999) on
-    held-out 256^2 fields (seeds 20000..20003, sigma 1.5, l 0.05): mixed
-    two-level RAS(16, overlap 2)-FGMRES(30) to an FP64-verified 1e-8 with and
-    without the learned stage.  The stage must cut the total iteration count;
-    the gain and the time ratio are printed (the network costs more than the
-    steps it saves at this size)."""
+    """Ablation with the shipped weights (trained on samples from the plain
+    solves of fields 30000..30999) on held-out 1024^2 fields (seeds 20000..
+    20002, sigma 1.5, correlation length 0.05), with the 1024^2 workload's
+    solver: mixed two-level RAS(256 subdomains, overlap 2, coefficient-weighted
+    partition)-FGMRES(30) to an FP64-verified 1e-8, with and without the
+    learned stage (network on 4 x 4 block averages).  The stage must cut the
+    total iteration count by more than 40% and the total time; both are
+    printed.  Each solve is timed after an untimed warm-up solve."""
     import time
     P, Lm = mods
     import __graft_entry__
-    params = np.load(os.path.join(__graft_entry__.PKG_DIR, "weights", "unet_grf256.npy"))
-    side = 256
+    params = np.load(os.path.join(__graft_entry__.PKG_DIR, "weights", "unet_grf1024_c4.npy"))
+    side = 1024
     net = sg.UNet(ctx, params)
     kw = dict(method="fgmres", restart=30, max_iter=3000, rtol=1e-8, ortho="cgs2")
     it_plain = it_learn = 0
     t_plain = t_learn = 0.0
-    for seed in range(20000, 20004):
+    for seed in range(20000, 20003):
         rp, ci, v, g, b = grf_problem(P, side, seed)
         A = sg.Csr(ctx, rp, ci, v)
-        part = sg.partition(rp, ci, 16, 0, vals=v, weighted=True)
-        base = sg.Precond(ctx, "twolevel_ras", nparts=16, overlap=2, precision="fp32",
+        part = sg.partition(rp, ci, 256, 0, vals=v, weighted=True)
+        base = sg.Precond(ctx, "twolevel_ras", nparts=256, overlap=2, precision="fp32",
                           partition=part, coarse_solve=1).setup(A)
-        stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side).setup(A)
+        stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, coarsen=4).setup(A)
         db = dev(sg, ctx, b)
+        x = dev(sg, ctx, np.zeros(A.n))
         for M, which in ((base, "plain"), (stage, "learned")):
-            x = dev(sg, ctx, np.zeros(A.n))
-            ctx.sync()
-            t0 = time.perf_counter()
-            r = sg.solve(A, db, x, M, **kw)
-            dt = time.perf_counter() - t0
+            for timed in (False, True):
+                x.zero()
+                ctx.sync()
+                t0 = time.perf_counter()
+                r = sg.solve(A, db, x, M, **kw)
+                dt = time.perf_counter() - t0
             assert r["converged"] and r["rel_true_residual"] <= 1e-8, (seed, which, r)
             if which == "plain":
                 it_plain += r["iterations"]
@@ -233,6 +237,8 @@ def test_shipped_network_gains_on_held_out_fields(sg, ctx, mods):
                 it_learn += r["iterations"]
                 t_learn += dt
     gain = 1.0 - it_learn / it_plain
-    print(f"held-out 256^2 fields: iterations {it_plain} plain, {it_learn} with the learned stage "
-          f"({100 * gain:.1f}% fewer); time {t_plain:.3f} s vs {t_learn:.3f} s")
-    assert it_learn < it_plain
+    print(f"held-out 1024^2 fields: iterations {it_plain} plain, {it_learn} with the learned "
+          f"stage ({100 * gain:.1f}% fewer); time {t_plain:.3f} s vs {t_learn:.3f} s "
+          f"({t_plain / t_learn:.2f}x)")
+    assert it_learn < 0.6 * it_plain
+    assert t_learn < t_plain
