@@ -74,46 +74,50 @@ struct View {
 // ---- 3x3 convolution on the matrix cores ------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int kTX = 32, kTY = 8;          // output tile: 8 rows x 32 pixels
-constexpr int kCC = 32;                   // input channels per staged chunk
 constexpr int kHX = kTX + 2, kHY = kTY + 2;
+
+// Input channels per staged chunk (CC): the whole padded input of every layer
+// of this network (16, 32, 48, 64, 96 or 128) is one chunk; the output
+// channels per workgroup are capped so that the chunk and its weights fit the
+// LDS (64 up to CC = 64, 32 at 96, 16 at 128).
+__host__ __device__ constexpr int cob_max(int cc) { return cc >= 128 ? 16 : (cc >= 96 ? 32 : 64); }
 
 __device__ __forceinline__ v8bf lds_v8(const uint16_t* p) {
     return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(p));
 }
 
-// LDS of one workgroup: the staged input chunk (pixel stride sp = C + 8) and
-// the chunk's weights for cob output channels (row stride swp).
+// LDS of one workgroup: the staged input chunk of C channels (pixel stride
+// sp = C + 8) and the chunk's weights for cob output channels (row stride swp).
 struct ConvLds {
     int sp, swp, bytes;
 };
-__host__ __device__ inline ConvLds conv_lds(int cinp, int cob) {
-    const int C = cinp < kCC ? cinp : kCC;
-    const int sp = C + 8, swp = (9 * C + 31) / 32 * 32 + 8;
-    return ConvLds{sp, swp, 2 * (kHY * kHX * sp + cob * swp)};
+__host__ __device__ constexpr ConvLds conv_lds(int C, int cob) {
+    return ConvLds{C + 8, (9 * C + 31) / 32 * 32 + 8,
+                   2 * (kHY * kHX * (C + 8) + cob * ((9 * C + 31) / 32 * 32 + 8))};
 }
 
 // MODE 0: out = relu(acc + bias); MODE 1: out = acc (data gradient).
 // Weights wt: [9][cout][cinp] bfloat16 (cinp = cin rounded up to 16; the
-// input view holds at least cinp channels, zero beyond cin).
+// input view holds at least cinp channels, zero beyond cin).  CC divides cinp.
 // Grid: x = B * tilesY * tilesX, y = output-channel blocks of cob (16, 32 or
-// 64); dynamic LDS conv_lds(cinp, cob).bytes.
+// 64, at most cob_max(CC)); dynamic LDS conv_lds(CC, cob).bytes.
 //
 // Fusions (forward, MODE 0): channels [0, cup) of the input are read from the
 // half-resolution view `up` at (y/2, x/2) (nearest upsampling of a decoder
-// input, cup a multiple of the 32-channel chunk); EPI 1 also writes the 2x2
+// input, cup a multiple of 8); EPI 1 also writes the 2x2
 // average pooling of the (bf16-rounded) output to `pout`; EPI 2 also writes
 // the 1x1 head hout = hw[16] + sum_c hw[c] out[c] over the bf16-rounded
 // outputs (cout = 16).  The pooling adds the four outputs of a 2x2 block in
 // row-major order and scales by 1/4 before rounding to bf16.
-template <int MODE, int EPI>
+template <int MODE, int EPI, int CC>
 __global__ void __launch_bounds__(256)
 k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restrict__ wt,
           const float* __restrict__ bias, View out, int cout, int H, int W, int cob, View pout,
           const float* __restrict__ hw, float* __restrict__ hout, const int* __restrict__ stop) {
     if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
-    const ConvLds L = conv_lds(cinp, cob);
-    const int kSP = L.sp, kSWP = L.swp;
+    constexpr ConvLds L = conv_lds(CC, cob_max(CC));
+    constexpr int kSP = L.sp, kSWP = L.swp, OCT = CC / 8, KC = (9 * CC + 31) / 32 * 32;
     uint16_t* s_in = lds_u16;
     uint16_t* s_w = lds_u16 + kHY * kHX * kSP;
     const int tilesX = (W + kTX - 1) / kTX, tilesY = (H + kTY - 1) / kTY;
@@ -134,73 +138,65 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int n = 0; n < 4; ++n) acc[r][p][n] = v4f{0.f, 0.f, 0.f, 0.f};
-    // input chunk c0 with its halo and the chunk's weights in registers: every
-    // 16-byte load is issued before the first use; the next chunk's loads
-    // are issued before the current chunk's MFMAs, so they are in flight
-    // while those run
-    constexpr int kJI = (kHY * kHX * (kCC / 8) + 255) / 256;
-    constexpr int kJW = (64 * 9 * (kCC / 8) + 255) / 256;
-    uint4 ri[kJI], rw[kJW];
-    auto load_chunk = [&](int c0) {
-        const int oct = min(kCC, cinp - c0) / 8;
-        const bool from_up = c0 < cup;  // the whole chunk comes from the upsampled view
+    // input chunk c0 with its halo and the chunk's weights, every 16-byte load
+    // issued before the first LDS store; branch-free: every lane loads (an
+    // invalid element from the tensor's first word) and zeroes what lies
+    // outside the image or the channels
+    constexpr int kJI = (kHY * kHX * OCT + 255) / 256;
+    constexpr int kJW = (cob_max(CC) * 9 * OCT + 255) / 256;
+    for (int c0 = 0; c0 < cinp; c0 += CC) {
+        uint4 ri[kJI], rw[kJW];
 #pragma unroll
         for (int jj = 0; jj < kJI; ++jj) {
             const int e = tid + 256 * jj;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (e < kHY * kHX * oct) {
-                const int q = e % oct, pix = e / oct;
-                const int gy = y0 + pix / kHX - 1, gx = x0 + pix % kHX - 1;
-                if (c0 + 8 * q < cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    v = from_up ? *reinterpret_cast<const uint4*>(
-                                      up.p + up.at(b, c0 + 8 * q, gy >> 1, gx >> 1, H >> 1, W >> 1))
-                                : *reinterpret_cast<const uint4*>(in.p + in.at(b, c0 + 8 * q, gy, gx, H, W));
-            }
-            ri[jj] = v;
+            const int q = e % OCT, pix = e / OCT;
+            const int gy = y0 + pix / kHX - 1, gx = x0 + pix % kHX - 1;
+            const int c = c0 + 8 * q;
+            const bool ok = e < kHY * kHX * OCT && c < cin && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            const bool fu = c < cup;  // from the half-resolution view
+            const uint16_t* src = fu ? up.p : in.p;
+            const int sct = fu ? up.ctot : in.ctot, sc0 = fu ? up.c0 : in.c0, sh = fu ? 1 : 0;
+            const size_t off = ok ? (((size_t)b * (H >> sh) + (gy >> sh)) * (W >> sh) + (gx >> sh)) * sct +
+                                        sc0 + c
+                                  : 0;
+            const uint4 v = *reinterpret_cast<const uint4*>(src + off);
+            ri[jj] = ok ? v : uint4{0u, 0u, 0u, 0u};
         }
 #pragma unroll
         for (int jj = 0; jj < kJW; ++jj) {
             const int e = tid + 256 * jj;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (e < ncot * 16 * 9 * oct) {
-                const int q = e % oct, r = e / oct;
-                v = *reinterpret_cast<const uint4*>(wt + ((size_t)(r % 9) * cout + co0 + r / 9) * cinp + c0 + 8 * q);
-            }
-            rw[jj] = v;
+            const int q = e % OCT, r = e / OCT;
+            const bool ok = e < ncot * 16 * 9 * OCT;
+            const size_t off = ok ? ((size_t)(r % 9) * cout + co0 + r / 9) * cinp + c0 + 8 * q : 0;
+            const uint4 v = *reinterpret_cast<const uint4*>(wt + off);
+            rw[jj] = ok ? v : uint4{0u, 0u, 0u, 0u};
         }
-    };
-    load_chunk(0);
-    for (int c0 = 0; c0 < cinp; c0 += kCC) {
-        const int C = min(kCC, cinp - c0);  // 16 or 32
-        const int oct = C / 8;
-        const int KC = (9 * C + 31) / 32 * 32;
         __syncthreads();  // the previous chunk's MFMAs are done with the LDS
 #pragma unroll
         for (int jj = 0; jj < kJI; ++jj) {
             const int e = tid + 256 * jj;
-            if (e < kHY * kHX * oct)
-                *reinterpret_cast<uint4*>(&s_in[(e / oct) * kSP + 8 * (e % oct)]) = ri[jj];
+            if (e < kHY * kHX * OCT)
+                *reinterpret_cast<uint4*>(&s_in[(e / OCT) * kSP + 8 * (e % OCT)]) = ri[jj];
         }
-        // weights: s_w[co][tap * C + ch]; K beyond 9 C is zero
+        // weights: s_w[co][tap * CC + ch]; K beyond 9 CC is zero
 #pragma unroll
         for (int jj = 0; jj < kJW; ++jj) {
             const int e = tid + 256 * jj;
-            if (e < ncot * 16 * 9 * oct) {
-                const int q = e % oct, r = e / oct;
-                *reinterpret_cast<uint4*>(&s_w[(r / 9) * kSWP + (r % 9) * C + 8 * q]) = rw[jj];
+            if (e < ncot * 16 * 9 * OCT) {
+                const int q = e % OCT, r = e / OCT;
+                *reinterpret_cast<uint4*>(&s_w[(r / 9) * kSWP + (r % 9) * CC + 8 * q]) = rw[jj];
             }
         }
-        for (int e = tid; e < ncot * 16 * ((KC - 9 * C) / 8); e += 256) {
-            const int per = (KC - 9 * C) / 8;
-            *reinterpret_cast<uint4*>(&s_w[(e / per) * kSWP + 9 * C + 8 * (e % per)]) =
-                uint4{0u, 0u, 0u, 0u};
-        }
+        constexpr int kPer = (KC - 9 * CC) / 8;
+        if (kPer > 0)
+            for (int e = tid; e < ncot * 16 * kPer; e += 256)
+                *reinterpret_cast<uint4*>(&s_w[(e / kPer) * kSWP + 9 * CC + 8 * (e % kPer)]) =
+                    uint4{0u, 0u, 0u, 0u};
         __syncthreads();
-        if (c0 + kCC < cinp) load_chunk(c0 + kCC);
         for (int ks = 0; ks < KC; ks += 32) {
-            // this lane's 8 K values: one tap, channels ch..ch+7 (C is a multiple of 8)
+            // this lane's 8 K values: one tap, channels ch..ch+7 (CC is a multiple of 8)
             const int k = ks + 8 * g;
-            const int tap = k / C, ch = k - tap * C;
+            const int tap = k / CC, ch = k - tap * CC;
             const bool live = tap < 9;  // the zero-padded tail of K
             const int tc = live ? tap : 8;
             const int ky = tc / 3, kx = tc % 3;
@@ -714,8 +710,15 @@ struct sg_unet {
 
 namespace sg {
 
+template <int MODE, int EPI>
+static void conv_allow_lds();
+
 static void unet_pack(sg_unet* U) {
     if (U->packed) return;
+    conv_allow_lds<0, 0>();
+    conv_allow_lds<0, 1>();
+    conv_allow_lds<0, 2>();
+    conv_allow_lds<1, 0>();
     hipStream_t s = U->ctx->stream;
     for (int i = 0; i < kUConvs; ++i) {
         const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci);
@@ -777,11 +780,67 @@ static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
 static View V(DevBuf<uint16_t>& d, int ctot, int c0 = 0) { return View{d.p, ctot, c0}; }
 
 // Output channels per workgroup: 64, halved (down to 16) while the grid has
-// fewer than 2048 workgroups, so the small deep layers still fill the CUs.
-static int conv_cob(int tiles, int cout) {
+// fewer than 2048 workgroups, so the small deep layers still fill the CUs;
+// at most cob_max(cc).
+static int conv_cob(int tiles, int cout, int cc) {
     int cob = cout >= 64 ? 64 : (cout >= 32 ? 32 : 16);
-    while (cob > 16 && (long long)tiles * ((cout + cob - 1) / cob) < 2048) cob /= 2;
+    while (cob > 16 && ((long long)tiles * ((cout + cob - 1) / cob) < 2048 || cob > cob_max(cc)))
+        cob /= 2;
     return cob;
+}
+
+// Chunk width for a padded input of cinp channels: all of it when it is one
+// of the instantiated widths (every layer of this network), else 16.
+static int conv_cc(int cinp) {
+    for (int cc : {128, 96, 64, 48, 32, 16})
+        if (cinp % cc == 0 && cinp / cc == 1) return cc;
+    return 16;
+}
+
+template <int MODE, int EPI, int CC>
+static void conv_go(dim3 grid, int cob, hipStream_t st, View in, View up, int cup, int ci, int cinp,
+                    const uint16_t* wt, const float* bias, View out, int co, int H, int W, View pout,
+                    const float* hw, float* hout, const int* stop) {
+    hipLaunchKernelGGL((k_conv3x3<MODE, EPI, CC>), grid, dim3(256), conv_lds(CC, cob).bytes, st, in, up,
+                       cup, ci, cinp, wt, bias, out, co, H, W, cob, pout, hw, hout, stop);
+}
+
+template <int MODE, int EPI>
+static void conv_dispatch(int cc, dim3 grid, int cob, hipStream_t st, View in, View up, int cup,
+                          int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
+                          int H, int W, View pout, const float* hw, float* hout, const int* stop) {
+#define SG_CONV_CC(C)                                                                              \
+    case C:                                                                                        \
+        conv_go<MODE, EPI, C>(grid, cob, st, in, up, cup, ci, cinp, wt, bias, out, co, H, W, pout, \
+                              hw, hout, stop);                                                     \
+        break;
+    switch (cc) {
+        SG_CONV_CC(16)
+        SG_CONV_CC(32)
+        SG_CONV_CC(48)
+        SG_CONV_CC(64)
+        SG_CONV_CC(96)
+        SG_CONV_CC(128)
+        default: fail(SG_E_ARG, "convolution: unsupported chunk width");
+    }
+#undef SG_CONV_CC
+}
+
+// LDS beyond the default 64 KB for the wide chunks (once per process and device
+// is enough; repeating it is harmless)
+template <int MODE, int EPI>
+static void conv_allow_lds() {
+#define SG_CONV_LDS(C)                                                                             \
+    SG_HIP(hipFuncSetAttribute((const void*)k_conv3x3<MODE, EPI, C>,                              \
+                               hipFuncAttributeMaxDynamicSharedMemorySize,                        \
+                               conv_lds(C, cob_max(C)).bytes));
+    SG_CONV_LDS(16)
+    SG_CONV_LDS(32)
+    SG_CONV_LDS(48)
+    SG_CONV_LDS(64)
+    SG_CONV_LDS(96)
+    SG_CONV_LDS(128)
+#undef SG_CONV_LDS
 }
 
 // Forward convolution i; `up`/`cup`: the input's first cup channels read from
@@ -789,24 +848,21 @@ static int conv_cob(int tiles, int cout) {
 template <int EPI = 0>
 static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop,
                      View up = View{nullptr, 0, 0}, int cup = 0, View pout = View{nullptr, 0, 0}) {
-    const int ci = kUSpec[i].cin, co = kUSpec[i].cout;
+    const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci), cc = conv_cc(cp);
     const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
-    const int cob = conv_cob(tiles, co);
-    hipLaunchKernelGGL((k_conv3x3<0, EPI>), dim3(tiles, (co + cob - 1) / cob), dim3(256),
-                       conv_lds(cin_pad(ci), cob).bytes, U->ctx->stream, in, up, cup, ci, cin_pad(ci),
-                       U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, cob, pout,
-                       (const float*)U->hw.p, U->out.p, stop);
+    const int cob = conv_cob(tiles, co, cc);
+    conv_dispatch<0, EPI>(cc, dim3(tiles, (co + cob - 1) / cob), cob, U->ctx->stream, in, up, cup, ci,
+                          cp, U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, pout,
+                          U->hw.p, U->out.p, stop);
 }
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
-    const int ci = kUSpec[i].cin, co = kUSpec[i].cout;  // dx has ci channels
+    const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cc = conv_cc(co);  // dx has ci channels
     const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
-    const int cob = conv_cob(tiles, ci);
+    const int cob = conv_cob(tiles, ci, cc);
     const View none{nullptr, 0, 0};
-    hipLaunchKernelGGL((k_conv3x3<1, 0>), dim3(tiles, (ci + cob - 1) / cob), dim3(256),
-                       conv_lds(co, cob).bytes, U->ctx->stream, dp, none, 0, co, co, U->wb[i].p,
-                       (const float*)nullptr, dx, ci, H, W, cob, none, (const float*)nullptr,
-                       (float*)nullptr, (const int*)nullptr);
+    conv_dispatch<1, 0>(cc, dim3(tiles, (ci + cob - 1) / cob), cob, U->ctx->stream, dp, none, 0, co, co,
+                        U->wb[i].p, nullptr, dx, ci, H, W, none, nullptr, nullptr, nullptr);
 }
 
 static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
