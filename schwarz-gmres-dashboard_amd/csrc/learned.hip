@@ -41,6 +41,7 @@ struct LearnedStage {
     int coarsen = 1, cnx = 0, cny = 0, cn = 0;  // the network's grid
     long long calls_host = 0;
     int nblk_r = 0, nblk_e = 0;      // blocks of the residual / A e kernels
+    bool sell = false, tile = false; // SELL-64 rows / 64 x 16 tiles usable (set at setup)
     DevBuf<double> s, sc, scal;      // scal: ss, a, cc, q
     DevBuf<double> part_r, part_e;   // block partials {ss, q}, {a, cc}
     DevBuf<float> lk;                // c x c block averages of log kappa [cny][cnx]
@@ -203,19 +204,44 @@ k_stage_in(int cn, int nblk, const double* __restrict__ part, const double* __re
 }
 
 // P o at fine cell j: cell-centred bilinear interpolation from the cnx x cny
-// net grid, clamped at the edges (C = 1: o itself)
+// net grid, clamped at the edges (C = 1: o itself).  Cell x of the grid sits
+// at (x + 1/2) / C - 1/2 = (2x + 1 - C) / (2C) on the net grid: its floor and
+// fraction come from an integer shift and mask (exact, the same values as the
+// floating-point form); the row of j from a reciprocal with a one-step fix.
 template <int C>
-__device__ inline double interp(const float* __restrict__ o, int j, int nx, int cnx, int cny) {
-    if (C == 1) return (double)o[j];
-    const int x = j % nx, y = j / nx;
-    const double fx = (x + 0.5) / C - 0.5, fy = (y + 0.5) / C - 0.5;
-    int x0 = (int)floor(fx), y0 = (int)floor(fy);
-    const double tx = fx - x0, ty = fy - y0;
+__device__ inline double interp_xy(const float* __restrict__ o, int x, int y, int cnx, int cny) {
+    constexpr int kSh = C == 2 ? 2 : 3;  // log2(2C)
+    const int tx2 = 2 * x + 1 - C, ty2 = 2 * y + 1 - C;
+    int x0 = tx2 >> kSh, y0 = ty2 >> kSh;  // arithmetic shifts: floor
+    const double tx = (double)(tx2 & (2 * C - 1)) * (1.0 / (2 * C));
+    const double ty = (double)(ty2 & (2 * C - 1)) * (1.0 / (2 * C));
     const int x1 = min(max(x0 + 1, 0), cnx - 1), y1 = min(max(y0 + 1, 0), cny - 1);
     x0 = min(max(x0, 0), cnx - 1);
     y0 = min(max(y0, 0), cny - 1);
     return (1.0 - ty) * ((1.0 - tx) * o[y0 * cnx + x0] + tx * o[y0 * cnx + x1]) +
            ty * ((1.0 - tx) * o[y1 * cnx + x0] + tx * o[y1 * cnx + x1]);
+}
+
+// grid coordinates of cell j: the row from a reciprocal with a one-step fix
+__device__ inline void cell_xy(int j, int nx, double rnx, int& x, int& y) {
+    y = (int)((double)j * rnx);
+    x = j - y * nx;
+    if (x < 0) {
+        --y;
+        x += nx;
+    } else if (x >= nx) {
+        ++y;
+        x -= nx;
+    }
+}
+
+template <int C>
+__device__ inline double interp(const float* __restrict__ o, int j, int nx, double rnx, int cnx,
+                                int cny) {
+    if (C == 1) return (double)o[j];
+    int x, y;
+    cell_xy(j, nx, rnx, x, y);
+    return interp_xy<C>(o, x, y, cnx, cny);
 }
 
 // block partials {s.(A e), (A e).(A e)} with e = sqrt(q/cn) P o, row i per thread
@@ -228,11 +254,11 @@ k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ c
            double* __restrict__ part, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     const int n = nx * ny, cnx = nx / C, cny = ny / C;
-    const double r = sqrt(scal[3] / ((double)cnx * cny));
+    const double r = sqrt(scal[3] / ((double)cnx * cny)), rnx = 1.0 / nx;
     double a = 0.0, c = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ae = row_sum<SELL>(i, rp, ci, v, sp, sci, sv, [&](int j) {
-            return r * interp<C>(o, j, nx, cnx, cny);
+            return r * interp<C>(o, j, nx, rnx, cnx, cny);
         });
         a += s[i] * ae;
         c += ae * ae;
@@ -241,6 +267,70 @@ k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ c
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = a;
         part[2 * blockIdx.x + 1] = c;
+    }
+}
+
+// The same partials for a grid whose SELL-64 slices are aligned 64-column
+// rows (nx a multiple of 64, ny of 16): a workgroup takes a 64 x 16 tile,
+// first interpolates e on the tile and its one-cell halo into LDS (once per
+// cell instead of once per matrix entry), then forms A e of its rows from
+// there (columns outside the halo, which a 5-point stencil never has, are
+// interpolated directly).  Each row's sum has the same terms in the same
+// order as k_stage_ae's.
+constexpr int kAX = 64, kAY = 16;
+template <int C>
+__global__ void __launch_bounds__(kET)
+k_stage_ae_tile(int nx, int ny, const int* __restrict__ sp, const int* __restrict__ sci,
+                const double* __restrict__ sv, const float* __restrict__ o,
+                const double* __restrict__ scal, const double* __restrict__ s,
+                double* __restrict__ part, DevGuard guard) {
+    if (guard_stop(guard, 0)) return;
+    constexpr int kLX = kAX + 2, kLY = kAY + 2;
+    __shared__ double et[kLY * kLX];
+    const int cnx = nx / C, cny = ny / C;
+    const double r = sqrt(scal[3] / ((double)cnx * cny)), rnx = 1.0 / nx;
+    const int X0 = blockIdx.x * kAX, Y0 = blockIdx.y * kAY;
+    for (int e = threadIdx.x; e < kLY * kLX; e += blockDim.x) {
+        const int gx = X0 - 1 + e % kLX, gy = Y0 - 1 + e / kLX;
+        double v = 0.0;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+            v = r * (C == 1 ? (double)o[gy * nx + gx] : interp_xy<C>(o, gx, gy, cnx, cny));
+        et[e] = v;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double a = 0.0, c = 0.0;
+    for (int k = 0; k < kAY / 4; ++k) {
+        const int ly = w + 4 * k, gx = X0 + tx, gy = Y0 + ly;
+        const int i = gy * nx + gx;
+        const double ae = row_sum<true>(i, nullptr, nullptr, nullptr, sp, sci, sv, [&](int j) {
+            const int d = j - i;
+            int jx, jy;
+            if (d == 0) {
+                jx = gx, jy = gy;
+            } else if (d == 1 && gx + 1 < nx) {
+                jx = gx + 1, jy = gy;
+            } else if (d == -1 && gx > 0) {
+                jx = gx - 1, jy = gy;
+            } else if (d == nx) {
+                jx = gx, jy = gy + 1;
+            } else if (d == -nx) {
+                jx = gx, jy = gy - 1;
+            } else {
+                cell_xy(j, nx, rnx, jx, jy);
+            }
+            const int lx = jx - X0 + 1, lyy = jy - Y0 + 1;
+            if (lx >= 0 && lx < kLX && lyy >= 0 && lyy < kLY) return et[lyy * kLX + lx];
+            return r * interp<C>(o, j, nx, rnx, cnx, cny);
+        });
+        a += s[i] * ae;
+        c += ae * ae;
+    }
+    block_sum2(a, c);
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        part[2 * b] = a;
+        part[2 * b + 1] = c;
     }
 }
 
@@ -270,9 +360,9 @@ k_stage_final(int nx, int ny, int nblk, const double* __restrict__ part,
     __syncthreads();
     if (!sh_ok) return;
     const int n = nx * ny, cnx = nx / C, cny = ny / C;
-    const double al = sh_al * sqrt(scal[3] / ((double)cnx * cny));
+    const double al = sh_al * sqrt(scal[3] / ((double)cnx * cny)), rnx = 1.0 / nx;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        y[i] += al * interp<C>(o, i, nx, cnx, cny);
+        y[i] += al * interp<C>(o, i, nx, rnx, cnx, cny);
 }
 
 __global__ void k_count_base(int* __restrict__ cnt, DevGuard guard) {
@@ -324,12 +414,12 @@ LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int n
         }
     L->lk.upload(lk_h, ctx->stream);
     L->nblk_r = ((nx + 63) / 64) * ((L->cny + 3) / 4);
-    L->nblk_e = grid_ae(L->n);
     L->s.alloc(L->n);
     L->sc.alloc(L->cn);
     L->scal.alloc(4);
     L->part_r.alloc((size_t)2 * L->nblk_r);
-    L->part_e.alloc((size_t)2 * L->nblk_e);
+    // enough for either A e kernel (the one used is chosen at setup)
+    L->part_e.alloc((size_t)2 * std::max(grid_ae(L->n), (nx / kAX) * (ny / kAY)));
     L->cnt.alloc(2);
     SG_HIP(hipMemsetAsync(L->cnt.p, 0, sizeof(int) * 2, ctx->stream));
     SG_HIP(hipStreamSynchronize(ctx->stream));
@@ -345,6 +435,11 @@ void learned_setup(LearnedStage* L, const sg_csr* A) {
     if (!precond_ready(L->base)) fail(SG_E_STATE, "learned stage: base preconditioner not set up");
     if (precond_n(L->base) != L->n) fail(SG_E_ARG, "learned stage: base preconditioner size mismatch");
     L->A = A;
+    // the SELL-64 copy serves whole wavefronts of 64 grid columns when its
+    // slices are the aligned 64-row blocks of one system
+    L->sell = A->nsl > 0 && A->sys_off.size() == 2 && L->nx % 64 == 0;
+    L->tile = L->sell && L->ny % kAY == 0;
+    L->nblk_e = L->tile ? (L->nx / kAX) * (L->ny / kAY) : grid_ae(L->n);
     learned_prepare(L);
 }
 
@@ -364,9 +459,14 @@ static void stage_launch(LearnedStage* L, const double* x, double* y, const DevG
                        dim3(kRT), 0, st, L->cn, L->nblk_r, L->part_r.p, L->sc.p, L->lk.p,
                        L->scal.p, unet_input_buffer(L->net, 1, L->cny, L->cnx), dg);
     unet_forward_dev(L->net, 1, L->cny, L->cnx, nullptr, false, dg.flag);
-    hipLaunchKernelGGL((k_stage_ae<C, SELL>), dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
-                       A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p,
-                       unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
+    if (SELL && L->tile)
+        hipLaunchKernelGGL((k_stage_ae_tile<C>), dim3(L->nx / kAX, L->ny / kAY), dim3(kET), 0, st,
+                           L->nx, L->ny, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p, unet_output(L->net),
+                           L->scal.p, L->s.p, L->part_e.p, dg);
+    else
+        hipLaunchKernelGGL((k_stage_ae<C, SELL>), dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
+                           A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p,
+                           unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
     hipLaunchKernelGGL(k_stage_final<C>, dim3(grid_final(L->n)), dim3(kET), 0, st, L->nx, L->ny,
                        L->nblk_e, L->part_e.p, L->scal.p, unet_output(L->net), y, L->min_gain,
                        L->cnt.p, dg);
@@ -380,10 +480,7 @@ void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& g) 
     if (!run_net) {
         hipLaunchKernelGGL(k_count_base, dim3(1), dim3(64), 0, st, L->cnt.p, dg);
     } else {
-        // the SELL-64 copy serves whole wavefronts of 64 grid columns when its
-        // slices are the aligned 64-row blocks of one system
-        const sg_csr* A = L->A;
-        const bool sell = A->nsl > 0 && A->sys_off.size() == 2 && L->nx % 64 == 0;
+        const bool sell = L->sell;
         if (L->coarsen == 4)
             sell ? stage_launch<4, true>(L, x, y, dg) : stage_launch<4, false>(L, x, y, dg);
         else if (L->coarsen == 2)

@@ -73,8 +73,8 @@ struct View {
 
 // ---- 3x3 convolution on the matrix cores ------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-constexpr int kTX = 32, kTY = 8;          // output tile: 8 rows x 32 pixels
-constexpr int kHX = kTX + 2, kHY = kTY + 2;
+constexpr int kTX = 32;                   // output tile: 4 RW rows x 32 pixels (RW rows per wave)
+constexpr int kHX = kTX + 2;
 
 // Input channels per staged chunk (CC): the whole padded input of every layer
 // of this network (16, 32, 48, 64, 96 or 128) is one chunk; the output
@@ -91,16 +91,17 @@ __device__ __forceinline__ v8bf lds_v8(const uint16_t* p) {
 struct ConvLds {
     int sp, swp, bytes;
 };
-__host__ __device__ constexpr ConvLds conv_lds(int C, int cob) {
+__host__ __device__ constexpr ConvLds conv_lds(int C, int cob, int rw) {
     return ConvLds{C + 8, (9 * C + 31) / 32 * 32 + 8,
-                   2 * (kHY * kHX * (C + 8) + cob * ((9 * C + 31) / 32 * 32 + 8))};
+                   2 * ((4 * rw + 2) * kHX * (C + 8) + cob * ((9 * C + 31) / 32 * 32 + 8))};
 }
 
 // MODE 0: out = relu(acc + bias); MODE 1: out = acc (data gradient).
 // Weights wt: [9][cout][cinp] bfloat16 (cinp = cin rounded up to 16; the
 // input view holds at least cinp channels, zero beyond cin).  CC divides cinp.
 // Grid: x = B * tilesY * tilesX, y = output-channel blocks of cob (16, 32 or
-// 64, at most cob_max(CC)); dynamic LDS conv_lds(CC, cob).bytes.
+// 64, at most cob_max(CC)); dynamic LDS conv_lds(CC, cob, RW).bytes.  A tile
+// is 4 RW rows x 32 pixels: wavefront w computes rows RW w .. RW w + RW - 1.
 //
 // Fusions (forward, MODE 0): channels [0, cup) of the input are read from the
 // half-resolution view `up` at (y/2, x/2) (nearest upsampling of a decoder
@@ -109,14 +110,16 @@ __host__ __device__ constexpr ConvLds conv_lds(int C, int cob) {
 // the 1x1 head hout = hw[16] + sum_c hw[c] out[c] over the bf16-rounded
 // outputs (cout = 16).  The pooling adds the four outputs of a 2x2 block in
 // row-major order and scales by 1/4 before rounding to bf16.
-template <int MODE, int EPI, int CC>
+template <int MODE, int EPI, int CC, int RW>
 __global__ void __launch_bounds__(256)
 k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restrict__ wt,
           const float* __restrict__ bias, View out, int cout, int H, int W, int cob, View pout,
           const float* __restrict__ hw, float* __restrict__ hout, const int* __restrict__ stop) {
     if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
-    constexpr ConvLds L = conv_lds(CC, cob_max(CC));
+    static_assert(EPI != 1 || RW == 2, "the pooling epilogue needs two rows per wavefront");
+    constexpr int kTY = 4 * RW, kHY = kTY + 2;
+    constexpr ConvLds L = conv_lds(CC, cob_max(CC), RW);
     constexpr int kSP = L.sp, kSWP = L.swp, OCT = CC / 8, KC = (9 * CC + 31) / 32 * 32;
     uint16_t* s_in = lds_u16;
     uint16_t* s_w = lds_u16 + kHY * kHX * kSP;
@@ -131,9 +134,9 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
     const int ncot = min(cob, cout - co0) / 16;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
-    v4f acc[2][2][4];
+    v4f acc[RW][2][4];
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -202,10 +205,10 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
             const int ky = tc / 3, kx = tc % 3;
             v8bf bfr[2][2];
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+            for (int r = 0; r < RW; ++r)
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
-                    const int py = 2 * wv + r + ky, px = 16 * p + li + kx;
+                    const int py = RW * wv + r + ky, px = 16 * p + li + kx;
                     const v8bf v = lds_v8(&s_in[(py * kHX + px) * kSP + (live ? ch : 0)]);
                     bfr[r][p] = live ? v : v8bf{};
                 }
@@ -214,7 +217,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                 if (n >= ncot) break;
                 const v8bf afr = lds_v8(&s_w[(n * 16 + li) * kSWP + k]);
 #pragma unroll
-                for (int r = 0; r < 2; ++r)
+                for (int r = 0; r < RW; ++r)
 #pragma unroll
                     for (int p = 0; p < 2; ++p)
                         acc[r][p][n] =
@@ -223,9 +226,9 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
         }
     }
     // D[co][pixel]: lane holds pixel li, channels 4 g + i of each block
-    uint32_t hv[2][2][4][2];  // bf16 outputs, two per word: [r][p][n][i / 2]
+    uint32_t hv[RW][2][4][2];  // bf16 outputs, two per word: [r][p][n][i / 2]
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -244,7 +247,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                 }
                 hv[r][p][n][0] = (unsigned)h[0] | ((unsigned)h[1] << 16);
                 hv[r][p][n][1] = (unsigned)h[2] | ((unsigned)h[3] << 16);
-                const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
+                const int y = y0 + RW * wv + r, x = x0 + 16 * p + li;
                 if (y < H && x < W)
                     *reinterpret_cast<uint2*>(out.p + out.at(b, co, y, x, H, W)) =
                         uint2{hv[r][p][n][0], hv[r][p][n][1]};
@@ -277,7 +280,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
     if (EPI == 2) {
         // cout = 16: channels 4 g + i of pixel li sit in lane li + 16 g
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < RW; ++r)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 float ch[16];
@@ -286,7 +289,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         ch[4 * gg + i] = __shfl(unpack(hv[r][p][0][i / 2], i & 1), li + 16 * gg);
-                const int y = y0 + 2 * wv + r, x = x0 + 16 * p + li;
+                const int y = y0 + RW * wv + r, x = x0 + 16 * p + li;
                 if (g == 0 && y < H && x < W) {
                     float sum = hw[16];
 #pragma unroll
@@ -797,22 +800,22 @@ static int conv_cc(int cinp) {
     return 16;
 }
 
-template <int MODE, int EPI, int CC>
+template <int MODE, int EPI, int CC, int RW>
 static void conv_go(dim3 grid, int cob, hipStream_t st, View in, View up, int cup, int ci, int cinp,
                     const uint16_t* wt, const float* bias, View out, int co, int H, int W, View pout,
                     const float* hw, float* hout, const int* stop) {
-    hipLaunchKernelGGL((k_conv3x3<MODE, EPI, CC>), grid, dim3(256), conv_lds(CC, cob).bytes, st, in, up,
-                       cup, ci, cinp, wt, bias, out, co, H, W, cob, pout, hw, hout, stop);
+    hipLaunchKernelGGL((k_conv3x3<MODE, EPI, CC, RW>), grid, dim3(256), conv_lds(CC, cob, RW).bytes, st,
+                       in, up, cup, ci, cinp, wt, bias, out, co, H, W, cob, pout, hw, hout, stop);
 }
 
-template <int MODE, int EPI>
-static void conv_dispatch(int cc, dim3 grid, int cob, hipStream_t st, View in, View up, int cup,
-                          int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
-                          int H, int W, View pout, const float* hw, float* hout, const int* stop) {
-#define SG_CONV_CC(C)                                                                              \
-    case C:                                                                                        \
-        conv_go<MODE, EPI, C>(grid, cob, st, in, up, cup, ci, cinp, wt, bias, out, co, H, W, pout, \
-                              hw, hout, stop);                                                     \
+template <int MODE, int EPI, int RW>
+static void conv_dispatch_rw(int cc, dim3 grid, int cob, hipStream_t st, View in, View up, int cup,
+                             int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
+                             int H, int W, View pout, const float* hw, float* hout, const int* stop) {
+#define SG_CONV_CC(C)                                                                                 \
+    case C:                                                                                           \
+        conv_go<MODE, EPI, C, RW>(grid, cob, st, in, up, cup, ci, cinp, wt, bias, out, co, H, W, pout, \
+                                  hw, hout, stop);                                                    \
         break;
     switch (cc) {
         SG_CONV_CC(16)
@@ -828,12 +831,12 @@ static void conv_dispatch(int cc, dim3 grid, int cob, hipStream_t st, View in, V
 
 // LDS beyond the default 64 KB for the wide chunks (once per process and device
 // is enough; repeating it is harmless)
-template <int MODE, int EPI>
-static void conv_allow_lds() {
+template <int MODE, int EPI, int RW>
+static void conv_allow_lds_rw() {
 #define SG_CONV_LDS(C)                                                                             \
-    SG_HIP(hipFuncSetAttribute((const void*)k_conv3x3<MODE, EPI, C>,                              \
+    SG_HIP(hipFuncSetAttribute((const void*)k_conv3x3<MODE, EPI, C, RW>,                          \
                                hipFuncAttributeMaxDynamicSharedMemorySize,                        \
-                               conv_lds(C, cob_max(C)).bytes));
+                               conv_lds(C, cob_max(C), RW).bytes));
     SG_CONV_LDS(16)
     SG_CONV_LDS(32)
     SG_CONV_LDS(48)
@@ -843,26 +846,61 @@ static void conv_allow_lds() {
 #undef SG_CONV_LDS
 }
 
+template <int MODE, int EPI>
+static void conv_allow_lds() {
+    conv_allow_lds_rw<MODE, EPI, 2>();
+    if constexpr (EPI != 1) conv_allow_lds_rw<MODE, EPI, 1>();
+}
+
+// Launch shape of a convolution: rows per wavefront (2, or 1 when the layer
+// has fewer than 512 workgroups of 8-row tiles and no pooling epilogue, so
+// small layers spread over twice the workgroups), tiles and output channels
+// per workgroup.
+struct ConvShape {
+    int rw, tiles, cob;
+};
+static ConvShape conv_shape(int B, int H, int W, int cout, int cc, bool pool) {
+    auto tiles_of = [&](int rw) { return B * ((H + 4 * rw - 1) / (4 * rw)) * ((W + kTX - 1) / kTX); };
+    int t = tiles_of(2), cob = conv_cob(t, cout, cc);
+    if (pool || (long long)t * ((cout + cob - 1) / cob) >= 512) return {2, t, cob};
+    t = tiles_of(1);
+    return {1, t, conv_cob(t, cout, cc)};
+}
+
+template <int MODE, int EPI>
+static void conv_dispatch(const ConvShape& sh, int cc, hipStream_t st, View in, View up, int cup,
+                          int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
+                          int H, int W, View pout, const float* hw, float* hout, const int* stop) {
+    const dim3 grid(sh.tiles, (co + sh.cob - 1) / sh.cob);
+    if constexpr (EPI != 1) {
+        if (sh.rw == 1) {
+            conv_dispatch_rw<MODE, EPI, 1>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out,
+                                           co, H, W, pout, hw, hout, stop);
+            return;
+        }
+    }
+    conv_dispatch_rw<MODE, EPI, 2>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out, co, H,
+                                   W, pout, hw, hout, stop);
+}
+
 // Forward convolution i; `up`/`cup`: the input's first cup channels read from
 // the half-resolution view up; EPI 1 with pooled output pout, EPI 2 with the head.
 template <int EPI = 0>
 static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop,
                      View up = View{nullptr, 0, 0}, int cup = 0, View pout = View{nullptr, 0, 0}) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci), cc = conv_cc(cp);
-    const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
-    const int cob = conv_cob(tiles, co, cc);
-    conv_dispatch<0, EPI>(cc, dim3(tiles, (co + cob - 1) / cob), cob, U->ctx->stream, in, up, cup, ci,
-                          cp, U->wf[i].p, U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, pout,
-                          U->hw.p, U->out.p, stop);
+    const ConvShape sh = conv_shape(U->B, H, W, co, cc, EPI == 1);
+    conv_dispatch<0, EPI>(sh, cc, U->ctx->stream, in, up, cup, ci, cp, U->wf[i].p,
+                          U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, pout, U->hw.p,
+                          U->out.p, stop);
 }
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cc = conv_cc(co);  // dx has ci channels
-    const int tiles = U->B * ((H + kTY - 1) / kTY) * ((W + kTX - 1) / kTX);
-    const int cob = conv_cob(tiles, ci, cc);
+    const ConvShape sh = conv_shape(U->B, H, W, ci, cc, false);
     const View none{nullptr, 0, 0};
-    conv_dispatch<1, 0>(cc, dim3(tiles, (ci + cob - 1) / cob), cob, U->ctx->stream, dp, none, 0, co, co,
-                        U->wb[i].p, nullptr, dx, ci, H, W, none, nullptr, nullptr, nullptr);
+    conv_dispatch<1, 0>(sh, cc, U->ctx->stream, dp, none, 0, co, co, U->wb[i].p, nullptr, dx, ci, H, W,
+                        none, nullptr, nullptr, nullptr);
 }
 
 static void conv_dw(sg_unet* U, int i, View in, View dp, int H, int W) {
