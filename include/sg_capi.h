@@ -359,6 +359,10 @@ sg_status sg_unet_set_params(sg_unet* net, const float* params);
 sg_status sg_unet_get_params(const sg_unet* net, float* params);
 /* device buffers */
 sg_status sg_unet_forward(sg_unet* net, int B, int H, int W, const float* d_in, float* d_out);
+/* loss kind 0: relative squared error against a target; 1: residual loss
+ * 1 - (s.Au)^2 / (|s|^2 |Au|^2) with u the output, A rebuilt from the log kappa
+ * channel (no target); 2: kind 1 plus 0.01 ln(|Au|^2 / |s|^2)^2, which holds
+ * the output's scale */
 sg_status sg_unet_set_loss(sg_unet* net, int kind);
 /* forward + backward; loss and grad (host, nparams floats) may be NULL;
  * d_target may be NULL with loss kind 1 */

@@ -25,6 +25,9 @@
 // operator (harmonic-mean faces 2 k_p k_q / (k_p + k_q), a boundary face adds
 // k_p to the diagonal), u = A out: mean over samples of 1 - (s.u)^2 / (|s|^2 |u|^2),
 // the fraction of |s|^2 that the best multiple of out leaves.
+// Loss kind 2: kind 1 plus mu ln(|u|^2 / |s|^2)^2 per sample (mean), mu =
+// kScaleMu: kind 1 does not see the scale of out, this term holds |A out| near
+// |s| so the scale cannot drift during training.
 //
 // With round_bf16 = 1 the oracle rounds to bfloat16 (round to nearest even)
 // exactly where the GPU stores bfloat16: every weight used by a convolution,
@@ -44,6 +47,7 @@ struct ConvSpec {
 };
 
 constexpr int kNumConv = 14;
+constexpr double kScaleMu = 0.01;  // weight of the scale term of loss kind 2
 const ConvSpec kConvs[kNumConv] = {{2, 16},  {16, 16}, {16, 32}, {32, 32}, {32, 64},
                                    {64, 64}, {64, 64}, {64, 64}, {128, 64}, {64, 64},
                                    {96, 32}, {32, 32}, {48, 16}, {16, 16}};
@@ -336,7 +340,15 @@ double or_unet_loss_grad_kind(const float* params, int B, int H, int W, const fl
             }
             loss += (1.0 - a * a / (ss * c)) / B;
             const double f = -2.0 * a / (ss * c) / B, q = a / c;
-            for (size_t i = 0; i < hw; ++i) g[i] = f * ((double)sb[i] - q * u[i]);
+            // kind 2 adds mu ln(c / ss)^2: it pins the output's scale (|A out| ~ |s|),
+            // to which the kind-1 loss is blind; d/du = 4 mu ln(c / ss) u / c
+            double h = 0.0;
+            if (kind == 2) {
+                const double l = std::log(c / ss);
+                loss += kScaleMu * l * l / B;
+                h = 4.0 * kScaleMu * l / c / B;
+            }
+            for (size_t i = 0; i < hw; ++i) g[i] = f * ((double)sb[i] - q * u[i]) + h * u[i];
             stencil(lk, g.data(), u.data(), H, W);  // dL/dout = A dL/du
             for (size_t i = 0; i < hw; ++i) dout[b * hw + i] = u[i];
         }

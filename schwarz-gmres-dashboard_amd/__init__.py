@@ -321,7 +321,8 @@ class UNet:
         self.ctx.check(lib().sg_unet_forward(self.h, B, H, W, dx.ptr, dy.ptr))
 
     def set_loss(self, kind):
-        """0: relative squared error against a target; 1: residual loss (no target)."""
+        """0: relative squared error against a target; 1: residual loss (no target);
+        2: residual loss plus a term that holds the output's scale."""
         self.ctx.check(lib().sg_unet_set_loss(self.h, int(kind)))
 
     def loss_grad_dev(self, B, H, W, dx, dt, want_grad=False):

@@ -941,8 +941,9 @@ struct Cycle {
             }
         } else {
             double* Zj = M ? Z + (size_t)j * ld : Vj;
-            if (M) precond_apply_launch(M, Vj, Zj, step_guard);
-            spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, step_guard, s);
+            // w = A z_j, unless the preconditioner formed it already (learned stage)
+            if (!(M && precond_apply_az(M, A, Vj, Zj, w, step_guard)))
+                spmv_launch(A, 1.0, Zj, 0.0, nullptr, w, step_guard, s);
         }
         {
             ProfScope ps(ctx, SG_K_ORTHO);

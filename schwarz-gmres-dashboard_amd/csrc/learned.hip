@@ -43,6 +43,7 @@ struct LearnedStage {
     int nblk_r = 0, nblk_e = 0;      // blocks of the residual / A e kernels
     bool sell = false, tile = false; // SELL-64 rows / 64 x 16 tiles usable (set at setup)
     DevBuf<double> s, sc, scal;      // scal: ss, a, cc, q
+    DevBuf<double> ae, azb;          // A e and A z_b, kept when the solver takes A z from the stage
     DevBuf<double> part_r, part_e;   // block partials {ss, q}, {a, cc}
     DevBuf<float> lk;                // c x c block averages of log kappa [cny][cnx]
     DevBuf<int> cnt;                 // applications, accepted corrections
@@ -132,16 +133,17 @@ __device__ inline double row_sum(int i, const int* __restrict__ rp, const int* _
 }
 
 // s = x - A z for the rows of a 64-column x (4 C)-row tile, the C x C block
-// averages sc of s, block partials {s.s, sc.sc}.  Thread (tx, ty): fine
-// column bx * 64 + tx, net-grid row by * 4 + ty (its C fine rows); the C
-// lanes of one net-grid cell add their column sums with shuffles.
+// averages sc of s, block partials {s.s, sc.sc}; azb (optional) keeps A z.
+// Thread (tx, ty): fine column bx * 64 + tx, net-grid row by * 4 + ty (its C
+// fine rows); the C lanes of one net-grid cell add their column sums with
+// shuffles.
 template <int C, bool SELL>
 __global__ void __launch_bounds__(kRT)
 k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ ci,
             const double* __restrict__ v, const int* __restrict__ sp, const int* __restrict__ sci,
             const double* __restrict__ sv, const double* __restrict__ x,
             const double* __restrict__ z, double* __restrict__ s, double* __restrict__ sc,
-            double* __restrict__ part, DevGuard guard) {
+            double* __restrict__ part, double* __restrict__ azb, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int X = blockIdx.x * 64 + tx, Y = blockIdx.y * 4 + ty;
@@ -153,6 +155,7 @@ k_stage_res(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ 
         for (int dy = 0; dy < C; ++dy) {
             const int i = (Y * C + dy) * nx + X;
             const double acc = row_sum<SELL>(i, rp, ci, v, sp, sci, sv, [&](int j) { return z[j]; });
+            if (azb) azb[i] = acc;  // the bits of the solver's own SpMV of z_b
             const double si = x[i] - acc;
             s[i] = si;
             ss += si * si;
@@ -251,7 +254,7 @@ k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ c
            const double* __restrict__ v, const int* __restrict__ sp, const int* __restrict__ sci,
            const double* __restrict__ sv, const float* __restrict__ o,
            const double* __restrict__ scal, const double* __restrict__ s,
-           double* __restrict__ part, DevGuard guard) {
+           double* __restrict__ part, double* __restrict__ ae_out, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     const int n = nx * ny, cnx = nx / C, cny = ny / C;
     const double r = sqrt(scal[3] / ((double)cnx * cny)), rnx = 1.0 / nx;
@@ -260,6 +263,7 @@ k_stage_ae(int nx, int ny, const int* __restrict__ rp, const int* __restrict__ c
         const double ae = row_sum<SELL>(i, rp, ci, v, sp, sci, sv, [&](int j) {
             return r * interp<C>(o, j, nx, rnx, cnx, cny);
         });
+        if (ae_out) ae_out[i] = ae;
         a += s[i] * ae;
         c += ae * ae;
     }
@@ -283,7 +287,7 @@ __global__ void __launch_bounds__(kET)
 k_stage_ae_tile(int nx, int ny, const int* __restrict__ sp, const int* __restrict__ sci,
                 const double* __restrict__ sv, const float* __restrict__ o,
                 const double* __restrict__ scal, const double* __restrict__ s,
-                double* __restrict__ part, DevGuard guard) {
+                double* __restrict__ part, double* __restrict__ ae_out, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     constexpr int kLX = kAX + 2, kLY = kAY + 2;
     __shared__ double et[kLY * kLX];
@@ -323,6 +327,7 @@ k_stage_ae_tile(int nx, int ny, const int* __restrict__ sp, const int* __restric
             if (lx >= 0 && lx < kLX && lyy >= 0 && lyy < kLY) return et[lyy * kLX + lx];
             return r * interp<C>(o, j, nx, rnx, cnx, cny);
         });
+        if (ae_out) ae_out[i] = ae;
         a += s[i] * ae;
         c += ae * ae;
     }
@@ -334,12 +339,15 @@ k_stage_ae_tile(int nx, int ny, const int* __restrict__ sp, const int* __restric
     }
 }
 
-// the acceptance test and z = z_b + (a / cc) e
+// the acceptance test and z = z_b + (a / cc) e; with az, also the solver's next
+// product A z: A z_b + (a / cc) A e by linearity from the stored products, or
+// A z_b itself (the bits of an SpMV of z_b) when the correction is refused
 template <int C>
 __global__ void __launch_bounds__(kET)
 k_stage_final(int nx, int ny, int nblk, const double* __restrict__ part,
               double* __restrict__ scal, const float* __restrict__ o, double* __restrict__ y,
-              double min_gain, int* __restrict__ cnt, DevGuard guard) {
+              double min_gain, int* __restrict__ cnt, const double* __restrict__ azb,
+              const double* __restrict__ ae, double* __restrict__ az, DevGuard guard) {
     if (guard_stop(guard, 0)) return;
     __shared__ double sh_al;
     __shared__ int sh_ok;
@@ -358,8 +366,13 @@ k_stage_final(int nx, int ny, int nblk, const double* __restrict__ part,
         }
     }
     __syncthreads();
-    if (!sh_ok) return;
     const int n = nx * ny, cnx = nx / C, cny = ny / C;
+    if (az) {
+        const double aa = sh_al;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            az[i] = sh_ok ? azb[i] + aa * ae[i] : azb[i];
+    }
+    if (!sh_ok) return;
     const double al = sh_al * sqrt(scal[3] / ((double)cnx * cny)), rnx = 1.0 / nx;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         y[i] += al * interp<C>(o, i, nx, rnx, cnx, cny);
@@ -415,6 +428,8 @@ LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int n
     L->lk.upload(lk_h, ctx->stream);
     L->nblk_r = ((nx + 63) / 64) * ((L->cny + 3) / 4);
     L->s.alloc(L->n);
+    L->ae.alloc(L->n);
+    L->azb.alloc(L->n);
     L->sc.alloc(L->cn);
     L->scal.alloc(4);
     L->part_r.alloc((size_t)2 * L->nblk_r);
@@ -449,12 +464,15 @@ void learned_prepare(LearnedStage* L) {
 }
 
 template <int C, bool SELL>
-static void stage_launch(LearnedStage* L, const double* x, double* y, const DevGuard& dg) {
+static void stage_launch(LearnedStage* L, const double* x, double* y, double* az,
+                         const DevGuard& dg) {
+    double* ae = az ? L->ae.p : nullptr;
+    double* azb = az ? L->azb.p : nullptr;
     hipStream_t st = L->ctx->stream;
     const sg_csr* A = L->A;
     hipLaunchKernelGGL((k_stage_res<C, SELL>), dim3((L->nx + 63) / 64, (L->cny + 3) / 4), dim3(kRT),
                        0, st, L->nx, L->ny, A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p,
-                       A->sell_v.p, x, y, L->s.p, L->sc.p, L->part_r.p, dg);
+                       A->sell_v.p, x, y, L->s.p, L->sc.p, L->part_r.p, azb, dg);
     hipLaunchKernelGGL(k_stage_in, dim3(std::max(1, std::min((L->cn + kRT - 1) / kRT, 256))),
                        dim3(kRT), 0, st, L->cn, L->nblk_r, L->part_r.p, L->sc.p, L->lk.p,
                        L->scal.p, unet_input_buffer(L->net, 1, L->cny, L->cnx), dg);
@@ -462,33 +480,44 @@ static void stage_launch(LearnedStage* L, const double* x, double* y, const DevG
     if (SELL && L->tile)
         hipLaunchKernelGGL((k_stage_ae_tile<C>), dim3(L->nx / kAX, L->ny / kAY), dim3(kET), 0, st,
                            L->nx, L->ny, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p, unet_output(L->net),
-                           L->scal.p, L->s.p, L->part_e.p, dg);
+                           L->scal.p, L->s.p, L->part_e.p, ae, dg);
     else
         hipLaunchKernelGGL((k_stage_ae<C, SELL>), dim3(L->nblk_e), dim3(kET), 0, st, L->nx, L->ny,
                            A->rp.p, A->ci.p, A->v.p, A->sell_ptr.p, A->sell_ci.p, A->sell_v.p,
-                           unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, dg);
+                           unet_output(L->net), L->scal.p, L->s.p, L->part_e.p, ae, dg);
     hipLaunchKernelGGL(k_stage_final<C>, dim3(grid_final(L->n)), dim3(kET), 0, st, L->nx, L->ny,
                        L->nblk_e, L->part_e.p, L->scal.p, unet_output(L->net), y, L->min_gain,
-                       L->cnt.p, dg);
+                       L->cnt.p, azb, ae, az, dg);
 }
 
-void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& g) {
+// One application; with az (and A the stage's own matrix) it also writes
+// az = A z from the stage's products and returns true, false when the caller
+// must form A z itself (base-only applications, another matrix, no az).
+bool learned_apply_az(LearnedStage* L, const sg_csr* A, const double* x, double* y, double* az,
+                      const Guard& g) {
     hipStream_t st = L->ctx->stream;
     precond_apply_launch(L->base, x, y, g);
     const DevGuard dg = dev_guard(g, nullptr);
     const bool run_net = (L->calls_host++ % L->every) == 0;
     if (!run_net) {
         hipLaunchKernelGGL(k_count_base, dim3(1), dim3(64), 0, st, L->cnt.p, dg);
-    } else {
-        const bool sell = L->sell;
-        if (L->coarsen == 4)
-            sell ? stage_launch<4, true>(L, x, y, dg) : stage_launch<4, false>(L, x, y, dg);
-        else if (L->coarsen == 2)
-            sell ? stage_launch<2, true>(L, x, y, dg) : stage_launch<2, false>(L, x, y, dg);
-        else
-            sell ? stage_launch<1, true>(L, x, y, dg) : stage_launch<1, false>(L, x, y, dg);
+        SG_HIP(hipGetLastError());
+        return false;
     }
+    double* w = (az && A == L->A) ? az : nullptr;
+    const bool sell = L->sell;
+    if (L->coarsen == 4)
+        sell ? stage_launch<4, true>(L, x, y, w, dg) : stage_launch<4, false>(L, x, y, w, dg);
+    else if (L->coarsen == 2)
+        sell ? stage_launch<2, true>(L, x, y, w, dg) : stage_launch<2, false>(L, x, y, w, dg);
+    else
+        sell ? stage_launch<1, true>(L, x, y, w, dg) : stage_launch<1, false>(L, x, y, w, dg);
     SG_HIP(hipGetLastError());
+    return w != nullptr;
+}
+
+void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& g) {
+    learned_apply_az(L, nullptr, x, y, nullptr, g);
 }
 
 void learned_counts(const LearnedStage* L, long long* calls, long long* accepted) {

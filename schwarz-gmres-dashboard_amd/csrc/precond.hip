@@ -3099,6 +3099,13 @@ using namespace sg;
 
 namespace sg {
 
+bool precond_apply_az(sg_precond* M, const sg_csr* A, const double* x, double* y, double* az,
+                      const Guard& guard) {
+    if (M->learned) return learned_apply_az(M->learned, A, x, y, az, guard);
+    precond_apply_launch(M, x, y, guard);
+    return false;
+}
+
 void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard) {
     if (M->learned) {
         learned_apply(M->learned, x, y, guard);

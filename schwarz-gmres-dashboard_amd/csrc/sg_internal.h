@@ -204,12 +204,19 @@ LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int n
 void learned_free(LearnedStage* L);
 void learned_setup(LearnedStage* L, const sg_csr* A);
 void learned_apply(LearnedStage* L, const double* x, double* y, const Guard& guard);
+bool learned_apply_az(LearnedStage* L, const sg_csr* A, const double* x, double* y, double* az,
+                      const Guard& guard);
 void learned_prepare(LearnedStage* L);
 void learned_counts(const LearnedStage* L, long long* calls, long long* accepted);
 sg_precond* learned_base(const LearnedStage* L);
 
 // ---- preconditioner apply (precond.hip) ----------------------------------------
 void precond_apply_launch(sg_precond* M, const double* x, double* y, const Guard& guard);
+// y = M x; a preconditioner that forms A y on the way (the learned stage, with
+// A its own matrix) also writes it to az and returns true; otherwise false
+// and az is untouched
+bool precond_apply_az(sg_precond* M, const sg_csr* A, const double* x, double* y, double* az,
+                      const Guard& guard);
 // Maps each subdomain to the system of a block-diagonal batch that holds it
 // (-1 if it spans several) for per-system guards.
 void precond_bind_systems(sg_precond* M, const std::vector<int>& sys_off);

@@ -117,7 +117,6 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
           const float* __restrict__ hw, float* __restrict__ hout, const int* __restrict__ stop) {
     if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
-    static_assert(EPI != 1 || RW == 2, "the pooling epilogue needs two rows per wavefront");
     constexpr int kTY = 4 * RW, kHY = kTY + 2;
     constexpr ConvLds L = conv_lds(CC, cob_max(CC), RW);
     constexpr int kSP = L.sp, kSWP = L.swp, OCT = CC / 8, KC = (9 * CC + 31) / 32 * 32;
@@ -253,7 +252,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                         uint2{hv[r][p][n][0], hv[r][p][n][1]};
             }
     auto unpack = [](uint32_t w, int k) { return bf2f((uint16_t)(k ? (w >> 16) : (w & 0xffffu))); };
-    if (EPI == 1) {
+    if (EPI == 1 && RW == 2) {
         // rows 2 wv, 2 wv + 1 are in this lane, columns x, x + 1 in lanes li, li ^ 1
         const int y = y0 + 2 * wv;  // even
 #pragma unroll
@@ -276,6 +275,43 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                         uint2{(unsigned)o[0] | ((unsigned)o[1] << 16),
                               (unsigned)o[2] | ((unsigned)o[3] << 16)};
             }
+    }
+    if (EPI == 1 && RW == 1) {
+        // the four rows of the tile sit in four wavefronts: the bf16 outputs go
+        // through the LDS (free now), then each thread pools 4 channels of one
+        // output pixel, adding in the same order as above
+        __syncthreads();  // every MFMA operand read of the LDS is done
+        uint16_t* s_o = lds_u16;  // [4 rows][32 pixels][cob channels]
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                if (n >= ncot) break;
+                *reinterpret_cast<uint2*>(&s_o[(wv * kTX + 16 * p + li) * cob + n * 16 + 4 * g]) =
+                    uint2{hv[0][p][n][0], hv[0][p][n][1]};
+            }
+        __syncthreads();
+        const int q4 = ncot * 4;  // 4-channel groups per pixel
+        for (int e = tid; e < 2 * (kTX / 2) * q4; e += 256) {
+            const int cq = e % q4, px = (e / q4) % (kTX / 2), pr = e / (q4 * (kTX / 2));
+            const int y = y0 + 2 * pr, x = x0 + 2 * px;
+            if (y + 1 >= H || x + 1 >= W) continue;
+            auto at = [&](int rr, int xx) {
+                return *reinterpret_cast<const uint2*>(&s_o[((2 * pr + rr) * kTX + 2 * px + xx) * cob + 4 * cq]);
+            };
+            const uint2 A0 = at(0, 0), C0 = at(0, 1), D0 = at(1, 0), F0 = at(1, 1);
+            uint16_t o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t wa = i < 2 ? A0.x : A0.y, wc = i < 2 ? C0.x : C0.y;
+                const uint32_t wd = i < 2 ? D0.x : D0.y, wf = i < 2 ? F0.x : F0.y;
+                const float a = unpack(wa, i & 1), c = unpack(wc, i & 1);
+                const float d = unpack(wd, i & 1), f = unpack(wf, i & 1);
+                o[i] = f2bf(0.25f * (a + c + d + f));
+            }
+            *reinterpret_cast<uint2*>(pout.p + pout.at(b, co0 + 4 * cq, y >> 1, x >> 1, H >> 1, W >> 1)) =
+                uint2{(unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16)};
+        }
     }
     if (EPI == 2) {
         // cout = 16: channels 4 g + i of pixel li sit in lane li + 16 g
@@ -573,9 +609,11 @@ k_rloss_part(int H, int W, const float* __restrict__ in, const float* __restrict
         for (int k = 0; k < 3; ++k) part[((size_t)b * kLB + blockIdx.x) * 3 + k] = r[k][0];
 }
 
-// loss = mean_b (1 - a^2/(ss c)); coef[b] = {f, q}: dL/du = f (s - q u)
-__global__ void k_rloss_fin(int B, const double* __restrict__ part, double* __restrict__ coef,
-                            double* __restrict__ loss) {
+// loss = mean_b (1 - a^2/(ss c)) [+ mu ln(c / ss)^2 for loss kind 2];
+// coef[b] = {f, q, h}: dL/du = f (s - q u) + h u
+constexpr double kScaleMu = 0.01;  // weight of the scale term of loss kind 2
+__global__ void k_rloss_fin(int B, int kind, const double* __restrict__ part,
+                            double* __restrict__ coef, double* __restrict__ loss) {
     if (threadIdx.x != 0) return;
     double l = 0.0;
     for (int b = 0; b < B; ++b) {
@@ -587,8 +625,14 @@ __global__ void k_rloss_fin(int B, const double* __restrict__ part, double* __re
         }
         const bool ok = ss > 0.0 && c > 0.0;
         l += ok ? (1.0 - a * a / (ss * c)) / B : 1.0 / B;
-        coef[2 * b] = ok ? -2.0 * a / (ss * c) / B : 0.0;
-        coef[2 * b + 1] = ok ? a / c : 0.0;
+        coef[3 * b] = ok ? -2.0 * a / (ss * c) / B : 0.0;
+        coef[3 * b + 1] = ok ? a / c : 0.0;
+        coef[3 * b + 2] = 0.0;
+        if (kind == 2 && ok) {
+            const double lg = log(c / ss);
+            l += kScaleMu * lg * lg / B;
+            coef[3 * b + 2] = 4.0 * kScaleMu * lg / c / B;
+        }
     }
     *loss = l;
 }
@@ -600,8 +644,8 @@ __global__ void k_rloss_g(int B, int H, int W, const float* __restrict__ in,
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const size_t p = e % hwn;
         const int b = (int)(e / hwn);
-        const double sv = in[(size_t)b * 2 * hwn + p];
-        g[e] = (float)(coef[2 * b] * (sv - coef[2 * b + 1] * (double)u[e]));
+        const double sv = in[(size_t)b * 2 * hwn + p], uv = u[e];
+        g[e] = (float)(coef[3 * b] * (sv - coef[3 * b + 1] * uv) + coef[3 * b + 2] * uv);
     }
 }
 
@@ -750,7 +794,7 @@ static void unet_shape(sg_unet* U, int B, int H, int W, bool grads) {
         A(U->a12, 16, hw0), A(U->a13, 16, hw0);
         U->out.alloc((size_t)B * hw0);
         U->lpart.alloc((size_t)B * kLB * 3);
-        U->lsums.alloc((size_t)B * 2);
+        U->lsums.alloc((size_t)B * 3);
         U->lloss.alloc(1);
     }
     if (grads && !U->grads_alloc) {
@@ -849,20 +893,19 @@ static void conv_allow_lds_rw() {
 template <int MODE, int EPI>
 static void conv_allow_lds() {
     conv_allow_lds_rw<MODE, EPI, 2>();
-    if constexpr (EPI != 1) conv_allow_lds_rw<MODE, EPI, 1>();
+    conv_allow_lds_rw<MODE, EPI, 1>();
 }
 
 // Launch shape of a convolution: rows per wavefront (2, or 1 when the layer
-// has fewer than 512 workgroups of 8-row tiles and no pooling epilogue, so
-// small layers spread over twice the workgroups), tiles and output channels
-// per workgroup.
+// has fewer than 512 workgroups of 8-row tiles, so small layers spread over
+// twice the workgroups), tiles and output channels per workgroup.
 struct ConvShape {
     int rw, tiles, cob;
 };
-static ConvShape conv_shape(int B, int H, int W, int cout, int cc, bool pool) {
+static ConvShape conv_shape(int B, int H, int W, int cout, int cc) {
     auto tiles_of = [&](int rw) { return B * ((H + 4 * rw - 1) / (4 * rw)) * ((W + kTX - 1) / kTX); };
     int t = tiles_of(2), cob = conv_cob(t, cout, cc);
-    if (pool || (long long)t * ((cout + cob - 1) / cob) >= 512) return {2, t, cob};
+    if ((long long)t * ((cout + cob - 1) / cob) >= 512) return {2, t, cob};
     t = tiles_of(1);
     return {1, t, conv_cob(t, cout, cc)};
 }
@@ -872,12 +915,10 @@ static void conv_dispatch(const ConvShape& sh, int cc, hipStream_t st, View in, 
                           int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
                           int H, int W, View pout, const float* hw, float* hout, const int* stop) {
     const dim3 grid(sh.tiles, (co + sh.cob - 1) / sh.cob);
-    if constexpr (EPI != 1) {
-        if (sh.rw == 1) {
-            conv_dispatch_rw<MODE, EPI, 1>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out,
-                                           co, H, W, pout, hw, hout, stop);
-            return;
-        }
+    if (sh.rw == 1) {
+        conv_dispatch_rw<MODE, EPI, 1>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out, co,
+                                       H, W, pout, hw, hout, stop);
+        return;
     }
     conv_dispatch_rw<MODE, EPI, 2>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out, co, H,
                                    W, pout, hw, hout, stop);
@@ -889,7 +930,7 @@ template <int EPI = 0>
 static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop,
                      View up = View{nullptr, 0, 0}, int cup = 0, View pout = View{nullptr, 0, 0}) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci), cc = conv_cc(cp);
-    const ConvShape sh = conv_shape(U->B, H, W, co, cc, EPI == 1);
+    const ConvShape sh = conv_shape(U->B, H, W, co, cc);
     conv_dispatch<0, EPI>(sh, cc, U->ctx->stream, in, up, cup, ci, cp, U->wf[i].p,
                           U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, pout, U->hw.p,
                           U->out.p, stop);
@@ -897,7 +938,7 @@ static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const i
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cc = conv_cc(co);  // dx has ci channels
-    const ConvShape sh = conv_shape(U->B, H, W, ci, cc, false);
+    const ConvShape sh = conv_shape(U->B, H, W, ci, cc);
     const View none{nullptr, 0, 0};
     conv_dispatch<1, 0>(sh, cc, U->ctx->stream, dp, none, 0, co, co, U->wb[i].p, nullptr, dx, ci, H, W,
                         none, nullptr, nullptr, nullptr);
@@ -986,7 +1027,8 @@ void unet_backward_dev(sg_unet* U, const float* d_in, const float* d_target) {
     } else {
         hipLaunchKernelGGL(k_stencil, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->out.p, U->lu.p);
         hipLaunchKernelGGL(k_rloss_part, dim3(kLB, B), dim3(256), 0, s, H, W, d_in, U->lu.p, U->lpart.p);
-        hipLaunchKernelGGL(k_rloss_fin, dim3(1), dim3(64), 0, s, B, U->lpart.p, U->lsums.p, U->lloss.p);
+        hipLaunchKernelGGL(k_rloss_fin, dim3(1), dim3(64), 0, s, B, U->loss_kind, U->lpart.p, U->lsums.p,
+                           U->lloss.p);
         hipLaunchKernelGGL(k_rloss_g, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->lu.p, U->lsums.p,
                            U->lg.p);
         hipLaunchKernelGGL(k_stencil, dim3(ge), dim3(256), 0, s, B, H, W, d_in, U->lg.p, U->dout.p);
@@ -1164,7 +1206,7 @@ sg_status sg_unet_set_weight_decay(sg_unet* U, float wd) {
 }
 
 sg_status sg_unet_set_loss(sg_unet* U, int kind) {
-    if (!U || kind < 0 || kind > 1) return SG_E_ARG;
+    if (!U || kind < 0 || kind > 2) return SG_E_ARG;
     U->loss_kind = kind;
     return SG_OK;
 }

@@ -200,12 +200,13 @@ class _Ptr:
 
 
 def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0.05, log=None,
-          wd=0.0, keep_best=True, on_epoch=None):
+          wd=0.0, keep_best=True, on_epoch=None, loss_kind=None):
     """Adam on device-resident data (loss kind 1 when T is None); returns
     (params, history): the parameters after the epoch with the lowest
     validation loss (keep_best, and a validation set) or after the last epoch.
     wd: decoupled weight decay of the Adam steps; on_epoch(epoch, net) runs
-    after each epoch."""
+    after each epoch; loss_kind: 2 for the residual loss with the scale term
+    (default: 0 with targets, 1 without)."""
     import ctypes as C
     N, _, H, W = X.shape
     nval = max(1, int(round(val * N))) if N > 1 else 0
@@ -216,7 +217,7 @@ def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0
     dX = sg.DeviceArray.from_host(ctx, X.reshape(-1))
     dT = sg.DeviceArray.from_host(ctx, T.reshape(-1)) if T is not None else None
     net = sg.UNet(ctx, init_params(seed) if params is None else params)
-    net.set_loss(0 if T is not None else 1)
+    net.set_loss(loss_kind if loss_kind is not None else (0 if T is not None else 1))
     net.set_weight_decay(wd)
     xs, ts = 2 * H * W * 4, H * W * 4
 

@@ -261,7 +261,8 @@ def unet_forward(params, x, round_bf16=True):
 
 
 def unet_loss_grad(params, x, t, round_bf16=True, kind=0):
-    """kind 0: relative squared error against t; kind 1: residual loss (t unused)."""
+    """kind 0: relative squared error against t; kind 1: residual loss (t unused);
+    kind 2: kind 1 plus 0.01 ln(|Au|^2/|s|^2)^2 (t unused)."""
     x = np.ascontiguousarray(x, np.float32)
     B, _, H, W = x.shape
     g = np.zeros(unet_nparams())

@@ -57,14 +57,15 @@ def test_gradient_matches_oracle(sg, ctx):
     assert rel(g[head], gref[head]) <= 2e-2
 
 
-def test_residual_loss_gradient_matches_oracle(sg, ctx):
+@pytest.mark.parametrize("kind", [1, 2])
+def test_residual_loss_gradient_matches_oracle(sg, ctx, kind):
     params = init_params(17)
     x, _ = sample(18, B=2, H=32, W=16)
     x[:, 1] *= 0.7
     net = sg.UNet(ctx, params)
-    net.set_loss(1)
+    net.set_loss(kind)
     loss, g = net.loss_grad(x, None)
-    lref, gref = O.unet_loss_grad(params, x, None, round_bf16=True, kind=1)
+    lref, gref = O.unet_loss_grad(params, x, None, round_bf16=True, kind=kind)
     assert abs(loss - lref) <= 2e-2 * lref
     assert rel(g, gref) <= 5e-2, rel(g, gref)
 

@@ -174,3 +174,15 @@ def test_residual_loss_matches_torch_and_scipy_matrix():
     l2, g = O.unet_loss_grad(params, x, None, round_bf16=False, kind=1)
     assert abs(l2 - loss.item()) <= 1e-10 * abs(loss.item())
     assert np.linalg.norm(g - p.grad.numpy()) <= 1e-9 * np.linalg.norm(p.grad.numpy())
+    # kind 2: the same plus 0.01 ln(|u|^2 / |s|^2)^2 per sample (the scale term)
+    p.grad = None
+    out = f(xt)
+    u = torch_stencil(xt[:, 1], out)
+    a = (s * u).sum((1, 2))
+    ss, cc = (s * s).sum((1, 2)), (u * u).sum((1, 2))
+    loss2 = (1 - a * a / (ss * cc) + 0.01 * torch.log(cc / ss) ** 2).mean()
+    loss2.backward()
+    assert abs(torch.log(cc / ss)).min().item() > 0.1  # the scale term is active
+    l3, g3 = O.unet_loss_grad(params, x, None, round_bf16=False, kind=2)
+    assert abs(l3 - loss2.item()) <= 1e-10 * abs(loss2.item())
+    assert np.linalg.norm(g3 - p.grad.numpy()) <= 1e-9 * np.linalg.norm(p.grad.numpy())
