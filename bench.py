@@ -480,31 +480,54 @@ def run_learned(args, sg, P, cfg):
 
 
 def unet_probe(args, sg, P, k):
-    """K forward passes of the learned stage's network on the 1024^2 GRF
-    workload's input (normalised random residual, log kappa) with the shipped
-    weights; prints the mean time per forward and its fraction of the bf16
-    dense MFMA peak (75.2 GFLOP per forward at 1024^2)."""
-    side = 1024
-    g = P.grf(side, 1.5, 0.05, 5)
-    r = P.normal(77, side * side)
-    x = np.stack([r / np.sqrt(np.mean(r * r)), g.ravel()]).astype(np.float32)
+    """The learned stage's network with the shipped weights: K forward passes
+    at 256^2 (the grid it runs on in the 1024^2 workload, coarsen 4) and at
+    1024^2 on that workload's input (normalised random residual, log kappa),
+    and K/4 training steps (forward, residual-loss backward, Adam) on batches
+    of 16 at 256^2; mean time of each and the forward's fraction of the bf16
+    dense MFMA peak."""
     ctx = sg.Context(0)
     wpath = args.weights or os.path.join(__graft_entry__.PKG_DIR, "weights", SHIPPED_WEIGHTS)
     net = sg.UNet(ctx, np.load(wpath))
-    dx = sg.DeviceArray.from_host(ctx, x.ravel())
-    dy = sg.DeviceArray(ctx, side * side, np.float32)
-    for _ in range(3):
-        net.forward_dev(1, side, side, dx, dy)
+    out = {}
+    for side in (256, 1024):
+        g = P.grf(side, 1.5, 0.05, 5)
+        r = P.normal(77, side * side)
+        x = np.stack([r / np.sqrt(np.mean(r * r)), g.ravel()]).astype(np.float32)
+        dx = sg.DeviceArray.from_host(ctx, x.ravel())
+        dy = sg.DeviceArray(ctx, side * side, np.float32)
+        for _ in range(3):
+            net.forward_dev(1, side, side, dx, dy)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            net.forward_dev(1, side, side, dx, dy)
+        ctx.sync()
+        t = (time.perf_counter() - t0) / k
+        flops = unet_flops(side, side)
+        out[f"forward_{side}"] = {"ms": round(t * 1e3, 4), "gflop": round(flops / 1e9, 2),
+                                  "tflop_s": round(flops / t / 1e12, 1),
+                                  "frac_bf16_peak": round(flops / t / 2.5e15, 4)}
+    # training steps: B = 16 samples of 256^2 (residual loss, as learned.py trains)
+    B, side = 16, 256
+    rng = np.random.default_rng(3)
+    xb = np.stack([np.stack([rng.standard_normal((side, side)),
+                             P.grf(side, 1.5, 0.05, 100 + q)]) for q in range(B)]).astype(np.float32)
+    dxb = sg.DeviceArray.from_host(ctx, xb.ravel())
+    net.set_loss(1)
+    kt = max(1, k // 4)
+    for _ in range(2):
+        net.loss_grad_dev(B, side, side, dxb, None)
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(k):
-        net.forward_dev(1, side, side, dx, dy)
+    for _ in range(kt):
+        net.loss_grad_dev(B, side, side, dxb, None)
+        net.adam(1e-6)
     ctx.sync()
-    t = (time.perf_counter() - t0) / k
-    flops = unet_flops(side, side)
-    print(json.dumps({"unet_forward_ms": round(t * 1e3, 4), "forwards": k, "gflop": round(flops / 1e9, 2),
-                      "tflop_s": round(flops / t / 1e12, 1),
-                      "frac_bf16_peak": round(flops / t / 2.5e15, 4)}))
+    t = (time.perf_counter() - t0) / kt
+    out["train_step_16x256"] = {"ms": round(t * 1e3, 3), "steps": kt}
+    out["forwards"] = k
+    print(json.dumps(out))
 
 
 def unet_flops(H, W):
@@ -745,7 +768,8 @@ def main():
                          "Poisson 256^2; c3: BiCGSTAB / analytics on the 160^3 jump case; "
                          "mm: Matrix Market reader); repeatable")
     ap.add_argument("--unet-probe", type=int, default=0,
-                    help="with --config c5: time this many network forwards at 1024^2 and exit")
+                    help="with --config c5: time this many network forwards (256^2 and 1024^2) "
+                         "and a quarter as many training steps, then exit")
     ap.add_argument("--graph", type=int, default=1,
                     help="single systems: 1 (default) = each restart cycle captured and "
                          "replayed as a hipGraph, 0 = launched kernel by kernel")

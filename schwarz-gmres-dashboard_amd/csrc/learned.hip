@@ -386,7 +386,7 @@ __global__ void k_count_base(int* __restrict__ cnt, DevGuard guard) {
 // blocks of the A e kernel (each leaves a partial pair) and of the update
 // kernel (each block adds all of those partials itself)
 static int grid_ae(int n) { return std::max(1, std::min((n + kET - 1) / kET, 1024)); }
-static int grid_final(int n) { return std::max(1, std::min((n + kET - 1) / kET, 256)); }
+static int grid_final(int n) { return std::max(1, std::min((n + kET - 1) / kET, 1024)); }
 
 LearnedStage* learned_new(sg_context* ctx, sg_precond* base, sg_unet* net, int nx, int ny,
                           const double* log_kappa, const sg_learned_opts* o) {
