@@ -93,7 +93,7 @@ struct ConvLds {
 };
 __host__ __device__ constexpr ConvLds conv_lds(int C, int cob, int rw) {
     return ConvLds{C + 8, (9 * C + 31) / 32 * 32 + 8,
-                   2 * ((rw ? 4 * rw + 2 : 3) * kHX * (C + 8) + cob * ((9 * C + 31) / 32 * 32 + 8))};
+                   2 * ((4 * rw + 2) * kHX * (C + 8) + cob * ((9 * C + 31) / 32 * 32 + 8))};
 }
 
 // MODE 0: out = relu(acc + bias); MODE 1: out = acc (data gradient).
@@ -102,9 +102,6 @@ __host__ __device__ constexpr ConvLds conv_lds(int C, int cob, int rw) {
 // Grid: x = B * tilesY * tilesX, y = output-channel blocks of cob (16, 32 or
 // 64, at most cob_max(CC)); dynamic LDS conv_lds(CC, cob, RW).bytes.  A tile
 // is 4 RW rows x 32 pixels: wavefront w computes rows RW w .. RW w + RW - 1.
-// RW = 0 splits K instead: the tile is one row and wavefront w takes every
-// fourth k-step from the w-th; the four partial sums are added in wavefront
-// order through the LDS and wavefront 0 runs the epilogue (no pooling).
 //
 // Fusions (forward, MODE 0): channels [0, cup) of the input are read from the
 // half-resolution view `up` at (y/2, x/2) (nearest upsampling of a decoder
@@ -120,9 +117,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
           const float* __restrict__ hw, float* __restrict__ hout, const int* __restrict__ stop) {
     if (stop && *stop == 0) return;
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_u16[];
-    static_assert(EPI != 1 || RW > 0, "the pooling epilogue needs whole tiles of rows");
-    constexpr int NR = RW ? RW : 1;  // rows per wavefront
-    constexpr int kTY = RW ? 4 * RW : 1, kHY = kTY + 2;
+    constexpr int kTY = 4 * RW, kHY = kTY + 2;
     constexpr ConvLds L = conv_lds(CC, cob_max(CC), RW);
     constexpr int kSP = L.sp, kSWP = L.swp, OCT = CC / 8, KC = (9 * CC + 31) / 32 * 32;
     uint16_t* s_in = lds_u16;
@@ -138,10 +133,9 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
     const int ncot = min(cob, cout - co0) / 16;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, g = lane >> 4;
-    const int rb = RW ? RW * wv : 0;  // this wavefront's first row of the tile
-    v4f acc[NR][2][4];
+    v4f acc[RW][2][4];
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
+    for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -201,7 +195,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                 *reinterpret_cast<uint4*>(&s_w[(e / kPer) * kSWP + 9 * CC + 8 * (e % kPer)]) =
                     uint4{0u, 0u, 0u, 0u};
         __syncthreads();
-        for (int ks = RW ? 0 : 32 * wv; ks < KC; ks += RW ? 32 : 128) {
+        for (int ks = 0; ks < KC; ks += 32) {
             // this lane's 8 K values: one tap, channels ch..ch+7 (CC is a multiple of 8)
             const int k = ks + 8 * g;
             const int tap = k / CC, ch = k - tap * CC;
@@ -210,10 +204,10 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
             const int ky = tc / 3, kx = tc % 3;
             v8bf bfr[2][2];
 #pragma unroll
-            for (int r = 0; r < NR; ++r)
+            for (int r = 0; r < RW; ++r)
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
-                    const int py = rb + r + ky, px = 16 * p + li + kx;
+                    const int py = RW * wv + r + ky, px = 16 * p + li + kx;
                     const v8bf v = lds_v8(&s_in[(py * kHX + px) * kSP + (live ? ch : 0)]);
                     bfr[r][p] = live ? v : v8bf{};
                 }
@@ -222,7 +216,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                 if (n >= ncot) break;
                 const v8bf afr = lds_v8(&s_w[(n * 16 + li) * kSWP + k]);
 #pragma unroll
-                for (int r = 0; r < NR; ++r)
+                for (int r = 0; r < RW; ++r)
 #pragma unroll
                     for (int p = 0; p < 2; ++p)
                         acc[r][p][n] =
@@ -230,35 +224,10 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
             }
         }
     }
-    if constexpr (RW == 0) {
-        // add the four wavefronts' partial sums in wavefront order
-        __syncthreads();  // every MFMA operand read of the LDS is done
-        // [wavefront 1..3][p][n < ncot][lane]: 6 KB per 16 output channels, within
-        // the chunk's LDS for every (CC, cob)
-        v4f* red = reinterpret_cast<v4f*>(lds_u16);
-        if (wv > 0) {
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    if (n >= ncot) break;
-                    red[(((wv - 1) * 2 + p) * ncot + n) * 64 + lane] = acc[0][p][n];
-                }
-        }
-        __syncthreads();
-        if (wv > 0) return;
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int n = 0; n < 4; ++n) {
-                if (n >= ncot) break;
-                for (int w = 1; w < 4; ++w) acc[0][p][n] += red[(((w - 1) * 2 + p) * ncot + n) * 64 + lane];
-            }
-    }
     // D[co][pixel]: lane holds pixel li, channels 4 g + i of each block
-    uint32_t hv[NR][2][4][2];  // bf16 outputs, two per word: [r][p][n][i / 2]
+    uint32_t hv[RW][2][4][2];  // bf16 outputs, two per word: [r][p][n][i / 2]
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
+    for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -277,7 +246,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
                 }
                 hv[r][p][n][0] = (unsigned)h[0] | ((unsigned)h[1] << 16);
                 hv[r][p][n][1] = (unsigned)h[2] | ((unsigned)h[3] << 16);
-                const int y = y0 + rb + r, x = x0 + 16 * p + li;
+                const int y = y0 + RW * wv + r, x = x0 + 16 * p + li;
                 if (y < H && x < W)
                     *reinterpret_cast<uint2*>(out.p + out.at(b, co, y, x, H, W)) =
                         uint2{hv[r][p][n][0], hv[r][p][n][1]};
@@ -347,7 +316,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
     if (EPI == 2) {
         // cout = 16: channels 4 g + i of pixel li sit in lane li + 16 g
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
+        for (int r = 0; r < RW; ++r)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 float ch[16];
@@ -356,7 +325,7 @@ k_conv3x3(View in, View up, int cup, int cin, int cinp, const uint16_t* __restri
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         ch[4 * gg + i] = __shfl(unpack(hv[r][p][0][i / 2], i & 1), li + 16 * gg);
-                const int y = y0 + rb + r, x = x0 + 16 * p + li;
+                const int y = y0 + RW * wv + r, x = x0 + 16 * p + li;
                 if (g == 0 && y < H && x < W) {
                     float sum = hw[16];
 #pragma unroll
@@ -925,31 +894,20 @@ template <int MODE, int EPI>
 static void conv_allow_lds() {
     conv_allow_lds_rw<MODE, EPI, 2>();
     conv_allow_lds_rw<MODE, EPI, 1>();
-    if constexpr (EPI != 1) conv_allow_lds_rw<MODE, EPI, 0>();
 }
 
 // Launch shape of a convolution: rows per wavefront (2, or 1 when the layer
 // has fewer than 512 workgroups of 8-row tiles, so small layers spread over
-// twice the workgroups, or 0: split K), tiles and output channels per workgroup.
+// twice the workgroups), tiles and output channels per workgroup.
 struct ConvShape {
     int rw, tiles, cob;
 };
-static ConvShape conv_shape(int B, int H, int W, int cout, int cc, bool pool) {
-    auto tiles_of = [&](int rw) {
-        const int ty = rw ? 4 * rw : 1;
-        return B * ((H + ty - 1) / ty) * ((W + kTX - 1) / kTX);
-    };
+static ConvShape conv_shape(int B, int H, int W, int cout, int cc) {
+    auto tiles_of = [&](int rw) { return B * ((H + 4 * rw - 1) / (4 * rw)) * ((W + kTX - 1) / kTX); };
     int t = tiles_of(2), cob = conv_cob(t, cout, cc);
     if ((long long)t * ((cout + cob - 1) / cob) >= 512) return {2, t, cob};
     t = tiles_of(1);
-    cob = conv_cob(t, cout, cc);
-    // deep inputs (K of 18 or more k-steps) on small grids: split K over the
-    // wavefronts of one-row tiles
-    if (!pool && cc >= 64 && (long long)t * ((cout + cob - 1) / cob) < 1024) {
-        t = tiles_of(0);
-        return {0, t, conv_cob(t, cout, cc)};
-    }
-    return {1, t, cob};
+    return {1, t, conv_cob(t, cout, cc)};
 }
 
 template <int MODE, int EPI>
@@ -957,13 +915,6 @@ static void conv_dispatch(const ConvShape& sh, int cc, hipStream_t st, View in, 
                           int ci, int cinp, const uint16_t* wt, const float* bias, View out, int co,
                           int H, int W, View pout, const float* hw, float* hout, const int* stop) {
     const dim3 grid(sh.tiles, (co + sh.cob - 1) / sh.cob);
-    if constexpr (EPI != 1) {
-        if (sh.rw == 0) {
-            conv_dispatch_rw<MODE, EPI, 0>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out,
-                                           co, H, W, pout, hw, hout, stop);
-            return;
-        }
-    }
     if (sh.rw == 1) {
         conv_dispatch_rw<MODE, EPI, 1>(cc, grid, sh.cob, st, in, up, cup, ci, cinp, wt, bias, out, co,
                                        H, W, pout, hw, hout, stop);
@@ -979,7 +930,7 @@ template <int EPI = 0>
 static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const int* stop,
                      View up = View{nullptr, 0, 0}, int cup = 0, View pout = View{nullptr, 0, 0}) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cp = cin_pad(ci), cc = conv_cc(cp);
-    const ConvShape sh = conv_shape(U->B, H, W, co, cc, EPI == 1);
+    const ConvShape sh = conv_shape(U->B, H, W, co, cc);
     conv_dispatch<0, EPI>(sh, cc, U->ctx->stream, in, up, cup, ci, cp, U->wf[i].p,
                           U->p.p + uconv_offset(i) + co * ci * 9, out, co, H, W, pout, U->hw.p,
                           U->out.p, stop);
@@ -987,7 +938,7 @@ static void conv_fwd(sg_unet* U, int i, View in, View out, int H, int W, const i
 
 static void conv_dx(sg_unet* U, int i, View dp, View dx, int H, int W) {
     const int ci = kUSpec[i].cin, co = kUSpec[i].cout, cc = conv_cc(co);  // dx has ci channels
-    const ConvShape sh = conv_shape(U->B, H, W, ci, cc, false);
+    const ConvShape sh = conv_shape(U->B, H, W, ci, cc);
     const View none{nullptr, 0, 0};
     conv_dispatch<1, 0>(sh, cc, U->ctx->stream, dp, none, 0, co, co, U->wb[i].p, nullptr, dx, ci, H, W,
                         none, nullptr, nullptr, nullptr);
