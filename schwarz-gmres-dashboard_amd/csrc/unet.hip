@@ -7,17 +7,20 @@
 // Storage: activations and activation gradients in bfloat16, NHWC (the
 // channels of a pixel contiguous, so a 16-byte load carries 8 channels); a
 // decoder's concatenation buffer holds the upsampled and the skip channels of
-// each pixel side by side.  Weights as an fp32 master copy plus bfloat16
-// copies laid out for the kernels; every sum in fp32.
+// each pixel side by side (training; inference reads the upsampled half
+// straight from the half-resolution tensor).  Weights as an fp32 master copy
+// plus bfloat16 copies laid out for the kernels; every sum in fp32.
 //
 // 3x3 convolutions (forward, and the data gradient as a convolution with the
 // flipped, transposed weights) are implicit GEMMs on the matrix cores with
 // v_mfma_f32_16x16x32_bf16: A = weights (16 output channels x 32 of K), B =
 // input (32 of K x 16 pixels of one image row), K running over (tap, input
-// channel) of a staged chunk.  A workgroup computes an 8 x 32 pixel tile for
-// up to 64 output channels (each wavefront two rows: 2 x 2 pixel blocks x 4
-// channel blocks of accumulators); input chunks of 32 channels (with the
-// one-pixel halo) and their weights are staged in LDS with 16-byte loads.
+// channel) of a staged chunk.  A workgroup computes a 4 RW x 32 pixel tile
+// for up to 64 output channels (each wavefront RW = 1 or 2 rows: RW x 2 pixel
+// blocks x 4 channel blocks of accumulators); the layer's whole padded input
+// (16..128 channels, with the one-pixel halo) and its weights are staged in
+// LDS with 16-byte loads as one chunk.  2x2 pooling and the 1x1 head run in
+// the epilogues of the convolutions that feed them.
 // The weight gradient runs on the vector ALUs in fp32 (each thread one
 // (output, input) channel pair, a 3x3 window slid along the tile rows) and
 // writes per-tile partial sums that a second kernel adds in a fixed order.

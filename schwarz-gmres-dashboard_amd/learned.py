@@ -21,10 +21,13 @@ Data comes from the plain two-level RAS-FGMRES solve itself.
   input (r_k / rms(r_k), g), target e_k / rms(r_k) with e_k = x* - x_k, loss
   kind 0 (relative squared error).
 
-Training: Adam on batches of samples with a cosine learning-rate decay; the
-parameters of the epoch with the lowest validation loss are returned.
+Training: Adam (lr 1e-3) on batches of 16 samples with a cosine learning-rate
+decay over 24 epochs, stopped after epoch 12 (the shipped weights: 1,400
+fields, 16,800 samples); the residual loss is a poor guide to the solver (it
+bottoms out after a few epochs while the iteration counts keep falling), so
+the stopping epoch was chosen by solving held-out fields.
 
-    python learned.py --fields 1000 --epochs 10 --out weights.npy
+    python learned.py --out weights.npy
 """
 import argparse
 import importlib.util
@@ -263,13 +266,16 @@ def train(sg, ctx, X, T, epochs=8, batch=16, lr=1e-3, seed=0, params=None, val=0
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--fields", type=int, default=1000)
+    ap.add_argument("--fields", type=int, default=1400)
     ap.add_argument("--side", type=int, default=1024)
     ap.add_argument("--coarsen", type=int, default=4)
     ap.add_argument("--per-field", type=int, default=10)
-    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=24, help="length of the cosine schedule")
+    ap.add_argument("--stop-epoch", type=int, default=12,
+                    help="keep the parameters after this epoch and stop (0: run all)")
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mode", default="krylov", choices=("krylov", "residual", "error"))
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
@@ -287,9 +293,27 @@ def main():
         X, T = generate_pairs(sg, ctx, P, args.fields, side=args.side, per_field=args.per_field,
                               log=log, mode=args.mode)
     tgen = time.perf_counter() - t0
-    params, hist = train(sg, ctx, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+
+    class _Stop(Exception):
+        pass
+
+    kept = {}
+
+    def at_epoch(ep, net):
+        if args.stop_epoch and ep >= args.stop_epoch:
+            kept["params"] = net.get_params()
+            raise _Stop()
+
+    hist = None
+    try:
+        params, hist = train(sg, ctx, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                             seed=args.seed, log=log, keep_best=False, on_epoch=at_epoch)
+    except _Stop:
+        params = kept["params"]
     np.save(args.out, params)
-    print(json.dumps({"pairs": int(X.shape[0]), "generate_s": round(tgen, 1), "history": hist}))
+    print(json.dumps({"pairs": int(X.shape[0]), "generate_s": round(tgen, 1),
+                      "epochs_run": args.stop_epoch or args.epochs,
+                      "history": hist}))
 
 
 if __name__ == "__main__":
