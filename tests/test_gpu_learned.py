@@ -119,8 +119,18 @@ def _interp(o, c):
     return (1 - ty)[:, None] * top + ty[:, None] * bot
 
 
+def grf_grid(P, nx, ny, seed):
+    """A GRF diffusion problem on an nx x ny grid (a window of a square field)."""
+    g = P.grf(max(nx, ny), 1.5, 0.05, seed)[:ny, :nx]
+    rp, ci, v = P.diffusion2d(np.exp(g))
+    return rp, ci, v, g
+
+
+# (nx, ny): 64 x 64 and 128 x 96 run the tiled A e kernel on the SELL-64 copy,
+# 96 x 64 the generic row kernels (nx not a multiple of 64)
+@pytest.mark.parametrize("nx,ny", [(64, 64), (128, 96), (96, 64)])
 @pytest.mark.parametrize("coarsen", [1, 2, 4])
-def test_stage_matches_its_definition(sg, ctx, mods, coarsen):
+def test_stage_matches_its_definition(sg, ctx, mods, coarsen, nx, ny):
     """One application against a numpy restatement of the stage: s = x - A z_b,
     its c x c block averages sc normalised by their rms (plus the averaged
     log kappa) into the network, e = rms * bilinear(net output), z = z_b +
@@ -128,13 +138,12 @@ def test_stage_matches_its_definition(sg, ctx, mods, coarsen):
     own forward on the restated input."""
     from scipy.sparse import csr_matrix
     P, Lm = mods
-    side = 64
-    rp, ci, v, g, _ = grf_problem(P, side, 24)
+    rp, ci, v, g = grf_grid(P, nx, ny, 24)
     A = sg.Csr(ctx, rp, ci, v)
     As = csr_matrix((v, ci, rp), shape=(A.n, A.n))
     base = sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=2, precision="fp32").setup(A)
     net = sg.UNet(ctx, Lm.init_params(6))
-    stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, min_gain=1e-12,
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), nx, ny, min_gain=1e-12,
                             coarsen=coarsen).setup(A)
     rng = np.random.default_rng(11 + coarsen)
     for trial in range(2):
@@ -144,7 +153,7 @@ def test_stage_matches_its_definition(sg, ctx, mods, coarsen):
         stage.apply(dev(sg, ctx, x), z)
         zb = zb.get()
         s = x - As @ zb
-        sc = _pool(s.reshape(side, side), coarsen)
+        sc = _pool(s.reshape(ny, nx), coarsen)
         q = float((sc * sc).sum())
         rms = np.sqrt(q / sc.size)
         nin = np.stack([sc / rms, _pool(g, coarsen)])[None].astype(np.float32)
@@ -156,6 +165,38 @@ def test_stage_matches_its_definition(sg, ctx, mods, coarsen):
         assert np.linalg.norm(corr - alpha * e) <= 1e-4 * np.linalg.norm(alpha * e), trial
     calls, acc = stage.counts()
     assert calls == 2 and acc == 2
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 96), (96, 64)])
+def test_stage_hands_fgmres_a_consistent_product(sg, ctx, mods, nx, ny):
+    """Inside FGMRES the stage returns w = A z by linearity (A z_b + alpha A e)
+    instead of the step's SpMV: the last cycle's basis must still satisfy the
+    Arnoldi relation A Z = V Hbar to rounding."""
+    from scipy.sparse import csr_matrix
+    P, Lm = mods
+    rp, ci, v, g = grf_grid(P, nx, ny, 25)
+    A = sg.Csr(ctx, rp, ci, v)
+    As = csr_matrix((v, ci, rp), shape=(A.n, A.n))
+    base = sg.Precond(ctx, "twolevel_ras", nparts=4, overlap=2, precision="fp32").setup(A)
+    net = sg.UNet(ctx, Lm.init_params(7))
+    stage = sg.LearnedStage(ctx, base, net, g.ravel(), nx, ny, min_gain=1e-12, coarsen=4).setup(A)
+    b = np.random.default_rng(3).standard_normal(A.n)
+    x = dev(sg, ctx, np.zeros(A.n))
+    r = sg.solve(A, dev(sg, ctx, b), x, stage, method="fgmres", restart=12, max_iter=1,
+                 tol=1e-300, ortho="cgs2")
+    B = sg.last_basis(ctx)
+    k, kv = B["k"], B["kv"]
+    assert r["iterations"] == 12 and k == 12 and kv == 12  # one full cycle
+    V, Z, H = B["V"], B["Z"], B["Hbar"]
+    AZ = As @ Z
+    # |A|_2 <= sqrt(|A|_1 |A|_inf)
+    Aabs = abs(As)
+    norm2 = np.sqrt(Aabs.sum(axis=0).max() * Aabs.sum(axis=1).max())
+    for j in range(kv - 1):
+        res = AZ[:, j] - V[:, :j + 2] @ H[:j + 2, j]
+        assert np.linalg.norm(res) <= 1e-12 * norm2 * np.linalg.norm(Z[:, j]), j
+    calls, acc = stage.counts()
+    assert calls == k and acc > 0
 
 
 def test_needs_fgmres(sg, ctx, mods):
