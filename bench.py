@@ -413,6 +413,9 @@ def learned_leg(args, sg, P, cfg, with_baseline):
                       precision="fp32", partition=part, coarse_solve=1).setup(A)
     ctx.sync()
     t_setup = time.perf_counter() - t0
+    if getattr(args, "sweep_clock", 0):
+        sweep_clock(sg, ctx, base, db, n, args.sweep_clock, args.clock_out)
+        return None
     kw = dict(method="fgmres", restart=cfg["restart"], max_iter=cfg["max_iter"], rtol=1e-8,
               ortho="cgs2")
 
@@ -461,6 +464,8 @@ def learned_leg(args, sg, P, cfg, with_baseline):
 def run_learned(args, sg, P, cfg):
     """The 1024^2 learned-stage workload as the bench line."""
     leg = learned_leg(args, sg, P, cfg, not args.no_baseline)
+    if leg is None:  # --sweep-clock: the base preconditioner's clock only
+        return
     value = leg["learned_stage"]["s"]
     out = {
         "metric": "time to ||r||/||b||<=1e-8 (s): mixed/learned FGMRES vs FP64 RAS-GMRES, 1 GPU",

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <functional>
 #include <numeric>
+#include <memory_resource>
 #include <set>
 #include <stdexcept>
 #include <thread>
@@ -59,11 +60,21 @@ Graph graph_from_csr(const HostCsr& A, bool weighted) {
     // row i, which a counting pass builds in ascending order.
     int n = A.n;
     double amin = 0.0;
-    if (weighted) {
-        amin = 1e300;
-        for (int i = 0; i < n; ++i)
-            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
-                if (A.ci[k] != i && A.v[k] != 0.0) amin = std::min(amin, std::fabs(A.v[k]));
+    if (weighted) {  // a minimum: the same whatever the chunking
+        const int nt = hw_threads();
+        std::vector<double> part_min(nt, 1e300);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+                double m = 1e300;
+                for (int i = lo; i < hi; ++i)
+                    for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                        if (A.ci[k] != i && A.v[k] != 0.0) m = std::min(m, std::fabs(A.v[k]));
+                part_min[t] = m;
+            });
+        for (auto& x : th) x.join();
+        amin = *std::min_element(part_min.begin(), part_min.end());
         if (!(amin < 1e300)) weighted = false;
     }
     std::vector<long long> wk;
@@ -78,21 +89,54 @@ Graph graph_from_csr(const HostCsr& A, bool weighted) {
     }
     auto wt = [&](int k) -> long long { return weighted ? wk[k] : 1; };
     auto valid = [&](int i, int j) { return j != i && j >= 0 && j < n; };
+    // A symmetric matrix (pattern and weights, rows strictly ascending) is its
+    // own transpose: the union of row i and column i is row i, so the
+    // transpose is not built (the graph is the same).
+    bool symmetric = true;
+    {
+        std::vector<char> ok(hw_threads(), 1);
+        std::vector<std::thread> th;
+        const int nt = (int)ok.size();
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+                for (int i = lo; i < hi && ok[t]; ++i)
+                    for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                        const int j = A.ci[k];
+                        if (k > A.rp[i] && A.ci[k - 1] >= j) { ok[t] = 0; break; }
+                        if (!valid(i, j)) continue;
+                        const int* b = A.ci.data() + A.rp[j];
+                        const int* e = A.ci.data() + A.rp[j + 1];
+                        const int* f = std::lower_bound(b, e, i);
+                        if (f == e || *f != i || wt((int)(f - A.ci.data())) != wt(k)) {
+                            ok[t] = 0;
+                            break;
+                        }
+                    }
+            });
+        for (auto& x : th) x.join();
+        for (char c : ok) symmetric = symmetric && c;
+    }
     // transpose pattern: tr row j lists every i with a valid entry (i, j), ascending i
     std::vector<int> tp(n + 1, 0);
-    for (int i = 0; i < n; ++i)
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
-            if (valid(i, A.ci[k])) tp[A.ci[k] + 1]++;
-    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
-    std::vector<int> ti(tp[n]), fill(tp.begin(), tp.end() - 1);
-    std::vector<long long> tw(tp[n]);
-    for (int i = 0; i < n; ++i)
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const int j = A.ci[k];
-            if (!valid(i, j)) continue;
-            ti[fill[j]] = i;
-            tw[fill[j]++] = wt(k);
-        }
+    std::vector<int> ti;
+    std::vector<long long> tw;
+    if (!symmetric) {
+        for (int i = 0; i < n; ++i)
+            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                if (valid(i, A.ci[k])) tp[A.ci[k] + 1]++;
+        for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+        ti.resize(tp[n]);
+        tw.resize(tp[n]);
+        std::vector<int> fill(tp.begin(), tp.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const int j = A.ci[k];
+                if (!valid(i, j)) continue;
+                ti[fill[j]] = i;
+                tw[fill[j]++] = wt(k);
+            }
+    }
     // row i's own entries, sorted by column with the largest weight first
     auto own_row = [&](int i, std::vector<std::pair<int, long long>>& r) {
         r.clear();
@@ -197,32 +241,103 @@ bool coarsen(const Graph& g, Graph& c, std::vector<int>& cmap) {
         if (first[cv] < 0) first[cv] = v; else second[cv] = v;
     }
     // coarse adjacency: the members' neighbours in order of first appearance,
-    // weights summed
-    c.xadj.assign(nc + 1, 0);
-    c.adj.clear();
-    c.ew.clear();
-    c.adj.reserve(g.adj.size() / 2);
-    c.ew.reserve(g.adj.size() / 2);
-    std::vector<int> mark(nc, -1), slot(nc, 0);
-    for (int cv = 0; cv < nc; ++cv) {
+    // weights summed.  Each coarse row depends only on its members' rows, so
+    // with several threads rows are counted, then filled, in parallel chunks
+    // (the same graph as the single pass).
+    auto row_of = [&](int cv, std::vector<std::pair<int, long long>>& r,
+                      std::vector<std::pair<int, int>>& srt) {
+        r.clear();
         for (int m = 0; m < 2; ++m) {
-            int v = m == 0 ? first[cv] : second[cv];
+            const int v = m == 0 ? first[cv] : second[cv];
             if (v < 0) continue;
             for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
-                int cu = cmap[g.adj[k]];
-                if (cu == cv) continue;
-                if (mark[cu] != cv) {
-                    mark[cu] = cv;
-                    slot[cu] = (int)c.adj.size();
-                    c.adj.push_back(cu);
-                    c.ew.push_back(g.ew[k]);
-                } else {
-                    c.ew[slot[cu]] += g.ew[k];
-                }
+                const int cu = cmap[g.adj[k]];
+                if (cu != cv) r.push_back({cu, g.ew[k]});
             }
         }
-        c.xadj[cv + 1] = (int)c.adj.size();
+        if (r.size() <= 64) {  // short rows: merge in place, first appearance kept
+            size_t w = 0;
+            for (size_t a = 0; a < r.size(); ++a) {
+                size_t q = 0;
+                while (q < w && r[q].first != r[a].first) ++q;
+                if (q < w) r[q].second += r[a].second;
+                else r[w++] = r[a];
+            }
+            r.resize(w);
+            return;
+        }
+        // long rows: sort (neighbour, position), sum runs, back to first-appearance order
+        srt.resize(r.size());
+        for (size_t a = 0; a < r.size(); ++a) srt[a] = {r[a].first, (int)a};
+        std::sort(srt.begin(), srt.end());
+        std::vector<std::pair<int, long long>> u;  // (first position, weight)
+        std::vector<int> who;
+        for (size_t a = 0; a < srt.size();) {
+            size_t b = a;
+            long long w = 0;
+            while (b < srt.size() && srt[b].first == srt[a].first) w += r[srt[b++].second].second;
+            u.push_back({srt[a].second, w});
+            who.push_back(srt[a].first);
+            a = b;
+        }
+        std::vector<int> ord(u.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int x, int y) { return u[x].first < u[y].first; });
+        r.resize(ord.size());
+        for (size_t a = 0; a < ord.size(); ++a) r[a] = {who[ord[a]], u[ord[a]].second};
+    };
+    c.xadj.assign(nc + 1, 0);
+    const int nt = tl_share > 0 ? tl_share : hw_threads();
+    if (nt == 1 || nc < 200000) {  // one thread: a single pass with a mark array
+        c.adj.clear();
+        c.ew.clear();
+        c.adj.reserve(g.adj.size() / 2);
+        c.ew.reserve(g.adj.size() / 2);
+        std::vector<int> mark(nc, -1), slot(nc, 0);
+        for (int cv = 0; cv < nc; ++cv) {
+            for (int m = 0; m < 2; ++m) {
+                const int v = m == 0 ? first[cv] : second[cv];
+                if (v < 0) continue;
+                for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) {
+                    const int cu = cmap[g.adj[k]];
+                    if (cu == cv) continue;
+                    if (mark[cu] != cv) {
+                        mark[cu] = cv;
+                        slot[cu] = (int)c.adj.size();
+                        c.adj.push_back(cu);
+                        c.ew.push_back(g.ew[k]);
+                    } else {
+                        c.ew[slot[cu]] += g.ew[k];
+                    }
+                }
+            }
+            c.xadj[cv + 1] = (int)c.adj.size();
+        }
+        return true;
     }
+    parallel_rows(nc, [&](int lo, int hi) {
+        std::vector<std::pair<int, long long>> r;
+        std::vector<std::pair<int, int>> srt;
+        for (int cv = lo; cv < hi; ++cv) {
+            row_of(cv, r, srt);
+            c.xadj[cv + 1] = (int)r.size();
+        }
+    });
+    for (int cv = 0; cv < nc; ++cv) c.xadj[cv + 1] += c.xadj[cv];
+    c.adj.resize(c.xadj[nc]);
+    c.ew.resize(c.xadj[nc]);
+    parallel_rows(nc, [&](int lo, int hi) {
+        std::vector<std::pair<int, long long>> r;
+        std::vector<std::pair<int, int>> srt;
+        for (int cv = lo; cv < hi; ++cv) {
+            row_of(cv, r, srt);
+            int pos = c.xadj[cv];
+            for (const auto& e : r) {
+                c.adj[pos] = e.first;
+                c.ew[pos++] = e.second;
+            }
+        }
+    });
     return true;
 }
 
@@ -240,6 +355,12 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
     int n = g.nv;
     long long w0 = 0;
     for (int v = 0; v < n; ++v) if (side[v] == 0) w0 += g.vw[v];
+    long long minvw = 1, maxvw = 1;
+    if (n > 0) {
+        const auto mm = std::minmax_element(g.vw.begin(), g.vw.end());
+        minvw = *mm.first;
+        maxvw = *mm.second;
+    }
     std::vector<long long> gain(n);
     std::vector<int> locked(n);
     auto compute_gain = [&](int v) {
@@ -253,20 +374,67 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
             if (side[g.adj[k]] != side[v]) return true;
         return false;
     };
-    std::vector<char> bnd(n);
+    // bnd[v] and gain[v] (for boundary v) hold at the start of every pass:
+    // the first pass computes them on all threads; a pass changes sides only
+    // of the vertices it moved, so before the next one they are recomputed
+    // for those and their neighbours only, and the boundary list is updated.
+    std::vector<char> bnd(n), inlist(n, 0), inq(n, 0);
+    std::vector<int> blist, stamp(n, -1);
+    parallel_rows(n, [&](int lo, int hi) {
+        for (int v = lo; v < hi; ++v) {
+            bnd[v] = boundary(v);
+            if (bnd[v]) gain[v] = compute_gain(v);
+        }
+    });
+    for (int v = 0; v < n; ++v)
+        if (bnd[v]) {
+            blist.push_back(v);
+            inlist[v] = 1;
+        }
+    std::vector<int> moved;
     for (int pass = 0; pass < passes; ++pass) {
-        std::set<std::pair<long long, int>> q[2];  // (-gain, v)
-        std::fill(locked.begin(), locked.end(), 0);
-        // boundary flags and gains on all threads, then the queues in index order
-        parallel_rows(n, [&](int lo, int hi) {
-            for (int v = lo; v < hi; ++v) {
+        if (pass > 0) {
+            for (int v : moved) locked[v] = 0;
+            std::vector<int> touched;
+            auto touch = [&](int v) {
+                if (stamp[v] == pass) return;
+                stamp[v] = pass;
                 bnd[v] = boundary(v);
                 if (bnd[v]) gain[v] = compute_gain(v);
+                if (bnd[v] && !inlist[v]) touched.push_back(v);
+            };
+            for (int v : moved) {
+                touch(v);
+                for (int k = g.xadj[v]; k < g.xadj[v + 1]; ++k) touch(g.adj[k]);
             }
-        });
-        for (int v = 0; v < n; ++v)
-            if (bnd[v]) q[side[v]].insert({-gain[v], v});
-        std::vector<int> moved;
+            size_t keep = 0;
+            for (int v : blist)
+                if (bnd[v]) blist[keep++] = v;
+                else inlist[v] = 0;
+            blist.resize(keep);
+            for (int v : touched) {
+                blist.push_back(v);
+                inlist[v] = 1;
+            }
+        }
+        // the queues are ordered sets (the order of insertion does not
+        // matter): filled from sorted keys with end hints, nodes from a pool
+        std::pmr::unsynchronized_pool_resource pool;
+        std::pmr::set<std::pair<long long, int>> q[2] = {
+            std::pmr::set<std::pair<long long, int>>(&pool),
+            std::pmr::set<std::pair<long long, int>>(&pool)};  // (-gain, v)
+        {
+            std::vector<std::pair<long long, int>> keys[2];
+            for (int v : blist) keys[side[v]].push_back({-gain[v], v});
+            for (int s = 0; s < 2; ++s) {
+                std::sort(keys[s].begin(), keys[s].end());
+                for (const auto& k : keys[s]) {
+                    q[s].insert(q[s].end(), k);
+                    inq[k.second] = 1;
+                }
+            }
+        }
+        moved.clear();
         long long cur = 0, best = 0;
         size_t best_len = 0;
         long long best_imb = std::llabs(w0 - (lo + hi) / 2);
@@ -278,13 +446,23 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
             int pick = -1;
             int pside = -1;
             for (int s = 0; s < 2; ++s) {
+                // feasible: the move keeps side 0's weight in the window, or
+                // moves it toward the window from outside.  With w0 fixed that
+                // is an interval [L, U] of the vertex's weight (weights >= 1).
+                long long L, U;
+                if (s == 0) {
+                    U = w0 - lo;
+                    L = w0 > hi ? 1 : w0 - hi;
+                } else {
+                    U = hi - w0;
+                    L = w0 < lo ? 1 : lo - w0;
+                }
+                if (U < L || U < minvw || L > maxvw) continue;  // no vertex fits
+                const bool all_fit = L <= minvw && maxvw <= U;   // every vertex fits
                 for (auto it = q[s].begin(); it != q[s].end(); ++it) {
                     int v = it->second;
+                    if (!all_fit && (g.vw[v] < L || g.vw[v] > U)) continue;
                     long long nw0 = s == 0 ? w0 - g.vw[v] : w0 + g.vw[v];
-                    // feasible: stays in the window, or moves toward it from outside
-                    bool inside = nw0 >= lo && nw0 <= hi;
-                    bool closer = (w0 > hi && nw0 < w0 && nw0 >= lo) || (w0 < lo && nw0 > w0 && nw0 <= hi);
-                    if (!inside && !closer) continue;
                     if (pick < 0 || gain[v] > gain[pick] ||
                         (gain[v] == gain[pick] && std::llabs(nw0 - (lo + hi) / 2) <
                                                       std::llabs((pside == 0 ? w0 - g.vw[pick] : w0 + g.vw[pick]) - (lo + hi) / 2))) {
@@ -296,18 +474,24 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
             }
             if (pick < 0) break;
             q[pside].erase({-gain[pick], pick});
+            inq[pick] = 0;
             cur -= gain[pick];
             w0 += pside == 0 ? -g.vw[pick] : g.vw[pick];
             side[pick] ^= 1;
             locked[pick] = 1;
             moved.push_back(pick);
             for (int k = g.xadj[pick]; k < g.xadj[pick + 1]; ++k) {
-                int u = g.adj[k];
+                const int u = g.adj[k];
                 if (locked[u]) continue;
-                bool inq = q[side[u]].count({-gain[u], u}) > 0;
-                if (inq) q[side[u]].erase({-gain[u], u});
-                gain[u] = compute_gain(u);
-                if (boundary(u)) q[side[u]].insert({-gain[u], u});
+                if (inq[u]) q[side[u]].erase({-gain[u], u});
+                // gain and boundary in one pass (edge weights are >= 1, so
+                // u is on the boundary exactly when ext > 0)
+                long long ext = 0, in = 0;
+                for (int kk = g.xadj[u]; kk < g.xadj[u + 1]; ++kk)
+                    (side[g.adj[kk]] != side[u] ? ext : in) += g.ew[kk];
+                gain[u] = ext - in;
+                inq[u] = ext > 0;
+                if (inq[u]) q[side[u]].insert({-gain[u], u});
             }
             long long imb = std::llabs(w0 - (lo + hi) / 2);
             bool ok_now = w0 >= lo && w0 <= hi;
@@ -321,6 +505,8 @@ void fm_refine(const Graph& g, std::vector<int>& side, long long lo, long long h
                 ++since_best;
             }
         }
+        for (int s2 = 0; s2 < 2; ++s2)
+            for (const auto& e : q[s2]) inq[e.second] = 0;
         // roll back past the best prefix
         for (size_t i = moved.size(); i > best_len; --i) {
             int v = moved[i - 1];
@@ -435,7 +621,9 @@ std::vector<int> bisect(const Graph& g, double frac, double eps) {
     for (int l = nlev - 2; l >= 0; --l) {
         const Graph& gl = level(l);
         std::vector<int> fine(gl.nv);
-        for (int v = 0; v < gl.nv; ++v) fine[v] = side[maps[l][v]];
+        parallel_rows(gl.nv, [&](int lo, int hi) {
+            for (int v = lo; v < hi; ++v) fine[v] = side[maps[l][v]];
+        });
         side.swap(fine);
         auto wl = window(gl);
         fm_refine(gl, side, wl.first, wl.second, l == 0 ? 6 : 3);
@@ -492,9 +680,10 @@ void recurse(const Graph& g, const std::vector<int>& ids, int nparts, int first,
     std::vector<int> ids0(v0.size()), ids1(v1.size());
     for (size_t i = 0; i < v0.size(); ++i) ids0[i] = ids[v0[i]];
     for (size_t i = 0; i < v1.size(); ++i) ids1[i] = ids[v1[i]];
-    // the two halves are independent: recurse concurrently near the top
-    if (depth < 3 && g.nv > 20000) {
-        const int share = tl_share > 0 ? tl_share : hw_threads();
+    // the two halves are independent: recurse concurrently while this
+    // thread's share of the cores has room for both
+    const int share = tl_share > 0 ? tl_share : hw_threads();
+    if (share >= 2 && g.nv > 20000) {
         const int half = std::max(1, share / 2);
         std::thread t([&, half] {
             tl_share = half;
