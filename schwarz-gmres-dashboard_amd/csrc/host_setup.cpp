@@ -2,6 +2,7 @@
 #include "host_setup.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <functional>
@@ -728,26 +729,41 @@ void build_sets(const HostCsr& A, const std::vector<int>& part, int nparts, int 
     owned.assign(nparts, {});
     for (int i = 0; i < n; ++i) owned[part[i]].push_back(i);
     dofs.assign(nparts, {});
-    std::vector<int> mark(std::max(n, A.ncols), -1);
-    for (int s = 0; s < nparts; ++s) {
-        std::vector<int> cur = owned[s], frontier = owned[s];
-        for (int i : cur) mark[i] = s;
-        // Layer l adds every column of every row already in the set that is
-        // not yet in it; only the previous layer can contribute new columns.
-        for (int layer = 0; layer < overlap; ++layer) {
-            std::vector<int> add;
-            for (int i : frontier)
-                for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-                    int c = A.ci[k];
-                    if (mark[c] != s) { mark[c] = s; add.push_back(c); }
-                }
-            cur.insert(cur.end(), add.begin(), add.end());
-            frontier.swap(add);
-            if (frontier.empty()) break;
+    // Subdomains are independent: threads take them from a shared counter,
+    // each with its own mark array (stamped with the subdomain), so the sets
+    // do not depend on which thread builds which subdomain.
+    std::atomic<int> next{0};
+    auto work = [&] {
+        std::vector<int> mark(std::max(n, A.ncols), -1);
+        for (int s = next++; s < nparts; s = next++) {
+            std::vector<int> cur = owned[s], frontier = owned[s];
+            for (int i : cur) mark[i] = s;
+            // Layer l adds every column of every row already in the set that
+            // is not yet in it; only the previous layer can contribute new
+            // columns.
+            for (int layer = 0; layer < overlap; ++layer) {
+                std::vector<int> add;
+                for (int i : frontier)
+                    for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                        const int c = A.ci[k];
+                        if (mark[c] != s) {
+                            mark[c] = s;
+                            add.push_back(c);
+                        }
+                    }
+                cur.insert(cur.end(), add.begin(), add.end());
+                frontier.swap(add);
+                if (frontier.empty()) break;
+            }
+            std::sort(cur.begin(), cur.end());
+            dofs[s] = std::move(cur);
         }
-        std::sort(cur.begin(), cur.end());
-        dofs[s] = std::move(cur);
-    }
+    };
+    const int nt = std::min(hw_threads(), std::max(1, nparts));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
 }
 
 HostCsr extract_submatrix(const HostCsr& A, const std::vector<int>& idx) {

@@ -2158,80 +2158,144 @@ static void setup_ilu_batch(sg_precond* M, const sg_csr* A, bool additive,
         }
     }
 
-    // concatenate
+    // concatenate: every subdomain's offsets first (in subdomain order), then
+    // each subdomain's segments filled in parallel
     Plan pl;
     pl.subs.resize(nsub);
-    long long row0 = 0, nnz0 = 0, zoff = 0, fpos = 0, bpos = 0, fell = 0, bell = 0;
-    pl.lrp.push_back(0);
     M->sub_nnz.assign(nsub, 0);
     M->levels_fwd = M->levels_bwd = 0;
     M->max_levels = 0;
+    std::vector<std::vector<int>> rfc, rbc, rbp;
+    if (M->ring) {
+        rfc.resize(nsub);
+        rbc.resize(nsub);
+        rbp.resize(nsub);
+        std::vector<int> rF(nsub);
+        run_parallel(nsub, [&](int s) {
+            rF[s] = ring_encode(fw[s], bw[s], local[s].n, M->ring_R, rfc[s], rbc[s], rbp[s]);
+        });
+        for (int s = 0; s < nsub; ++s) M->ring_F = std::min(M->ring_F, rF[s]);
+    }
+    struct Off {
+        long long flev, blev, frow, brow, fcol, bcol, fsrc, bsrc, bdsrc, fbpos, rows, nnz, own;
+    };
+    std::vector<Off> off(nsub + 1, Off{});
+    long long zoff = 0;
     for (int s = 0; s < nsub; ++s) {
         const HostCsr& L = local[s];
+        if (!global_ilu && (int)M->dofs[s].size() != L.n) fail(SG_E_ARG, "subdomain size mismatch");
+        const Off& o = off[s];
+        Off& e = off[s + 1];
+        e.flev = o.flev + (long long)fw[s].lev.size();
+        e.blev = o.blev + (long long)bw[s].lev.size();
+        e.frow = o.frow + (long long)fw[s].row.size();
+        e.brow = o.brow + (long long)bw[s].row.size();
+        e.fcol = o.fcol + (long long)(M->ring ? rfc[s].size() : fw[s].col.size());
+        e.bcol = o.bcol + (long long)(M->ring ? rbc[s].size() : bw[s].col.size());
+        e.fsrc = o.fsrc + (long long)fw[s].src.size();
+        e.bsrc = o.bsrc + (long long)bw[s].src.size();
+        e.bdsrc = o.bdsrc + (long long)bw[s].dsrc.size();
+        e.fbpos = o.fbpos + (long long)(M->ring ? rbp[s].size() : 0);
+        e.rows = o.rows + L.n;
+        e.nnz = o.nnz + L.nnz();
+        e.own = o.own + (long long)own[s].size();
         SubInfo& si = pl.subs[s];
         si.ns = L.n;
-        si.row0 = (int)row0;
+        si.row0 = (int)o.rows;
         si.zoff = (int)zoff;
-        si.flev0 = (int)pl.flev.size();
+        si.flev0 = (int)o.flev;
         si.nflev = (int)fw[s].lev.size();
-        si.blev0 = (int)pl.blev.size();
+        si.blev0 = (int)o.blev;
         si.nblev = (int)bw[s].lev.size();
-        si.own0 = (int)pl.own.size();
+        si.own0 = (int)o.own;
         si.nown = (int)own[s].size();
-        for (Level lv : fw[s].lev) pl.flev.push_back(Level{lv.start + (int)fpos, lv.size, lv.K, lv.ell + (int)fell});
-        for (Level lv : bw[s].lev) pl.blev.push_back(Level{lv.start + (int)bpos, lv.size, lv.K, lv.ell + (int)bell});
-        for (int dir = 0; dir < 2; ++dir) {
-            const auto& lev = dir ? bw[s].lev : fw[s].lev;
-            auto& item = dir ? pl.b_item : pl.f_item;
-            int acc = 0;
-            for (const Level& lv : lev) {
-                item.push_back(acc);
-                acc += (lv.size + 63) / 64;
-            }
-        }
-        pl.f_row.insert(pl.f_row.end(), fw[s].row.begin(), fw[s].row.end());
-        pl.b_row.insert(pl.b_row.end(), bw[s].row.begin(), bw[s].row.end());
-        if (M->ring) {
-            std::vector<int> fc, bc, bp;
-            M->ring_F = std::min(M->ring_F, ring_encode(fw[s], bw[s], L.n, M->ring_R, fc, bc, bp));
-            pl.f_col.insert(pl.f_col.end(), fc.begin(), fc.end());
-            pl.b_col.insert(pl.b_col.end(), bc.begin(), bc.end());
-            pl.f_bpos.insert(pl.f_bpos.end(), bp.begin(), bp.end());
-        } else {
-            pl.f_col.insert(pl.f_col.end(), fw[s].col.begin(), fw[s].col.end());
-            pl.b_col.insert(pl.b_col.end(), bw[s].col.begin(), bw[s].col.end());
-        }
-        for (int k : fw[s].src) pl.f_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
-        for (int k : bw[s].src) pl.b_src.push_back(k >= 0 ? (int)(k + nnz0) : -1);
-        for (int k : bw[s].dsrc) pl.b_dsrc.push_back((int)(k + nnz0));
-        for (int i = 0; i < L.n; ++i) {
-            pl.lrp.push_back((int)(L.rp[i + 1] + nnz0));
-            pl.ldiag.push_back((int)(diag[s][i] + nnz0));
-        }
-        pl.lci.insert(pl.lci.end(), L.ci.begin(), L.ci.end());
-        pl.lval.insert(pl.lval.end(), L.v.begin(), L.v.end());
-        if (global_ilu) {
-            for (int i = 0; i < L.n; ++i) pl.dofs.push_back(i);
-        } else {
-            pl.dofs.insert(pl.dofs.end(), M->dofs[s].begin(), M->dofs[s].end());
-        }
-        pl.own.insert(pl.own.end(), own[s].begin(), own[s].end());
-        for (int li : own[s]) pl.own_gid.push_back(global_ilu ? li : M->dofs[s][li]);
         M->sub_nnz[s] = L.nnz();
         M->levels_fwd += si.nflev;
         M->levels_bwd += si.nblev;
         M->max_levels = std::max(M->max_levels, std::max(si.nflev, si.nblev));
         pl.max_ns = std::max(pl.max_ns, L.n);
-        row0 += L.n;
-        nnz0 += L.nnz();
         zoff += L.n + 1;
-        fpos += fw[s].row.size();
-        bpos += bw[s].row.size();
-        fell += fw[s].col.size();
-        bell += bw[s].col.size();
     }
-    if (nnz0 > INT32_MAX || fell > INT32_MAX || bell > INT32_MAX || zoff > INT32_MAX)
+    const Off& tot = off[nsub];
+    const long long row0 = tot.rows, nnz0 = tot.nnz;
+    // the 64-bit offsets (the level fields are 32-bit) are checked before use
+    if (nnz0 > INT32_MAX || tot.fcol > INT32_MAX || tot.bcol > INT32_MAX || zoff > INT32_MAX ||
+        tot.frow > INT32_MAX || tot.brow > INT32_MAX)
         fail(SG_E_ARG, "subdomain data exceeds 32-bit indexing");
+    pl.flev.resize(tot.flev);
+    pl.blev.resize(tot.blev);
+    pl.f_item.resize(tot.flev);
+    pl.b_item.resize(tot.blev);
+    pl.f_row.resize(tot.frow);
+    pl.b_row.resize(tot.brow);
+    pl.f_col.resize(tot.fcol);
+    pl.b_col.resize(tot.bcol);
+    pl.f_src.resize(tot.fsrc);
+    pl.b_src.resize(tot.bsrc);
+    pl.b_dsrc.resize(tot.bdsrc);
+    if (M->ring) pl.f_bpos.resize(tot.fbpos);
+    pl.lrp.resize(row0 + 1);
+    pl.lrp[0] = 0;
+    pl.ldiag.resize(row0);
+    pl.lci.resize(nnz0);
+    pl.lval.resize(nnz0);
+    pl.dofs.resize(row0);
+    pl.own.resize(tot.own);
+    pl.own_gid.resize(tot.own);
+    run_parallel(nsub, [&](int s) {
+        const HostCsr& L = local[s];
+        const Off& o = off[s];
+        const int fpos = (int)o.frow, bpos = (int)o.brow, fell = (int)o.fcol, bell = (int)o.bcol;
+        const int nz0 = (int)o.nnz;
+        for (size_t l = 0; l < fw[s].lev.size(); ++l) {
+            const Level lv = fw[s].lev[l];
+            pl.flev[o.flev + l] = Level{lv.start + fpos, lv.size, lv.K, lv.ell + fell};
+        }
+        for (size_t l = 0; l < bw[s].lev.size(); ++l) {
+            const Level lv = bw[s].lev[l];
+            pl.blev[o.blev + l] = Level{lv.start + bpos, lv.size, lv.K, lv.ell + bell};
+        }
+        for (int dir = 0; dir < 2; ++dir) {
+            const auto& lev = dir ? bw[s].lev : fw[s].lev;
+            int* item = (dir ? pl.b_item.data() + o.blev : pl.f_item.data() + o.flev);
+            int acc = 0;
+            for (size_t l = 0; l < lev.size(); ++l) {
+                item[l] = acc;
+                acc += (lev[l].size + 63) / 64;
+            }
+        }
+        std::copy(fw[s].row.begin(), fw[s].row.end(), pl.f_row.begin() + o.frow);
+        std::copy(bw[s].row.begin(), bw[s].row.end(), pl.b_row.begin() + o.brow);
+        if (M->ring) {
+            std::copy(rfc[s].begin(), rfc[s].end(), pl.f_col.begin() + o.fcol);
+            std::copy(rbc[s].begin(), rbc[s].end(), pl.b_col.begin() + o.bcol);
+            std::copy(rbp[s].begin(), rbp[s].end(), pl.f_bpos.begin() + o.fbpos);
+        } else {
+            std::copy(fw[s].col.begin(), fw[s].col.end(), pl.f_col.begin() + o.fcol);
+            std::copy(bw[s].col.begin(), bw[s].col.end(), pl.b_col.begin() + o.bcol);
+        }
+        for (size_t i = 0; i < fw[s].src.size(); ++i) {
+            const int k = fw[s].src[i];
+            pl.f_src[o.fsrc + i] = k >= 0 ? k + nz0 : -1;
+        }
+        for (size_t i = 0; i < bw[s].src.size(); ++i) {
+            const int k = bw[s].src[i];
+            pl.b_src[o.bsrc + i] = k >= 0 ? k + nz0 : -1;
+        }
+        for (size_t i = 0; i < bw[s].dsrc.size(); ++i) pl.b_dsrc[o.bdsrc + i] = bw[s].dsrc[i] + nz0;
+        for (int i = 0; i < L.n; ++i) {
+            pl.lrp[o.rows + 1 + i] = L.rp[i + 1] + nz0;
+            pl.ldiag[o.rows + i] = diag[s][i] + nz0;
+            pl.dofs[o.rows + i] = global_ilu ? i : M->dofs[s][i];
+        }
+        std::copy(L.ci.begin(), L.ci.begin() + L.nnz(), pl.lci.begin() + o.nnz);
+        std::copy(L.v.begin(), L.v.begin() + L.nnz(), pl.lval.begin() + o.nnz);
+        for (size_t q = 0; q < own[s].size(); ++q) {
+            const int li = own[s][q];
+            pl.own[o.own + q] = li;
+            pl.own_gid[o.own + q] = global_ilu ? li : M->dofs[s][li];
+        }
+    });
     // padding read (never used) by the branch-free prefetch of idle threads
     pl.f_row.resize(pl.f_row.size() + kPadRows, 0);
     if (M->ring) pl.f_bpos.resize(pl.f_row.size(), 0);
@@ -2700,10 +2764,12 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
     std::vector<double> c(nc);
     for (int s = 0; s < nc; ++s) c[s] = 1.0 / std::sqrt((double)M->owned[s].size());
     M->A0.assign((size_t)nc * nc, 0.0);
-    std::vector<double> aphi(nc, 0.0);
-    std::vector<char> seen(nc, 0);
-    std::vector<int> touched;
-    for (int s = 0; s < nc; ++s)
+    // row s of A0 comes from subdomain s's owned rows alone: rows in parallel,
+    // each summed in the owned rows' order (the same values on any thread count)
+    run_parallel(nc, [&](int s) {
+        std::vector<double> aphi(nc, 0.0);
+        std::vector<char> seen(nc, 0);
+        std::vector<int> touched;
         for (int k : M->owned[s]) {
             touched.clear();
             for (int j = H.rp[k]; j < H.rp[k + 1]; ++j) {
@@ -2716,6 +2782,7 @@ static void setup_coarse(sg_precond* M, const sg_csr* A) {
                 seen[t] = 0;
             }
         }
+    });
     // device LU (FP64) with partial pivoting
     DevBuf<double> dA;
     dA.upload(M->A0, st);
