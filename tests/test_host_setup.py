@@ -86,3 +86,32 @@ def test_partition_errors(sg):
     with pytest.raises(sg.SgError) as e:
         sg.partition(A.rp, A.ci, 17)
     assert e.value.status == 4
+
+
+def test_partition_graph_paths_agree_and_output_is_pinned(sg):
+    """A symmetric matrix takes the graph build without a transpose; the same
+    matrix with some rows stored in reverse order takes the general build
+    (transpose, merge): same graph, so the same parts.  The parts of this
+    490,000-row case (large enough for the parallel coarsening and concurrent
+    bisections) are pinned by hash to those of the single-threaded code, since
+    the solver's iteration counts depend on the exact partition."""
+    import hashlib
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "p", os.path.join(ROOT, "schwarz-gmres-dashboard_amd", "problems.py"))
+    P = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(P)
+    rp, ci, v = P.diffusion2d(np.exp(P.grf(700, 1.5, 0.05, 17)))
+    rp, ci, v = np.asarray(rp), np.asarray(ci), np.asarray(v, np.float64)
+    pw = sg.partition(rp, ci, 64, 0, vals=v, weighted=True)
+    pu = sg.partition(rp, ci, 64, 0)
+    ci2, v2 = ci.copy(), v.copy()
+    for i in range(0, len(rp) - 1, 7):
+        a, b = rp[i], rp[i + 1]
+        ci2[a:b] = ci2[a:b][::-1]
+        v2[a:b] = v2[a:b][::-1]
+    assert np.array_equal(pw, sg.partition(rp, ci2, 64, 0, vals=v2, weighted=True))
+    assert np.array_equal(pu, sg.partition(rp, ci2, 64, 0))
+    h = lambda p: hashlib.sha256(np.asarray(p, np.int32).tobytes()).hexdigest()[:16]
+    assert h(pw) == "5087a635ba343713"
+    assert h(pu) == "ed9b319f20ca816f"
