@@ -350,7 +350,25 @@ def run_batch(args, sg, P, cfg):
     its = [r["iterations"] for r in rs]
     print(f"# batched fgmres+ras fp32: {np.mean(times):.4f} s, iterations "
           f"{min(its)}..{max(its)}, max rel {max(rels):.2e}", file=sys.stderr, flush=True)
-    extra = {}
+    # solution error against an FP64 reference: the same batch with the FP64
+    # preconditioner to ||b - A x|| / ||b|| <= 1e-12 per system
+    M64 = sg.Precond(ctx, "ras", nparts=int(sum(nparts)), overlap=cfg["overlap"],
+                     precision="fp64", partition=part).setup(B)
+    dx.zero()
+    rr = sg.solve_batch(B, db, dx, M64, **dict(kw, rtol=1e-12, max_iter=4 * cfg["max_iter"]))
+    ctx.sync()
+    xref = dx.get()
+    del M64
+    xerr, rref = [], []
+    for q, (rp, ci, v, bq) in enumerate(systems):
+        lo, hi = B.sys_rows[q], B.sys_rows[q + 1]
+        xerr.append(float(np.linalg.norm(x[lo:hi] - xref[lo:hi]) / np.linalg.norm(xref[lo:hi])))
+        rref.append(np.linalg.norm(bq - spmv_np(rp, ci, v, xref[lo:hi])) / np.linalg.norm(bq))
+    extra = {"x_reference": {"converged": all(r["converged"] for r in rr),
+                             "iterations": f"{min(r['iterations'] for r in rr)}.."
+                                           f"{max(r['iterations'] for r in rr)}",
+                             "max_rel_true_residual": float(max(rref)),
+                             "what": "the same batch with the FP64 preconditioner, to 1e-12"}}
     if not args.no_baseline:
         tb, bits = 0.0, []
         for (rp, ci, v, bq), pq, k in zip(systems, np.split(part, B.sys_rows[1:-1]), nparts):
@@ -380,6 +398,8 @@ def run_batch(args, sg, P, cfg):
                    "nnz": B.nnz, "solver": f"batched fgmres({cfg['restart']})+ras",
                    "nparts": f"{min(nparts)}..{max(nparts)} per system", "overlap": cfg["overlap"],
                    "iterations": f"{min(its)}..{max(its)}", "max_rel_true_residual": max(rels),
+                   "max_x_rel_error": max(xerr), "median_x_rel_error": float(np.median(xerr)),
+                   "x_rel_error_vs": "FP64-preconditioned batch solution at 1e-12",
                    "setup_s": round(t_setup, 3), "graph": True, "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
@@ -429,9 +449,9 @@ def learned_leg(args, sg, P, cfg, with_baseline):
         t = float(np.median([q["wall_s"] for q in rs]))
         print(f"# {label}: {r['iterations']} it, {t:.4f} s, rel {rel:.2e}", file=sys.stderr,
               flush=True)
-        return t, r["iterations"], rel
+        return t, r["iterations"], rel, x
 
-    t_plain, it_plain, rel_p = timed(base, "plain mixed two-level RAS-FGMRES")
+    t_plain, it_plain, rel_p, x_plain = timed(base, "plain mixed two-level RAS-FGMRES")
     out = {"workload": "diffusion2d_grf_1024^2", "n": n, "nnz": nnz,
            "setup_s": round(t_setup, 3),
            "plain_mixed_twolevel_fgmres": {"s": round(t_plain, 4), "iterations": it_plain,
@@ -440,13 +460,28 @@ def learned_leg(args, sg, P, cfg, with_baseline):
     net = sg.UNet(ctx, np.load(wpath))
     stage = sg.LearnedStage(ctx, base, net, g.ravel(), side, side, min_gain=args.min_gain,
                             every=args.every, coarsen=args.coarsen).setup(A)
-    t_l, it_l, rel_l = timed(stage, "learned stage FGMRES")
+    t_l, it_l, rel_l, x_l = timed(stage, "learned stage FGMRES")
     calls, acc = stage.counts()
     out["learned_stage"] = {"s": round(t_l, 4), "iterations": it_l, "rel_true_residual": rel_l,
                             "coarsen": args.coarsen, "every": args.every,
                             "stage_calls": calls, "stage_accepted": acc,
                             "weights": os.path.basename(wpath)}
     out["learned_speedup_vs_plain"] = round(t_plain / t_l, 3)
+    # solution error against an FP64 reference: FP64 two-level RAS-FGMRES on the
+    # same partition to ||b - A x|| / ||b|| <= 1e-12
+    M64 = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                     precision="fp64", partition=part, coarse_solve=1).setup(A)
+    rr = run_solves(sg, ctx, A, M64, db, dx, 1, **dict(kw, rtol=1e-12,
+                                                       max_iter=4 * cfg["max_iter"]))[0]
+    xref = dx.get()
+    nref = np.linalg.norm(xref)
+    out["x_reference"] = {"iterations": rr["iterations"], "converged": rr["converged"],
+                          "rel_true_residual": float(np.linalg.norm(b - spmv_np(rp, ci, v, xref))
+                                                     / np.linalg.norm(b)),
+                          "what": "FP64 two-level RAS-FGMRES, same partition, to 1e-12"}
+    out["plain_mixed_twolevel_fgmres"]["x_rel_error"] = float(np.linalg.norm(x_plain - xref) / nref)
+    out["learned_stage"]["x_rel_error"] = float(np.linalg.norm(x_l - xref) / nref)
+    del M64
     if with_baseline:
         Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
                         precision="fp64", partition=part).setup(A)
