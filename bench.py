@@ -821,8 +821,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
-              "multi-GPU path (RCCL over xGMI) is a design in DESIGN.md, not built.",
+        print("bench.py measures one GPU only: the solver runs on a single GPU; a "
+              "multi-GPU path (RCCL over xGMI) is designed but not built.",
               file=sys.stderr)
         sys.exit(1)
 
