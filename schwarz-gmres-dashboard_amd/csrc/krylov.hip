@@ -291,7 +291,8 @@ __device__ __forceinline__ void gram_dot_cols(int cb, int ncols, int p0, int p1,
 #pragma unroll
         for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+            for (int r = 0; r < kEoG; ++r)
+                a[c][r] = __builtin_nontemporal_load(&V[(size_t)(cb + c) * ld + ee[r]]);
         double wr[kEoG], vr[kEoG];
 #pragma unroll
         for (int r = 0; r < kEoG; ++r) {
@@ -455,7 +456,7 @@ __device__ __forceinline__ void gram_update_cols(const double* __restrict__ V, s
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-        for (int r = 0; r < kEoG; ++r) a[c][r] = V[(size_t)(cb + c) * ld + ee[r]];
+        for (int r = 0; r < kEoG; ++r) a[c][r] = __builtin_nontemporal_load(&V[(size_t)(cb + c) * ld + ee[r]]);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int c = 0; c < CB; ++c) {

@@ -1013,7 +1013,7 @@ __device__ __forceinline__ void stage_load(StageRegs<T, NT>& R, const SChunk& c,
     const int n = stage_quads_n<T, KR>(c.nrow, c.bwd);
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i)
-        R.v[i] = src[min((int)threadIdx.x + i * NT, n - 1)];
+        R.v[i] = __builtin_nontemporal_load(&src[min((int)threadIdx.x + i * NT, n - 1)]);
 }
 
 // Words past the chunk go to a trash word (straight-line code: a branch here
@@ -1467,7 +1467,7 @@ __device__ __forceinline__ void sring_load(StageRegs<T, NT>& R, const SChunkR& c
 #pragma unroll
     for (int i = 0; i < StageRegs<T, NT>::kV; ++i) {
         const int u = min((int)threadIdx.x + i * NT, n - 1);
-        R.v[i] = *(gq*)((u < nrec ? bi : bo) + 16 * (size_t)u);
+        R.v[i] = __builtin_nontemporal_load((gq*)((u < nrec ? bi : bo) + 16 * (size_t)u));
     }
 }
 
