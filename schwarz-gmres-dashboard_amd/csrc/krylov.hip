@@ -780,7 +780,7 @@ k_update(Segs sp, double* __restrict__ x, const double* __restrict__ B, size_t l
     __syncthreads();
     for (int e = br.start + threadIdx.x; e < br.end; e += kT) {
         double xe = x[e];
-        for (int i = 0; i < k; ++i) xe += ys[i] * B[(size_t)i * ld + e];
+        for (int i = 0; i < k; ++i) xe += ys[i] * __builtin_nontemporal_load(&B[(size_t)i * ld + e]);
         x[e] = xe;
     }
 }
