@@ -561,7 +561,7 @@ k_norm_scale(Segs sp, Segs sp_norm, const double* __restrict__ u, double* __rest
     const int tid = threadIdx.x;
     double pre[U];
 #pragma unroll
-    for (int r = 0; r < U; ++r) pre[r] = u[min(br.start + tid + r * kT, br.end - 1)];
+    for (int r = 0; r < U; ++r) pre[r] = __builtin_nontemporal_load(&u[min(br.start + tid + r * kT, br.end - 1)]);
     __shared__ double nrm;
     const SysBlocks sb = sp_norm.sys[br.sys];
     if (threadIdx.x < 64) {
@@ -597,7 +597,7 @@ k_norm_scale(Segs sp, Segs sp_norm, const double* __restrict__ u, double* __rest
     for (int e0 = br.start + tid + U * kT; e0 < br.end; e0 += U * kT) {
         double a[U];
 #pragma unroll
-        for (int r = 0; r < U; ++r) a[r] = u[min(e0 + r * kT, br.end - 1)];
+        for (int r = 0; r < U; ++r) a[r] = __builtin_nontemporal_load(&u[min(e0 + r * kT, br.end - 1)]);
 #pragma unroll
         for (int r = 0; r < U; ++r)
             if (e0 + r * kT < br.end) v[e0 + r * kT] = div ? a[r] / sc : a[r] * sc;
