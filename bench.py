@@ -14,7 +14,8 @@ The dominant kernel family's achieved bandwidth is measured live with HIP
 events on the solver stream; the CPU oracle (the reference algorithm restated
 in C++) is timed on a bounded number of iterations on the host cores.
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--no-baseline] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c1|c4|c5] [--no-baseline] [--no-cpu]
+                    [--graph 0|1] [--no-learned]
 """
 import argparse
 import ctypes as C
@@ -331,7 +332,7 @@ def run_batch(args, sg, P, cfg):
     db = sg.DeviceArray.from_host(ctx, b)
     dx = sg.DeviceArray(ctx, B.n)
     kw = dict(method="fgmres", restart=cfg["restart"], max_iter=cfg["max_iter"], rtol=1e-8,
-              ortho="cgs2", graph=True)
+              ortho="cgs2", graph=bool(args.graph))
     times, rs = [], None
     for it in range(args.warmup + args.steps):
         dx.zero()
@@ -351,24 +352,27 @@ def run_batch(args, sg, P, cfg):
     print(f"# batched fgmres+ras fp32: {np.mean(times):.4f} s, iterations "
           f"{min(its)}..{max(its)}, max rel {max(rels):.2e}", file=sys.stderr, flush=True)
     # solution error against an FP64 reference: the same batch with the FP64
-    # preconditioner to ||b - A x|| / ||b|| <= 1e-12 per system
-    M64 = sg.Precond(ctx, "ras", nparts=int(sum(nparts)), overlap=cfg["overlap"],
-                     precision="fp64", partition=part).setup(B)
-    dx.zero()
-    rr = sg.solve_batch(B, db, dx, M64, **dict(kw, rtol=1e-12, max_iter=4 * cfg["max_iter"]))
-    ctx.sync()
-    xref = dx.get()
-    del M64
-    xerr, rref = [], []
-    for q, (rp, ci, v, bq) in enumerate(systems):
-        lo, hi = B.sys_rows[q], B.sys_rows[q + 1]
-        xerr.append(float(np.linalg.norm(x[lo:hi] - xref[lo:hi]) / np.linalg.norm(xref[lo:hi])))
-        rref.append(np.linalg.norm(bq - spmv_np(rp, ci, v, xref[lo:hi])) / np.linalg.norm(bq))
-    extra = {"x_reference": {"converged": all(r["converged"] for r in rr),
-                             "iterations": f"{min(r['iterations'] for r in rr)}.."
-                                           f"{max(r['iterations'] for r in rr)}",
-                             "max_rel_true_residual": float(max(rref)),
-                             "what": "the same batch with the FP64 preconditioner, to 1e-12"}}
+    # preconditioner to ||b - A x|| / ||b|| <= 1e-12 per system (skipped with
+    # --no-baseline, as the other reference legs are)
+    xerr, extra = [], {}
+    if not args.no_baseline:
+        M64 = sg.Precond(ctx, "ras", nparts=int(sum(nparts)), overlap=cfg["overlap"],
+                         precision="fp64", partition=part).setup(B)
+        dx.zero()
+        rr = sg.solve_batch(B, db, dx, M64, **dict(kw, rtol=1e-12, max_iter=4 * cfg["max_iter"]))
+        ctx.sync()
+        xref = dx.get()
+        del M64
+        rref = []
+        for q, (rp, ci, v, bq) in enumerate(systems):
+            lo, hi = B.sys_rows[q], B.sys_rows[q + 1]
+            xerr.append(float(np.linalg.norm(x[lo:hi] - xref[lo:hi]) / np.linalg.norm(xref[lo:hi])))
+            rref.append(np.linalg.norm(bq - spmv_np(rp, ci, v, xref[lo:hi])) / np.linalg.norm(bq))
+        extra["x_reference"] = {"converged": all(r["converged"] for r in rr),
+                                "iterations": f"{min(r['iterations'] for r in rr)}.."
+                                              f"{max(r['iterations'] for r in rr)}",
+                                "max_rel_true_residual": float(max(rref)),
+                                "what": "the same batch with the FP64 preconditioner, to 1e-12"}
     if not args.no_baseline:
         tb, bits = 0.0, []
         for (rp, ci, v, bq), pq, k in zip(systems, np.split(part, B.sys_rows[1:-1]), nparts):
@@ -398,9 +402,11 @@ def run_batch(args, sg, P, cfg):
                    "nnz": B.nnz, "solver": f"batched fgmres({cfg['restart']})+ras",
                    "nparts": f"{min(nparts)}..{max(nparts)} per system", "overlap": cfg["overlap"],
                    "iterations": f"{min(its)}..{max(its)}", "max_rel_true_residual": max(rels),
-                   "max_x_rel_error": max(xerr), "median_x_rel_error": float(np.median(xerr)),
+                   "max_x_rel_error": max(xerr) if xerr else None,
+                   "median_x_rel_error": float(np.median(xerr)) if xerr else None,
                    "x_rel_error_vs": "FP64-preconditioned batch solution at 1e-12",
-                   "setup_s": round(t_setup, 3), "graph": True, "parallelism": "single GPU"},
+                   "setup_s": round(t_setup, 3), "graph": bool(args.graph),
+                   "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
     out.update(extra)
@@ -468,20 +474,23 @@ def learned_leg(args, sg, P, cfg, with_baseline):
                             "weights": os.path.basename(wpath)}
     out["learned_speedup_vs_plain"] = round(t_plain / t_l, 3)
     # solution error against an FP64 reference: FP64 two-level RAS-FGMRES on the
-    # same partition to ||b - A x|| / ||b|| <= 1e-12
-    M64 = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
-                     precision="fp64", partition=part, coarse_solve=1).setup(A)
-    rr = run_solves(sg, ctx, A, M64, db, dx, 1, **dict(kw, rtol=1e-12,
-                                                       max_iter=4 * cfg["max_iter"]))[0]
-    xref = dx.get()
-    nref = np.linalg.norm(xref)
-    out["x_reference"] = {"iterations": rr["iterations"], "converged": rr["converged"],
-                          "rel_true_residual": float(np.linalg.norm(b - spmv_np(rp, ci, v, xref))
-                                                     / np.linalg.norm(b)),
-                          "what": "FP64 two-level RAS-FGMRES, same partition, to 1e-12"}
-    out["plain_mixed_twolevel_fgmres"]["x_rel_error"] = float(np.linalg.norm(x_plain - xref) / nref)
-    out["learned_stage"]["x_rel_error"] = float(np.linalg.norm(x_l - xref) / nref)
-    del M64
+    # same partition to ||b - A x|| / ||b|| <= 1e-12 (skipped with --no-baseline)
+    if not getattr(args, "no_baseline", False):
+        M64 = sg.Precond(ctx, "twolevel_ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
+                         precision="fp64", partition=part, coarse_solve=1).setup(A)
+        rr = run_solves(sg, ctx, A, M64, db, dx, 1, **dict(kw, rtol=1e-12,
+                                                           max_iter=4 * cfg["max_iter"]))[0]
+        xref = dx.get()
+        nref = np.linalg.norm(xref)
+        out["x_reference"] = {
+            "iterations": rr["iterations"], "converged": rr["converged"],
+            "rel_true_residual": float(np.linalg.norm(b - spmv_np(rp, ci, v, xref))
+                                       / np.linalg.norm(b)),
+            "what": "FP64 two-level RAS-FGMRES, same partition, to 1e-12"}
+        out["plain_mixed_twolevel_fgmres"]["x_rel_error"] = float(
+            np.linalg.norm(x_plain - xref) / nref)
+        out["learned_stage"]["x_rel_error"] = float(np.linalg.norm(x_l - xref) / nref)
+        del M64
     if with_baseline:
         Mb = sg.Precond(ctx, "ras", nparts=cfg["nparts"], overlap=cfg["overlap"],
                         precision="fp64", partition=part).setup(A)
